@@ -1,0 +1,117 @@
+"""ctypes binding of the C ABI declared in ``include/gigalens_amd.h``.
+
+The HIP library is the only compute path: importing this module does not
+load it, but every call that needs it raises ``NativeLibraryMissing`` when
+``gigalens_amd/_lib/libgigalens_amd.so`` has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libgigalens_amd.so")
+
+GL_MAX_COMPONENTS = 8
+GL_MAX_COMPONENT_PARAMS = 8
+GL_MAX_PARAMS = 128
+GL_MAX_EPL_ITER = 100
+
+GL_EPL, GL_SHEAR, GL_SIS = 1, 2, 3
+GL_SERSIC, GL_SERSIC_ELLIPSE, GL_SHAPELETS = 10, 11, 12
+GL_ROLE_LENS, GL_ROLE_LENS_LIGHT, GL_ROLE_SOURCE = 0, 1, 2
+GL_PRIOR_NORMAL, GL_PRIOR_LOGNORMAL, GL_PRIOR_UNIFORM, GL_PRIOR_TRUNCNORMAL = 1, 2, 3, 4
+GL_LIK_FORWARD, GL_LIK_BACKWARD = 0, 1
+
+EXPORTED_SYMBOLS = (
+    "gl_last_error",
+    "gl_plan_create",
+    "gl_plan_destroy",
+    "gl_simulate",
+    "gl_simulate_vjp",
+    "gl_log_prob",
+    "gl_map_adam_step",
+)
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class GlError(RuntimeError):
+    pass
+
+
+class ComponentDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("role", ctypes.c_int),
+        ("param_col", ctypes.c_int * GL_MAX_COMPONENT_PARAMS),
+        ("param_const", ctypes.c_float * GL_MAX_COMPONENT_PARAMS),
+        ("niter", ctypes.c_int),
+        ("n_max", ctypes.c_int),
+        ("interpolate", ctypes.c_int),
+        ("use_lstsq", ctypes.c_int),
+        ("amp_col0", ctypes.c_int),
+    ]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_pix", ctypes.c_int),
+        ("supersample", ctypes.c_int),
+        ("transform_sub", ctypes.c_float * 4),
+        ("origin", ctypes.c_float * 2),
+        ("conversion_factor", ctypes.c_float),
+        ("kernel_size", ctypes.c_int),
+        ("kernel", ctypes.POINTER(ctypes.c_float)),
+        ("n_components", ctypes.c_int),
+        ("components", ctypes.POINTER(ComponentDesc)),
+        ("n_params", ctypes.c_int),
+        ("prior_kind", ctypes.POINTER(ctypes.c_int)),
+        ("prior_args", ctypes.POINTER(ctypes.c_float)),
+        ("observed", ctypes.POINTER(ctypes.c_float)),
+        ("background_rms", ctypes.c_float),
+        ("exp_time", ctypes.c_float),
+        ("likelihood", ctypes.c_int),
+        ("fldj_at_x", ctypes.c_int),
+        ("epl_batch_series", ctypes.c_int),
+    ]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+_vp = ctypes.c_void_p
+_SIGS = {
+    "gl_last_error": ([], ctypes.c_char_p),
+    "gl_plan_create": ([ctypes.POINTER(PlanDesc), ctypes.POINTER(_vp)], ctypes.c_int),
+    "gl_plan_destroy": ([_vp], ctypes.c_int),
+    "gl_simulate": ([_vp, _vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "gl_simulate_vjp": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+    "gl_log_prob": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "gl_map_adam_step": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                f"{LIB_PATH} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `python -m gigalens_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().gl_last_error().decode(errors="replace")
+        raise GlError(f"{what} failed (code {rc}): {msg}")

@@ -1,0 +1,423 @@
+// C ABI of the lens-model hot path: plan creation and launches.
+// Declarations and reference citations: include/gigalens_amd.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "lens_kernels.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define GL_HIP(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) return fail(GL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 512;
+
+using LaunchFn = hipError_t (*)(int grid, size_t lds, hipStream_t, const gl::DPlan&, const gl::KArgs&, bool grad);
+using AttrFn = hipError_t (*)(size_t lds);
+
+template <class M>
+hipError_t launch_model(int grid, size_t lds, hipStream_t s, const gl::DPlan& P, const gl::KArgs& A, bool grad) {
+    if (grad)
+        hipLaunchKernelGGL((gl::particle_kernel<kBlock, M, true>), dim3(grid), dim3(kBlock), lds, s, P, A);
+    else
+        hipLaunchKernelGGL((gl::particle_kernel<kBlock, M, false>), dim3(grid), dim3(kBlock), lds, s, P, A);
+    return hipGetLastError();
+}
+
+template <class M>
+hipError_t attr_model(size_t lds) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::particle_kernel<kBlock, M, true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::particle_kernel<kBlock, M, false>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+struct ModelEntry {
+    int sig[7];  // lens0..2, lens light 0..1, source 0..1
+    LaunchFn launch;
+    AttrFn attr;
+};
+
+constexpr int EPL = glm::K_EPL, SHR = glm::K_SHEAR, SIS = glm::K_SIS, SE = glm::K_SERSIC_ELLIPSE,
+              SC = glm::K_SERSIC;
+
+#define GL_ENTRY(L0, L1, L2, A0, A1, S0, S1)                                      \
+    {{L0, L1, L2, A0, A1, S0, S1},                                                \
+     &launch_model<gl::ModelT<L0, L1, L2, A0, A1, S0, S1>>,                        \
+     &attr_model<gl::ModelT<L0, L1, L2, A0, A1, S0, S1>>}
+
+const ModelEntry kModels[] = {
+    GL_ENTRY(EPL, SHR, 0, SE, 0, SE, 0),   // reference default model (tests/conftest.py:76-80)
+    GL_ENTRY(EPL, 0, 0, SE, 0, SE, 0),
+    GL_ENTRY(EPL, SHR, 0, 0, 0, SE, 0),
+    GL_ENTRY(EPL, 0, 0, 0, 0, SE, 0),
+    GL_ENTRY(SIS, 0, 0, 0, 0, SE, 0),
+    GL_ENTRY(SIS, SHR, 0, SE, 0, SE, 0),
+    GL_ENTRY(EPL, SHR, 0, SE, 0, SC, 0),
+    GL_ENTRY(EPL, SHR, 0, SE, SE, SE, 0),
+};
+
+}  // namespace
+
+struct gl_plan {
+    gl::DPlan P;
+    const ModelEntry* model;
+    size_t lds_bytes;
+    int epl_batch;
+    std::vector<void*> dev;  // owned device buffers
+};
+
+extern "C" const char* gl_last_error(void) { return g_err.c_str(); }
+
+static int upload(gl_plan* p, const void* host, size_t bytes, void** out) {
+    void* d = nullptr;
+    GL_HIP(hipMalloc(&d, bytes ? bytes : 4));
+    p->dev.push_back(d);
+    if (bytes) GL_HIP(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+    *out = d;
+    return GL_OK;
+}
+
+static int n_params_of(int kind) {
+    switch (kind) {
+    case glm::K_EPL: return 6;
+    case glm::K_SHEAR: return 2;
+    case glm::K_SIS: return 3;
+    case glm::K_SERSIC: return 5;
+    case glm::K_SERSIC_ELLIPSE: return 7;
+    default: return -1;
+    }
+}
+
+extern "C" int gl_plan_destroy(gl_plan* plan) {
+    if (!plan) return GL_OK;
+    for (void* d : plan->dev) (void)hipFree(d);
+    delete plan;
+    return GL_OK;
+}
+
+extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
+    if (!desc || !out) return fail(GL_ERR_ARG, "gl_plan_create: null argument");
+    *out = nullptr;
+    const int N = desc->num_pix, ss = desc->supersample;
+    if (N <= 0 || ss <= 0) return fail(GL_ERR_ARG, "num_pix and supersample must be positive");
+    if (desc->n_params <= 0 || desc->n_params > GL_MAX_PARAMS)
+        return fail(GL_ERR_ARG, "n_params must be in [1, " + std::to_string(GL_MAX_PARAMS) + "]");
+    if (desc->n_components <= 0 || desc->n_components > GL_MAX_COMPONENTS)
+        return fail(GL_ERR_ARG, "n_components must be in [1, 8]");
+
+    gl_plan* p = new gl_plan();
+    gl::DPlan& P = p->P;
+    memset(&P, 0, sizeof(P));
+    P.N = N; P.ss = ss; P.nsub = N * ss;
+    P.M00 = desc->transform_sub[0]; P.M01 = desc->transform_sub[1];
+    P.M10 = desc->transform_sub[2]; P.M11 = desc->transform_sub[3];
+    P.ox = desc->origin[0]; P.oy = desc->origin[1];
+    P.cf = desc->conversion_factor;
+    P.d = desc->n_params;
+    P.lik = desc->likelihood;
+    P.fldj_at_x = desc->fldj_at_x;
+    P.bkg2 = desc->background_rms * desc->background_rms;
+    P.inv_texp = 1.0f / desc->exp_time;
+
+    // ---- components and model signature -----------------------------------
+    int sig[7] = {0, 0, 0, 0, 0, 0, 0};
+    int nl = 0, nll = 0, ns = 0, last_role = 0, max_niter = 0;
+    P.ncomp = desc->n_components;
+    for (int i = 0; i < desc->n_components; ++i) {
+        const gl_component_desc& c = desc->components[i];
+        if (c.role < last_role) {
+            gl_plan_destroy(p);
+            return fail(GL_ERR_ARG, "components must be ordered lenses, lens light, source");
+        }
+        last_role = c.role;
+        if (n_params_of(c.kind) < 0) {
+            gl_plan_destroy(p);
+            return fail(GL_ERR_UNSUPPORTED, "profile kind " + std::to_string(c.kind) +
+                                                " is not supported by the resident-image kernel");
+        }
+        for (int j = 0; j < n_params_of(c.kind); ++j)
+            if (c.param_col[j] >= desc->n_params) {
+                gl_plan_destroy(p);
+                return fail(GL_ERR_ARG, "param_col out of range");
+            }
+        if (c.role == GL_ROLE_LENS) {
+            if (c.kind != glm::K_EPL && c.kind != glm::K_SHEAR && c.kind != glm::K_SIS) {
+                gl_plan_destroy(p);
+                return fail(GL_ERR_ARG, "light profile given as a lens");
+            }
+            if (nl >= 3) { gl_plan_destroy(p); return fail(GL_ERR_UNSUPPORTED, "at most 3 lens components"); }
+            sig[nl++] = c.kind;
+        } else {
+            if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE) {
+                gl_plan_destroy(p);
+                return fail(GL_ERR_ARG, "mass profile given as a light component");
+            }
+            if (c.role == GL_ROLE_LENS_LIGHT) {
+                if (nll >= 2) { gl_plan_destroy(p); return fail(GL_ERR_UNSUPPORTED, "at most 2 lens-light components"); }
+                sig[3 + nll++] = c.kind;
+            } else {
+                if (ns >= 2) { gl_plan_destroy(p); return fail(GL_ERR_UNSUPPORTED, "at most 2 source components"); }
+                sig[5 + ns++] = c.kind;
+            }
+        }
+        gl::DComp& dc = P.comp[i];
+        dc.kind = c.kind; dc.role = c.role; dc.niter = c.niter; dc.n_max = c.n_max;
+        dc.interpolate = c.interpolate; dc.lstsq = c.use_lstsq; dc.amp_col0 = c.amp_col0;
+        for (int j = 0; j < GL_MAX_COMPONENT_PARAMS; ++j) { dc.col[j] = c.param_col[j]; dc.cst[j] = c.param_const[j]; }
+        if (c.kind == glm::K_EPL) {
+            if (c.niter < 0 || c.niter > GL_MAX_EPL_ITER) {
+                gl_plan_destroy(p);
+                return fail(GL_ERR_ARG, "EPL niter must be in [0, " + std::to_string(GL_MAX_EPL_ITER) + "]");
+            }
+            if (c.niter > max_niter) max_niter = c.niter;
+        }
+    }
+    P.nlens = nl; P.nll = nll; P.nsrc = ns;
+    p->model = nullptr;
+    for (const ModelEntry& e : kModels) {
+        bool ok = true;
+        for (int j = 0; j < 7; ++j) ok = ok && e.sig[j] == sig[j];
+        if (ok) { p->model = &e; break; }
+    }
+    if (!p->model) {
+        gl_plan_destroy(p);
+        return fail(GL_ERR_UNSUPPORTED, "no compiled kernel for this combination of profiles");
+    }
+
+    // ---- PSF folded with the pooling box (correlation weights) -----------------
+    const int k = desc->kernel_size > 0 ? desc->kernel_size : 1;
+    std::vector<double> K(k * k, 0.0);
+    if (desc->kernel_size > 0) {
+        if (!desc->kernel) { gl_plan_destroy(p); return fail(GL_ERR_ARG, "kernel_size > 0 but kernel is null"); }
+        for (int i = 0; i < k * k; ++i) K[i] = desc->kernel[i];
+    } else {
+        K[0] = 1.0;
+    }
+    const int kf = k + ss - 1, pb = (k - 1) / 2;
+    P.k = k; P.kf = kf; P.pb = pb;
+    std::vector<float> W(kf * kf, 0.0f);
+    for (int pp = 0; pp < kf; ++pp)
+        for (int qq = 0; qq < kf; ++qq) {
+            double acc = 0.0;
+            for (int a = 0; a < ss; ++a)
+                for (int b = 0; b < ss; ++b) {
+                    int u = pp - a, v = qq - b;
+                    if (u >= 0 && u < k && v >= 0 && v < k) acc += K[(k - 1 - u) * k + (k - 1 - v)];  // flipped
+                }
+            W[pp * kf + qq] = (float)(acc / (ss * ss));
+        }
+
+    // ---- LDS layout --------------------------------------------------------
+    const int nsub = P.nsub;
+    const int nstr = (N + gl::kRB - 1) / gl::kRB;
+    const int overshoot = ss * (nstr * gl::kRB - N);
+    int H = pb > (k - 1 - pb + overshoot) ? pb : (k - 1 - pb + overshoot);
+    int PW = (nsub + H + 3) & ~3;
+    P.PW = PW;
+    P.sub_base = pb;
+    const int sub_img = pb + (nsub + k - 1) * PW;
+    // adjoint image padding
+    int gp0 = 0, dmax = 0;
+    for (int a = 0; a < ss; ++a) {
+        int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
+        if (Ma - 1 - da > gp0) gp0 = Ma - 1 - da;
+        if (da > dmax) dmax = da;
+    }
+    const int xstr = (N + gl::kRB2 - 1) / gl::kRB2;
+    const int GW = gp0 + xstr * gl::kRB2 + dmax;
+    const int grows = gp0 + N + dmax;
+    P.GW = GW; P.gp0 = gp0; P.g_size = GW * grows;
+    P.sub_size = sub_img > P.g_size ? sub_img : P.g_size;
+    int off = (P.sub_size + 3) & ~3;
+    P.off_w = off; off += (kf * kf + 3) & ~3;
+    P.off_x = off; off += (6 * P.d + 3) & ~3;
+    P.off_c = off; off += gl::kMaxComp * gl::kConstStride;
+    P.off_coef = off;
+    P.max_n = max_niter;
+    const int coef_block = ((max_niter + 4) & ~3) + 4 * (max_niter + 1);
+    for (int i = 0; i < 3; ++i) {
+        P.coef_off[i] = (sig[i] == glm::K_EPL) ? (off - P.off_coef) : 0;
+        if (sig[i] == glm::K_EPL) off += (coef_block + 3) & ~3;
+    }
+    P.off_red = off;
+    const int nred = 2 + 8 * 3 + 8 * 4;
+    off += (kBlock / 64 + 1) * nred;
+
+    // ---- bounds proof of every LDS index the kernel forms --------------------
+    {
+        // forward conv: base = sub_base - pb; reads base + (ss I + p) PW + ss J0 + c, c < ss(RB-1)+kf
+        const long j0max = (long)(nstr - 1) * gl::kRB;
+        const long seg = (long)ss * (gl::kRB - 1) + kf;
+        const long rmax = (long)P.sub_base - pb + (long)(ss * (N - 1) + kf - 1) * PW + ss * j0max + seg - 1;
+        const long wmax = (long)P.sub_base + (long)(nsub - 1 + pb) * PW + nsub - 1;
+        bool ok = rmax < P.sub_size && wmax < P.sub_size && P.sub_base - pb >= 0;
+        // image column reach must stay inside the zero halo of the same row
+        ok = ok && (ss * j0max + seg - 1 - pb) < PW && pb <= PW - nsub;
+        // adjoint: reads (Y + da - m + gp0) GW + X0 + db - (Mb-1) + gp0 + j, j < RB2 + Mb - 1
+        const long x0max = (long)(xstr - 1) * gl::kRB2;
+        for (int a = 0; a < ss && ok; ++a) {
+            int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
+            long rlo = (long)(0 + da - (Ma - 1) + gp0), rhi = (long)(N - 1 + da + gp0);
+            long clo = (long)(0 + da - (Ma - 1) + gp0), chi = x0max + da + gp0 + gl::kRB2 - 1;
+            ok = ok && rlo >= 0 && clo >= 0 && rhi < grows && chi < GW;
+        }
+        ok = ok && (long)(N - 1 + gp0) * GW + (N - 1 + gp0) < P.g_size && P.g_size <= P.off_w;
+        if (!ok) {
+            gl_plan_destroy(p);
+            return fail(GL_ERR_UNSUPPORTED, "internal LDS layout check failed for this image size / PSF");
+        }
+    }
+    P.lds_floats = off;
+    p->lds_bytes = (size_t)off * sizeof(float);
+    if (p->lds_bytes > 160 * 1024) {
+        gl_plan_destroy(p);
+        return fail(GL_ERR_UNSUPPORTED, "image too large for the LDS-resident kernel (" +
+                                            std::to_string(p->lds_bytes) + " bytes of LDS)");
+    }
+    if (N * nstr > gl::kMaxT * kBlock) {
+        gl_plan_destroy(p);
+        return fail(GL_ERR_UNSUPPORTED, "image too large for the LDS-resident kernel (output strips)");
+    }
+
+    // ---- device constants ---------------------------------------------------
+    void* dptr = nullptr;
+    int rc;
+    if ((rc = upload(p, W.data(), W.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
+    P.w = (const float*)dptr;
+    std::vector<float> obs(N * N, 0.0f), ivar(N * N, 0.0f), lsig(N * N, 0.0f);
+    if (desc->observed) {
+        for (int i = 0; i < N * N; ++i) {
+            obs[i] = desc->observed[i];
+            // sqrt(bkg^2 + clip(obs, 0, inf) / t_exp), tf/model.py:113-115
+            float var = P.bkg2 + fmaxf(obs[i], 0.0f) / desc->exp_time;
+            ivar[i] = 1.0f / var;
+            lsig[i] = logf(sqrtf(var));
+        }
+    }
+    if ((rc = upload(p, obs.data(), obs.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
+    P.obs = (const float*)dptr;
+    if ((rc = upload(p, ivar.data(), ivar.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
+    P.inv_var_obs = (const float*)dptr;
+    if ((rc = upload(p, lsig.data(), lsig.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
+    P.log_sig_obs = (const float*)dptr;
+    std::vector<int> pk(P.d, 0);
+    std::vector<float> pa(8 * P.d, 0.0f);
+    if (desc->prior_kind) {
+        for (int j = 0; j < P.d; ++j) {
+            pk[j] = desc->prior_kind[j];
+            for (int t = 0; t < 4; ++t) pa[8 * j + t] = desc->prior_args[4 * j + t];
+            if (pk[j] == GL_PRIOR_TRUNCNORMAL) {
+                double loc = pa[8 * j], sc = pa[8 * j + 1], lo = pa[8 * j + 2], hi = pa[8 * j + 3];
+                double Z = 0.5 * (erf((hi - loc) / sc / sqrt(2.0)) - erf((lo - loc) / sc / sqrt(2.0)));
+                pa[8 * j + 4] = (float)log(Z);
+            }
+        }
+    }
+    if ((rc = upload(p, pk.data(), pk.size() * sizeof(int), &dptr))) { gl_plan_destroy(p); return rc; }
+    P.prior_kind = (const int*)dptr;
+    if ((rc = upload(p, pa.data(), pa.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
+    P.prior_args = (const float*)dptr;
+    p->epl_batch = desc->epl_batch_series;
+    P.batch_n = nullptr;
+    if (p->epl_batch) {
+        std::vector<int> z8(8, -1);
+        if ((rc = upload(p, z8.data(), z8.size() * sizeof(int), &dptr))) { gl_plan_destroy(p); return rc; }
+        P.batch_n = (const int*)dptr;
+    }
+    hipError_t e = p->model->attr(p->lds_bytes);
+    if (e != hipSuccess) {
+        gl_plan_destroy(p);
+        return fail(GL_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    return GL_OK;
+}
+
+static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream) {
+    if (!plan) return fail(GL_ERR_ARG, "null plan");
+    if (A.bs <= 0) return fail(GL_ERR_ARG, "bs must be positive");
+    if (!A.in) return fail(GL_ERR_ARG, "null parameter pointer");
+    if (plan->epl_batch) {
+        const int use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
+        hipLaunchKernelGGL(gl::epl_batch_series_kernel, dim3(1), dim3(256), 0, stream, plan->P, A.in, A.bs, use_z,
+                           const_cast<int*>(plan->P.batch_n));
+        GL_HIP(hipGetLastError());
+    }
+    GL_HIP(plan->model->launch(A.bs, plan->lds_bytes, stream, plan->P, A, grad));
+    return GL_OK;
+}
+
+static gl::KArgs blank(int mode, int bs) {
+    gl::KArgs A;
+    memset(&A, 0, sizeof(A));
+    A.mode = mode;
+    A.bs = bs;
+    return A;
+}
+
+extern "C" int gl_simulate(const gl_plan* plan, const float* x, int bs, float* img, hipStream_t stream) {
+    if (!img) return fail(GL_ERR_ARG, "null image pointer");
+    gl::KArgs A = blank(gl::M_SIM, bs);
+    A.in = x;
+    A.img = img;
+    return run(plan, A, false, stream);
+}
+
+extern "C" int gl_simulate_vjp(const gl_plan* plan, const float* x, int bs, const float* g_img, float* grad_x,
+                               hipStream_t stream) {
+    if (!g_img || !grad_x) return fail(GL_ERR_ARG, "null gradient pointer");
+    gl::KArgs A = blank(gl::M_VJP, bs);
+    A.in = x;
+    A.g_img = g_img;
+    A.grad = grad_x;
+    return run(plan, A, true, stream);
+}
+
+extern "C" int gl_log_prob(const gl_plan* plan, const float* z, int bs, float* lp, float* chi2, const float* g_lp,
+                           const float* g_chi2, float* grad_z, hipStream_t stream) {
+    gl::KArgs A = blank(gl::M_LOGPROB, bs);
+    A.in = z;
+    A.lp = lp;
+    A.chi2 = chi2;
+    A.g_lp = g_lp;
+    A.g_chi2 = g_chi2;
+    A.grad = grad_z;
+    return run(plan, A, grad_z != nullptr, stream);
+}
+
+extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, int step, float lr,
+                                float beta1, float beta2, float epsilon, float* lp, float* chi2, hipStream_t stream) {
+    if (!z || !m || !v) return fail(GL_ERR_ARG, "null optimizer state");
+    if (step < 1) return fail(GL_ERR_ARG, "step is 1-based");
+    gl::KArgs A = blank(gl::M_MAP, bs);
+    A.in = z;
+    A.z_io = z;
+    A.m = m;
+    A.v = v;
+    A.lp = lp;
+    A.chi2 = chi2;
+    double b1t = pow((double)beta1, step), b2t = pow((double)beta2, step);
+    A.lr_t = (float)((double)lr * sqrt(1.0 - b2t) / (1.0 - b1t));
+    A.beta1 = beta1;
+    A.beta2 = beta2;
+    A.eps = epsilon;
+    A.gscale = -1.0f / ((float)bs * (float)(plan ? plan->P.N * plan->P.N : 1));
+    return run(plan, A, true, stream);
+}

@@ -1,0 +1,631 @@
+// Fused per-particle lens-model kernels for gfx950 (MI355X).
+//
+// One workgroup owns one parameter particle and keeps the particle's whole
+// supersampled image resident in LDS:
+//   A  prologue   z -> x (bijectors), log prior, per-component constants,
+//                 EPL series coefficient tables (LDS)
+//   B  ray-trace  every sub-pixel: lens light(theta) + source(beta), NaN -> 0,
+//                 written into a zero-haloed LDS image
+//   C  conv+pool  stride-ss correlation with the PSF pre-folded with the ss x ss
+//                 box (== conv2d SAME + avg_pool2d, tf/simulator.py:83-96),
+//                 x conversion factor, then the Gaussian log-likelihood terms
+//   D  adjoint    (optional) transposed strided correlation of dL/dimage, in
+//                 polyphase order so the tap weights are wave-uniform
+//   E  recompute  (optional) ray-trace again per sub-pixel and accumulate the
+//                 reverse-mode statistics of every component
+//   F  reduce     wavefront shuffles + LDS, per-particle chain rule, Adam
+//
+// Reference semantics: src/gigalens/tf/simulator.py:57-97 (simulate),
+// src/gigalens/tf/model.py:53-86 (ForwardProbModel.log_prob),
+// src/gigalens/tf/inference.py:28-35 (MAP train_step).
+#include <hip/hip_runtime.h>
+#include "lens_math.h"
+#include "lens_plan.h"
+
+namespace gl {
+
+enum Mode { M_SIM = 0, M_LOGPROB = 1, M_VJP = 2, M_MAP = 3 };
+
+struct KArgs {
+    const float* in;      // x (SIM, VJP) or z (LOGPROB, MAP): bs x d
+    float* z_io;          // MAP: z updated in place
+    float* img;           // SIM: bs x N x N
+    const float* g_img;   // VJP: upstream image gradient
+    float* lp;
+    float* chi2;
+    const float* g_lp;
+    const float* g_chi2;
+    float* grad;          // LOGPROB: grad_z, VJP: grad_x
+    float* m;
+    float* v;
+    float lr_t, beta1, beta2, eps, gscale;
+    int mode;
+    int bs;
+};
+
+template <int K> struct LensAcc { static constexpr int n = 0; };
+template <> struct LensAcc<glm::K_EPL> { static constexpr int n = 8; };
+template <> struct LensAcc<glm::K_SHEAR> { static constexpr int n = 2; };
+template <> struct LensAcc<glm::K_SIS> { static constexpr int n = 3; };
+template <int K> struct LightAcc { static constexpr int n = (K == glm::K_SERSIC || K == glm::K_SERSIC_ELLIPSE) ? 8 : 0; };
+
+template <int L0, int L1, int L2, int A0, int A1, int S0, int S1>
+struct ModelT {
+    static constexpr int lens[3] = {L0, L1, L2};
+    static constexpr int light[4] = {A0, A1, S0, S1};  // lens light then source
+    static constexpr int NL = (L0 != 0) + (L1 != 0) + (L2 != 0);
+    static constexpr int NLL = (A0 != 0) + (A1 != 0);
+    static constexpr int NS = (S0 != 0) + (S1 != 0);
+    static constexpr int lens_off(int i) {
+        return i == 0 ? 0 : i == 1 ? LensAcc<L0>::n : LensAcc<L0>::n + LensAcc<L1>::n;
+    }
+    static constexpr int NLENSACC = LensAcc<L0>::n + LensAcc<L1>::n + LensAcc<L2>::n;
+    static constexpr int light_off(int i) { return NLENSACC + 8 * i; }
+    static constexpr int NACC = NLENSACC + LightAcc<A0>::n + LightAcc<A1>::n + LightAcc<S0>::n + LightAcc<S1>::n;
+    static constexpr int NRED = NACC + 2;  // + log-likelihood, chi^2
+};
+
+// Component index (in the plan's component table) of light slot i of the model.
+template <class M>
+__device__ __forceinline__ int light_comp_index(int i) {
+    // lens-light slots map to comp[NL + i], source slots to comp[NL + NLL + (i-2)]
+    return i < 2 ? M::NL + i : M::NL + M::NLL + (i - 2);
+}
+
+// ---------------------------------------------------------------------------
+// Conv + pool (forward) for one strip of RB outputs in row I.
+// ---------------------------------------------------------------------------
+template <int SS, int KF, int RB>
+__device__ __forceinline__ void conv_strip_static(const float* __restrict__ sub, const float* __restrict__ w,
+                                                  int PW, int I, int J0, float* acc) {
+    constexpr int SEG = SS * (RB - 1) + KF;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) acc[r] = 0.0f;
+    const float* base = sub + (SS * I) * PW + SS * J0;
+    for (int p = 0; p < KF; ++p) {
+        const float* row = base + p * PW;
+        float seg[SEG];
+#pragma unroll
+        for (int c = 0; c < SEG; ++c) seg[c] = row[c];
+        const float* wp = w + p * KF;
+#pragma unroll
+        for (int q = 0; q < KF; ++q) {
+            float wq = wp[q];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc[r] = fmaf(wq, seg[SS * r + q], acc[r]);
+        }
+    }
+}
+
+template <int RB>
+__device__ __forceinline__ void conv_strip_dyn(const float* __restrict__ sub, const float* __restrict__ w, int PW,
+                                               int ss, int kf, int I, int J0, float* acc) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r) acc[r] = 0.0f;
+    const float* base = sub + (ss * I) * PW + ss * J0;
+    for (int p = 0; p < kf; ++p) {
+        const float* row = base + p * PW;
+        const float* wp = w + p * kf;
+        for (int q = 0; q < kf; ++q) {
+            float wq = wp[q];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc[r] = fmaf(wq, row[ss * r + q], acc[r]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint conv for one strip of RB sub-pixels of phase (a, b):
+//   gs[r] = sum_{m,n} W[ss m + ra][ss n + rb] G[Y + da - m][X0 + r + db - n]
+// G: padded adjoint image, idx(I,J) = (I+gp0)*GW + (J+gp0).
+// ---------------------------------------------------------------------------
+template <int SS, int KF, int RB>
+__device__ __forceinline__ void adj_strip_static(const float* __restrict__ G, const float* __restrict__ w, int GW,
+                                                 int gp0, int a, int b, int Y, int X0, float* gs) {
+    constexpr int PB = (KF - SS) / 2;  // (k-1)/2 with k = KF - SS + 1
+    const int ra = (a + PB) % SS, da = (a + PB) / SS;
+    const int rb = (b + PB) % SS, db = (b + PB) / SS;
+    const int Ma = (KF - ra + SS - 1) / SS;
+    constexpr int MMAX = (KF + SS - 1) / SS;
+    const int Mb = (KF - rb + SS - 1) / SS;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) gs[r] = 0.0f;
+    for (int m = 0; m < Ma; ++m) {
+        const float* row = G + (Y + da - m + gp0) * GW + (X0 + db - (Mb - 1) + gp0);
+        const float* wr = w + (SS * m + ra) * KF + rb;
+        float seg[RB + MMAX - 1];
+#pragma unroll
+        for (int j = 0; j < RB + MMAX - 1; ++j) seg[j] = (j < RB + Mb - 1) ? row[j] : 0.0f;
+#pragma unroll
+        for (int n = 0; n < MMAX; ++n) {
+            if (n < Mb) {
+                float wq = wr[SS * n];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) gs[r] = fmaf(wq, seg[r + Mb - 1 - n], gs[r]);
+            }
+        }
+    }
+}
+
+template <int RB>
+__device__ __forceinline__ void adj_strip_dyn(const float* __restrict__ G, const float* __restrict__ w, int GW,
+                                              int gp0, int ss, int kf, int pb, int a, int b, int Y, int X0,
+                                              float* gs) {
+    const int ra = (a + pb) % ss, da = (a + pb) / ss;
+    const int rb = (b + pb) % ss, db = (b + pb) / ss;
+    const int Ma = (kf - ra + ss - 1) / ss;
+    const int Mb = (kf - rb + ss - 1) / ss;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) gs[r] = 0.0f;
+    for (int m = 0; m < Ma; ++m) {
+        const float* row = G + (Y + da - m + gp0) * GW + (X0 + db + gp0);
+        const float* wr = w + (ss * m + ra) * kf + rb;
+        for (int n = 0; n < Mb; ++n) {
+            float wq = wr[ss * n];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) gs[r] = fmaf(wq, row[r - n], gs[r]);
+        }
+    }
+}
+
+// Light slot i (0,1 lens light; 2,3 source) -> accumulator slot index.
+// Lens-light slots occupy 0..NLL-1, source slots NLL..NLL+NS-1.
+template <class M>
+__host__ __device__ constexpr int light_acc_slot(int i) {
+    return i < 2 ? i : M::NLL + (i - 2);
+}
+
+// ---------------------------------------------------------------------------
+// Per-sub-pixel model evaluation with the slot structure of M.
+// ---------------------------------------------------------------------------
+template <class M>
+struct Consts {
+    glm::EplC epl[3];
+    glm::ShearC shr[3];
+    glm::SisC sis[3];
+    glm::SerC ser[4];
+    const float* coefA[3];
+    const float4* coefB[3];
+};
+
+template <class M>
+__device__ __forceinline__ void load_consts(const float* s_c, const float* s_coef, const DPlan& P, Consts<M>& C) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int k = M::lens[i];
+        if (k == glm::K_EPL) {
+            C.epl[i] = *reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
+            C.coefA[i] = s_coef + P.coef_off[i];
+            C.coefB[i] = reinterpret_cast<const float4*>(s_coef + P.coef_off[i] + ((P.max_n + 4) & ~3));
+        } else if (k == glm::K_SHEAR) {
+            C.shr[i] = *reinterpret_cast<const glm::ShearC*>(s_c + i * kConstStride);
+        } else if (k == glm::K_SIS) {
+            C.sis[i] = *reinterpret_cast<const glm::SisC*>(s_c + i * kConstStride);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (M::light[i] != 0) {
+            const int ci = light_comp_index<M>(i);
+            C.ser[i] = *reinterpret_cast<const glm::SerC*>(s_c + ci * kConstStride);
+        }
+    }
+}
+
+template <class M>
+__device__ __forceinline__ float eval_forward(const Consts<M>& C, float X, float Y) {
+    float val = 0.0f;
+    glm::SerState st;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], X, Y, st);
+    float bx = X, by = Y;
+    glm::EplState es;
+    float sis_st[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float ax = 0.0f, ay = 0.0f;
+        if (M::lens[i] == glm::K_EPL) glm::epl_forward<false>(C.epl[i], C.coefA[i], nullptr, X, Y, ax, ay, es);
+        else if (M::lens[i] == glm::K_SHEAR) glm::shear_forward(C.shr[i], X, Y, ax, ay);
+        else if (M::lens[i] == glm::K_SIS) glm::sis_forward(C.sis[i], X, Y, ax, ay, sis_st);
+        bx -= ax;
+        by -= ay;
+    }
+#pragma unroll
+    for (int i = 2; i < 4; ++i)
+        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], bx, by, st);
+    return val;
+}
+
+template <class M>
+__device__ __forceinline__ void eval_adjoint(const Consts<M>& C, float X, float Y, float g, float* acc) {
+    float val = 0.0f;
+    glm::SerState lst[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], X, Y, lst[i]);
+    float bx = X, by = Y;
+    glm::EplState es[3];
+    float sis_st[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float ax = 0.0f, ay = 0.0f;
+        if (M::lens[i] == glm::K_EPL) glm::epl_forward<true>(C.epl[i], C.coefA[i], C.coefB[i], X, Y, ax, ay, es[i]);
+        else if (M::lens[i] == glm::K_SHEAR) glm::shear_forward(C.shr[i], X, Y, ax, ay);
+        else if (M::lens[i] == glm::K_SIS) glm::sis_forward(C.sis[i], X, Y, ax, ay, sis_st[i]);
+        bx -= ax;
+        by -= ay;
+    }
+#pragma unroll
+    for (int i = 2; i < 4; ++i)
+        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], bx, by, lst[i]);
+    if (isnan(val)) return;  // zeroed in the forward pass: no contribution
+    float gbx = 0.0f, gby = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (M::light[i] != 0) {
+            float gdx, gdy;
+            glm::sersic_adjoint(C.ser[i], lst[i], g, acc + M::light_off(light_acc_slot<M>(i)), gdx, gdy);
+            if (i >= 2) { gbx += gdx; gby += gdy; }
+        }
+    }
+    const float Hx = -gbx, Hy = -gby;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (M::lens[i] == glm::K_EPL) glm::epl_adjoint(C.epl[i], es[i], Hx, Hy, acc + M::lens_off(i));
+        else if (M::lens[i] == glm::K_SHEAR) glm::shear_adjoint(X, Y, Hx, Hy, acc + M::lens_off(i));
+        else if (M::lens[i] == glm::K_SIS) glm::sis_adjoint(C.sis[i], sis_st[i], Hx, Hy, acc + M::lens_off(i));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The kernel
+// ---------------------------------------------------------------------------
+constexpr int kRB = 4;    // outputs per thread strip (forward conv)
+constexpr int kRB2 = 4;   // sub-pixels per thread strip (adjoint conv)
+constexpr int kMaxT = 4;  // forward strips per thread held in registers
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int BLOCK, class M, bool GRAD>
+__global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KArgs A) {
+    extern __shared__ float smem[];
+    const int pid = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int d = P.d;
+    float* s_sub = smem;
+    float* s_w = smem + P.off_w;
+    float* s_x = smem + P.off_x;  // [x | dxdz | dlpdx | dfldjdz | gx | lpj]
+    float* s_c = smem + P.off_c;
+    float* s_coef = smem + P.off_coef;
+    float* s_red = smem + P.off_red;
+    const bool use_z = (A.mode == M_LOGPROB || A.mode == M_MAP);
+
+    // ---- A: prologue -----------------------------------------------------
+    for (int i = tid; i < P.kf * P.kf; i += BLOCK) s_w[i] = P.w[i];
+    for (int i = tid; i < P.sub_size; i += BLOCK) s_sub[i] = 0.0f;
+    if (tid < d) {
+        float zin = A.in[(size_t)pid * d + tid];
+        glm::PriorOut o;
+        if (use_z) {
+            glm::prior_eval(P.prior_kind[tid], P.prior_args + 8 * tid, zin, P.fldj_at_x, o);
+        } else {
+            o.x = zin; o.dxdz = 1.0f; o.lp = 0.0f; o.dlpdx = 0.0f; o.fldj = 0.0f; o.dfldjdz = 0.0f;
+        }
+        s_x[tid] = o.x;
+        s_x[d + tid] = o.dxdz;
+        s_x[2 * d + tid] = o.dlpdx;
+        s_x[3 * d + tid] = o.dfldjdz;
+        s_x[4 * d + tid] = 0.0f;
+        s_x[5 * d + tid] = o.lp + o.fldj;
+    }
+    __syncthreads();
+    if (tid < P.ncomp) {
+        const DComp& cd = P.comp[tid];
+        float pv[GL_MAX_COMPONENT_PARAMS];
+#pragma unroll
+        for (int j = 0; j < GL_MAX_COMPONENT_PARAMS; ++j) pv[j] = cd.col[j] >= 0 ? s_x[cd.col[j]] : cd.cst[j];
+        float* dst = s_c + tid * kConstStride;
+        if (cd.kind == glm::K_EPL) {
+            glm::EplC c;
+            int nov = P.batch_n ? P.batch_n[tid] : -1;
+            glm::epl_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.niter, nov, c);
+            if (c.N > P.max_n) c.N = P.max_n;
+            *reinterpret_cast<glm::EplC*>(dst) = c;
+        } else if (cd.kind == glm::K_SHEAR) {
+            glm::ShearC c{pv[0], pv[1]};
+            *reinterpret_cast<glm::ShearC*>(dst) = c;
+        } else if (cd.kind == glm::K_SIS) {
+            glm::SisC c{pv[0], pv[1], pv[2]};
+            *reinterpret_cast<glm::SisC*>(dst) = c;
+        } else if (cd.kind == glm::K_SERSIC_ELLIPSE) {
+            glm::SerC c;
+            glm::sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.lstsq ? 1.0f : pv[6], 1, c);
+            *reinterpret_cast<glm::SerC*>(dst) = c;
+        } else if (cd.kind == glm::K_SERSIC) {
+            glm::SerC c;
+            glm::sersic_prepare(pv[0], pv[1], 0.0f, 0.0f, pv[2], pv[3], cd.lstsq ? 1.0f : pv[4], 0, c);
+            *reinterpret_cast<glm::SerC*>(dst) = c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (M::lens[i] == glm::K_EPL) {
+            const glm::EplC* c = reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
+            float* ca = s_coef + P.coef_off[i];
+            float4* cb = reinterpret_cast<float4*>(ca + ((P.max_n + 4) & ~3));
+            for (int n = tid; n <= c->N; n += BLOCK) {
+                float cv, dt, df;
+                glm::epl_coef(n, (double)c->f, (double)c->t, cv, dt, df);
+                ca[n] = cv;
+                cb[n] = make_float4(cv, dt, df, 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+
+    Consts<M> C;
+    load_consts<M>(s_c, s_coef, P, C);
+
+    const int N = P.N, ss = P.ss, nsub = P.nsub, PW = P.PW;
+    const float* sub0 = s_sub + P.sub_base;  // sub0[(y+pb)*PW + x]
+
+    // ---- B: ray-trace into LDS -----------------------------------------------
+    {
+        int i = tid / nsub, j = tid - (tid / nsub) * nsub;
+        const int nsub2 = nsub * nsub;
+        for (int s = tid; s < nsub2; s += BLOCK) {
+            float X = fmaf(P.M00, (float)j, fmaf(P.M01, (float)i, P.ox));
+            float Y = fmaf(P.M10, (float)j, fmaf(P.M11, (float)i, P.oy));
+            float val = eval_forward<M>(C, X, Y);
+            if (isnan(val)) val = 0.0f;
+            s_sub[P.sub_base + (i + P.pb) * PW + j] = val;
+            j += BLOCK;
+            while (j >= nsub) { j -= nsub; ++i; }
+        }
+    }
+    __syncthreads();
+
+    // ---- C: conv + pool + likelihood ------------------------------------------
+    const int NSTR = (N + kRB - 1) / kRB;
+    const int ntask = N * NSTR;
+    float gv[kMaxT][kRB];
+    float ll_acc = 0.0f, chi_acc = 0.0f;
+    const float cf = P.cf;
+    const float inv_np = 1.0f / (float)(N * N);
+    const float glp = A.mode == M_MAP ? A.gscale : (A.g_lp ? A.g_lp[pid] : 1.0f);
+    const float gch = (A.mode == M_LOGPROB && A.g_chi2) ? A.g_chi2[pid] : 0.0f;
+    const int ssk = ss * 1000 + P.kf;
+#pragma unroll
+    for (int tt = 0; tt < kMaxT; ++tt) {
+        const int task = tid + tt * BLOCK;
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) gv[tt][r] = 0.0f;
+        if (task < ntask) {
+            const int I = task / NSTR;
+            const int J0 = (task - I * NSTR) * kRB;
+            float acc[kRB];
+            const float* base = s_sub + P.sub_base - P.pb;  // column offset -pb folded in
+            if (ssk == 2022) conv_strip_static<2, 22, kRB>(base, s_w, PW, I, J0, acc);
+            else if (ssk == 2026) conv_strip_static<2, 26, kRB>(base, s_w, PW, I, J0, acc);
+            else if (ssk == 1021) conv_strip_static<1, 21, kRB>(base, s_w, PW, I, J0, acc);
+            else conv_strip_dyn<kRB>(base, s_w, PW, ss, P.kf, I, J0, acc);
+#pragma unroll
+            for (int r = 0; r < kRB; ++r) {
+                const int J = J0 + r;
+                if (J < N) {
+                    const float im = cf * acc[r];
+                    const int pix = I * N + J;
+                    if (A.mode == M_SIM) {
+                        A.img[(size_t)pid * N * N + pix] = im;
+                    } else if (A.mode == M_VJP) {
+                        gv[tt][r] = cf * A.g_img[(size_t)pid * N * N + pix];
+                    } else {
+                        float ll, r2, dll, dr2;
+                        if (P.lik == GL_LIK_FORWARD)
+                            glm::lik_forward(im, P.obs[pix], P.bkg2, P.inv_texp, ll, r2, dll, dr2);
+                        else
+                            glm::lik_backward(im, P.obs[pix], P.inv_var_obs[pix], P.log_sig_obs[pix], ll, r2, dll,
+                                              dr2);
+                        ll_acc += ll;
+                        chi_acc += r2;
+                        gv[tt][r] = cf * (glp * dll + gch * inv_np * dr2);
+                    }
+                }
+            }
+        }
+    }
+
+    if (A.mode == M_SIM) return;
+
+    float acc[M::NACC > 0 ? M::NACC : 1];
+#pragma unroll
+    for (int e = 0; e < M::NACC; ++e) acc[e] = 0.0f;
+
+    if (GRAD && A.mode != M_SIM) {
+        __syncthreads();
+        for (int i = tid; i < P.g_size; i += BLOCK) s_sub[i] = 0.0f;
+        __syncthreads();
+        const int GW = P.GW, gp0 = P.gp0;
+#pragma unroll
+        for (int tt = 0; tt < kMaxT; ++tt) {
+            const int task = tid + tt * BLOCK;
+            if (task < ntask) {
+                const int I = task / NSTR;
+                const int J0 = (task - I * NSTR) * kRB;
+#pragma unroll
+                for (int r = 0; r < kRB; ++r)
+                    if (J0 + r < N) s_sub[(I + gp0) * GW + (J0 + r + gp0)] = gv[tt][r];
+            }
+        }
+        __syncthreads();
+
+        // ---- D + E: adjoint conv (polyphase) and recompute ---------------------
+        const int XSTR = (N + kRB2 - 1) / kRB2;
+        const int ptask = N * XSTR;
+        for (int a = 0; a < ss; ++a) {
+            for (int b = 0; b < ss; ++b) {
+                for (int task = tid; task < ptask; task += BLOCK) {
+                    const int Yc = task / XSTR;
+                    const int X0 = (task - Yc * XSTR) * kRB2;
+                    float gs[kRB2];
+                    if (ssk == 2022) adj_strip_static<2, 22, kRB2>(s_sub, s_w, GW, gp0, a, b, Yc, X0, gs);
+                    else if (ssk == 2026) adj_strip_static<2, 26, kRB2>(s_sub, s_w, GW, gp0, a, b, Yc, X0, gs);
+                    else if (ssk == 1021) adj_strip_static<1, 21, kRB2>(s_sub, s_w, GW, gp0, a, b, Yc, X0, gs);
+                    else adj_strip_dyn<kRB2>(s_sub, s_w, GW, gp0, ss, P.kf, P.pb, a, b, Yc, X0, gs);
+                    const int yi = ss * Yc + a;
+#pragma unroll
+                    for (int r = 0; r < kRB2; ++r) {
+                        const int xj = ss * (X0 + r) + b;
+                        if (X0 + r < N) {
+                            float X = fmaf(P.M00, (float)xj, fmaf(P.M01, (float)yi, P.ox));
+                            float Y = fmaf(P.M10, (float)xj, fmaf(P.M11, (float)yi, P.oy));
+                            eval_adjoint<M>(C, X, Y, gs[r], acc);
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- F: reduction ------------------------------------------------------
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = BLOCK / 64;
+    float part = wave_sum(ll_acc);
+    if (lane == 0) s_red[wave * M::NRED + 0] = part;
+    part = wave_sum(chi_acc);
+    if (lane == 0) s_red[wave * M::NRED + 1] = part;
+    if (GRAD && A.mode != M_SIM) {
+#pragma unroll
+        for (int e = 0; e < M::NACC; ++e) {
+            part = wave_sum(acc[e]);
+            if (lane == 0) s_red[wave * M::NRED + 2 + e] = part;
+        }
+    }
+    __syncthreads();
+    float* s_tot = s_red + NW * M::NRED;
+    if (tid < M::NRED) {
+        float t = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += s_red[w * M::NRED + tid];
+        s_tot[tid] = t;
+    }
+    __syncthreads();
+
+    if (tid == 0) {
+        float lp_like = s_tot[0] - (float)(N * N) * glm::kHalfLn2Pi;
+        float chi2 = s_tot[1] * inv_np;
+        float lp_prior = 0.0f;
+        for (int j = 0; j < d; ++j) lp_prior += s_x[5 * d + j];
+        if (A.mode == M_LOGPROB || A.mode == M_MAP) {
+            if (A.lp) A.lp[pid] = lp_like + lp_prior;
+            if (A.chi2) A.chi2[pid] = chi2;
+        }
+        if (GRAD && A.mode != M_SIM) {
+            const float* tot = s_tot + 2;
+            float* gx = s_x + 4 * d;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int k = M::lens[i];
+                if (k == 0) continue;
+                const DComp& cd = P.comp[i];
+                float g[8];
+                int np = 0;
+                if (k == glm::K_EPL) { glm::epl_finish(C.epl[i], tot + M::lens_off(i), g); np = 6; }
+                else if (k == glm::K_SHEAR) { g[0] = tot[M::lens_off(i)]; g[1] = tot[M::lens_off(i) + 1]; np = 2; }
+                else if (k == glm::K_SIS) { glm::sis_finish(tot + M::lens_off(i), g); np = 3; }
+                for (int p = 0; p < np; ++p)
+                    if (cd.col[p] >= 0) gx[cd.col[p]] += g[p];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (M::light[i] == 0) continue;
+                const DComp& cd = P.comp[light_comp_index<M>(i)];
+                float g[8];
+                glm::sersic_finish(C.ser[i], tot + M::light_off(light_acc_slot<M>(i)), g);
+                if (M::light[i] == glm::K_SERSIC_ELLIPSE) {
+                    for (int p = 0; p < (cd.lstsq ? 6 : 7); ++p)
+                        if (cd.col[p] >= 0) gx[cd.col[p]] += g[p];
+                } else {
+                    const int map[5] = {0, 1, 4, 5, 6};
+                    for (int p = 0; p < (cd.lstsq ? 4 : 5); ++p)
+                        if (cd.col[p] >= 0) gx[cd.col[p]] += g[map[p]];
+                }
+            }
+        }
+    }
+    if (GRAD && A.mode != M_SIM) {
+        __syncthreads();
+        if (tid < d) {
+            const float gxj = s_x[4 * d + tid];
+            if (A.mode == M_VJP) {
+                A.grad[(size_t)pid * d + tid] = gxj;
+            } else {
+                const float gz = (gxj + glp * s_x[2 * d + tid]) * s_x[d + tid] + glp * s_x[3 * d + tid];
+                if (A.mode == M_LOGPROB) {
+                    A.grad[(size_t)pid * d + tid] = gz;
+                } else {  // M_MAP: Keras Adam
+                    const size_t o = (size_t)pid * d + tid;
+                    float mm = A.beta1 * A.m[o] + (1.0f - A.beta1) * gz;
+                    float vv = A.beta2 * A.v[o] + (1.0f - A.beta2) * gz * gz;
+                    A.m[o] = mm;
+                    A.v[o] = vv;
+                    A.z_io[o] = A.in[o] - A.lr_t * mm / (sqrtf(vv) + A.eps);
+                }
+            }
+        }
+    }
+}
+
+// Batch-coupled EPL series length (epl.py:41): one block reduces max f over
+// the batch for every EPL component and stores the loop count.
+__global__ void epl_batch_series_kernel(const DPlan P, const float* in, int bs, int use_z, int* out) {
+    __shared__ float red[256];
+    for (int ci = 0; ci < P.nlens; ++ci) {
+        const DComp& cd = P.comp[ci];
+        if (cd.kind != glm::K_EPL) {
+            if (threadIdx.x == 0) out[ci] = -1;
+            continue;
+        }
+        float fmax = -1.0f;
+        bool any_nan = false;
+        for (int p = threadIdx.x; p < bs; p += blockDim.x) {
+            float pv[4];
+            for (int j = 2; j < 4; ++j) {
+                int col = cd.col[j];
+                float v = col >= 0 ? in[(size_t)p * P.d + col] : cd.cst[j];
+                if (col >= 0 && use_z) {
+                    glm::PriorOut o;
+                    glm::prior_eval(P.prior_kind[col], P.prior_args + 8 * col, v, 0, o);
+                    v = o.x;
+                }
+                pv[j] = v;
+            }
+            float c = fminf(fmaxf(sqrtf(pv[2] * pv[2] + pv[3] * pv[3]), 0.0f), 1.0f);
+            float q = (1.0f - c) / (1.0f + c);
+            float f = (1.0f - q) / (1.0f + q);
+            if (isnan(f)) any_nan = true;
+            fmax = fmaxf(fmax, f);
+        }
+        if (any_nan) fmax = NAN;
+        red[threadIdx.x] = fmax;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                float a = red[threadIdx.x], b = red[threadIdx.x + s];
+                red[threadIdx.x] = (isnan(a) || isnan(b)) ? NAN : fmaxf(a, b);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[ci] = glm::epl_series_len((double)red[0], cd.niter);
+        __syncthreads();
+    }
+}
+
+}  // namespace gl

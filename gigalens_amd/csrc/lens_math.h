@@ -1,0 +1,477 @@
+// Per-particle and per-sub-pixel arithmetic of the lens model, forward and
+// adjoint.  All functions are __host__ __device__ so the same code is unit
+// tested on the CPU and used by the gfx950 kernels.
+//
+// Reference semantics (file:line in the reference repository):
+//   EPL            src/gigalens/tf/profiles/mass/epl.py:23-68
+//   Shear          src/gigalens/tf/profiles/mass/shear.py:12-14
+//   SIS            src/gigalens/tf/profiles/mass/sis.py:14-18
+//   Sersic(Ellip.) src/gigalens/tf/profiles/light/sersic.py:21-72
+//   likelihood     src/gigalens/tf/model.py:77-86, 113-118, 159-163
+//   prior/bijector src/gigalens/tf/model.py:40-51,75,81-83 (TFP default event-space bijectors)
+//
+// Formulation (own design; the reference evaluates these as unfused tensor ops):
+//  * EPL: the series sum_n Omega_n = e^{i phi} sum_n c_n w^n with w = e^{2 i phi}
+//    and per-particle real coefficients c_n is evaluated by a Clenshaw
+//    recurrence (2 flops / term); cos/sin phi come from q x / R and y / R
+//    instead of atan2/cos/sin; (b/R)^(t-1) is one exp2/log2 pair.
+//  * Adjoint: reverse mode with per-particle "sufficient statistics": every
+//    sub-pixel adds a handful of scalars (e.g. h . alpha, h . alpha log R) to
+//    per-thread accumulators, and the per-particle Jacobians (d/de1, d/dq, ...)
+//    are applied once in the epilogue.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define GLM __host__ __device__ __forceinline__
+
+namespace glm {
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLn2 = 0.6931471805599453f;
+constexpr float kHalfLn2Pi = 0.91893853320467274f;
+
+// ---- transcendental wrappers: hardware instructions on the device --------
+GLM float ex2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_exp2f(x);
+#else
+    return exp2f(x);
+#endif
+}
+GLM float lg2(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_logf(x);
+#else
+    return log2f(x);
+#endif
+}
+GLM float rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+GLM float rsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Component constants (prepared once per particle)
+// ---------------------------------------------------------------------------
+enum Kind { K_EPL = 1, K_SHEAR = 2, K_SIS = 3, K_SERSIC = 10, K_SERSIC_ELLIPSE = 11, K_SHAPELETS = 12 };
+
+// float slots of a component's constant block
+struct EplC {
+    float cx, cy, cphi, sphi, q, f, t, tm1, K0;
+    float thetaE, e1, e2, c, lnb, craw;
+    int N;
+};
+struct SerC {
+    float cx, cy, Axx, Axy, Ayy, k1, k0, m1, m0, Ie;
+    float bn, n, Rs, e1, e2, q, cphi, sphi, c;
+    int elliptical;
+};
+struct ShearC { float g1, g2; };
+struct SisC { float thetaE, cx, cy; };
+
+// Series length rule of epl.py:41,51-58 for a given f: the loop runs for
+// n = 1, 2, ... while n < log(1e-12)/log(f) + 2, at most niter times.
+GLM int epl_series_len(double f, int niter) {
+    if (!(f > 0.0)) {
+        // log(0) = -inf -> limit 2 -> one iteration; NaN -> zero iterations
+        if (f == 0.0) return niter < 1 ? niter : 1;
+        return 0;
+    }
+    double lim = log(1e-12) / log(f) + 2.0;  // f<1: positive; f>=1: -inf / negative
+    if (!(lim > 1.0)) return 0;
+    double n = ceil(lim) - 1.0;
+    if (n > niter) n = niter;
+    return (int)n;
+}
+
+// EPL constants, epl.py:25-30,37,59.  b = theta_E sqrt(q) is the algebraic
+// simplification of epl.py:28-29.
+GLM void epl_prepare(float thetaE, float gamma, float e1, float e2, float cx, float cy, int niter,
+                     int n_override, EplC& C) {
+    float phi = atan2f(e2, e1) * 0.5f;
+    float craw = sqrtf(e1 * e1 + e2 * e2);
+    float c = fminf(fmaxf(craw, 0.0f), 1.0f);
+    float q = (1.0f - c) / (1.0f + c);
+    float b = thetaE * sqrtf(q);
+    float t = gamma - 1.0f;
+    C.cx = cx; C.cy = cy;
+    C.cphi = cosf(phi); C.sphi = sinf(phi);
+    C.q = q;
+    C.f = (1.0f - q) / (1.0f + q);
+    C.t = t; C.tm1 = t - 1.0f;
+    C.lnb = logf(b);
+    C.K0 = log2f(2.0f / (1.0f + q)) + t * log2f(b);
+    C.thetaE = thetaE; C.e1 = e1; C.e2 = e2; C.c = c; C.craw = craw;
+    C.N = n_override >= 0 ? n_override : epl_series_len((double)C.f, niter);
+}
+
+// Coefficient n of the three series (value, d/dt, d/df):
+//   c_n   = prod_{k<=n} (-f)(2k-2+t)/(2k+2-t),  c_0 = 1
+//   dt_n  = d c_n / dt,  df_n = d c_n / df  (recurrences avoid 1/f, 1/(2k-2+t)).
+GLM void epl_coef(int n, double f, double t, float& c, float& dt, float& df) {
+    double a = 1.0, da = 0.0, pw = 1.0;  // a_n (without f), d a_n/dt, (-f)^(n-1)
+    for (int k = 1; k <= n; ++k) {
+        double den = 2.0 * k + 2.0 - t;
+        double r = (2.0 * k - 2.0 + t) / den;
+        double dr = 4.0 * k / (den * den);
+        da = da * r + a * dr;
+        a = a * r;
+        if (k > 1) pw *= -f;
+    }
+    if (n == 0) { c = 1.0f; dt = 0.0f; df = 0.0f; return; }
+    c = (float)(-f * pw * a);
+    dt = (float)(-f * pw * da);
+    df = (float)(-(double)n * pw * a);
+}
+
+// Sersic constants, sersic.py:33-59 (no clip on |e|), sersic.py:66-72.
+GLM void sersic_prepare(float Rs, float n, float e1, float e2, float cx, float cy, float Ie, int elliptical,
+                        SerC& C) {
+    C.cx = cx; C.cy = cy; C.Rs = Rs; C.n = n; C.Ie = Ie; C.e1 = e1; C.e2 = e2;
+    C.elliptical = elliptical;
+    if (elliptical) {
+        float phi = atan2f(e2, e1) * 0.5f;
+        float c = sqrtf(e1 * e1 + e2 * e2);
+        float q = (1.0f - c) / (1.0f + c);
+        float cp = cosf(phi), sp = sinf(phi);
+        C.Axx = q * cp * cp + sp * sp / q;
+        C.Ayy = q * sp * sp + cp * cp / q;
+        C.Axy = cp * sp * (q - 1.0f / q);
+        C.q = q; C.cphi = cp; C.sphi = sp; C.c = c;
+    } else {
+        C.Axx = 1.0f; C.Ayy = 1.0f; C.Axy = 0.0f;
+        C.q = 1.0f; C.cphi = 1.0f; C.sphi = 0.0f; C.c = 0.0f;
+    }
+    C.bn = 1.9992f * n - 0.3271f;
+    C.k1 = 0.5f / n;
+    C.k0 = -log2f(Rs) / n;
+    C.m1 = -C.bn * kLog2e;
+    C.m0 = C.bn * kLog2e;
+}
+
+// ---------------------------------------------------------------------------
+// Per sub-pixel forward
+// ---------------------------------------------------------------------------
+struct EplState {
+    float u, v, qu, r2, r2c, L, A, cp, sp, inrange;
+    float Fr, Fi, aLu, aLv;
+    float Etr, Eti, Efr, Efi;
+};
+
+// Clenshaw evaluation of sum_{n=0..N} c_n w^n, w = cos th + i sin th.
+// coef: c_n (float).  Returns real/imag parts.
+GLM void clenshaw1(const float* __restrict__ coef, int N, float c2, float s2, float& re, float& im) {
+    float x2 = c2 + c2;
+    float b1 = 0.0f, b2 = 0.0f;
+    for (int k = N; k >= 1; --k) {
+        float b0 = fmaf(x2, b1, coef[k]) - b2;
+        b2 = b1;
+        b1 = b0;
+    }
+    re = fmaf(c2, b1, coef[0]) - b2;
+    im = s2 * b1;
+}
+
+// Three series at once; coef4[n] = (c_n, dc_n/dt, dc_n/df, 0).
+GLM void clenshaw3(const float4* __restrict__ coef4, int N, float c2, float s2, float& re, float& im,
+                   float& tre, float& tim, float& fre, float& fim) {
+    float x2 = c2 + c2;
+    float a1 = 0.f, a2 = 0.f, d1 = 0.f, d2 = 0.f, g1 = 0.f, g2 = 0.f;
+    for (int k = N; k >= 1; --k) {
+        float4 cc = coef4[k];
+        float a0 = fmaf(x2, a1, cc.x) - a2;
+        float d0 = fmaf(x2, d1, cc.y) - d2;
+        float g0 = fmaf(x2, g1, cc.z) - g2;
+        a2 = a1; a1 = a0; d2 = d1; d1 = d0; g2 = g1; g1 = g0;
+    }
+    re = fmaf(c2, a1, coef4[0].x) - a2;
+    im = s2 * a1;
+    tre = c2 * d1 - d2;  // dc_0/dt = 0
+    tim = s2 * d1;
+    fre = c2 * g1 - g2;
+    fim = s2 * g1;
+}
+
+// EPL deflection (image frame).  Fills the state needed by the adjoint when
+// coef4 != nullptr (then the two derivative series are evaluated too).
+template <bool ADJ>
+GLM void epl_forward(const EplC& C, const float* __restrict__ coef, const float4* __restrict__ coef4, float X,
+                     float Y, float& ax, float& ay, EplState& S) {
+    float dx = X - C.cx, dy = Y - C.cy;
+    float u = fmaf(dx, C.cphi, dy * C.sphi);
+    float v = fmaf(-dx, C.sphi, dy * C.cphi);
+    float qu = C.q * u;
+    float r2 = fmaf(qu, qu, v * v);
+    float cp, sp;
+    if (r2 > 0.0f) {
+        float ir = rsq(r2);
+        cp = qu * ir;
+        sp = v * ir;
+    } else {  // atan2(0, +-0)
+        cp = signbit(qu) ? -1.0f : 1.0f;
+        sp = 0.0f;
+    }
+    float r2c = fminf(fmaxf(r2, 1e-20f), 1e20f);
+    float L = lg2(r2c);
+    float A = ex2(fmaf(-C.tm1 * 0.5f, L, C.K0));
+    float c2 = fmaf(cp, cp, -sp * sp);
+    float s2 = 2.0f * cp * sp;
+    float gr, gi;
+    if (ADJ) {
+        float tr, ti, fr, fi;
+        clenshaw3(coef4, C.N, c2, s2, gr, gi, tr, ti, fr, fi);
+        S.Etr = cp * tr - sp * ti; S.Eti = sp * tr + cp * ti;
+        S.Efr = cp * fr - sp * fi; S.Efi = sp * fr + cp * fi;
+    } else {
+        clenshaw1(coef, C.N, c2, s2, gr, gi);
+    }
+    float Fr = cp * gr - sp * gi;
+    float Fi = sp * gr + cp * gi;
+    float aLu = A * Fr, aLv = A * Fi;
+    ax = aLu * C.cphi - aLv * C.sphi;
+    ay = aLu * C.sphi + aLv * C.cphi;
+    if (ADJ) {
+        S.u = u; S.v = v; S.qu = qu; S.r2 = r2; S.r2c = r2c; S.L = L; S.A = A; S.cp = cp; S.sp = sp;
+        S.inrange = (r2 >= 1e-20f && r2 <= 1e20f) ? 1.0f : 0.0f;
+        S.Fr = Fr; S.Fi = Fi; S.aLu = aLu; S.aLv = aLv;
+    }
+}
+
+// EPL adjoint accumulation; H = dL/d alpha_image at this sub-pixel.
+// acc[0..7] = (sum h.a, sum h.a log2 R^2, S_t, S_f, S_q, S_u, S_v, S_phi).
+GLM void epl_adjoint(const EplC& C, const EplState& S, float Hx, float Hy, float* acc) {
+    float hu = fmaf(Hx, C.cphi, Hy * C.sphi);
+    float hv = fmaf(-Hx, C.sphi, Hy * C.cphi);
+    float sa = fmaf(hu, S.aLu, hv * S.aLv);
+    acc[0] += sa;
+    acc[1] = fmaf(sa, S.L, acc[1]);
+    acc[2] = fmaf(S.A, fmaf(hu, S.Etr, hv * S.Eti), acc[2]);
+    float sf = fmaf(hu, S.Efr, hv * S.Efi);
+    acc[3] = fmaf(S.A, sf, acc[3]);
+    float f2 = C.f + C.f;
+    float Yr = fmaf(f2, S.Efr, S.Fr);
+    float Yi = fmaf(f2, S.Efi, S.Fi);
+    float sphi = S.A * fmaf(-hu, Yi, hv * Yr);
+    float ir2 = S.r2 > 0.0f ? rcp(S.r2) : 0.0f;
+    float sR = (1.0f - C.t) * sa * S.inrange;
+    float dLdu = (sR * C.q * S.qu - sphi * C.q * S.v) * ir2;
+    float dLdv = (sR * S.v + sphi * S.qu) * ir2;
+    acc[4] = fmaf((sR * S.qu - sphi * S.v) * S.u, ir2, acc[4]);
+    acc[5] += dLdu;
+    acc[6] += dLdv;
+    acc[7] += dLdu * S.v - dLdv * S.u + hv * S.aLu - hu * S.aLv;
+}
+
+// Epilogue: statistics -> d/d(theta_E, gamma, e1, e2, center_x, center_y).
+GLM void epl_finish(const EplC& C, const float* acc, float* g) {
+    float t = C.t, q = C.q;
+    float dK = acc[0];
+    g[0] = dK * t / C.thetaE;
+    g[1] = dK * C.lnb - 0.5f * kLn2 * acc[1] + acc[2];
+    float dq = dK * (t / (2.0f * q) - 1.0f / (1.0f + q)) + acc[4];
+    float dc = (C.craw <= 1.0f) ? (dq * (-2.0f / ((1.0f + C.c) * (1.0f + C.c))) + acc[3]) : 0.0f;
+    float dphi = acc[7];
+    float cc = C.e1 * C.e1 + C.e2 * C.e2;
+    float inv2c2 = 0.5f / cc;
+    float ic = 1.0f / C.craw;
+    g[2] = dphi * (-C.e2 * inv2c2) + dc * C.e1 * ic;
+    g[3] = dphi * (C.e1 * inv2c2) + dc * C.e2 * ic;
+    g[4] = -(acc[5] * C.cphi - acc[6] * C.sphi);
+    g[5] = -(acc[5] * C.sphi + acc[6] * C.cphi);
+}
+
+GLM void shear_forward(const ShearC& C, float X, float Y, float& ax, float& ay) {
+    ax = fmaf(C.g1, X, C.g2 * Y);
+    ay = fmaf(C.g2, X, -C.g1 * Y);
+}
+GLM void shear_adjoint(float X, float Y, float Hx, float Hy, float* acc) {
+    acc[0] += Hx * X - Hy * Y;
+    acc[1] += Hx * Y + Hy * X;
+}
+
+GLM void sis_forward(const SisC& C, float X, float Y, float& ax, float& ay, float* st) {
+    float dx = X - C.cx, dy = Y - C.cy;
+    float r2 = dx * dx + dy * dy;
+    float ir = r2 > 0.0f ? rsq(r2) : 0.0f;
+    float a = C.thetaE * ir;
+    ax = a * dx; ay = a * dy;
+    st[0] = dx * ir; st[1] = dy * ir; st[2] = ir;
+}
+// acc: (sum H.n, S_dx, S_dy) with dL/d(dx,dy) = thetaE/R (H - n (n.H))
+GLM void sis_adjoint(const SisC& C, const float* st, float Hx, float Hy, float* acc) {
+    float nh = st[0] * Hx + st[1] * Hy;
+    acc[0] += nh;
+    float s = C.thetaE * st[2];
+    acc[1] += s * (Hx - st[0] * nh);
+    acc[2] += s * (Hy - st[1] * nh);
+}
+GLM void sis_finish(const float* acc, float* g) {
+    g[0] = acc[0];
+    g[1] = -acc[1];
+    g[2] = -acc[2];
+}
+
+struct SerState { float dx, dy, gx, gy, Q, LQ, P, I; };
+
+GLM float sersic_forward(const SerC& C, float X, float Y, SerState& S) {
+    float dx = X - C.cx, dy = Y - C.cy;
+    float gx = fmaf(C.Axx, dx, C.Axy * dy);
+    float gy = fmaf(C.Axy, dx, C.Ayy * dy);
+    float Q = fmaf(dx, gx, dy * gy);
+    float LQ = lg2(Q);
+    float P = ex2(fmaf(C.k1, LQ, C.k0));
+    float I = C.Ie * ex2(fmaf(C.m1, P, C.m0));
+    S.dx = dx; S.dy = dy; S.gx = gx; S.gy = gy; S.Q = Q; S.LQ = LQ; S.P = P; S.I = I;
+    return I;
+}
+
+// acc[0..7] = (S_I, S_P, S_PL, S_xx, S_xy, S_yy, S_dx, S_dy); returns dL/d(dx,dy)
+// of this sub-pixel in (gdx, gdy) (needed to chain a source into the lenses).
+GLM void sersic_adjoint(const SerC& C, const SerState& S, float g, float* acc, float& gdx, float& gdy) {
+    float gI = g * S.I;
+    float gP = gI * S.P;
+    acc[0] += gI;
+    acc[1] += gP;
+    acc[2] = fmaf(gP, S.LQ, acc[2]);
+    float w = gP * rcp(S.Q);
+    acc[3] = fmaf(w * S.dx, S.dx, acc[3]);
+    acc[4] = fmaf(w * S.dx, S.dy, acc[4]);
+    acc[5] = fmaf(w * S.dy, S.dy, acc[5]);
+    acc[6] = fmaf(w, S.gx, acc[6]);
+    acc[7] = fmaf(w, S.gy, acc[7]);
+    float Ks2 = -2.0f * C.bn * C.k1;
+    gdx = Ks2 * w * S.gx;
+    gdy = Ks2 * w * S.gy;
+}
+
+// Epilogue: g receives d/d(R_sersic, n_sersic, e1, e2, center_x, center_y, Ie)
+// (e1,e2 only for the elliptical profile).
+GLM void sersic_finish(const SerC& C, const float* acc, float* g) {
+    float bn = C.bn, n = C.n;
+    float dbn = -(acc[1] - acc[0]);
+    float dk0 = -bn * kLn2 * acc[1];
+    float dk1 = -bn * kLn2 * acc[2];
+    float Ks = -bn * C.k1;
+    g[0] = dk0 * (-1.0f / (n * C.Rs * kLn2));
+    g[1] = 1.9992f * dbn + dk1 * (-0.5f / (n * n)) + dk0 * (log2f(C.Rs) / (n * n));
+    if (C.elliptical) {
+        float dAxx = Ks * acc[3], dAxy = 2.0f * Ks * acc[4], dAyy = Ks * acc[5];
+        float q = C.q, cp = C.cphi, sp = C.sphi;
+        float s2p = 2.0f * sp * cp, c2p = cp * cp - sp * sp;
+        float iq = 1.0f / q, iq2 = iq * iq;
+        float dq = dAxx * (cp * cp - sp * sp * iq2) + dAyy * (sp * sp - cp * cp * iq2) +
+                   dAxy * 0.5f * s2p * (1.0f + iq2);
+        float dphi = dAxx * s2p * (iq - q) + dAyy * s2p * (q - iq) + dAxy * c2p * (q - iq);
+        float c = C.c;
+        float dc = dq * (-2.0f / ((1.0f + c) * (1.0f + c)));
+        float cc = C.e1 * C.e1 + C.e2 * C.e2;
+        g[2] = dphi * (-C.e2 * 0.5f / cc) + dc * C.e1 / c;
+        g[3] = dphi * (C.e1 * 0.5f / cc) + dc * C.e2 / c;
+    } else {
+        g[2] = 0.0f; g[3] = 0.0f;
+    }
+    g[4] = -2.0f * Ks * acc[6];
+    g[5] = -2.0f * Ks * acc[7];
+    g[6] = acc[0] / C.Ie;
+}
+
+// ---------------------------------------------------------------------------
+// Prior + bijector (per parameter), TFP semantics
+// ---------------------------------------------------------------------------
+// args: a0..a3, a4 = precomputed log normaliser (TruncatedNormal: log Z).
+struct PriorOut { float x, dxdz, lp, dlpdx, fldj, dfldjdz; };
+
+GLM float softplusf(float v) { return v > 20.0f ? v : log1pf(expf(v)); }
+GLM float sigmoidf_(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+GLM void prior_eval(int kind, const float* a, float z, int fldj_at_x, PriorOut& o) {
+    switch (kind) {
+    case 1: {  // Normal, Identity
+        o.x = z; o.dxdz = 1.0f;
+        float r = (z - a[0]) / a[1];
+        o.lp = -0.5f * r * r - logf(a[1]) - kHalfLn2Pi;
+        o.dlpdx = -r / a[1];
+        o.fldj = 0.0f; o.dfldjdz = 0.0f;
+        break;
+    }
+    case 2: {  // LogNormal, Exp
+        float x = expf(z);
+        o.x = x; o.dxdz = x;
+        float lx = logf(x);
+        float r = (lx - a[0]) / a[1];
+        o.lp = -0.5f * r * r - logf(a[1]) - kHalfLn2Pi - lx;
+        o.dlpdx = (-r / a[1] - 1.0f) / x;
+        if (fldj_at_x) { o.fldj = x; o.dfldjdz = x; }
+        else { o.fldj = z; o.dfldjdz = 1.0f; }
+        break;
+    }
+    case 3:
+    case 4: {  // Uniform / TruncatedNormal, Sigmoid(low, high)
+        float lo = kind == 3 ? a[0] : a[2];
+        float hi = kind == 3 ? a[1] : a[3];
+        float s = sigmoidf_(z), sm = sigmoidf_(-z);
+        float x = hi * s + lo * sm;
+        o.x = x;
+        o.dxdz = (hi - lo) * s * sm;
+        if (kind == 3) {
+            bool in = (x >= lo) && (x < hi);
+            o.lp = in ? -logf(hi - lo) : -INFINITY;
+            o.dlpdx = 0.0f;
+        } else {
+            bool in = (x >= lo) && (x <= hi);
+            float r = (x - a[0]) / a[1];
+            o.lp = in ? (-0.5f * r * r - logf(a[1]) - kHalfLn2Pi - a[4]) : -INFINITY;
+            o.dlpdx = -r / a[1];
+        }
+        float w = fldj_at_x ? x : z;
+        o.fldj = logf(hi - lo) - softplusf(-w) - softplusf(w);
+        float dw = 1.0f - 2.0f * sigmoidf_(w);
+        o.dfldjdz = fldj_at_x ? dw * o.dxdz : dw;
+        break;
+    }
+    default:
+        o.x = z; o.dxdz = 1.0f; o.lp = 0.0f; o.dlpdx = 0.0f; o.fldj = 0.0f; o.dfldjdz = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Likelihood per pixel
+// ---------------------------------------------------------------------------
+// Forward model (sigma from the model image, tf/model.py:77-80):
+// returns log-pdf without the -1/2 ln 2 pi constant; r2 = r^2; dll / dchi
+// are d(log-pdf)/d im and d(r^2)/d im.
+GLM void lik_forward(float im, float obs, float bkg2, float inv_texp, float& ll, float& r2, float& dll,
+                     float& dr2) {
+    float d = obs - im;
+    float pos = im >= 0.0f ? 1.0f : 0.0f;
+    float var = fmaf(fmaxf(im, 0.0f), inv_texp, bkg2);
+    float iv = 1.0f / var;
+    float d2iv = d * d * iv;
+    r2 = d2iv;
+    ll = -0.5f * d2iv - 0.5f * logf(var);
+    float dv = pos * inv_texp;
+    dll = d * iv + 0.5f * (d2iv * iv - iv) * dv;
+    dr2 = -2.0f * d * iv - d2iv * iv * dv;
+}
+// Backward model (sigma from the observed image, tf/model.py:113-118):
+GLM void lik_backward(float im, float obs, float iv, float log_sig, float& ll, float& r2, float& dll, float& dr2) {
+    float d = im - obs;
+    r2 = d * d * iv;
+    ll = -0.5f * r2 - log_sig;
+    dll = -d * iv;
+    dr2 = 2.0f * d * iv;
+}
+
+}  // namespace glm

@@ -1,0 +1,117 @@
+"""Physical and probabilistic models (mirrors ``gigalens.model`` and
+``gigalens.tf.model``).
+
+``ForwardProbModel.log_prob(simulator, z)`` returns ``(lp, chi2)`` like
+src/gigalens/tf/model.py:53-86 and is differentiable: its backward is the
+hand-written adjoint kernel, which recomputes the ray-trace instead of
+storing per-particle grids.
+"""
+from __future__ import annotations
+
+from abc import ABC, abstractmethod
+from typing import List
+
+import torch
+
+from . import _native as nat
+from . import distributions as dist
+from .plan import Plan, layout_from_paths
+
+
+class PhysicalModel:
+    """``gigalens.model.PhysicalModel`` (src/gigalens/model.py:7-29)."""
+
+    def __init__(self, lenses: List, lens_light: List, source_light: List):
+        self.lenses = lenses
+        self.lens_light = lens_light
+        self.source_light = source_light
+
+
+class ProbabilisticModel(ABC):
+    """``gigalens.model.ProbabilisticModel`` (src/gigalens/model.py:32-58)."""
+
+    def __init__(self, prior, bij=None, *args):
+        self.prior = prior
+        self.bij = bij
+
+    @abstractmethod
+    def log_prob(self, simulator, z):
+        pass
+
+
+class _LogProbFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, plan, z):
+        lp, chi2, _ = plan.log_prob(z)
+        ctx.plan = plan
+        ctx.save_for_backward(z)
+        return lp, chi2
+
+    @staticmethod
+    def backward(ctx, g_lp, g_chi2):
+        (z,) = ctx.saved_tensors
+        g_lp = torch.zeros_like(z[:, 0]) if g_lp is None else g_lp.contiguous()
+        g_chi2 = None if g_chi2 is None else g_chi2.contiguous()
+        _, _, grad = ctx.plan.log_prob(z, g_lp=g_lp, g_chi2=g_chi2, want_grad=True)
+        return None, grad
+
+
+class _ProbModelBase(ProbabilisticModel):
+    _likelihood = nat.GL_LIK_FORWARD
+    _fldj_at_x = False
+
+    def __init__(self, prior, observed_image=None, background_rms=None, exp_time=None):
+        super().__init__(prior)
+        self.observed_image = torch.as_tensor(observed_image, dtype=torch.float32)
+        self.background_rms = float(background_rms)
+        self.exp_time = float(exp_time)
+        self.pack_bij = dist.PackBijector(prior)
+        self.unconstraining_bij = prior.experimental_default_event_space_bijector()
+        self.bij = dist.ChainBijector(self.unconstraining_bij, self.pack_bij)
+        self._paths = [p for p, _ in dist.leaves(prior)]
+        self._plans = {}
+
+    def plan_for(self, simulator, epl_batch_series=False) -> Plan:
+        key = (id(simulator), bool(epl_batch_series))
+        if key not in self._plans:
+            lay = layout_from_paths(simulator.phys_model, self._paths)
+            self._plans[key] = (simulator, Plan(simulator.phys_model, simulator.camera, lay, prior=self.prior,
+                                                observed=self.observed_image.cpu().numpy(),
+                                                background_rms=self.background_rms, exp_time=self.exp_time,
+                                                likelihood=self._likelihood, fldj_at_x=self._fldj_at_x,
+                                                epl_batch_series=epl_batch_series))
+        return self._plans[key][1]
+
+    def log_prob(self, simulator, z):
+        """Log posterior of unconstrained ``z`` (bs, d): ``(lp, chi2)``."""
+        plan = self.plan_for(simulator)
+        z = torch.as_tensor(z, dtype=torch.float32, device=simulator.device).contiguous()
+        return _LogProbFn.apply(plan, z)
+
+    def log_prob_and_grad(self, simulator, z, g_lp=None):
+        """Fused value and gradient in one launch: ``(lp, chi2, g_lp * dlp/dz)``."""
+        plan = self.plan_for(simulator)
+        z = torch.as_tensor(z, dtype=torch.float32, device=simulator.device).contiguous()
+        return plan.log_prob(z, g_lp=g_lp, want_grad=True)
+
+
+class ForwardProbModel(_ProbModelBase):
+    """Noise from the simulated image; no least squares (tf/model.py:9-86)."""
+
+
+class BackwardProbModel(_ProbModelBase):
+    """Noise from the observed image; linear light amplitudes by least squares
+    (tf/model.py:89-163).  Its Jacobian term is evaluated at the constrained
+    parameters, as in tf/model.py:160."""
+
+    _likelihood = nat.GL_LIK_BACKWARD
+    _fldj_at_x = True
+
+    def __init__(self, prior, observed_image, background_rms, exp_time):
+        super().__init__(prior, observed_image, background_rms, exp_time)
+        obs = self.observed_image
+        self.err_map = torch.sqrt(self.background_rms ** 2 + torch.clamp(obs, min=0) / self.exp_time)
+
+    def log_prob(self, simulator, z):
+        from .lstsq import backward_log_prob
+        return backward_log_prob(self, simulator, z)

@@ -1,0 +1,142 @@
+/*
+ * gigalens_amd -- C ABI of the MI355X (gfx950) lens-model hot path.
+ *
+ * The ABI is what a binding of the reference's LensSimulator / prob-model
+ * path would call.  Every tensor argument is caller-allocated DEVICE memory
+ * (row-major float32 unless stated), every call is stream-ordered and
+ * asynchronous, and no call allocates device memory or synchronises the
+ * host except gl_plan_create / gl_plan_destroy.  Errors are reported as a
+ * non-zero return code; gl_last_error() returns a thread-local message.
+ *
+ * Parameter vectors use the reference's flatten order (tf.nest order of a
+ * prior sample: component lists in order, dict keys sorted), i.e. the
+ * column order of z in src/gigalens/tf/model.py:40-51.
+ *
+ * Citations are file:line in the reference repository.
+ */
+#ifndef GIGALENS_AMD_H
+#define GIGALENS_AMD_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GL_OK 0
+#define GL_ERR_ARG 1
+#define GL_ERR_HIP 2
+#define GL_ERR_UNSUPPORTED 3
+
+#define GL_MAX_COMPONENTS 8
+#define GL_MAX_COMPONENT_PARAMS 8
+#define GL_MAX_PARAMS 128
+#define GL_MAX_EPL_ITER 100
+
+/* Profile kinds (src/gigalens/tf/profiles/{mass,light}/...). */
+enum gl_profile_kind {
+    GL_EPL = 1,            /* params: theta_E, gamma, e1, e2, center_x, center_y  (epl.py:17) */
+    GL_SHEAR = 2,          /* params: gamma1, gamma2                              (shear.py:10) */
+    GL_SIS = 3,            /* params: theta_E, center_x, center_y                 (sis.py:11) */
+    GL_SERSIC = 10,        /* params: R_sersic, n_sersic, center_x, center_y, Ie   (sersic.py:23) */
+    GL_SERSIC_ELLIPSE = 11,/* params: R_sersic, n_sersic, e1, e2, center_x, center_y, Ie (sersic.py:64) */
+    GL_SHAPELETS = 12      /* params: beta, center_x, center_y (+ amplitudes)      (shapelets.py:17) */
+};
+
+/* Roles: which list of PhysicalModel the component belongs to (model.py:21-29). */
+enum gl_role { GL_ROLE_LENS = 0, GL_ROLE_LENS_LIGHT = 1, GL_ROLE_SOURCE = 2 };
+
+/* Prior families of the reference default prior (tests/conftest.py:20-73). */
+enum gl_prior_kind {
+    GL_PRIOR_NORMAL = 1,        /* args: loc, scale          bijector Identity */
+    GL_PRIOR_LOGNORMAL = 2,     /* args: loc, scale          bijector Exp */
+    GL_PRIOR_UNIFORM = 3,       /* args: low, high           bijector Sigmoid(low, high) */
+    GL_PRIOR_TRUNCNORMAL = 4    /* args: loc, scale, low, high  bijector Sigmoid(low, high) */
+};
+
+enum gl_likelihood {
+    GL_LIK_FORWARD = 0,   /* sigma from the simulated image (tf/model.py:77) */
+    GL_LIK_BACKWARD = 1   /* sigma from the observed image  (tf/model.py:113-115) */
+};
+
+typedef struct gl_component_desc {
+    int kind;                                   /* gl_profile_kind */
+    int role;                                   /* gl_role */
+    int param_col[GL_MAX_COMPONENT_PARAMS];     /* column of x for each profile parameter, -1: constant */
+    float param_const[GL_MAX_COMPONENT_PARAMS]; /* value used when param_col is -1 */
+    int niter;                                  /* EPL: maximum series length (epl.py:19) */
+    int n_max;                                  /* SHAPELETS: n_max */
+    int interpolate;                            /* SHAPELETS: table mode (shapelets.py:20,55-65) */
+    int use_lstsq;                              /* light: unit amplitude, solved by least squares */
+    int amp_col0;                               /* SHAPELETS (not lstsq): column of amp00, contiguous */
+} gl_component_desc;
+
+typedef struct gl_plan_desc {
+    /* camera: src/gigalens/simulator.py:11-28, tf/simulator.py:17-55 */
+    int num_pix;
+    int supersample;
+    float transform_sub[4];   /* sub-pixel pix->angle matrix (T / ss), row-major */
+    float origin[2];          /* angle at sub-pixel (0,0): ra_at_xy_0, dec_at_xy_0 */
+    float conversion_factor;  /* det(T) (tf/simulator.py:24-26) */
+    int kernel_size;          /* k of the sub-grid PSF (0: no PSF) */
+    const float* kernel;      /* HOST k*k sub-grid PSF, NOT flipped (true convolution) */
+    /* model */
+    int n_components;
+    const gl_component_desc* components; /* HOST; lens comps first, then lens light, then source */
+    /* parameters / prior */
+    int n_params;             /* d */
+    const int* prior_kind;    /* HOST d entries, gl_prior_kind */
+    const float* prior_args;  /* HOST d*4 */
+    /* data */
+    const float* observed;    /* HOST N*N, may be NULL for simulate-only plans */
+    float background_rms;
+    float exp_time;
+    int likelihood;           /* gl_likelihood */
+    int fldj_at_x;            /* 1: evaluate fldj at constrained x (tf/model.py:160 quirk) */
+    int epl_batch_series;     /* 1: reference's batch-coupled EPL series length (epl.py:41) */
+} gl_plan_desc;
+
+typedef struct gl_plan gl_plan;
+
+/* Message of the last failing call on this thread. */
+const char* gl_last_error(void);
+
+/* Build an immutable device plan: copies PSF (pre-folded with the ss x ss
+ * pooling box), observed image, prior table and component table to device
+ * memory.  Replaces LensSimulator.__init__ (tf/simulator.py:11-55) plus the
+ * constant state of ForwardProbModel/BackwardProbModel.__init__
+ * (tf/model.py:29-51, 109-132). */
+int gl_plan_create(const gl_plan_desc* desc, gl_plan** out);
+int gl_plan_destroy(gl_plan* plan);
+
+/* Batched forward model on physical parameters x (bs x d):
+ * img (bs x N x N).  Replaces LensSimulator.simulate (tf/simulator.py:65-97). */
+int gl_simulate(const gl_plan* plan, const float* x, int bs, float* img, hipStream_t stream);
+
+/* Vector-Jacobian product of gl_simulate: grad_x (bs x d) = d<g_img, img>/dx.
+ * Backward of simulate for autograd (the reference's GradientTape through
+ * tf/simulator.py:65-97).  Recomputes the ray-trace; stores no grids. */
+int gl_simulate_vjp(const gl_plan* plan, const float* x, int bs, const float* g_img,
+                    float* grad_x, hipStream_t stream);
+
+/* Log posterior on unconstrained z (bs x d): lp (bs), chi2 = mean r^2 (bs).
+ * Replaces ForwardProbModel.log_prob (tf/model.py:53-86) and, for plans with
+ * GL_LIK_BACKWARD and least-squares components, BackwardProbModel.log_prob
+ * (tf/model.py:134-163).  If grad_z is non-NULL it receives
+ * g_lp[i] * dlp_i/dz_i + g_chi2[i] * dchi2_i/dz_i (g_lp NULL: 1, g_chi2 NULL: 0),
+ * computed by the hand-written adjoint in the same launch.  lp/chi2 may be NULL. */
+int gl_log_prob(const gl_plan* plan, const float* z, int bs, float* lp, float* chi2,
+                const float* g_lp, const float* g_chi2, float* grad_z, hipStream_t stream);
+
+/* One multi-start MAP step with Keras-Adam semantics, fused with log_prob and
+ * its adjoint: loss = mean_i(-lp_i / N^2) (tf/inference.py:28-35), then
+ * m, v, z updated in place (tf.keras Adam, epsilon outside the sqrt).
+ * step is 1-based.  lp / chi2 (bs) receive the pre-update values. */
+int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, int step,
+                     float lr, float beta1, float beta2, float epsilon,
+                     float* lp, float* chi2, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
