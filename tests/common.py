@@ -109,3 +109,14 @@ def epl_shear_problem(bs=16, seed=2, obs_seed=1):
     obs = wl.noisy_observation(clean, 0.2, 100.0, seed=obs_seed)
     x = draw_prior(prior, bs, seed=seed)
     return dict(prior=prior, phys=phys, cfg=cfg, cam=cam, ocam=ocam, obs=obs, x=x, truth_x=truth_x)
+
+
+def oracle_nested_prior(prior):
+    """Nested (kind_name, args) leaves in the structure of the prior (torch_ref format)."""
+    def rec(node):
+        if isinstance(node, tfd.JointDistributionSequential):
+            return [rec(m) for m in node.model]
+        if isinstance(node, tfd.JointDistributionNamed):
+            return {k: rec(node.model[k]) for k in sorted(node.model)}
+        return (KIND_NAME[node.kind], tuple(node.args()))
+    return rec(prior)
