@@ -1,0 +1,202 @@
+"""Benchmark: sim+grad particle-evals/s of multi-start MAP on one MI355X.
+
+Workload (the batched-MAP configuration): 60x60 pixels at 0.065"/px,
+supersample 2, Gaussian PSF FWHM 0.1" (11x11, sub-grid 21x21), EPL + shear
+lens, elliptical Sersic lens light and source; 4,000 particles drawn from the
+prior (seed 3); Keras-Adam lr 5e-3.  One step = one fused launch that
+simulates every particle, evaluates log_prob, back-propagates with the
+hand-written adjoint and applies the Adam update.  Inputs are resident in
+HBM before the timed region.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from gigalens_amd import ForwardProbModel, LensSimulator  # noqa: E402
+from gigalens_amd import distributions as tfd  # noqa: E402
+from gigalens_amd import flops as fl  # noqa: E402
+from gigalens_amd import workloads as wl  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 (vector / matrix)
+BASELINE_EVALS = 2200.0   # reference notebook MAP throughput (tf-demo.ipynb:808)
+
+
+def flat_sample(prior, n, seed):
+    s = prior.sample(n, seed=seed)
+    return torch.stack([torch.as_tensor(v, dtype=torch.float32) for v in tfd.flatten(s)], dim=1)
+
+
+def setup(bs, device):
+    prior = wl.epl_shear_prior()
+    phys = wl.epl_shear_model()
+    cfg = wl.epl_shear_config()
+    sim = LensSimulator(phys, cfg, bs=bs, device=device)
+    truth = flat_sample(prior, 1, seed=0).to(device)
+    clean = sim.simulate(truth).cpu().numpy()
+    obs = wl.noisy_observation(clean, 0.2, 100.0, seed=1)
+    pm = ForwardProbModel(prior, obs, 0.2, 100.0)
+    x0 = flat_sample(prior, bs, seed=3)
+    z0 = pm.bij.inverse(tfd.pack_as(prior, [x0[:, j] for j in range(x0.shape[1])])).to(device).contiguous()
+    return prior, phys, cfg, sim, pm, obs, z0
+
+
+def cpu_baseline(obs, budget_s=20.0):
+    """Oracle port (torch-CPU FP32, op-for-op restatement of the reference's TF
+    path, autograd) timed on this host: 2 warm-ups, median of 5 runs."""
+    from oracle import torch_ref
+    from oracle import np64
+    prior = wl.epl_shear_prior()
+    cfg = wl.epl_shear_config()
+    from gigalens_amd.simulator import Camera
+    cam = Camera(cfg)
+    ocam = np64.Camera(cfg.delta_pix, cfg.num_pix, cfg.supersample, cfg.kernel, None, subgrid=cam.subgrid_kernel)
+    model = {"lenses": [("EPL", {"niter": 50}), ("SHEAR", {})], "lens_light": [("SERSIC_ELLIPSE", {})],
+             "source_light": [("SERSIC_ELLIPSE", {})]}
+    kind = {1: "Normal", 2: "LogNormal", 3: "Uniform", 4: "TruncatedNormal"}
+
+    def rec(node):
+        if isinstance(node, tfd.JointDistributionSequential):
+            return [rec(m) for m in node.model]
+        if isinstance(node, tfd.JointDistributionNamed):
+            return {k: rec(node.model[k]) for k in sorted(node.model)}
+        return (kind[node.kind], tuple(node.args()))
+    nested = rec(prior)
+    sim = torch_ref.Sim(model, ocam, dtype=torch.float32)
+    x = flat_sample(prior, 4000, seed=3)
+    z_all = torch.as_tensor(torch_ref.x_to_z(nested, x.double().numpy()), dtype=torch.float32)
+
+    def run(bs, off):
+        z = z_all[off:off + bs].clone().requires_grad_(True)
+        t = time.perf_counter()
+        lp, _ = torch_ref.log_prob(sim, nested, z, obs, 0.2, 100.0)
+        (-lp.mean()).backward()
+        return time.perf_counter() - t
+    t1 = run(16, 0)
+    bs = int(max(16, min(512, 16 * budget_s / 7.0 / max(t1, 1e-3))))
+    times = [run(bs, 16 + i * bs) for i in range(7)]
+    med = float(np.median(times[2:]))
+    return {"value": bs / med, "unit": "particle-evals/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"{bs} prior particles per run, sim+log_prob+autograd grad in FP32 (torch CPU, "
+                      f"op-for-op restatement of tf/simulator.py + tf/model.py), 2 warm-ups, median of 5 runs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=350)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--particles", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=5e-3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="also launch a 256 MiB device copy (PMC calibration)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    bs = args.particles
+    prior, phys, cfg, sim, pm, obs, z0 = setup(bs, dev)
+    plan = pm.plan_for(sim)
+
+    # warm-up on a copy (also compiles/loads everything)
+    zw, mw, vw = z0.clone(), torch.zeros_like(z0), torch.zeros_like(z0)
+    lpw = torch.empty(bs, device=dev)
+    chw = torch.empty(bs, device=dev)
+    for s in range(1, args.warmup + 1):
+        plan.map_adam_step(zw, mw, vw, s, args.lr, lp=lpw, chi2=chw)
+    torch.cuda.synchronize()
+
+    K = args.steps
+    z, m, v = z0.clone(), torch.zeros_like(z0), torch.zeros_like(z0)
+    lp_hist = torch.empty((K, bs), device=dev)
+    chi_hist = torch.empty((K, bs), device=dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(K):
+        ev[s][0].record()
+        plan.map_adam_step(z, m, v, s + 1, args.lr, lp=lp_hist[s], chi2=chi_hist[s])
+        ev[s][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    if args.copy_probe:
+        a = torch.empty(64 * 1024 * 1024, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+
+    # algorithmic work of one launch: per-particle series length at the start and end
+    x_start = pm.bij.forward(z0.cpu())
+    x_end = pm.bij.forward(z.cpu())
+    kinds, roles = fl.model_kinds(phys)
+    ksz = sim.camera.subgrid_kernel.shape[0]
+    tot = 0.0
+    tot_tr = 0.0
+    tot_survey = 0.0
+    for xs in (x_start, x_end):
+        e1 = xs[0][0]["e1"].numpy()
+        e2 = xs[0][0]["e2"].numpy()
+        n = fl.series_length(e1, e2, 50)
+        c = fl.per_particle(kinds, roles, cfg.num_pix, cfg.supersample, ksz, n, grad=True)
+        tot += 0.5 * float(c["flops"].sum())
+        tot_tr += 0.5 * float(c["transcendentals"].sum())
+        tot_survey += 0.5 * float(fl.survey_formula_per_particle(cfg.num_pix, cfg.supersample, ksz, n).sum())
+    achieved = tot / (kern_ms * 1e-3) / 1e12
+
+    chi = chi_hist.cpu().numpy()
+    best = np.nanmin(chi, axis=1)
+    hit = np.nonzero(best <= 1.1)[0]
+    ms_per_step = wall / K * 1e3
+    value = bs * K / wall
+    out = {
+        "metric": "sim+grad particle-evals/s",
+        "value": value,
+        "unit": "particle-evals/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / BASELINE_EVALS,
+        "dtype": "f32",
+        "data": "synthetic (prior draws; observed image = seeded prior draw + Poisson + Gaussian noise)",
+        "config": {"workload": "multi-start MAP, 60x60 px, ss2, 21x21 PSF, EPL+shear + 2 Sersic, "
+                               f"{bs} particles, Keras Adam lr {args.lr}",
+                   "particles": bs, "num_pix": cfg.num_pix, "supersample": cfg.supersample,
+                   "psf_subgrid": ksz, "params": z0.shape[1]},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "gl::particle_kernel<512, EPL+SHEAR | SERSIC_ELLIPSE | SERSIC_ELLIPSE, grad>",
+                     "kernel_ms": kern_ms, "flop_per_launch": tot, "transcendentals_per_launch": tot_tr,
+                     "reference_formulation_flop_per_launch": tot_survey},
+        "time_to_map": {"chi2_threshold": 1.1,
+                        "steps": int(hit[0]) + 1 if len(hit) else None,
+                        "seconds": (int(hit[0]) + 1) * ms_per_step / 1e3 if len(hit) else None,
+                        "best_chi2_final": float(best[-1]), "init": "prior"},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(obs)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -241,12 +241,23 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         if (da > dmax) dmax = da;
     }
     const int xstr = (N + gl::kRB2 - 1) / gl::kRB2;
-    const int GW = gp0 + xstr * gl::kRB2 + dmax;
+    // width covers the widest vector read of an adjoint strip (rounded to 4 floats)
+    const int mmax = (kf + ss - 1) / ss;
+    const int sega4 = ((gl::kRB2 + mmax - 1 + 3) / 4) * 4;
+    const int GW = (gp0 + xstr * gl::kRB2 + dmax + sega4 + 3) & ~3;
     const int grows = gp0 + N + dmax;
     P.GW = GW; P.gp0 = gp0; P.g_size = GW * grows;
+    {
+        bool al = true;
+        for (int b = 0; b < ss; ++b) {
+            int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+            al = al && ((db - (Mb - 1) + gp0) % 4 + 4) % 4 == 0;
+        }
+        P.adj_aligned = al ? 1 : 0;
+    }
     P.sub_size = sub_img > P.g_size ? sub_img : P.g_size;
     int off = (P.sub_size + 3) & ~3;
-    P.off_w = off; off += (kf * kf + 3) & ~3;
+    P.off_w = off;
     P.off_x = off; off += (6 * P.d + 3) & ~3;
     P.off_c = off; off += gl::kMaxComp * gl::kConstStride;
     P.off_coef = off;
@@ -264,7 +275,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     {
         // forward conv: base = sub_base - pb; reads base + (ss I + p) PW + ss J0 + c, c < ss(RB-1)+kf
         const long j0max = (long)(nstr - 1) * gl::kRB;
-        const long seg = (long)ss * (gl::kRB - 1) + kf;
+        const long seg = (((long)ss * (gl::kRB - 1) + kf + 3) / 4) * 4;  // vector reads round up
         const long rmax = (long)P.sub_base - pb + (long)(ss * (N - 1) + kf - 1) * PW + ss * j0max + seg - 1;
         const long wmax = (long)P.sub_base + (long)(nsub - 1 + pb) * PW + nsub - 1;
         bool ok = rmax < P.sub_size && wmax < P.sub_size && P.sub_base - pb >= 0;
@@ -275,7 +286,8 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         for (int a = 0; a < ss && ok; ++a) {
             int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
             long rlo = (long)(0 + da - (Ma - 1) + gp0), rhi = (long)(N - 1 + da + gp0);
-            long clo = (long)(0 + da - (Ma - 1) + gp0), chi = x0max + da + gp0 + gl::kRB2 - 1;
+            long clo = (long)(0 + da - (Ma - 1) + gp0), chi = x0max + da - (Ma - 1) + gp0 + sega4 - 1;
+            if (chi < x0max + da + gp0 + gl::kRB2 - 1) chi = x0max + da + gp0 + gl::kRB2 - 1;
             ok = ok && rlo >= 0 && clo >= 0 && rhi < grows && chi < GW;
         }
         ok = ok && (long)(N - 1 + gp0) * GW + (N - 1 + gp0) < P.g_size && P.g_size <= P.off_w;

@@ -76,21 +76,27 @@ __device__ __forceinline__ int light_comp_index(int i) {
 // Conv + pool (forward) for one strip of RB outputs in row I.
 // ---------------------------------------------------------------------------
 template <int SS, int KF, int RB>
-__device__ __forceinline__ void conv_strip_static(const float* __restrict__ sub, const float* __restrict__ w,
+__device__ __forceinline__ void conv_strip_static(const float* __restrict__ sub, const float* __restrict__ wg,
                                                   int PW, int I, int J0, float* acc) {
+    // sub: LDS image, row (SS*I + p) of the strip starts 16-byte aligned (PW % 4 == 0,
+    // SS*J0 % 4 == 0); wg: folded weights in global memory (wave-uniform -> scalar loads)
     constexpr int SEG = SS * (RB - 1) + KF;
+    constexpr int SEG4 = (SEG + 3) / 4;
 #pragma unroll
     for (int r = 0; r < RB; ++r) acc[r] = 0.0f;
     const float* base = sub + (SS * I) * PW + SS * J0;
     for (int p = 0; p < KF; ++p) {
-        const float* row = base + p * PW;
-        float seg[SEG];
+        const float4* row4 = reinterpret_cast<const float4*>(base + p * PW);
+        float seg[SEG4 * 4];
 #pragma unroll
-        for (int c = 0; c < SEG; ++c) seg[c] = row[c];
-        const float* wp = w + p * KF;
+        for (int c = 0; c < SEG4; ++c) {
+            float4 t = row4[c];
+            seg[4 * c] = t.x; seg[4 * c + 1] = t.y; seg[4 * c + 2] = t.z; seg[4 * c + 3] = t.w;
+        }
+        const float* wp = wg + p * KF;
 #pragma unroll
         for (int q = 0; q < KF; ++q) {
-            float wq = wp[q];
+            const float wq = wp[q];
 #pragma unroll
             for (int r = 0; r < RB; ++r) acc[r] = fmaf(wq, seg[SS * r + q], acc[r]);
         }
@@ -120,26 +126,37 @@ __device__ __forceinline__ void conv_strip_dyn(const float* __restrict__ sub, co
 // G: padded adjoint image, idx(I,J) = (I+gp0)*GW + (J+gp0).
 // ---------------------------------------------------------------------------
 template <int SS, int KF, int RB>
-__device__ __forceinline__ void adj_strip_static(const float* __restrict__ G, const float* __restrict__ w, int GW,
-                                                 int gp0, int a, int b, int Y, int X0, float* gs) {
+__device__ __forceinline__ void adj_strip_static(const float* __restrict__ G, const float* __restrict__ wg, int GW,
+                                                 int gp0, bool aligned, int a, int b, int Y, int X0, float* gs) {
     constexpr int PB = (KF - SS) / 2;  // (k-1)/2 with k = KF - SS + 1
     const int ra = (a + PB) % SS, da = (a + PB) / SS;
     const int rb = (b + PB) % SS, db = (b + PB) / SS;
     const int Ma = (KF - ra + SS - 1) / SS;
     constexpr int MMAX = (KF + SS - 1) / SS;
+    constexpr int SEGA = RB + MMAX - 1;
+    constexpr int SEGA4 = (SEGA + 3) / 4;
     const int Mb = (KF - rb + SS - 1) / SS;
 #pragma unroll
     for (int r = 0; r < RB; ++r) gs[r] = 0.0f;
     for (int m = 0; m < Ma; ++m) {
         const float* row = G + (Y + da - m + gp0) * GW + (X0 + db - (Mb - 1) + gp0);
-        const float* wr = w + (SS * m + ra) * KF + rb;
-        float seg[RB + MMAX - 1];
+        const float* wr = wg + (SS * m + ra) * KF + rb;
+        float seg[SEGA4 * 4];
+        if (aligned) {
+            const float4* row4 = reinterpret_cast<const float4*>(row);
 #pragma unroll
-        for (int j = 0; j < RB + MMAX - 1; ++j) seg[j] = (j < RB + Mb - 1) ? row[j] : 0.0f;
+            for (int c = 0; c < SEGA4; ++c) {
+                float4 t = row4[c];
+                seg[4 * c] = t.x; seg[4 * c + 1] = t.y; seg[4 * c + 2] = t.z; seg[4 * c + 3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SEGA4 * 4; ++j) seg[j] = (j < RB + Mb - 1) ? row[j] : 0.0f;
+        }
 #pragma unroll
         for (int n = 0; n < MMAX; ++n) {
             if (n < Mb) {
-                float wq = wr[SS * n];
+                const float wq = wr[SS * n];
 #pragma unroll
                 for (int r = 0; r < RB; ++r) gs[r] = fmaf(wq, seg[r + Mb - 1 - n], gs[r]);
             }
@@ -188,26 +205,38 @@ struct Consts {
     const float4* coefB[3];
 };
 
+// Read a constant block from LDS into wave-uniform (scalar) registers.
+template <class T>
+__device__ __forceinline__ T uniform_load(const float* src) {
+    static_assert(sizeof(T) % 4 == 0, "constant blocks are 32-bit words");
+    T out;
+    int* o = reinterpret_cast<int*>(&out);
+    const int* w = reinterpret_cast<const int*>(src);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    return out;
+}
+
 template <class M>
 __device__ __forceinline__ void load_consts(const float* s_c, const float* s_coef, const DPlan& P, Consts<M>& C) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int k = M::lens[i];
         if (k == glm::K_EPL) {
-            C.epl[i] = *reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
+            C.epl[i] = uniform_load<glm::EplC>(s_c + i * kConstStride);
             C.coefA[i] = s_coef + P.coef_off[i];
             C.coefB[i] = reinterpret_cast<const float4*>(s_coef + P.coef_off[i] + ((P.max_n + 4) & ~3));
         } else if (k == glm::K_SHEAR) {
-            C.shr[i] = *reinterpret_cast<const glm::ShearC*>(s_c + i * kConstStride);
+            C.shr[i] = uniform_load<glm::ShearC>(s_c + i * kConstStride);
         } else if (k == glm::K_SIS) {
-            C.sis[i] = *reinterpret_cast<const glm::SisC*>(s_c + i * kConstStride);
+            C.sis[i] = uniform_load<glm::SisC>(s_c + i * kConstStride);
         }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (M::light[i] != 0) {
             const int ci = light_comp_index<M>(i);
-            C.ser[i] = *reinterpret_cast<const glm::SerC*>(s_c + ci * kConstStride);
+            C.ser[i] = uniform_load<glm::SerC>(s_c + ci * kConstStride);
         }
     }
 }
@@ -293,12 +322,12 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 template <int BLOCK, class M, bool GRAD>
 __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KArgs A) {
-    extern __shared__ float smem[];
+    extern __shared__ float4 smem4[];
+    float* smem = reinterpret_cast<float*>(smem4);
     const int pid = blockIdx.x;
     const int tid = threadIdx.x;
     const int d = P.d;
     float* s_sub = smem;
-    float* s_w = smem + P.off_w;
     float* s_x = smem + P.off_x;  // [x | dxdz | dlpdx | dfldjdz | gx | lpj]
     float* s_c = smem + P.off_c;
     float* s_coef = smem + P.off_coef;
@@ -306,7 +335,6 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
     const bool use_z = (A.mode == M_LOGPROB || A.mode == M_MAP);
 
     // ---- A: prologue -----------------------------------------------------
-    for (int i = tid; i < P.kf * P.kf; i += BLOCK) s_w[i] = P.w[i];
     for (int i = tid; i < P.sub_size; i += BLOCK) s_sub[i] = 0.0f;
     if (tid < d) {
         float zin = A.in[(size_t)pid * d + tid];
@@ -411,10 +439,10 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
             const int J0 = (task - I * NSTR) * kRB;
             float acc[kRB];
             const float* base = s_sub + P.sub_base - P.pb;  // column offset -pb folded in
-            if (ssk == 2022) conv_strip_static<2, 22, kRB>(base, s_w, PW, I, J0, acc);
-            else if (ssk == 2026) conv_strip_static<2, 26, kRB>(base, s_w, PW, I, J0, acc);
-            else if (ssk == 1021) conv_strip_static<1, 21, kRB>(base, s_w, PW, I, J0, acc);
-            else conv_strip_dyn<kRB>(base, s_w, PW, ss, P.kf, I, J0, acc);
+            if (ssk == 2022) conv_strip_static<2, 22, kRB>(base, P.w, PW, I, J0, acc);
+            else if (ssk == 2026) conv_strip_static<2, 26, kRB>(base, P.w, PW, I, J0, acc);
+            else if (ssk == 1021) conv_strip_static<1, 21, kRB>(base, P.w, PW, I, J0, acc);
+            else conv_strip_dyn<kRB>(base, P.w, PW, ss, P.kf, I, J0, acc);
 #pragma unroll
             for (int r = 0; r < kRB; ++r) {
                 const int J = J0 + r;
@@ -474,10 +502,10 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
                     const int Yc = task / XSTR;
                     const int X0 = (task - Yc * XSTR) * kRB2;
                     float gs[kRB2];
-                    if (ssk == 2022) adj_strip_static<2, 22, kRB2>(s_sub, s_w, GW, gp0, a, b, Yc, X0, gs);
-                    else if (ssk == 2026) adj_strip_static<2, 26, kRB2>(s_sub, s_w, GW, gp0, a, b, Yc, X0, gs);
-                    else if (ssk == 1021) adj_strip_static<1, 21, kRB2>(s_sub, s_w, GW, gp0, a, b, Yc, X0, gs);
-                    else adj_strip_dyn<kRB2>(s_sub, s_w, GW, gp0, ss, P.kf, P.pb, a, b, Yc, X0, gs);
+                    if (ssk == 2022) adj_strip_static<2, 22, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
+                    else if (ssk == 2026) adj_strip_static<2, 26, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
+                    else if (ssk == 1021) adj_strip_static<1, 21, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
+                    else adj_strip_dyn<kRB2>(s_sub, P.w, GW, gp0, ss, P.kf, P.pb, a, b, Yc, X0, gs);
                     const int yi = ss * Yc + a;
 #pragma unroll
                     for (int r = 0; r < kRB2; ++r) {

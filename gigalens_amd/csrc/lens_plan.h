@@ -26,6 +26,7 @@ struct DPlan {
     // LDS layout of the resident-image kernel (floats)
     int PW, sub_base, sub_size;  // padded sub-image: idx(y,x) = sub_base + (y+pb)*PW + x
     int GW, gp0, g_size;         // padded adjoint image: idx(I,J) = (I+gp0)*GW + (J+gp0)
+    int adj_aligned;             // adjoint strip rows start 16-byte aligned (vector loads)
     int off_w, off_x, off_c, off_coef, off_red, lds_floats;
     int coef_off[kMaxLens];      // per lens slot: offset of its coefficient tables in the coef area
     int max_n;                   // max EPL series length supported by the coef tables

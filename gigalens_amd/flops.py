@@ -1,0 +1,106 @@
+"""Algorithmic operation counts of the fused lens-model kernels.
+
+Counting rule: add/sub/mul = 1, FMA = 2, each transcendental (exp2, log2,
+rsqrt, rcp, sqrt, log) = 1; comparisons, selects, LDS/global moves are not
+counted.  Transcendentals are also reported separately.  The counts are the
+operation lists of the implemented formulation (gigalens_amd/csrc/lens_math.h
+and lens_kernels.hip), per sub-pixel ``S = (ss N)^2`` and per output pixel
+``P = N^2``; the EPL series costs 3 flops per term (Clenshaw) and its length
+``n_p`` is per particle.
+
+Forward (simulate + log-likelihood), per particle:
+  S * (8 coords + sum_c fwd_c + 1 per light (accumulate) + 2 per lens (beta))
+  + P * (2 kf^2 conv + 1 scale + 12 likelihood)
+Gradient launch (forward + adjoint), additionally:
+  S * (sum_c adj_c + 2 kf^2/ss^2 adjoint conv + 2) + P * 15 likelihood adjoint
+with kf = k + ss - 1 the PSF size after folding in the ss x ss pooling box.
+"""
+from __future__ import annotations
+
+from typing import Dict, Iterable, Sequence
+
+import numpy as np
+
+from . import _native as nat
+
+# (flops, transcendentals) per sub-pixel
+_FWD = {
+    nat.GL_SERSIC: (19, 3),
+    nat.GL_SERSIC_ELLIPSE: (19, 3),
+    nat.GL_SHEAR: (6, 0),
+    nat.GL_SIS: (12, 1),
+    nat.GL_EPL: (44, 3),  # + 3 per series term
+}
+# adjoint recompute + reverse statistics (the forward is recomputed)
+_ADJ = {
+    nat.GL_SERSIC: (19 + 24, 3 + 1),
+    nat.GL_SERSIC_ELLIPSE: (19 + 24, 3 + 1),
+    nat.GL_SHEAR: (6 + 8, 0),
+    nat.GL_SIS: (12 + 12, 1),
+    nat.GL_EPL: (44 + 12 + 59, 3 + 1),  # + 9 per series term (three series)
+}
+
+
+def series_length(e1: np.ndarray, e2: np.ndarray, niter: int) -> np.ndarray:
+    """Per-particle EPL series length (epl.py:41 rule on the particle's own f)."""
+    c = np.clip(np.hypot(np.asarray(e1, np.float64), np.asarray(e2, np.float64)), 0, 1)
+    q = (1 - c) / (1 + c)
+    f = (1 - q) / (1 + q)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lim = np.log(1e-12) / np.log(f) + 2.0
+    n = np.where(f == 0, 1, np.where(lim > 1, np.ceil(lim) - 1, 0))
+    return np.minimum(n, niter).astype(np.int64)
+
+
+def per_particle(kinds: Sequence[int], roles: Sequence[int], num_pix: int, ss: int, kernel_size: int,
+                 n_terms: np.ndarray, grad: bool) -> Dict[str, np.ndarray]:
+    """Flops and transcendentals per particle (arrays over particles)."""
+    S = (ss * num_pix) ** 2
+    P = num_pix ** 2
+    k = kernel_size if kernel_size > 0 else 1
+    kf = k + ss - 1
+    n_terms = np.asarray(n_terms, np.float64)
+    f = np.full_like(n_terms, 8.0)
+    tr = np.zeros_like(n_terms)
+    for kind, role in zip(kinds, roles):
+        a, t = _FWD[kind]
+        f += a + (2 if role == nat.GL_ROLE_LENS else 1)
+        tr += t
+        if kind == nat.GL_EPL:
+            f += 3 * n_terms
+    flops = S * f + P * (2 * kf * kf + 1 + 12)
+    trans = S * tr + P * 1
+    if grad:
+        g = np.full_like(n_terms, 8.0 + 2.0 + 2.0 * kf * kf / (ss * ss))
+        gt = np.zeros_like(n_terms)
+        for kind, role in zip(kinds, roles):
+            a, t = _ADJ[kind]
+            g += a
+            gt += t
+            if kind == nat.GL_EPL:
+                g += 9 * n_terms
+        flops = flops + S * g + P * 15
+        trans = trans + S * gt
+    return {"flops": flops, "transcendentals": trans}
+
+
+def model_kinds(phys_model):
+    kinds, roles = [], []
+    for role, lst in ((nat.GL_ROLE_LENS, phys_model.lenses), (nat.GL_ROLE_LENS_LIGHT, phys_model.lens_light),
+                      (nat.GL_ROLE_SOURCE, phys_model.source_light)):
+        for prof in lst:
+            kinds.append(prof._kind)
+            roles.append(role)
+    return kinds, roles
+
+
+def survey_formula_per_particle(num_pix: int, ss: int, kernel_size: int, n_terms: np.ndarray) -> np.ndarray:
+    """The reference-formulation estimate (naive EPL recurrence at 10 flops per
+    term, unfused elementwise ops): forward S (71 + 10 n) + P (2 kf^2 + 10),
+    sim+grad = forward + conv adjoint + 3 x ray/light part."""
+    S = (ss * num_pix) ** 2
+    P = num_pix ** 2
+    kf = (kernel_size if kernel_size > 0 else 1) + ss - 1
+    ray = S * (71 + 10 * np.asarray(n_terms, np.float64))
+    conv = P * 2 * kf * kf
+    return ray + conv + P * 10 + conv + 3 * ray
