@@ -10,7 +10,8 @@ import ctypes
 import os
 from typing import Optional
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libgigalens_amd.so")
+LIB_PATH = os.environ.get("GIGALENS_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib",
+                                                                 "libgigalens_amd.so")
 
 GL_MAX_COMPONENTS = 8
 GL_MAX_COMPONENT_PARAMS = 8

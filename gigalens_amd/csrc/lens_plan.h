@@ -1,7 +1,7 @@
 // Device-side plan shared by the kernels and the C ABI.
 #pragma once
 #include <hip/hip_runtime.h>
-#include "../../include/gigalens_amd.h"
+#include "gigalens_amd.h"
 
 namespace gl {
 
