@@ -77,6 +77,7 @@ class PlanDesc(ctypes.Structure):
         ("likelihood", ctypes.c_int),
         ("fldj_at_x", ctypes.c_int),
         ("epl_batch_series", ctypes.c_int),
+        ("psf_rank_tol", ctypes.c_float),
     ]
 
 

@@ -105,6 +105,64 @@ static int n_params_of(int kind) {
     }
 }
 
+// One-sided Jacobi SVD of a small n x n matrix (row-major, double):
+// A = U diag(s) V^T.  Returns s (descending), U columns, V columns.
+static void jacobi_svd(int n, const std::vector<double>& A, std::vector<double>& s, std::vector<double>& U,
+                       std::vector<double>& V) {
+    std::vector<double> W(A);  // columns of W converge to U * s
+    V.assign(n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double a = 0, b = 0, c = 0;
+                for (int i = 0; i < n; ++i) {
+                    a += W[i * n + p] * W[i * n + p];
+                    b += W[i * n + q] * W[i * n + q];
+                    c += W[i * n + p] * W[i * n + q];
+                }
+                if (fabs(c) <= 1e-300 || fabs(c) <= 1e-17 * sqrt(a * b)) continue;
+                off = fmax(off, fabs(c) / sqrt(a * b));
+                double zeta = (b - a) / (2.0 * c);
+                double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                for (int i = 0; i < n; ++i) {
+                    double wp = W[i * n + p], wq = W[i * n + q];
+                    W[i * n + p] = cs * wp - sn * wq;
+                    W[i * n + q] = sn * wp + cs * wq;
+                    double vp = V[i * n + p], vq = V[i * n + q];
+                    V[i * n + p] = cs * vp - sn * vq;
+                    V[i * n + q] = sn * vp + cs * vq;
+                }
+            }
+        if (off < 1e-15) break;
+    }
+    std::vector<int> order(n);
+    std::vector<double> norms(n);
+    for (int j = 0; j < n; ++j) {
+        double acc = 0;
+        for (int i = 0; i < n; ++i) acc += W[i * n + j] * W[i * n + j];
+        norms[j] = sqrt(acc);
+        order[j] = j;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (norms[order[j]] > norms[order[i]]) std::swap(order[i], order[j]);
+    s.assign(n, 0.0);
+    U.assign(n * n, 0.0);
+    std::vector<double> V2(n * n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        int j = order[k];
+        s[k] = norms[j];
+        for (int i = 0; i < n; ++i) {
+            U[i * n + k] = norms[j] > 0 ? W[i * n + j] / norms[j] : 0.0;
+            V2[i * n + k] = V[i * n + j];
+        }
+    }
+    V.swap(V2);
+}
+
 extern "C" int gl_plan_destroy(gl_plan* plan) {
     if (!plan) return GL_OK;
     for (void* d : plan->dev) (void)hipFree(d);
@@ -308,11 +366,50 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         return fail(GL_ERR_UNSUPPORTED, "image too large for the LDS-resident kernel (output strips)");
     }
 
+    // ---- separable (low-rank) factorisation of the folded PSF ----------------
+    std::vector<float> sa, sb;
+    P.sep_rank = 0;
+    {
+        const float tol_in = desc->psf_rank_tol;
+        const bool have_sep = (ss == 2 && kf == 22) || (ss == 1 && kf == 21) || (ss == 3 && kf == 23);
+        if (tol_in >= 0.0f && have_sep) {
+            const double tol = tol_in > 0.0f ? tol_in : 1e-6;
+            std::vector<double> Wd(kf * kf), sv, Uv, Vv;
+            for (int i = 0; i < kf * kf; ++i) Wd[i] = W[i];
+            jacobi_svd(kf, Wd, sv, Uv, Vv);
+            int R = 0;
+            while (R < kf && sv[R] >= tol * sv[0]) ++R;
+            const int TR = nsub + k - 1;
+            const int xs = (N + gl::kTB - 1) / gl::kTB;
+            const int urows = gp0 + N + dmax;
+            const bool fits = R >= 1 && R <= gl::kMaxRank && (long)R * TR * N <= P.sub_size &&
+                              (long)R * urows * nsub <= P.sub_size && (long)TR * nstr <= (long)gl::kMaxTT * kBlock &&
+                              (long)N * ss * xs <= (long)gl::kMaxU * kBlock && (long)N * nstr <= (long)gl::kMaxT * kBlock;
+            if (fits) {
+                P.sep_rank = R;
+                P.urows = urows;
+                sa.resize(R * kf);
+                sb.resize(R * kf);
+                for (int r = 0; r < R; ++r)
+                    for (int i = 0; i < kf; ++i) {
+                        sa[r * kf + i] = (float)(sv[r] * Uv[i * kf + r]);
+                        sb[r * kf + i] = (float)Vv[i * kf + r];
+                    }
+            }
+        }
+    }
+
     // ---- device constants ---------------------------------------------------
     void* dptr = nullptr;
     int rc;
     if ((rc = upload(p, W.data(), W.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
     P.w = (const float*)dptr;
+    if (P.sep_rank > 0) {
+        if ((rc = upload(p, sa.data(), sa.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
+        P.sa = (const float*)dptr;
+        if ((rc = upload(p, sb.data(), sb.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
+        P.sb = (const float*)dptr;
+    }
     std::vector<float> obs(N * N, 0.0f), ivar(N * N, 0.0f), lsig(N * N, 0.0f);
     if (desc->observed) {
         for (int i = 0; i < N * N; ++i) {

@@ -192,6 +192,129 @@ __host__ __device__ constexpr int light_acc_slot(int i) {
     return i < 2 ? i : M::NLL + (i - 2);
 }
 
+
+// ---------------------------------------------------------------------------
+// Separable (low-rank) PSF path: W = sum_r a_r b_r^T with R <= kMaxRank.
+//   forward  T_r[y][J]  = sum_q b_r[q] sub[y][ss J + q - pb]          (horizontal)
+//            out[I][J]  = sum_r sum_p a_r[p] T_r[ss I + p - pb][J]       (vertical)
+//   adjoint  U_r[I][x]  = sum_J b_r[x + pb - ss J] G[I][J]              (transposed horizontal)
+//            gsub[y][x] = sum_r sum_I a_r[y + pb - ss I] U_r[I][x]       (transposed vertical)
+// ---------------------------------------------------------------------------
+constexpr int kMaxRank = 2;
+constexpr int kTB = 4;     // J per horizontal-pass task
+constexpr int kMaxTT = 5;  // horizontal-pass tasks per thread
+constexpr int kRV = 4;     // output rows per vertical-pass task
+constexpr int kMaxU = 4;   // transposed-horizontal tasks per thread
+
+template <int SS, int KF>
+__device__ __forceinline__ void sep_hpass(const float* __restrict__ sub, const float* __restrict__ sb, int R, int PW,
+                                          int yy, int J0, float (&t)[kMaxRank][kTB]) {
+    constexpr int SEG = SS * (kTB - 1) + KF;
+    constexpr int SEG4 = (SEG + 3) / 4;
+    const float4* row4 = reinterpret_cast<const float4*>(sub + yy * PW + SS * J0);
+    float seg[SEG4 * 4];
+#pragma unroll
+    for (int c = 0; c < SEG4; ++c) {
+        float4 v = row4[c];
+        seg[4 * c] = v.x; seg[4 * c + 1] = v.y; seg[4 * c + 2] = v.z; seg[4 * c + 3] = v.w;
+    }
+#pragma unroll
+    for (int r = 0; r < kMaxRank; ++r) {
+        if (r < R) {
+#pragma unroll
+            for (int j = 0; j < kTB; ++j) t[r][j] = 0.0f;
+#pragma unroll
+            for (int q = 0; q < KF; ++q) {
+                const float w = sb[r * KF + q];
+#pragma unroll
+                for (int j = 0; j < kTB; ++j) t[r][j] = fmaf(w, seg[SS * j + q], t[r][j]);
+            }
+        }
+    }
+}
+
+template <int SS, int KF>
+__device__ __forceinline__ void sep_vpass(const float* __restrict__ T, const float* __restrict__ sa, int R, int TR,
+                                          int TW, int I0, int J, float (&o)[kRV]) {
+#pragma unroll
+    for (int i = 0; i < kRV; ++i) o[i] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < kMaxRank; ++r) {
+        if (r < R) {
+            const float* col = T + r * TR * TW + J;
+#pragma unroll
+            for (int c = 0; c < SS * (kRV - 1) + KF; ++c) {
+                const int row = SS * I0 + c;
+                const float v = row < TR ? col[row * TW] : 0.0f;
+#pragma unroll
+                for (int i = 0; i < kRV; ++i) {
+                    const int pp = c - SS * i;
+                    if (pp >= 0 && pp < KF) o[i] = fmaf(sa[r * KF + pp], v, o[i]);
+                }
+            }
+        }
+    }
+}
+
+template <int SS, int KF>
+__device__ __forceinline__ void sep_upass(const float* __restrict__ G, const float* __restrict__ sb, int R, int GW,
+                                          int gp0, bool aligned, int b, int I, int X0, float (&u)[kMaxRank][kTB]) {
+    constexpr int PB = (KF - SS) / 2;
+    constexpr int MMAX = (KF + SS - 1) / SS;
+    constexpr int SEGA = kTB + MMAX - 1;
+    constexpr int SEGA4 = (SEGA + 3) / 4;
+    const int rb = (b + PB) % SS, db = (b + PB) / SS;
+    const int Mb = (KF - rb + SS - 1) / SS;
+    const float* row = G + (I + gp0) * GW + (X0 + db - (Mb - 1) + gp0);
+    float seg[SEGA4 * 4];
+    if (aligned) {
+        const float4* row4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+        for (int c = 0; c < SEGA4; ++c) {
+            float4 v = row4[c];
+            seg[4 * c] = v.x; seg[4 * c + 1] = v.y; seg[4 * c + 2] = v.z; seg[4 * c + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < SEGA4 * 4; ++j) seg[j] = (j < kTB + Mb - 1) ? row[j] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < kMaxRank; ++r) {
+        if (r < R) {
+#pragma unroll
+            for (int j = 0; j < kTB; ++j) u[r][j] = 0.0f;
+#pragma unroll
+            for (int n = 0; n < MMAX; ++n) {
+                if (n < Mb) {
+                    const float w = sb[r * KF + SS * n + rb];
+#pragma unroll
+                    for (int j = 0; j < kTB; ++j) u[r][j] = fmaf(w, seg[j + Mb - 1 - n], u[r][j]);
+                }
+            }
+        }
+    }
+}
+
+template <int SS, int KF>
+__device__ __forceinline__ float sep_gsub(const float* __restrict__ U, const float* __restrict__ sa, int R, int UR,
+                                          int UW, int up0, int a, int Yc, int x) {
+    constexpr int PB = (KF - SS) / 2;
+    constexpr int MMAX = (KF + SS - 1) / SS;
+    const int ra = (a + PB) % SS, da = (a + PB) / SS;
+    const int Ma = (KF - ra + SS - 1) / SS;
+    float g = 0.0f;
+#pragma unroll
+    for (int r = 0; r < kMaxRank; ++r) {
+        if (r < R) {
+            const float* col = U + r * UR * UW + (Yc + da + up0) * UW + x;
+#pragma unroll
+            for (int m = 0; m < MMAX; ++m)
+                if (m < Ma) g = fmaf(sa[r * KF + SS * m + ra], col[-m * UW], g);
+        }
+    }
+    return g;
+}
+
 // ---------------------------------------------------------------------------
 // Per-sub-pixel model evaluation with the slot structure of M.
 // ---------------------------------------------------------------------------
@@ -420,51 +543,111 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
     __syncthreads();
 
     // ---- C: conv + pool + likelihood ------------------------------------------
-    const int NSTR = (N + kRB - 1) / kRB;
-    const int ntask = N * NSTR;
-    float gv[kMaxT][kRB];
     float ll_acc = 0.0f, chi_acc = 0.0f;
     const float cf = P.cf;
     const float inv_np = 1.0f / (float)(N * N);
     const float glp = A.mode == M_MAP ? A.gscale : (A.g_lp ? A.g_lp[pid] : 1.0f);
     const float gch = (A.mode == M_LOGPROB && A.g_chi2) ? A.g_chi2[pid] : 0.0f;
     const int ssk = ss * 1000 + P.kf;
+    const int sepk = P.sep_rank > 0 ? ssk : 0;  // separable path specialisation (0: direct)
+    const int R = P.sep_rank;
+
+    // per-output pixel: write the image or the likelihood terms; returns cf * dL/dimage
+    auto out_pixel = [&](int I, int J, float conv) -> float {
+        const float im = cf * conv;
+        const int pix = I * N + J;
+        if (A.mode == M_SIM) {
+            A.img[(size_t)pid * N * N + pix] = im;
+            return 0.0f;
+        }
+        if (A.mode == M_VJP) return cf * A.g_img[(size_t)pid * N * N + pix];
+        float ll, r2, dll, dr2;
+        if (P.lik == GL_LIK_FORWARD)
+            glm::lik_forward(im, P.obs[pix], P.bkg2, P.inv_texp, ll, r2, dll, dr2);
+        else
+            glm::lik_backward(im, P.obs[pix], P.inv_var_obs[pix], P.log_sig_obs[pix], ll, r2, dll, dr2);
+        ll_acc += ll;
+        chi_acc += r2;
+        return cf * (glp * dll + gch * inv_np * dr2);
+    };
+
+    // forward outputs of this thread; gv[tt][r] belongs to output
+    //   direct:    (I, J0 + r) with task = I*NSTR + J0/kRB
+    //   separable: (I0 + r, J) with task = (I0/kRV)*N + J
+    float gv[kMaxT][kRB];
+    const int NSTR = (N + kRB - 1) / kRB;
+    const int ntask = N * NSTR;
+    static_assert(kRB == kRV, "register layout shared by both paths");
 #pragma unroll
-    for (int tt = 0; tt < kMaxT; ++tt) {
-        const int task = tid + tt * BLOCK;
+    for (int tt = 0; tt < kMaxT; ++tt)
 #pragma unroll
         for (int r = 0; r < kRB; ++r) gv[tt][r] = 0.0f;
-        if (task < ntask) {
-            const int I = task / NSTR;
-            const int J0 = (task - I * NSTR) * kRB;
-            float acc[kRB];
-            const float* base = s_sub + P.sub_base - P.pb;  // column offset -pb folded in
-            if (ssk == 2022) conv_strip_static<2, 22, kRB>(base, P.w, PW, I, J0, acc);
-            else if (ssk == 2026) conv_strip_static<2, 26, kRB>(base, P.w, PW, I, J0, acc);
-            else if (ssk == 1021) conv_strip_static<1, 21, kRB>(base, P.w, PW, I, J0, acc);
-            else conv_strip_dyn<kRB>(base, P.w, PW, ss, P.kf, I, J0, acc);
+
+    if (sepk) {
+        // C1: horizontal pass over all padded sub rows -> registers
+        const int TR = P.nsub + P.k - 1;
+        const int TW = N;
+        const int htask = TR * NSTR;
+        float tv[kMaxTT][kMaxRank][kTB];
 #pragma unroll
-            for (int r = 0; r < kRB; ++r) {
-                const int J = J0 + r;
-                if (J < N) {
-                    const float im = cf * acc[r];
-                    const int pix = I * N + J;
-                    if (A.mode == M_SIM) {
-                        A.img[(size_t)pid * N * N + pix] = im;
-                    } else if (A.mode == M_VJP) {
-                        gv[tt][r] = cf * A.g_img[(size_t)pid * N * N + pix];
-                    } else {
-                        float ll, r2, dll, dr2;
-                        if (P.lik == GL_LIK_FORWARD)
-                            glm::lik_forward(im, P.obs[pix], P.bkg2, P.inv_texp, ll, r2, dll, dr2);
-                        else
-                            glm::lik_backward(im, P.obs[pix], P.inv_var_obs[pix], P.log_sig_obs[pix], ll, r2, dll,
-                                              dr2);
-                        ll_acc += ll;
-                        chi_acc += r2;
-                        gv[tt][r] = cf * (glp * dll + gch * inv_np * dr2);
-                    }
-                }
+        for (int tt = 0; tt < kMaxTT; ++tt) {
+            const int task = tid + tt * BLOCK;
+            if (task < htask) {
+                const int yy = task / NSTR;
+                const int J0 = (task - yy * NSTR) * kTB;
+                if (sepk == 2022) sep_hpass<2, 22>(s_sub, P.sb, R, PW, yy, J0, tv[tt]);
+                else if (sepk == 1021) sep_hpass<1, 21>(s_sub, P.sb, R, PW, yy, J0, tv[tt]);
+                else sep_hpass<3, 23>(s_sub, P.sb, R, PW, yy, J0, tv[tt]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < kMaxTT; ++tt) {
+            const int task = tid + tt * BLOCK;
+            if (task < htask) {
+                const int yy = task / NSTR;
+                const int J0 = (task - yy * NSTR) * kTB;
+#pragma unroll
+                for (int r = 0; r < kMaxRank; ++r)
+                    if (r < R)
+#pragma unroll
+                        for (int j = 0; j < kTB; ++j)
+                            if (J0 + j < N) s_sub[r * TR * TW + yy * TW + J0 + j] = tv[tt][r][j];
+            }
+        }
+        __syncthreads();
+        // C2: vertical pass, lanes along J
+#pragma unroll
+        for (int tt = 0; tt < kMaxT; ++tt) {
+            const int task = tid + tt * BLOCK;
+            if (task < ntask) {
+                const int I0 = (task / N) * kRV;
+                const int J = task - (task / N) * N;
+                float o[kRV];
+                if (sepk == 2022) sep_vpass<2, 22>(s_sub, P.sa, R, TR, TW, I0, J, o);
+                else if (sepk == 1021) sep_vpass<1, 21>(s_sub, P.sa, R, TR, TW, I0, J, o);
+                else sep_vpass<3, 23>(s_sub, P.sa, R, TR, TW, I0, J, o);
+#pragma unroll
+                for (int i = 0; i < kRV; ++i)
+                    if (I0 + i < N) gv[tt][i] = out_pixel(I0 + i, J, o[i]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int tt = 0; tt < kMaxT; ++tt) {
+            const int task = tid + tt * BLOCK;
+            if (task < ntask) {
+                const int I = task / NSTR;
+                const int J0 = (task - I * NSTR) * kRB;
+                float acc[kRB];
+                const float* base = s_sub + P.sub_base - P.pb;  // column offset -pb folded in
+                if (ssk == 2022) conv_strip_static<2, 22, kRB>(base, P.w, PW, I, J0, acc);
+                else if (ssk == 2026) conv_strip_static<2, 26, kRB>(base, P.w, PW, I, J0, acc);
+                else if (ssk == 1021) conv_strip_static<1, 21, kRB>(base, P.w, PW, I, J0, acc);
+                else conv_strip_dyn<kRB>(base, P.w, PW, ss, P.kf, I, J0, acc);
+#pragma unroll
+                for (int r = 0; r < kRB; ++r)
+                    if (J0 + r < N) gv[tt][r] = out_pixel(I, J0 + r, acc[r]);
             }
         }
     }
@@ -475,7 +658,7 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
 #pragma unroll
     for (int e = 0; e < M::NACC; ++e) acc[e] = 0.0f;
 
-    if (GRAD && A.mode != M_SIM) {
+    if (GRAD) {
         __syncthreads();
         for (int i = tid; i < P.g_size; i += BLOCK) s_sub[i] = 0.0f;
         __syncthreads();
@@ -484,36 +667,105 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
         for (int tt = 0; tt < kMaxT; ++tt) {
             const int task = tid + tt * BLOCK;
             if (task < ntask) {
-                const int I = task / NSTR;
-                const int J0 = (task - I * NSTR) * kRB;
+                if (sepk) {
+                    const int I0 = (task / N) * kRV;
+                    const int J = task - (task / N) * N;
 #pragma unroll
-                for (int r = 0; r < kRB; ++r)
-                    if (J0 + r < N) s_sub[(I + gp0) * GW + (J0 + r + gp0)] = gv[tt][r];
+                    for (int i = 0; i < kRV; ++i)
+                        if (I0 + i < N) s_sub[(I0 + i + gp0) * GW + (J + gp0)] = gv[tt][i];
+                } else {
+                    const int I = task / NSTR;
+                    const int J0 = (task - I * NSTR) * kRB;
+#pragma unroll
+                    for (int r = 0; r < kRB; ++r)
+                        if (J0 + r < N) s_sub[(I + gp0) * GW + (J0 + r + gp0)] = gv[tt][r];
+                }
             }
         }
         __syncthreads();
 
-        // ---- D + E: adjoint conv (polyphase) and recompute ---------------------
-        const int XSTR = (N + kRB2 - 1) / kRB2;
-        const int ptask = N * XSTR;
-        for (int a = 0; a < ss; ++a) {
-            for (int b = 0; b < ss; ++b) {
-                for (int task = tid; task < ptask; task += BLOCK) {
-                    const int Yc = task / XSTR;
-                    const int X0 = (task - Yc * XSTR) * kRB2;
-                    float gs[kRB2];
-                    if (ssk == 2022) adj_strip_static<2, 22, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
-                    else if (ssk == 2026) adj_strip_static<2, 26, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
-                    else if (ssk == 1021) adj_strip_static<1, 21, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
-                    else adj_strip_dyn<kRB2>(s_sub, P.w, GW, gp0, ss, P.kf, P.pb, a, b, Yc, X0, gs);
-                    const int yi = ss * Yc + a;
+        if (sepk) {
+            // ---- D1: transposed horizontal pass G -> U (registers), lanes along X ----
+            const int XS = (N + kTB - 1) / kTB;
+            const int utask = N * ss * XS;
+            float uv[kMaxU][kMaxRank][kTB];
 #pragma unroll
-                    for (int r = 0; r < kRB2; ++r) {
-                        const int xj = ss * (X0 + r) + b;
-                        if (X0 + r < N) {
-                            float X = fmaf(P.M00, (float)xj, fmaf(P.M01, (float)yi, P.ox));
-                            float Y = fmaf(P.M10, (float)xj, fmaf(P.M11, (float)yi, P.oy));
-                            eval_adjoint<M>(C, X, Y, gs[r], acc);
+            for (int tt = 0; tt < kMaxU; ++tt) {
+                const int task = tid + tt * BLOCK;
+                if (task < utask) {
+                    const int I = task / (ss * XS);
+                    const int rem = task - I * ss * XS;
+                    const int b = rem / XS;
+                    const int X0 = (rem - b * XS) * kTB;
+                    if (sepk == 2022) sep_upass<2, 22>(s_sub, P.sb, R, GW, gp0, P.adj_aligned, b, I, X0, uv[tt]);
+                    else if (sepk == 1021) sep_upass<1, 21>(s_sub, P.sb, R, GW, gp0, P.adj_aligned, b, I, X0, uv[tt]);
+                    else sep_upass<3, 23>(s_sub, P.sb, R, GW, gp0, P.adj_aligned, b, I, X0, uv[tt]);
+                }
+            }
+            __syncthreads();
+            const int UR = P.urows, UW = P.nsub, up0 = gp0;
+            // zero the padding rows of U, then store
+            for (int i = tid; i < R * UR * UW; i += BLOCK) {
+                const int row = (i / UW) % UR;
+                if (row < up0 || row >= up0 + N) s_sub[i] = 0.0f;
+            }
+#pragma unroll
+            for (int tt = 0; tt < kMaxU; ++tt) {
+                const int task = tid + tt * BLOCK;
+                if (task < utask) {
+                    const int I = task / (ss * XS);
+                    const int rem = task - I * ss * XS;
+                    const int b = rem / XS;
+                    const int X0 = (rem - b * XS) * kTB;
+#pragma unroll
+                    for (int r = 0; r < kMaxRank; ++r)
+                        if (r < R)
+#pragma unroll
+                            for (int j = 0; j < kTB; ++j)
+                                if (X0 + j < N) s_sub[r * UR * UW + (I + up0) * UW + ss * (X0 + j) + b] = uv[tt][r][j];
+                }
+            }
+            __syncthreads();
+            // ---- D2 + E: per sub-pixel transposed vertical pass + recompute, one row per wave ----
+            const int lane_ = tid & 63, wave_ = tid >> 6;
+            const int H = (nsub + 63) / 64;
+            for (int wt = wave_; wt < nsub * H; wt += BLOCK / 64) {
+                const int y = wt / H;
+                const int x = (wt - y * H) * 64 + lane_;
+                const int a = y % ss, Yc = y / ss;
+                if (x < nsub) {
+                    float g;
+                    if (sepk == 2022) g = sep_gsub<2, 22>(s_sub, P.sa, R, UR, UW, up0, a, Yc, x);
+                    else if (sepk == 1021) g = sep_gsub<1, 21>(s_sub, P.sa, R, UR, UW, up0, a, Yc, x);
+                    else g = sep_gsub<3, 23>(s_sub, P.sa, R, UR, UW, up0, a, Yc, x);
+                    float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
+                    float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
+                    eval_adjoint<M>(C, X, Y, g, acc);
+                }
+            }
+        } else {
+            // ---- D + E: adjoint conv (polyphase) and recompute ---------------------
+            const int XSTR = (N + kRB2 - 1) / kRB2;
+            const int ptask = N * XSTR;
+            for (int a = 0; a < ss; ++a) {
+                for (int b = 0; b < ss; ++b) {
+                    for (int task = tid; task < ptask; task += BLOCK) {
+                        const int Yc = task / XSTR;
+                        const int X0 = (task - Yc * XSTR) * kRB2;
+                        float gs[kRB2];
+                        if (ssk == 2022) adj_strip_static<2, 22, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
+                        else if (ssk == 2026) adj_strip_static<2, 26, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
+                        else if (ssk == 1021) adj_strip_static<1, 21, kRB2>(s_sub, P.w, GW, gp0, P.adj_aligned, a, b, Yc, X0, gs);
+                        else adj_strip_dyn<kRB2>(s_sub, P.w, GW, gp0, ss, P.kf, P.pb, a, b, Yc, X0, gs);
+                        const int yi = ss * Yc + a;
+#pragma unroll
+                        for (int r = 0; r < kRB2; ++r) {
+                            const int xj = ss * (X0 + r) + b;
+                            if (X0 + r < N) {
+                                float X = fmaf(P.M00, (float)xj, fmaf(P.M01, (float)yi, P.ox));
+                                float Y = fmaf(P.M10, (float)xj, fmaf(P.M11, (float)yi, P.oy));
+                                eval_adjoint<M>(C, X, Y, gs[r], acc);
+                            }
                         }
                     }
                 }
@@ -529,7 +781,7 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
     if (lane == 0) s_red[wave * M::NRED + 0] = part;
     part = wave_sum(chi_acc);
     if (lane == 0) s_red[wave * M::NRED + 1] = part;
-    if (GRAD && A.mode != M_SIM) {
+    if (GRAD) {
 #pragma unroll
         for (int e = 0; e < M::NACC; ++e) {
             part = wave_sum(acc[e]);

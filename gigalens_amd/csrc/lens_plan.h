@@ -37,6 +37,10 @@ struct DPlan {
     const int* prior_kind;       // d
     const float* prior_args;     // d*8
     const int* batch_n;          // device int per EPL slot when batch_series (else null)
+    int sep_rank;                // > 0: separable PSF path with this many rank-1 terms
+    int urows;                   // rows of the transposed-horizontal buffer U (separable adjoint)
+    const float* sa;             // sep_rank x kf vertical factors  (W ~= sum_r sa_r sb_r^T)
+    const float* sb;             // sep_rank x kf horizontal factors
     DComp comp[kMaxComp];
 };
 

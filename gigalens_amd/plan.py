@@ -91,7 +91,8 @@ def _dev(t: torch.Tensor, name: str) -> torch.Tensor:
 
 class Plan:
     def __init__(self, phys_model, camera, layout: Layout, prior=None, observed=None, background_rms=1.0,
-                 exp_time=1.0, likelihood=nat.GL_LIK_FORWARD, fldj_at_x=False, epl_batch_series=False):
+                 exp_time=1.0, likelihood=nat.GL_LIK_FORWARD, fldj_at_x=False, epl_batch_series=False,
+                 psf_rank_tol=0.0):
         self.layout = layout
         self.d = len(layout)
         self.N = camera.num_pix
@@ -165,6 +166,7 @@ class Plan:
         desc.likelihood = int(likelihood)
         desc.fldj_at_x = int(bool(fldj_at_x))
         desc.epl_batch_series = int(bool(epl_batch_series))
+        desc.psf_rank_tol = float(psf_rank_tol)
         lib = nat.lib()
         h = ctypes.c_void_p()
         nat.check(lib.gl_plan_create(ctypes.byref(desc), ctypes.byref(h)), "gl_plan_create")

@@ -94,6 +94,8 @@ typedef struct gl_plan_desc {
     int likelihood;           /* gl_likelihood */
     int fldj_at_x;            /* 1: evaluate fldj at constrained x (tf/model.py:160 quirk) */
     int epl_batch_series;     /* 1: reference's batch-coupled EPL series length (epl.py:41) */
+    float psf_rank_tol;       /* separable PSF: keep singular values >= tol * s_max of the folded
+                                 PSF; 0 selects 1e-6, a negative value forces the direct 2-D path */
 } gl_plan_desc;
 
 typedef struct gl_plan gl_plan;
