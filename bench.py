@@ -35,6 +35,18 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 (vector / matrix)
 BASELINE_EVALS = 2200.0   # reference notebook MAP throughput (tf-demo.ipynb:808)
 
 
+def measured_traffic():
+    """HBM bytes per launch of the particle kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated)."""
+    import re
+    f = os.path.join(ROOT, "profiles", "particle_kernel_map4000.md")
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(f).read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def flat_sample(prior, n, seed):
     s = prior.sample(n, seed=seed)
     return torch.stack([torch.as_tensor(v, dtype=torch.float32) for v in tfd.flatten(s)], dim=1)
@@ -155,7 +167,8 @@ def main():
         e1 = xs[0][0]["e1"].numpy()
         e2 = xs[0][0]["e2"].numpy()
         n = fl.series_length(e1, e2, 50)
-        c = fl.per_particle(kinds, roles, cfg.num_pix, cfg.supersample, ksz, n, grad=True)
+        c = fl.per_particle(kinds, roles, cfg.num_pix, cfg.supersample, ksz, n, grad=True,
+                            sep_rank=plan.info()["sep_rank"])
         tot += 0.5 * float(c["flops"].sum())
         tot_tr += 0.5 * float(c["transcendentals"].sum())
         tot_survey += 0.5 * float(fl.survey_formula_per_particle(cfg.num_pix, cfg.supersample, ksz, n).sum())
@@ -184,9 +197,12 @@ def main():
                    "particles": bs, "num_pix": cfg.num_pix, "supersample": cfg.supersample,
                    "psf_subgrid": ksz, "params": z0.shape[1]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": measured_traffic() if bs == 4000 else None, "traffic_unit": "bytes",
                      "kernel": "gl::particle_kernel<512, EPL+SHEAR | SERSIC_ELLIPSE | SERSIC_ELLIPSE, grad>",
                      "kernel_ms": kern_ms, "flop_per_launch": tot, "transcendentals_per_launch": tot_tr,
+                     "psf_path": ("separable rank %d" % plan.info()["sep_rank"]) if plan.info()["sep_rank"]
+                     else "direct 2-D",
                      "reference_formulation_flop_per_launch": tot_survey},
         "time_to_map": {"chi2_threshold": 1.1,
                         "steps": int(hit[0]) + 1 if len(hit) else None,

@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
     "gl_last_error",
     "gl_plan_create",
     "gl_plan_destroy",
+    "gl_plan_query",
     "gl_simulate",
     "gl_simulate_vjp",
     "gl_log_prob",
@@ -81,6 +82,11 @@ class PlanDesc(ctypes.Structure):
     ]
 
 
+class PlanInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("num_pix", "supersample", "n_params", "kernel_size", "folded_size",
+                                            "sep_rank", "lds_bytes", "block")]
+
+
 _lib: Optional[ctypes.CDLL] = None
 
 _vp = ctypes.c_void_p
@@ -88,6 +94,7 @@ _SIGS = {
     "gl_last_error": ([], ctypes.c_char_p),
     "gl_plan_create": ([ctypes.POINTER(PlanDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "gl_plan_destroy": ([_vp], ctypes.c_int),
+    "gl_plan_query": ([_vp, ctypes.POINTER(PlanInfo)], ctypes.c_int),
     "gl_simulate": ([_vp, _vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "gl_simulate_vjp": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
     "gl_log_prob": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),

@@ -459,6 +459,20 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     return GL_OK;
 }
 
+extern "C" int gl_plan_query(const gl_plan* plan, gl_plan_info* info) {
+    if (!plan || !info) return fail(GL_ERR_ARG, "gl_plan_query: null argument");
+    const gl::DPlan& P = plan->P;
+    info->num_pix = P.N;
+    info->supersample = P.ss;
+    info->n_params = P.d;
+    info->kernel_size = P.k;
+    info->folded_size = P.kf;
+    info->sep_rank = P.sep_rank;
+    info->lds_bytes = (int)plan->lds_bytes;
+    info->block = kBlock;
+    return GL_OK;
+}
+
 static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream) {
     if (!plan) return fail(GL_ERR_ARG, "null plan");
     if (A.bs <= 0) return fail(GL_ERR_ARG, "bs must be positive");

@@ -52,8 +52,27 @@ def series_length(e1: np.ndarray, e2: np.ndarray, niter: int) -> np.ndarray:
     return np.minimum(n, niter).astype(np.int64)
 
 
+def conv_flops(num_pix: int, ss: int, kernel_size: int, sep_rank: int = 0, grad: bool = False) -> float:
+    """Flops of the PSF+pool convolution of one particle (forward, plus the
+    adjoint when ``grad``).  Direct path: 2 kf^2 per output pixel each way.
+    Separable rank-R path: horizontal pass over the (ss N + k - 1) x N
+    intermediate and vertical pass over N x N, 2 kf flops per tap each; the
+    adjoint passes use ceil(kf/ss) taps over N x ssN and ssN x ssN."""
+    N = num_pix
+    k = kernel_size if kernel_size > 0 else 1
+    kf = k + ss - 1
+    if sep_rank <= 0:
+        fwd = 2.0 * kf * kf * N * N
+        adj = 2.0 * kf * kf * N * N
+    else:
+        m = -(-kf // ss)
+        fwd = sep_rank * (2.0 * kf * (ss * N + k - 1) * N + 2.0 * kf * N * N)
+        adj = sep_rank * (2.0 * m * N * ss * N + 2.0 * m * (ss * N) ** 2)
+    return fwd + (adj if grad else 0.0)
+
+
 def per_particle(kinds: Sequence[int], roles: Sequence[int], num_pix: int, ss: int, kernel_size: int,
-                 n_terms: np.ndarray, grad: bool) -> Dict[str, np.ndarray]:
+                 n_terms: np.ndarray, grad: bool, sep_rank: int = 0) -> Dict[str, np.ndarray]:
     """Flops and transcendentals per particle (arrays over particles)."""
     S = (ss * num_pix) ** 2
     P = num_pix ** 2
@@ -68,10 +87,10 @@ def per_particle(kinds: Sequence[int], roles: Sequence[int], num_pix: int, ss: i
         tr += t
         if kind == nat.GL_EPL:
             f += 3 * n_terms
-    flops = S * f + P * (2 * kf * kf + 1 + 12)
+    flops = S * f + P * (1 + 12) + conv_flops(num_pix, ss, kernel_size, sep_rank, grad)
     trans = S * tr + P * 1
     if grad:
-        g = np.full_like(n_terms, 8.0 + 2.0 + 2.0 * kf * kf / (ss * ss))
+        g = np.full_like(n_terms, 8.0 + 2.0)
         gt = np.zeros_like(n_terms)
         for kind, role in zip(kinds, roles):
             a, t = _ADJ[kind]

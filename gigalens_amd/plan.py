@@ -182,6 +182,11 @@ class Plan:
                 pass
             self._h = None
 
+    def info(self) -> dict:
+        pi = nat.PlanInfo()
+        nat.check(self._lib.gl_plan_query(self._h, ctypes.byref(pi)), "gl_plan_query")
+        return {n: getattr(pi, n) for n, _ in pi._fields_}
+
     # ---- launches --------------------------------------------------------
     def simulate(self, x: torch.Tensor) -> torch.Tensor:
         x = _dev(x, "x")

@@ -100,6 +100,16 @@ typedef struct gl_plan_desc {
 
 typedef struct gl_plan gl_plan;
 
+/* Read-only facts about a built plan (for tooling / FLOP accounting). */
+typedef struct gl_plan_info {
+    int num_pix, supersample, n_params;
+    int kernel_size;      /* sub-grid PSF size k */
+    int folded_size;      /* kf = k + ss - 1 */
+    int sep_rank;         /* > 0: separable PSF path with this rank, 0: direct 2-D path */
+    int lds_bytes;        /* LDS per workgroup of the particle kernel */
+    int block;            /* threads per workgroup */
+} gl_plan_info;
+
 /* Message of the last failing call on this thread. */
 const char* gl_last_error(void);
 
@@ -110,6 +120,7 @@ const char* gl_last_error(void);
  * (tf/model.py:29-51, 109-132). */
 int gl_plan_create(const gl_plan_desc* desc, gl_plan** out);
 int gl_plan_destroy(gl_plan* plan);
+int gl_plan_query(const gl_plan* plan, gl_plan_info* info);
 
 /* Batched forward model on physical parameters x (bs x d):
  * img (bs x N x N).  Replaces LensSimulator.simulate (tf/simulator.py:65-97). */
