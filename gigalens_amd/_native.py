@@ -33,6 +33,7 @@ EXPORTED_SYMBOLS = (
     "gl_simulate_vjp",
     "gl_log_prob",
     "gl_map_adam_step",
+    "gl_lstsq_simulate",
 )
 
 
@@ -79,6 +80,8 @@ class PlanDesc(ctypes.Structure):
         ("fldj_at_x", ctypes.c_int),
         ("epl_batch_series", ctypes.c_int),
         ("psf_rank_tol", ctypes.c_float),
+        ("max_batch", ctypes.c_int),
+        ("noise_map", ctypes.POINTER(ctypes.c_float)),
     ]
 
 
@@ -98,6 +101,7 @@ _SIGS = {
     "gl_simulate": ([_vp, _vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "gl_simulate_vjp": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
     "gl_log_prob": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "gl_lstsq_simulate": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
     "gl_map_adam_step": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
 }

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "lens_kernels.hip"
+#include "lstsq_kernels.hip"
 
 namespace {
 
@@ -80,6 +81,11 @@ struct gl_plan {
     const ModelEntry* model;
     size_t lds_bytes;
     int epl_batch;
+    int mc;                  // 1: multi-channel least-squares plan
+    gl::McPlan M;
+    int max_batch;
+    int kf4;                 // conv template parameter of the multi-channel path
+    size_t gram_lds, solve_lds;
     std::vector<void*> dev;  // owned device buffers
 };
 
@@ -101,6 +107,7 @@ static int n_params_of(int kind) {
     case glm::K_SIS: return 3;
     case glm::K_SERSIC: return 5;
     case glm::K_SERSIC_ELLIPSE: return 7;
+    case glm::K_SHAPELETS: return 3;
     default: return -1;
     }
 }
@@ -170,6 +177,262 @@ extern "C" int gl_plan_destroy(gl_plan* plan) {
     return GL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Multi-channel (least-squares) plans
+// ---------------------------------------------------------------------------
+static int upload_common(const gl_plan_desc* desc, gl_plan* p);
+template <int SS, int KF4>
+static hipError_t launch_mc_conv(int grid, hipStream_t st, const gl::DPlan& P, const gl::McPlan& M, int nb) {
+    hipLaunchKernelGGL((gl::mc_conv<SS, KF4>), dim3(grid), dim3(256), 0, st, P, M, nb);
+    return hipGetLastError();
+}
+
+static hipError_t mc_conv_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::DPlan& P,
+                                   const gl::McPlan& M, int nb) {
+#define GL_MCC(S, K) if (ss == S && kf4 == K) return launch_mc_conv<S, K>(grid, st, P, M, nb)
+    GL_MCC(1, 8); GL_MCC(1, 16); GL_MCC(1, 22); GL_MCC(1, 32);
+    GL_MCC(2, 8); GL_MCC(2, 16); GL_MCC(2, 22); GL_MCC(2, 32);
+    GL_MCC(3, 8); GL_MCC(3, 16); GL_MCC(3, 22); GL_MCC(3, 32);
+    GL_MCC(4, 8); GL_MCC(4, 16); GL_MCC(4, 22); GL_MCC(4, 32);
+#undef GL_MCC
+    return hipErrorInvalidValue;
+}
+
+static int pick_kf4(int kf) {
+    const int need = (kf + 3) / 4;
+    for (int c : {8, 16, 22, 32})
+        if (need <= c) return c;
+    return -1;
+}
+
+// Orthonormal Hermite functions on linspace(-5, 5, 6000) (lenstronomy Shapelets.phi_n,
+// shapelets.py:39-40) by the stable three-term recurrence in FP64.
+static void hermite_table(int n_max, std::vector<float>& tab) {
+    const int T = glm::kShapeletTable;
+    tab.assign((size_t)(n_max + 1) * T, 0.0f);
+    for (int i = 0; i < T; ++i) {
+        const double x = -5.0 + 10.0 * i / (T - 1);
+        double pm2 = 0.0, pm1 = pow(M_PI, -0.25) * exp(-0.5 * x * x);
+        tab[i] = (float)pm1;
+        for (int n = 1; n <= n_max; ++n) {
+            const double pn = sqrt(2.0 / n) * x * pm1 - sqrt((n - 1.0) / n) * pm2;
+            tab[(size_t)n * T + i] = (float)pn;
+            pm2 = pm1;
+            pm1 = pn;
+        }
+    }
+}
+
+static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<float>& W, int max_niter) {
+    gl::DPlan& P = p->P;
+    gl::McPlan& M = p->M;
+    memset(&M, 0, sizeof(M));
+    const int N = P.N, ss = P.ss, nsub = P.nsub, kf = P.kf, pb = P.pb;
+    int rc;
+    void* dptr = nullptr;
+    if (ss > 4) return fail(GL_ERR_UNSUPPORTED, "least-squares plans support supersample <= 4");
+    p->kf4 = pick_kf4(kf);
+    if (p->kf4 < 0) return fail(GL_ERR_UNSUPPORTED, "PSF too large for the least-squares path (kf > 128)");
+    if (P.ncomp > gl::kMaxComp) return fail(GL_ERR_UNSUPPORTED, "too many components");
+    // channels
+    int D = 0, nshp = 0;
+    M.shp_comp = -1;
+    for (int i = P.nlens; i < P.ncomp; ++i) {
+        const gl::DComp& c = P.comp[i];
+        if (c.kind == glm::K_SHAPELETS) {
+            if (++nshp > 1) return fail(GL_ERR_UNSUPPORTED, "at most one shapelet component");
+            if (c.n_max < 0 || c.n_max > glm::kMaxShapeletN)
+                return fail(GL_ERR_UNSUPPORTED, "shapelet n_max must be in [0, " + std::to_string(glm::kMaxShapeletN) + "]");
+            M.shp_comp = i;
+            M.shp_nmax = c.n_max;
+            M.shp_interp = c.interpolate;
+            M.shp_chan0 = D;
+            D += (c.n_max + 1) * (c.n_max + 2) / 2;
+        } else {
+            D += 1;
+        }
+    }
+    if (D < 1 || D > gl::kMcMaxChan) return fail(GL_ERR_UNSUPPORTED, "number of linear channels out of range");
+    M.D = D;
+    // per-particle constants: prior arrays, component blocks, EPL coefficient tables
+    P.max_n = max_niter;
+    const int coef_block = ((max_niter + 4) & ~3) + 4 * (max_niter + 1);
+    const int prior_f = (6 * P.d + 3) & ~3;
+    M.coef_base = prior_f + gl::kMaxComp * gl::kConstStride;
+    int coff = 0;
+    for (int i = 0; i < 3; ++i) {
+        P.coef_off[i] = coff;
+        if (i < P.nlens && P.comp[i].kind == glm::K_EPL) coff += (coef_block + 3) & ~3;
+    }
+    M.pstride = (M.coef_base + coff + 63) & ~63;
+    // staged row width: lanes read SS*J + q for q < 4*KF4, J < N
+    M.BW = ((ss * (N - 1) + 4 * p->kf4 > pb + nsub ? ss * (N - 1) + 4 * p->kf4 : pb + nsub) + 3) & ~3;
+    // padded dL/dimage
+    int gp = 0;
+    for (int a = 0; a < ss; ++a) {
+        const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
+        if (Ma - 1 - da > gp) gp = Ma - 1 - da;
+        if (da > gp) gp = da;
+    }
+    M.gp = gp;
+    const int GW = N + 2 * gp;
+    const int mb = desc->max_batch > 0 ? desc->max_batch : 64;
+    p->max_batch = mb;
+    M.rcond = 1e-6f;
+    // folded weights padded to the conv's row width
+    M.wpad_w = 4 * p->kf4;
+    std::vector<float> wp((size_t)kf * M.wpad_w, 0.0f);
+    for (int a = 0; a < kf; ++a)
+        for (int b = 0; b < kf; ++b) wp[(size_t)a * M.wpad_w + b] = W[a * kf + b];
+    if ((rc = upload(p, wp.data(), wp.size() * sizeof(float), &dptr))) return rc;
+    M.wpad = (const float*)dptr;
+    if ((rc = upload(p, W.data(), W.size() * sizeof(float), &dptr))) return rc;
+    P.w = (const float*)dptr;
+    if ((rc = upload_common(desc, p))) return rc;
+    // shapelet tables
+    if (M.shp_comp >= 0) {
+        std::vector<float> tab;
+        hermite_table(M.shp_nmax, tab);
+        if ((rc = upload(p, tab.data(), tab.size() * sizeof(float), &dptr))) return rc;
+        M.shp_table = (const float*)dptr;
+        std::vector<float> pref(M.shp_nmax + 2);
+        for (int n = 0; n <= M.shp_nmax + 1; ++n)
+            pref[n] = (float)(1.0 / sqrt(pow(2.0, n) * sqrt(M_PI) * tgamma(n + 1.0)));
+        if ((rc = upload(p, pref.data(), pref.size() * sizeof(float), &dptr))) return rc;
+        M.shp_pref = (const float*)dptr;
+    }
+    // workspace for max_batch particles (allocated here once; calls never allocate)
+    auto alloc = [&](size_t bytes, void** outp, bool zero) -> int {
+        void* d = nullptr;
+        GL_HIP(hipMalloc(&d, bytes ? bytes : 4));
+        p->dev.push_back(d);
+        if (zero) GL_HIP(hipMemset(d, 0, bytes));
+        *outp = d;
+        return GL_OK;
+    };
+    if ((rc = alloc((size_t)mb * M.pstride * 4, &dptr, true))) return rc;
+    M.consts = (float*)dptr;
+    if ((rc = alloc((size_t)mb * D * nsub * M.BW * 4, &dptr, true))) return rc;  // zero margins stay zero
+    M.B = (float*)dptr;
+    if ((rc = alloc((size_t)mb * D * N * N * 4, &dptr, false))) return rc;
+    M.X = (float*)dptr;
+    if ((rc = alloc((size_t)mb * D * D * 8, &dptr, false))) return rc;
+    M.G = (double*)dptr;
+    if ((rc = alloc((size_t)mb * D * 8, &dptr, false))) return rc;
+    M.rhs = (double*)dptr;
+    if ((rc = alloc((size_t)mb * D * 4, &dptr, false))) return rc;
+    M.coef = (float*)dptr;
+    if ((rc = alloc((size_t)mb * GW * GW * 4, &dptr, true))) return rc;  // zero margins stay zero
+    M.gimg = (float*)dptr;
+    if ((rc = alloc((size_t)mb * nsub * nsub * 4, &dptr, false))) return rc;
+    M.gsub = (float*)dptr;
+    if ((rc = alloc((size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
+    M.stats = (float*)dptr;
+    p->gram_lds = (size_t)(D * 64 + 128) * 4;
+    const int n2 = D + (D & 1);
+    p->solve_lds = (size_t)(2 * n2 * n2 + n2) * 8 + (size_t)n2 * 4 + 64;
+    GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_gram), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)p->gram_lds));
+    GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_solve), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)p->solve_lds));
+    p->epl_batch = 0;
+    p->lds_bytes = 0;
+    P.sep_rank = 0;
+    GL_HIP(hipDeviceSynchronize());
+    return GL_OK;
+}
+
+// Run the multi-channel pipeline over bs particles in chunks of max_batch.
+static int run_mc(const gl_plan* plan, const float* in, int use_z, int bs, float* lp, float* chi2, float* img,
+                  float* coef, float* stacked, const float* g_lp, const float* g_chi2, float* grad,
+                  hipStream_t st) {
+    const gl::DPlan& P = plan->P;
+    const gl::McPlan& M = plan->M;
+    const int N = P.N, nsub = P.nsub, D = M.D, d = P.d;
+    const int nib = (N + gl::kMcIB - 1) / gl::kMcIB, njb = (N + 63) / 64;
+    for (int off = 0; off < bs; off += plan->max_batch) {
+        const int nb = bs - off < plan->max_batch ? bs - off : plan->max_batch;
+        gl::McArgs A;
+        memset(&A, 0, sizeof(A));
+        A.in = in + (size_t)off * d;
+        A.use_z = use_z;
+        A.bs = nb;
+        A.lp = lp ? lp + off : nullptr;
+        A.chi2 = chi2 ? chi2 + off : nullptr;
+        A.img = img ? img + (size_t)off * N * N : nullptr;
+        A.coef_out = coef ? coef + (size_t)off * D : nullptr;
+        A.stacked = stacked ? stacked + (size_t)off * N * N * D : nullptr;
+        A.g_lp = g_lp ? g_lp + off : nullptr;
+        A.g_chi2 = g_chi2 ? g_chi2 + off : nullptr;
+        A.grad = grad ? grad + (size_t)off * d : nullptr;
+        hipLaunchKernelGGL(gl::mc_prologue, dim3(nb), dim3(128), 0, st, P, M, A);
+        GL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gl::mc_basis, dim3((nsub * nsub + 255) / 256, nb), dim3(256), 0, st, P, M);
+        GL_HIP(hipGetLastError());
+        const long waves = (long)nb * D * nib * njb;
+        GL_HIP(mc_conv_dispatch(P.ss, plan->kf4, (int)((waves + 3) / 4), st, P, M, nb));
+        hipLaunchKernelGGL(gl::mc_gram, dim3(nb), dim3(256), plan->gram_lds, st, P, M);
+        GL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gl::mc_solve, dim3(nb), dim3(256), plan->solve_lds, st, M, M.rcond);
+        GL_HIP(hipGetLastError());
+        const int want_grad = grad != nullptr;
+        hipLaunchKernelGGL(gl::mc_image, dim3(nb), dim3(256), 0, st, P, M, A, want_grad);
+        GL_HIP(hipGetLastError());
+        if (want_grad) {
+            hipLaunchKernelGGL(gl::mc_adjoint, dim3(nb * nsub * P.ss, (N + 63) / 64), dim3(64), 0, st, P, M);
+            GL_HIP(hipGetLastError());
+            hipLaunchKernelGGL(gl::mc_grad, dim3((nsub * nsub + 255) / 256, nb), dim3(256), 0, st, P, M);
+            GL_HIP(hipGetLastError());
+            hipLaunchKernelGGL(gl::mc_finish, dim3(nb), dim3(128), 0, st, P, M, A);
+            GL_HIP(hipGetLastError());
+        }
+    }
+    return GL_OK;
+}
+
+// Observed image, its noise map (tf/model.py:113-115) and the prior table.
+static int upload_common(const gl_plan_desc* desc, gl_plan* p) {
+    gl::DPlan& P = p->P;
+    const int N = P.N;
+    void* dptr = nullptr;
+    int rc;
+    std::vector<float> obs(N * N, 0.0f), ivar(N * N, 0.0f), lsig(N * N, 0.0f);
+    if (desc->observed) {
+        for (int i = 0; i < N * N; ++i) {
+            obs[i] = desc->observed[i];
+            // sqrt(bkg^2 + clip(obs, 0, inf) / t_exp), tf/model.py:113-115, or the given map
+            float var = desc->noise_map ? desc->noise_map[i] * desc->noise_map[i]
+                                        : P.bkg2 + fmaxf(obs[i], 0.0f) / desc->exp_time;
+            ivar[i] = 1.0f / var;
+            lsig[i] = logf(sqrtf(var));
+        }
+    }
+    if ((rc = upload(p, obs.data(), obs.size() * sizeof(float), &dptr))) return rc;
+    P.obs = (const float*)dptr;
+    if ((rc = upload(p, ivar.data(), ivar.size() * sizeof(float), &dptr))) return rc;
+    P.inv_var_obs = (const float*)dptr;
+    if ((rc = upload(p, lsig.data(), lsig.size() * sizeof(float), &dptr))) return rc;
+    P.log_sig_obs = (const float*)dptr;
+    std::vector<int> pk(P.d, 0);
+    std::vector<float> pa(8 * P.d, 0.0f);
+    if (desc->prior_kind) {
+        for (int j = 0; j < P.d; ++j) {
+            pk[j] = desc->prior_kind[j];
+            for (int t = 0; t < 4; ++t) pa[8 * j + t] = desc->prior_args[4 * j + t];
+            if (pk[j] == GL_PRIOR_TRUNCNORMAL) {
+                double loc = pa[8 * j], sc = pa[8 * j + 1], lo = pa[8 * j + 2], hi = pa[8 * j + 3];
+                double Z = 0.5 * (erf((hi - loc) / sc / sqrt(2.0)) - erf((lo - loc) / sc / sqrt(2.0)));
+                pa[8 * j + 4] = (float)log(Z);
+            }
+        }
+    }
+    if ((rc = upload(p, pk.data(), pk.size() * sizeof(int), &dptr))) return rc;
+    P.prior_kind = (const int*)dptr;
+    if ((rc = upload(p, pa.data(), pa.size() * sizeof(float), &dptr))) return rc;
+    P.prior_args = (const float*)dptr;
+    return GL_OK;
+}
+
 extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     if (!desc || !out) return fail(GL_ERR_ARG, "gl_plan_create: null argument");
     *out = nullptr;
@@ -198,6 +461,9 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     int sig[7] = {0, 0, 0, 0, 0, 0, 0};
     int nl = 0, nll = 0, ns = 0, last_role = 0, max_niter = 0;
     P.ncomp = desc->n_components;
+    p->mc = 0;
+    for (int i = 0; i < desc->n_components; ++i)
+        if (desc->components[i].use_lstsq || desc->components[i].kind == glm::K_SHAPELETS) p->mc = 1;
     for (int i = 0; i < desc->n_components; ++i) {
         const gl_component_desc& c = desc->components[i];
         if (c.role < last_role) {
@@ -223,9 +489,13 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             if (nl >= 3) { gl_plan_destroy(p); return fail(GL_ERR_UNSUPPORTED, "at most 3 lens components"); }
             sig[nl++] = c.kind;
         } else {
-            if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE) {
+            if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE && c.kind != glm::K_SHAPELETS) {
                 gl_plan_destroy(p);
                 return fail(GL_ERR_ARG, "mass profile given as a light component");
+            }
+            if (p->mc && !c.use_lstsq) {
+                gl_plan_destroy(p);
+                return fail(GL_ERR_UNSUPPORTED, "least-squares plans need every light component linear (use_lstsq)");
             }
             if (c.role == GL_ROLE_LENS_LIGHT) {
                 if (nll >= 2) { gl_plan_destroy(p); return fail(GL_ERR_UNSUPPORTED, "at most 2 lens-light components"); }
@@ -249,14 +519,16 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     }
     P.nlens = nl; P.nll = nll; P.nsrc = ns;
     p->model = nullptr;
-    for (const ModelEntry& e : kModels) {
-        bool ok = true;
-        for (int j = 0; j < 7; ++j) ok = ok && e.sig[j] == sig[j];
-        if (ok) { p->model = &e; break; }
-    }
-    if (!p->model) {
-        gl_plan_destroy(p);
-        return fail(GL_ERR_UNSUPPORTED, "no compiled kernel for this combination of profiles");
+    if (!p->mc) {
+        for (const ModelEntry& e : kModels) {
+            bool ok = true;
+            for (int j = 0; j < 7; ++j) ok = ok && e.sig[j] == sig[j];
+            if (ok) { p->model = &e; break; }
+        }
+        if (!p->model) {
+            gl_plan_destroy(p);
+            return fail(GL_ERR_UNSUPPORTED, "no compiled kernel for this combination of profiles");
+        }
     }
 
     // ---- PSF folded with the pooling box (correlation weights) -----------------
@@ -281,6 +553,13 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
                 }
             W[pp * kf + qq] = (float)(acc / (ss * ss));
         }
+
+    if (p->mc) {
+        int rc = mc_create(desc, p, W, max_niter);
+        if (rc) { gl_plan_destroy(p); return rc; }
+        *out = p;
+        return GL_OK;
+    }
 
     // ---- LDS layout --------------------------------------------------------
     const int nsub = P.nsub;
@@ -410,39 +689,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         if ((rc = upload(p, sb.data(), sb.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
         P.sb = (const float*)dptr;
     }
-    std::vector<float> obs(N * N, 0.0f), ivar(N * N, 0.0f), lsig(N * N, 0.0f);
-    if (desc->observed) {
-        for (int i = 0; i < N * N; ++i) {
-            obs[i] = desc->observed[i];
-            // sqrt(bkg^2 + clip(obs, 0, inf) / t_exp), tf/model.py:113-115
-            float var = P.bkg2 + fmaxf(obs[i], 0.0f) / desc->exp_time;
-            ivar[i] = 1.0f / var;
-            lsig[i] = logf(sqrtf(var));
-        }
-    }
-    if ((rc = upload(p, obs.data(), obs.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
-    P.obs = (const float*)dptr;
-    if ((rc = upload(p, ivar.data(), ivar.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
-    P.inv_var_obs = (const float*)dptr;
-    if ((rc = upload(p, lsig.data(), lsig.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
-    P.log_sig_obs = (const float*)dptr;
-    std::vector<int> pk(P.d, 0);
-    std::vector<float> pa(8 * P.d, 0.0f);
-    if (desc->prior_kind) {
-        for (int j = 0; j < P.d; ++j) {
-            pk[j] = desc->prior_kind[j];
-            for (int t = 0; t < 4; ++t) pa[8 * j + t] = desc->prior_args[4 * j + t];
-            if (pk[j] == GL_PRIOR_TRUNCNORMAL) {
-                double loc = pa[8 * j], sc = pa[8 * j + 1], lo = pa[8 * j + 2], hi = pa[8 * j + 3];
-                double Z = 0.5 * (erf((hi - loc) / sc / sqrt(2.0)) - erf((lo - loc) / sc / sqrt(2.0)));
-                pa[8 * j + 4] = (float)log(Z);
-            }
-        }
-    }
-    if ((rc = upload(p, pk.data(), pk.size() * sizeof(int), &dptr))) { gl_plan_destroy(p); return rc; }
-    P.prior_kind = (const int*)dptr;
-    if ((rc = upload(p, pa.data(), pa.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
-    P.prior_args = (const float*)dptr;
+    if ((rc = upload_common(desc, p))) { gl_plan_destroy(p); return rc; }
     p->epl_batch = desc->epl_batch_series;
     P.batch_n = nullptr;
     if (p->epl_batch) {
@@ -475,6 +722,12 @@ extern "C" int gl_plan_query(const gl_plan* plan, gl_plan_info* info) {
 
 static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream) {
     if (!plan) return fail(GL_ERR_ARG, "null plan");
+    if (plan->mc) {
+        if (A.mode != gl::M_LOGPROB)
+            return fail(GL_ERR_UNSUPPORTED, "least-squares plans support gl_log_prob and gl_lstsq_simulate");
+        if (A.bs <= 0 || !A.in) return fail(GL_ERR_ARG, "bad batch");
+        return run_mc(plan, A.in, 1, A.bs, A.lp, A.chi2, nullptr, nullptr, nullptr, A.g_lp, A.g_chi2, A.grad, stream);
+    }
     if (A.bs <= 0) return fail(GL_ERR_ARG, "bs must be positive");
     if (!A.in) return fail(GL_ERR_ARG, "null parameter pointer");
     if (plan->epl_batch) {
@@ -523,6 +776,14 @@ extern "C" int gl_log_prob(const gl_plan* plan, const float* z, int bs, float* l
     A.g_chi2 = g_chi2;
     A.grad = grad_z;
     return run(plan, A, grad_z != nullptr, stream);
+}
+
+extern "C" int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, float* img, float* coeffs,
+                                 float* stacked, hipStream_t stream) {
+    if (!plan) return fail(GL_ERR_ARG, "null plan");
+    if (!plan->mc) return fail(GL_ERR_ARG, "gl_lstsq_simulate needs a plan with linear (use_lstsq) light components");
+    if (bs <= 0 || !x) return fail(GL_ERR_ARG, "bad batch");
+    return run_mc(plan, x, 0, bs, nullptr, nullptr, img, coeffs, stacked, nullptr, nullptr, nullptr, stream);
 }
 
 extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, int step, float lr,

@@ -474,4 +474,69 @@ GLM void lik_backward(float im, float obs, float iv, float log_sig, float& ll, f
     dr2 = 2.0f * d * iv;
 }
 
+
+// ---------------------------------------------------------------------------
+// Shapelets (src/gigalens/tf/profiles/light/shapelets.py:53-85)
+// ---------------------------------------------------------------------------
+constexpr int kShapeletTable = 6000;  // points of linspace(-5, 5, 6000) (shapelets.py:39-40)
+constexpr int kMaxShapeletN = 12;     // n_max supported
+
+// 1-D Hermite functions phi_n(u), n = 0..n_max, and their derivatives.
+// Table mode: linear interpolation on the regular [-5, 5] grid with zero
+// fill outside (tfp.math.interp_regular_1d_grid, shapelets.py:55-60); the
+// derivative is the slope of the segment (zero outside).
+GLM void shapelet_1d_table(const float* __restrict__ table, int n_max, float u, float* phi, float* dphi) {
+    const float scale = (kShapeletTable - 1) / 10.0f;
+    const float xi = (u + 5.0f) * scale;
+    const bool out = !(xi >= 0.0f && xi <= (float)(kShapeletTable - 1));
+    float xc = fminf(fmaxf(xi, 0.0f), (float)(kShapeletTable - 1));
+    int lo = (int)floorf(xc);
+    int hi = lo + 1 < kShapeletTable ? lo + 1 : kShapeletTable - 1;
+    lo = hi - 1 > 0 ? hi - 1 : 0;
+    const float a = xc - (float)lo;
+#pragma unroll
+    for (int n = 0; n <= kMaxShapeletN; ++n) {
+        if (n <= n_max) {
+            const float y0 = table[n * kShapeletTable + lo];
+            const float y1 = table[n * kShapeletTable + hi];
+            phi[n] = out ? 0.0f : fmaf(a, y1 - y0, y0);
+            if (dphi) dphi[n] = out ? 0.0f : (y1 - y0) * scale;
+        }
+    }
+}
+
+// Direct mode: phi_n(u) = H_n(u) exp(-u^2/2) / sqrt(2^n sqrt(pi) n!) by the
+// recurrence H_n = 2 (u H_{n-1} - (n-1) H_{n-2}) (shapelets.py:66-85);
+// phi_n' = sqrt(n/2) phi_{n-1} - sqrt((n+1)/2) phi_{n+1}.
+GLM void shapelet_1d_direct(const float* __restrict__ pref, int n_max, float u, float* phi, float* dphi) {
+    const float g = expf(-0.5f * u * u);
+    float h[kMaxShapeletN + 2];
+    h[0] = 1.0f;
+    h[1] = 2.0f * u;
+#pragma unroll
+    for (int n = 2; n <= kMaxShapeletN + 1; ++n) h[n] = 2.0f * (u * h[n - 1] - (float)(n - 1) * h[n - 2]);
+#pragma unroll
+    for (int n = 0; n <= kMaxShapeletN; ++n)
+        if (n <= n_max) phi[n] = pref[n] * h[n] * g;
+    if (dphi) {
+#pragma unroll
+        for (int n = 0; n <= kMaxShapeletN; ++n) {
+            if (n <= n_max) {
+                const float up = pref[n + 1] * h[n + 1] * g;  // phi_{n+1}
+                const float dn = n > 0 ? phi[n - 1] : 0.0f;
+                dphi[n] = sqrtf(0.5f * n) * dn - sqrtf(0.5f * (n + 1)) * up;
+            }
+        }
+    }
+}
+
+// Basis order (n1, n2): (0,0),(1,0),(0,1),(2,0),(1,1),(0,2),... (shapelets.py:27-46)
+GLM void shapelet_order(int k, int& n1, int& n2) {
+    int s = 0;
+    while ((s + 1) * (s + 2) / 2 <= k) ++s;
+    const int j = k - s * (s + 1) / 2;  // position within the shell of total order s
+    n1 = s - j;
+    n2 = j;
+}
+
 }  // namespace glm

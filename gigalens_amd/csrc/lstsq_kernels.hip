@@ -1,0 +1,755 @@
+// Multi-channel (linear-amplitude) lens-model path for gfx950:
+// LensSimulator.lstsq_simulate (src/gigalens/tf/simulator.py:99-165) and
+// BackwardProbModel.log_prob (src/gigalens/tf/model.py:134-163).
+//
+// Pipeline, one launch each, stream-ordered:
+//   mc_prologue   per particle: z -> x (bijectors), prior terms, component
+//                 constants, EPL coefficient tables  -> workspace
+//   mc_basis      per sub-pixel: unit-amplitude channels (lens light on theta,
+//                 sources -- Sersic or the shapelet basis -- on beta), NaN -> 0,
+//                 staged in HBM with zero column margins
+//   mc_conv       per (particle, channel, 4 output rows, 64 output columns):
+//                 stride-ss correlation with the PSF folded with the pooling box
+//                 (depthwise conv2d SAME + avg_pool2d); one lane per output
+//                 column streams the contributing sub-rows through L1
+//   mc_gram       per particle: A = X^T W^2 X, b = X^T W^2 obs in FP64
+//   mc_solve      per particle: FP64 parallel cyclic Jacobi eigensolver, then
+//                 the pseudo-inverse with the reference's cutoff
+//                 (singular values <= 1e-6 * max dropped, tf/simulator.py:161)
+//   mc_image      per particle: image = sum_c coef_c X_c, likelihood with the
+//                 observed-image noise map, chi^2, prior -> lp; dL/dimage
+//   mc_adjoint    per sub-pixel: transposed correlation of dL/dimage
+//   mc_grad       per sub-pixel: recompute with the solved amplitudes and
+//                 accumulate reverse-mode statistics (the amplitudes are held
+//                 at their least-squares optimum: envelope theorem)
+//   mc_finish     per particle: chain rule -> dlp/dz
+#include <hip/hip_runtime.h>
+#include "lens_math.h"
+#include "lens_plan.h"
+
+namespace gl {
+
+// mc_gram gives each of its 256 threads 12 of the D(D+1)/2 + D Gram/rhs entries,
+// so D(D+3)/2 <= 3072, i.e. D <= 76 (shapelets up to n_max = 10 plus 10 more).
+constexpr int kMcMaxChan = 76;
+constexpr int kMcStat = 64;  // statistics floats per particle
+constexpr int kMcIB = 4;     // output rows per conv lane
+
+struct McPlan {
+    int D;              // channels
+    int BW;             // staged row width (floats): pb + nsub + right margin, multiple of 4
+    int pstride;        // floats of per-particle constants
+    int coef_base;      // offset of EPL coefficient tables in a particle's constants
+    int shp_comp;       // component index of the shapelet source (-1: none)
+    int shp_nmax, shp_interp, shp_chan0;
+    int rcond_mode;     // 0: reference cutoff
+    float rcond;
+    int lik_cf;         // 1: multiply by the conversion factor (never for lstsq)
+    const float* shp_table;  // (n_max+1) x 6000
+    const float* shp_pref;   // n_max+2
+    const float* wpad;       // kf x (4*KF4max) folded weights, zero padded columns
+    int wpad_w;              // row width of wpad
+    float* consts;           // bs x pstride
+    float* B;                // bs x D x nsub x BW
+    float* X;                // bs x D x N x N
+    double* G;               // bs x D x D
+    double* rhs;             // bs x D
+    float* coef;             // bs x D
+    float* gimg;             // bs x (N + 2 gp) x (N + 2 gp) padded dL/dimage
+    int gp;                  // padding of gimg
+    float* gsub;             // bs x nsub x nsub
+    float* stats;            // bs x kMcStat
+    float* lp_like;          // bs (log-likelihood + chi^2 scratch)
+};
+
+struct McArgs {
+    const float* in;  // z (log_prob) or x (simulate)
+    int use_z;
+    int bs;
+    float* lp;
+    float* chi2;
+    float* img;        // optional image out (bs x N x N)
+    float* coef_out;   // optional (bs x D)
+    float* stacked;    // optional (bs x N x N x D)
+    const float* g_lp;
+    const float* g_chi2;
+    float* grad;       // optional dlp/dz
+};
+
+// ---------------------------------------------------------------------------
+// constants of particle p
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float* pconst(const McPlan& M, int p) { return M.consts + (size_t)p * M.pstride; }
+
+__global__ void mc_prologue(const DPlan P, const McPlan M, const McArgs A) {
+    const int p = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int d = P.d;
+    float* pc = pconst(M, p);
+    float* s_x = pc;  // [x | dxdz | dlpdx | dfldjdz | gx | lpj]
+    float* s_c = pc + ((6 * d + 3) & ~3);
+    if (tid < d) {
+        const float zin = A.in[(size_t)p * d + tid];
+        glm::PriorOut o;
+        if (A.use_z) {
+            glm::prior_eval(P.prior_kind[tid], P.prior_args + 8 * tid, zin, P.fldj_at_x, o);
+        } else {
+            o.x = zin; o.dxdz = 1.0f; o.lp = 0.0f; o.dlpdx = 0.0f; o.fldj = 0.0f; o.dfldjdz = 0.0f;
+        }
+        s_x[tid] = o.x;
+        s_x[d + tid] = o.dxdz;
+        s_x[2 * d + tid] = o.dlpdx;
+        s_x[3 * d + tid] = o.dfldjdz;
+        s_x[4 * d + tid] = 0.0f;
+        s_x[5 * d + tid] = o.lp + o.fldj;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (tid < P.ncomp) {
+        const DComp& cd = P.comp[tid];
+        float pv[GL_MAX_COMPONENT_PARAMS];
+        for (int j = 0; j < GL_MAX_COMPONENT_PARAMS; ++j) pv[j] = cd.col[j] >= 0 ? s_x[cd.col[j]] : cd.cst[j];
+        float* dst = s_c + tid * kConstStride;
+        if (cd.kind == glm::K_EPL) {
+            glm::EplC c;
+            glm::epl_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.niter, -1, c);
+            if (c.N > P.max_n) c.N = P.max_n;
+            *reinterpret_cast<glm::EplC*>(dst) = c;
+        } else if (cd.kind == glm::K_SHEAR) {
+            glm::ShearC c{pv[0], pv[1]};
+            *reinterpret_cast<glm::ShearC*>(dst) = c;
+        } else if (cd.kind == glm::K_SIS) {
+            glm::SisC c{pv[0], pv[1], pv[2]};
+            *reinterpret_cast<glm::SisC*>(dst) = c;
+        } else if (cd.kind == glm::K_SERSIC_ELLIPSE) {
+            glm::SerC c;
+            glm::sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.lstsq ? 1.0f : pv[6], 1, c);
+            *reinterpret_cast<glm::SerC*>(dst) = c;
+        } else if (cd.kind == glm::K_SERSIC) {
+            glm::SerC c;
+            glm::sersic_prepare(pv[0], pv[1], 0.0f, 0.0f, pv[2], pv[3], cd.lstsq ? 1.0f : pv[4], 0, c);
+            *reinterpret_cast<glm::SerC*>(dst) = c;
+        } else if (cd.kind == glm::K_SHAPELETS) {
+            dst[0] = pv[0];  // beta
+            dst[1] = pv[1];  // center_x
+            dst[2] = pv[2];  // center_y
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int i = 0; i < P.nlens; ++i) {
+        if (P.comp[i].kind != glm::K_EPL) continue;
+        const glm::EplC* c = reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
+        float* ca = pc + M.coef_base + P.coef_off[i];
+        float4* cb = reinterpret_cast<float4*>(ca + ((P.max_n + 4) & ~3));
+        for (int n = tid; n <= c->N; n += blockDim.x) {
+            float cv, dt, df;
+            glm::epl_coef(n, (double)c->f, (double)c->t, cv, dt, df);
+            ca[n] = cv;
+            cb[n] = make_float4(cv, dt, df, 0.0f);
+        }
+    }
+    if (tid < kMcStat) M.stats[(size_t)p * kMcStat + tid] = 0.0f;
+}
+
+// Deflection of all lenses of particle p at (X, Y); keeps the EPL states when ADJ.
+template <bool ADJ>
+__device__ __forceinline__ void mc_beta(const DPlan& P, const McPlan& M, const float* pc, float X, float Y, float& bx,
+                                        float& by, glm::EplState* es, float (*sis_st)[3]) {
+    const float* s_c = pc + ((6 * P.d + 3) & ~3);
+    bx = X;
+    by = Y;
+#pragma unroll
+    for (int i = 0; i < kMaxLens; ++i) {
+        if (i >= P.nlens) break;
+        const int kind = P.comp[i].kind;
+        const float* cst = s_c + i * kConstStride;
+        float ax = 0.0f, ay = 0.0f;
+        if (kind == glm::K_EPL) {
+            const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
+            const float* ca = pc + M.coef_base + P.coef_off[i];
+            const float4* cb = reinterpret_cast<const float4*>(ca + ((P.max_n + 4) & ~3));
+            glm::EplState st;
+            glm::epl_forward<ADJ>(C, ca, cb, X, Y, ax, ay, st);
+            if (ADJ) es[i] = st;
+        } else if (kind == glm::K_SHEAR) {
+            const glm::ShearC C = *reinterpret_cast<const glm::ShearC*>(cst);
+            glm::shear_forward(C, X, Y, ax, ay);
+        } else if (kind == glm::K_SIS) {
+            const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
+            float st[3];
+            glm::sis_forward(C, X, Y, ax, ay, st);
+            if (ADJ) { sis_st[i][0] = st[0]; sis_st[i][1] = st[1]; sis_st[i][2] = st[2]; }
+        }
+        bx -= ax;
+        by -= ay;
+    }
+}
+
+// Shapelet 1-D functions of the plan's mode.
+__device__ __forceinline__ void mc_phi(const McPlan& M, float u, float* phi, float* dphi) {
+    if (M.shp_interp) glm::shapelet_1d_table(M.shp_table, M.shp_nmax, u, phi, dphi);
+    else glm::shapelet_1d_direct(M.shp_pref, M.shp_nmax, u, phi, dphi);
+}
+
+// ---------------------------------------------------------------------------
+// mc_basis: grid (bs, ceil(nsub*nsub/256)), block 256
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mc_basis(const DPlan P, const McPlan M) {
+    const int p = blockIdx.y;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int nsub = P.nsub;
+    if (s >= nsub * nsub) return;
+    const int y = s / nsub, x = s - (s / nsub) * nsub;
+    const float* pc = pconst(M, p);
+    const float* s_c = pc + ((6 * P.d + 3) & ~3);
+    const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
+    const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
+    float bx, by;
+    mc_beta<false>(P, M, pc, X, Y, bx, by, nullptr, nullptr);
+    float* Bp = M.B + (size_t)p * M.D * nsub * M.BW + (size_t)y * M.BW + P.pb + x;
+    const size_t cstride = (size_t)nsub * M.BW;
+    int ch = 0;
+    for (int i = P.nlens; i < P.ncomp; ++i) {
+        const DComp& cd = P.comp[i];
+        const float* cst = s_c + i * kConstStride;
+        const bool src = cd.role == GL_ROLE_SOURCE;
+        const float ux = src ? bx : X, uy = src ? by : Y;
+        if (cd.kind == glm::K_SERSIC || cd.kind == glm::K_SERSIC_ELLIPSE) {
+            const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
+            glm::SerState st;
+            float v = glm::sersic_forward(C, ux, uy, st);
+            Bp[cstride * ch++] = isnan(v) ? 0.0f : v;
+        } else if (cd.kind == glm::K_SHAPELETS) {
+            const float beta = cst[0];
+            const float u = (ux - cst[1]) / beta, w = (uy - cst[2]) / beta;
+            float pu[glm::kMaxShapeletN + 1], pw[glm::kMaxShapeletN + 1];
+            mc_phi(M, u, pu, nullptr);
+            mc_phi(M, w, pw, nullptr);
+#pragma unroll
+            for (int so = 0; so <= glm::kMaxShapeletN; ++so) {
+                if (so > M.shp_nmax) break;
+#pragma unroll
+                for (int j = 0; j <= so; ++j) {  // (n1, n2) = (so - j, j): order of shapelets.py:27-46
+                    const float v = pu[so - j] * pw[j];
+                    Bp[cstride * (ch + so * (so + 1) / 2 + j)] = isnan(v) ? 0.0f : v;
+                }
+            }
+            ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mc_conv: one wave = (particle, channel, kMcIB output rows, 64 output columns)
+// ---------------------------------------------------------------------------
+template <int SS, int KF4>
+__global__ __launch_bounds__(256) void mc_conv(const DPlan P, const McPlan M, int bs) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int N = P.N, nsub = P.nsub, kf = P.kf, pb = P.pb;
+    const int nib = (N + kMcIB - 1) / kMcIB, njb = (N + 63) / 64;
+    int task = blockIdx.x * 4 + wave;
+    const int total = bs * M.D * nib * njb;
+    if (task >= total) return;
+    const int jb = task % njb; task /= njb;
+    const int ib = task % nib; task /= nib;
+    const int c = task % M.D;
+    const int p = task / M.D;
+    const int J = jb * 64 + lane;
+    const int I0 = ib * kMcIB;
+    const bool active = J < N;
+    const float* Bc = M.B + ((size_t)p * M.D + c) * nsub * M.BW;
+    float acc[kMcIB];
+#pragma unroll
+    for (int i = 0; i < kMcIB; ++i) acc[i] = 0.0f;
+    const int nrows = SS * (kMcIB - 1) + kf;
+    for (int r = 0; r < nrows; ++r) {
+        const int y = SS * I0 + r - pb;
+        if (y < 0 || y >= nsub) continue;
+        // staged row: column x sits at index pb + x; this lane needs x = SS*J + q - pb
+        const float* row = Bc + (size_t)y * M.BW + SS * J;
+        float v[4 * KF4];
+        if (active) {
+            if (SS == 4) {
+                const float4* r4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+                for (int j = 0; j < KF4; ++j) {
+                    float4 t = r4[j];
+                    v[4 * j] = t.x; v[4 * j + 1] = t.y; v[4 * j + 2] = t.z; v[4 * j + 3] = t.w;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4 * KF4; ++j) v[j] = row[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4 * KF4; ++j) v[j] = 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < kMcIB; ++i) {
+            const int pp = r - SS * i;
+            if (pp >= 0 && pp < kf) {
+                const float* w = M.wpad + pp * M.wpad_w;
+                float a = acc[i];
+#pragma unroll
+                for (int j = 0; j < 4 * KF4; ++j) a = fmaf(w[j], v[j], a);
+                acc[i] = a;
+            }
+        }
+    }
+    if (active) {
+        float* Xc = M.X + ((size_t)p * M.D + c) * N * N;
+#pragma unroll
+        for (int i = 0; i < kMcIB; ++i) {
+            const int I = I0 + i;
+            if (I < N) {
+                const float v = acc[i];
+                Xc[I * N + J] = isnan(v) ? 0.0f : v;  // NaN -> 0 after pooling (tf/simulator.py:153)
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mc_gram: grid bs, block 256.  A = sum_pix w2 X_c X_d,  b = sum_pix w2 obs X_c
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mc_gram(const DPlan P, const McPlan M) {
+    constexpr int CH = 64;  // pixels per chunk
+    extern __shared__ float s_mc[];
+    const int p = blockIdx.x;
+    const int D = M.D, NP = P.N * P.N;
+    float* xs = s_mc;              // D x CH
+    float* ws = s_mc + D * CH;     // CH  (w^2)
+    float* os = ws + CH;           // CH  (w^2 obs)
+    const int npair = D * (D + 1) / 2 + D;
+    constexpr int PPT = 12;        // pairs per thread (D <= 72)
+    double acc[PPT];
+    int pi[PPT], pj[PPT];
+    for (int t = 0; t < PPT; ++t) {
+        acc[t] = 0.0;
+        int k = threadIdx.x + t * 256;
+        pi[t] = -1; pj[t] = -1;
+        if (k < npair) {
+            if (k < D * (D + 1) / 2) {
+                int i = 0;
+                while ((i + 1) * (i + 2) / 2 <= k) ++i;
+                pi[t] = i;
+                pj[t] = k - i * (i + 1) / 2;
+            } else {
+                pi[t] = k - D * (D + 1) / 2;
+                pj[t] = D;  // rhs
+            }
+        }
+    }
+    const float* Xp = M.X + (size_t)p * D * NP;
+    for (int base = 0; base < NP; base += CH) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < D * CH; e += 256) {
+            const int c = e / CH, k = e - (e / CH) * CH;
+            xs[e] = base + k < NP ? Xp[(size_t)c * NP + base + k] : 0.0f;
+        }
+        for (int k = threadIdx.x; k < CH; k += 256) {
+            const int pix = base + k;
+            const float w2 = pix < NP ? P.inv_var_obs[pix] : 0.0f;
+            ws[k] = w2;
+            os[k] = pix < NP ? w2 * P.obs[pix] : 0.0f;
+        }
+        __syncthreads();
+        for (int t = 0; t < PPT; ++t) {
+            if (pi[t] < 0) continue;
+            const float* a = xs + pi[t] * CH;
+            const float* b = pj[t] < D ? xs + pj[t] * CH : os;
+            const float* w = pj[t] < D ? ws : nullptr;
+            double s = 0.0;
+            if (w) {
+                for (int k = 0; k < CH; ++k) s += (double)(a[k] * w[k]) * (double)b[k];
+            } else {
+                for (int k = 0; k < CH; ++k) s += (double)a[k] * (double)b[k];
+            }
+            acc[t] += s;
+        }
+    }
+    double* Gp = M.G + (size_t)p * D * D;
+    double* rp = M.rhs + (size_t)p * D;
+    for (int t = 0; t < PPT; ++t) {
+        if (pi[t] < 0) continue;
+        if (pj[t] < D) {
+            Gp[pi[t] * D + pj[t]] = acc[t];
+            Gp[pj[t] * D + pi[t]] = acc[t];
+        } else {
+            rp[pi[t]] = acc[t];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mc_solve: grid bs, block 256.  FP64 parallel cyclic Jacobi + pseudo-inverse.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mc_solve(const McPlan M, float rcond) {
+    extern __shared__ double s_d[];
+    const int p = blockIdx.x;
+    const int D = M.D;
+    const int n = D + (D & 1);  // even size (dummy index n-1 when D is odd)
+    double* A = s_d;            // n x n
+    double* V = A + n * n;      // n x n
+    double* rot = V + n * n;    // (n/2) x 2
+    int* pr = reinterpret_cast<int*>(rot + n);  // (n/2) x 2 pairs
+    double* red = reinterpret_cast<double*>(pr + n + (n & 1));
+    const double* Gp = M.G + (size_t)p * D * D;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - (e / n) * n;
+        A[e] = (i < D && j < D) ? Gp[i * D + j] : 0.0;
+        V[e] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int half = n / 2;
+    for (int sweep = 0; sweep < 20; ++sweep) {
+        // convergence: off-diagonal mass relative to the diagonal
+        if (threadIdx.x == 0) { red[0] = 0.0; red[1] = 0.0; }
+        __syncthreads();
+        double off = 0.0, dia = 0.0;
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const int i = e / n, j = e - (e / n) * n;
+            const double a = A[e];
+            if (i == j) dia += a * a; else off += a * a;
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            off += __shfl_xor(off, o, 64);
+            dia += __shfl_xor(dia, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&red[0], off);
+            atomicAdd(&red[1], dia);
+        }
+        __syncthreads();
+        const bool done = red[0] <= 1e-30 * red[1] || red[1] == 0.0;
+        __syncthreads();
+        if (done) break;
+        for (int round = 0; round < n - 1; ++round) {
+            // round-robin pairing: index 0 fixed, the others rotate
+            if (threadIdx.x < half) {
+                const int k = threadIdx.x;
+                int a = k == 0 ? 0 : 1 + (k - 1 + round) % (n - 1);
+                int b = 1 + (n - 2 - k + round) % (n - 1);
+                if (a > b) { int t = a; a = b; b = t; }
+                double c = 1.0, s = 0.0;
+                const double apq = A[a * n + b];
+                if (apq != 0.0) {
+                    const double th = (A[b * n + b] - A[a * n + a]) / (2.0 * apq);
+                    const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                    c = 1.0 / sqrt(t * t + 1.0);
+                    s = t * c;
+                }
+                rot[2 * k] = c;
+                rot[2 * k + 1] = s;
+                pr[2 * k] = a;
+                pr[2 * k + 1] = b;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int e = threadIdx.x; e < half * n; e += blockDim.x) {
+                const int k = e / n, j = e - (e / n) * n;
+                const int a = pr[2 * k], b = pr[2 * k + 1];
+                const double c = rot[2 * k], s = rot[2 * k + 1];
+                const double x = A[a * n + j], y = A[b * n + j];
+                A[a * n + j] = c * x - s * y;
+                A[b * n + j] = s * x + c * y;
+            }
+            __syncthreads();
+            // columns: A <- A J, V <- V J
+            for (int e = threadIdx.x; e < half * n; e += blockDim.x) {
+                const int k = e / n, i = e - (e / n) * n;
+                const int a = pr[2 * k], b = pr[2 * k + 1];
+                const double c = rot[2 * k], s = rot[2 * k + 1];
+                const double x = A[i * n + a], y = A[i * n + b];
+                A[i * n + a] = c * x - s * y;
+                A[i * n + b] = s * x + c * y;
+                const double vx = V[i * n + a], vy = V[i * n + b];
+                V[i * n + a] = c * vx - s * vy;
+                V[i * n + b] = s * vx + c * vy;
+            }
+            __syncthreads();
+        }
+    }
+    // pseudo-inverse: coef = sum_{|l_i| > rcond max|l|} (v_i . b / l_i) v_i
+    double* w = rot;  // reuse: n doubles of projections
+    if (threadIdx.x == 0) {
+        double mx = 0.0;
+        for (int i = 0; i < D; ++i) mx = fmax(mx, fabs(A[i * n + i]));
+        red[0] = mx * (double)rcond;
+    }
+    __syncthreads();
+    const double cut = red[0];
+    const double* bp = M.rhs + (size_t)p * D;
+    for (int i = threadIdx.x; i < D; i += blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < D; ++j) s += V[j * n + i] * bp[j];
+        const double l = A[i * n + i];
+        w[i] = fabs(l) > cut ? s / l : 0.0;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < D; j += blockDim.x) {
+        double s = 0.0;
+        for (int i = 0; i < D; ++i) s += V[j * n + i] * w[i];
+        M.coef[(size_t)p * D + j] = (float)s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mc_image: grid bs, block 256.  image, likelihood, chi^2, lp, dL/dimage.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mc_image(const DPlan P, const McPlan M, const McArgs A, int want_grad) {
+    const int p = blockIdx.x;
+    const int N = P.N, NP = N * N, D = M.D;
+    const float* Xp = M.X + (size_t)p * D * NP;
+    const float* cp = M.coef + (size_t)p * D;
+    const float glp = A.g_lp ? A.g_lp[p] : 1.0f;
+    const float gch = A.g_chi2 ? A.g_chi2[p] : 0.0f;
+    const float inv_np = 1.0f / (float)NP;
+    const int GW = N + 2 * M.gp;
+    float ll = 0.0f, chi = 0.0f;
+    for (int pix = threadIdx.x; pix < NP; pix += blockDim.x) {
+        float im = 0.0f;
+        for (int c = 0; c < D; ++c) im = fmaf(Xp[(size_t)c * NP + pix], cp[c], im);
+        if (A.img) A.img[(size_t)p * NP + pix] = im;
+        if (A.stacked)
+            for (int c = 0; c < D; ++c) A.stacked[((size_t)p * NP + pix) * D + c] = Xp[(size_t)c * NP + pix];
+        float l, r2, dll, dr2;
+        glm::lik_backward(im, P.obs[pix], P.inv_var_obs[pix], P.log_sig_obs[pix], l, r2, dll, dr2);
+        ll += l;
+        chi += r2;
+        if (want_grad) {
+            const int I = pix / N, J = pix - (pix / N) * N;
+            M.gimg[(size_t)p * GW * GW + (I + M.gp) * GW + (J + M.gp)] = glp * dll + gch * inv_np * dr2;
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        ll += __shfl_xor(ll, o, 64);
+        chi += __shfl_xor(chi, o, 64);
+    }
+    __shared__ float red[2][8];
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = ll;
+        red[1][threadIdx.x >> 6] = chi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float L = 0.0f, Cq = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { L += red[0][w]; Cq += red[1][w]; }
+        L -= (float)NP * glm::kHalfLn2Pi;
+        const float* px = pconst(M, p);
+        float lpri = 0.0f;
+        for (int j = 0; j < P.d; ++j) lpri += px[5 * P.d + j];
+        if (A.lp) A.lp[p] = L + lpri;
+        if (A.chi2) A.chi2[p] = Cq * inv_np;
+    }
+    if (A.coef_out)
+        for (int c = threadIdx.x; c < D; c += blockDim.x) A.coef_out[(size_t)p * D + c] = cp[c];
+}
+
+// ---------------------------------------------------------------------------
+// mc_adjoint: transposed correlation of dL/dimage onto the sub-grid.
+// grid (bs * nsub * ss, ceil(N/64)), block 64: one wave per (particle, row y,
+// column phase b, 64 columns X); x = ss X + b, so the weights are wave-uniform.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void mc_adjoint(const DPlan P, const McPlan M) {
+    const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb;
+    int t = blockIdx.x;
+    const int b = t % ss; t /= ss;
+    const int y = t % nsub;
+    const int p = t / nsub;
+    const int Xc = blockIdx.y * 64 + threadIdx.x;
+    const int a = y % ss, Yc = y / ss;
+    const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
+    const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+    const int GW = N + 2 * M.gp;
+    const float* G = M.gimg + (size_t)p * GW * GW;
+    float g = 0.0f;
+    if (Xc < N) {
+        for (int m = 0; m < Ma; ++m) {
+            const float* row = G + (Yc + da - m + M.gp) * GW + (Xc + db + M.gp);
+            const float* wr = P.w + (ss * m + ra) * kf + rb;
+            for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
+        }
+        M.gsub[((size_t)p * nsub + y) * nsub + ss * Xc + b] = g;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mc_grad: grid (bs, ceil(nsub*nsub/256)), block 256.  Reverse-mode statistics
+// with the solved amplitudes; block-reduced, then atomically added per particle.
+// Statistics layout: per lens slot 8 (EPL 8 / shear 2 / SIS 3), per light slot 8
+// (Sersic) or 3 (shapelets: sum g L_u, sum g L_v, sum g (u L_u + v L_v)).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mc_grad(const DPlan P, const McPlan M) {
+    const int p = blockIdx.y;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int nsub = P.nsub;
+    float st[kMcStat];
+#pragma unroll
+    for (int e = 0; e < kMcStat; ++e) st[e] = 0.0f;
+    const float* pc = pconst(M, p);
+    const float* s_c = pc + ((6 * P.d + 3) & ~3);
+    const float* cp = M.coef + (size_t)p * M.D;
+    if (s < nsub * nsub) {
+        const int y = s / nsub, x = s - (s / nsub) * nsub;
+        const float g = M.gsub[(size_t)p * nsub * nsub + s];
+        const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
+        const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
+        glm::EplState es[3];
+        float sis_st[3][3];
+        float bx, by;
+        mc_beta<true>(P, M, pc, X, Y, bx, by, es, sis_st);
+        // forward value (NaN sub-pixels contribute nothing) and light adjoints
+        float val = 0.0f;
+        int ch = 0;
+        float gbx = 0.0f, gby = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kMaxComp; ++i) {
+            if (i < P.nlens) continue;
+            if (i >= P.ncomp) break;
+            const DComp& cd = P.comp[i];
+            const float* cst = s_c + i * kConstStride;
+            const bool src = cd.role == GL_ROLE_SOURCE;
+            const float ux = src ? bx : X, uy = src ? by : Y;
+            float* acc = st + 8 * i;
+            if (cd.kind == glm::K_SERSIC || cd.kind == glm::K_SERSIC_ELLIPSE) {
+                const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
+                glm::SerState ss_;
+                const float v = glm::sersic_forward(C, ux, uy, ss_);
+                if (isnan(v)) { ++ch; continue; }
+                val += cp[ch] * v;
+                float gdx, gdy;
+                glm::sersic_adjoint(C, ss_, g * cp[ch], acc, gdx, gdy);
+                if (src) { gbx += gdx; gby += gdy; }
+                ++ch;
+            } else if (cd.kind == glm::K_SHAPELETS) {
+                const float beta = cst[0];
+                const float ib = 1.0f / beta;
+                const float u = (ux - cst[1]) * ib, w = (uy - cst[2]) * ib;
+                float pu[glm::kMaxShapeletN + 1], pw[glm::kMaxShapeletN + 1];
+                float du[glm::kMaxShapeletN + 1], dw[glm::kMaxShapeletN + 1];
+                mc_phi(M, u, pu, du);
+                mc_phi(M, w, pw, dw);
+                float L = 0.0f, Lu = 0.0f, Lw = 0.0f;
+#pragma unroll
+                for (int so = 0; so <= glm::kMaxShapeletN; ++so) {
+                    if (so > M.shp_nmax) break;
+#pragma unroll
+                    for (int j = 0; j <= so; ++j) {
+                        const float a = cp[ch + so * (so + 1) / 2 + j];
+                        L = fmaf(a, pu[so - j] * pw[j], L);
+                        Lu = fmaf(a, du[so - j] * pw[j], Lu);
+                        Lw = fmaf(a, pu[so - j] * dw[j], Lw);
+                    }
+                }
+                ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
+                val += L;
+                acc[0] += g * Lu;
+                acc[1] += g * Lw;
+                acc[2] += g * (u * Lu + w * Lw);
+                if (src) { gbx += g * Lu * ib; gby += g * Lw * ib; }
+            }
+        }
+        if (!isnan(val)) {
+            const float Hx = -gbx, Hy = -gby;
+#pragma unroll
+            for (int i = 0; i < kMaxLens; ++i) {
+                if (i >= P.nlens) break;
+                const int kind = P.comp[i].kind;
+                const float* cst = s_c + i * kConstStride;
+                float* acc = st + 8 * i;
+                if (kind == glm::K_EPL) {
+                    const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
+                    glm::epl_adjoint(C, es[i], Hx, Hy, acc);
+                } else if (kind == glm::K_SHEAR) {
+                    glm::shear_adjoint(X, Y, Hx, Hy, acc);
+                } else if (kind == glm::K_SIS) {
+                    const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
+                    glm::sis_adjoint(C, sis_st[i], Hx, Hy, acc);
+                }
+            }
+        } else {
+            // a NaN light: drop this sub-pixel's contributions entirely
+#pragma unroll
+            for (int e = 0; e < kMcStat; ++e) st[e] = 0.0f;
+        }
+    }
+    __shared__ float red[kMcStat][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nst = 8 * P.ncomp;
+#pragma unroll
+    for (int e = 0; e < kMcStat; ++e) {
+        if (e < nst) {
+            float v = st[e];
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[e][wv] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nst) {
+        const float v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+        atomicAdd(M.stats + (size_t)p * kMcStat + threadIdx.x, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mc_finish: grid bs, block 64.  Statistics -> dlp/dz (or dL/dx).
+// ---------------------------------------------------------------------------
+__global__ void mc_finish(const DPlan P, const McPlan M, const McArgs A) {
+    const int p = blockIdx.x;
+    float* px = pconst(M, p);
+    const int d = P.d;
+    float* gx = px + 4 * d;
+    const float* s_c = px + ((6 * d + 3) & ~3);
+    const float* st = M.stats + (size_t)p * kMcStat;
+    const float* cp = M.coef + (size_t)p * M.D;
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < d; ++j) gx[j] = 0.0f;
+        int ch = 0;
+        for (int i = 0; i < P.ncomp; ++i) {
+            const DComp& cd = P.comp[i];
+            const float* cst = s_c + i * kConstStride;
+            const float* acc = st + 8 * i;
+            float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int np_ = 0;
+            bool circ = false;  // circular Sersic: canonical (Rs, n, cx, cy) -> finish order (Rs, n, e1, e2, cx, cy)
+            if (cd.kind == glm::K_EPL) {
+                glm::epl_finish(*reinterpret_cast<const glm::EplC*>(cst), acc, g);
+                np_ = 6;
+            } else if (cd.kind == glm::K_SHEAR) {
+                g[0] = acc[0]; g[1] = acc[1];
+                np_ = 2;
+            } else if (cd.kind == glm::K_SIS) {
+                glm::sis_finish(acc, g);
+                np_ = 3;
+            } else if (cd.kind == glm::K_SERSIC_ELLIPSE || cd.kind == glm::K_SERSIC) {
+                glm::sersic_finish(*reinterpret_cast<const glm::SerC*>(cst), acc, g);
+                np_ = cd.kind == glm::K_SERSIC_ELLIPSE ? 6 : 4;  // no Ie in least-squares mode
+                circ = cd.kind == glm::K_SERSIC;
+                ++ch;
+            } else if (cd.kind == glm::K_SHAPELETS) {
+                const float beta = cst[0];
+                g[0] = -acc[2] / beta;  // d/d beta
+                g[1] = -acc[0] / beta;  // d/d center_x
+                g[2] = -acc[1] / beta;  // d/d center_y
+                np_ = 3;
+                ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
+            }
+            for (int q = 0; q < np_; ++q)
+                if (cd.col[q] >= 0) gx[cd.col[q]] += circ ? g[q < 2 ? q : q + 2] : g[q];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < d && A.grad) {
+        const int j = threadIdx.x;
+        const float glp = A.g_lp ? A.g_lp[p] : 1.0f;
+        if (!A.use_z) {
+            A.grad[(size_t)p * d + j] = gx[j];
+        } else {
+            A.grad[(size_t)p * d + j] = (gx[j] + glp * px[2 * d + j]) * px[d + j] + glp * px[3 * d + j];
+        }
+    }
+}
+
+}  // namespace gl

@@ -112,6 +112,3 @@ class BackwardProbModel(_ProbModelBase):
         obs = self.observed_image
         self.err_map = torch.sqrt(self.background_rms ** 2 + torch.clamp(obs, min=0) / self.exp_time)
 
-    def log_prob(self, simulator, z):
-        from .lstsq import backward_log_prob
-        return backward_log_prob(self, simulator, z)

@@ -92,7 +92,7 @@ def _dev(t: torch.Tensor, name: str) -> torch.Tensor:
 class Plan:
     def __init__(self, phys_model, camera, layout: Layout, prior=None, observed=None, background_rms=1.0,
                  exp_time=1.0, likelihood=nat.GL_LIK_FORWARD, fldj_at_x=False, epl_batch_series=False,
-                 psf_rank_tol=0.0):
+                 psf_rank_tol=0.0, max_batch=0, noise_map=None):
         self.layout = layout
         self.d = len(layout)
         self.N = camera.num_pix
@@ -167,6 +167,15 @@ class Plan:
         desc.fldj_at_x = int(bool(fldj_at_x))
         desc.epl_batch_series = int(bool(epl_batch_series))
         desc.psf_rank_tol = float(psf_rank_tol)
+        desc.max_batch = int(max_batch)
+        if noise_map is not None:
+            nm = np.ascontiguousarray(np.asarray(noise_map, dtype=np.float32))
+            if nm.shape != (self.N, self.N):
+                raise ValueError(f"noise map must be {self.N}x{self.N}")
+            self._nm = nm
+            desc.noise_map = nm.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        self.depth = sum(getattr(prof, "depth", 1) if getattr(prof, "use_lstsq", False) or prof._kind == nat.GL_SHAPELETS
+                         else 1 for role, i, prof in comps if role != nat.GL_ROLE_LENS)
         lib = nat.lib()
         h = ctypes.c_void_p()
         nat.check(lib.gl_plan_create(ctypes.byref(desc), ctypes.byref(h)), "gl_plan_create")
@@ -212,6 +221,18 @@ class Plan:
         nat.check(self._lib.gl_log_prob(self._h, _ptr(z), bs, _ptr(lp), _ptr(chi2), _ptr(g_lp), _ptr(g_chi2),
                                         _ptr(grad), _stream()), "gl_log_prob")
         return lp, chi2, grad
+
+    def lstsq_simulate(self, x: torch.Tensor, want_image=True, want_coeffs=False, want_stacked=False):
+        """Least-squares simulation (tf/simulator.py:99-165): (image, coeffs, stacked)."""
+        x = _dev(x, "x")
+        bs = x.shape[0]
+        D = self.depth
+        img = torch.empty((bs, self.N, self.N), device=x.device) if want_image else None
+        coef = torch.empty((bs, D), device=x.device) if want_coeffs else None
+        st = torch.empty((bs, self.N, self.N, D), device=x.device) if want_stacked else None
+        nat.check(self._lib.gl_lstsq_simulate(self._h, _ptr(x), bs, _ptr(img), _ptr(coef), _ptr(st), _stream()),
+                  "gl_lstsq_simulate")
+        return img, coef, st
 
     def map_adam_step(self, z, m, v, step: int, lr: float, beta1=0.9, beta2=0.999, epsilon=1e-7, lp=None,
                       chi2=None):

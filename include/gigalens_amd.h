@@ -96,6 +96,9 @@ typedef struct gl_plan_desc {
     int epl_batch_series;     /* 1: reference's batch-coupled EPL series length (epl.py:41) */
     float psf_rank_tol;       /* separable PSF: keep singular values >= tol * s_max of the folded
                                  PSF; 0 selects 1e-6, a negative value forces the direct 2-D path */
+    int max_batch;            /* least-squares plans: particles per internal chunk (workspace is
+                                 allocated for this many at plan creation); 0 selects 64 */
+    const float* noise_map;   /* HOST N*N sigma per pixel; NULL: sqrt(bkg^2 + clip(obs,0)/t_exp) */
 } gl_plan_desc;
 
 typedef struct gl_plan gl_plan;
@@ -140,6 +143,18 @@ int gl_simulate_vjp(const gl_plan* plan, const float* x, int bs, const float* g_
  * computed by the hand-written adjoint in the same launch.  lp/chi2 may be NULL. */
 int gl_log_prob(const gl_plan* plan, const float* z, int bs, float* lp, float* chi2,
                 const float* g_lp, const float* g_chi2, float* grad_z, hipStream_t stream);
+
+/* Least-squares simulation on physical parameters x (bs x d) of a plan whose light
+ * components are linear (use_lstsq): every component is simulated with unit
+ * amplitude, convolved and pooled, the amplitudes are solved per particle by
+ * weighted least squares against the plan's observed image and noise map
+ * (pseudo-inverse cutoff 1e-6 of the largest singular value, FP64 solve), and
+ * the outputs are written when non-NULL: img (bs x N x N), coeffs (bs x D),
+ * stacked (bs x N x N x D, the unit-amplitude components).  Replaces
+ * LensSimulator.lstsq_simulate (tf/simulator.py:99-165) with its
+ * return_stacked / return_coeffs outputs. */
+int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, float* img, float* coeffs,
+                      float* stacked, hipStream_t stream);
 
 /* One multi-start MAP step with Keras-Adam semantics, fused with log_prob and
  * its adjoint: loss = mean_i(-lp_i / N^2) (tf/inference.py:28-35), then

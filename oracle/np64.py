@@ -478,8 +478,11 @@ def conv_pool(img: np.ndarray, kernel: Optional[np.ndarray], ss: int) -> np.ndar
         # SAME zero padding; for even sizes TF pads (k-1)//2 before
         pb = (kh - 1) // 2
         out = np.empty_like(flat, dtype=np.float64)
+        from scipy.signal import fftconvolve
         for i in range(flat.shape[0]):
-            full = convolve2d(flat[i].astype(np.float64), k, mode="full")
+            # FFT for large kernels (exact to FP64 rounding); direct otherwise
+            full = (fftconvolve(flat[i].astype(np.float64), k, mode="full") if kh > 15
+                    else convolve2d(flat[i].astype(np.float64), k, mode="full"))
             # full conv index = out index + (k-1) - pb_conv; true conv with flipped
             # correlation offset: correlation pad pb  -> conv offset kh-1-pb
             off = kh - 1 - pb

@@ -120,3 +120,62 @@ def oracle_nested_prior(prior):
             return {k: rec(node.model[k]) for k in sorted(node.model)}
         return (KIND_NAME[node.kind], tuple(node.args()))
     return rec(prior)
+
+
+def oracle_lstsq_lists(phys_model, nested):
+    """Nested params of a least-squares model -> oracle (kind, params[, n_max, interp]) lists."""
+    has_ll = len(phys_model.lens_light) > 0
+    groups = (nested[0], nested[1], nested[2]) if has_ll else (nested[0], [], nested[1])
+    lenses = []
+    for prof, p in zip(phys_model.lenses, groups[0]):
+        lenses.append(("EPL", dict(p, niter=prof.niter)) if prof._name == "EPL" else (prof._name, dict(p)))
+
+    def light(prof, p):
+        if prof._name == "SHAPELETS":
+            return ("SHAPELETS", dict(p), prof.n_max, prof.interpolate)
+        return (prof._name, dict(p))
+    ll = [light(prof, p) for prof, p in zip(phys_model.lens_light, groups[1])]
+    src = [light(prof, p) for prof, p in zip(phys_model.source_light, groups[2])]
+    return lenses, ll, src
+
+
+def oracle_lstsq(cam, phys_model, nested, obs, err):
+    """FP64 lstsq_simulate (tf/simulator.py:99-165): (stacked, coeffs, image), per-particle EPL series."""
+    lenses, ll, src = oracle_lstsq_lists(phys_model, nested)
+    pp = []
+    for kind, p in lenses:
+        if kind == "EPL":
+            p = dict(p)
+            niter = p.pop("niter")
+            pp.append(("EPL", dict(p, niter=niter)))
+        else:
+            pp.append((kind, p))
+    # per-particle series length: evaluate each particle separately
+    bs = len(np.atleast_1d(next(iter(nested[0][0].values()))))
+    st, co, im = [], [], []
+    for i in range(bs):
+        def pick(d):
+            return {k: (v if k == "niter" else np.atleast_1d(np.asarray(v, np.float64))[i:i + 1])
+                    for k, v in d.items()}
+        L = [(k, pick(p)) for k, p in pp]
+        LL = [(it[0], pick(it[1])) + tuple(it[2:]) for it in ll]
+        S = [(it[0], pick(it[1])) + tuple(it[2:]) for it in src]
+        stacked = np64.lstsq_components(cam, L, LL, S)
+        img, coeffs = np64.lstsq_solve(stacked, obs, err)
+        st.append(stacked[0]); co.append(coeffs[0]); im.append(img[0])
+    return np.stack(st), np.stack(co), np.stack(im)
+
+
+def oracle_backward_log_prob(cam, phys_model, prior, z, obs, bkg, texp):
+    """FP64 BackwardProbModel.log_prob (tf/model.py:134-163) incl. fldj evaluated at x (model.py:160)."""
+    op = oracle_prior(prior)
+    x = np.zeros_like(z, dtype=np.float64)
+    lpri = 0.0
+    for j, (_, kind, args) in enumerate(op):
+        x[:, j], _ = np64.bijector_forward(kind, args, z[:, j])
+        _, fl_at_x = np64.bijector_forward(kind, args, x[:, j])
+        lpri = lpri + fl_at_x + np64.prior_logpdf(kind, args, x[:, j])
+    err = np64.backward_err_map(obs, bkg, texp)
+    _, _, img = oracle_lstsq(cam, phys_model, nested_from_x(prior, x), obs, err)
+    ll, chi2 = np64.backward_loglike(img, obs, err)
+    return ll + lpri, chi2, img
