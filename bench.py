@@ -10,6 +10,11 @@ HBM before the timed region.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 Prints one JSON line.
+
+``--workload shapelets`` measures the least-squares configuration instead
+(100x100 px, supersample 4, 21x21 Moffat PSF, EPL+shear, Sersic lens light and
+an n_max=10 shapelet source with linear amplitudes; 512 prior particles,
+BackwardProbModel log_prob + gradient per step).
 """
 from __future__ import annotations
 
@@ -26,7 +31,7 @@ import torch
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from gigalens_amd import ForwardProbModel, LensSimulator  # noqa: E402
+from gigalens_amd import BackwardProbModel, ForwardProbModel, LensSimulator  # noqa: E402
 from gigalens_amd import distributions as tfd  # noqa: E402
 from gigalens_amd import flops as fl  # noqa: E402
 from gigalens_amd import workloads as wl  # noqa: E402
@@ -106,6 +111,123 @@ def cpu_baseline(obs, budget_s=20.0):
                       f"op-for-op restatement of tf/simulator.py + tf/model.py), 2 warm-ups, median of 5 runs"}
 
 
+def shapelet_setup(bs, device, interpolate=True):
+    """Shapelet workload: truth = prior draw (seed 0) with the lens light scaled
+    to a peak of 100 and shapelet amplitudes from ``shapelet_amplitudes`` scaled
+    to a source peak of 50; observed = Poisson + Gaussian noise (seed 1)."""
+    prior = wl.shapelet_prior()
+    phys = wl.shapelet_model(10, interpolate=interpolate)
+    cfg = wl.shapelet_config()
+    sim = LensSimulator(phys, cfg, bs=bs, device=device)
+    truth = flat_sample(prior, 1, seed=0).to(device)
+    N = cfg.num_pix
+    st = sim.lstsq_simulate(truth, np.zeros((N, N), np.float32), np.ones((N, N), np.float32),
+                            return_stacked=True)[0].double().cpu().numpy()
+    amps = wl.shapelet_amplitudes(10, seed=0)
+    amps[0] = 100.0 / max(st[..., 0].max(), 1e-30)
+    src = st[..., 1:] @ amps[1:]
+    amps[1:] *= 50.0 / max(np.abs(src).max(), 1e-30)
+    clean = st @ amps
+    obs = wl.noisy_observation(clean, 0.2, 100.0, seed=1)
+    pm = BackwardProbModel(prior, obs, 0.2, 100.0)
+    x0 = flat_sample(prior, bs, seed=3)
+    z0 = pm.bij.inverse(tfd.pack_as(prior, [x0[:, j] for j in range(x0.shape[1])])).to(device).contiguous()
+    return prior, phys, cfg, sim, pm, obs, z0
+
+
+def shapelet_cpu_baseline(prior, phys, cfg, sim, obs, z0, n=2):
+    """Oracle port timed on the host: BackwardProbModel log_prob (lstsq with
+    torch.linalg.pinv) + autograd gradient for ``n`` particles, FP32, one run
+    after one warm-up."""
+    from oracle import torch_ref
+    from oracle import np64
+    cam = sim.camera
+    ocam = np64.Camera(cfg.delta_pix, cfg.num_pix, cfg.supersample, cfg.kernel, None, subgrid=cam.subgrid_kernel)
+    model = {"lenses": [("EPL", {"niter": 50}), ("SHEAR", {})], "lens_light": [("SERSIC_ELLIPSE", {})],
+             "source_light": [("SHAPELETS", {"n_max": 10, "interpolate": phys.source_light[0].interpolate})]}
+    kind = {1: "Normal", 2: "LogNormal", 3: "Uniform", 4: "TruncatedNormal"}
+
+    def rec(node):
+        if isinstance(node, tfd.JointDistributionSequential):
+            return [rec(m) for m in node.model]
+        if isinstance(node, tfd.JointDistributionNamed):
+            return {k: rec(node.model[k]) for k in sorted(node.model)}
+        return (kind[node.kind], tuple(node.args()))
+    nested = rec(prior)
+    osim = torch_ref.Sim(model, ocam, dtype=torch.float32)
+    zc = z0[:n].float().cpu()
+
+    def run():
+        z = zc.clone().requires_grad_(True)
+        t = time.perf_counter()
+        lp, _ = torch_ref.log_prob(osim, nested, z, obs, 0.2, 100.0, backward=True)
+        (-lp.sum()).backward()
+        return time.perf_counter() - t
+    run()
+    dt = run()
+    return {"value": n / dt, "unit": "particle-evals/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"{n} prior particles, BackwardProbModel log_prob + autograd gradient in FP32 (torch CPU "
+                      f"restatement of tf/simulator.py lstsq_simulate + tf/model.py), 1 warm-up, 1 timed run"}
+
+
+def run_shapelets(args, dev):
+    bs = args.particles if args.particles != 4000 else 512
+    prior, phys, cfg, sim, pm, obs, z0 = shapelet_setup(bs, dev)
+    plan = pm.plan_for(sim)
+    for _ in range(args.warmup):
+        pm.log_prob_and_grad(sim, z0)
+    torch.cuda.synchronize()
+    K = args.steps
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(K):
+        ev[s][0].record(stream)
+        lp, chi2, g = pm.log_prob_and_grad(sim, z0)
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    x0 = pm.bij.forward(z0.cpu())
+    n = fl.series_length(x0[0][0]["e1"].numpy(), x0[0][0]["e2"].numpy(), 50)
+    ksz = sim.camera.subgrid_kernel.shape[0]
+    c = fl.lstsq_per_particle(phys, cfg.num_pix, cfg.supersample, ksz, n, grad=True)
+    fp32 = float(c["flops"].sum())
+    achieved = fp32 / (step_ms * 1e-3) / 1e12
+    value = bs * K / wall
+    out = {
+        "metric": "sim+grad particle-evals/s",
+        "value": value,
+        "unit": "particle-evals/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (prior draws; observed = seeded truth with fixed amplitudes + Poisson + Gaussian noise)",
+        "config": {"workload": "least-squares shapelets, 100x100 px, ss4, 83x83 sub-grid Moffat PSF, EPL+shear, "
+                               f"Sersic lens light + n_max=10 shapelets (D={c['channels']}), {bs} particles, "
+                               "BackwardProbModel log_prob + gradient",
+                   "particles": bs, "num_pix": cfg.num_pix, "supersample": cfg.supersample,
+                   "psf_subgrid": ksz, "params": z0.shape[1], "channels": c["channels"]},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "whole pipeline (mc_prologue .. mc_finish); mc_conv dominates",
+                     "step_ms": step_ms, "fp32_flop_per_step": fp32,
+                     "conv_flop_per_step": c["conv_flops"] * bs,
+                     "mfma_flop_per_step": float(c["mfma"].sum()), "fp64_flop_per_step": float(c["fp64"].sum()),
+                     "transcendentals_per_step": float(c["transcendentals"].sum())},
+        "chi2_median": float(chi2.median()),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = shapelet_cpu_baseline(prior, phys, cfg, sim, obs, z0)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +237,7 @@ def main():
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="also launch a 256 MiB device copy (PMC calibration)")
+    ap.add_argument("--workload", choices=["map", "shapelets"], default="map")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -122,6 +245,10 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     dev = torch.device("cuda:0")
+    if args.workload == "shapelets":
+        if args.steps == 350:
+            args.steps = 10
+        return run_shapelets(args, dev)
     bs = args.particles
     prior, phys, cfg, sim, pm, obs, z0 = setup(bs, dev)
     plan = pm.plan_for(sim)

@@ -4,6 +4,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -85,7 +86,8 @@ struct gl_plan {
     gl::McPlan M;
     int max_batch;
     int kf4;                 // conv template parameter of the multi-channel path
-    size_t gram_lds, solve_lds;
+    size_t solve_lds;
+    int ksplit_cap;          // Gram pixel-range splits allowed by the image size
     std::vector<void*> dev;  // owned device buffers
 };
 
@@ -198,6 +200,18 @@ static hipError_t mc_conv_dispatch(int ss, int kf4, int grid, hipStream_t st, co
     return hipErrorInvalidValue;
 }
 
+static hipError_t mc_gram_dispatch(int nbt, int nb, int ks, hipStream_t st, const gl::DPlan& P, const gl::McPlan& M) {
+    switch (nbt) {
+        case 1: hipLaunchKernelGGL((gl::mc_gram<1>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
+        case 2: hipLaunchKernelGGL((gl::mc_gram<2>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
+        case 3: hipLaunchKernelGGL((gl::mc_gram<3>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
+        case 4: hipLaunchKernelGGL((gl::mc_gram<4>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
+        case 5: hipLaunchKernelGGL((gl::mc_gram<5>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 static int pick_kf4(int kf) {
     const int need = (kf + 3) / 4;
     for (int c : {8, 16, 22, 32})
@@ -276,7 +290,17 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     }
     M.gp = gp;
     const int GW = N + 2 * gp;
-    const int mb = desc->max_batch > 0 ? desc->max_batch : 64;
+    // Gram tiles and pixel-range splits
+    M.gram_nb = (D + 1 + 15) / 16;
+    const int gram_nt = M.gram_nb * (M.gram_nb + 1) / 2;
+    const int nchunk = (N * N + 63) / 64;
+    p->ksplit_cap = std::max(1, std::min(gl::kMcGramMaxSplit, nchunk / 4));
+    // particles per chunk: the caller's, else as many as fit a 32 GiB workspace (at most 1024)
+    const double per_particle = 4.0 * ((double)M.pstride + (double)D * nsub * M.BW + (double)D * N * N + D +
+                                       (double)GW * GW + (double)nsub * nsub + gl::kMcStat) +
+                                8.0 * 4 * p->ksplit_cap * gram_nt * 256;
+    int mb = desc->max_batch;
+    if (mb <= 0) mb = (int)std::max(1.0, std::min(1024.0, 32.0 * 1073741824.0 / per_particle));
     p->max_batch = mb;
     M.rcond = 1e-6f;
     // folded weights padded to the conv's row width
@@ -316,10 +340,25 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     M.B = (float*)dptr;
     if ((rc = alloc((size_t)mb * D * N * N * 4, &dptr, false))) return rc;
     M.X = (float*)dptr;
-    if ((rc = alloc((size_t)mb * D * D * 8, &dptr, false))) return rc;
-    M.G = (double*)dptr;
-    if ((rc = alloc((size_t)mb * D * 8, &dptr, false))) return rc;
-    M.rhs = (double*)dptr;
+    if ((rc = alloc((size_t)std::min(mb * p->ksplit_cap, 1024 + mb) * 4 * gram_nt * 256 * 8, &dptr, false)))
+        return rc;
+    M.gpart = (double*)dptr;
+    {
+        // W = 1 / err and obs / err (tf/simulator.py:157-158), err from the noise map or tf/model.py:113-115
+        std::vector<float> sw(N * N, 0.0f), swo(N * N, 0.0f);
+        for (int i = 0; i < N * N; ++i) {
+            const double o = desc->observed ? desc->observed[i] : 0.0;
+            const double var = desc->noise_map ? (double)desc->noise_map[i] * desc->noise_map[i]
+                                               : (double)P.bkg2 + std::max(o, 0.0) / desc->exp_time;
+            const float w = (float)(1.0 / sqrt(var));
+            sw[i] = w;
+            swo[i] = (float)o * w;
+        }
+        if ((rc = upload(p, sw.data(), sw.size() * sizeof(float), &dptr))) return rc;
+        M.sw = (const float*)dptr;
+        if ((rc = upload(p, swo.data(), swo.size() * sizeof(float), &dptr))) return rc;
+        M.swobs = (const float*)dptr;
+    }
     if ((rc = alloc((size_t)mb * D * 4, &dptr, false))) return rc;
     M.coef = (float*)dptr;
     if ((rc = alloc((size_t)mb * GW * GW * 4, &dptr, true))) return rc;  // zero margins stay zero
@@ -328,11 +367,9 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     M.gsub = (float*)dptr;
     if ((rc = alloc((size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
     M.stats = (float*)dptr;
-    p->gram_lds = (size_t)(D * 64 + 128) * 4;
     const int n2 = D + (D & 1);
-    p->solve_lds = (size_t)(2 * n2 * n2 + n2) * 8 + (size_t)n2 * 4 + 64;
-    GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_gram), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)p->gram_lds));
+    // A, V (n2 x n2), rotations (n2), pairs (n2 ints), 2 reduction slots, rhs (n2)
+    p->solve_lds = (size_t)(2 * n2 * n2 + n2) * 8 + (size_t)n2 * 4 + 16 + (size_t)n2 * 8 + 64;
     GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_solve), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)p->solve_lds));
     p->epl_batch = 0;
@@ -371,9 +408,9 @@ static int run_mc(const gl_plan* plan, const float* in, int use_z, int bs, float
         GL_HIP(hipGetLastError());
         const long waves = (long)nb * D * nib * njb;
         GL_HIP(mc_conv_dispatch(P.ss, plan->kf4, (int)((waves + 3) / 4), st, P, M, nb));
-        hipLaunchKernelGGL(gl::mc_gram, dim3(nb), dim3(256), plan->gram_lds, st, P, M);
-        GL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gl::mc_solve, dim3(nb), dim3(256), plan->solve_lds, st, M, M.rcond);
+        const int ks = std::max(1, std::min(plan->ksplit_cap, (1024 + nb - 1) / nb));
+        GL_HIP(mc_gram_dispatch(M.gram_nb, nb, ks, st, P, M));
+        hipLaunchKernelGGL(gl::mc_solve, dim3(nb), dim3(256), plan->solve_lds, st, M, M.rcond, 4 * ks);
         GL_HIP(hipGetLastError());
         const int want_grad = grad != nullptr;
         hipLaunchKernelGGL(gl::mc_image, dim3(nb), dim3(256), 0, st, P, M, A, want_grad);

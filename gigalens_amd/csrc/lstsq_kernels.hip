@@ -12,7 +12,9 @@
 //                 stride-ss correlation with the PSF folded with the pooling box
 //                 (depthwise conv2d SAME + avg_pool2d); one lane per output
 //                 column streams the contributing sub-rows through L1
-//   mc_gram       per particle: A = X^T W^2 X, b = X^T W^2 obs in FP64
+//   mc_gram       per particle and pixel range: [X | obs]^T W^2 [X | obs] on
+//                 FP32 MFMA (16x16x4) tiles, flushed into FP64 every 128
+//                 pixels; one FP64 partial per wave
 //   mc_solve      per particle: FP64 parallel cyclic Jacobi eigensolver, then
 //                 the pseudo-inverse with the reference's cutoff
 //                 (singular values <= 1e-6 * max dropped, tf/simulator.py:161)
@@ -29,9 +31,11 @@
 
 namespace gl {
 
-// mc_gram gives each of its 256 threads 12 of the D(D+1)/2 + D Gram/rhs entries,
-// so D(D+3)/2 <= 3072, i.e. D <= 76 (shapelets up to n_max = 10 plus 10 more).
-constexpr int kMcMaxChan = 76;
+// Channels plus the observed-image column are tiled by 16 for the Gram MFMAs
+// (at most 5 tiles); the solver keeps two FP64 (D+1)^2 matrices in LDS.
+constexpr int kMcMaxChan = 79;
+constexpr int kMcGramMaxNB = 5;
+constexpr int kMcGramMaxSplit = 8;  // pixel-range splits per particle (x 4 waves = partials)
 constexpr int kMcStat = 64;  // statistics floats per particle
 constexpr int kMcIB = 4;     // output rows per conv lane
 
@@ -52,8 +56,10 @@ struct McPlan {
     float* consts;           // bs x pstride
     float* B;                // bs x D x nsub x BW
     float* X;                // bs x D x N x N
-    double* G;               // bs x D x D
-    double* rhs;             // bs x D
+    const float* sw;         // N x N: 1 / err
+    const float* swobs;      // N x N: obs / err
+    int gram_nb;             // 16-column tiles of [X | obs]
+    double* gpart;           // bs x parts x (nb(nb+1)/2) x 256 FP64 Gram partials
     float* coef;             // bs x D
     float* gimg;             // bs x (N + 2 gp) x (N + 2 gp) padded dL/dimage
     int gp;                  // padding of gimg
@@ -311,81 +317,112 @@ __global__ __launch_bounds__(256) void mc_conv(const DPlan P, const McPlan M, in
 }
 
 // ---------------------------------------------------------------------------
-// mc_gram: grid bs, block 256.  A = sum_pix w2 X_c X_d,  b = sum_pix w2 obs X_c
+// mc_gram: grid (bs, ksplit), block 256 (4 waves).  Y = [X | obs] / err is
+// staged 64 pixels at a time in LDS as Y^T (channel rows); each wave owns 16
+// of the 64 pixels and accumulates the upper-triangle 16x16 tiles of Y^T Y with
+// v_mfma_f32_16x16x4f32.  Lane l of a wave reads the float4 of pixels
+// 4(l/16)..+3 of channel row l%16 of a tile; MFMA step s uses element s, so
+// A and B see the same pixel order and the K sum covers the 16 pixels once.
+// FP32 tiles are added into FP64 accumulators every kFlush chunks.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void mc_gram(const DPlan P, const McPlan M) {
-    constexpr int CH = 64;  // pixels per chunk
-    extern __shared__ float s_mc[];
-    const int p = blockIdx.x;
+typedef float mc_v4f __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ __launch_bounds__(256) void mc_gram(const DPlan P, const McPlan M, int ksplit) {
+    constexpr int NT = NB * (NB + 1) / 2;
+    constexpr int CP = 16 * NB;
+    constexpr int KC = 64;
+    constexpr int LS = KC + 4;
+    constexpr int LPT = CP * KC / 256;
+    constexpr int kFlush = 8;
+    __shared__ __attribute__((aligned(16))) float yt[CP * LS];
+    const int p = blockIdx.x, ks = blockIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int D = M.D, NP = P.N * P.N;
-    float* xs = s_mc;              // D x CH
-    float* ws = s_mc + D * CH;     // CH  (w^2)
-    float* os = ws + CH;           // CH  (w^2 obs)
-    const int npair = D * (D + 1) / 2 + D;
-    constexpr int PPT = 12;        // pairs per thread (D <= 72)
-    double acc[PPT];
-    int pi[PPT], pj[PPT];
-    for (int t = 0; t < PPT; ++t) {
-        acc[t] = 0.0;
-        int k = threadIdx.x + t * 256;
-        pi[t] = -1; pj[t] = -1;
-        if (k < npair) {
-            if (k < D * (D + 1) / 2) {
-                int i = 0;
-                while ((i + 1) * (i + 2) / 2 <= k) ++i;
-                pi[t] = i;
-                pj[t] = k - i * (i + 1) / 2;
-            } else {
-                pi[t] = k - D * (D + 1) / 2;
-                pj[t] = D;  // rhs
-            }
-        }
-    }
+    const int nchunk = (NP + KC - 1) / KC;
+    const int c0 = (int)((long)nchunk * ks / ksplit), c1 = (int)((long)nchunk * (ks + 1) / ksplit);
     const float* Xp = M.X + (size_t)p * D * NP;
-    for (int base = 0; base < NP; base += CH) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < D * CH; e += 256) {
-            const int c = e / CH, k = e - (e / CH) * CH;
-            xs[e] = base + k < NP ? Xp[(size_t)c * NP + base + k] : 0.0f;
-        }
-        for (int k = threadIdx.x; k < CH; k += 256) {
-            const int pix = base + k;
-            const float w2 = pix < NP ? P.inv_var_obs[pix] : 0.0f;
-            ws[k] = w2;
-            os[k] = pix < NP ? w2 * P.obs[pix] : 0.0f;
-        }
-        __syncthreads();
-        for (int t = 0; t < PPT; ++t) {
-            if (pi[t] < 0) continue;
-            const float* a = xs + pi[t] * CH;
-            const float* b = pj[t] < D ? xs + pj[t] * CH : os;
-            const float* w = pj[t] < D ? ws : nullptr;
-            double s = 0.0;
-            if (w) {
-                for (int k = 0; k < CH; ++k) s += (double)(a[k] * w[k]) * (double)b[k];
-            } else {
-                for (int k = 0; k < CH; ++k) s += (double)a[k] * (double)b[k];
+    mc_v4f acc[NT];
+    double acc64[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        acc[t] = mc_v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[t][r] = 0.0;
+    }
+    float pre[LPT];
+    auto fetch = [&](int chunk) {
+#pragma unroll
+        for (int t = 0; t < LPT; ++t) {
+            const int e = threadIdx.x + 256 * t;
+            const int c = e / KC, k = e - (e / KC) * KC;
+            const int pix = chunk * KC + k;
+            float v = 0.0f;
+            if (pix < NP) {
+                if (c < D) v = Xp[(size_t)c * NP + pix] * M.sw[pix];
+                else if (c == D) v = M.swobs[pix];
             }
-            acc[t] += s;
+            pre[t] = v;
+        }
+    };
+    if (c0 < c1) fetch(c0);
+    for (int ch = c0; ch < c1; ++ch) {
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < LPT; ++t) {
+            const int e = threadIdx.x + 256 * t;
+            const int c = e / KC, k = e - (e / KC) * KC;
+            yt[c * LS + k] = pre[t];
+        }
+        __syncthreads();
+        if (ch + 1 < c1) fetch(ch + 1);
+        mc_v4f a[NB];
+#pragma unroll
+        for (int I = 0; I < NB; ++I)
+            a[I] = *reinterpret_cast<const mc_v4f*>(yt + (16 * I + (lane & 15)) * LS + wave * 16 + 4 * (lane >> 4));
+        int t = 0;
+#pragma unroll
+        for (int I = 0; I < NB; ++I) {
+#pragma unroll
+            for (int J = I; J < NB; ++J) {
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[I][s4], a[J][s4], acc[t], 0, 0, 0);
+                ++t;
+            }
+        }
+        if ((ch - c0) % kFlush == kFlush - 1 || ch == c1 - 1) {
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc64[u][r] += (double)acc[u][r];
+                acc[u] = mc_v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            }
         }
     }
-    double* Gp = M.G + (size_t)p * D * D;
-    double* rp = M.rhs + (size_t)p * D;
-    for (int t = 0; t < PPT; ++t) {
-        if (pi[t] < 0) continue;
-        if (pj[t] < D) {
-            Gp[pi[t] * D + pj[t]] = acc[t];
-            Gp[pj[t] * D + pi[t]] = acc[t];
-        } else {
-            rp[pi[t]] = acc[t];
-        }
-    }
+    // tile element (i, j) = (4 (lane / 16) + r, lane % 16)
+    double* out = M.gpart + ((size_t)(p * ksplit + ks) * 4 + wave) * NT * 256;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[t * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc64[t][r];
+}
+
+// Entry (i, j) of [X | obs]^T W^2 [X | obs] summed over the partials of particle p.
+__device__ __forceinline__ double mc_gram_entry(const McPlan& M, int p, int parts, int i, int j) {
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    const int nb = M.gram_nb, I = a >> 4, J = b >> 4;
+    const int t = I * nb - I * (I - 1) / 2 + (J - I);
+    const int nt = nb * (nb + 1) / 2;
+    const double* g = M.gpart + (size_t)p * parts * nt * 256 + t * 256 + (a & 15) * 16 + (b & 15);
+    double s = 0.0;
+    for (int q = 0; q < parts; ++q) s += g[(size_t)q * nt * 256];
+    return s;
 }
 
 // ---------------------------------------------------------------------------
 // mc_solve: grid bs, block 256.  FP64 parallel cyclic Jacobi + pseudo-inverse.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void mc_solve(const McPlan M, float rcond) {
+__global__ __launch_bounds__(256) void mc_solve(const McPlan M, float rcond, int parts) {
     extern __shared__ double s_d[];
     const int p = blockIdx.x;
     const int D = M.D;
@@ -395,11 +432,17 @@ __global__ __launch_bounds__(256) void mc_solve(const McPlan M, float rcond) {
     double* rot = V + n * n;    // (n/2) x 2
     int* pr = reinterpret_cast<int*>(rot + n);  // (n/2) x 2 pairs
     double* red = reinterpret_cast<double*>(pr + n + (n & 1));
-    const double* Gp = M.G + (size_t)p * D * D;
+    double* bv = red + 2;       // n: X^T W^2 obs
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int i = e / n, j = e - (e / n) * n;
-        A[e] = (i < D && j < D) ? Gp[i * D + j] : 0.0;
+        A[e] = (i < D && j < D && i <= j) ? mc_gram_entry(M, p, parts, i, j) : 0.0;
         V[e] = i == j ? 1.0 : 0.0;
+    }
+    for (int i = threadIdx.x; i < D; i += blockDim.x) bv[i] = mc_gram_entry(M, p, parts, i, D);
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e - (e / n) * n;
+        if (i > j) A[e] = A[j * n + i];
     }
     __syncthreads();
     const int half = n / 2;
@@ -480,10 +523,9 @@ __global__ __launch_bounds__(256) void mc_solve(const McPlan M, float rcond) {
     }
     __syncthreads();
     const double cut = red[0];
-    const double* bp = M.rhs + (size_t)p * D;
     for (int i = threadIdx.x; i < D; i += blockDim.x) {
         double s = 0.0;
-        for (int j = 0; j < D; ++j) s += V[j * n + i] * bp[j];
+        for (int j = 0; j < D; ++j) s += V[j * n + i] * bv[j];
         const double l = A[i * n + i];
         w[i] = fabs(l) > cut ? s / l : 0.0;
     }

@@ -123,3 +123,66 @@ def survey_formula_per_particle(num_pix: int, ss: int, kernel_size: int, n_terms
     ray = S * (71 + 10 * np.asarray(n_terms, np.float64))
     conv = P * 2 * kf * kf
     return ray + conv + P * 10 + conv + 3 * ray
+
+
+def shapelet_1d(n_max: int, interpolate: bool, deriv: bool = False):
+    """(flops, transcendentals) of one 1-D shapelet evaluation up to n_max:
+    table mode = index + n_max+1 lerps (3 flops; a slope when ``deriv``),
+    direct mode = Hermite recurrence (4 flops per order) + Gaussian + scaling
+    (+ 3 flops per order for the derivative)."""
+    m = n_max + 1
+    if interpolate:
+        return 6 + 3 * m + (2 * m if deriv else 0), 0
+    return 6 + 5 * m + (3 * m if deriv else 0), 1
+
+
+def lstsq_per_particle(phys_model, num_pix: int, ss: int, kernel_size: int, n_terms: np.ndarray,
+                       grad: bool) -> Dict[str, np.ndarray]:
+    """Operation counts of the least-squares pipeline for one particle.
+    ``flops`` are FP32 vector flops (basis, D-channel conv, image and
+    likelihood, adjoint conv, gradient recompute), ``mfma`` the Gram product
+    [X | obs]^T [X | obs] (2 flops per pair per pixel, upper triangle), ``fp64``
+    the eigen-solve estimate (8 Jacobi sweeps of 6 (D+1)^3 flops)."""
+    S = (ss * num_pix) ** 2
+    P = num_pix ** 2
+    k = kernel_size if kernel_size > 0 else 1
+    kf = k + ss - 1
+    kinds, roles = model_kinds(phys_model)
+    n_terms = np.asarray(n_terms, np.float64)
+    D = 0
+    f = np.full_like(n_terms, 8.0)
+    tr = np.zeros_like(n_terms)
+    g = np.full_like(n_terms, 8.0 + 2.0)
+    gt = np.zeros_like(n_terms)
+    for prof, kind, role in zip([*phys_model.lenses, *phys_model.lens_light, *phys_model.source_light], kinds, roles):
+        if kind == nat.GL_SHAPELETS:
+            ds = (prof.n_max + 1) * (prof.n_max + 2) // 2
+            D += ds
+            a, t = shapelet_1d(prof.n_max, prof.interpolate)
+            f += 4 + 2 * a + ds
+            tr += 2 * t
+            ad, atr = shapelet_1d(prof.n_max, prof.interpolate, deriv=True)
+            g += 4 + 2 * ad + 6 * ds + 8
+            gt += 2 * atr
+            continue
+        if role != nat.GL_ROLE_LENS:
+            D += 1
+        a, t = _FWD[kind]
+        f += a + (2 if role == nat.GL_ROLE_LENS else 0)
+        tr += t
+        a2, t2 = _ADJ[kind]
+        g += a2
+        gt += t2
+        if kind == nat.GL_EPL:
+            f += 3 * n_terms
+            g += 9 * n_terms
+    flops = S * f + D * P * 2.0 * kf * kf + P * (2.0 * D + 12)
+    trans = S * tr + P
+    mfma = P * (D + 1.0) * (D + 2.0) + P * (D + 1.0)
+    fp64 = 8 * 6.0 * (D + 1) ** 3
+    if grad:
+        flops = flops + P * 2.0 * kf * kf + S * g + P * 15
+        trans = trans + S * gt
+    return {"flops": flops, "transcendentals": trans, "mfma": np.full_like(n_terms, mfma),
+            "fp64": np.full_like(n_terms, fp64), "channels": D,
+            "conv_flops": float(D * P * 2.0 * kf * kf)}

@@ -137,5 +137,37 @@ def shapelet_model(n_max: int = 10, interpolate: bool = True, niter: int = 50):
 
 
 def shapelet_config():
+    """100x100 px at 0.04"/px, supersample 4, 21x21 Moffat PSF (FWHM 0.08", beta 3)."""
     kernel = psf.moffat(0.08, 3.0, 0.04, 21)
     return SimulatorConfig(delta_pix=0.04, num_pix=100, supersample=4, kernel=kernel)
+
+
+def shapelet_prior():
+    """Prior of the shapelet workload: the EPL+shear lens of the 60x60 workload,
+    a Sersic lens light and a shapelet source with linear amplitudes (so no Ie
+    and no shapelet amplitudes among the sampled parameters; 17 in all)."""
+    lens = epl_shear_prior().model[0]
+    lens_light = tfd.JointDistributionSequential([tfd.JointDistributionNamed(dict(
+        R_sersic=tfd.LogNormal(math.log(1.0), 0.15),
+        n_sersic=tfd.Uniform(0.5, 8),
+        e1=tfd.Normal(0, 0.1),
+        e2=tfd.Normal(0, 0.1),
+        center_x=tfd.Normal(0, 0.05),
+        center_y=tfd.Normal(0, 0.05),
+    ))])
+    source = tfd.JointDistributionSequential([tfd.JointDistributionNamed(dict(
+        beta=tfd.LogNormal(math.log(0.15), 0.2),
+        center_x=tfd.Normal(0, 0.25),
+        center_y=tfd.Normal(0, 0.25),
+    ))])
+    return tfd.JointDistributionSequential([lens, lens_light, source])
+
+
+def shapelet_amplitudes(n_max: int, seed: int, lens_light_amp: float = 500.0, source_amp: float = 150.0):
+    """Linear amplitudes of a synthetic truth: the lens light, then shapelet
+    coefficients N(0, 1) * source_amp * exp(-order / 3) in the basis order."""
+    rng = np.random.default_rng(seed)
+    amps = [lens_light_amp]
+    for order in range(n_max + 1):
+        amps.extend(rng.normal(0.0, 1.0, size=order + 1) * source_amp * math.exp(-order / 3.0))
+    return np.asarray(amps, np.float64)
