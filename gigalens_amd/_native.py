@@ -34,6 +34,14 @@ EXPORTED_SYMBOLS = (
     "gl_log_prob",
     "gl_map_adam_step",
     "gl_lstsq_simulate",
+    "gl_cnn_create",
+    "gl_cnn_destroy",
+    "gl_cnn_num_params",
+    "gl_cnn_set_params",
+    "gl_cnn_get_params",
+    "gl_cnn_forward",
+    "gl_cnn_loss_grad",
+    "gl_cnn_adamw_step",
 )
 
 
@@ -90,6 +98,18 @@ class PlanInfo(ctypes.Structure):
                                             "sep_rank", "lds_bytes", "block")]
 
 
+class CnnDesc(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("channels", ctypes.c_int * 4),
+        ("hidden", ctypes.c_int),
+        ("n_out", ctypes.c_int),
+        ("max_batch", ctypes.c_int),
+        ("in_scale", ctypes.c_float),
+    ]
+
+
 _lib: Optional[ctypes.CDLL] = None
 
 _vp = ctypes.c_void_p
@@ -104,6 +124,15 @@ _SIGS = {
     "gl_lstsq_simulate": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
     "gl_map_adam_step": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
+    "gl_cnn_create": ([ctypes.POINTER(CnnDesc), ctypes.POINTER(_vp)], ctypes.c_int),
+    "gl_cnn_destroy": ([_vp], ctypes.c_int),
+    "gl_cnn_num_params": ([_vp], ctypes.c_int),
+    "gl_cnn_set_params": ([_vp, _vp, _vp], ctypes.c_int),
+    "gl_cnn_get_params": ([_vp, _vp, _vp], ctypes.c_int),
+    "gl_cnn_forward": ([_vp, _vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "gl_cnn_loss_grad": ([_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+    "gl_cnn_adamw_step": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_float, _vp], ctypes.c_int),
 }
 
 

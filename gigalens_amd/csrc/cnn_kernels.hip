@@ -1,0 +1,432 @@
+// Learned initialiser: a small convolutional network that maps an observed
+// image to a diagonal Gaussian over the unconstrained lens parameters.  It
+// has no counterpart in the reference; it seeds the multi-start MAP of
+// ModellingSequence.MAP (src/gigalens/tf/inference.py:17-41), whose prior
+// draws stay the fallback.
+//
+// Architecture: 4 x (3x3 conv, pad 1, bias, tanh-GELU), strides 1,2,1,2,
+// global average pool, Linear(C4 -> hidden) + GELU, Linear(hidden -> 2 d)
+// giving (mean, log-std).  Activations are NHWC bfloat16; every convolution
+// pass is one BF16 MFMA GEMM (v_mfma_f32_16x16x32_bf16, FP32 accumulation)
+// whose operand loaders gather straight from the NHWC tensors:
+//   forward      C[m][co]  = sum_kk X~(m, kk) W[co][kk]           (m = output pixel)
+//   data grad    C[mi][ci] = sum_k  dZ~(mi, k) W2T[ci][k]         (mi = input pixel)
+//   weight grad  C[co][kk] = sum_m  dZ[m][co] X~(m, kk)           (split over m)
+// with kk = (kh*3 + kw)*Cin + ci and k = (kh*3 + kw)*Cout + co.  The head,
+// the Gaussian NLL and AdamW run in FP32 on the vector ALUs.
+//
+// Rounding points (the CPU oracle applies the same ones): the input
+// asinh(img / scale), the weights, Z = conv + bias and Y = gelu(Z) are
+// rounded to bfloat16; dZ = dY * gelu'(Z) is rounded to bfloat16; dY, the
+// weight gradients and the head are FP32.
+#include <hip/hip_runtime.h>
+
+namespace gl {
+namespace cnn {
+
+typedef unsigned short u16;
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned short v8u __attribute__((ext_vector_type(8)));
+
+__host__ __device__ inline u16 f2bf(float f) {
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return 0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                           // round to nearest even
+    return (u16)(u >> 16);
+}
+__host__ __device__ inline float bf2f(u16 h) { return __builtin_bit_cast(float, (unsigned int)h << 16); }
+
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
+constexpr float kGeluA = 0.044715f;
+__device__ inline float gelu(float x) {
+    const float u = kGeluC * (x + kGeluA * x * x * x);
+    return 0.5f * x * (1.0f + tanhf(u));
+}
+__device__ inline float gelu_grad(float x) {
+    const float u = kGeluC * (x + kGeluA * x * x * x);
+    const float t = tanhf(u);
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
+}
+
+// ---------------------------------------------------------------------------
+// Operand loaders: load8(major, minor8) returns the 8 consecutive minor-index
+// elements at one major index (zeros outside the operand).
+// ---------------------------------------------------------------------------
+struct LdPlain {  // element (major, minor) = p[major * ld + minor], minor < cols (cols % 8 == 0)
+    const u16* p;
+    int ld, cols;
+    __device__ v8u load8(int major, int minor8) const {
+        if (minor8 + 8 <= cols) return *reinterpret_cast<const v8u*>(p + (size_t)major * ld + minor8);
+        return v8u{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+};
+
+// im2col view of an NHWC tensor: (m = (b, ho, wo), kk = (kh*3+kw)*cin + ci);
+// column kk == 9 cin reads 1 when `ones` (bias gradient column).
+struct LdIm2col {
+    const u16* x;
+    int H, W, cin, Ho, Wo, s, ones;
+    __device__ v8u load8(int m, int kk8) const {
+        v8u v = {0, 0, 0, 0, 0, 0, 0, 0};
+        const int hw = Ho * Wo;
+        const int b = m / hw, r = m - b * hw;
+        const int ho = r / Wo, wo = r - ho * Wo;
+        const int kv = 9 * cin;
+        if ((cin & 7) == 0) {
+            if (kk8 < kv) {
+                const int khw = kk8 / cin, ci = kk8 - khw * cin;
+                const int kh = khw / 3, kw = khw - 3 * kh;
+                const int hi = ho * s + kh - 1, wi = wo * s + kw - 1;
+                if (hi >= 0 && hi < H && wi >= 0 && wi < W)
+                    v = *reinterpret_cast<const v8u*>(x + ((size_t)(b * H + hi) * W + wi) * cin + ci);
+            } else if (ones && kk8 == kv) {
+                v[0] = 0x3f80;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = kk8 + e;
+                if (kk < kv) {
+                    const int khw = kk / cin, ci = kk - khw * cin;
+                    const int kh = khw / 3, kw = khw - 3 * kh;
+                    const int hi = ho * s + kh - 1, wi = wo * s + kw - 1;
+                    if (hi >= 0 && hi < H && wi >= 0 && wi < W) v[e] = x[((size_t)(b * H + hi) * W + wi) * cin + ci];
+                } else if (ones && kk == kv) {
+                    v[e] = 0x3f80;
+                }
+            }
+        }
+        return v;
+    }
+};
+
+// Transposed-convolution view of dZ (NHWC over Ho x Wo x cout) seen from the
+// input pixel mi = (b, hi, wi): k = (kh*3+kw)*cout + co reads dZ at
+// ho = (hi + 1 - kh) / s, wo = (wi + 1 - kw) / s when those divide exactly.
+struct LdDzGather {
+    const u16* dz;
+    int H, W, Ho, Wo, cout, s;
+    __device__ v8u load8(int mi, int k8) const {
+        const int hw = H * W;
+        const int b = mi / hw, r = mi - b * hw;
+        const int hi = r / W, wi = r - hi * W;
+        const int khw = k8 / cout, co = k8 - khw * cout;
+        const int kh = khw / 3, kw = khw - 3 * kh;
+        const int hn = hi + 1 - kh, wn = wi + 1 - kw;
+        if (khw >= 9 || hn < 0 || wn < 0 || hn % s || wn % s) return v8u{0, 0, 0, 0, 0, 0, 0, 0};
+        const int ho = hn / s, wo = wn / s;
+        if (ho >= Ho || wo >= Wo) return v8u{0, 0, 0, 0, 0, 0, 0, 0};
+        return *reinterpret_cast<const v8u*>(dz + ((size_t)(b * Ho + ho) * Wo + wo) * cout + co);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Epilogues
+// ---------------------------------------------------------------------------
+struct EpFwd {  // Z = bf16(acc + bias), Y = bf16(gelu(Z)), row-major [M][N]
+    const float* bias;
+    u16* Z;
+    u16* Y;
+    int N;
+    __device__ void operator()(int m, int n, float acc, int) const {
+        const u16 zb = f2bf(acc + bias[n]);
+        Z[(size_t)m * N + n] = zb;
+        Y[(size_t)m * N + n] = f2bf(gelu(bf2f(zb)));
+    }
+};
+struct EpDataGrad {  // dZprev = bf16(acc * gelu'(Zprev))
+    const u16* Zp;
+    u16* dZp;
+    int N;
+    __device__ void operator()(int m, int n, float acc, int) const {
+        const size_t i = (size_t)m * N + n;
+        dZp[i] = f2bf(acc * gelu_grad(bf2f(Zp[i])));
+    }
+};
+struct EpPartial {  // split-K partial sums part[split][M][N]
+    float* part;
+    int M, N;
+    __device__ void operator()(int m, int n, float acc, int split) const {
+        part[((size_t)split * M + m) * N + n] = acc;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// GEMM: C[M][N] = sum_{k in split} A(m, k) B(k, n), 64x64x32 tiles, 4 waves of
+// 32x32 (2x2 MFMA 16x16x32 tiles).  Operands are staged in LDS as [row][k]
+// (row = m for A, n for B), padded to 40 elements.  KMAJ: the loader's 8-vector
+// runs along k (major = row); otherwise along rows (major = k).
+// MFMA lane layout: lane l feeds row (l % 16) with the 8 k-values 8(l/16)..+7
+// for both operands (the same k order on both sides, so the sum covers the 32
+// k-values of the step once); the 16x16 result holds (4(l/16) + r, l % 16).
+// ---------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 32, GLS = GBK + 8;
+
+template <bool KMAJ, class L>
+__device__ __forceinline__ void stage(const L& ld, u16* S, int r0, int k0, int rows, int kend) {
+    const int t = threadIdx.x;
+    if (KMAJ) {
+        const int row = t >> 2, k8 = (t & 3) * 8;
+        v8u v = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (r0 + row < rows && k0 + k8 < kend) v = ld.load8(r0 + row, k0 + k8);
+        *reinterpret_cast<v8u*>(S + row * GLS + k8) = v;
+    } else {
+        const int kk = t >> 3, r8 = (t & 7) * 8;
+        v8u v = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (k0 + kk < kend) v = ld.load8(k0 + kk, r0 + r8);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) S[(r8 + e) * GLS + kk] = v[e];
+    }
+}
+
+template <bool AK, bool BK, class LA, class LB, class EP>
+__global__ __launch_bounds__(256) void gemm(LA la, LB lb, EP ep, int M, int N, int K, int kchunk) {
+    __shared__ __attribute__((aligned(16))) u16 As[GBM * GLS];
+    __shared__ __attribute__((aligned(16))) u16 Bs[GBN * GLS];
+    const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = K < kbeg + kchunk ? K : kbeg + kchunk;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    v4f acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = kbeg; k0 < kend; k0 += GBK) {
+        stage<AK>(la, As, m0, k0, M, kend);
+        stage<BK>(lb, Bs, n0, k0, N, kend);
+        __syncthreads();
+        v8bf a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const v8u ua = *reinterpret_cast<const v8u*>(As + (wm * 32 + i * 16 + (lane & 15)) * GLS + 8 * (lane >> 4));
+            const v8u ub = *reinterpret_cast<const v8u*>(Bs + (wn * 32 + i * 16 + (lane & 15)) * GLS + 8 * (lane >> 4));
+            a[i] = __builtin_bit_cast(v8bf, ua);
+            b[i] = __builtin_bit_cast(v8bf, ub);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+                if (m < M && n < N) ep(m, n, acc[i][j][r], blockIdx.z);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels
+// ---------------------------------------------------------------------------
+// X0 = bf16(asinh(img / scale)), NHWC with one channel
+__global__ void input_kernel(const float* img, u16* x0, long n, float inv_scale) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x0[i] = f2bf(asinhf(img[i] * inv_scale));
+}
+
+// split-K reduction of the weight gradient: part[S][cout][9 cin + 1] -> grad
+__global__ void wgrad_reduce(const float* part, int splits, int cout, int ncol, float* gw, float* gb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cout * ncol) return;
+    float s = 0.0f;
+    for (int q = 0; q < splits; ++q) s += part[(size_t)q * cout * ncol + i];
+    const int co = i / ncol, kk = i - co * ncol;
+    if (kk < ncol - 1) gw[co * (ncol - 1) + kk] = s;
+    else gb[co] = s;
+}
+
+// bf16 GEMM operands from the FP32 master weights W[co][9 cin]:
+// wt[co][kp] (zero padded to kp) and w2t[ci][(kh*3+kw)*cout + co]
+__global__ void pack_weights(const float* w, int cin, int cout, int kp, u16* wt, u16* w2t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cout * kp) return;
+    const int co = i / kp, kk = i - co * kp;
+    const int kv = 9 * cin;
+    const float v = kk < kv ? w[co * kv + kk] : 0.0f;
+    wt[i] = f2bf(v);
+    if (kk < kv && w2t) {
+        const int khw = kk / cin, ci = kk - khw * cin;
+        w2t[(size_t)ci * 9 * cout + khw * cout + co] = f2bf(v);
+    }
+}
+
+// Head, one workgroup (256 threads) per image: global average pool over P
+// pixels of C channels, Linear(C -> Hd) + GELU, Linear(Hd -> 2 d).  With a
+// target: Gaussian NLL per image (log-std clamped to [-7, 5]), its gradient
+// (scaled by inv_bs), the head's per-image back-propagation and dZ of the
+// last convolution.
+struct HeadArgs {
+    const u16* y4;  // [bs][P][C]
+    const u16* z4;  // [bs][P][C]
+    int P, C, Hd, d;
+    const float* w1;  // [Hd][C]
+    const float* b1;
+    const float* w2;  // [2d][Hd]
+    const float* b2;
+    float* h0;    // [bs][C]
+    float* h1;    // [bs][Hd]
+    float* da1;   // [bs][Hd]
+    float* out;   // [bs][2d]
+    float* dout;  // [bs][2d]
+    float* lossb; // [bs]
+    u16* dz4;     // [bs][P][C]
+    const float* target;  // [bs][d] or null
+    float inv_bs;
+};
+
+constexpr int kHeadMaxC = 256, kHeadMaxHd = 512, kHeadMaxOut = 256;
+
+__global__ __launch_bounds__(256) void head_kernel(const HeadArgs A) {
+    __shared__ float s_h0[kHeadMaxC];
+    __shared__ float s_part[4][kHeadMaxC];
+    __shared__ float s_a1[kHeadMaxHd];
+    __shared__ float s_h1[kHeadMaxHd];
+    __shared__ float s_out[kHeadMaxOut];
+    __shared__ float s_dout[kHeadMaxOut];
+    __shared__ float s_da1[kHeadMaxHd];
+    __shared__ float s_dh0[kHeadMaxC];
+    __shared__ float s_red[256];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int P = A.P, C = A.C, Hd = A.Hd, d = A.d;
+    const u16* y = A.y4 + (size_t)b * P * C;
+    // global average pool: 4 pixel groups x 64 channel lanes
+    for (int c0 = 0; c0 < C; c0 += 64) {
+        const int c = c0 + (t & 63), g = t >> 6;
+        float s = 0.0f;
+        if (c < C)
+            for (int p = g; p < P; p += 4) s += bf2f(y[(size_t)p * C + c]);
+        if (c < C) s_part[g][c] = s;
+    }
+    __syncthreads();
+    for (int c = t; c < C; c += 256) {
+        const float h = (s_part[0][c] + s_part[1][c] + s_part[2][c] + s_part[3][c]) / (float)P;
+        s_h0[c] = h;
+        A.h0[(size_t)b * C + c] = h;
+    }
+    __syncthreads();
+    for (int j = t; j < Hd; j += 256) {
+        float a = A.b1[j];
+        for (int c = 0; c < C; ++c) a = fmaf(A.w1[j * C + c], s_h0[c], a);
+        s_a1[j] = a;
+        const float h = gelu(a);
+        s_h1[j] = h;
+        A.h1[(size_t)b * Hd + j] = h;
+    }
+    __syncthreads();
+    for (int o = t; o < 2 * d; o += 256) {
+        float a = A.b2[o];
+        for (int j = 0; j < Hd; ++j) a = fmaf(A.w2[o * Hd + j], s_h1[j], a);
+        s_out[o] = a;
+        A.out[(size_t)b * 2 * d + o] = a;
+    }
+    __syncthreads();
+    if (!A.target) return;
+    // Gaussian NLL: sum_o [ls + 0.5 (t - mu)^2 exp(-2 ls)], ls = clamp(raw, -7, 5)
+    float l = 0.0f;
+    for (int o = t; o < d; o += 256) {
+        const float mu = s_out[o], raw = s_out[d + o];
+        const float ls = fminf(fmaxf(raw, -7.0f), 5.0f);
+        const float r = A.target[(size_t)b * d + o] - mu;
+        const float iv = expf(-2.0f * ls);
+        l += ls + 0.5f * r * r * iv;
+        s_dout[o] = -r * iv * A.inv_bs;
+        const float inside = (raw > -7.0f && raw < 5.0f) ? 1.0f : 0.0f;
+        s_dout[d + o] = (1.0f - r * r * iv) * inside * A.inv_bs;
+    }
+    s_red[t] = l;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s_red[t] += s_red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) A.lossb[b] = s_red[0];
+    for (int o = t; o < 2 * d; o += 256) A.dout[(size_t)b * 2 * d + o] = s_dout[o];
+    for (int j = t; j < Hd; j += 256) {
+        float g = 0.0f;
+        for (int o = 0; o < 2 * d; ++o) g = fmaf(A.w2[o * Hd + j], s_dout[o], g);
+        const float da = g * gelu_grad(s_a1[j]);
+        s_da1[j] = da;
+        A.da1[(size_t)b * Hd + j] = da;
+    }
+    __syncthreads();
+    for (int c = t; c < C; c += 256) {
+        float g = 0.0f;
+        for (int j = 0; j < Hd; ++j) g = fmaf(A.w1[j * C + c], s_da1[j], g);
+        s_dh0[c] = g / (float)P;
+    }
+    __syncthreads();
+    const u16* z = A.z4 + (size_t)b * P * C;
+    u16* dz = A.dz4 + (size_t)b * P * C;
+    for (int i = t; i < P * C; i += 256) {
+        const int c = i % C;
+        dz[i] = f2bf(s_dh0[c] * gelu_grad(bf2f(z[i])));
+    }
+}
+
+// Head weight gradients: sums over the batch of the per-image outer products.
+__global__ void head_wgrad(const HeadArgs A, int bs, float* gw1, float* gb1, float* gw2, float* gb2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = A.C, Hd = A.Hd, d2 = 2 * A.d;
+    const int n1 = Hd * C, n2 = Hd, n3 = d2 * Hd, n4 = d2;
+    float s = 0.0f;
+    if (i < n1) {
+        const int j = i / C, c = i - j * C;
+        for (int b = 0; b < bs; ++b) s = fmaf(A.da1[(size_t)b * Hd + j], A.h0[(size_t)b * C + c], s);
+        gw1[i] = s;
+    } else if (i < n1 + n2) {
+        const int j = i - n1;
+        for (int b = 0; b < bs; ++b) s += A.da1[(size_t)b * Hd + j];
+        gb1[j] = s;
+    } else if (i < n1 + n2 + n3) {
+        const int q = i - n1 - n2;
+        const int o = q / Hd, j = q - o * Hd;
+        for (int b = 0; b < bs; ++b) s = fmaf(A.dout[(size_t)b * d2 + o], A.h1[(size_t)b * Hd + j], s);
+        gw2[q] = s;
+    } else if (i < n1 + n2 + n3 + n4) {
+        const int o = i - n1 - n2 - n3;
+        for (int b = 0; b < bs; ++b) s += A.dout[(size_t)b * d2 + o];
+        gb2[o] = s;
+    }
+}
+
+// mean of the per-image losses (one workgroup)
+__global__ void loss_mean(const float* lossb, int bs, float* loss) {
+    __shared__ float r[256];
+    float s = 0.0f;
+    for (int b = threadIdx.x; b < bs; b += 256) s += lossb[b];
+    r[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = r[0] / (float)bs;
+}
+
+// AdamW with torch.optim.AdamW semantics: p *= 1 - lr wd; m, v moments;
+// p -= lr / (1 - b1^t) * m / (sqrt(v) / sqrt(1 - b2^t) + eps)
+__global__ void adamw(float* p, float* m, float* v, const float* g, int n, float lr, float b1, float b2, float eps,
+                      float wd, float bc1, float bc2_sqrt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    float pi = p[i] * (1.0f - lr * wd);
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    pi -= (lr / bc1) * mi / (sqrtf(vi) / bc2_sqrt + eps);
+    p[i] = pi;
+}
+
+}  // namespace cnn
+}  // namespace gl

@@ -147,6 +147,7 @@ class ModellingSequence(ModellingSequenceInterface):
         super().__init__(phys_model, prob_model, sim_config)
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.chi2_history = None
+        self.lp_history = None
 
     def _sim(self, bs):
         return LensSimulator(self.phys_model, self.sim_config, bs=bs, device=self.device)
@@ -162,8 +163,9 @@ class ModellingSequence(ModellingSequenceInterface):
         """Multi-start MAP (tf/inference.py:17-41): returns unconstrained z (n, d).
 
         With a Keras-style ``Adam`` each step is one fused launch (log_prob,
-        adjoint, Adam update).  ``chi2_history`` holds the per-particle mean r^2
-        of every recorded step (device tensor)."""
+        adjoint, Adam update).  ``chi2_history`` and ``lp_history`` hold the
+        per-particle mean r^2 and log posterior of every recorded step (device
+        tensors, values before that step's update)."""
         optimizer = optimizer if optimizer is not None else Adam(1e-2)
         if start is None:
             start = self.prob_model.prior.sample(n_samples, seed=seed)
@@ -175,21 +177,26 @@ class ModellingSequence(ModellingSequenceInterface):
             return self._map_autograd(optimizer, z, sim, num_steps)
         plan = pm.plan_for(sim)
         m, v = torch.zeros_like(z), torch.zeros_like(z)
-        lp = torch.empty(n, device=self.device)
-        hist = torch.empty((num_steps // record_every + 1, n), device=self.device)
+        nrec = (num_steps - 1) // record_every + 1
+        hist = torch.empty((nrec, n), device=self.device)
+        lp_hist = torch.empty((nrec, n), device=self.device)
+        scratch = torch.empty((2, n), device=self.device)
         it = range(num_steps)
         if progress:
             from tqdm.auto import trange
             it = trange(num_steps)
         for s in it:
             step = optimizer.iterations + 1
-            row = hist[s // record_every] if s % record_every == 0 else torch.empty(n, device=self.device)
+            rec = s % record_every == 0
+            row = hist[s // record_every] if rec else scratch[0]
+            lp = lp_hist[s // record_every] if rec else scratch[1]
             plan.map_adam_step(z, m, v, step, optimizer.lr(optimizer.iterations), optimizer.beta_1,
                                optimizer.beta_2, optimizer.epsilon, lp=lp, chi2=row)
             optimizer.iterations += 1
             if progress and s % 10 == 0:
                 it.set_description(f"Chi Squared: {float(torch.nan_to_num(row, nan=float('inf')).min()):.4f}")
-        self.chi2_history = hist[: (num_steps - 1) // record_every + 1]
+        self.chi2_history = hist
+        self.lp_history = lp_hist
         return z
 
     def _map_autograd(self, optimizer, z, sim, num_steps):

@@ -1,0 +1,252 @@
+"""Learned initialiser for the multi-start MAP (no counterpart in the reference).
+
+A small convolutional network -- 4 x (3x3 conv + GELU) with strides 1,2,1,2,
+global average pool, a 256-wide hidden layer and a (mean, log-std) head --
+maps an observed image to a diagonal Gaussian over the standardised
+unconstrained parameters z of a prior.  Its forward and backward passes are
+the BF16 MFMA kernels behind ``gl_cnn_*`` (FP32 accumulation); training data
+are simulated on the device from the prior with fresh Poisson + Gaussian
+noise every batch.  ``LearnedInitializer.propose`` draws MAP starting points
+from the network and mixes in prior draws, which stay the fallback: every
+start is judged by the simulator's log-posterior through the MAP itself
+(``ModellingSequence.MAP``, src/gigalens/tf/inference.py:17-41).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import distributions as dist
+from .plan import _ptr, _stream
+from .simulator import LensSimulator
+
+STRIDES = (1, 2, 1, 2)
+
+
+class ConvNet:
+    """The network behind ``gl_cnn_create`` (weights, AdamW state and
+    activation workspace live on the device)."""
+
+    def __init__(self, height: int, width: int, n_out: int, channels: Sequence[int] = (32, 32, 64, 64),
+                 hidden: int = 256, max_batch: int = 1024, in_scale: float = 0.2, seed: int = 0, device="cuda"):
+        self.height, self.width, self.n_out = int(height), int(width), int(n_out)
+        self.channels = tuple(int(c) for c in channels)
+        self.hidden = int(hidden)
+        self.max_batch = int(max_batch)
+        self.in_scale = float(in_scale)
+        self.device = torch.device(device)
+        desc = nat.CnnDesc()
+        desc.height, desc.width = self.height, self.width
+        for i, c in enumerate(self.channels):
+            desc.channels[i] = c
+        desc.hidden, desc.n_out, desc.max_batch, desc.in_scale = self.hidden, self.n_out, self.max_batch, self.in_scale
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().gl_cnn_create(ctypes.byref(desc), ctypes.byref(h)), "gl_cnn_create")
+        self._h = h
+        self.n_params = int(nat.lib().gl_cnn_num_params(h))
+        self.layout = self.param_layout(self.channels, self.hidden, self.n_out)
+        assert self.layout[-1][2] + int(np.prod(self.layout[-1][1])) == self.n_params
+        self.set_params(self.init_params(seed))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                nat.lib().gl_cnn_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @staticmethod
+    def param_layout(channels, hidden, n_out):
+        """[(name, shape, offset)] of the parameter vector, in the order of the
+        C ABI: per conv layer W[co][(kh*3+kw)*cin + ci] and b[co], then the head."""
+        out = []
+        off = 0
+        cin = 1
+        for i, c in enumerate(channels):
+            out.append((f"conv{i}.w", (c, 9 * cin), off))
+            off += c * 9 * cin
+            out.append((f"conv{i}.b", (c,), off))
+            off += c
+            cin = c
+        for name, shape in (("fc1.w", (hidden, cin)), ("fc1.b", (hidden,)), ("fc2.w", (2 * n_out, hidden)),
+                            ("fc2.b", (2 * n_out,))):
+            out.append((name, shape, off))
+            off += int(np.prod(shape))
+        return out
+
+    def init_params(self, seed: int) -> torch.Tensor:
+        """He-normal convolutions and hidden layer, a small output layer, zero biases."""
+        rng = np.random.default_rng(seed)
+        p = np.zeros(self.n_params, np.float32)
+        for name, shape, off in self.layout:
+            n = int(np.prod(shape))
+            if name.endswith(".w"):
+                fan_in = shape[1]
+                std = math.sqrt(2.0 / fan_in) if not name.startswith("fc2") else 0.1 / math.sqrt(fan_in)
+                p[off:off + n] = rng.normal(0.0, std, size=n)
+        return torch.as_tensor(p, device=self.device)
+
+    def unpack(self, p: torch.Tensor) -> dict:
+        return {name: p[off:off + int(np.prod(shape))].reshape(shape) for name, shape, off in self.layout}
+
+    def set_params(self, p: torch.Tensor) -> None:
+        p = p.to(self.device, torch.float32).contiguous()
+        assert p.numel() == self.n_params
+        nat.check(nat.lib().gl_cnn_set_params(self._h, _ptr(p), _stream()), "gl_cnn_set_params")
+        torch.cuda.current_stream().synchronize()
+
+    def get_params(self) -> torch.Tensor:
+        p = torch.empty(self.n_params, device=self.device)
+        nat.check(nat.lib().gl_cnn_get_params(self._h, _ptr(p), _stream()), "gl_cnn_get_params")
+        return p
+
+    def _images(self, images: torch.Tensor) -> torch.Tensor:
+        images = images.to(self.device, torch.float32).contiguous()
+        if images.dim() == 2:
+            images = images[None]
+        if images.shape[-2:] != (self.height, self.width):
+            raise ValueError(f"images must be (bs, {self.height}, {self.width})")
+        return images
+
+    def forward(self, images: torch.Tensor) -> torch.Tensor:
+        """(bs, 2 n_out): mean and log-std outputs."""
+        images = self._images(images)
+        bs = images.shape[0]
+        out = torch.empty((bs, 2 * self.n_out), device=self.device)
+        for o in range(0, bs, self.max_batch):
+            n = min(self.max_batch, bs - o)
+            nat.check(nat.lib().gl_cnn_forward(self._h, _ptr(images[o:o + n]), n, _ptr(out[o:o + n]), _stream()),
+                      "gl_cnn_forward")
+        return out
+
+    def loss_grad(self, images: torch.Tensor, target: torch.Tensor, want_grad: bool = False):
+        """Mean Gaussian NLL (device scalar) and, when asked, its gradient (P,)."""
+        images = self._images(images)
+        target = target.to(self.device, torch.float32).contiguous()
+        bs = images.shape[0]
+        if bs > self.max_batch or target.shape != (bs, self.n_out):
+            raise ValueError("batch larger than max_batch or target shape mismatch")
+        loss = torch.empty(1, device=self.device)
+        grad = torch.empty(self.n_params, device=self.device) if want_grad else None
+        nat.check(nat.lib().gl_cnn_loss_grad(self._h, _ptr(images), _ptr(target), bs, _ptr(loss), _ptr(grad),
+                                             _stream()), "gl_cnn_loss_grad")
+        return loss, grad
+
+    def adamw_step(self, step: int, lr: float, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.01) -> None:
+        nat.check(nat.lib().gl_cnn_adamw_step(self._h, int(step), float(lr), float(beta1), float(beta2), float(eps),
+                                              float(weight_decay), _stream()), "gl_cnn_adamw_step")
+
+
+def _flat(prior, sample) -> torch.Tensor:
+    return torch.stack([torch.as_tensor(v, dtype=torch.float32) for v in dist.flatten(sample)], dim=1)
+
+
+class LearnedInitializer:
+    """q(z | image): a diagonal Gaussian over the unconstrained parameters,
+    predicted by ``ConvNet`` and standardised with prior moments of z."""
+
+    def __init__(self, phys_model, sim_config, prior, background_rms: float, exp_time: float,
+                 channels=(32, 32, 64, 64), hidden: int = 256, max_batch: int = 1024, seed: int = 0,
+                 n_stats: int = 65536, device="cuda"):
+        self.phys_model = phys_model
+        self.sim_config = sim_config
+        self.prior = prior
+        self.background_rms = float(background_rms)
+        self.exp_time = float(exp_time)
+        self.device = torch.device(device)
+        self.bij = dist.ChainBijector(prior.experimental_default_event_space_bijector(), dist.PackBijector(prior))
+        self.d = len(dist.leaves(prior))
+        self.max_batch = int(max_batch)
+        self.sim = LensSimulator(phys_model, sim_config, bs=self.max_batch, device=self.device)
+        z = self.bij.inverse(prior.sample(n_stats, seed=seed + 7919)).double()
+        self.z_mean = z.mean(0).float().to(self.device)
+        self.z_std = z.std(0).clamp_min(1e-6).float().to(self.device)
+        N = sim_config.num_pix
+        self.net = ConvNet(N, N, self.d, channels, hidden, self.max_batch, self.background_rms, seed, self.device)
+        self.losses: List[float] = []
+
+    # ---- data ------------------------------------------------------------------
+    def simulate_batch(self, n: int, seed: int):
+        """n prior draws: noisy images (n, N, N) on the device and their z (n, d)."""
+        x_nested = self.prior.sample(n, seed=seed)
+        z = self.bij.inverse(x_nested).to(self.device, torch.float32)
+        x = _flat(self.prior, x_nested).to(self.device).contiguous()
+        img = self.sim.plan.simulate(x)
+        g = torch.Generator(device=self.device)
+        g.manual_seed(int(seed) * 2654435761 % (2 ** 62))
+        lam = torch.clamp(img, min=0.0) * self.exp_time
+        noisy = torch.poisson(lam, generator=g) / self.exp_time
+        noisy = noisy + self.background_rms * torch.randn(img.shape, device=self.device, generator=g)
+        return noisy.contiguous(), z
+
+    def standardise(self, z: torch.Tensor) -> torch.Tensor:
+        return (z - self.z_mean) / self.z_std
+
+    # ---- training --------------------------------------------------------------
+    def train(self, n_images: int = 2 ** 20, batch: int = 1024, lr: float = 3e-4, weight_decay: float = 0.01,
+              seed: int = 10, log_every: int = 0) -> List[float]:
+        """AdamW on the Gaussian NLL, fresh simulated batches (seeds seed, seed+1, ...)."""
+        batch = min(int(batch), self.max_batch)
+        steps = max(1, int(n_images) // batch)
+        loss_acc = []
+        for s in range(1, steps + 1):
+            images, z = self.simulate_batch(batch, seed=seed * 1_000_003 + s)
+            loss, _ = self.net.loss_grad(images, self.standardise(z))
+            self.net.adamw_step(len(self.losses) + len(loss_acc) + 1, lr, weight_decay=weight_decay)
+            loss_acc.append(loss)
+            if log_every and s % log_every == 0:
+                print(f"step {s}/{steps}: loss {float(loss):.4f}", flush=True)
+        vals = torch.cat(loss_acc).cpu().tolist()
+        self.losses.extend(vals)
+        return vals
+
+    # ---- inference -------------------------------------------------------------
+    def predict(self, images: torch.Tensor):
+        """(mean, std) of z for each image (bs, d)."""
+        out = self.net.forward(images)
+        d = self.d
+        mean = self.z_mean + self.z_std * out[:, :d]
+        std = self.z_std * torch.exp(torch.clamp(out[:, d:], -7.0, 5.0))
+        return mean, std
+
+    def propose(self, image: torch.Tensor, n: int, prior_fraction: float = 0.25, widen: float = 1.0,
+                seed: int = 0) -> torch.Tensor:
+        """n starting points in z: (1 - prior_fraction) n from the network's
+        Gaussian (std scaled by ``widen``), the rest prior draws."""
+        mean, std = self.predict(image)
+        n_prior = int(round(prior_fraction * n))
+        n_net = n - n_prior
+        g = torch.Generator(device=self.device)
+        g.manual_seed(int(seed) + 12345)
+        z_net = mean[0] + widen * std[0] * torch.randn((n_net, self.d), device=self.device, generator=g)
+        parts = [z_net]
+        if n_prior > 0:
+            parts.append(self.bij.inverse(self.prior.sample(n_prior, seed=seed + 1)).to(self.device, torch.float32))
+        return torch.cat(parts, 0).contiguous()
+
+
+def map_trace(fitter, start: torch.Tensor, num_steps: int, lr: float):
+    """Run ``fitter.MAP`` from ``start`` and return (best lp per step, best chi2
+    per step, seconds per step), all on the host."""
+    from .inference import Adam
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fitter.MAP(optimizer=Adam(lr), start=start, num_steps=num_steps)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / num_steps
+    lp = torch.nan_to_num(fitter.lp_history, nan=-float("inf")).max(1).values.cpu().numpy()
+    chi = torch.nan_to_num(fitter.chi2_history, nan=float("inf")).min(1).values.cpu().numpy()
+    return lp, chi, dt
+
+
+def first_step(values: np.ndarray, ok) -> Optional[int]:
+    idx = np.nonzero(ok(values))[0]
+    return int(idx[0]) + 1 if len(idx) else None

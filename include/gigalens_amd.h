@@ -164,6 +164,44 @@ int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, 
                      float lr, float beta1, float beta2, float epsilon,
                      float* lp, float* chi2, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Learned initialiser.  No counterpart in the reference: a small BF16 CNN
+ * (hand-written MFMA kernels, FP32 accumulation) mapping an observed image to
+ * a diagonal Gaussian (mean, log-std) over the unconstrained parameters, used
+ * to seed ModellingSequence.MAP (tf/inference.py:17-41) next to prior draws.
+ * Architecture: 4 x (3x3 conv pad 1 + bias + tanh-GELU), strides 1,2,1,2,
+ * global average pool, Linear(c4 -> hidden) + GELU, Linear(hidden -> 2 n_out).
+ * Parameters are one FP32 vector, per conv layer W[co][(kh*3+kw)*cin + ci]
+ * then b[co], then W1[hidden][c4], b1, W2[2 n_out][hidden], b2.
+ * ------------------------------------------------------------------------- */
+typedef struct gl_cnn gl_cnn;
+
+typedef struct gl_cnn_desc {
+    int height, width;    /* input image */
+    int channels[4];      /* conv widths, multiples of 32 */
+    int hidden;           /* MLP width, <= 512 */
+    int n_out;            /* d: parameters predicted (outputs 2 d), 2 d <= 256 */
+    int max_batch;        /* images per call (workspace is sized for it) */
+    float in_scale;       /* input transform asinh(img / in_scale) */
+} gl_cnn_desc;
+
+int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out);
+int gl_cnn_destroy(gl_cnn* net);
+int gl_cnn_num_params(const gl_cnn* net);
+/* Copy a parameter vector in (device memory) and reset the AdamW moments. */
+int gl_cnn_set_params(gl_cnn* net, const float* params, hipStream_t stream);
+int gl_cnn_get_params(const gl_cnn* net, float* params, hipStream_t stream);
+/* out (bs x 2 n_out): mean, then log-std (unclamped) of each image (bs x H x W). */
+int gl_cnn_forward(gl_cnn* net, const float* images, int bs, float* out, hipStream_t stream);
+/* Mean Gaussian NLL over the batch against target (bs x n_out), log-std clamped to
+ * [-7, 5]; its gradient is kept for gl_cnn_adamw_step and copied to grad when
+ * non-NULL.  loss: one float (device), may be NULL. */
+int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* target, int bs, float* loss, float* grad,
+                     hipStream_t stream);
+/* AdamW (torch.optim.AdamW semantics, step 1-based) on the kept gradient. */
+int gl_cnn_adamw_step(gl_cnn* net, int step, float lr, float beta1, float beta2, float eps,
+                      float weight_decay, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
