@@ -34,6 +34,8 @@ EXPORTED_SYMBOLS = (
     "gl_log_prob",
     "gl_map_adam_step",
     "gl_lstsq_simulate",
+    "gl_log_prob_lenses",
+    "gl_map_adam_step_lenses",
     "gl_cnn_create",
     "gl_cnn_destroy",
     "gl_cnn_num_params",
@@ -90,6 +92,7 @@ class PlanDesc(ctypes.Structure):
         ("psf_rank_tol", ctypes.c_float),
         ("max_batch", ctypes.c_int),
         ("noise_map", ctypes.POINTER(ctypes.c_float)),
+        ("n_observed", ctypes.c_int),
     ]
 
 
@@ -124,6 +127,9 @@ _SIGS = {
     "gl_lstsq_simulate": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
     "gl_map_adam_step": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
+    "gl_log_prob_lenses": ([_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "gl_map_adam_step_lenses": ([_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                 ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
     "gl_cnn_create": ([ctypes.POINTER(CnnDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "gl_cnn_destroy": ([_vp], ctypes.c_int),
     "gl_cnn_num_params": ([_vp], ctypes.c_int),

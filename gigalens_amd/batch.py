@@ -1,0 +1,87 @@
+"""Fitting many lens systems at once (no counterpart in the reference, which
+builds one simulator and one probabilistic model per lens).
+
+``MultiLensMAP`` runs the multi-start MAP of ``ModellingSequence.MAP``
+(src/gigalens/tf/inference.py:17-41) for several lens systems that share a
+physical model, camera and prior but have their own observed images: one plan
+holds all observed images, every particle carries the index of its lens, and
+each Adam step is one fused launch over all particles of all lenses.  With
+per-particle loss weights 1 / (particles of its lens) every lens follows
+exactly the optimisation it would follow alone.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import distributions as dist
+from .inference import Adam
+from .plan import Plan, layout_from_paths
+from .simulator import Camera
+
+
+class MultiLensMAP:
+    def __init__(self, phys_model, sim_config, prior, observed_images, background_rms: float, exp_time: float,
+                 device="cuda"):
+        self.device = torch.device(device)
+        self.prior = prior
+        obs = np.ascontiguousarray(np.asarray(observed_images, np.float32))
+        if obs.ndim == 2:
+            obs = obs[None]
+        self.n_lenses = obs.shape[0]
+        self.bij = dist.ChainBijector(prior.experimental_default_event_space_bijector(), dist.PackBijector(prior))
+        paths = [p for p, _ in dist.leaves(prior)]
+        self.plan = Plan(phys_model, Camera(sim_config), layout_from_paths(phys_model, paths), prior=prior,
+                         observed=obs, background_rms=background_rms, exp_time=exp_time,
+                         likelihood=nat.GL_LIK_FORWARD)
+        self.lp_history: Optional[torch.Tensor] = None
+        self.chi2_history: Optional[torch.Tensor] = None
+
+    def log_prob(self, z: torch.Tensor, lens: torch.Tensor):
+        lp, chi2, _ = self.plan.log_prob(z.contiguous(), lens=lens)
+        return lp, chi2
+
+    def run(self, z0: torch.Tensor, lens: torch.Tensor, num_steps: int = 350, optimizer: Optional[Adam] = None,
+            record_every: int = 1) -> torch.Tensor:
+        """Adam (Keras semantics) from z0 (n, d); lens (n,) int32.  Returns z;
+        ``lp_history`` / ``chi2_history`` hold the per-particle values of every
+        recorded step (before that step's update)."""
+        optimizer = optimizer if optimizer is not None else Adam(1e-2)
+        z = z0.to(self.device, torch.float32).contiguous().clone()
+        lens = lens.to(self.device, torch.int32).contiguous()
+        n = z.shape[0]
+        counts = torch.bincount(lens.long(), minlength=self.n_lenses).float()
+        if int(lens.min()) < 0 or int(lens.max()) >= self.n_lenses:
+            raise ValueError("lens index out of range")
+        weight = (1.0 / counts[lens.long()]).contiguous()
+        m, v = torch.zeros_like(z), torch.zeros_like(z)
+        nrec = (num_steps - 1) // record_every + 1
+        lp_h = torch.empty((nrec, n), device=self.device)
+        chi_h = torch.empty((nrec, n), device=self.device)
+        scratch = torch.empty((2, n), device=self.device)
+        for s in range(num_steps):
+            rec = s % record_every == 0
+            lp = lp_h[s // record_every] if rec else scratch[0]
+            chi = chi_h[s // record_every] if rec else scratch[1]
+            self.plan.map_adam_step(z, m, v, optimizer.iterations + 1, optimizer.lr(optimizer.iterations),
+                                    optimizer.beta_1, optimizer.beta_2, optimizer.epsilon, lp=lp, chi2=chi,
+                                    lens=lens, weight=weight)
+            optimizer.iterations += 1
+        self.lp_history, self.chi2_history = lp_h, chi_h
+        return z
+
+    def per_lens_best(self, lens: torch.Tensor):
+        """(best lp per recorded step, best chi2 per recorded step), each (steps, lenses), on the host."""
+        lens = lens.to(self.device).long()
+        L = self.n_lenses
+        lp = torch.nan_to_num(self.lp_history, nan=-float("inf"))
+        chi = torch.nan_to_num(self.chi2_history, nan=float("inf"))
+        idx = lens[None].expand_as(lp)
+        best_lp = torch.full((lp.shape[0], L), -float("inf"), device=self.device).scatter_reduce(
+            1, idx, lp, reduce="amax")
+        best_chi = torch.full((chi.shape[0], L), float("inf"), device=self.device).scatter_reduce(
+            1, idx, chi, reduce="amin")
+        return best_lp.cpu().numpy(), best_chi.cpu().numpy()

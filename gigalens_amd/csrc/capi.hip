@@ -434,9 +434,12 @@ static int upload_common(const gl_plan_desc* desc, gl_plan* p) {
     const int N = P.N;
     void* dptr = nullptr;
     int rc;
-    std::vector<float> obs(N * N, 0.0f), ivar(N * N, 0.0f), lsig(N * N, 0.0f);
+    const int nobs = desc->n_observed > 0 ? desc->n_observed : 1;
+    P.n_obs = nobs;
+    std::vector<float> obs((size_t)nobs * N * N, 0.0f), ivar((size_t)nobs * N * N, 0.0f),
+        lsig((size_t)nobs * N * N, 0.0f);
     if (desc->observed) {
-        for (int i = 0; i < N * N; ++i) {
+        for (size_t i = 0; i < (size_t)nobs * N * N; ++i) {
             obs[i] = desc->observed[i];
             // sqrt(bkg^2 + clip(obs, 0, inf) / t_exp), tf/model.py:113-115, or the given map
             float var = desc->noise_map ? desc->noise_map[i] * desc->noise_map[i]
@@ -592,6 +595,10 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             W[pp * kf + qq] = (float)(acc / (ss * ss));
         }
 
+    if (p->mc && desc->n_observed > 1) {
+        gl_plan_destroy(p);
+        return fail(GL_ERR_UNSUPPORTED, "least-squares plans take one observed image");
+    }
     if (p->mc) {
         int rc = mc_create(desc, p, W, max_niter);
         if (rc) { gl_plan_destroy(p); return rc; }
@@ -816,6 +823,21 @@ extern "C" int gl_log_prob(const gl_plan* plan, const float* z, int bs, float* l
     return run(plan, A, grad_z != nullptr, stream);
 }
 
+extern "C" int gl_log_prob_lenses(const gl_plan* plan, const float* z, const int* lens, int bs, float* lp,
+                                  float* chi2, const float* g_lp, const float* g_chi2, float* grad_z,
+                                  hipStream_t stream) {
+    if (plan && plan->mc && lens) return fail(GL_ERR_UNSUPPORTED, "least-squares plans take one observed image");
+    gl::KArgs A = blank(gl::M_LOGPROB, bs);
+    A.in = z;
+    A.lp = lp;
+    A.chi2 = chi2;
+    A.g_lp = g_lp;
+    A.g_chi2 = g_chi2;
+    A.grad = grad_z;
+    A.lens = lens;
+    return run(plan, A, grad_z != nullptr, stream);
+}
+
 extern "C" int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, float* img, float* coeffs,
                                  float* stacked, hipStream_t stream) {
     if (!plan) return fail(GL_ERR_ARG, "null plan");
@@ -824,8 +846,9 @@ extern "C" int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, fl
     return run_mc(plan, x, 0, bs, nullptr, nullptr, img, coeffs, stacked, nullptr, nullptr, nullptr, stream);
 }
 
-extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, int step, float lr,
-                                float beta1, float beta2, float epsilon, float* lp, float* chi2, hipStream_t stream) {
+extern "C" int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, float* v, const int* lens,
+                                       const float* weight, int bs, int step, float lr, float beta1, float beta2,
+                                       float epsilon, float* lp, float* chi2, hipStream_t stream) {
     if (!z || !m || !v) return fail(GL_ERR_ARG, "null optimizer state");
     if (step < 1) return fail(GL_ERR_ARG, "step is 1-based");
     gl::KArgs A = blank(gl::M_MAP, bs);
@@ -840,8 +863,17 @@ extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* 
     A.beta1 = beta1;
     A.beta2 = beta2;
     A.eps = epsilon;
-    A.gscale = -1.0f / ((float)bs * (float)(plan ? plan->P.N * plan->P.N : 1));
+    A.gscale = weight ? -1.0f / (float)(plan ? plan->P.N * plan->P.N : 1)
+                      : -1.0f / ((float)bs * (float)(plan ? plan->P.N * plan->P.N : 1));
+    A.lens = lens;
+    A.weight = weight;
     return run(plan, A, true, stream);
+}
+
+extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, int step, float lr,
+                                float beta1, float beta2, float epsilon, float* lp, float* chi2, hipStream_t stream) {
+    return gl_map_adam_step_lenses(plan, z, m, v, nullptr, nullptr, bs, step, lr, beta1, beta2, epsilon, lp, chi2,
+                                   stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -38,6 +38,8 @@ struct KArgs {
     float* grad;          // LOGPROB: grad_z, VJP: grad_x
     float* m;
     float* v;
+    const int* lens;      // optional per-particle observed-image index (bs)
+    const float* weight;  // MAP: optional per-particle loss weight (bs); default 1/bs
     float lr_t, beta1, beta2, eps, gscale;
     int mode;
     int bs;
@@ -546,7 +548,16 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
     float ll_acc = 0.0f, chi_acc = 0.0f;
     const float cf = P.cf;
     const float inv_np = 1.0f / (float)(N * N);
-    const float glp = A.mode == M_MAP ? A.gscale : (A.g_lp ? A.g_lp[pid] : 1.0f);
+    // observed image of this particle's lens; an index out of range gives NaN lp / chi2
+    int lens_i = A.lens ? A.lens[pid] : 0;
+    const bool bad_lens = lens_i < 0 || lens_i >= P.n_obs;
+    if (bad_lens) lens_i = 0;
+    const size_t obs_off = (size_t)lens_i * N * N;
+    const float* obs_p = P.obs + obs_off;
+    const float* ivar_p = P.inv_var_obs + obs_off;
+    const float* lsig_p = P.log_sig_obs + obs_off;
+    const float glp = A.mode == M_MAP ? (A.weight ? A.gscale * A.weight[pid] : A.gscale)
+                                      : (A.g_lp ? A.g_lp[pid] : 1.0f);
     const float gch = (A.mode == M_LOGPROB && A.g_chi2) ? A.g_chi2[pid] : 0.0f;
     const int ssk = ss * 1000 + P.kf;
     const int sepk = P.sep_rank > 0 ? ssk : 0;  // separable path specialisation (0: direct)
@@ -563,9 +574,9 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
         if (A.mode == M_VJP) return cf * A.g_img[(size_t)pid * N * N + pix];
         float ll, r2, dll, dr2;
         if (P.lik == GL_LIK_FORWARD)
-            glm::lik_forward(im, P.obs[pix], P.bkg2, P.inv_texp, ll, r2, dll, dr2);
+            glm::lik_forward(im, obs_p[pix], P.bkg2, P.inv_texp, ll, r2, dll, dr2);
         else
-            glm::lik_backward(im, P.obs[pix], P.inv_var_obs[pix], P.log_sig_obs[pix], ll, r2, dll, dr2);
+            glm::lik_backward(im, obs_p[pix], ivar_p[pix], lsig_p[pix], ll, r2, dll, dr2);
         ll_acc += ll;
         chi_acc += r2;
         return cf * (glp * dll + gch * inv_np * dr2);
@@ -804,8 +815,8 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
         float lp_prior = 0.0f;
         for (int j = 0; j < d; ++j) lp_prior += s_x[5 * d + j];
         if (A.mode == M_LOGPROB || A.mode == M_MAP) {
-            if (A.lp) A.lp[pid] = lp_like + lp_prior;
-            if (A.chi2) A.chi2[pid] = chi2;
+            if (A.lp) A.lp[pid] = bad_lens ? __builtin_nanf("") : lp_like + lp_prior;
+            if (A.chi2) A.chi2[pid] = bad_lens ? __builtin_nanf("") : chi2;
         }
         if (GRAD && A.mode != M_SIM) {
             const float* tot = s_tot + 2;

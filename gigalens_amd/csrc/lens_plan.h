@@ -32,6 +32,7 @@ struct DPlan {
     int max_n;                   // max EPL series length supported by the coef tables
     const float* w;              // kf*kf folded PSF+pool weights (correlation form, incl. 1/ss^2)
     const float* obs;            // N*N observed image
+    int n_obs;                   // observed images stacked in obs / inv_var_obs / log_sig_obs
     const float* inv_var_obs;    // N*N (backward likelihood)
     const float* log_sig_obs;    // N*N
     const int* prior_kind;       // d

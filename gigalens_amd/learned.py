@@ -250,3 +250,98 @@ def map_trace(fitter, start: torch.Tensor, num_steps: int, lr: float):
 def first_step(values: np.ndarray, ok) -> Optional[int]:
     idx = np.nonzero(ok(values))[0]
     return int(idx[0]) + 1 if len(idx) else None
+
+
+def steps_to(trace: np.ndarray, ok) -> np.ndarray:
+    """Per column of a (steps, lenses) trace: first 1-based step where ``ok``
+    holds, or -1."""
+    hit = ok(trace)
+    first = np.argmax(hit, axis=0) + 1
+    return np.where(hit.any(axis=0), first, -1)
+
+
+def compare_initialisation(init: "LearnedInitializer", n_lenses: int = 256, n_particles: int = 1000,
+                           num_steps: int = 350, lr: float = 5e-3, prior_fraction: float = 0.25,
+                           seed0: int = 1000, chi2_threshold: float = 1.1):
+    """Held-out evaluation of learned vs prior-sampled MAP starts.
+
+    Lens k is a prior draw simulated with fresh noise (seed seed0 + k).  Both
+    methods fit all lenses in one multi-lens MAP (``n_particles`` starts each,
+    Keras Adam ``lr``); per lens we record the steps until the best particle
+    reaches chi2 <= ``chi2_threshold`` and until the learned run comes within
+    1 nat of the prior run's final best log-posterior.  Wall times are those
+    steps times the measured time of one single-lens MAP step of
+    ``n_particles`` particles, plus (learned) the network's inference time."""
+    from .batch import MultiLensMAP
+    from .inference import Adam, ModellingSequence
+    from .model import ForwardProbModel
+    dev = init.device
+    imgs = []
+    for k in range(n_lenses):
+        im, _ = init.simulate_batch(1, seed=seed0 + k)
+        imgs.append(im[0])
+    images = torch.stack(imgs).contiguous()
+    lens = torch.arange(n_lenses, device=dev, dtype=torch.int32).repeat_interleave(n_particles)
+    # learned starts (the network's inference is timed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mean, std = init.predict(images)
+    torch.cuda.synchronize()
+    t_net = time.perf_counter() - t0
+    n_prior = int(round(prior_fraction * n_particles))
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed0 + 7)
+    z_l = mean.repeat_interleave(n_particles - n_prior, 0) + \
+        std.repeat_interleave(n_particles - n_prior, 0) * torch.randn(
+            (n_lenses * (n_particles - n_prior), init.d), device=dev, generator=g)
+    z_fallback = init.bij.inverse(init.prior.sample(n_lenses * n_prior, seed=seed0 + 11)).to(dev, torch.float32)
+    z_l = torch.cat([z_l.reshape(n_lenses, n_particles - n_prior, init.d),
+                     z_fallback.reshape(n_lenses, n_prior, init.d)], 1).reshape(-1, init.d)
+    z_p = init.bij.inverse(init.prior.sample(n_lenses * n_particles, seed=seed0 + 13)).to(dev, torch.float32)
+    mlm = MultiLensMAP(init.phys_model, init.sim_config, init.prior, images.cpu().numpy(), init.background_rms,
+                       init.exp_time, device=dev)
+    out = {}
+    for name, z0 in (("learned", z_l), ("prior", z_p)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mlm.run(z0, lens, num_steps, Adam(lr))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        best_lp, best_chi = mlm.per_lens_best(lens)
+        out[name] = {"best_lp": best_lp, "best_chi": best_chi, "batched_seconds": wall}
+    # one single-lens MAP step of n_particles (the per-lens cost of each step)
+    pm = ForwardProbModel(init.prior, images[0].cpu().numpy(), init.background_rms, init.exp_time)
+    fitter = ModellingSequence(init.phys_model, pm, init.sim_config, device=dev)
+    fitter.MAP(optimizer=Adam(lr), start=z_p[:n_particles], num_steps=5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fitter.MAP(optimizer=Adam(lr), start=z_p[:n_particles], num_steps=50)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / 50
+    L, P = out["learned"], out["prior"]
+    s_chi_l = steps_to(L["best_chi"], lambda c: c <= chi2_threshold)
+    s_chi_p = steps_to(P["best_chi"], lambda c: c <= chi2_threshold)
+    target = P["best_lp"][-1] - 1.0
+    s_lp_l = steps_to(L["best_lp"], lambda v: v >= target[None, :])
+    s_lp_p = steps_to(P["best_lp"], lambda v: v >= target[None, :])
+
+    def stats(s, extra=0.0):
+        ok = s > 0
+        if not ok.any():
+            return {"reached": 0}
+        sec = s[ok] * t_step + extra
+        return {"reached": int(ok.sum()), "steps_p50": float(np.median(s[ok])),
+                "steps_p95": float(np.percentile(s[ok], 95)), "seconds_p50": float(np.median(sec)),
+                "seconds_p95": float(np.percentile(sec, 95))}
+    both = (s_lp_l > 0) & (s_lp_p > 0)
+    ratio = (s_lp_l[both] * t_step + t_net / n_lenses) / (s_lp_p[both] * t_step)
+    return {
+        "lenses": n_lenses, "particles": n_particles, "steps": num_steps, "lr": lr, "prior_fraction": prior_fraction,
+        "chi2_threshold": chi2_threshold, "single_lens_step_seconds": t_step,
+        "network_seconds_per_lens": t_net / n_lenses,
+        "time_to_chi2": {"learned": stats(s_chi_l, t_net / n_lenses), "prior": stats(s_chi_p)},
+        "time_to_prior_best_lp_within_1nat": {"learned": stats(s_lp_l, t_net / n_lenses), "prior": stats(s_lp_p),
+                                             "time_ratio_p50": float(np.median(ratio)) if both.any() else None},
+        "final_best_lp_gain_p50": float(np.median(L["best_lp"][-1] - P["best_lp"][-1])),
+        "batched_seconds": {"learned": L["batched_seconds"], "prior": P["batched_seconds"]},
+    }

@@ -89,6 +89,12 @@ def _dev(t: torch.Tensor, name: str) -> torch.Tensor:
     return t
 
 
+def _lens(t: torch.Tensor, bs: int) -> torch.Tensor:
+    if not t.is_cuda or t.dtype != torch.int32 or not t.is_contiguous() or t.numel() != bs:
+        raise TypeError("lens must be a contiguous int32 CUDA tensor with one entry per particle")
+    return t
+
+
 class Plan:
     def __init__(self, phys_model, camera, layout: Layout, prior=None, observed=None, background_rms=1.0,
                  exp_time=1.0, likelihood=nat.GL_LIK_FORWARD, fldj_at_x=False, epl_batch_series=False,
@@ -155,12 +161,16 @@ class Plan:
         self._pa = (ctypes.c_float * (4 * self.d))(*args)
         desc.prior_kind = self._pk
         desc.prior_args = self._pa
+        self.n_observed = 1
         if observed is not None:
             obs = np.ascontiguousarray(np.asarray(observed, dtype=np.float32))
-            if obs.shape != (self.N, self.N):
-                raise ValueError(f"observed image must be {self.N}x{self.N}")
+            if obs.ndim == 3 and obs.shape[1:] == (self.N, self.N):
+                self.n_observed = obs.shape[0]
+            elif obs.shape != (self.N, self.N):
+                raise ValueError(f"observed image must be {self.N}x{self.N} or (lenses, {self.N}, {self.N})")
             self._obs = obs
             desc.observed = obs.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+            desc.n_observed = self.n_observed
         desc.background_rms = float(background_rms)
         desc.exp_time = float(exp_time)
         desc.likelihood = int(likelihood)
@@ -170,8 +180,8 @@ class Plan:
         desc.max_batch = int(max_batch)
         if noise_map is not None:
             nm = np.ascontiguousarray(np.asarray(noise_map, dtype=np.float32))
-            if nm.shape != (self.N, self.N):
-                raise ValueError(f"noise map must be {self.N}x{self.N}")
+            if nm.size != self.n_observed * self.N * self.N:
+                raise ValueError(f"noise map must hold {self.n_observed} image(s) of {self.N}x{self.N}")
             self._nm = nm
             desc.noise_map = nm.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         self.depth = sum(getattr(prof, "depth", 1) if getattr(prof, "use_lstsq", False) or prof._kind == nat.GL_SHAPELETS
@@ -212,14 +222,21 @@ class Plan:
                   "gl_simulate_vjp")
         return grad
 
-    def log_prob(self, z: torch.Tensor, g_lp=None, g_chi2=None, want_grad=False):
+    def log_prob(self, z: torch.Tensor, g_lp=None, g_chi2=None, want_grad=False, lens=None):
+        """lp, chi2 (and dlp/dz when ``want_grad``); ``lens`` (bs int32) picks
+        each particle's observed image of a multi-lens plan."""
         z = _dev(z, "z")
         bs = z.shape[0]
         lp = torch.empty(bs, device=z.device, dtype=torch.float32)
         chi2 = torch.empty(bs, device=z.device, dtype=torch.float32)
         grad = torch.empty_like(z) if want_grad else None
-        nat.check(self._lib.gl_log_prob(self._h, _ptr(z), bs, _ptr(lp), _ptr(chi2), _ptr(g_lp), _ptr(g_chi2),
-                                        _ptr(grad), _stream()), "gl_log_prob")
+        if lens is None:
+            nat.check(self._lib.gl_log_prob(self._h, _ptr(z), bs, _ptr(lp), _ptr(chi2), _ptr(g_lp), _ptr(g_chi2),
+                                            _ptr(grad), _stream()), "gl_log_prob")
+        else:
+            lens = _lens(lens, bs)
+            nat.check(self._lib.gl_log_prob_lenses(self._h, _ptr(z), _ptr(lens), bs, _ptr(lp), _ptr(chi2), _ptr(g_lp),
+                                                   _ptr(g_chi2), _ptr(grad), _stream()), "gl_log_prob_lenses")
         return lp, chi2, grad
 
     def lstsq_simulate(self, x: torch.Tensor, want_image=True, want_coeffs=False, want_stacked=False):
@@ -235,8 +252,17 @@ class Plan:
         return img, coef, st
 
     def map_adam_step(self, z, m, v, step: int, lr: float, beta1=0.9, beta2=0.999, epsilon=1e-7, lp=None,
-                      chi2=None):
+                      chi2=None, lens=None, weight=None):
         bs = z.shape[0]
-        nat.check(self._lib.gl_map_adam_step(self._h, _ptr(z), _ptr(m), _ptr(v), bs, int(step), float(lr),
-                                             float(beta1), float(beta2), float(epsilon), _ptr(lp), _ptr(chi2),
-                                             _stream()), "gl_map_adam_step")
+        if lens is None and weight is None:
+            nat.check(self._lib.gl_map_adam_step(self._h, _ptr(z), _ptr(m), _ptr(v), bs, int(step), float(lr),
+                                                 float(beta1), float(beta2), float(epsilon), _ptr(lp), _ptr(chi2),
+                                                 _stream()), "gl_map_adam_step")
+        else:
+            lens = None if lens is None else _lens(lens, bs)
+            if weight is not None:
+                weight = _dev(weight, "weight")
+            nat.check(self._lib.gl_map_adam_step_lenses(self._h, _ptr(z), _ptr(m), _ptr(v), _ptr(lens), _ptr(weight),
+                                                        bs, int(step), float(lr), float(beta1), float(beta2),
+                                                        float(epsilon), _ptr(lp), _ptr(chi2), _stream()),
+                      "gl_map_adam_step_lenses")

@@ -99,6 +99,9 @@ typedef struct gl_plan_desc {
     int max_batch;            /* least-squares plans: particles per internal chunk (workspace is
                                  allocated for this many at plan creation); 0 selects 64 */
     const float* noise_map;   /* HOST N*N sigma per pixel; NULL: sqrt(bkg^2 + clip(obs,0)/t_exp) */
+    int n_observed;           /* observed images stacked in `observed` (and `noise_map`), each
+                                 N*N; 0 selects 1.  Particles pick one with the lens index of
+                                 gl_log_prob_lenses / gl_map_adam_step_lenses */
 } gl_plan_desc;
 
 typedef struct gl_plan gl_plan;
@@ -156,6 +159,13 @@ int gl_log_prob(const gl_plan* plan, const float* z, int bs, float* lp, float* c
 int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, float* img, float* coeffs,
                       float* stacked, hipStream_t stream);
 
+/* gl_log_prob over several lens systems of one shape in one launch: particle i
+ * is compared with observed image lens[i] of the plan (lens NULL: image 0).
+ * An index outside [0, n_observed) gives NaN lp and chi2 for that particle.
+ * No counterpart in the reference (which builds one simulator per lens). */
+int gl_log_prob_lenses(const gl_plan* plan, const float* z, const int* lens, int bs, float* lp, float* chi2,
+                       const float* g_lp, const float* g_chi2, float* grad_z, hipStream_t stream);
+
 /* One multi-start MAP step with Keras-Adam semantics, fused with log_prob and
  * its adjoint: loss = mean_i(-lp_i / N^2) (tf/inference.py:28-35), then
  * m, v, z updated in place (tf.keras Adam, epsilon outside the sqrt).
@@ -163,6 +173,14 @@ int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, float* img, f
 int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, int step,
                      float lr, float beta1, float beta2, float epsilon,
                      float* lp, float* chi2, hipStream_t stream);
+
+/* gl_map_adam_step over several lens systems of one shape in one launch
+ * (lens as in gl_log_prob_lenses).  Particle i's loss term is
+ * -weight[i] * lp_i / N^2 (weight NULL: 1/bs), so weight[i] = 1 / (particles of
+ * its lens) reproduces independent per-lens MAP runs (tf/inference.py:28-35). */
+int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, float* v, const int* lens,
+                            const float* weight, int bs, int step, float lr, float beta1, float beta2,
+                            float epsilon, float* lp, float* chi2, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Learned initialiser.  No counterpart in the reference: a small BF16 CNN

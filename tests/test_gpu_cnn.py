@@ -117,3 +117,17 @@ def test_training_beats_prior_starts(cuda):
         _print(f"lens {k}: start lp learned {lp_l[0]:.1f} prior {lp_p[0]:.1f}; steps to chi2<=1.1 learned {sl} prior {sp}")
     assert wins_lp >= n_lens - 1
     assert np.median(steps_l) <= np.median(steps_p)
+
+
+def test_compare_initialisation_small(cuda):
+    """The held-out learned-vs-prior harness on 8 lenses x 256 particles."""
+    from gigalens_amd.learned import compare_initialisation
+    prior = wl.epl_shear_prior()
+    init = LearnedInitializer(wl.epl_shear_model(), wl.epl_shear_config(), prior, 0.2, 100.0, max_batch=256,
+                              seed=0, device=cuda)
+    init.train(n_images=1200 * 256, batch=256, lr=1e-3, seed=10)
+    r = compare_initialisation(init, n_lenses=8, n_particles=256, num_steps=200, lr=5e-3)
+    _print(r)
+    tl, tp = r["time_to_chi2"]["learned"], r["time_to_chi2"]["prior"]
+    assert tl["reached"] >= tp["reached"]
+    assert r["final_best_lp_gain_p50"] > -1.0
