@@ -237,6 +237,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="also launch a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--workload", choices=["map", "shapelets"], default="map")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-prior time-to-MAP evaluation")
+    ap.add_argument("--learned-images", type=int, default=2 ** 20)
+    ap.add_argument("--learned-lenses", type=int, default=256)
+    ap.add_argument("--learned-lr", type=float, default=1e-3)
+    ap.add_argument("--learned-schedule", choices=["constant", "cosine"], default="constant")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -335,9 +340,32 @@ def main():
                         "seconds": (int(hit[0]) + 1) * ms_per_step / 1e3 if len(hit) else None,
                         "best_chi2_final": float(best[-1]), "init": "prior"},
     }
+    if not args.no_learned:
+        out["time_to_map"]["learned_vs_prior"] = learned_time_to_map(prior, phys, cfg, dev, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(obs)
     print(json.dumps(out), flush=True)
+
+
+def learned_time_to_map(prior, phys, cfg, dev, args):
+    """Learned initialiser: trained on ``--learned-images`` simulated images
+    (batch 1024, AdamW lr ``--learned-lr``, seed 10), then held-out lenses
+    (seeds 1000..) fitted by 1,000-particle MAP from screened network
+    proposals, from prior draws and from screened prior draws (one multi-lens
+    launch per step for each)."""
+    from gigalens_amd.learned import LearnedInitializer, compare_initialisation
+    init = LearnedInitializer(phys, cfg, prior, 0.2, 100.0, max_batch=1024, seed=0, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = init.train(n_images=args.learned_images, batch=1024, lr=args.learned_lr, weight_decay=0.01, seed=10,
+                        schedule=args.learned_schedule)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    r = compare_initialisation(init, n_lenses=args.learned_lenses, n_particles=1000, num_steps=350, lr=args.lr)
+    r["training"] = {"images": args.learned_images, "batch": 1024, "lr": args.learned_lr,
+                     "schedule": args.learned_schedule, "seconds": t_train,
+                     "nll_first10": float(np.mean(losses[:10])), "nll_last10": float(np.mean(losses[-10:]))}
+    return r
 
 
 if __name__ == "__main__":

@@ -192,15 +192,20 @@ class LearnedInitializer:
 
     # ---- training --------------------------------------------------------------
     def train(self, n_images: int = 2 ** 20, batch: int = 1024, lr: float = 3e-4, weight_decay: float = 0.01,
-              seed: int = 10, log_every: int = 0) -> List[float]:
-        """AdamW on the Gaussian NLL, fresh simulated batches (seeds seed, seed+1, ...)."""
+              seed: int = 10, log_every: int = 0, schedule: str = "constant", warmup: int = 50) -> List[float]:
+        """AdamW on the Gaussian NLL, fresh simulated batches (seeds seed, seed+1, ...).
+        ``schedule`` "cosine": linear warm-up over ``warmup`` steps to ``lr``,
+        then cosine decay to 0 at the last step."""
         batch = min(int(batch), self.max_batch)
         steps = max(1, int(n_images) // batch)
         loss_acc = []
         for s in range(1, steps + 1):
             images, z = self.simulate_batch(batch, seed=seed * 1_000_003 + s)
             loss, _ = self.net.loss_grad(images, self.standardise(z))
-            self.net.adamw_step(len(self.losses) + len(loss_acc) + 1, lr, weight_decay=weight_decay)
+            lr_s = lr
+            if schedule == "cosine":
+                lr_s = lr * min(1.0, s / max(1, warmup)) * 0.5 * (1.0 + math.cos(math.pi * (s - 1) / steps))
+            self.net.adamw_step(len(self.losses) + len(loss_acc) + 1, lr_s, weight_decay=weight_decay)
             loss_acc.append(loss)
             if log_every and s % log_every == 0:
                 print(f"step {s}/{steps}: loss {float(loss):.4f}", flush=True)
@@ -218,9 +223,17 @@ class LearnedInitializer:
         return mean, std
 
     def propose(self, image: torch.Tensor, n: int, prior_fraction: float = 0.25, widen: float = 1.0,
-                seed: int = 0) -> torch.Tensor:
-        """n starting points in z: (1 - prior_fraction) n from the network's
-        Gaussian (std scaled by ``widen``), the rest prior draws."""
+                seed: int = 0, prob_model=None, oversample: int = 8) -> torch.Tensor:
+        """n starting points in z: (1 - prior_fraction) from the network's
+        Gaussian (std scaled by ``widen``), the rest prior draws.  With a
+        ``prob_model`` (whose observed image is ``image``) ``oversample * n``
+        candidates are drawn and the n with the highest log-posterior kept."""
+        if prob_model is not None and oversample > 1:
+            cand = self.propose(image, oversample * n, prior_fraction, widen, seed)
+            sim = LensSimulator(self.phys_model, self.sim_config, bs=cand.shape[0], device=self.device)
+            lp, _, _ = prob_model.plan_for(sim).log_prob(cand)
+            keep = torch.topk(torch.nan_to_num(lp, nan=-float("inf")), n).indices
+            return cand[keep].contiguous()
         mean, std = self.predict(image)
         n_prior = int(round(prior_fraction * n))
         n_net = n - n_prior
@@ -260,72 +273,95 @@ def steps_to(trace: np.ndarray, ok) -> np.ndarray:
     return np.where(hit.any(axis=0), first, -1)
 
 
+def screen(log_prob_fn, z: torch.Tensor, lens: torch.Tensor, n_lenses: int, per_lens: int, keep: int):
+    """Keep the ``keep`` highest-log-posterior candidates of each lens.  The
+    candidates are grouped by lens (``per_lens`` each, lens-major order)."""
+    lp, _ = log_prob_fn(z, lens)
+    lp = torch.nan_to_num(lp, nan=-float("inf")).reshape(n_lenses, per_lens)
+    idx = torch.topk(lp, keep, dim=1).indices + per_lens * torch.arange(n_lenses, device=z.device)[:, None]
+    return z[idx.reshape(-1)].contiguous()
+
+
 def compare_initialisation(init: "LearnedInitializer", n_lenses: int = 256, n_particles: int = 1000,
                            num_steps: int = 350, lr: float = 5e-3, prior_fraction: float = 0.25,
-                           seed0: int = 1000, chi2_threshold: float = 1.1):
+                           oversample: int = 8, seed0: int = 1000, chi2_threshold: float = 1.1):
     """Held-out evaluation of learned vs prior-sampled MAP starts.
 
-    Lens k is a prior draw simulated with fresh noise (seed seed0 + k).  Both
-    methods fit all lenses in one multi-lens MAP (``n_particles`` starts each,
-    Keras Adam ``lr``); per lens we record the steps until the best particle
-    reaches chi2 <= ``chi2_threshold`` and until the learned run comes within
-    1 nat of the prior run's final best log-posterior.  Wall times are those
-    steps times the measured time of one single-lens MAP step of
-    ``n_particles`` particles, plus (learned) the network's inference time."""
+    Lens k is a prior draw simulated with fresh noise (seed seed0 + k).  Three
+    start strategies, each fitting all lenses in one multi-lens MAP
+    (``n_particles`` starts per lens, Keras Adam ``lr``):
+
+      * ``prior``          -- prior draws (the reference's MAP, tf/inference.py:21-24)
+      * ``learned``        -- ``oversample * n_particles`` candidates, (1 - prior_fraction)
+                              from the network's Gaussian and the rest prior draws,
+                              screened by the log-posterior (keep the best n_particles)
+      * ``prior_screened`` -- the same screening applied to prior draws only
+                              (what the screening alone buys)
+
+    Per lens: steps until the best particle reaches chi2 <= ``chi2_threshold``
+    and until it comes within 1 nat of the ``prior`` run's final best
+    log-posterior.  Wall times are steps times the measured time of one
+    single-lens MAP step of ``n_particles``, plus the measured per-lens
+    start-up cost (network inference, screening evaluation)."""
     from .batch import MultiLensMAP
     from .inference import Adam, ModellingSequence
     from .model import ForwardProbModel
     dev = init.device
+    d = init.d
     imgs = []
     for k in range(n_lenses):
         im, _ = init.simulate_batch(1, seed=seed0 + k)
         imgs.append(im[0])
     images = torch.stack(imgs).contiguous()
+    mlm = MultiLensMAP(init.phys_model, init.sim_config, init.prior, images.cpu().numpy(), init.background_rms,
+                       init.exp_time, device=dev)
     lens = torch.arange(n_lenses, device=dev, dtype=torch.int32).repeat_interleave(n_particles)
-    # learned starts (the network's inference is timed)
+    K = oversample * n_particles
+    lens_c = torch.arange(n_lenses, device=dev, dtype=torch.int32).repeat_interleave(K)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed0 + 7)
+    n_prior = int(round(prior_fraction * K))
+    n_net = K - n_prior
+    # learned candidates: network Gaussian + prior fallback, then screening
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     mean, std = init.predict(images)
+    z_net = (mean[:, None, :] + std[:, None, :] * torch.randn((n_lenses, n_net, d), device=dev, generator=g))
+    z_net[:, 0] = mean  # the network's point estimate is one of the candidates
+    z_fb = init.bij.inverse(init.prior.sample(n_lenses * n_prior, seed=seed0 + 11)).to(dev, torch.float32)
+    z_c = torch.cat([z_net, z_fb.reshape(n_lenses, n_prior, d)], 1).reshape(-1, d).contiguous()
+    z_learned = screen(mlm.log_prob, z_c, lens_c, n_lenses, K, n_particles)
     torch.cuda.synchronize()
-    t_net = time.perf_counter() - t0
-    n_prior = int(round(prior_fraction * n_particles))
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed0 + 7)
-    z_l = mean.repeat_interleave(n_particles - n_prior, 0) + \
-        std.repeat_interleave(n_particles - n_prior, 0) * torch.randn(
-            (n_lenses * (n_particles - n_prior), init.d), device=dev, generator=g)
-    z_fallback = init.bij.inverse(init.prior.sample(n_lenses * n_prior, seed=seed0 + 11)).to(dev, torch.float32)
-    z_l = torch.cat([z_l.reshape(n_lenses, n_particles - n_prior, init.d),
-                     z_fallback.reshape(n_lenses, n_prior, init.d)], 1).reshape(-1, init.d)
-    z_p = init.bij.inverse(init.prior.sample(n_lenses * n_particles, seed=seed0 + 13)).to(dev, torch.float32)
-    mlm = MultiLensMAP(init.phys_model, init.sim_config, init.prior, images.cpu().numpy(), init.background_rms,
-                       init.exp_time, device=dev)
-    out = {}
-    for name, z0 in (("learned", z_l), ("prior", z_p)):
+    t_learned_setup = (time.perf_counter() - t0) / n_lenses
+    z_pc = init.bij.inverse(init.prior.sample(n_lenses * K, seed=seed0 + 17)).to(dev, torch.float32).contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    z_pscreen = screen(mlm.log_prob, z_pc, lens_c, n_lenses, K, n_particles)
+    torch.cuda.synchronize()
+    t_screen_setup = (time.perf_counter() - t0) / n_lenses
+    z_prior = init.bij.inverse(init.prior.sample(n_lenses * n_particles, seed=seed0 + 13)).to(dev, torch.float32)
+    runs = {}
+    for name, z0 in (("learned", z_learned), ("prior", z_prior), ("prior_screened", z_pscreen)):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         mlm.run(z0, lens, num_steps, Adam(lr))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         best_lp, best_chi = mlm.per_lens_best(lens)
-        out[name] = {"best_lp": best_lp, "best_chi": best_chi, "batched_seconds": wall}
+        runs[name] = {"best_lp": best_lp, "best_chi": best_chi, "batched_seconds": wall}
     # one single-lens MAP step of n_particles (the per-lens cost of each step)
     pm = ForwardProbModel(init.prior, images[0].cpu().numpy(), init.background_rms, init.exp_time)
     fitter = ModellingSequence(init.phys_model, pm, init.sim_config, device=dev)
-    fitter.MAP(optimizer=Adam(lr), start=z_p[:n_particles], num_steps=5)
+    fitter.MAP(optimizer=Adam(lr), start=z_prior[:n_particles], num_steps=5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fitter.MAP(optimizer=Adam(lr), start=z_p[:n_particles], num_steps=50)
+    fitter.MAP(optimizer=Adam(lr), start=z_prior[:n_particles], num_steps=50)
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t0) / 50
-    L, P = out["learned"], out["prior"]
-    s_chi_l = steps_to(L["best_chi"], lambda c: c <= chi2_threshold)
-    s_chi_p = steps_to(P["best_chi"], lambda c: c <= chi2_threshold)
-    target = P["best_lp"][-1] - 1.0
-    s_lp_l = steps_to(L["best_lp"], lambda v: v >= target[None, :])
-    s_lp_p = steps_to(P["best_lp"], lambda v: v >= target[None, :])
+    setup = {"learned": t_learned_setup, "prior": 0.0, "prior_screened": t_screen_setup}
+    target = runs["prior"]["best_lp"][-1] - 1.0
 
-    def stats(s, extra=0.0):
+    def stats(s, extra):
         ok = s > 0
         if not ok.any():
             return {"reached": 0}
@@ -333,15 +369,23 @@ def compare_initialisation(init: "LearnedInitializer", n_lenses: int = 256, n_pa
         return {"reached": int(ok.sum()), "steps_p50": float(np.median(s[ok])),
                 "steps_p95": float(np.percentile(s[ok], 95)), "seconds_p50": float(np.median(sec)),
                 "seconds_p95": float(np.percentile(sec, 95))}
-    both = (s_lp_l > 0) & (s_lp_p > 0)
-    ratio = (s_lp_l[both] * t_step + t_net / n_lenses) / (s_lp_p[both] * t_step)
-    return {
-        "lenses": n_lenses, "particles": n_particles, "steps": num_steps, "lr": lr, "prior_fraction": prior_fraction,
-        "chi2_threshold": chi2_threshold, "single_lens_step_seconds": t_step,
-        "network_seconds_per_lens": t_net / n_lenses,
-        "time_to_chi2": {"learned": stats(s_chi_l, t_net / n_lenses), "prior": stats(s_chi_p)},
-        "time_to_prior_best_lp_within_1nat": {"learned": stats(s_lp_l, t_net / n_lenses), "prior": stats(s_lp_p),
-                                             "time_ratio_p50": float(np.median(ratio)) if both.any() else None},
-        "final_best_lp_gain_p50": float(np.median(L["best_lp"][-1] - P["best_lp"][-1])),
-        "batched_seconds": {"learned": L["batched_seconds"], "prior": P["batched_seconds"]},
-    }
+    res = {"lenses": n_lenses, "particles": n_particles, "steps": num_steps, "lr": lr,
+           "prior_fraction": prior_fraction, "oversample": oversample, "chi2_threshold": chi2_threshold,
+           "single_lens_step_seconds": t_step, "setup_seconds_per_lens": setup,
+           "time_to_chi2": {}, "time_to_prior_best_lp_within_1nat": {}, "final_best_lp_gain_vs_prior_p50": {},
+           "batched_seconds": {}}
+    steps_lp = {}
+    for name, r in runs.items():
+        s_chi = steps_to(r["best_chi"], lambda c: c <= chi2_threshold)
+        s_lp = steps_to(r["best_lp"], lambda v: v >= target[None, :])
+        steps_lp[name] = s_lp
+        res["time_to_chi2"][name] = stats(s_chi, setup[name])
+        res["time_to_prior_best_lp_within_1nat"][name] = stats(s_lp, setup[name])
+        res["final_best_lp_gain_vs_prior_p50"][name] = float(np.median(r["best_lp"][-1] - runs["prior"]["best_lp"][-1]))
+        res["batched_seconds"][name] = r["batched_seconds"]
+    for name in ("learned", "prior_screened"):
+        both = (steps_lp[name] > 0) & (steps_lp["prior"] > 0)
+        ratio = (steps_lp[name][both] * t_step + setup[name]) / (steps_lp["prior"][both] * t_step)
+        res["time_to_prior_best_lp_within_1nat"][name]["time_ratio_vs_prior_p50"] = \
+            float(np.median(ratio)) if both.any() else None
+    return res

@@ -130,4 +130,4 @@ def test_compare_initialisation_small(cuda):
     _print(r)
     tl, tp = r["time_to_chi2"]["learned"], r["time_to_chi2"]["prior"]
     assert tl["reached"] >= tp["reached"]
-    assert r["final_best_lp_gain_p50"] > -1.0
+    assert r["final_best_lp_gain_vs_prior_p50"]["learned"] > -1.0
