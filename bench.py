@@ -11,6 +11,9 @@ HBM before the timed region.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 Prints one JSON line.
 
+``--workload cnn`` measures the learned initialiser's training step (1,024
+simulated 60x60 images, BF16 MFMA network, AdamW).
+
 ``--workload shapelets`` measures the least-squares configuration instead
 (100x100 px, supersample 4, 21x21 Moffat PSF, EPL+shear, Sersic lens light and
 an n_max=10 shapelet source with linear amplitudes; 512 prior particles,
@@ -227,6 +230,71 @@ def run_shapelets(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_cnn(args, dev):
+    """Learned-initialiser training throughput: one step = simulate 1,024 prior
+    images on the device (particle kernel + Poisson/Gaussian noise) and one
+    AdamW step of the BF16 MFMA network on them."""
+    from gigalens_amd.learned import LearnedInitializer
+    from oracle import cnn_ref
+    prior, phys, cfg = wl.epl_shear_prior(), wl.epl_shear_model(), wl.epl_shear_config()
+    bs = 1024
+    init = LearnedInitializer(phys, cfg, prior, 0.2, 100.0, max_batch=bs, seed=0, device=dev)
+    K = args.steps
+    init.train(n_images=bs * args.warmup, batch=bs, lr=1e-3, seed=3)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+    imgs, z = init.simulate_batch(bs, seed=1)
+    t = init.standardise(z)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(K):
+        ev[s][0].record(stream)
+        init.net.loss_grad(imgs, t)
+        init.net.adamw_step(args.warmup + s + 1, 1e-3)
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall_net = (time.perf_counter() - t0) / K
+    net_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    t0 = time.perf_counter()
+    init.train(n_images=bs * K, batch=bs, lr=1e-3, seed=4)
+    torch.cuda.synchronize()
+    wall_step = (time.perf_counter() - t0) / K
+    N = cfg.num_pix
+    ch = init.net.channels
+    fwd = 0.0
+    H, cin = N, 1
+    for i, c in enumerate(ch):
+        Ho = (H - 1) // (1, 2, 1, 2)[i] + 1
+        fwd += 2.0 * Ho * Ho * c * 9 * cin
+        H, cin = Ho, c
+    first = 2.0 * N * N * ch[0] * 9
+    flop_img = fwd + 2 * fwd - first
+    achieved = flop_img * bs / (net_ms * 1e-3) / 1e12
+    out = {"metric": "learned-initialiser training images/s", "value": bs / wall_step, "unit": "images/s",
+           "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": wall_step * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+           "data": "synthetic (prior draws simulated on the device, Poisson + Gaussian noise)",
+           "config": {"workload": "BF16 CNN 4x conv3x3 (32/32/64/64, strides 1,2,1,2) + GAP + MLP 256 -> 2x22 on "
+                                  "60x60 images, batch 1024, AdamW", "batch": bs, "num_pix": N},
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": achieved / 2500.0, "traffic": None,
+                        "kernel": "network step (gl::cnn::gemm x 7 + head + AdamW)", "network_ms": net_ms,
+                        "network_wall_ms": wall_net * 1e3, "flop_per_image": flop_img}}
+    if not args.no_cpu_baseline:
+        n = 64
+        p = init.net.get_params().cpu()
+        x = imgs[:n].cpu()
+        tt = t[:n].cpu()
+        cnn_ref.loss_and_grad(p, init.net.layout, x, tt, 0.2, ch)
+        t0 = time.perf_counter()
+        cnn_ref.loss_and_grad(p, init.net.layout, x, tt, 0.2, ch)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "images/s", "cores": torch.get_num_threads(), "kind": "port",
+                               "sample": f"{n} images, forward + NLL + autograd backward of the torch-CPU restatement "
+                                         "(FP32 with the same bf16 rounding points), 1 warm-up, 1 timed run"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,7 +304,7 @@ def main():
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="also launch a 256 MiB device copy (PMC calibration)")
-    ap.add_argument("--workload", choices=["map", "shapelets"], default="map")
+    ap.add_argument("--workload", choices=["map", "shapelets", "cnn"], default="map")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-prior time-to-MAP evaluation")
     ap.add_argument("--learned-images", type=int, default=2 ** 20)
     ap.add_argument("--learned-lenses", type=int, default=256)
@@ -253,6 +321,10 @@ def main():
         if args.steps == 350:
             args.steps = 10
         return run_shapelets(args, dev)
+    if args.workload == "cnn":
+        if args.steps == 350:
+            args.steps = 100
+        return run_cnn(args, dev)
     bs = args.particles
     prior, phys, cfg, sim, pm, obs, z0 = setup(bs, dev)
     plan = pm.plan_for(sim)
