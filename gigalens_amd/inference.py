@@ -110,6 +110,17 @@ class MultivariateNormalTriL:
         eps = torch.randn((int(n), self.d), generator=generator, device="cpu").to(self.loc)
         return self.loc + eps @ self.scale_tril().T
 
+    def sample_and_log_prob(self, n, generator: Optional[torch.Generator] = None):
+        """Reparameterised samples z = loc + L eps and log q(z) evaluated from eps,
+        -|eps|^2/2 - log det L - d/2 log 2 pi: the same value and total gradient
+        as log_prob(z), without the triangular solve (which loses all precision in
+        FP32 once L is ill-conditioned)."""
+        eps = torch.randn((int(n), self.d), generator=generator, device="cpu").to(self.loc)
+        L = self.scale_tril()
+        z = self.loc + eps @ L.T
+        lq = -0.5 * (eps ** 2).sum(1) - torch.log(torch.diagonal(L)).sum() - 0.5 * self.d * math.log(2 * math.pi)
+        return z, lq
+
     def log_prob(self, z: torch.Tensor) -> torch.Tensor:
         L = self.scale_tril()
         diff = (z - self.loc).T
@@ -229,9 +240,9 @@ class ModellingSequence(ModellingSequenceInterface):
         gen = torch.Generator().manual_seed(int(seed))
         losses = []
         for _ in range(num_steps):
-            z = q.sample(n_vi, generator=gen).contiguous()
-            lp, _ = self.prob_model.log_prob(sim, z)
-            loss = torch.mean(q.log_prob(z) - lp)
+            z, lq = q.sample_and_log_prob(n_vi, generator=gen)
+            lp, _ = self.prob_model.log_prob(sim, z.contiguous())
+            loss = torch.mean(lq - lp)
             grads = torch.autograd.grad(loss, q.trainable_variables())
             optimizer.apply_gradients(list(zip(grads, q.trainable_variables())))
             losses.append(loss.detach())
