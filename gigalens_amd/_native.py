@@ -36,6 +36,11 @@ EXPORTED_SYMBOLS = (
     "gl_lstsq_simulate",
     "gl_log_prob_lenses",
     "gl_map_adam_step_lenses",
+    "gl_batch_create",
+    "gl_batch_destroy",
+    "gl_batch_size",
+    "gl_batch_log_prob",
+    "gl_batch_map_adam_step",
     "gl_cnn_create",
     "gl_cnn_destroy",
     "gl_cnn_num_params",
@@ -93,6 +98,7 @@ class PlanDesc(ctypes.Structure):
         ("max_batch", ctypes.c_int),
         ("noise_map", ctypes.POINTER(ctypes.c_float)),
         ("n_observed", ctypes.c_int),
+        ("staged", ctypes.c_int),
     ]
 
 
@@ -130,6 +136,13 @@ _SIGS = {
     "gl_log_prob_lenses": ([_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "gl_map_adam_step_lenses": ([_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
+    "gl_batch_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_vp)],
+                        ctypes.c_int),
+    "gl_batch_destroy": ([_vp], ctypes.c_int),
+    "gl_batch_size": ([_vp], ctypes.c_int),
+    "gl_batch_log_prob": ([_vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "gl_batch_map_adam_step": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
     "gl_cnn_create": ([ctypes.POINTER(CnnDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "gl_cnn_destroy": ([_vp], ctypes.c_int),
     "gl_cnn_num_params": ([_vp], ctypes.c_int),

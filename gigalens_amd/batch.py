@@ -1,6 +1,11 @@
 """Fitting many lens systems at once (no counterpart in the reference, which
 builds one simulator and one probabilistic model per lens).
 
+``LensBatch`` takes lens systems with their own image sizes, supersampling,
+PSFs, observed images and particle counts: each lens is an HBM-staged plan and
+every stage of a log_prob / MAP step is one launch over all lenses
+(``gl_batch_*``).
+
 ``MultiLensMAP`` runs the multi-start MAP of ``ModellingSequence.MAP``
 (src/gigalens/tf/inference.py:17-41) for several lens systems that share a
 physical model, camera and prior but have their own observed images: one plan
@@ -13,13 +18,16 @@ from __future__ import annotations
 
 from typing import Optional
 
+import ctypes
+from typing import Sequence
+
 import numpy as np
 import torch
 
 from . import _native as nat
 from . import distributions as dist
 from .inference import Adam
-from .plan import Plan, layout_from_paths
+from .plan import Plan, _ptr, _stream, layout_from_paths
 from .simulator import Camera
 
 
@@ -85,3 +93,78 @@ class MultiLensMAP:
         best_chi = torch.full((chi.shape[0], L), float("inf"), device=self.device).scatter_reduce(
             1, idx, chi, reduce="amin")
         return best_lp.cpu().numpy(), best_chi.cpu().numpy()
+
+
+class LensBatch:
+    """Lens systems with different cameras and particle counts in one launch per
+    stage.  ``lenses`` is a sequence of (sim_config, observed_image, n_particles);
+    all share the physical model, the prior and the noise model
+    (``ForwardProbModel``, tf/model.py:53-86).  Particles are lens-major."""
+
+    def __init__(self, phys_model, prior, lenses: Sequence, background_rms: float, exp_time: float, device="cuda"):
+        self.device = torch.device(device)
+        self.prior = prior
+        self.bij = dist.ChainBijector(prior.experimental_default_event_space_bijector(), dist.PackBijector(prior))
+        paths = [p for p, _ in dist.leaves(prior)]
+        self.d = len(paths)
+        self.plans = []
+        self.counts = []
+        for cfg, obs, n in lenses:
+            self.plans.append(Plan(phys_model, Camera(cfg), layout_from_paths(phys_model, paths), prior=prior,
+                                   observed=np.asarray(obs, np.float32), background_rms=background_rms,
+                                   exp_time=exp_time, likelihood=nat.GL_LIK_FORWARD, staged=True, max_batch=int(n)))
+            self.counts.append(int(n))
+        self.n_lenses = len(self.plans)
+        handles = (ctypes.c_void_p * self.n_lenses)(*[p._h.value for p in self.plans])
+        counts = (ctypes.c_int * self.n_lenses)(*self.counts)
+        h = ctypes.c_void_p()
+        lib = nat.lib()
+        nat.check(lib.gl_batch_create(handles, self.n_lenses, counts, ctypes.byref(h)), "gl_batch_create")
+        self._h = h
+        self._lib = lib
+        self.total = int(lib.gl_batch_size(h))
+        self.lens = torch.arange(self.n_lenses, device=self.device, dtype=torch.int32).repeat_interleave(
+            torch.as_tensor(self.counts, device=self.device))
+        self.lp_history = None
+        self.chi2_history = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.gl_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def log_prob(self, z: torch.Tensor, want_grad: bool = False):
+        z = z.to(self.device, torch.float32).contiguous()
+        assert z.shape == (self.total, self.d)
+        lp = torch.empty(self.total, device=self.device)
+        chi2 = torch.empty(self.total, device=self.device)
+        grad = torch.empty_like(z) if want_grad else None
+        nat.check(self._lib.gl_batch_log_prob(self._h, _ptr(z), _ptr(lp), _ptr(chi2), _ptr(grad), _stream()),
+                  "gl_batch_log_prob")
+        return lp, chi2, grad
+
+    def run(self, z0: torch.Tensor, num_steps: int = 350, optimizer: Optional[Adam] = None,
+            record_every: int = 1) -> torch.Tensor:
+        """Keras-Adam MAP of every lens at once from z0 (total, d); histories as in MultiLensMAP."""
+        optimizer = optimizer if optimizer is not None else Adam(1e-2)
+        z = z0.to(self.device, torch.float32).contiguous().clone()
+        m, v = torch.zeros_like(z), torch.zeros_like(z)
+        nrec = (num_steps - 1) // record_every + 1
+        lp_h = torch.empty((nrec, self.total), device=self.device)
+        chi_h = torch.empty((nrec, self.total), device=self.device)
+        scratch = torch.empty((2, self.total), device=self.device)
+        for s in range(num_steps):
+            rec = s % record_every == 0
+            lp = lp_h[s // record_every] if rec else scratch[0]
+            chi = chi_h[s // record_every] if rec else scratch[1]
+            nat.check(self._lib.gl_batch_map_adam_step(
+                self._h, _ptr(z), _ptr(m), _ptr(v), optimizer.iterations + 1, float(optimizer.lr(optimizer.iterations)),
+                float(optimizer.beta_1), float(optimizer.beta_2), float(optimizer.epsilon), _ptr(lp), _ptr(chi),
+                _stream()), "gl_batch_map_adam_step")
+            optimizer.iterations += 1
+        self.lp_history, self.chi2_history = lp_h, chi_h
+        return z

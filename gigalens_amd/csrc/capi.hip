@@ -89,6 +89,7 @@ struct gl_plan {
     int kf4;                 // conv template parameter of the multi-channel path
     size_t solve_lds;
     int ksplit_cap;          // Gram pixel-range splits allowed by the image size
+    std::vector<float> W_host;  // folded PSF (kf x kf) of staged plans, for batches
     std::vector<void*> dev;  // owned device buffers
 };
 
@@ -181,18 +182,21 @@ extern "C" int gl_plan_destroy(gl_plan* plan) {
 }
 
 // ---------------------------------------------------------------------------
-// Multi-channel (least-squares) plans
+// HBM-staged plans: least squares (linear amplitudes) and fixed amplitudes
+// for images / models the LDS-resident kernel does not take
 // ---------------------------------------------------------------------------
 static int upload_common(const gl_plan_desc* desc, gl_plan* p);
-template <int SS, int KF4>
-static hipError_t launch_mc_conv(int grid, hipStream_t st, const gl::DPlan& P, const gl::McPlan& M, int nb) {
-    hipLaunchKernelGGL((gl::mc_conv<SS, KF4>), dim3(grid), dim3(256), 0, st, P, M, nb);
+
+template <int SS, int KF4, bool MULTI>
+static hipError_t launch_mc_conv(int grid, hipStream_t st, const gl::McTable& T) {
+    hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI>), dim3(grid), dim3(256), 0, st, T);
     return hipGetLastError();
 }
 
-static hipError_t mc_conv_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::DPlan& P,
-                                   const gl::McPlan& M, int nb) {
-#define GL_MCC(S, K) if (ss == S && kf4 == K) return launch_mc_conv<S, K>(grid, st, P, M, nb)
+template <bool MULTI>
+static hipError_t mc_conv_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::McTable& T) {
+#define GL_MCC(S, K) if (ss == S && kf4 == K) return launch_mc_conv<S, K, MULTI>(grid, st, T)
+    GL_MCC(0, 8); GL_MCC(0, 16); GL_MCC(0, 22); GL_MCC(0, 32);
     GL_MCC(1, 8); GL_MCC(1, 16); GL_MCC(1, 22); GL_MCC(1, 32);
     GL_MCC(2, 8); GL_MCC(2, 16); GL_MCC(2, 22); GL_MCC(2, 32);
     GL_MCC(3, 8); GL_MCC(3, 16); GL_MCC(3, 22); GL_MCC(3, 32);
@@ -201,13 +205,13 @@ static hipError_t mc_conv_dispatch(int ss, int kf4, int grid, hipStream_t st, co
     return hipErrorInvalidValue;
 }
 
-static hipError_t mc_gram_dispatch(int nbt, int nb, int ks, hipStream_t st, const gl::DPlan& P, const gl::McPlan& M) {
+static hipError_t mc_gram_dispatch(int nbt, int nb, int ks, hipStream_t st, const gl::McTable& T) {
     switch (nbt) {
-        case 1: hipLaunchKernelGGL((gl::mc_gram<1>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
-        case 2: hipLaunchKernelGGL((gl::mc_gram<2>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
-        case 3: hipLaunchKernelGGL((gl::mc_gram<3>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
-        case 4: hipLaunchKernelGGL((gl::mc_gram<4>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
-        case 5: hipLaunchKernelGGL((gl::mc_gram<5>), dim3(nb, ks), dim3(256), 0, st, P, M, ks); break;
+        case 1: hipLaunchKernelGGL((gl::mc_gram<1, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
+        case 2: hipLaunchKernelGGL((gl::mc_gram<2, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
+        case 3: hipLaunchKernelGGL((gl::mc_gram<3, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
+        case 4: hipLaunchKernelGGL((gl::mc_gram<4, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
+        case 5: hipLaunchKernelGGL((gl::mc_gram<5, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -238,22 +242,79 @@ static void hermite_table(int n_max, std::vector<float>& tab) {
     }
 }
 
-static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<float>& W, int max_niter) {
+static int dev_alloc(std::vector<void*>& owned, size_t bytes, void** out, bool zero) {
+    void* d = nullptr;
+    GL_HIP(hipMalloc(&d, bytes ? bytes : 4));
+    owned.push_back(d);
+    if (zero) GL_HIP(hipMemset(d, 0, bytes ? bytes : 4));
+    *out = d;
+    return GL_OK;
+}
+
+// Workspace of a staged lens for `mb` particles, with the conv launched at
+// template width kf4 (>= the lens's own): staged row width BW and the zero-padded
+// weight rows follow it.  W: the lens's folded weights (kf x kf, host).
+static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int ksplit_cap,
+                        const std::vector<float>& W, std::vector<void*>& owned) {
+    const int N = P.N, ss = P.ss, nsub = P.nsub, kf = P.kf, pb = P.pb, D = M.D;
+    int rc;
+    void* dptr = nullptr;
+    M.BW = ((ss * (N - 1) + 4 * kf4 > pb + nsub ? ss * (N - 1) + 4 * kf4 : pb + nsub) + 3) & ~3;
+    M.wpad_w = 4 * kf4;
+    std::vector<float> wp((size_t)kf * M.wpad_w, 0.0f);
+    for (int a = 0; a < kf; ++a)
+        for (int b = 0; b < kf; ++b) wp[(size_t)a * M.wpad_w + b] = W[a * kf + b];
+    if ((rc = dev_alloc(owned, wp.size() * sizeof(float), &dptr, false))) return rc;
+    GL_HIP(hipMemcpy(dptr, wp.data(), wp.size() * sizeof(float), hipMemcpyHostToDevice));
+    M.wpad = (const float*)dptr;
+    const int GW = N + 2 * M.gp;
+    const int gram_nt = M.gram_nb * (M.gram_nb + 1) / 2;
+    if ((rc = dev_alloc(owned, (size_t)mb * M.pstride * 4, &dptr, true))) return rc;
+    M.consts = (float*)dptr;
+    if ((rc = dev_alloc(owned, (size_t)mb * D * nsub * M.BW * 4, &dptr, true))) return rc;  // zero margins stay zero
+    M.B = (float*)dptr;
+    if ((rc = dev_alloc(owned, (size_t)mb * D * N * N * 4, &dptr, false))) return rc;
+    M.X = (float*)dptr;
+    M.gpart = nullptr;
+    if (M.lstsq) {
+        if ((rc = dev_alloc(owned, (size_t)std::min(mb * ksplit_cap, 1024 + mb) * 4 * gram_nt * 256 * 8, &dptr,
+                            false)))
+            return rc;
+        M.gpart = (double*)dptr;
+    }
+    if ((rc = dev_alloc(owned, (size_t)mb * D * 4, &dptr, false))) return rc;
+    M.coef = (float*)dptr;
+    if ((rc = dev_alloc(owned, (size_t)mb * GW * GW * 4, &dptr, true))) return rc;  // zero margins stay zero
+    M.gimg = (float*)dptr;
+    if ((rc = dev_alloc(owned, (size_t)mb * nsub * nsub * 4, &dptr, false))) return rc;
+    M.gsub = (float*)dptr;
+    if ((rc = dev_alloc(owned, (size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
+    M.stats = (float*)dptr;
+    return GL_OK;
+}
+
+static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<float>& W, int max_niter, int lstsq) {
     gl::DPlan& P = p->P;
     gl::McPlan& M = p->M;
     memset(&M, 0, sizeof(M));
+    M.lstsq = lstsq;
     const int N = P.N, ss = P.ss, nsub = P.nsub, kf = P.kf, pb = P.pb;
     int rc;
     void* dptr = nullptr;
-    if (ss > 4) return fail(GL_ERR_UNSUPPORTED, "least-squares plans support supersample <= 4");
+    if (ss > 4) return fail(GL_ERR_UNSUPPORTED, "the staged path supports supersample <= 4");
     p->kf4 = pick_kf4(kf);
-    if (p->kf4 < 0) return fail(GL_ERR_UNSUPPORTED, "PSF too large for the least-squares path (kf > 128)");
+    if (p->kf4 < 0) return fail(GL_ERR_UNSUPPORTED, "PSF too large for the staged path (kf > 128)");
     if (P.ncomp > gl::kMaxComp) return fail(GL_ERR_UNSUPPORTED, "too many components");
     // channels
     int D = 0, nshp = 0;
     M.shp_comp = -1;
     for (int i = P.nlens; i < P.ncomp; ++i) {
         const gl::DComp& c = P.comp[i];
+        if (!lstsq) {
+            if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE)
+                return fail(GL_ERR_UNSUPPORTED, "fixed-amplitude staged plans take Sersic lights");
+            continue;
+        }
         if (c.kind == glm::K_SHAPELETS) {
             if (++nshp > 1) return fail(GL_ERR_UNSUPPORTED, "at most one shapelet component");
             if (c.n_max < 0 || c.n_max > glm::kMaxShapeletN)
@@ -267,6 +328,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
             D += 1;
         }
     }
+    if (!lstsq) D = 1;
     if (D < 1 || D > gl::kMcMaxChan) return fail(GL_ERR_UNSUPPORTED, "number of linear channels out of range");
     M.D = D;
     // per-particle constants: prior arrays, component blocks, EPL coefficient tables
@@ -280,8 +342,6 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
         if (i < P.nlens && P.comp[i].kind == glm::K_EPL) coff += (coef_block + 3) & ~3;
     }
     M.pstride = (M.coef_base + coff + 63) & ~63;
-    // staged row width: lanes read SS*J + q for q < 4*KF4, J < N
-    M.BW = ((ss * (N - 1) + 4 * p->kf4 > pb + nsub ? ss * (N - 1) + 4 * p->kf4 : pb + nsub) + 3) & ~3;
     // padded dL/dimage
     int gp = 0;
     for (int a = 0; a < ss; ++a) {
@@ -292,59 +352,24 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     M.gp = gp;
     const int GW = N + 2 * gp;
     // Gram tiles and pixel-range splits
-    M.gram_nb = (D + 1 + 15) / 16;
+    M.gram_nb = lstsq ? (D + 1 + 15) / 16 : 0;
     const int gram_nt = M.gram_nb * (M.gram_nb + 1) / 2;
     const int nchunk = (N * N + 63) / 64;
     p->ksplit_cap = std::max(1, std::min(gl::kMcGramMaxSplit, nchunk / 4));
     // particles per chunk: the caller's, else as many as fit a 32 GiB workspace (at most 1024)
-    const double per_particle = 4.0 * ((double)M.pstride + (double)D * nsub * M.BW + (double)D * N * N + D +
+    const int BW = ((ss * (N - 1) + 4 * p->kf4 > pb + nsub ? ss * (N - 1) + 4 * p->kf4 : pb + nsub) + 3) & ~3;
+    const double per_particle = 4.0 * ((double)M.pstride + (double)D * nsub * BW + (double)D * N * N + D +
                                        (double)GW * GW + (double)nsub * nsub + gl::kMcStat) +
                                 8.0 * 4 * p->ksplit_cap * gram_nt * 256;
     int mb = desc->max_batch;
     if (mb <= 0) mb = (int)std::max(1.0, std::min(1024.0, 32.0 * 1073741824.0 / per_particle));
     p->max_batch = mb;
     M.rcond = 1e-6f;
-    // folded weights padded to the conv's row width
-    M.wpad_w = 4 * p->kf4;
-    std::vector<float> wp((size_t)kf * M.wpad_w, 0.0f);
-    for (int a = 0; a < kf; ++a)
-        for (int b = 0; b < kf; ++b) wp[(size_t)a * M.wpad_w + b] = W[a * kf + b];
-    if ((rc = upload(p, wp.data(), wp.size() * sizeof(float), &dptr))) return rc;
-    M.wpad = (const float*)dptr;
     if ((rc = upload(p, W.data(), W.size() * sizeof(float), &dptr))) return rc;
     P.w = (const float*)dptr;
+    p->W_host = W;
     if ((rc = upload_common(desc, p))) return rc;
-    // shapelet tables
-    if (M.shp_comp >= 0) {
-        std::vector<float> tab;
-        hermite_table(M.shp_nmax, tab);
-        if ((rc = upload(p, tab.data(), tab.size() * sizeof(float), &dptr))) return rc;
-        M.shp_table = (const float*)dptr;
-        std::vector<float> pref(M.shp_nmax + 2);
-        for (int n = 0; n <= M.shp_nmax + 1; ++n)
-            pref[n] = (float)(1.0 / sqrt(pow(2.0, n) * sqrt(M_PI) * tgamma(n + 1.0)));
-        if ((rc = upload(p, pref.data(), pref.size() * sizeof(float), &dptr))) return rc;
-        M.shp_pref = (const float*)dptr;
-    }
-    // workspace for max_batch particles (allocated here once; calls never allocate)
-    auto alloc = [&](size_t bytes, void** outp, bool zero) -> int {
-        void* d = nullptr;
-        GL_HIP(hipMalloc(&d, bytes ? bytes : 4));
-        p->dev.push_back(d);
-        if (zero) GL_HIP(hipMemset(d, 0, bytes));
-        *outp = d;
-        return GL_OK;
-    };
-    if ((rc = alloc((size_t)mb * M.pstride * 4, &dptr, true))) return rc;
-    M.consts = (float*)dptr;
-    if ((rc = alloc((size_t)mb * D * nsub * M.BW * 4, &dptr, true))) return rc;  // zero margins stay zero
-    M.B = (float*)dptr;
-    if ((rc = alloc((size_t)mb * D * N * N * 4, &dptr, false))) return rc;
-    M.X = (float*)dptr;
-    if ((rc = alloc((size_t)std::min(mb * p->ksplit_cap, 1024 + mb) * 4 * gram_nt * 256 * 8, &dptr, false)))
-        return rc;
-    M.gpart = (double*)dptr;
-    {
+    if (lstsq) {
         // W = 1 / err and obs / err (tf/simulator.py:157-158), err from the noise map or tf/model.py:113-115
         std::vector<float> sw(N * N, 0.0f), swo(N * N, 0.0f);
         for (int i = 0; i < N * N; ++i) {
@@ -360,19 +385,26 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
         if ((rc = upload(p, swo.data(), swo.size() * sizeof(float), &dptr))) return rc;
         M.swobs = (const float*)dptr;
     }
-    if ((rc = alloc((size_t)mb * D * 4, &dptr, false))) return rc;
-    M.coef = (float*)dptr;
-    if ((rc = alloc((size_t)mb * GW * GW * 4, &dptr, true))) return rc;  // zero margins stay zero
-    M.gimg = (float*)dptr;
-    if ((rc = alloc((size_t)mb * nsub * nsub * 4, &dptr, false))) return rc;
-    M.gsub = (float*)dptr;
-    if ((rc = alloc((size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
-    M.stats = (float*)dptr;
+    // shapelet tables
+    if (M.shp_comp >= 0) {
+        std::vector<float> tab;
+        hermite_table(M.shp_nmax, tab);
+        if ((rc = upload(p, tab.data(), tab.size() * sizeof(float), &dptr))) return rc;
+        M.shp_table = (const float*)dptr;
+        std::vector<float> pref(M.shp_nmax + 2);
+        for (int n = 0; n <= M.shp_nmax + 1; ++n)
+            pref[n] = (float)(1.0 / sqrt(pow(2.0, n) * sqrt(M_PI) * tgamma(n + 1.0)));
+        if ((rc = upload(p, pref.data(), pref.size() * sizeof(float), &dptr))) return rc;
+        M.shp_pref = (const float*)dptr;
+    }
+    // workspace for max_batch particles (allocated here once; calls never allocate)
+    if ((rc = mc_workspace(P, M, mb, p->kf4, p->ksplit_cap, W, p->dev))) return rc;
     const int n2 = D + (D & 1);
     // A, V (n2 x n2), rotations (n2), pairs (n2 ints), 2 reduction slots, rhs (n2)
     p->solve_lds = (size_t)(2 * n2 * n2 + n2) * 8 + (size_t)n2 * 4 + 16 + (size_t)n2 * 8 + 64;
-    GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_solve), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)p->solve_lds));
+    if (lstsq)
+        GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_solve<false>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->solve_lds));
     p->epl_batch = 0;
     p->lds_bytes = 0;
     P.sep_rank = 0;
@@ -380,50 +412,89 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     return GL_OK;
 }
 
-// Run the multi-channel pipeline over bs particles in chunks of max_batch.
-static int run_mc(const gl_plan* plan, const float* in, int use_z, int bs, float* lp, float* chi2, float* img,
-                  float* coef, float* stacked, const float* g_lp, const float* g_chi2, float* grad,
-                  hipStream_t st) {
+// Work-unit totals of a lens with `count` particles (the prefix increments of a table).
+struct McUnits {
+    long part, basis, conv, adj;
+};
+static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
+    McUnits u;
+    const long njb = (P.N + 63) / 64;
+    u.part = count;
+    u.basis = (long)count * ((P.nsub * P.nsub + 255) / 256);
+    u.conv = (long)count * M.D * ((P.N + gl::kMcIB - 1) / gl::kMcIB) * njb;
+    u.adj = (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
+    return u;
+}
+
+// One pass of the staged pipeline over the lenses of table T.
+template <bool MULTI>
+static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch, int kf4, int lstsq, int gram_nb,
+                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st) {
+    const int np = T.tot_part;
+    hipLaunchKernelGGL((gl::mc_prologue<MULTI>), dim3(np), dim3(128), 0, st, T, A);
+    GL_HIP(hipGetLastError());
+    hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), 0, st, T);
+    GL_HIP(hipGetLastError());
+    GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, (T.tot_conv + 3) / 4, st, T));
+    if (lstsq) {
+        if (MULTI) return fail(GL_ERR_UNSUPPORTED, "least-squares lenses are not batched");
+        const int ks = std::max(1, std::min(ksplit_cap, (1024 + np - 1) / np));
+        GL_HIP(mc_gram_dispatch(gram_nb, np, ks, st, T));
+        hipLaunchKernelGGL((gl::mc_solve<MULTI>), dim3(np), dim3(256), solve_lds, st, T, T.one.M.rcond, 4 * ks);
+        GL_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((gl::mc_image<MULTI>), dim3(np), dim3(256), 0, st, T, A, grad ? 1 : 0);
+    GL_HIP(hipGetLastError());
+    if (grad) {
+        hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), 0, st, T);
+        GL_HIP(hipGetLastError());
+        hipLaunchKernelGGL((gl::mc_grad<MULTI>), dim3(T.tot_basis), dim3(256), 0, st, T);
+        GL_HIP(hipGetLastError());
+        hipLaunchKernelGGL((gl::mc_finish<MULTI>), dim3(np), dim3(128), 0, st, T, A);
+        GL_HIP(hipGetLastError());
+    }
+    return GL_OK;
+}
+
+// Run a staged plan over bs particles in chunks of max_batch.  A carries the
+// full-batch pointers; map: MAP step with loss weight 1/bs per particle.
+static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStream_t st) {
     const gl::DPlan& P = plan->P;
     const gl::McPlan& M = plan->M;
-    const int N = P.N, nsub = P.nsub, D = M.D, d = P.d;
-    const int nib = (N + gl::kMcIB - 1) / gl::kMcIB, njb = (N + 63) / 64;
+    const int N = P.N, D = M.D, d = P.d, bs = A0.bs;
+    if (bs <= 0 || !A0.in) return fail(GL_ERR_ARG, "bad batch");
     for (int off = 0; off < bs; off += plan->max_batch) {
         const int nb = bs - off < plan->max_batch ? bs - off : plan->max_batch;
-        gl::McArgs A;
-        memset(&A, 0, sizeof(A));
-        A.in = in + (size_t)off * d;
-        A.use_z = use_z;
+        gl::McArgs A = A0;
+        A.in = A0.in + (size_t)off * d;
         A.bs = nb;
-        A.lp = lp ? lp + off : nullptr;
-        A.chi2 = chi2 ? chi2 + off : nullptr;
-        A.img = img ? img + (size_t)off * N * N : nullptr;
-        A.coef_out = coef ? coef + (size_t)off * D : nullptr;
-        A.stacked = stacked ? stacked + (size_t)off * N * N * D : nullptr;
-        A.g_lp = g_lp ? g_lp + off : nullptr;
-        A.g_chi2 = g_chi2 ? g_chi2 + off : nullptr;
-        A.grad = grad ? grad + (size_t)off * d : nullptr;
-        hipLaunchKernelGGL(gl::mc_prologue, dim3(nb), dim3(128), 0, st, P, M, A);
-        GL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gl::mc_basis, dim3((nsub * nsub + 255) / 256, nb), dim3(256), 0, st, P, M);
-        GL_HIP(hipGetLastError());
-        const long waves = (long)nb * D * nib * njb;
-        GL_HIP(mc_conv_dispatch(P.ss, plan->kf4, (int)((waves + 3) / 4), st, P, M, nb));
-        const int ks = std::max(1, std::min(plan->ksplit_cap, (1024 + nb - 1) / nb));
-        GL_HIP(mc_gram_dispatch(M.gram_nb, nb, ks, st, P, M));
-        hipLaunchKernelGGL(gl::mc_solve, dim3(nb), dim3(256), plan->solve_lds, st, M, M.rcond, 4 * ks);
-        GL_HIP(hipGetLastError());
-        const int want_grad = grad != nullptr;
-        hipLaunchKernelGGL(gl::mc_image, dim3(nb), dim3(256), 0, st, P, M, A, want_grad);
-        GL_HIP(hipGetLastError());
-        if (want_grad) {
-            hipLaunchKernelGGL(gl::mc_adjoint, dim3(nb * nsub * P.ss, (N + 63) / 64), dim3(64), 0, st, P, M);
-            GL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(gl::mc_grad, dim3((nsub * nsub + 255) / 256, nb), dim3(256), 0, st, P, M);
-            GL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(gl::mc_finish, dim3(nb), dim3(128), 0, st, P, M, A);
-            GL_HIP(hipGetLastError());
-        }
+        A.lp = A0.lp ? A0.lp + off : nullptr;
+        A.chi2 = A0.chi2 ? A0.chi2 + off : nullptr;
+        A.img = A0.img ? A0.img + (size_t)off * N * N : nullptr;
+        A.coef_out = A0.coef_out ? A0.coef_out + (size_t)off * D : nullptr;
+        A.stacked = A0.stacked ? A0.stacked + (size_t)off * N * N * D : nullptr;
+        A.g_lp = A0.g_lp ? A0.g_lp + off : nullptr;
+        A.g_chi2 = A0.g_chi2 ? A0.g_chi2 + off : nullptr;
+        A.g_img = A0.g_img ? A0.g_img + (size_t)off * N * N : nullptr;
+        A.grad = A0.grad ? A0.grad + (size_t)off * d : nullptr;
+        A.z_io = A0.z_io ? A0.z_io + (size_t)off * d : nullptr;
+        A.m = A0.m ? A0.m + (size_t)off * d : nullptr;
+        A.v = A0.v ? A0.v + (size_t)off * d : nullptr;
+        gl::McTable T;
+        memset(&T, 0, sizeof(T));
+        T.one.P = P;
+        T.one.M = M;
+        T.one.first = 0;
+        T.one.count = nb;
+        T.one.weight = 1.0f / (float)bs;
+        T.n = 1;
+        const McUnits u = mc_units(P, M, nb);
+        if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX)
+            return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
+        T.tot_part = nb; T.tot_basis = (int)u.basis; T.tot_conv = (int)u.conv; T.tot_adj = (int)u.adj;
+        int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
+                                    grad, st);
+        if (rc) return rc;
     }
     return GL_OK;
 }
@@ -560,16 +631,15 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     }
     P.nlens = nl; P.nll = nll; P.nsrc = ns;
     p->model = nullptr;
-    if (!p->mc) {
+    const int lstsq = p->mc;
+    bool staged = p->mc || desc->staged;
+    if (!staged) {
         for (const ModelEntry& e : kModels) {
             bool ok = true;
             for (int j = 0; j < 7; ++j) ok = ok && e.sig[j] == sig[j];
             if (ok) { p->model = &e; break; }
         }
-        if (!p->model) {
-            gl_plan_destroy(p);
-            return fail(GL_ERR_UNSUPPORTED, "no compiled kernel for this combination of profiles");
-        }
+        staged = !p->model;  // no fused kernel for this combination: the staged pipeline
     }
 
     // ---- PSF folded with the pooling box (correlation weights) -----------------
@@ -595,16 +665,21 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             W[pp * kf + qq] = (float)(acc / (ss * ss));
         }
 
-    if (p->mc && desc->n_observed > 1) {
-        gl_plan_destroy(p);
-        return fail(GL_ERR_UNSUPPORTED, "least-squares plans take one observed image");
-    }
-    if (p->mc) {
-        int rc = mc_create(desc, p, W, max_niter);
+    // HBM-staged pipeline: least squares, forced, no fused kernel, or (below) an
+    // image the LDS-resident kernel cannot hold
+    auto create_staged = [&]() -> int {
+        if (desc->n_observed > 1) {
+            gl_plan_destroy(p);
+            return fail(GL_ERR_UNSUPPORTED, "staged plans take one observed image (use gl_batch_* for many lenses)");
+        }
+        p->mc = 1;
+        p->model = nullptr;
+        int rc = mc_create(desc, p, W, max_niter, lstsq);
         if (rc) { gl_plan_destroy(p); return rc; }
         *out = p;
         return GL_OK;
-    }
+    };
+    if (staged) return create_staged();
 
     // ---- LDS layout --------------------------------------------------------
     const int nsub = P.nsub;
@@ -673,22 +748,11 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             ok = ok && rlo >= 0 && clo >= 0 && rhi < grows && chi < GW;
         }
         ok = ok && (long)(N - 1 + gp0) * GW + (N - 1 + gp0) < P.g_size && P.g_size <= P.off_w;
-        if (!ok) {
-            gl_plan_destroy(p);
-            return fail(GL_ERR_UNSUPPORTED, "internal LDS layout check failed for this image size / PSF");
-        }
+        if (!ok) return create_staged();
     }
     P.lds_floats = off;
     p->lds_bytes = (size_t)off * sizeof(float);
-    if (p->lds_bytes > 160 * 1024) {
-        gl_plan_destroy(p);
-        return fail(GL_ERR_UNSUPPORTED, "image too large for the LDS-resident kernel (" +
-                                            std::to_string(p->lds_bytes) + " bytes of LDS)");
-    }
-    if (N * nstr > gl::kMaxT * kBlock) {
-        gl_plan_destroy(p);
-        return fail(GL_ERR_UNSUPPORTED, "image too large for the LDS-resident kernel (output strips)");
-    }
+    if (p->lds_bytes > 160 * 1024 || N * nstr > gl::kMaxT * kBlock) return create_staged();
 
     // ---- separable (low-rank) factorisation of the folded PSF ----------------
     std::vector<float> sa, sb;
@@ -768,10 +832,38 @@ extern "C" int gl_plan_query(const gl_plan* plan, gl_plan_info* info) {
 static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream) {
     if (!plan) return fail(GL_ERR_ARG, "null plan");
     if (plan->mc) {
-        if (A.mode != gl::M_LOGPROB)
-            return fail(GL_ERR_UNSUPPORTED, "least-squares plans support gl_log_prob and gl_lstsq_simulate");
-        if (A.bs <= 0 || !A.in) return fail(GL_ERR_ARG, "bad batch");
-        return run_mc(plan, A.in, 1, A.bs, A.lp, A.chi2, nullptr, nullptr, nullptr, A.g_lp, A.g_chi2, A.grad, stream);
+        if (A.lens) return fail(GL_ERR_UNSUPPORTED, "staged plans take one observed image (use gl_batch_*)");
+        gl::McArgs B;
+        memset(&B, 0, sizeof(B));
+        B.in = A.in;
+        B.bs = A.bs;
+        B.use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
+        if (A.mode == gl::M_SIM) {
+            if (plan->M.lstsq) return fail(GL_ERR_UNSUPPORTED, "least-squares plans: use gl_lstsq_simulate");
+            B.img = A.img;
+        } else if (A.mode == gl::M_VJP) {
+            if (plan->M.lstsq) return fail(GL_ERR_UNSUPPORTED, "least-squares plans have no simulate VJP");
+            B.g_img = A.g_img;
+            B.grad = A.grad;
+        } else {
+            B.lp = A.lp;
+            B.chi2 = A.chi2;
+            B.g_lp = A.g_lp;
+            B.g_chi2 = A.g_chi2;
+            B.grad = A.grad;
+            if (A.mode == gl::M_MAP) {
+                if (A.weight) return fail(GL_ERR_UNSUPPORTED, "staged plans: per-particle MAP weights need gl_batch_*");
+                B.map = 1;
+                B.z_io = A.z_io;
+                B.m = A.m;
+                B.v = A.v;
+                B.lr_t = A.lr_t;
+                B.beta1 = A.beta1;
+                B.beta2 = A.beta2;
+                B.eps = A.eps;
+            }
+        }
+        return run_mc(plan, B, grad, stream);
     }
     if (A.bs <= 0) return fail(GL_ERR_ARG, "bs must be positive");
     if (!A.in) return fail(GL_ERR_ARG, "null parameter pointer");
@@ -842,8 +934,15 @@ extern "C" int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, fl
                                  float* stacked, hipStream_t stream) {
     if (!plan) return fail(GL_ERR_ARG, "null plan");
     if (!plan->mc) return fail(GL_ERR_ARG, "gl_lstsq_simulate needs a plan with linear (use_lstsq) light components");
-    if (bs <= 0 || !x) return fail(GL_ERR_ARG, "bad batch");
-    return run_mc(plan, x, 0, bs, nullptr, nullptr, img, coeffs, stacked, nullptr, nullptr, nullptr, stream);
+    if (!plan->M.lstsq) return fail(GL_ERR_ARG, "gl_lstsq_simulate needs a plan with linear (use_lstsq) light components");
+    gl::McArgs B;
+    memset(&B, 0, sizeof(B));
+    B.in = x;
+    B.bs = bs;
+    B.img = img;
+    B.coef_out = coeffs;
+    B.stacked = stacked;
+    return run_mc(plan, B, false, stream);
 }
 
 extern "C" int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, float* v, const int* lens,
@@ -874,6 +973,137 @@ extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* 
                                 float beta1, float beta2, float epsilon, float* lp, float* chi2, hipStream_t stream) {
     return gl_map_adam_step_lenses(plan, z, m, v, nullptr, nullptr, bs, step, lr, beta1, beta2, epsilon, lp, chi2,
                                    stream);
+}
+
+// ---------------------------------------------------------------------------
+// Batches of lens systems with their own image sizes, PSFs and particle counts
+// (no reference counterpart): every stage of the staged pipeline is one launch
+// over all lenses, addressed through a device lens table.
+// ---------------------------------------------------------------------------
+struct gl_batch {
+    int n, d, total, ss_launch, kf4;
+    gl::McTable T;
+    std::vector<int> counts;
+    std::vector<void*> dev;
+};
+
+extern "C" int gl_batch_destroy(gl_batch* b) {
+    if (!b) return GL_OK;
+    for (void* p : b->dev) (void)hipFree(p);
+    delete b;
+    return GL_OK;
+}
+
+extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* counts, gl_batch** out) {
+    if (!plans || !counts || !out || n <= 0) return fail(GL_ERR_ARG, "gl_batch_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        const gl_plan* q = plans[i];
+        if (!q || !q->mc || q->M.lstsq)
+            return fail(GL_ERR_ARG, "gl_batch_create: every plan must be a fixed-amplitude staged plan (desc.staged = 1)");
+        if (counts[i] <= 0) return fail(GL_ERR_ARG, "gl_batch_create: particle counts must be positive");
+        if (q->P.d != plans[0]->P.d || q->P.lik != plans[0]->P.lik)
+            return fail(GL_ERR_ARG, "gl_batch_create: plans must share the parameter count and likelihood");
+    }
+    gl_batch* b = new gl_batch();
+    b->n = n;
+    b->d = plans[0]->P.d;
+    b->kf4 = 0;
+    b->ss_launch = plans[0]->P.ss;
+    for (int i = 0; i < n; ++i) {
+        b->kf4 = std::max(b->kf4, plans[i]->kf4);
+        if (plans[i]->P.ss != b->ss_launch) b->ss_launch = 0;  // mixed supersampling: runtime ss
+    }
+    std::vector<gl::McLens> lens(n);
+    std::vector<int> pre_part(n + 1, 0), pre_basis(n + 1, 0), pre_conv(n + 1, 0), pre_adj(n + 1, 0);
+    long acc[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        gl::McLens& E = lens[i];
+        memset(&E, 0, sizeof(E));
+        E.P = plans[i]->P;
+        E.M = plans[i]->M;
+        int rc = mc_workspace(E.P, E.M, counts[i], b->kf4, 1, plans[i]->W_host, b->dev);
+        if (rc) { gl_batch_destroy(b); return rc; }
+        E.first = (int)acc[0];
+        E.count = counts[i];
+        E.weight = 1.0f / (float)counts[i];
+        const McUnits u = mc_units(E.P, E.M, counts[i]);
+        acc[0] += u.part; acc[1] += u.basis; acc[2] += u.conv; acc[3] += u.adj;
+        for (int k = 0; k < 4; ++k)
+            if (acc[k] > INT32_MAX) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: batch too large"); }
+        pre_part[i + 1] = (int)acc[0];
+        pre_basis[i + 1] = (int)acc[1];
+        pre_conv[i + 1] = (int)acc[2];
+        pre_adj[i + 1] = (int)acc[3];
+    }
+    b->total = (int)acc[0];
+    b->counts.assign(counts, counts + n);
+    memset(&b->T, 0, sizeof(b->T));
+    void* dptr = nullptr;
+    int rc;
+    auto put = [&](const void* host, size_t bytes) -> void* {
+        if ((rc = dev_alloc(b->dev, bytes, &dptr, false))) return nullptr;
+        if (hipMemcpy(dptr, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(GL_ERR_HIP, "gl_batch_create: upload");
+            return nullptr;
+        }
+        return dptr;
+    };
+    rc = GL_OK;
+    b->T.lens = (const gl::McLens*)put(lens.data(), lens.size() * sizeof(gl::McLens));
+    if (!rc) b->T.pre_part = (const int*)put(pre_part.data(), pre_part.size() * sizeof(int));
+    if (!rc) b->T.pre_basis = (const int*)put(pre_basis.data(), pre_basis.size() * sizeof(int));
+    if (!rc) b->T.pre_conv = (const int*)put(pre_conv.data(), pre_conv.size() * sizeof(int));
+    if (!rc) b->T.pre_adj = (const int*)put(pre_adj.data(), pre_adj.size() * sizeof(int));
+    if (rc) { gl_batch_destroy(b); return rc; }
+    b->T.n = n;
+    b->T.tot_part = (int)acc[0];
+    b->T.tot_basis = (int)acc[1];
+    b->T.tot_conv = (int)acc[2];
+    b->T.tot_adj = (int)acc[3];
+    if (hipDeviceSynchronize() != hipSuccess) { gl_batch_destroy(b); return fail(GL_ERR_HIP, "gl_batch_create: sync"); }
+    *out = b;
+    return GL_OK;
+}
+
+extern "C" int gl_batch_size(const gl_batch* b) { return b ? b->total : -1; }
+
+extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, float* chi2, float* grad_z,
+                                 hipStream_t stream) {
+    if (!b || !z) return fail(GL_ERR_ARG, "gl_batch_log_prob: null argument");
+    gl::McArgs A;
+    memset(&A, 0, sizeof(A));
+    A.in = z;
+    A.use_z = 1;
+    A.bs = b->total;
+    A.lp = lp;
+    A.chi2 = chi2;
+    A.grad = grad_z;
+    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream);
+}
+
+extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, float* v, int step, float lr,
+                                      float beta1, float beta2, float epsilon, float* lp, float* chi2,
+                                      hipStream_t stream) {
+    if (!b || !z || !m || !v) return fail(GL_ERR_ARG, "gl_batch_map_adam_step: null argument");
+    if (step < 1) return fail(GL_ERR_ARG, "step is 1-based");
+    gl::McArgs A;
+    memset(&A, 0, sizeof(A));
+    A.in = z;
+    A.use_z = 1;
+    A.bs = b->total;
+    A.lp = lp;
+    A.chi2 = chi2;
+    A.map = 1;
+    A.z_io = z;
+    A.m = m;
+    A.v = v;
+    const double b1t = pow((double)beta1, step), b2t = pow((double)beta2, step);
+    A.lr_t = (float)((double)lr * sqrt(1.0 - b2t) / (1.0 - b1t));
+    A.beta1 = beta1;
+    A.beta2 = beta2;
+    A.eps = epsilon;
+    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,30 +1,42 @@
-// Multi-channel (linear-amplitude) lens-model path for gfx950:
-// LensSimulator.lstsq_simulate (src/gigalens/tf/simulator.py:99-165) and
-// BackwardProbModel.log_prob (src/gigalens/tf/model.py:134-163).
+// HBM-staged lens-model pipeline for gfx950.  It serves three callers:
+//   * LensSimulator.lstsq_simulate (src/gigalens/tf/simulator.py:99-165) and
+//     BackwardProbModel.log_prob (src/gigalens/tf/model.py:134-163): linear
+//     light amplitudes solved per particle by least squares;
+//   * LensSimulator.simulate / ForwardProbModel.log_prob (tf/simulator.py:65-97,
+//     tf/model.py:53-86) for images too large for the LDS-resident particle
+//     kernel or profile combinations it is not compiled for: fixed amplitudes
+//     (the light parameters include Ie), one channel;
+//   * batches of lens systems with different image sizes, PSFs and particle
+//     counts: every stage is one launch over all lenses.
 //
-// Pipeline, one launch each, stream-ordered:
+// Work is addressed through a lens table (McTable): per lens a DPlan, a
+// McPlan (its own workspace) and prefix sums of each stage's work units over
+// the lenses; a unit finds its lens by binary search and its particle inside
+// the lens by division.  A single plan is a table of one lens passed by value.
+//
+// Stages, one launch each, stream-ordered:
 //   mc_prologue   per particle: z -> x (bijectors), prior terms, component
-//                 constants, EPL coefficient tables  -> workspace
-//   mc_basis      per sub-pixel: unit-amplitude channels (lens light on theta,
-//                 sources -- Sersic or the shapelet basis -- on beta), NaN -> 0,
-//                 staged in HBM with zero column margins
+//                 constants, EPL coefficient tables -> workspace
+//   mc_basis      per sub-pixel: the unit-amplitude channels (lens light on
+//                 theta, sources -- Sersic or the shapelet basis -- on beta) or,
+//                 with fixed amplitudes, their sum; NaN -> 0; staged in HBM
+//                 with zero column margins
 //   mc_conv       per (particle, channel, 4 output rows, 64 output columns):
 //                 stride-ss correlation with the PSF folded with the pooling box
 //                 (depthwise conv2d SAME + avg_pool2d); one lane per output
 //                 column streams the contributing sub-rows through L1
-//   mc_gram       per particle and pixel range: [X | obs]^T W^2 [X | obs] on
-//                 FP32 MFMA (16x16x4) tiles, flushed into FP64 every 128
-//                 pixels; one FP64 partial per wave
-//   mc_solve      per particle: FP64 parallel cyclic Jacobi eigensolver, then
-//                 the pseudo-inverse with the reference's cutoff
-//                 (singular values <= 1e-6 * max dropped, tf/simulator.py:161)
-//   mc_image      per particle: image = sum_c coef_c X_c, likelihood with the
-//                 observed-image noise map, chi^2, prior -> lp; dL/dimage
-//   mc_adjoint    per sub-pixel: transposed correlation of dL/dimage
-//   mc_grad       per sub-pixel: recompute with the solved amplitudes and
-//                 accumulate reverse-mode statistics (the amplitudes are held
-//                 at their least-squares optimum: envelope theorem)
-//   mc_finish     per particle: chain rule -> dlp/dz
+//   mc_gram       least squares: per particle and pixel range [X | obs]^T W^2 [X | obs]
+//                 on FP32 MFMA (16x16x4) tiles, flushed into FP64 every 128 pixels
+//   mc_solve      least squares: per particle FP64 parallel cyclic Jacobi, then the
+//                 pseudo-inverse with the reference's cutoff (1e-6, tf/simulator.py:161)
+//   mc_image      per particle: image = sum_c coef_c X_c, likelihood (forward: sigma
+//                 from the model image; backward: from the observed image), chi^2,
+//                 prior -> lp; dL/dimage
+//   mc_adjoint    per sub-pixel row segment: transposed correlation of dL/dimage
+//   mc_grad       per sub-pixel: recompute and accumulate reverse-mode statistics
+//                 (least squares: amplitudes held at their optimum, envelope theorem)
+//   mc_finish     per particle: chain rule -> dlp/dz, optionally the Keras-Adam
+//                 update of a MAP step (tf/inference.py:28-35)
 #include <hip/hip_runtime.h>
 #include "lens_math.h"
 #include "lens_plan.h"
@@ -40,6 +52,7 @@ constexpr int kMcStat = 64;  // statistics floats per particle
 constexpr int kMcIB = 4;     // output rows per conv lane
 
 struct McPlan {
+    int lstsq;          // 1: linear amplitudes solved by least squares; 0: fixed amplitudes (one channel)
     int D;              // channels
     int BW;             // staged row width (floats): pb + nsub + right margin, multiple of 4
     int pstride;        // floats of per-particle constants
@@ -80,18 +93,81 @@ struct McArgs {
     const float* g_lp;
     const float* g_chi2;
     float* grad;       // optional dlp/dz
+    const float* g_img;  // VJP of simulate: upstream image gradient (bs x N x N); no likelihood
+    int map;           // 1: MAP step -- the loss is -weight * lp / N^2 and z, m, v get a Keras-Adam update
+    float* z_io;
+    float* m;
+    float* v;
+    float lr_t, beta1, beta2, eps;
 };
+
+// One lens of a launch: its plan, its workspace, and its particles
+// [first, first + count) of the launch; weight = MAP loss weight per particle.
+struct McLens {
+    DPlan P;
+    McPlan M;
+    int first, count;
+    float weight;
+    int pad_;
+};
+
+// Lens table of a launch.  Single-lens launches use `one` (a kernel argument);
+// multi-lens launches use lens[0..n) and the n + 1 prefix sums pre_* of each
+// stage's units over the lenses (device memory).
+struct McTable {
+    McLens one;
+    const McLens* lens;
+    int n;
+    const int* pre_part;   // particles
+    const int* pre_basis;  // 256-thread blocks of sub-pixels (mc_basis, mc_grad)
+    const int* pre_conv;   // conv waves
+    const int* pre_adj;    // adjoint waves (a multiple of 4 per particle)
+    int tot_part, tot_basis, tot_conv, tot_adj;
+};
+
+// lens holding unit x of a stage, and x's offset inside that lens.  MULTI =
+// false: the single lens passed by value (its fields stay kernel arguments).
+template <bool MULTI>
+__device__ __forceinline__ const McLens& mc_locate(const McTable& T, const int* pre, int x, int& local) {
+    if (!MULTI) {
+        local = x;
+        return T.one;
+    }
+    int lo = 0, hi = T.n - 1;
+    while (lo < hi) {  // largest L with pre[L] <= x
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    local = x - pre[lo];
+    return T.lens[lo];
+}
+
+__device__ __forceinline__ int mc_bpp(const DPlan& P) { return (P.nsub * P.nsub + 255) / 256; }
+__device__ __forceinline__ int mc_adj_wpp4(const DPlan& P) {
+    return ((P.nsub * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
+}
+
+// gradient weight of particle p's log posterior: MAP -> -weight / N^2, else g_lp (default 1)
+__device__ __forceinline__ float mc_glp(const McArgs& A, const McLens& E, int p) {
+    if (A.map) return -E.weight / (float)(E.P.N * E.P.N);
+    return A.g_lp ? A.g_lp[p] : 1.0f;
+}
 
 // ---------------------------------------------------------------------------
 // constants of particle p
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float* pconst(const McPlan& M, int p) { return M.consts + (size_t)p * M.pstride; }
 
-__global__ void mc_prologue(const DPlan P, const McPlan M, const McArgs A) {
-    const int p = blockIdx.x;
+template <bool MULTI>
+__global__ void mc_prologue(const McTable T, const McArgs A) {
+    const int p = blockIdx.x;  // particle of the launch (inputs, outputs)
+    int q;                     // particle inside its lens (workspace)
+    const McLens& E = mc_locate<MULTI>(T, T.pre_part, p, q);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
     const int tid = threadIdx.x;
     const int d = P.d;
-    float* pc = pconst(M, p);
+    float* pc = pconst(M, q);
     float* s_x = pc;  // [x | dxdz | dlpdx | dfldjdz | gx | lpj]
     float* s_c = pc + ((6 * d + 3) & ~3);
     if (tid < d) {
@@ -111,11 +187,15 @@ __global__ void mc_prologue(const DPlan P, const McPlan M, const McArgs A) {
     }
     __threadfence_block();
     __syncthreads();
-    if (tid < P.ncomp) {
-        const DComp& cd = P.comp[tid];
+    // component constants, one thread per component (compile-time component index)
+#pragma unroll
+    for (int ci = 0; ci < kMaxComp; ++ci) {
+        if (ci != tid || ci >= P.ncomp) continue;
+        const DComp& cd = P.comp[ci];
         float pv[GL_MAX_COMPONENT_PARAMS];
+#pragma unroll
         for (int j = 0; j < GL_MAX_COMPONENT_PARAMS; ++j) pv[j] = cd.col[j] >= 0 ? s_x[cd.col[j]] : cd.cst[j];
-        float* dst = s_c + tid * kConstStride;
+        float* dst = s_c + ci * kConstStride;
         if (cd.kind == glm::K_EPL) {
             glm::EplC c;
             glm::epl_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.niter, -1, c);
@@ -143,8 +223,9 @@ __global__ void mc_prologue(const DPlan P, const McPlan M, const McArgs A) {
     }
     __threadfence_block();
     __syncthreads();
-    for (int i = 0; i < P.nlens; ++i) {
-        if (P.comp[i].kind != glm::K_EPL) continue;
+#pragma unroll
+    for (int i = 0; i < kMaxLens; ++i) {
+        if (i >= P.nlens || P.comp[i].kind != glm::K_EPL) continue;
         const glm::EplC* c = reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
         float* ca = pc + M.coef_base + P.coef_off[i];
         float4* cb = reinterpret_cast<float4*>(ca + ((P.max_n + 4) & ~3));
@@ -155,7 +236,8 @@ __global__ void mc_prologue(const DPlan P, const McPlan M, const McArgs A) {
             cb[n] = make_float4(cv, dt, df, 0.0f);
         }
     }
-    if (tid < kMcStat) M.stats[(size_t)p * kMcStat + tid] = 0.0f;
+    if (tid < kMcStat) M.stats[(size_t)q * kMcStat + tid] = 0.0f;
+    if (!M.lstsq && tid == 0) M.coef[(size_t)q * M.D] = P.cf;  // fixed amplitudes: image = cf * conv
 }
 
 // Deflection of all lenses of particle p at (X, Y); keeps the EPL states when ADJ.
@@ -201,9 +283,15 @@ __device__ __forceinline__ void mc_phi(const McPlan& M, float u, float* phi, flo
 // ---------------------------------------------------------------------------
 // mc_basis: grid (bs, ceil(nsub*nsub/256)), block 256
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void mc_basis(const DPlan P, const McPlan M) {
-    const int p = blockIdx.y;
-    const int s = blockIdx.x * 256 + threadIdx.x;
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_basis(const McTable T) {
+    int local;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_basis, blockIdx.x, local);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
+    const int bpp = mc_bpp(P);
+    const int p = local / bpp;
+    const int s = (local - p * bpp) * 256 + threadIdx.x;
     const int nsub = P.nsub;
     if (s >= nsub * nsub) return;
     const int y = s / nsub, x = s - (s / nsub) * nsub;
@@ -215,6 +303,19 @@ __global__ __launch_bounds__(256) void mc_basis(const DPlan P, const McPlan M) {
     mc_beta<false>(P, M, pc, X, Y, bx, by, nullptr, nullptr);
     float* Bp = M.B + (size_t)p * M.D * nsub * M.BW + (size_t)y * M.BW + P.pb + x;
     const size_t cstride = (size_t)nsub * M.BW;
+    if (!M.lstsq) {
+        // fixed amplitudes: the sum of every light, NaN -> 0 (tf/simulator.py:74-81)
+        float v = 0.0f;
+        for (int i = P.nlens; i < P.ncomp; ++i) {
+            const DComp& cd = P.comp[i];
+            const glm::SerC C = *reinterpret_cast<const glm::SerC*>(s_c + i * kConstStride);
+            const bool src = cd.role == GL_ROLE_SOURCE;
+            glm::SerState st;
+            v += glm::sersic_forward(C, src ? bx : X, src ? by : Y, st);
+        }
+        Bp[0] = isnan(v) ? 0.0f : v;
+        return;
+    }
     int ch = 0;
     for (int i = P.nlens; i < P.ncomp; ++i) {
         const DComp& cd = P.comp[i];
@@ -249,14 +350,19 @@ __global__ __launch_bounds__(256) void mc_basis(const DPlan P, const McPlan M) {
 // ---------------------------------------------------------------------------
 // mc_conv: one wave = (particle, channel, kMcIB output rows, 64 output columns)
 // ---------------------------------------------------------------------------
-template <int SS, int KF4>
-__global__ __launch_bounds__(256) void mc_conv(const DPlan P, const McPlan M, int bs) {
+// SS = 0: the supersampling factor is read from the plan (lenses of several ss)
+template <int SS, int KF4, bool MULTI>
+__global__ __launch_bounds__(256) void mc_conv(const McTable T) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gtask = blockIdx.x * 4 + wave;
+    if (gtask >= T.tot_conv) return;
+    int task;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_conv, gtask, task);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, kf = P.kf, pb = P.pb;
+    const int ss = SS ? SS : P.ss;
     const int nib = (N + kMcIB - 1) / kMcIB, njb = (N + 63) / 64;
-    int task = blockIdx.x * 4 + wave;
-    const int total = bs * M.D * nib * njb;
-    if (task >= total) return;
     const int jb = task % njb; task /= njb;
     const int ib = task % nib; task /= nib;
     const int c = task % M.D;
@@ -268,12 +374,12 @@ __global__ __launch_bounds__(256) void mc_conv(const DPlan P, const McPlan M, in
     float acc[kMcIB];
 #pragma unroll
     for (int i = 0; i < kMcIB; ++i) acc[i] = 0.0f;
-    const int nrows = SS * (kMcIB - 1) + kf;
+    const int nrows = ss * (kMcIB - 1) + kf;
     for (int r = 0; r < nrows; ++r) {
-        const int y = SS * I0 + r - pb;
+        const int y = ss * I0 + r - pb;
         if (y < 0 || y >= nsub) continue;
-        // staged row: column x sits at index pb + x; this lane needs x = SS*J + q - pb
-        const float* row = Bc + (size_t)y * M.BW + SS * J;
+        // staged row: column x sits at index pb + x; this lane needs x = ss*J + q - pb
+        const float* row = Bc + (size_t)y * M.BW + ss * J;
         float v[4 * KF4];
         if (active) {
             if (SS == 4) {
@@ -293,7 +399,7 @@ __global__ __launch_bounds__(256) void mc_conv(const DPlan P, const McPlan M, in
         }
 #pragma unroll
         for (int i = 0; i < kMcIB; ++i) {
-            const int pp = r - SS * i;
+            const int pp = r - ss * i;
             if (pp >= 0 && pp < kf) {
                 const float* w = M.wpad + pp * M.wpad_w;
                 float a = acc[i];
@@ -327,8 +433,8 @@ __global__ __launch_bounds__(256) void mc_conv(const DPlan P, const McPlan M, in
 // ---------------------------------------------------------------------------
 typedef float mc_v4f __attribute__((ext_vector_type(4)));
 
-template <int NB>
-__global__ __launch_bounds__(256) void mc_gram(const DPlan P, const McPlan M, int ksplit) {
+template <int NB, bool MULTI>
+__global__ __launch_bounds__(256) void mc_gram(const McTable T, int ksplit) {
     constexpr int NT = NB * (NB + 1) / 2;
     constexpr int CP = 16 * NB;
     constexpr int KC = 64;
@@ -336,7 +442,11 @@ __global__ __launch_bounds__(256) void mc_gram(const DPlan P, const McPlan M, in
     constexpr int LPT = CP * KC / 256;
     constexpr int kFlush = 8;
     __shared__ __attribute__((aligned(16))) float yt[CP * LS];
-    const int p = blockIdx.x, ks = blockIdx.y;
+    int p;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_part, blockIdx.x, p);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
+    const int ks = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int D = M.D, NP = P.N * P.N;
     const int nchunk = (NP + KC - 1) / KC;
@@ -422,9 +532,11 @@ __device__ __forceinline__ double mc_gram_entry(const McPlan& M, int p, int part
 // ---------------------------------------------------------------------------
 // mc_solve: grid bs, block 256.  FP64 parallel cyclic Jacobi + pseudo-inverse.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void mc_solve(const McPlan M, float rcond, int parts) {
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, int parts) {
     extern __shared__ double s_d[];
-    const int p = blockIdx.x;
+    int p;
+    const McPlan& M = mc_locate<MULTI>(T, T.pre_part, blockIdx.x, p).M;
     const int D = M.D;
     const int n = D + (D & 1);  // even size (dummy index n-1 when D is odd)
     double* A = s_d;            // n x n
@@ -540,24 +652,39 @@ __global__ __launch_bounds__(256) void mc_solve(const McPlan M, float rcond, int
 // ---------------------------------------------------------------------------
 // mc_image: grid bs, block 256.  image, likelihood, chi^2, lp, dL/dimage.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void mc_image(const DPlan P, const McPlan M, const McArgs A, int want_grad) {
-    const int p = blockIdx.x;
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_image(const McTable T, const McArgs A, int want_grad) {
+    const int gp = blockIdx.x;  // particle of the launch
+    int p;                      // particle inside its lens
+    const McLens& E = mc_locate<MULTI>(T, T.pre_part, gp, p);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
     const int N = P.N, NP = N * N, D = M.D;
     const float* Xp = M.X + (size_t)p * D * NP;
     const float* cp = M.coef + (size_t)p * D;
-    const float glp = A.g_lp ? A.g_lp[p] : 1.0f;
-    const float gch = A.g_chi2 ? A.g_chi2[p] : 0.0f;
+    const float glp = mc_glp(A, E, gp);
+    const float gch = (!A.map && A.g_chi2) ? A.g_chi2[gp] : 0.0f;
     const float inv_np = 1.0f / (float)NP;
     const int GW = N + 2 * M.gp;
     float ll = 0.0f, chi = 0.0f;
     for (int pix = threadIdx.x; pix < NP; pix += blockDim.x) {
         float im = 0.0f;
         for (int c = 0; c < D; ++c) im = fmaf(Xp[(size_t)c * NP + pix], cp[c], im);
-        if (A.img) A.img[(size_t)p * NP + pix] = im;
+        if (A.img) A.img[(size_t)gp * NP + pix] = im;
         if (A.stacked)
-            for (int c = 0; c < D; ++c) A.stacked[((size_t)p * NP + pix) * D + c] = Xp[(size_t)c * NP + pix];
+            for (int c = 0; c < D; ++c) A.stacked[((size_t)gp * NP + pix) * D + c] = Xp[(size_t)c * NP + pix];
+        if (A.g_img) {
+            if (want_grad) {
+                const int I = pix / N, J = pix - (pix / N) * N;
+                M.gimg[(size_t)p * GW * GW + (I + M.gp) * GW + (J + M.gp)] = A.g_img[(size_t)gp * NP + pix];
+            }
+            continue;
+        }
         float l, r2, dll, dr2;
-        glm::lik_backward(im, P.obs[pix], P.inv_var_obs[pix], P.log_sig_obs[pix], l, r2, dll, dr2);
+        if (P.lik == GL_LIK_FORWARD)
+            glm::lik_forward(im, P.obs[pix], P.bkg2, P.inv_texp, l, r2, dll, dr2);
+        else
+            glm::lik_backward(im, P.obs[pix], P.inv_var_obs[pix], P.log_sig_obs[pix], l, r2, dll, dr2);
         ll += l;
         chi += r2;
         if (want_grad) {
@@ -582,25 +709,36 @@ __global__ __launch_bounds__(256) void mc_image(const DPlan P, const McPlan M, c
         const float* px = pconst(M, p);
         float lpri = 0.0f;
         for (int j = 0; j < P.d; ++j) lpri += px[5 * P.d + j];
-        if (A.lp) A.lp[p] = L + lpri;
-        if (A.chi2) A.chi2[p] = Cq * inv_np;
+        if (A.lp) A.lp[gp] = L + lpri;
+        if (A.chi2) A.chi2[gp] = Cq * inv_np;
     }
     if (A.coef_out)
-        for (int c = threadIdx.x; c < D; c += blockDim.x) A.coef_out[(size_t)p * D + c] = cp[c];
+        for (int c = threadIdx.x; c < D; c += blockDim.x) A.coef_out[(size_t)gp * D + c] = cp[c];
 }
 
 // ---------------------------------------------------------------------------
 // mc_adjoint: transposed correlation of dL/dimage onto the sub-grid.
-// grid (bs * nsub * ss, ceil(N/64)), block 64: one wave per (particle, row y,
-// column phase b, 64 columns X); x = ss X + b, so the weights are wave-uniform.
+// block 256 = 4 waves; one wave per (particle, row y, column phase b, 64 columns
+// X); x = ss X + b, so the weights are wave-uniform.  Each particle owns a
+// multiple of 4 waves, so a block never spans two particles.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void mc_adjoint(const DPlan P, const McPlan M) {
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
+    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gw >= T.tot_adj) return;
+    int local;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_adj, gw, local);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb;
-    int t = blockIdx.x;
+    const int njb = (N + 63) / 64, wpp4 = mc_adj_wpp4(P);
+    const int p = local / wpp4;
+    int t = local - p * wpp4;
+    if (t >= nsub * ss * njb) return;
+    const int jb = t % njb; t /= njb;
     const int b = t % ss; t /= ss;
-    const int y = t % nsub;
-    const int p = t / nsub;
-    const int Xc = blockIdx.y * 64 + threadIdx.x;
+    const int y = t;
+    const int Xc = jb * 64 + (threadIdx.x & 63);
     const int a = y % ss, Yc = y / ss;
     const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
     const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
@@ -623,9 +761,15 @@ __global__ __launch_bounds__(64) void mc_adjoint(const DPlan P, const McPlan M) 
 // Statistics layout: per lens slot 8 (EPL 8 / shear 2 / SIS 3), per light slot 8
 // (Sersic) or 3 (shapelets: sum g L_u, sum g L_v, sum g (u L_u + v L_v)).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void mc_grad(const DPlan P, const McPlan M) {
-    const int p = blockIdx.y;
-    const int s = blockIdx.x * 256 + threadIdx.x;
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_grad(const McTable T) {
+    int local;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_basis, blockIdx.x, local);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
+    const int bpp = mc_bpp(P);
+    const int p = local / bpp;
+    const int s = (local - p * bpp) * 256 + threadIdx.x;
     const int nsub = P.nsub;
     float st[kMcStat];
 #pragma unroll
@@ -659,6 +803,14 @@ __global__ __launch_bounds__(256) void mc_grad(const DPlan P, const McPlan M) {
                 const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
                 glm::SerState ss_;
                 const float v = glm::sersic_forward(C, ux, uy, ss_);
+                if (!M.lstsq) {
+                    // one channel: a NaN light zeroes the whole sub-pixel (via val below)
+                    val += v;
+                    float gdx, gdy;
+                    glm::sersic_adjoint(C, ss_, g * cp[0], acc, gdx, gdy);
+                    if (src) { gbx += gdx; gby += gdy; }
+                    continue;
+                }
                 if (isnan(v)) { ++ch; continue; }
                 val += cp[ch] * v;
                 float gdx, gdy;
@@ -738,8 +890,13 @@ __global__ __launch_bounds__(256) void mc_grad(const DPlan P, const McPlan M) {
 // ---------------------------------------------------------------------------
 // mc_finish: grid bs, block 64.  Statistics -> dlp/dz (or dL/dx).
 // ---------------------------------------------------------------------------
-__global__ void mc_finish(const DPlan P, const McPlan M, const McArgs A) {
-    const int p = blockIdx.x;
+template <bool MULTI>
+__global__ void mc_finish(const McTable T, const McArgs A) {
+    const int gp = blockIdx.x;
+    int p;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_part, gp, p);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
     float* px = pconst(M, p);
     const int d = P.d;
     float* gx = px + 4 * d;
@@ -768,6 +925,7 @@ __global__ void mc_finish(const DPlan P, const McPlan M, const McArgs A) {
             } else if (cd.kind == glm::K_SERSIC_ELLIPSE || cd.kind == glm::K_SERSIC) {
                 glm::sersic_finish(*reinterpret_cast<const glm::SerC*>(cst), acc, g);
                 np_ = cd.kind == glm::K_SERSIC_ELLIPSE ? 6 : 4;  // no Ie in least-squares mode
+                if (!cd.lstsq) np_ += 1;                        // fixed amplitudes: Ie too
                 circ = cd.kind == glm::K_SERSIC;
                 ++ch;
             } else if (cd.kind == glm::K_SHAPELETS) {
@@ -783,13 +941,21 @@ __global__ void mc_finish(const DPlan P, const McPlan M, const McArgs A) {
         }
     }
     __syncthreads();
-    if (threadIdx.x < d && A.grad) {
+    if (threadIdx.x < d && (A.grad || A.map)) {
         const int j = threadIdx.x;
-        const float glp = A.g_lp ? A.g_lp[p] : 1.0f;
-        if (!A.use_z) {
-            A.grad[(size_t)p * d + j] = gx[j];
-        } else {
-            A.grad[(size_t)p * d + j] = (gx[j] + glp * px[2 * d + j]) * px[d + j] + glp * px[3 * d + j];
+        const float glp = mc_glp(A, E, gp);
+        float gz;
+        if (!A.use_z) gz = gx[j];
+        else gz = (gx[j] + glp * px[2 * d + j]) * px[d + j] + glp * px[3 * d + j];
+        if (A.grad) A.grad[(size_t)gp * d + j] = gz;
+        if (A.map) {
+            // Keras Adam (epsilon outside the sqrt; lr_t carries the bias corrections)
+            const size_t i = (size_t)gp * d + j;
+            const float m = A.beta1 * A.m[i] + (1.0f - A.beta1) * gz;
+            const float v = A.beta2 * A.v[i] + (1.0f - A.beta2) * gz * gz;
+            A.m[i] = m;
+            A.v[i] = v;
+            A.z_io[i] -= A.lr_t * m / (sqrtf(v) + A.eps);
         }
     }
 }

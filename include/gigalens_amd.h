@@ -102,6 +102,9 @@ typedef struct gl_plan_desc {
     int n_observed;           /* observed images stacked in `observed` (and `noise_map`), each
                                  N*N; 0 selects 1.  Particles pick one with the lens index of
                                  gl_log_prob_lenses / gl_map_adam_step_lenses */
+    int staged;               /* 1: always use the HBM-staged pipeline (otherwise it is chosen when
+                                 the LDS-resident kernel cannot hold the image, has no compiled
+                                 kernel for the profile combination, or lights are linear) */
 } gl_plan_desc;
 
 typedef struct gl_plan gl_plan;
@@ -181,6 +184,29 @@ int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, 
 int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, float* v, const int* lens,
                             const float* weight, int bs, int step, float lr, float beta1, float beta2,
                             float epsilon, float* lp, float* chi2, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Batches of lens systems with different image sizes, supersampling, PSFs and
+ * particle counts.  No counterpart in the reference (one simulator per lens,
+ * tf/inference.py:21).  Each lens is a fixed-amplitude staged plan
+ * (gl_plan_desc.staged = 1, same parameter count and likelihood); the plans
+ * must outlive the batch.  Particles are lens-major: lens 0's counts[0]
+ * particles first.  Every stage of the staged pipeline is one launch over all
+ * lenses; the batch owns the workspace for its particle counts.
+ * ------------------------------------------------------------------------- */
+typedef struct gl_batch gl_batch;
+int gl_batch_create(const gl_plan* const* plans, int n_lenses, const int* counts, gl_batch** out);
+int gl_batch_destroy(gl_batch* batch);
+int gl_batch_size(const gl_batch* batch);  /* total particles */
+/* lp, chi2 (total) and, when grad_z is non-NULL, dlp/dz (total x d) */
+int gl_batch_log_prob(const gl_batch* batch, const float* z, float* lp, float* chi2, float* grad_z,
+                      hipStream_t stream);
+/* One MAP step of every lens at once: lens L's loss is mean over its particles of
+ * -lp / N_L^2 (tf/inference.py:28-35) with Keras Adam; lp / chi2 get the
+ * pre-update values. */
+int gl_batch_map_adam_step(const gl_batch* batch, float* z, float* m, float* v, int step, float lr,
+                           float beta1, float beta2, float epsilon, float* lp, float* chi2,
+                           hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Learned initialiser.  No counterpart in the reference: a small BF16 CNN
