@@ -14,6 +14,9 @@ Prints one JSON line.
 ``--workload cnn`` measures the learned initialiser's training step (1,024
 simulated 60x60 images, BF16 MFMA network, AdamW).
 
+``--workload uneven`` measures batched fitting of many lens systems with
+different image sizes, supersampling and particle counts (lenses/hour).
+
 ``--workload shapelets`` measures the least-squares configuration instead
 (100x100 px, supersample 4, 21x21 Moffat PSF, EPL+shear, Sersic lens light and
 an n_max=10 shapelet source with linear amplitudes; 512 prior particles,
@@ -295,6 +298,107 @@ def run_cnn(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_uneven(args, dev):
+    """Uneven lens batch (config-0-style lenses, 0.065"/px, 11x11 Gaussian PSF):
+    ``--uneven-lenses`` lenses (seeds 2000..) with N in {48,64,80,100,128}, ss in
+    {2,3,4} and U{256..4096} particles (times ``--uneven-scale``).  Pipeline per
+    lens: 350 MAP steps from prior draws, then 1,500 diagonal-SVI steps with 64
+    samples.  Both the batched pipeline (every stage one launch over all lenses)
+    and the sequential per-lens pipeline (each lens alone, on the LDS-resident
+    kernel where it fits) are timed over ``--steps`` steps of each phase after one
+    warm-up step and extrapolated to the protocol's step counts."""
+    from gigalens_amd import SimulatorConfig, psf
+    from gigalens_amd.batch import LensBatch
+    from gigalens_amd.inference import Adam
+    from gigalens_amd.plan import Plan, layout_from_paths
+    from gigalens_amd.simulator import Camera
+    prior, phys = wl.epl_shear_prior(), wl.epl_shear_model()
+    kernel = psf.gaussian(0.1, 0.065, 11)
+    rng = np.random.default_rng(2000)
+    L = args.uneven_lenses
+    specs = []
+    for k in range(L):
+        n_pix = int(rng.choice([48, 64, 80, 100, 128]))
+        ss = int(rng.choice([2, 3, 4]))
+        cnt = max(1, int(round(int(rng.integers(256, 4097)) * args.uneven_scale)))
+        specs.append((n_pix, ss, cnt))
+    paths = [pth for pth, _ in tfd.leaves(prior)]
+    lenses = []
+    for k, (n_pix, ss, cnt) in enumerate(specs):
+        cfg = SimulatorConfig(delta_pix=0.065, num_pix=n_pix, supersample=ss, kernel=kernel)
+        gen = Plan(phys, Camera(cfg), layout_from_paths(phys, paths), staged=True, max_batch=1)
+        truth = flat_sample(prior, 1, seed=2000 + k).to(dev).contiguous()
+        clean = gen.simulate(truth).cpu().numpy()[0]
+        del gen
+        lenses.append((cfg, wl.noisy_observation(clean, 0.2, 100.0, seed=3000 + k), cnt))
+    total = sum(c for *_, c in lenses)
+    K = args.steps
+    n_map, n_svi, n_vi = 350, 1500, 64
+
+    def timed(fn, k):
+        fn(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(k)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    # batched pipeline
+    tb = time.perf_counter()
+    batch = LensBatch(phys, prior, lenses, 0.2, 100.0, device=dev)
+    t_create = time.perf_counter() - tb
+    z0 = batch.bij.inverse(prior.sample(total, seed=4000)).to(dev, torch.float32).contiguous()
+    t_map_b = timed(lambda k: batch.run(z0, k, Adam(5e-3)), K)
+    svi_b = LensBatch(phys, prior, [(c, o, n_vi) for c, o, _ in lenses], 0.2, 100.0, device=dev)
+    loc0 = z0[torch.as_tensor(np.cumsum([0] + [c for *_, c in lenses[:-1]]), device=dev)]
+    t_svi_b = timed(lambda k: svi_b.svi_diag(loc0, k, Adam(1e-3)), K)
+    del batch, svi_b
+    batch_total = n_map * t_map_b + n_svi * t_svi_b
+    # sequential per-lens pipeline
+    seq = []
+    off = 0
+    from gigalens_amd.model import ForwardProbModel
+    for k, (cfg, obs, cnt) in enumerate(lenses):
+        pm = ForwardProbModel(prior, obs, 0.2, 100.0)
+        plan = Plan(phys, Camera(cfg), layout_from_paths(phys, paths), prior=prior, observed=obs,
+                    background_rms=0.2, exp_time=100.0, max_batch=cnt)
+        zk = z0[off:off + cnt].clone()
+        mk, vk = torch.zeros_like(zk), torch.zeros_like(zk)
+
+        def map_steps(n):
+            for s_ in range(1, n + 1):
+                plan.map_adam_step(zk, mk, vk, s_, 5e-3)
+        t_map = timed(map_steps, K)
+        zs = zk[:n_vi].contiguous()
+
+        def svi_steps(n):
+            for _ in range(n):
+                plan.log_prob(zs, want_grad=True)
+        t_svi = timed(svi_steps, K)
+        seq.append({"n_pix": cfg.num_pix, "ss": cfg.supersample, "particles": cnt,
+                    "fused": plan.info()["lds_bytes"] > 0, "map_step_s": t_map, "svi_step_s": t_svi,
+                    "total_s": n_map * t_map + n_svi * t_svi})
+        off += cnt
+        del plan
+    seq_tot = np.array([r["total_s"] for r in seq])
+    out = {"metric": "lenses/hour (uneven batch, MAP 350 + diagonal SVI 1500 steps)",
+           "value": L / batch_total * 3600.0, "unit": "lenses/hour", "n_gpus": 1, "steps": K, "warmup": 1,
+           "ms_per_step": t_map_b * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (prior draws, Poisson + Gaussian noise; seeds 2000..)",
+           "config": {"workload": f"{L} lenses, N in {{48..128}}, ss 2-4, particles U{{256..4096}} x "
+                                  f"{args.uneven_scale}, 11x11 Gaussian PSF; timed over {K} steps per phase and "
+                                  "extrapolated to 350 MAP + 1500 SVI steps", "lenses": L, "particles": total},
+           "batched": {"map_step_s": t_map_b, "svi_step_s": t_svi_b, "total_s": batch_total,
+                       "latency_p50_s": batch_total, "latency_p95_s": batch_total, "create_s": t_create,
+                       "particle_evals_per_s": total / t_map_b},
+           "sequential": {"total_s": float(seq_tot.sum()), "lenses_per_hour": L / float(seq_tot.sum()) * 3600.0,
+                          "latency_p50_s": float(np.median(seq_tot)),
+                          "latency_p95_s": float(np.percentile(seq_tot, 95)),
+                          "fused_lenses": int(sum(r["fused"] for r in seq))},
+           "speedup_vs_sequential": float(seq_tot.sum()) / batch_total}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,7 +408,9 @@ def main():
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="also launch a 256 MiB device copy (PMC calibration)")
-    ap.add_argument("--workload", choices=["map", "shapelets", "cnn"], default="map")
+    ap.add_argument("--workload", choices=["map", "shapelets", "cnn", "uneven"], default="map")
+    ap.add_argument("--uneven-lenses", type=int, default=128)
+    ap.add_argument("--uneven-scale", type=float, default=1.0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-prior time-to-MAP evaluation")
     ap.add_argument("--learned-images", type=int, default=2 ** 20)
     ap.add_argument("--learned-lenses", type=int, default=256)
@@ -325,6 +431,10 @@ def main():
         if args.steps == 350:
             args.steps = 100
         return run_cnn(args, dev)
+    if args.workload == "uneven":
+        if args.steps == 350:
+            args.steps = 3
+        return run_uneven(args, dev)
     bs = args.particles
     prior, phys, cfg, sim, pm, obs, z0 = setup(bs, dev)
     plan = pm.plan_for(sim)

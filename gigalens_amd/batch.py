@@ -110,9 +110,10 @@ class LensBatch:
         self.plans = []
         self.counts = []
         for cfg, obs, n in lenses:
+            # the batch owns the workspace of its particles: the plan itself keeps one particle's
             self.plans.append(Plan(phys_model, Camera(cfg), layout_from_paths(phys_model, paths), prior=prior,
                                    observed=np.asarray(obs, np.float32), background_rms=background_rms,
-                                   exp_time=exp_time, likelihood=nat.GL_LIK_FORWARD, staged=True, max_batch=int(n)))
+                                   exp_time=exp_time, likelihood=nat.GL_LIK_FORWARD, staged=True, max_batch=1))
             self.counts.append(int(n))
         self.n_lenses = len(self.plans)
         handles = (ctypes.c_void_p * self.n_lenses)(*[p._h.value for p in self.plans])
@@ -168,3 +169,43 @@ class LensBatch:
             optimizer.iterations += 1
         self.lp_history, self.chi2_history = lp_h, chi_h
         return z
+
+    def svi_diag(self, loc: torch.Tensor, num_steps: int = 1500, optimizer: Optional[Adam] = None,
+                 init_scale: float = 1e-3, seed: int = 2):
+        """Diagonal-Gaussian SVI of every lens at once (the surrogate of
+        tf/inference.py:43-73 restricted to a diagonal scale).  The batch's
+        particle count per lens is the number of Monte-Carlo samples per step.
+        loc: (lenses, d) starting means.  Reparameterised ELBO gradient from one
+        batched log_prob + gradient launch per step:
+          d/d loc = mean_s grad_z lp,  d/d log sigma = mean_s grad_z lp * eps * sigma + 1.
+        Returns (loc, log_sigma, losses (steps, lenses))."""
+        optimizer = optimizer if optimizer is not None else Adam(1e-3)
+        L, d = loc.shape
+        counts = torch.as_tensor(self.counts, device=self.device)
+        loc = loc.to(self.device, torch.float32).clone()
+        logs = torch.full_like(loc, float(np.log(init_scale)))
+        g = torch.Generator(device=self.device)
+        g.manual_seed(int(seed))
+        idx = self.lens.long()
+        inv_n = (1.0 / counts.float())[:, None]
+        m = [torch.zeros_like(loc), torch.zeros_like(logs)]
+        v = [torch.zeros_like(loc), torch.zeros_like(logs)]
+        losses = torch.empty((num_steps, L), device=self.device)
+        for s in range(num_steps):
+            eps = torch.randn((self.total, d), device=self.device, generator=g)
+            sig = torch.exp(logs)
+            z = (loc[idx] + sig[idx] * eps).contiguous()
+            lp, _, gz = self.log_prob(z, want_grad=True)
+            # loss = -ELBO = -(mean lp + sum log sigma) + const
+            lp_mean = torch.zeros(L, device=self.device).index_add_(0, idx, lp) * inv_n[:, 0]
+            losses[s] = -(lp_mean + logs.sum(1))
+            g_loc = -torch.zeros_like(loc).index_add_(0, idx, gz) * inv_n
+            g_logs = -(torch.zeros_like(logs).index_add_(0, idx, gz * eps * sig[idx]) * inv_n + 1.0)
+            lr_t = optimizer.lr(optimizer.iterations) * np.sqrt(1 - optimizer.beta_2 ** (s + 1)) / \
+                (1 - optimizer.beta_1 ** (s + 1))
+            for k, (par, gr) in enumerate(((loc, g_loc), (logs, g_logs))):
+                m[k].mul_(optimizer.beta_1).add_((1 - optimizer.beta_1) * gr)
+                v[k].mul_(optimizer.beta_2).add_((1 - optimizer.beta_2) * gr * gr)
+                par.sub_(lr_t * m[k] / (torch.sqrt(v[k]) + optimizer.epsilon))
+            optimizer.iterations += 1
+        return loc, logs, losses

@@ -286,8 +286,6 @@ static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int 
     M.coef = (float*)dptr;
     if ((rc = dev_alloc(owned, (size_t)mb * GW * GW * 4, &dptr, true))) return rc;  // zero margins stay zero
     M.gimg = (float*)dptr;
-    if ((rc = dev_alloc(owned, (size_t)mb * nsub * nsub * 4, &dptr, false))) return rc;
-    M.gsub = (float*)dptr;
     if ((rc = dev_alloc(owned, (size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
     M.stats = (float*)dptr;
     return GL_OK;
@@ -359,7 +357,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     // particles per chunk: the caller's, else as many as fit a 32 GiB workspace (at most 1024)
     const int BW = ((ss * (N - 1) + 4 * p->kf4 > pb + nsub ? ss * (N - 1) + 4 * p->kf4 : pb + nsub) + 3) & ~3;
     const double per_particle = 4.0 * ((double)M.pstride + (double)D * nsub * BW + (double)D * N * N + D +
-                                       (double)GW * GW + (double)nsub * nsub + gl::kMcStat) +
+                                       (double)GW * GW + gl::kMcStat) +
                                 8.0 * 4 * p->ksplit_cap * gram_nt * 256;
     int mb = desc->max_batch;
     if (mb <= 0) mb = (int)std::max(1.0, std::min(1024.0, 32.0 * 1073741824.0 / per_particle));
@@ -429,11 +427,11 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
 // One pass of the staged pipeline over the lenses of table T.
 template <bool MULTI>
 static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch, int kf4, int lstsq, int gram_nb,
-                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st) {
+                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st, bool shp = false) {
     const int np = T.tot_part;
     hipLaunchKernelGGL((gl::mc_prologue<MULTI>), dim3(np), dim3(128), 0, st, T, A);
     GL_HIP(hipGetLastError());
-    hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), 0, st, T);
+    hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
     GL_HIP(hipGetLastError());
     GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, (T.tot_conv + 3) / 4, st, T));
     if (lstsq) {
@@ -448,7 +446,10 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
     if (grad) {
         hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), 0, st, T);
         GL_HIP(hipGetLastError());
-        hipLaunchKernelGGL((gl::mc_grad<MULTI>), dim3(T.tot_basis), dim3(256), 0, st, T);
+        if (shp)
+            hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
+        else
+            hipLaunchKernelGGL((gl::mc_grad<MULTI, false>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
         GL_HIP(hipGetLastError());
         hipLaunchKernelGGL((gl::mc_finish<MULTI>), dim3(np), dim3(128), 0, st, T, A);
         GL_HIP(hipGetLastError());
@@ -492,8 +493,9 @@ static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStrea
         if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX)
             return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
         T.tot_part = nb; T.tot_basis = (int)u.basis; T.tot_conv = (int)u.conv; T.tot_adj = (int)u.adj;
+        T.lds_consts = M.pstride * 4;
         int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
-                                    grad, st);
+                                    grad, st, M.shp_comp >= 0);
         if (rc) return rc;
     }
     return GL_OK;
@@ -1061,6 +1063,9 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     b->T.tot_basis = (int)acc[1];
     b->T.tot_conv = (int)acc[2];
     b->T.tot_adj = (int)acc[3];
+    b->T.lds_consts = 0;
+    for (int i = 0; i < n; ++i) b->T.lds_consts = std::max(b->T.lds_consts, lens[i].M.pstride * 4);
+    if (b->T.lds_consts > 64 * 1024) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: constant block too large"); }
     if (hipDeviceSynchronize() != hipSuccess) { gl_batch_destroy(b); return fail(GL_ERR_HIP, "gl_batch_create: sync"); }
     *out = b;
     return GL_OK;

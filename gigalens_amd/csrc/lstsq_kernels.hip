@@ -76,7 +76,8 @@ struct McPlan {
     float* coef;             // bs x D
     float* gimg;             // bs x (N + 2 gp) x (N + 2 gp) padded dL/dimage
     int gp;                  // padding of gimg
-    float* gsub;             // bs x nsub x nsub
+    // the sub-grid adjoint image dL/dsub reuses B (dead after mc_conv): channel 0's
+    // interior, row stride BW, column offset pb -- the zero margins stay untouched
     float* stats;            // bs x kMcStat
     float* lp_like;          // bs (log-likelihood + chi^2 scratch)
 };
@@ -123,6 +124,7 @@ struct McTable {
     const int* pre_conv;   // conv waves
     const int* pre_adj;    // adjoint waves (a multiple of 4 per particle)
     int tot_part, tot_basis, tot_conv, tot_adj;
+    int lds_consts;        // bytes of LDS for one particle's constant block (max over lenses)
 };
 
 // lens holding unit x of a stage, and x's offset inside that lens.  MULTI =
@@ -138,11 +140,28 @@ __device__ __forceinline__ const McLens& mc_locate(const McTable& T, const int* 
         const int mid = (lo + hi + 1) >> 1;
         if (pre[mid] <= x) lo = mid; else hi = mid - 1;
     }
+    lo = __builtin_amdgcn_readfirstlane(lo);  // x is uniform over the caller's wave
     local = x - pre[lo];
-    return T.lens[lo];
+    // the table is read-only for the launch: address it in the constant space so
+    // the fields come in through scalar loads
+    typedef const McLens __attribute__((address_space(4))) ConstLens;
+    const ConstLens* tab = (const ConstLens*)T.lens;
+    return *(const McLens*)(tab + lo);
 }
 
 __device__ __forceinline__ int mc_bpp(const DPlan& P) { return (P.nsub * P.nsub + 255) / 256; }
+
+// Copy particle p's constant block (pstride floats: prior terms, component
+// constants, EPL coefficient tables) into LDS; every thread of the block calls
+// it.  The per-sub-pixel kernels then read constants and series coefficients
+// from LDS (wave-uniform addresses, broadcast) instead of global memory.
+__device__ __forceinline__ const float* mc_stage_consts(const McPlan& M, int p, float* lds) {
+    const float4* src = reinterpret_cast<const float4*>(M.consts + (size_t)p * M.pstride);
+    float4* dst = reinterpret_cast<float4*>(lds);
+    for (int i = threadIdx.x; i < M.pstride / 4; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+    return lds;
+}
 __device__ __forceinline__ int mc_adj_wpp4(const DPlan& P) {
     return ((P.nsub * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
 }
@@ -293,9 +312,10 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
     const int p = local / bpp;
     const int s = (local - p * bpp) * 256 + threadIdx.x;
     const int nsub = P.nsub;
+    extern __shared__ float4 mc_dyn4[];
+    const float* pc = mc_stage_consts(M, p, reinterpret_cast<float*>(mc_dyn4));
     if (s >= nsub * nsub) return;
     const int y = s / nsub, x = s - (s / nsub) * nsub;
-    const float* pc = pconst(M, p);
     const float* s_c = pc + ((6 * P.d + 3) & ~3);
     const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
     const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
@@ -751,7 +771,7 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
             const float* wr = P.w + (ss * m + ra) * kf + rb;
             for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
         }
-        M.gsub[((size_t)p * nsub + y) * nsub + ss * Xc + b] = g;
+        M.B[((size_t)p * M.D * nsub + y) * M.BW + pb + ss * Xc + b] = g;
     }
 }
 
@@ -761,7 +781,8 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
 // Statistics layout: per lens slot 8 (EPL 8 / shear 2 / SIS 3), per light slot 8
 // (Sersic) or 3 (shapelets: sum g L_u, sum g L_v, sum g (u L_u + v L_v)).
 // ---------------------------------------------------------------------------
-template <bool MULTI>
+// SHP: the plan has a shapelet source (otherwise that branch is compiled out)
+template <bool MULTI, bool SHP>
 __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
     int local;
     const McLens& E = mc_locate<MULTI>(T, T.pre_basis, blockIdx.x, local);
@@ -774,12 +795,13 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
     float st[kMcStat];
 #pragma unroll
     for (int e = 0; e < kMcStat; ++e) st[e] = 0.0f;
-    const float* pc = pconst(M, p);
+    extern __shared__ float4 mc_dyn4[];
+    const float* pc = mc_stage_consts(M, p, reinterpret_cast<float*>(mc_dyn4));
     const float* s_c = pc + ((6 * P.d + 3) & ~3);
     const float* cp = M.coef + (size_t)p * M.D;
     if (s < nsub * nsub) {
         const int y = s / nsub, x = s - (s / nsub) * nsub;
-        const float g = M.gsub[(size_t)p * nsub * nsub + s];
+        const float g = M.B[((size_t)p * M.D * nsub + y) * M.BW + P.pb + x];
         const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
         const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
         glm::EplState es[3];
@@ -817,7 +839,7 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
                 glm::sersic_adjoint(C, ss_, g * cp[ch], acc, gdx, gdy);
                 if (src) { gbx += gdx; gby += gdy; }
                 ++ch;
-            } else if (cd.kind == glm::K_SHAPELETS) {
+            } else if (SHP && cd.kind == glm::K_SHAPELETS) {
                 const float beta = cst[0];
                 const float ib = 1.0f / beta;
                 const float u = (ux - cst[1]) * ib, w = (uy - cst[2]) * ib;
