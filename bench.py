@@ -228,9 +228,44 @@ def run_shapelets(args, dev):
                      "transcendentals_per_step": float(c["transcendentals"].sum())},
         "chi2_median": float(chi2.median()),
     }
+    if args.hmc:
+        out["hmc"] = shapelet_hmc(prior, phys, cfg, pm, z0, dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = shapelet_cpu_baseline(prior, phys, cfg, sim, obs, z0)
     print(json.dumps(out), flush=True)
+
+
+def shapelet_hmc(prior, phys, cfg, pm, z0, dev):
+    """MAP of the 512 particles (fused steps, Keras Adam lr 5e-3, 200 steps), a
+    diagonal SVI from the best particle (300 steps, 64 samples) for the mass
+    matrix, then HMC with 256 chains x 1,000 leapfrog steps at step size 0.01
+    (100 draws of 10 leapfrog steps) from the surrogate's draws."""
+    from gigalens_amd.inference import Adam, ModellingSequence, PolynomialDecay
+    seq = ModellingSequence(phys, pm, cfg, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    z = seq.MAP(Adam(5e-3), start=z0, num_steps=200)
+    sim = LensSimulator(phys, cfg, bs=z.shape[0], device=dev)
+    lp, chi2 = pm.log_prob(sim, z)
+    torch.cuda.synchronize()
+    t_map = time.perf_counter() - t0
+    best = torch.argmax(torch.nan_to_num(lp.detach(), nan=-1e30))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q, losses = seq.SVI(Adam(PolynomialDecay(0.0, 100, 2e-3, 2)), z[best].detach().cpu(), n_vi=64, num_steps=300,
+                        full_rank=False)
+    torch.cuda.synchronize()
+    t_svi = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    samples, stats = seq.HMC(q, init_eps=0.01, init_l=10, n_hmc=256, num_burnin_steps=0, num_results=100,
+                             adapt=False)
+    torch.cuda.synchronize()
+    t_hmc = time.perf_counter() - t0
+    return {"map_steps": 200, "map_seconds": t_map, "best_chi2": float(chi2.detach()[best]),
+            "svi_steps": 300, "svi_samples": 64, "svi_seconds": t_svi,
+            "chains": 256, "leapfrog_steps": 1000, "step_size": 0.01, "hmc_seconds": t_hmc,
+            "leapfrog_particle_evals_per_s": 256 * 1000 / t_hmc,
+            "accept_rate_mean": float(np.mean(stats["accept_rate"]))}
 
 
 def run_cnn(args, dev):
@@ -415,6 +450,7 @@ def main():
     ap.add_argument("--learned-images", type=int, default=2 ** 20)
     ap.add_argument("--learned-lenses", type=int, default=256)
     ap.add_argument("--learned-lr", type=float, default=1e-3)
+    ap.add_argument("--hmc", action="store_true", help="shapelets: also MAP + 256-chain HMC (1,000 leapfrog steps)")
     ap.add_argument("--learned-schedule", choices=["constant", "cosine"], default="constant")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))

@@ -184,7 +184,7 @@ class ModellingSequence(ModellingSequenceInterface):
         n = z.shape[0]
         sim = self._sim(n)
         pm = self.prob_model
-        if not isinstance(optimizer, Adam) or not hasattr(pm, "plan_for") or getattr(pm, "_likelihood", 0) != 0:
+        if not isinstance(optimizer, Adam) or not hasattr(pm, "plan_for"):
             return self._map_autograd(optimizer, z, sim, num_steps)
         plan = pm.plan_for(sim)
         m, v = torch.zeros_like(z), torch.zeros_like(z)
@@ -250,21 +250,31 @@ class ModellingSequence(ModellingSequenceInterface):
 
     # ---- HMC ----------------------------------------------------------------
     def HMC(self, q_z, init_eps=0.3, init_l=3, n_hmc=50, num_burnin_steps=250, num_results=750,
-            max_leapfrog_steps=30, seed=3, adapt=True, target_accept=0.75):
+            max_leapfrog_steps=30, seed=3, adapt=True, target_accept=0.75, start=None):
         """Preconditioned HMC (tf/inference.py:75-141): momentum ~ N(0, Sigma^-1)
         with Sigma the surrogate covariance (kinetic energy p^T Sigma p / 2),
         dual-averaging step-size adaptation and ChEES trajectory-length
         adaptation during 80% of the burn-in.  ``adapt=False`` runs fixed
-        ``init_eps`` / ``init_l``.  Returns ``(samples (num_results, n_hmc, d), stats)``."""
+        ``init_eps`` / ``init_l``.  ``q_z=None`` gives plain HMC (identity mass);
+        ``start`` (n_hmc, d) overrides the chains' initial states (default: draws
+        from q_z).  Returns ``(samples (num_results, n_hmc, d), stats)``."""
         gen = torch.Generator().manual_seed(int(seed))
-        # Sigma = L L^T with L the surrogate's scale_tril (positive diagonal): L is
-        # already the Cholesky factor; build Sigma in FP64 for the kinetic energy
-        chol = q_z.scale_tril().detach().to(self.device, torch.float64)
+        if q_z is None:
+            d0 = start.shape[1]
+            chol = torch.eye(d0, dtype=torch.float64, device=self.device)
+        else:
+            # Sigma = L L^T with L the surrogate's scale_tril (positive diagonal): L is
+            # already the Cholesky factor; build Sigma in FP64 for the kinetic energy
+            chol = q_z.scale_tril().detach().to(self.device, torch.float64)
         cov = (chol @ chol.T).to(torch.float32)
         chol_inv = torch.linalg.solve_triangular(chol, torch.eye(chol.shape[0], dtype=torch.float64,
                                                                   device=self.device), upper=False)
         chol_inv = chol_inv.to(torch.float32)
-        z = q_z.sample(n_hmc, generator=gen).detach().contiguous()
+        if start is not None:
+            z = torch.as_tensor(start, dtype=torch.float32).to(self.device).contiguous()
+            n_hmc = z.shape[0]
+        else:
+            z = q_z.sample(n_hmc, generator=gen).detach().contiguous()
         sim = self._sim(n_hmc)
         pm = self.prob_model
         d = z.shape[1]

@@ -42,7 +42,7 @@ def shapelet_prior(n_max):
     return tfd.JointDistributionSequential([lens, ll, src])
 
 
-def setup(num_pix, ss, delta, psf_size, n_max, interpolate, bs, seed=1):
+def setup(num_pix, ss, delta, psf_size, n_max, interpolate, bs, seed=1):  # noqa: C901
     prior = shapelet_prior(n_max)
     phys = PhysicalModel([EPL(50), Shear()], [SersicEllipse(use_lstsq=True)],
                          [Shapelets(n_max, use_lstsq=True, interpolate=interpolate)])
@@ -127,3 +127,19 @@ def test_lstsq_full_size_config(cuda):
     assert stacked.shape == (2, 100, 100, 67)
     assert e_st < 1e-4
     assert e_im < 1e-4
+
+
+def test_backward_map_fused_matches_autograd(cuda):
+    """MAP on the least-squares model: the fused step (pipeline + Adam in one
+    pass) against autograd through log_prob with the same Keras Adam."""
+    from gigalens_amd.inference import Adam, ModellingSequence
+    P = setup(30, 2, 0.08, 7, 4, True, bs=16, seed=5)
+    pm = BackwardProbModel(P["prior"], P["obs"], 0.2, 100.0)
+    seq = ModellingSequence(P["phys"], pm, P["cfg"], device=cuda)
+    z0 = pm.bij.inverse(C.nested_from_x(P["prior"], P["x"])).to(cuda).contiguous()
+    z1 = seq.MAP(Adam(5e-3), start=z0.clone(), num_steps=5)
+    sim = LensSimulator(P["phys"], P["cfg"], bs=16, device=cuda)
+    z2 = seq._map_autograd(Adam(5e-3), z0.clone(), sim, 5)
+    err = float((z1 - z2).abs().max())
+    _print("least-squares MAP fused vs autograd max |z diff|:", err)
+    assert err < 0.1 * 5 * 5e-3
