@@ -53,7 +53,7 @@ class MultiLensMAP:
         return lp, chi2
 
     def run(self, z0: torch.Tensor, lens: torch.Tensor, num_steps: int = 350, optimizer: Optional[Adam] = None,
-            record_every: int = 1) -> torch.Tensor:
+            record_every: int = 1, graph: bool = False) -> torch.Tensor:
         """Adam (Keras semantics) from z0 (n, d); lens (n,) int32.  Returns z;
         ``lp_history`` / ``chi2_history`` hold the per-particle values of every
         recorded step (before that step's update)."""
@@ -70,14 +70,21 @@ class MultiLensMAP:
         lp_h = torch.empty((nrec, n), device=self.device)
         chi_h = torch.empty((nrec, n), device=self.device)
         scratch = torch.empty((2, n), device=self.device)
-        for s in range(num_steps):
-            rec = s % record_every == 0
-            lp = lp_h[s // record_every] if rec else scratch[0]
-            chi = chi_h[s // record_every] if rec else scratch[1]
-            self.plan.map_adam_step(z, m, v, optimizer.iterations + 1, optimizer.lr(optimizer.iterations),
-                                    optimizer.beta_1, optimizer.beta_2, optimizer.epsilon, lp=lp, chi2=chi,
-                                    lens=lens, weight=weight)
-            optimizer.iterations += 1
+
+        def steps():
+            for s in range(num_steps):
+                rec = s % record_every == 0
+                lp = lp_h[s // record_every] if rec else scratch[0]
+                chi = chi_h[s // record_every] if rec else scratch[1]
+                self.plan.map_adam_step(z, m, v, optimizer.iterations + 1, optimizer.lr(optimizer.iterations),
+                                        optimizer.beta_1, optimizer.beta_2, optimizer.epsilon, lp=lp, chi2=chi,
+                                        lens=lens, weight=weight)
+                optimizer.iterations += 1
+        if graph:
+            from .graphs import capture
+            capture(steps).replay()
+        else:
+            steps()
         self.lp_history, self.chi2_history = lp_h, chi_h
         return z
 
@@ -149,8 +156,9 @@ class LensBatch:
         return lp, chi2, grad
 
     def run(self, z0: torch.Tensor, num_steps: int = 350, optimizer: Optional[Adam] = None,
-            record_every: int = 1) -> torch.Tensor:
-        """Keras-Adam MAP of every lens at once from z0 (total, d); histories as in MultiLensMAP."""
+            record_every: int = 1, graph: bool = False) -> torch.Tensor:
+        """Keras-Adam MAP of every lens at once from z0 (total, d); histories as in
+        MultiLensMAP; ``graph=True`` records the steps into one HIP graph first."""
         optimizer = optimizer if optimizer is not None else Adam(1e-2)
         z = z0.to(self.device, torch.float32).contiguous().clone()
         m, v = torch.zeros_like(z), torch.zeros_like(z)
@@ -158,15 +166,22 @@ class LensBatch:
         lp_h = torch.empty((nrec, self.total), device=self.device)
         chi_h = torch.empty((nrec, self.total), device=self.device)
         scratch = torch.empty((2, self.total), device=self.device)
-        for s in range(num_steps):
-            rec = s % record_every == 0
-            lp = lp_h[s // record_every] if rec else scratch[0]
-            chi = chi_h[s // record_every] if rec else scratch[1]
-            nat.check(self._lib.gl_batch_map_adam_step(
-                self._h, _ptr(z), _ptr(m), _ptr(v), optimizer.iterations + 1, float(optimizer.lr(optimizer.iterations)),
-                float(optimizer.beta_1), float(optimizer.beta_2), float(optimizer.epsilon), _ptr(lp), _ptr(chi),
-                _stream()), "gl_batch_map_adam_step")
-            optimizer.iterations += 1
+
+        def steps():
+            for s in range(num_steps):
+                rec = s % record_every == 0
+                lp = lp_h[s // record_every] if rec else scratch[0]
+                chi = chi_h[s // record_every] if rec else scratch[1]
+                nat.check(self._lib.gl_batch_map_adam_step(
+                    self._h, _ptr(z), _ptr(m), _ptr(v), optimizer.iterations + 1,
+                    float(optimizer.lr(optimizer.iterations)), float(optimizer.beta_1), float(optimizer.beta_2),
+                    float(optimizer.epsilon), _ptr(lp), _ptr(chi), _stream()), "gl_batch_map_adam_step")
+                optimizer.iterations += 1
+        if graph:
+            from .graphs import capture
+            capture(steps).replay()
+        else:
+            steps()
         self.lp_history, self.chi2_history = lp_h, chi_h
         return z
 

@@ -170,13 +170,15 @@ class ModellingSequence(ModellingSequenceInterface):
 
     # ---- MAP ----------------------------------------------------------------
     def MAP(self, optimizer=None, start=None, n_samples=500, num_steps=350, seed=0, progress=False,
-            record_every=1):
+            record_every=1, graph=False):
         """Multi-start MAP (tf/inference.py:17-41): returns unconstrained z (n, d).
 
         With a Keras-style ``Adam`` each step is one fused launch (log_prob,
         adjoint, Adam update).  ``chi2_history`` and ``lp_history`` hold the
         per-particle mean r^2 and log posterior of every recorded step (device
-        tensors, values before that step's update)."""
+        tensors, values before that step's update).  ``graph=True`` records the
+        ``num_steps`` launches (with their learning rates) into one HIP graph
+        and replays it: the same kernels and results without per-step host work."""
         optimizer = optimizer if optimizer is not None else Adam(1e-2)
         if start is None:
             start = self.prob_model.prior.sample(n_samples, seed=seed)
@@ -192,20 +194,26 @@ class ModellingSequence(ModellingSequenceInterface):
         hist = torch.empty((nrec, n), device=self.device)
         lp_hist = torch.empty((nrec, n), device=self.device)
         scratch = torch.empty((2, n), device=self.device)
-        it = range(num_steps)
-        if progress:
-            from tqdm.auto import trange
-            it = trange(num_steps)
-        for s in it:
-            step = optimizer.iterations + 1
-            rec = s % record_every == 0
-            row = hist[s // record_every] if rec else scratch[0]
-            lp = lp_hist[s // record_every] if rec else scratch[1]
-            plan.map_adam_step(z, m, v, step, optimizer.lr(optimizer.iterations), optimizer.beta_1,
-                               optimizer.beta_2, optimizer.epsilon, lp=lp, chi2=row)
-            optimizer.iterations += 1
-            if progress and s % 10 == 0:
-                it.set_description(f"Chi Squared: {float(torch.nan_to_num(row, nan=float('inf')).min()):.4f}")
+        def steps(it):
+            for s in it:
+                step = optimizer.iterations + 1
+                rec = s % record_every == 0
+                row = hist[s // record_every] if rec else scratch[0]
+                lp = lp_hist[s // record_every] if rec else scratch[1]
+                plan.map_adam_step(z, m, v, step, optimizer.lr(optimizer.iterations), optimizer.beta_1,
+                                   optimizer.beta_2, optimizer.epsilon, lp=lp, chi2=row)
+                optimizer.iterations += 1
+                if progress and s % 10 == 0:
+                    it.set_description(f"Chi Squared: {float(torch.nan_to_num(row, nan=float('inf')).min()):.4f}")
+        if graph:
+            from .graphs import capture
+            capture(lambda: steps(range(num_steps))).replay()
+        else:
+            it = range(num_steps)
+            if progress:
+                from tqdm.auto import trange
+                it = trange(num_steps)
+            steps(it)
         self.chi2_history = hist
         self.lp_history = lp_hist
         return z
