@@ -449,7 +449,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-prior time-to-MAP evaluation")
     ap.add_argument("--learned-images", type=int, default=2 ** 20)
     ap.add_argument("--learned-lenses", type=int, default=256)
-    ap.add_argument("--learned-lr", type=float, default=1e-3)
+    ap.add_argument("--learned-lr", type=float, default=2e-3)
+    ap.add_argument("--learned-epochs", type=int, default=4)
     ap.add_argument("--hmc", action="store_true", help="shapelets: also MAP + 256-chain HMC (1,000 leapfrog steps)")
     ap.add_argument("--learned-schedule", choices=["constant", "cosine"], default="constant")
     args = ap.parse_args()
@@ -586,8 +587,9 @@ def main():
 
 
 def learned_time_to_map(prior, phys, cfg, dev, args):
-    """Learned initialiser: trained on ``--learned-images`` simulated images
-    (batch 1024, AdamW lr ``--learned-lr``, seed 10), then held-out lenses
+    """Learned initialiser: trained on ``--learned-images`` prior draws simulated
+    on the device, ``--learned-epochs`` passes with fresh noise each (batch 1024,
+    AdamW lr ``--learned-lr``, seed 10), then held-out lenses
     (seeds 1000..) fitted by 1,000-particle MAP from screened network
     proposals, from prior draws and from screened prior draws (one multi-lens
     launch per step for each)."""
@@ -596,11 +598,13 @@ def learned_time_to_map(prior, phys, cfg, dev, args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     losses = init.train(n_images=args.learned_images, batch=1024, lr=args.learned_lr, weight_decay=0.01, seed=10,
-                        schedule=args.learned_schedule)
+                        schedule=args.learned_schedule, epochs=args.learned_epochs)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     r = compare_initialisation(init, n_lenses=args.learned_lenses, n_particles=1000, num_steps=350, lr=args.lr)
-    r["training"] = {"images": args.learned_images, "batch": 1024, "lr": args.learned_lr,
+    r["training"] = {"images": args.learned_images, "epochs": args.learned_epochs,
+                     "note": "each epoch re-simulates the same prior draws with fresh noise",
+                     "batch": 1024, "lr": args.learned_lr,
                      "schedule": args.learned_schedule, "seconds": t_train,
                      "nll_first10": float(np.mean(losses[:10])), "nll_last10": float(np.mean(losses[-10:]))}
     return r

@@ -174,14 +174,16 @@ class LearnedInitializer:
         self.losses: List[float] = []
 
     # ---- data ------------------------------------------------------------------
-    def simulate_batch(self, n: int, seed: int):
-        """n prior draws: noisy images (n, N, N) on the device and their z (n, d)."""
+    def simulate_batch(self, n: int, seed: int, noise_seed: Optional[int] = None):
+        """n prior draws (seed): noisy images (n, N, N) on the device and their z
+        (n, d); the noise realisation follows ``noise_seed`` (default: seed)."""
         x_nested = self.prior.sample(n, seed=seed)
         z = self.bij.inverse(x_nested).to(self.device, torch.float32)
         x = _flat(self.prior, x_nested).to(self.device).contiguous()
         img = self.sim.plan.simulate(x)
         g = torch.Generator(device=self.device)
-        g.manual_seed(int(seed) * 2654435761 % (2 ** 62))
+        ns = seed if noise_seed is None else noise_seed
+        g.manual_seed(int(ns) * 2654435761 % (2 ** 62))
         lam = torch.clamp(img, min=0.0) * self.exp_time
         noisy = torch.poisson(lam, generator=g) / self.exp_time
         noisy = noisy + self.background_rms * torch.randn(img.shape, device=self.device, generator=g)
@@ -192,15 +194,21 @@ class LearnedInitializer:
 
     # ---- training --------------------------------------------------------------
     def train(self, n_images: int = 2 ** 20, batch: int = 1024, lr: float = 3e-4, weight_decay: float = 0.01,
-              seed: int = 10, log_every: int = 0, schedule: str = "constant", warmup: int = 50) -> List[float]:
-        """AdamW on the Gaussian NLL, fresh simulated batches (seeds seed, seed+1, ...).
+              seed: int = 10, log_every: int = 0, schedule: str = "constant", warmup: int = 50,
+              epochs: int = 1) -> List[float]:
+        """AdamW on the Gaussian NLL over ``n_images`` prior draws (batch k uses
+        prior seed seed*1000003 + k), simulated on the device; each of the
+        ``epochs`` passes draws fresh noise for the same lens configurations.
         ``schedule`` "cosine": linear warm-up over ``warmup`` steps to ``lr``,
         then cosine decay to 0 at the last step."""
         batch = min(int(batch), self.max_batch)
-        steps = max(1, int(n_images) // batch)
+        per_epoch = max(1, int(n_images) // batch)
+        steps = per_epoch * max(1, int(epochs))
         loss_acc = []
         for s in range(1, steps + 1):
-            images, z = self.simulate_batch(batch, seed=seed * 1_000_003 + s)
+            k, e = (s - 1) % per_epoch + 1, (s - 1) // per_epoch
+            images, z = self.simulate_batch(batch, seed=seed * 1_000_003 + k,
+                                            noise_seed=(seed * 1_000_003 + k) * 7919 + e)
             loss, _ = self.net.loss_grad(images, self.standardise(z))
             lr_s = lr
             if schedule == "cosine":
