@@ -412,13 +412,14 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
 
 // Work-unit totals of a lens with `count` particles (the prefix increments of a table).
 struct McUnits {
-    long part, basis, conv, adj;
+    long part, basis, conv, adj, grad;
 };
 static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     McUnits u;
     const long njb = (P.N + 63) / 64;
     u.part = count;
     u.basis = (long)count * ((P.nsub * P.nsub + 255) / 256);
+    u.grad = (long)count * (((long)P.nsub * P.nsub + 256 * gl::kMcGradK - 1) / (256 * gl::kMcGradK));
     u.conv = (long)count * M.D * ((P.N + gl::kMcIB - 1) / gl::kMcIB) * njb;
     u.adj = (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
     return u;
@@ -447,9 +448,9 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
         hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), 0, st, T);
         GL_HIP(hipGetLastError());
         if (shp)
-            hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
+            hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
         else
-            hipLaunchKernelGGL((gl::mc_grad<MULTI, false>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
+            hipLaunchKernelGGL((gl::mc_grad<MULTI, false>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
         GL_HIP(hipGetLastError());
         hipLaunchKernelGGL((gl::mc_finish<MULTI>), dim3(np), dim3(128), 0, st, T, A);
         GL_HIP(hipGetLastError());
@@ -493,6 +494,7 @@ static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStrea
         if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX)
             return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
         T.tot_part = nb; T.tot_basis = (int)u.basis; T.tot_conv = (int)u.conv; T.tot_adj = (int)u.adj;
+        T.tot_grad = (int)u.grad;
         T.lds_consts = M.pstride * 4;
         int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
                                     grad, st, M.shp_comp >= 0);
@@ -1017,8 +1019,9 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         if (plans[i]->P.ss != b->ss_launch) b->ss_launch = 0;  // mixed supersampling: runtime ss
     }
     std::vector<gl::McLens> lens(n);
-    std::vector<int> pre_part(n + 1, 0), pre_basis(n + 1, 0), pre_conv(n + 1, 0), pre_adj(n + 1, 0);
-    long acc[4] = {0, 0, 0, 0};
+    std::vector<int> pre_part(n + 1, 0), pre_basis(n + 1, 0), pre_conv(n + 1, 0), pre_adj(n + 1, 0),
+        pre_grad(n + 1, 0);
+    long acc[5] = {0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         gl::McLens& E = lens[i];
         memset(&E, 0, sizeof(E));
@@ -1030,13 +1033,14 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         E.count = counts[i];
         E.weight = 1.0f / (float)counts[i];
         const McUnits u = mc_units(E.P, E.M, counts[i]);
-        acc[0] += u.part; acc[1] += u.basis; acc[2] += u.conv; acc[3] += u.adj;
-        for (int k = 0; k < 4; ++k)
+        acc[0] += u.part; acc[1] += u.basis; acc[2] += u.conv; acc[3] += u.adj; acc[4] += u.grad;
+        for (int k = 0; k < 5; ++k)
             if (acc[k] > INT32_MAX) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: batch too large"); }
         pre_part[i + 1] = (int)acc[0];
         pre_basis[i + 1] = (int)acc[1];
         pre_conv[i + 1] = (int)acc[2];
         pre_adj[i + 1] = (int)acc[3];
+        pre_grad[i + 1] = (int)acc[4];
     }
     b->total = (int)acc[0];
     b->counts.assign(counts, counts + n);
@@ -1057,12 +1061,14 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     if (!rc) b->T.pre_basis = (const int*)put(pre_basis.data(), pre_basis.size() * sizeof(int));
     if (!rc) b->T.pre_conv = (const int*)put(pre_conv.data(), pre_conv.size() * sizeof(int));
     if (!rc) b->T.pre_adj = (const int*)put(pre_adj.data(), pre_adj.size() * sizeof(int));
+    if (!rc) b->T.pre_grad = (const int*)put(pre_grad.data(), pre_grad.size() * sizeof(int));
     if (rc) { gl_batch_destroy(b); return rc; }
     b->T.n = n;
     b->T.tot_part = (int)acc[0];
     b->T.tot_basis = (int)acc[1];
     b->T.tot_conv = (int)acc[2];
     b->T.tot_adj = (int)acc[3];
+    b->T.tot_grad = (int)acc[4];
     b->T.lds_consts = 0;
     for (int i = 0; i < n; ++i) b->T.lds_consts = std::max(b->T.lds_consts, lens[i].M.pstride * 4);
     if (b->T.lds_consts > 64 * 1024) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: constant block too large"); }

@@ -123,7 +123,8 @@ struct McTable {
     const int* pre_basis;  // 256-thread blocks of sub-pixels (mc_basis, mc_grad)
     const int* pre_conv;   // conv waves
     const int* pre_adj;    // adjoint waves (a multiple of 4 per particle)
-    int tot_part, tot_basis, tot_conv, tot_adj;
+    const int* pre_grad;   // mc_grad blocks (256 x kMcGradK sub-pixels each)
+    int tot_part, tot_basis, tot_conv, tot_adj, tot_grad;
     int lds_consts;        // bytes of LDS for one particle's constant block (max over lenses)
 };
 
@@ -150,6 +151,8 @@ __device__ __forceinline__ const McLens& mc_locate(const McTable& T, const int* 
 }
 
 __device__ __forceinline__ int mc_bpp(const DPlan& P) { return (P.nsub * P.nsub + 255) / 256; }
+constexpr int kMcGradK = 8;  // sub-pixels per thread in mc_grad
+__device__ __forceinline__ int mc_gpp(const DPlan& P) { return (P.nsub * P.nsub + 256 * kMcGradK - 1) / (256 * kMcGradK); }
 
 // Copy particle p's constant block (pstride floats: prior terms, component
 // constants, EPL coefficient tables) into LDS; every thread of the block calls
@@ -781,16 +784,21 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
 // Statistics layout: per lens slot 8 (EPL 8 / shear 2 / SIS 3), per light slot 8
 // (Sersic) or 3 (shapelets: sum g L_u, sum g L_v, sum g (u L_u + v L_v)).
 // ---------------------------------------------------------------------------
-// SHP: the plan has a shapelet source (otherwise that branch is compiled out)
+// SHP: the plan has a shapelet source (otherwise that branch is compiled out).
+// Each thread accumulates kMcGradK sub-pixels (256 apart) before the block
+// reduction, so the 8-per-component statistics are reduced once per 2,048
+// sub-pixels.  A sub-pixel whose forward value was zeroed for NaN contributes
+// nothing: per channel with least squares (the channel was zeroed), the whole
+// sub-pixel with fixed amplitudes (their sum was zeroed, tf/simulator.py:81).
 template <bool MULTI, bool SHP>
 __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
     int local;
-    const McLens& E = mc_locate<MULTI>(T, T.pre_basis, blockIdx.x, local);
+    const McLens& E = mc_locate<MULTI>(T, T.pre_grad, blockIdx.x, local);
     const DPlan& P = E.P;
     const McPlan& M = E.M;
-    const int bpp = mc_bpp(P);
-    const int p = local / bpp;
-    const int s = (local - p * bpp) * 256 + threadIdx.x;
+    const int gpp = mc_gpp(P);
+    const int p = local / gpp;
+    const int blk = local - p * gpp;
     const int nsub = P.nsub;
     float st[kMcStat];
 #pragma unroll
@@ -799,7 +807,9 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
     const float* pc = mc_stage_consts(M, p, reinterpret_cast<float*>(mc_dyn4));
     const float* s_c = pc + ((6 * P.d + 3) & ~3);
     const float* cp = M.coef + (size_t)p * M.D;
-    if (s < nsub * nsub) {
+    for (int it = 0; it < kMcGradK; ++it) {
+        const int s = (blk * kMcGradK + it) * 256 + threadIdx.x;
+        if (s >= nsub * nsub) break;
         const int y = s / nsub, x = s - (s / nsub) * nsub;
         const float g = M.B[((size_t)p * M.D * nsub + y) * M.BW + P.pb + x];
         const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
@@ -808,8 +818,17 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
         float sis_st[3][3];
         float bx, by;
         mc_beta<true>(P, M, pc, X, Y, bx, by, es, sis_st);
-        // forward value (NaN sub-pixels contribute nothing) and light adjoints
-        float val = 0.0f;
+        if (!M.lstsq) {
+            // the sum of the lights: a NaN zeroed the whole sub-pixel in the forward pass
+            float v = 0.0f;
+            for (int i = P.nlens; i < P.ncomp; ++i) {
+                const glm::SerC C = *reinterpret_cast<const glm::SerC*>(s_c + i * kConstStride);
+                const bool src = P.comp[i].role == GL_ROLE_SOURCE;
+                glm::SerState ss0;
+                v += glm::sersic_forward(C, src ? bx : X, src ? by : Y, ss0);
+            }
+            if (isnan(v)) continue;
+        }
         int ch = 0;
         float gbx = 0.0f, gby = 0.0f;
 #pragma unroll
@@ -825,20 +844,11 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
                 const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
                 glm::SerState ss_;
                 const float v = glm::sersic_forward(C, ux, uy, ss_);
-                if (!M.lstsq) {
-                    // one channel: a NaN light zeroes the whole sub-pixel (via val below)
-                    val += v;
-                    float gdx, gdy;
-                    glm::sersic_adjoint(C, ss_, g * cp[0], acc, gdx, gdy);
-                    if (src) { gbx += gdx; gby += gdy; }
-                    continue;
-                }
-                if (isnan(v)) { ++ch; continue; }
-                val += cp[ch] * v;
+                const int c = M.lstsq ? ch++ : 0;
+                if (isnan(v)) continue;  // least squares: this channel was zeroed
                 float gdx, gdy;
-                glm::sersic_adjoint(C, ss_, g * cp[ch], acc, gdx, gdy);
+                glm::sersic_adjoint(C, ss_, g * cp[c], acc, gdx, gdy);
                 if (src) { gbx += gdx; gby += gdy; }
-                ++ch;
             } else if (SHP && cd.kind == glm::K_SHAPELETS) {
                 const float beta = cst[0];
                 const float ib = 1.0f / beta;
@@ -847,48 +857,42 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
                 float du[glm::kMaxShapeletN + 1], dw[glm::kMaxShapeletN + 1];
                 mc_phi(M, u, pu, du);
                 mc_phi(M, w, pw, dw);
-                float L = 0.0f, Lu = 0.0f, Lw = 0.0f;
+                float Lu = 0.0f, Lw = 0.0f;
 #pragma unroll
                 for (int so = 0; so <= glm::kMaxShapeletN; ++so) {
                     if (so > M.shp_nmax) break;
 #pragma unroll
                     for (int j = 0; j <= so; ++j) {
                         const float a = cp[ch + so * (so + 1) / 2 + j];
-                        L = fmaf(a, pu[so - j] * pw[j], L);
                         Lu = fmaf(a, du[so - j] * pw[j], Lu);
                         Lw = fmaf(a, pu[so - j] * dw[j], Lw);
                     }
                 }
                 ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
-                val += L;
+                if (isnan(u) || isnan(w)) continue;  // the basis was zeroed
                 acc[0] += g * Lu;
                 acc[1] += g * Lw;
                 acc[2] += g * (u * Lu + w * Lw);
                 if (src) { gbx += g * Lu * ib; gby += g * Lw * ib; }
             }
         }
-        if (!isnan(val)) {
-            const float Hx = -gbx, Hy = -gby;
+        if (isnan(bx) || isnan(by)) continue;  // no deflection adjoint through a NaN ray
+        const float Hx = -gbx, Hy = -gby;
 #pragma unroll
-            for (int i = 0; i < kMaxLens; ++i) {
-                if (i >= P.nlens) break;
-                const int kind = P.comp[i].kind;
-                const float* cst = s_c + i * kConstStride;
-                float* acc = st + 8 * i;
-                if (kind == glm::K_EPL) {
-                    const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
-                    glm::epl_adjoint(C, es[i], Hx, Hy, acc);
-                } else if (kind == glm::K_SHEAR) {
-                    glm::shear_adjoint(X, Y, Hx, Hy, acc);
-                } else if (kind == glm::K_SIS) {
-                    const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
-                    glm::sis_adjoint(C, sis_st[i], Hx, Hy, acc);
-                }
+        for (int i = 0; i < kMaxLens; ++i) {
+            if (i >= P.nlens) break;
+            const int kind = P.comp[i].kind;
+            const float* cst = s_c + i * kConstStride;
+            float* acc = st + 8 * i;
+            if (kind == glm::K_EPL) {
+                const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
+                glm::epl_adjoint(C, es[i], Hx, Hy, acc);
+            } else if (kind == glm::K_SHEAR) {
+                glm::shear_adjoint(X, Y, Hx, Hy, acc);
+            } else if (kind == glm::K_SIS) {
+                const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
+                glm::sis_adjoint(C, sis_st[i], Hx, Hy, acc);
             }
-        } else {
-            // a NaN light: drop this sub-pixel's contributions entirely
-#pragma unroll
-            for (int e = 0; e < kMcStat; ++e) st[e] = 0.0f;
         }
     }
     __shared__ float red[kMcStat][4];
