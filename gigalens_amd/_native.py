@@ -99,6 +99,7 @@ class PlanDesc(ctypes.Structure):
         ("noise_map", ctypes.POINTER(ctypes.c_float)),
         ("n_observed", ctypes.c_int),
         ("staged", ctypes.c_int),
+        ("conv_rows", ctypes.c_int),
     ]
 
 

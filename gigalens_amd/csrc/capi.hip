@@ -188,14 +188,21 @@ extern "C" int gl_plan_destroy(gl_plan* plan) {
 static int upload_common(const gl_plan_desc* desc, gl_plan* p);
 
 template <int SS, int KF4, bool MULTI>
-static hipError_t launch_mc_conv(int grid, hipStream_t st, const gl::McTable& T) {
-    hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI>), dim3(grid), dim3(256), 0, st, T);
+static hipError_t launch_mc_conv(int ib, int grid, hipStream_t st, const gl::McTable& T) {
+    if (MULTI || ib == 4)
+        hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI, 4>), dim3(grid), dim3(256), 0, st, T);
+    else if (ib == 2)
+        hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI, 2>), dim3(grid), dim3(256), 0, st, T);
+    else if (ib == 8)
+        hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI, 8>), dim3(grid), dim3(256), 0, st, T);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 template <bool MULTI>
-static hipError_t mc_conv_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::McTable& T) {
-#define GL_MCC(S, K) if (ss == S && kf4 == K) return launch_mc_conv<S, K, MULTI>(grid, st, T)
+static hipError_t mc_conv_dispatch(int ss, int kf4, int ib, int grid, hipStream_t st, const gl::McTable& T) {
+#define GL_MCC(S, K) if (ss == S && kf4 == K) return launch_mc_conv<S, K, MULTI>(ib, grid, st, T)
     GL_MCC(0, 8); GL_MCC(0, 16); GL_MCC(0, 22); GL_MCC(0, 32);
     GL_MCC(1, 8); GL_MCC(1, 16); GL_MCC(1, 22); GL_MCC(1, 32);
     GL_MCC(2, 8); GL_MCC(2, 16); GL_MCC(2, 22); GL_MCC(2, 32);
@@ -296,6 +303,9 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     gl::McPlan& M = p->M;
     memset(&M, 0, sizeof(M));
     M.lstsq = lstsq;
+    M.conv_ib = desc->conv_rows > 0 ? desc->conv_rows : gl::kMcIB;
+    if (M.conv_ib != 2 && M.conv_ib != 4 && M.conv_ib != 8)
+        return fail(GL_ERR_ARG, "conv_rows must be 0, 2, 4 or 8");
     const int N = P.N, ss = P.ss, nsub = P.nsub, kf = P.kf, pb = P.pb;
     int rc;
     void* dptr = nullptr;
@@ -420,7 +430,7 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     u.part = count;
     u.basis = (long)count * ((P.nsub * P.nsub + 255) / 256);
     u.grad = (long)count * (((long)P.nsub * P.nsub + 256 * gl::kMcGradK - 1) / (256 * gl::kMcGradK));
-    u.conv = (long)count * M.D * ((P.N + gl::kMcIB - 1) / gl::kMcIB) * njb;
+    u.conv = (long)count * M.D * ((P.N + M.conv_ib - 1) / M.conv_ib) * njb;
     u.adj = (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
     return u;
 }
@@ -434,7 +444,7 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
     GL_HIP(hipGetLastError());
     hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
     GL_HIP(hipGetLastError());
-    GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, (T.tot_conv + 3) / 4, st, T));
+    GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, MULTI ? 4 : T.one.M.conv_ib, (T.tot_conv + 3) / 4, st, T));
     if (lstsq) {
         if (MULTI) return fail(GL_ERR_UNSUPPORTED, "least-squares lenses are not batched");
         const int ks = std::max(1, std::min(ksplit_cap, (1024 + np - 1) / np));
@@ -1027,6 +1037,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         memset(&E, 0, sizeof(E));
         E.P = plans[i]->P;
         E.M = plans[i]->M;
+        E.M.conv_ib = gl::kMcIB;  // batches launch the conv at the default rows per lane
         int rc = mc_workspace(E.P, E.M, counts[i], b->kf4, 1, plans[i]->W_host, b->dev);
         if (rc) { gl_batch_destroy(b); return rc; }
         E.first = (int)acc[0];

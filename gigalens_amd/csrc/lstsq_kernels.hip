@@ -49,10 +49,11 @@ constexpr int kMcMaxChan = 79;
 constexpr int kMcGramMaxNB = 5;
 constexpr int kMcGramMaxSplit = 8;  // pixel-range splits per particle (x 4 waves = partials)
 constexpr int kMcStat = 64;  // statistics floats per particle
-constexpr int kMcIB = 4;     // output rows per conv lane
+constexpr int kMcIB = 4;     // default output rows per conv lane (McPlan.conv_ib: 2, 4 or 8)
 
 struct McPlan {
     int lstsq;          // 1: linear amplitudes solved by least squares; 0: fixed amplitudes (one channel)
+    int conv_ib;        // output rows per lane of mc_conv (a tuning choice: 2, 4 or 8)
     int D;              // channels
     int BW;             // staged row width (floats): pb + nsub + right margin, multiple of 4
     int pstride;        // floats of per-particle constants
@@ -371,10 +372,10 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
 }
 
 // ---------------------------------------------------------------------------
-// mc_conv: one wave = (particle, channel, kMcIB output rows, 64 output columns)
+// mc_conv: one wave = (particle, channel, IB output rows, 64 output columns)
 // ---------------------------------------------------------------------------
 // SS = 0: the supersampling factor is read from the plan (lenses of several ss)
-template <int SS, int KF4, bool MULTI>
+template <int SS, int KF4, bool MULTI, int IB>
 __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int gtask = blockIdx.x * 4 + wave;
@@ -385,19 +386,19 @@ __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
     const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, kf = P.kf, pb = P.pb;
     const int ss = SS ? SS : P.ss;
-    const int nib = (N + kMcIB - 1) / kMcIB, njb = (N + 63) / 64;
+    const int nib = (N + IB - 1) / IB, njb = (N + 63) / 64;
     const int jb = task % njb; task /= njb;
     const int ib = task % nib; task /= nib;
     const int c = task % M.D;
     const int p = task / M.D;
     const int J = jb * 64 + lane;
-    const int I0 = ib * kMcIB;
+    const int I0 = ib * IB;
     const bool active = J < N;
     const float* Bc = M.B + ((size_t)p * M.D + c) * nsub * M.BW;
-    float acc[kMcIB];
+    float acc[IB];
 #pragma unroll
-    for (int i = 0; i < kMcIB; ++i) acc[i] = 0.0f;
-    const int nrows = ss * (kMcIB - 1) + kf;
+    for (int i = 0; i < IB; ++i) acc[i] = 0.0f;
+    const int nrows = ss * (IB - 1) + kf;
     for (int r = 0; r < nrows; ++r) {
         const int y = ss * I0 + r - pb;
         if (y < 0 || y >= nsub) continue;
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
             for (int j = 0; j < 4 * KF4; ++j) v[j] = 0.0f;
         }
 #pragma unroll
-        for (int i = 0; i < kMcIB; ++i) {
+        for (int i = 0; i < IB; ++i) {
             const int pp = r - ss * i;
             if (pp >= 0 && pp < kf) {
                 const float* w = M.wpad + pp * M.wpad_w;
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
     if (active) {
         float* Xc = M.X + ((size_t)p * M.D + c) * N * N;
 #pragma unroll
-        for (int i = 0; i < kMcIB; ++i) {
+        for (int i = 0; i < IB; ++i) {
             const int I = I0 + i;
             if (I < N) {
                 const float v = acc[i];

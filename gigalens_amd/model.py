@@ -8,6 +8,7 @@ storing per-particle grids.
 """
 from __future__ import annotations
 
+import os
 from abc import ABC, abstractmethod
 from typing import List
 
@@ -71,15 +72,24 @@ class _ProbModelBase(ProbabilisticModel):
         self._paths = [p for p, _ in dist.leaves(prior)]
         self._plans = {}
 
+    #: pick the fastest execution variant of each new plan by timing (autotune.py); also
+    #: switched on by the environment variable GIGALENS_AMD_AUTOTUNE=1
+    autotune = False
+
     def plan_for(self, simulator, epl_batch_series=False) -> Plan:
         key = (id(simulator), bool(epl_batch_series))
         if key not in self._plans:
             lay = layout_from_paths(simulator.phys_model, self._paths)
-            self._plans[key] = (simulator, Plan(simulator.phys_model, simulator.camera, lay, prior=self.prior,
-                                                observed=self.observed_image.cpu().numpy(),
-                                                background_rms=self.background_rms, exp_time=self.exp_time,
-                                                likelihood=self._likelihood, fldj_at_x=self._fldj_at_x,
-                                                epl_batch_series=epl_batch_series))
+            kw = dict(prior=self.prior, observed=self.observed_image.cpu().numpy(),
+                      background_rms=self.background_rms, exp_time=self.exp_time, likelihood=self._likelihood,
+                      fldj_at_x=self._fldj_at_x, epl_batch_series=epl_batch_series)
+            variant = {}
+            if self.autotune or os.environ.get("GIGALENS_AMD_AUTOTUNE") == "1":
+                from .autotune import tune
+                bs = max(1, min(int(getattr(simulator, "bs", 256)), 1024))
+                z = self.bij.inverse(self.prior.sample(bs, seed=12345)).to(simulator.device, torch.float32)
+                variant, _ = tune(simulator.phys_model, simulator.camera, lay, z.contiguous(), kw)
+            self._plans[key] = (simulator, Plan(simulator.phys_model, simulator.camera, lay, **kw, **variant))
         return self._plans[key][1]
 
     def log_prob(self, simulator, z):

@@ -98,7 +98,7 @@ def _lens(t: torch.Tensor, bs: int) -> torch.Tensor:
 class Plan:
     def __init__(self, phys_model, camera, layout: Layout, prior=None, observed=None, background_rms=1.0,
                  exp_time=1.0, likelihood=nat.GL_LIK_FORWARD, fldj_at_x=False, epl_batch_series=False,
-                 psf_rank_tol=0.0, max_batch=0, noise_map=None, staged=False):
+                 psf_rank_tol=0.0, max_batch=0, noise_map=None, staged=False, conv_rows=0):
         self.layout = layout
         self.d = len(layout)
         self.N = camera.num_pix
@@ -179,6 +179,7 @@ class Plan:
         desc.psf_rank_tol = float(psf_rank_tol)
         desc.max_batch = int(max_batch)
         desc.staged = int(bool(staged))
+        desc.conv_rows = int(conv_rows)
         if noise_map is not None:
             nm = np.ascontiguousarray(np.asarray(noise_map, dtype=np.float32))
             if nm.size != self.n_observed * self.N * self.N:

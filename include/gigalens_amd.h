@@ -105,6 +105,8 @@ typedef struct gl_plan_desc {
     int staged;               /* 1: always use the HBM-staged pipeline (otherwise it is chosen when
                                  the LDS-resident kernel cannot hold the image, has no compiled
                                  kernel for the profile combination, or lights are linear) */
+    int conv_rows;            /* staged plans: output rows per lane of the channel convolution
+                                 (2, 4 or 8; 0 selects 4) -- a tuning choice, same results */
 } gl_plan_desc;
 
 typedef struct gl_plan gl_plan;

@@ -156,3 +156,20 @@ def test_uneven_batch_matches_per_lens_plans(cuda):
         _print(f"lens {k}: MAP max |z diff| {e:.2e} after 5 steps of lr 5e-3")
         assert e < 0.1 * 5 * 5e-3  # gradients agree to ~1e-6 (atomic-add order); Adam steps ~lr sign(g)
         off += n
+
+
+def test_conv_rows_variants_agree(cuda):
+    """The staged conv's rows-per-lane tuning choice (2, 4, 8) does not change results."""
+    prob = C.epl_shear_problem(bs=8, seed=12)
+    prior, phys = prob["prior"], prob["phys"]
+    paths = [p for p, _ in tfd.leaves(prior)]
+    pm = ForwardProbModel(prior, prob["obs"], 0.2, 100.0)
+    z = pm.bij.inverse(C.nested_from_x(prior, prob["x"])).to(cuda).contiguous()
+    out = {}
+    for r in (2, 4, 8):
+        plan = Plan(phys, prob["cam"], layout_from_paths(phys, paths), prior=prior, observed=prob["obs"],
+                    background_rms=0.2, exp_time=100.0, staged=True, conv_rows=r)
+        out[r] = plan.log_prob(z, want_grad=True)
+    for r in (2, 8):
+        assert torch.equal(out[r][0], out[4][0]), r
+        assert float((out[r][2] - out[4][2]).abs().max() / out[4][2].abs().max()) < 1e-5
