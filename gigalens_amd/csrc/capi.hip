@@ -312,6 +312,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     if (ss > 4) return fail(GL_ERR_UNSUPPORTED, "the staged path supports supersample <= 4");
     p->kf4 = pick_kf4(kf);
     if (p->kf4 < 0) return fail(GL_ERR_UNSUPPORTED, "PSF too large for the staged path (kf > 128)");
+    if (kf * kf * 4 > 64 * 1024) return fail(GL_ERR_UNSUPPORTED, "folded PSF too large for the staged adjoint (kf > 128)");
     if (P.ncomp > gl::kMaxComp) return fail(GL_ERR_UNSUPPORTED, "too many components");
     // channels
     int D = 0, nshp = 0;
@@ -455,7 +456,7 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
     hipLaunchKernelGGL((gl::mc_image<MULTI>), dim3(np), dim3(256), 0, st, T, A, grad ? 1 : 0);
     GL_HIP(hipGetLastError());
     if (grad) {
-        hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), 0, st, T);
+        hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), T.lds_w, st, T);
         GL_HIP(hipGetLastError());
         if (shp)
             hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
@@ -506,6 +507,7 @@ static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStrea
         T.tot_part = nb; T.tot_basis = (int)u.basis; T.tot_conv = (int)u.conv; T.tot_adj = (int)u.adj;
         T.tot_grad = (int)u.grad;
         T.lds_consts = M.pstride * 4;
+        T.lds_w = P.kf * P.kf * 4;
         int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
                                     grad, st, M.shp_comp >= 0);
         if (rc) return rc;
@@ -1081,7 +1083,11 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     b->T.tot_adj = (int)acc[3];
     b->T.tot_grad = (int)acc[4];
     b->T.lds_consts = 0;
-    for (int i = 0; i < n; ++i) b->T.lds_consts = std::max(b->T.lds_consts, lens[i].M.pstride * 4);
+    b->T.lds_w = 0;
+    for (int i = 0; i < n; ++i) {
+        b->T.lds_consts = std::max(b->T.lds_consts, lens[i].M.pstride * 4);
+        b->T.lds_w = std::max(b->T.lds_w, lens[i].P.kf * lens[i].P.kf * 4);
+    }
     if (b->T.lds_consts > 64 * 1024) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: constant block too large"); }
     if (hipDeviceSynchronize() != hipSuccess) { gl_batch_destroy(b); return fail(GL_ERR_HIP, "gl_batch_create: sync"); }
     *out = b;

@@ -127,6 +127,7 @@ struct McTable {
     const int* pre_grad;   // mc_grad blocks (256 x kMcGradK sub-pixels each)
     int tot_part, tot_basis, tot_conv, tot_adj, tot_grad;
     int lds_consts;        // bytes of LDS for one particle's constant block (max over lenses)
+    int lds_w;             // bytes of LDS for the folded weights kf x kf (max over lenses)
 };
 
 // lens holding unit x of a stage, and x's offset inside that lens.  MULTI =
@@ -742,19 +743,25 @@ __global__ __launch_bounds__(256) void mc_image(const McTable T, const McArgs A,
 
 // ---------------------------------------------------------------------------
 // mc_adjoint: transposed correlation of dL/dimage onto the sub-grid.
-// block 256 = 4 waves; one wave per (particle, row y, column phase b, 64 columns
-// X); x = ss X + b, so the weights are wave-uniform.  Each particle owns a
-// multiple of 4 waves, so a block never spans two particles.
+// block 256 = 4 waves; one wave per (particle, sub-row y, column phase b, 64
+// columns Xc); lane = sub-pixel x = ss Xc + b.  For row phase a = y % ss the
+// contributing image rows are Yc + da - m (m < Ma) and, in each, the columns
+// Xc + db - nn (nn < Mb); the weights W[ss m + ra][ss nn + rb] are wave-uniform
+// and read from an LDS copy of the folded PSF.  Each particle owns a multiple
+// of 4 waves, so a block never spans two particles (or lenses).
 // ---------------------------------------------------------------------------
 template <bool MULTI>
 __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
-    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (gw >= T.tot_adj) return;
     int local;
-    const McLens& E = mc_locate<MULTI>(T, T.pre_adj, gw, local);
+    const McLens& E = mc_locate<MULTI>(T, T.pre_adj, blockIdx.x * 4, local);
     const DPlan& P = E.P;
     const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb;
+    extern __shared__ float4 mc_dyn4[];
+    float* s_w = reinterpret_cast<float*>(mc_dyn4);
+    for (int i = threadIdx.x; i < kf * kf; i += 256) s_w[i] = P.w[i];
+    __syncthreads();
+    local += threadIdx.x >> 6;
     const int njb = (N + 63) / 64, wpp4 = mc_adj_wpp4(P);
     const int p = local / wpp4;
     int t = local - p * wpp4;
@@ -763,25 +770,25 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
     const int b = t % ss; t /= ss;
     const int y = t;
     const int Xc = jb * 64 + (threadIdx.x & 63);
+    if (Xc >= N) return;
     const int a = y % ss, Yc = y / ss;
     const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
     const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
     const int GW = N + 2 * M.gp;
     const float* G = M.gimg + (size_t)p * GW * GW;
     float g = 0.0f;
-    if (Xc < N) {
-        for (int m = 0; m < Ma; ++m) {
-            const float* row = G + (Yc + da - m + M.gp) * GW + (Xc + db + M.gp);
-            const float* wr = P.w + (ss * m + ra) * kf + rb;
-            for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
-        }
-        M.B[((size_t)p * M.D * nsub + y) * M.BW + pb + ss * Xc + b] = g;
+    for (int m = 0; m < Ma; ++m) {
+        const float* row = G + (size_t)(Yc + da - m + M.gp) * GW + (Xc + db + M.gp);
+        const float* wr = s_w + (ss * m + ra) * kf + rb;
+        for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
     }
+    M.B[((size_t)p * M.D * nsub + y) * M.BW + pb + ss * Xc + b] = g;
 }
 
 // ---------------------------------------------------------------------------
-// mc_grad: grid (bs, ceil(nsub*nsub/256)), block 256.  Reverse-mode statistics
-// with the solved amplitudes; block-reduced, then atomically added per particle.
+// mc_grad: block 256, each block 256 x kMcGradK sub-pixels of one particle.
+// Reverse-mode statistics with the solved amplitudes; block-reduced, then
+// atomically added per particle.
 // Statistics layout: per lens slot 8 (EPL 8 / shear 2 / SIS 3), per light slot 8
 // (Sersic) or 3 (shapelets: sum g L_u, sum g L_v, sum g (u L_u + v L_v)).
 // ---------------------------------------------------------------------------
