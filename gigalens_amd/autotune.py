@@ -6,7 +6,8 @@ Every variant computes the same model (results agree to FP32 rounding):
   (``psf_rank_tol=0``) or the direct 2-D path (``psf_rank_tol=-1``), when the
   image fits in LDS;
 * the HBM-staged pipeline with 2, 4 or 8 output rows per lane in its channel
-  convolution (``staged=True, conv_rows=...``).
+  convolution (``staged=True, conv_rows=...``), or with its separable
+  channel convolution (``staged=True, psf_rank_tol=1e-6``).
 
 ``tune`` builds each candidate plan, warms it up and times ``log_prob`` with its
 gradient on a batch of the caller's z; the fastest configuration is kept per
@@ -62,6 +63,7 @@ def candidates(fused_fits: bool, lstsq: bool) -> List[dict]:
     if fused_fits and not lstsq:
         c += [{"psf_rank_tol": 0.0}, {"psf_rank_tol": -1.0}]
     c += [{"staged": True, "conv_rows": r} for r in (2, 4, 8)]
+    c += [{"staged": True, "psf_rank_tol": 1e-6}]
     return c
 
 

@@ -212,6 +212,25 @@ static hipError_t mc_conv_dispatch(int ss, int kf4, int ib, int grid, hipStream_
     return hipErrorInvalidValue;
 }
 
+template <int SS, int KF4>
+static hipError_t launch_mc_sep(int gh, int gv, hipStream_t st, const gl::McTable& T) {
+    hipLaunchKernelGGL((gl::mc_sep_h<SS, KF4>), dim3(gh), dim3(256), 0, st, T);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((gl::mc_sep_v<SS>), dim3(gv), dim3(256), 0, st, T);
+    return hipGetLastError();
+}
+
+static hipError_t mc_sep_dispatch(int ss, int kf4, int gh, int gv, hipStream_t st, const gl::McTable& T) {
+#define GL_MCS(S, K) if (ss == S && kf4 == K) return launch_mc_sep<S, K>(gh, gv, st, T)
+    GL_MCS(1, 8); GL_MCS(1, 16); GL_MCS(1, 22); GL_MCS(1, 32);
+    GL_MCS(2, 8); GL_MCS(2, 16); GL_MCS(2, 22); GL_MCS(2, 32);
+    GL_MCS(3, 8); GL_MCS(3, 16); GL_MCS(3, 22); GL_MCS(3, 32);
+    GL_MCS(4, 8); GL_MCS(4, 16); GL_MCS(4, 22); GL_MCS(4, 32);
+#undef GL_MCS
+    return hipErrorInvalidValue;
+}
+
 static hipError_t mc_gram_dispatch(int nbt, int nb, int ks, hipStream_t st, const gl::McTable& T) {
     switch (nbt) {
         case 1: hipLaunchKernelGGL((gl::mc_gram<1, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
@@ -365,10 +384,37 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     const int gram_nt = M.gram_nb * (M.gram_nb + 1) / 2;
     const int nchunk = (N * N + 63) / 64;
     p->ksplit_cap = std::max(1, std::min(gl::kMcGramMaxSplit, nchunk / 4));
+    // separable channel convolution, on request (psf_rank_tol > 0): when the folded
+    // PSF has rank <= 8 at that tolerance and the flop count is lower.  Not the
+    // default: its passes read a PSF factor per FMA and run slower than the direct
+    // conv on the shapelet configuration (the autotuner times both).
+    M.sep_rank = 0;
+    std::vector<float> sa, sbp;
+    if (desc->psf_rank_tol > 0.0f) {
+        const double tol = desc->psf_rank_tol;
+        std::vector<double> Wd(kf * kf), sv, Uv, Vv;
+        for (int i = 0; i < kf * kf; ++i) Wd[i] = W[i];
+        jacobi_svd(kf, Wd, sv, Uv, Vv);
+        int R = 0;
+        while (R < kf && sv[R] >= tol * sv[0]) ++R;
+        const double ratio = (double)R * (nsub + N) / ((double)N * kf);
+        if (R >= 1 && R <= gl::kMcMaxRank && ratio < 0.8) {
+            M.sep_rank = R;
+            const int wpw = 4 * p->kf4;
+            sa.assign((size_t)R * kf, 0.0f);
+            sbp.assign((size_t)R * wpw, 0.0f);
+            for (int r = 0; r < R; ++r)
+                for (int i = 0; i < kf; ++i) {
+                    sa[(size_t)r * kf + i] = (float)(sv[r] * Uv[i * kf + r]);
+                    sbp[(size_t)r * wpw + i] = (float)Vv[i * kf + r];
+                }
+        }
+    }
     // particles per chunk: the caller's, else as many as fit a 32 GiB workspace (at most 1024)
     const int BW = ((ss * (N - 1) + 4 * p->kf4 > pb + nsub ? ss * (N - 1) + 4 * p->kf4 : pb + nsub) + 3) & ~3;
     const double per_particle = 4.0 * ((double)M.pstride + (double)D * nsub * BW + (double)D * N * N + D +
-                                       (double)GW * GW + gl::kMcStat) +
+                                       (double)GW * GW + gl::kMcStat +
+                                       (double)D * M.sep_rank * nsub * N) +
                                 8.0 * 4 * p->ksplit_cap * gram_nt * 256;
     int mb = desc->max_batch;
     if (mb <= 0) mb = (int)std::max(1.0, std::min(1024.0, 32.0 * 1073741824.0 / per_particle));
@@ -408,6 +454,14 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     }
     // workspace for max_batch particles (allocated here once; calls never allocate)
     if ((rc = mc_workspace(P, M, mb, p->kf4, p->ksplit_cap, W, p->dev))) return rc;
+    if (M.sep_rank > 0) {
+        if ((rc = upload(p, sa.data(), sa.size() * sizeof(float), &dptr))) return rc;
+        M.sa = (const float*)dptr;
+        if ((rc = upload(p, sbp.data(), sbp.size() * sizeof(float), &dptr))) return rc;
+        M.sbp = (const float*)dptr;
+        if ((rc = dev_alloc(p->dev, (size_t)mb * D * M.sep_rank * nsub * N * 4, &dptr, false))) return rc;
+        M.H = (float*)dptr;
+    }
     const int n2 = D + (D & 1);
     // A, V (n2 x n2), rotations (n2), pairs (n2 ints), 2 reduction slots, rhs (n2)
     p->solve_lds = (size_t)(2 * n2 * n2 + n2) * 8 + (size_t)n2 * 4 + 16 + (size_t)n2 * 8 + 64;
@@ -445,7 +499,16 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
     GL_HIP(hipGetLastError());
     hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
     GL_HIP(hipGetLastError());
-    GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, MULTI ? 4 : T.one.M.conv_ib, (T.tot_conv + 3) / 4, st, T));
+    if (!MULTI && T.one.M.sep_rank > 0) {
+        const gl::DPlan& P = T.one.P;
+        const long per = (long)np * T.one.M.D * ((P.N + 63) / 64);
+        const long gh = (per * ((P.nsub + 3) / 4) + 3) / 4;
+        const long gv = (per * ((P.N + gl::kSepIB - 1) / gl::kSepIB) + 3) / 4;
+        if (gh > INT32_MAX) return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
+        GL_HIP(mc_sep_dispatch(ss_launch, kf4, (int)gh, (int)gv, st, T));
+    } else {
+        GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, MULTI ? 4 : T.one.M.conv_ib, (T.tot_conv + 3) / 4, st, T));
+    }
     if (lstsq) {
         if (MULTI) return fail(GL_ERR_UNSUPPORTED, "least-squares lenses are not batched");
         const int ks = std::max(1, std::min(ksplit_cap, (1024 + np - 1) / np));
@@ -839,7 +902,7 @@ extern "C" int gl_plan_query(const gl_plan* plan, gl_plan_info* info) {
     info->n_params = P.d;
     info->kernel_size = P.k;
     info->folded_size = P.kf;
-    info->sep_rank = P.sep_rank;
+    info->sep_rank = plan->mc ? plan->M.sep_rank : P.sep_rank;
     info->lds_bytes = (int)plan->lds_bytes;
     info->block = kBlock;
     return GL_OK;
@@ -1039,7 +1102,8 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         memset(&E, 0, sizeof(E));
         E.P = plans[i]->P;
         E.M = plans[i]->M;
-        E.M.conv_ib = gl::kMcIB;  // batches launch the conv at the default rows per lane
+        E.M.conv_ib = gl::kMcIB;  // batches launch the direct conv at the default rows per lane
+        E.M.sep_rank = 0;
         int rc = mc_workspace(E.P, E.M, counts[i], b->kf4, 1, plans[i]->W_host, b->dev);
         if (rc) { gl_batch_destroy(b); return rc; }
         E.first = (int)acc[0];

@@ -49,11 +49,16 @@ constexpr int kMcMaxChan = 79;
 constexpr int kMcGramMaxNB = 5;
 constexpr int kMcGramMaxSplit = 8;  // pixel-range splits per particle (x 4 waves = partials)
 constexpr int kMcStat = 64;  // statistics floats per particle
+constexpr int kMcMaxRank = 8;  // separable channel convolution: at most this rank
 constexpr int kMcIB = 4;     // default output rows per conv lane (McPlan.conv_ib: 2, 4 or 8)
 
 struct McPlan {
     int lstsq;          // 1: linear amplitudes solved by least squares; 0: fixed amplitudes (one channel)
     int conv_ib;        // output rows per lane of mc_conv (a tuning choice: 2, 4 or 8)
+    int sep_rank;       // > 0: separable channel convolution with this rank (W ~= sum_r sa_r sb_r^T)
+    const float* sa;    // sep_rank x kf vertical factors
+    const float* sbp;   // sep_rank x wpad_w horizontal factors, zero padded
+    float* H;           // bs x D x sep_rank x nsub x N horizontal-pass intermediate
     int D;              // channels
     int BW;             // staged row width (floats): pb + nsub + right margin, multiple of 4
     int pstride;        // floats of per-particle constants
@@ -444,6 +449,106 @@ __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
                 Xc[I * N + J] = isnan(v) ? 0.0f : v;  // NaN -> 0 after pooling (tf/simulator.py:153)
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Separable channel convolution (single plans): W ~= sum_r sa_r sb_r^T, so the
+// stride-ss correlation splits into a horizontal pass over every sub-row,
+//   H_r[y][J] = sum_q sb_r[q] B[y][ss J + q - pb],
+// and a vertical pass over the output rows,
+//   X[I][J] = sum_r sum_p sa_r[p] H_r[ss I + p - pb][J].
+// mc_sep_h: one wave per (particle, channel, 4 sub-rows, 64 columns J).
+// ---------------------------------------------------------------------------
+template <int SS, int KF4>
+__global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int task = blockIdx.x * 4 + wave;
+    const DPlan& P = T.one.P;
+    const McPlan& M = T.one.M;
+    const int N = P.N, nsub = P.nsub, R = M.sep_rank;
+    const int nyb = (nsub + 3) / 4, njb = (N + 63) / 64;
+    if (task >= T.tot_part * M.D * nyb * njb) return;
+    const int jb = task % njb; task /= njb;
+    const int yb = task % nyb; task /= nyb;
+    const int c = task % M.D;
+    const int p = task / M.D;
+    const int J = jb * 64 + lane;
+    if (J >= N) return;
+    const float* Bc = M.B + ((size_t)p * M.D + c) * nsub * M.BW;
+    float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int y = yb * 4 + i;
+        if (y >= nsub) break;
+        const float* row = Bc + (size_t)y * M.BW + SS * J;
+        float v[4 * KF4];
+        if (SS == 4) {
+            const float4* r4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+            for (int j = 0; j < KF4; ++j) {
+                const float4 t = r4[j];
+                v[4 * j] = t.x; v[4 * j + 1] = t.y; v[4 * j + 2] = t.z; v[4 * j + 3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4 * KF4; ++j) v[j] = row[j];
+        }
+#pragma unroll
+        for (int r = 0; r < kMcMaxRank; ++r) {
+            if (r >= R) break;
+            const float* w = M.sbp + r * M.wpad_w;
+            float h = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4 * KF4; ++j) h = fmaf(w[j], v[j], h);
+            Hc[((size_t)r * nsub + y) * N + J] = h;
+        }
+    }
+}
+
+// mc_sep_v: one wave per (particle, channel, kSepIB output rows, 64 columns J);
+// each intermediate row read serves up to kSepIB outputs of the lane.
+constexpr int kSepIB = 16;
+template <int SS>
+__global__ __launch_bounds__(256) void mc_sep_v(const McTable T) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int task = blockIdx.x * 4 + wave;
+    const DPlan& P = T.one.P;
+    const McPlan& M = T.one.M;
+    const int N = P.N, nsub = P.nsub, kf = P.kf, pb = P.pb, R = M.sep_rank;
+    const int nib = (N + kSepIB - 1) / kSepIB, njb = (N + 63) / 64;
+    if (task >= T.tot_part * M.D * nib * njb) return;
+    const int jb = task % njb; task /= njb;
+    const int ib = task % nib; task /= nib;
+    const int c = task % M.D;
+    const int p = task / M.D;
+    const int J = jb * 64 + lane;
+    if (J >= N) return;
+    const int I0 = ib * kSepIB;
+    const float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;
+    float acc[kSepIB];
+#pragma unroll
+    for (int i = 0; i < kSepIB; ++i) acc[i] = 0.0f;
+    const int nrows = SS * (kSepIB - 1) + kf;
+    for (int r = 0; r < R; ++r) {
+        const float* Hr = Hc + (size_t)r * nsub * N;
+        const float* ar = M.sa + r * kf;
+        for (int rr = 0; rr < nrows; ++rr) {
+            const int y = SS * I0 + rr - pb;
+            if (y < 0 || y >= nsub) continue;
+            const float v = Hr[(size_t)y * N + J];
+#pragma unroll
+            for (int i = 0; i < kSepIB; ++i) {
+                const int pp = rr - SS * i;
+                if (pp >= 0 && pp < kf) acc[i] = fmaf(ar[pp], v, acc[i]);
+            }
+        }
+    }
+    float* Xc = M.X + ((size_t)p * M.D + c) * N * N;
+#pragma unroll
+    for (int i = 0; i < kSepIB; ++i) {
+        const int I = I0 + i;
+        if (I < N) Xc[I * N + J] = isnan(acc[i]) ? 0.0f : acc[i];  // NaN -> 0 after pooling
     }
 }
 

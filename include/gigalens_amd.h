@@ -95,7 +95,9 @@ typedef struct gl_plan_desc {
     int fldj_at_x;            /* 1: evaluate fldj at constrained x (tf/model.py:160 quirk) */
     int epl_batch_series;     /* 1: reference's batch-coupled EPL series length (epl.py:41) */
     float psf_rank_tol;       /* separable PSF: keep singular values >= tol * s_max of the folded
-                                 PSF; 0 selects 1e-6, a negative value forces the direct 2-D path */
+                                 PSF; 0 selects 1e-6, a negative value forces the direct 2-D path.
+                                 Staged plans use their separable channel convolution only for an
+                                 explicit tol > 0 */
     int max_batch;            /* least-squares plans: particles per internal chunk (workspace is
                                  allocated for this many at plan creation); 0 selects 64 */
     const float* noise_map;   /* HOST N*N sigma per pixel; NULL: sqrt(bkg^2 + clip(obs,0)/t_exp) */
@@ -116,7 +118,8 @@ typedef struct gl_plan_info {
     int num_pix, supersample, n_params;
     int kernel_size;      /* sub-grid PSF size k */
     int folded_size;      /* kf = k + ss - 1 */
-    int sep_rank;         /* > 0: separable PSF path with this rank, 0: direct 2-D path */
+    int sep_rank;         /* > 0: separable PSF path with this rank, 0: direct 2-D path (fused kernel or,
+                             for staged plans, their channel convolution) */
     int lds_bytes;        /* LDS per workgroup of the particle kernel */
     int block;            /* threads per workgroup */
 } gl_plan_info;

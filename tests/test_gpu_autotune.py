@@ -25,7 +25,7 @@ def test_autotune_picks_a_variant(cuda, tmp_path, monkeypatch):
     plan = pm2.plan_for(LensSimulator(prob["phys"], prob["cfg"], bs=256, device=cuda))
     rec = list(autotune._memory.values())[0]
     print("autotune timings (ms):", rec["timings"], "best:", rec["best"], flush=True)
-    assert len(rec["timings"]) == 5  # separable, direct, staged x 3
+    assert len(rec["timings"]) == 6  # fused separable / direct, staged conv rows x 3, staged separable
     lp1, _, _ = plan.log_prob(z)
     assert float(((lp1 - lp0).abs() / lp0.abs()).max()) < 1e-5
     # cached: a second tuning of the same key does not time again
