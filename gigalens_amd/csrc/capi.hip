@@ -849,12 +849,29 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             const int TR = nsub + k - 1;
             const int xs = (N + gl::kTB - 1) / gl::kTB;
             const int urows = gp0 + N + dmax;
-            const bool fits = R >= 1 && R <= gl::kMaxRank && (long)R * TR * N <= P.sub_size &&
-                              (long)R * urows * nsub <= P.sub_size && (long)TR * nstr <= (long)gl::kMaxTT * kBlock &&
-                              (long)N * ss * xs <= (long)gl::kMaxU * kBlock && (long)N * nstr <= (long)gl::kMaxT * kBlock;
+            // transposed horizontal pass (gl::SepGeom): phase b reads G columns from
+            // X0 + off_b + gp0, off_b = db - (Mb - 1); one segment covers every phase
+            int omin = 1 << 30, omax = -(1 << 30);
+            for (int b = 0; b < ss; ++b) {
+                const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+                const int o = db - (Mb - 1);
+                omin = o < omin ? o : omin;
+                omax = o > omax ? o : omax;
+            }
+            const int sega4 = ((gl::kTB + mmax - 1 + omax - omin + 3) / 4) * 4;
+            const int u_off = (P.g_size + 3) & ~3;
+            const bool fits = R >= 1 && R <= gl::kMaxRank &&
+                              // forward: R planes of T plus the rows read for discarded outputs
+                              (long)(R * TR + ss * (gl::kRV - 1)) * N <= P.sub_size &&
+                              (long)TR * nstr <= (long)gl::kMaxTT * kBlock && (long)N * nstr <= (long)gl::kMaxT * kBlock &&
+                              // adjoint: G segments stay inside their row, U after G
+                              omin + gp0 >= 0 && (long)(xs - 1) * gl::kTB + omin + gp0 + sega4 - 1 < GW &&
+                              (long)u_off + (long)R * urows * nsub <= P.sub_size;
             if (fits) {
                 P.sep_rank = R;
                 P.urows = urows;
+                P.u_off = u_off;
+                P.sep_aligned = ((omin + gp0) % 4 == 0) ? 1 : 0;
                 sa.resize(R * kf);
                 sb.resize(R * kf);
                 for (int r = 0; r < R; ++r)

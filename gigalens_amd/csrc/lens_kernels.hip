@@ -206,7 +206,6 @@ constexpr int kMaxRank = 2;
 constexpr int kTB = 4;     // J per horizontal-pass task
 constexpr int kMaxTT = 5;  // horizontal-pass tasks per thread
 constexpr int kRV = 4;     // output rows per vertical-pass task
-constexpr int kMaxU = 4;   // transposed-horizontal tasks per thread
 
 template <int SS, int KF>
 __device__ __forceinline__ void sep_hpass(const float* __restrict__ sub, const float* __restrict__ sb, int R, int PW,
@@ -243,11 +242,12 @@ __device__ __forceinline__ void sep_vpass(const float* __restrict__ T, const flo
 #pragma unroll
     for (int r = 0; r < kMaxRank; ++r) {
         if (r < R) {
-            const float* col = T + r * TR * TW + J;
+            // rows at or beyond TR only reach outputs I0 + i >= N, which are discarded; the
+            // plan keeps SS (kRV - 1) rows of LDS after the last rank for them
+            const float* col = T + r * TR * TW + SS * I0 * TW + J;
 #pragma unroll
             for (int c = 0; c < SS * (kRV - 1) + KF; ++c) {
-                const int row = SS * I0 + c;
-                const float v = row < TR ? col[row * TW] : 0.0f;
+                const float v = col[c * TW];
 #pragma unroll
                 for (int i = 0; i < kRV; ++i) {
                     const int pp = c - SS * i;
@@ -258,40 +258,83 @@ __device__ __forceinline__ void sep_vpass(const float* __restrict__ T, const flo
     }
 }
 
+// Geometry of the transposed horizontal pass for phase b (sub-pixel column x = SS X + b):
+// taps b_r[SS n + rb], n < Mb, read G columns X + db - n; sep_off(b) = db - (Mb - 1).
 template <int SS, int KF>
-__device__ __forceinline__ void sep_upass(const float* __restrict__ G, const float* __restrict__ sb, int R, int GW,
-                                          int gp0, bool aligned, int b, int I, int X0, float (&u)[kMaxRank][kTB]) {
-    constexpr int PB = (KF - SS) / 2;
-    constexpr int MMAX = (KF + SS - 1) / SS;
-    constexpr int SEGA = kTB + MMAX - 1;
-    constexpr int SEGA4 = (SEGA + 3) / 4;
-    const int rb = (b + PB) % SS, db = (b + PB) / SS;
-    const int Mb = (KF - rb + SS - 1) / SS;
-    const float* row = G + (I + gp0) * GW + (X0 + db - (Mb - 1) + gp0);
-    float seg[SEGA4 * 4];
+struct SepGeom {
+    static constexpr int PB = (KF - SS) / 2;
+    static constexpr int MMAX = (KF + SS - 1) / SS;
+    static constexpr int rb(int b) { return (b + PB) % SS; }
+    static constexpr int db(int b) { return (b + PB) / SS; }
+    static constexpr int Mb(int b) { return (KF - rb(b) + SS - 1) / SS; }
+    static constexpr int off(int b) { return db(b) - (Mb(b) - 1); }
+    static constexpr int off_min() {
+        int m = off(0);
+        for (int b = 1; b < SS; ++b) m = off(b) < m ? off(b) : m;
+        return m;
+    }
+    static constexpr int off_max() {
+        int m = off(0);
+        for (int b = 1; b < SS; ++b) m = off(b) > m ? off(b) : m;
+        return m;
+    }
+    static constexpr int SEGA = kTB + MMAX - 1 + off_max() - off_min();
+    static constexpr int SEGA4 = (SEGA + 3) / 4;
+};
+
+// One task: kTB adjoint columns X0.. of output row I, every phase b, every rank;
+// writes U_r[I][SS X + b] for X < N.  The phase loop is unrolled, so every tap
+// index is a compile-time constant and the factors arrive by scalar loads.
+template <int SS, int KF>
+__device__ __forceinline__ void sep_upass(const float* __restrict__ G, float* __restrict__ U,
+                                          const float* __restrict__ sb, int R, int GW, int gp0, int ustride,
+                                          int UW, bool aligned, bool vstore, int I, int X0, int N) {
+    using Gm = SepGeom<SS, KF>;
+    constexpr int OMIN = Gm::off_min();
+    const float* row = G + (I + gp0) * GW + (X0 + OMIN + gp0);
+    float seg[Gm::SEGA4 * 4];
     if (aligned) {
         const float4* row4 = reinterpret_cast<const float4*>(row);
 #pragma unroll
-        for (int c = 0; c < SEGA4; ++c) {
+        for (int c = 0; c < Gm::SEGA4; ++c) {
             float4 v = row4[c];
             seg[4 * c] = v.x; seg[4 * c + 1] = v.y; seg[4 * c + 2] = v.z; seg[4 * c + 3] = v.w;
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < SEGA4 * 4; ++j) seg[j] = (j < kTB + Mb - 1) ? row[j] : 0.0f;
+        for (int j = 0; j < Gm::SEGA4 * 4; ++j) seg[j] = j < Gm::SEGA ? row[j] : 0.0f;
     }
+    const bool full = vstore && X0 + kTB <= N;
 #pragma unroll
     for (int r = 0; r < kMaxRank; ++r) {
         if (r < R) {
+            float o[SS * kTB];
 #pragma unroll
-            for (int j = 0; j < kTB; ++j) u[r][j] = 0.0f;
+            for (int b = 0; b < SS; ++b) {
+                const int sh = Gm::off(b) - OMIN;
 #pragma unroll
-            for (int n = 0; n < MMAX; ++n) {
-                if (n < Mb) {
-                    const float w = sb[r * KF + SS * n + rb];
+                for (int j = 0; j < kTB; ++j) o[SS * j + b] = 0.0f;
 #pragma unroll
-                    for (int j = 0; j < kTB; ++j) u[r][j] = fmaf(w, seg[j + Mb - 1 - n], u[r][j]);
+                for (int n = 0; n < Gm::MMAX; ++n) {
+                    if (n < Gm::Mb(b)) {
+                        const float w = sb[r * KF + SS * n + Gm::rb(b)];
+#pragma unroll
+                        for (int j = 0; j < kTB; ++j)
+                            o[SS * j + b] = fmaf(w, seg[sh + j + Gm::Mb(b) - 1 - n], o[SS * j + b]);
+                    }
                 }
+            }
+            float* dst = U + r * ustride + SS * X0;
+            if (full) {
+#pragma unroll
+                for (int c = 0; c < SS * kTB / 4; ++c)
+                    reinterpret_cast<float4*>(dst)[c] = make_float4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < kTB; ++j)
+                    if (X0 + j < N)
+#pragma unroll
+                        for (int b = 0; b < SS; ++b) dst[SS * j + b] = o[SS * j + b];
             }
         }
     }
@@ -439,10 +482,48 @@ constexpr int kRB = 4;    // outputs per thread strip (forward conv)
 constexpr int kRB2 = 4;   // sub-pixels per thread strip (adjoint conv)
 constexpr int kMaxT = 4;  // forward strips per thread held in registers
 
+// Sum over the 64 lanes of a wave, returned wave-uniform.  Every lane must be
+// active.  DPP moves within each row of 16 lanes (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_ror 4, row_ror 8) leave the row sum in all 16
+// lanes; the four row sums are then read from lanes 0, 16, 32 and 48.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
 __device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x124>(v);
+    v += dpp_f<0x128>(v);
+    const int b = __builtin_bit_cast(int, v);
+    return (__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 0)) +
+            __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 16))) +
+           (__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 32)) +
+            __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 48)));
+}
+
+// EPL coefficient n = lane (0 <= n <= NN < 64) of glm::epl_coef, computed by one
+// wave: the serial recurrence (a, da) -> (a r_k, da r_k + a dr_k) is associative in
+// the pairs (r_k, dr_k), so an inclusive scan over the lanes (r_k, dr_k scaled by -f
+// for k >= 2, identity elsewhere) gives pw a_n and pw da_n in log2(64) steps.
+// Every lane of the wave must be active.
+__device__ __forceinline__ void epl_coef_scan(int n, int NN, double f, double t, float& c, float& dt, float& df) {
+    double r = 1.0, dr = 0.0;
+    if (n >= 1 && n <= NN) {
+        const double inv = 1.0 / (2.0 * n + 2.0 - t);
+        r = (2.0 * n - 2.0 + t) * inv;
+        dr = 4.0 * n * inv * inv;
+        if (n > 1) { r *= -f; dr *= -f; }
+    }
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const double pr = __shfl_up(r, o, 64), pdr = __shfl_up(dr, o, 64);
+        if (n >= o) { dr = pdr * r + pr * dr; r = pr * r; }
+    }
+    if (n == 0) { c = 1.0f; dt = 0.0f; df = 0.0f; return; }
+    c = (float)(-f * r);
+    dt = (float)(-f * dr);
+    df = (float)(-(double)n * r);
 }
 
 template <int BLOCK, class M, bool GRAD>
@@ -460,9 +541,15 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
     const bool use_z = (A.mode == M_LOGPROB || A.mode == M_MAP);
 
     // ---- A: prologue -----------------------------------------------------
-    for (int i = tid; i < P.sub_size; i += BLOCK) s_sub[i] = 0.0f;
+    // z, and for MAP the Adam moments, are loaded once here and kept in registers
+    float zin = 0.0f, m_in = 0.0f, v_in = 0.0f;
     if (tid < d) {
-        float zin = A.in[(size_t)pid * d + tid];
+        const size_t o = (size_t)pid * d + tid;
+        zin = A.in[o];
+        if (A.mode == M_MAP) { m_in = A.m[o]; v_in = A.v[o]; }
+    }
+    for (int i = tid; i < (P.sub_size + 3) / 4; i += BLOCK) smem4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (tid < d) {
         glm::PriorOut o;
         if (use_z) {
             glm::prior_eval(P.prior_kind[tid], P.prior_args + 8 * tid, zin, P.fldj_at_x, o);
@@ -477,15 +564,17 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
         s_x[5 * d + tid] = o.lp + o.fldj;
     }
     __syncthreads();
-    if (tid < P.ncomp) {
-        const DComp& cd = P.comp[tid];
+    // component ci is prepared by lane 0 of wave ci (the components run side by side)
+    if ((tid & 63) == 0 && (tid >> 6) < P.ncomp) {
+        const int ci = tid >> 6;
+        const DComp& cd = P.comp[ci];
         float pv[GL_MAX_COMPONENT_PARAMS];
 #pragma unroll
         for (int j = 0; j < GL_MAX_COMPONENT_PARAMS; ++j) pv[j] = cd.col[j] >= 0 ? s_x[cd.col[j]] : cd.cst[j];
-        float* dst = s_c + tid * kConstStride;
+        float* dst = s_c + ci * kConstStride;
         if (cd.kind == glm::K_EPL) {
             glm::EplC c;
-            int nov = P.batch_n ? P.batch_n[tid] : -1;
+            int nov = P.batch_n ? P.batch_n[ci] : -1;
             glm::epl_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.niter, nov, c);
             if (c.N > P.max_n) c.N = P.max_n;
             *reinterpret_cast<glm::EplC*>(dst) = c;
@@ -508,15 +597,27 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        if (M::lens[i] == glm::K_EPL) {
+        // the table of EPL slot i is built by wave i
+        if (M::lens[i] == glm::K_EPL && (tid >> 6) == i) {
             const glm::EplC* c = reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
             float* ca = s_coef + P.coef_off[i];
             float4* cb = reinterpret_cast<float4*>(ca + ((P.max_n + 4) & ~3));
-            for (int n = tid; n <= c->N; n += BLOCK) {
+            const int NN = c->N;
+            const int n = tid & 63;
+            if (NN < 64) {
                 float cv, dt, df;
-                glm::epl_coef(n, (double)c->f, (double)c->t, cv, dt, df);
-                ca[n] = cv;
-                cb[n] = make_float4(cv, dt, df, 0.0f);
+                epl_coef_scan(n, NN, (double)c->f, (double)c->t, cv, dt, df);
+                if (n <= NN) {
+                    ca[n] = cv;
+                    cb[n] = make_float4(cv, dt, df, 0.0f);
+                }
+            } else {
+                for (int k = n; k <= NN; k += 64) {
+                    float cv, dt, df;
+                    glm::epl_coef(k, (double)c->f, (double)c->t, cv, dt, df);
+                    ca[k] = cv;
+                    cb[k] = make_float4(cv, dt, df, 0.0f);
+                }
             }
         }
     }
@@ -671,7 +772,7 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
 
     if (GRAD) {
         __syncthreads();
-        for (int i = tid; i < P.g_size; i += BLOCK) s_sub[i] = 0.0f;
+        for (int i = tid; i < P.g_size / 4; i += BLOCK) smem4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __syncthreads();
         const int GW = P.GW, gp0 = P.gp0;
 #pragma unroll
@@ -696,49 +797,32 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
         __syncthreads();
 
         if (sepk) {
-            // ---- D1: transposed horizontal pass G -> U (registers), lanes along X ----
+            // ---- D1: transposed horizontal pass G -> U, lanes along X ---------------
+            // U lives after G in LDS (no overlap), so tasks store straight to it.
             const int XS = (N + kTB - 1) / kTB;
-            const int utask = N * ss * XS;
-            float uv[kMaxU][kMaxRank][kTB];
-#pragma unroll
-            for (int tt = 0; tt < kMaxU; ++tt) {
-                const int task = tid + tt * BLOCK;
-                if (task < utask) {
-                    const int I = task / (ss * XS);
-                    const int rem = task - I * ss * XS;
-                    const int b = rem / XS;
-                    const int X0 = (rem - b * XS) * kTB;
-                    if (sepk == 2022) sep_upass<2, 22>(s_sub, P.sb, R, GW, gp0, P.adj_aligned, b, I, X0, uv[tt]);
-                    else if (sepk == 1021) sep_upass<1, 21>(s_sub, P.sb, R, GW, gp0, P.adj_aligned, b, I, X0, uv[tt]);
-                    else sep_upass<3, 23>(s_sub, P.sb, R, GW, gp0, P.adj_aligned, b, I, X0, uv[tt]);
-                }
-            }
-            __syncthreads();
+            const int utask = N * XS;
             const int UR = P.urows, UW = P.nsub, up0 = gp0;
-            // zero the padding rows of U, then store
-            for (int i = tid; i < R * UR * UW; i += BLOCK) {
-                const int row = (i / UW) % UR;
-                if (row < up0 || row >= up0 + N) s_sub[i] = 0.0f;
+            float* s_u = s_sub + P.u_off;
+            for (int r = 0; r < R; ++r) {
+                float* ur = s_u + r * UR * UW;
+                for (int i = tid; i < up0 * UW; i += BLOCK) ur[i] = 0.0f;
+                for (int i = tid; i < (UR - up0 - N) * UW; i += BLOCK) ur[(up0 + N) * UW + i] = 0.0f;
             }
-#pragma unroll
-            for (int tt = 0; tt < kMaxU; ++tt) {
-                const int task = tid + tt * BLOCK;
-                if (task < utask) {
-                    const int I = task / (ss * XS);
-                    const int rem = task - I * ss * XS;
-                    const int b = rem / XS;
-                    const int X0 = (rem - b * XS) * kTB;
-#pragma unroll
-                    for (int r = 0; r < kMaxRank; ++r)
-                        if (r < R)
-#pragma unroll
-                            for (int j = 0; j < kTB; ++j)
-                                if (X0 + j < N) s_sub[r * UR * UW + (I + up0) * UW + ss * (X0 + j) + b] = uv[tt][r][j];
-                }
+            const bool vstore = (UW & 3) == 0;
+            for (int task = tid; task < utask; task += BLOCK) {
+                const int I = task / XS;
+                const int X0 = (task - I * XS) * kTB;
+                float* urow = s_u + (I + up0) * UW;
+                if (sepk == 2022)
+                    sep_upass<2, 22>(s_sub, urow, P.sb, R, GW, gp0, UR * UW, UW, P.sep_aligned, vstore, I, X0, N);
+                else if (sepk == 1021)
+                    sep_upass<1, 21>(s_sub, urow, P.sb, R, GW, gp0, UR * UW, UW, P.sep_aligned, vstore, I, X0, N);
+                else
+                    sep_upass<3, 23>(s_sub, urow, P.sb, R, GW, gp0, UR * UW, UW, P.sep_aligned, vstore, I, X0, N);
             }
             __syncthreads();
             // ---- D2 + E: per sub-pixel transposed vertical pass + recompute, one row per wave ----
-            const int lane_ = tid & 63, wave_ = tid >> 6;
+            const int lane_ = tid & 63, wave_ = __builtin_amdgcn_readfirstlane(tid >> 6);
             const int H = (nsub + 63) / 64;
             for (int wt = wave_; wt < nsub * H; wt += BLOCK / 64) {
                 const int y = wt / H;
@@ -746,9 +830,9 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
                 const int a = y % ss, Yc = y / ss;
                 if (x < nsub) {
                     float g;
-                    if (sepk == 2022) g = sep_gsub<2, 22>(s_sub, P.sa, R, UR, UW, up0, a, Yc, x);
-                    else if (sepk == 1021) g = sep_gsub<1, 21>(s_sub, P.sa, R, UR, UW, up0, a, Yc, x);
-                    else g = sep_gsub<3, 23>(s_sub, P.sa, R, UR, UW, up0, a, Yc, x);
+                    if (sepk == 2022) g = sep_gsub<2, 22>(s_u, P.sa, R, UR, UW, up0, a, Yc, x);
+                    else if (sepk == 1021) g = sep_gsub<1, 21>(s_u, P.sa, R, UR, UW, up0, a, Yc, x);
+                    else g = sep_gsub<3, 23>(s_u, P.sa, R, UR, UW, up0, a, Yc, x);
                     float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
                     float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
                     eval_adjoint<M>(C, X, Y, g, acc);
@@ -818,13 +902,18 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
             if (A.lp) A.lp[pid] = bad_lens ? __builtin_nanf("") : lp_like + lp_prior;
             if (A.chi2) A.chi2[pid] = bad_lens ? __builtin_nanf("") : chi2;
         }
-        if (GRAD && A.mode != M_SIM) {
-            const float* tot = s_tot + 2;
-            float* gx = s_x + 4 * d;
+    }
+    if (GRAD && A.mode != M_SIM) {
+        // per-component chain rule: lens slot i on lane 0 of wave 1 + i, light slot i on
+        // lane 0 of wave 4 + i; contributions meet in gx through LDS atomics
+        const float* tot = s_tot + 2;
+        float* gx = s_x + 4 * d;
+        const int w = tid >> 6;
+        if ((tid & 63) == 0) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const int k = M::lens[i];
-                if (k == 0) continue;
+                if (k == 0 || w != 1 + i) continue;
                 const DComp& cd = P.comp[i];
                 float g[8];
                 int np = 0;
@@ -832,26 +921,24 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
                 else if (k == glm::K_SHEAR) { g[0] = tot[M::lens_off(i)]; g[1] = tot[M::lens_off(i) + 1]; np = 2; }
                 else if (k == glm::K_SIS) { glm::sis_finish(tot + M::lens_off(i), g); np = 3; }
                 for (int p = 0; p < np; ++p)
-                    if (cd.col[p] >= 0) gx[cd.col[p]] += g[p];
+                    if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[p]);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                if (M::light[i] == 0) continue;
+                if (M::light[i] == 0 || w != 4 + i) continue;
                 const DComp& cd = P.comp[light_comp_index<M>(i)];
                 float g[8];
                 glm::sersic_finish(C.ser[i], tot + M::light_off(light_acc_slot<M>(i)), g);
                 if (M::light[i] == glm::K_SERSIC_ELLIPSE) {
                     for (int p = 0; p < (cd.lstsq ? 6 : 7); ++p)
-                        if (cd.col[p] >= 0) gx[cd.col[p]] += g[p];
+                        if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[p]);
                 } else {
                     const int map[5] = {0, 1, 4, 5, 6};
                     for (int p = 0; p < (cd.lstsq ? 4 : 5); ++p)
-                        if (cd.col[p] >= 0) gx[cd.col[p]] += g[map[p]];
+                        if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[map[p]]);
                 }
             }
         }
-    }
-    if (GRAD && A.mode != M_SIM) {
         __syncthreads();
         if (tid < d) {
             const float gxj = s_x[4 * d + tid];
@@ -863,11 +950,11 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
                     A.grad[(size_t)pid * d + tid] = gz;
                 } else {  // M_MAP: Keras Adam
                     const size_t o = (size_t)pid * d + tid;
-                    float mm = A.beta1 * A.m[o] + (1.0f - A.beta1) * gz;
-                    float vv = A.beta2 * A.v[o] + (1.0f - A.beta2) * gz * gz;
+                    float mm = A.beta1 * m_in + (1.0f - A.beta1) * gz;
+                    float vv = A.beta2 * v_in + (1.0f - A.beta2) * gz * gz;
                     A.m[o] = mm;
                     A.v[o] = vv;
-                    A.z_io[o] = A.in[o] - A.lr_t * mm / (sqrtf(vv) + A.eps);
+                    A.z_io[o] = zin - A.lr_t * mm / (sqrtf(vv) + A.eps);
                 }
             }
         }
