@@ -57,8 +57,8 @@ struct ModelEntry {
     AttrFn attr;
 };
 
-constexpr int EPL = glm::K_EPL, SHR = glm::K_SHEAR, SIS = glm::K_SIS, SE = glm::K_SERSIC_ELLIPSE,
-              SC = glm::K_SERSIC;
+constexpr int EPL = glm::K_EPL, SHR = glm::K_SHEAR, SIS = glm::K_SIS, SIE = glm::K_SIE, TNFW = glm::K_TNFW,
+              SE = glm::K_SERSIC_ELLIPSE, SC = glm::K_SERSIC;
 
 #define GL_ENTRY(L0, L1, L2, A0, A1, S0, S1)                                      \
     {{L0, L1, L2, A0, A1, S0, S1},                                                \
@@ -74,6 +74,8 @@ const ModelEntry kModels[] = {
     GL_ENTRY(SIS, SHR, 0, SE, 0, SE, 0),
     GL_ENTRY(EPL, SHR, 0, SE, 0, SC, 0),
     GL_ENTRY(EPL, SHR, 0, SE, SE, SE, 0),
+    GL_ENTRY(SIE, SHR, 0, SE, 0, SE, 0),
+    GL_ENTRY(EPL, SHR, TNFW, SE, 0, SE, 0),  // main deflector + one truncated-NFW subhalo
 };
 
 }  // namespace
@@ -109,6 +111,8 @@ static int n_params_of(int kind) {
     case glm::K_EPL: return 6;
     case glm::K_SHEAR: return 2;
     case glm::K_SIS: return 3;
+    case glm::K_SIE: return 5;
+    case glm::K_TNFW: return 5;
     case glm::K_SERSIC: return 5;
     case glm::K_SERSIC_ELLIPSE: return 7;
     case glm::K_SHAPELETS: return 3;
@@ -673,7 +677,8 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
                 return fail(GL_ERR_ARG, "param_col out of range");
             }
         if (c.role == GL_ROLE_LENS) {
-            if (c.kind != glm::K_EPL && c.kind != glm::K_SHEAR && c.kind != glm::K_SIS) {
+            if (c.kind != glm::K_EPL && c.kind != glm::K_SHEAR && c.kind != glm::K_SIS && c.kind != glm::K_SIE &&
+                c.kind != glm::K_TNFW) {
                 gl_plan_destroy(p);
                 return fail(GL_ERR_ARG, "light profile given as a lens");
             }

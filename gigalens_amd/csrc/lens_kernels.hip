@@ -49,6 +49,8 @@ template <int K> struct LensAcc { static constexpr int n = 0; };
 template <> struct LensAcc<glm::K_EPL> { static constexpr int n = 8; };
 template <> struct LensAcc<glm::K_SHEAR> { static constexpr int n = 2; };
 template <> struct LensAcc<glm::K_SIS> { static constexpr int n = 3; };
+template <> struct LensAcc<glm::K_SIE> { static constexpr int n = 5; };
+template <> struct LensAcc<glm::K_TNFW> { static constexpr int n = 5; };
 template <int K> struct LightAcc { static constexpr int n = (K == glm::K_SERSIC || K == glm::K_SERSIC_ELLIPSE) ? 8 : 0; };
 
 template <int L0, int L1, int L2, int A0, int A1, int S0, int S1>
@@ -368,6 +370,8 @@ struct Consts {
     glm::EplC epl[3];
     glm::ShearC shr[3];
     glm::SisC sis[3];
+    glm::SieC sie[3];
+    glm::TnfwC tnfw[3];
     glm::SerC ser[4];
     const float* coefA[3];
     const float4* coefB[3];
@@ -398,6 +402,10 @@ __device__ __forceinline__ void load_consts(const float* s_c, const float* s_coe
             C.shr[i] = uniform_load<glm::ShearC>(s_c + i * kConstStride);
         } else if (k == glm::K_SIS) {
             C.sis[i] = uniform_load<glm::SisC>(s_c + i * kConstStride);
+        } else if (k == glm::K_SIE) {
+            C.sie[i] = uniform_load<glm::SieC>(s_c + i * kConstStride);
+        } else if (k == glm::K_TNFW) {
+            C.tnfw[i] = uniform_load<glm::TnfwC>(s_c + i * kConstStride);
         }
     }
 #pragma unroll
@@ -419,12 +427,16 @@ __device__ __forceinline__ float eval_forward(const Consts<M>& C, float X, float
     float bx = X, by = Y;
     glm::EplState es;
     float sis_st[3];
+    glm::SieState sie_st;
+    glm::TnfwState tn_st;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         float ax = 0.0f, ay = 0.0f;
         if (M::lens[i] == glm::K_EPL) glm::epl_forward<false>(C.epl[i], C.coefA[i], nullptr, X, Y, ax, ay, es);
         else if (M::lens[i] == glm::K_SHEAR) glm::shear_forward(C.shr[i], X, Y, ax, ay);
         else if (M::lens[i] == glm::K_SIS) glm::sis_forward(C.sis[i], X, Y, ax, ay, sis_st);
+        else if (M::lens[i] == glm::K_SIE) glm::sie_forward<false>(C.sie[i], X, Y, ax, ay, sie_st);
+        else if (M::lens[i] == glm::K_TNFW) glm::tnfw_forward<false>(C.tnfw[i], X, Y, ax, ay, tn_st);
         bx -= ax;
         by -= ay;
     }
@@ -444,12 +456,16 @@ __device__ __forceinline__ void eval_adjoint(const Consts<M>& C, float X, float 
     float bx = X, by = Y;
     glm::EplState es[3];
     float sis_st[3][3];
+    glm::SieState sie_st[3];
+    glm::TnfwState tn_st[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         float ax = 0.0f, ay = 0.0f;
         if (M::lens[i] == glm::K_EPL) glm::epl_forward<true>(C.epl[i], C.coefA[i], C.coefB[i], X, Y, ax, ay, es[i]);
         else if (M::lens[i] == glm::K_SHEAR) glm::shear_forward(C.shr[i], X, Y, ax, ay);
         else if (M::lens[i] == glm::K_SIS) glm::sis_forward(C.sis[i], X, Y, ax, ay, sis_st[i]);
+        else if (M::lens[i] == glm::K_SIE) glm::sie_forward<true>(C.sie[i], X, Y, ax, ay, sie_st[i]);
+        else if (M::lens[i] == glm::K_TNFW) glm::tnfw_forward<true>(C.tnfw[i], X, Y, ax, ay, tn_st[i]);
         bx -= ax;
         by -= ay;
     }
@@ -472,6 +488,8 @@ __device__ __forceinline__ void eval_adjoint(const Consts<M>& C, float X, float 
         if (M::lens[i] == glm::K_EPL) glm::epl_adjoint(C.epl[i], es[i], Hx, Hy, acc + M::lens_off(i));
         else if (M::lens[i] == glm::K_SHEAR) glm::shear_adjoint(X, Y, Hx, Hy, acc + M::lens_off(i));
         else if (M::lens[i] == glm::K_SIS) glm::sis_adjoint(C.sis[i], sis_st[i], Hx, Hy, acc + M::lens_off(i));
+        else if (M::lens[i] == glm::K_SIE) glm::sie_adjoint(C.sie[i], sie_st[i], Hx, Hy, acc + M::lens_off(i));
+        else if (M::lens[i] == glm::K_TNFW) glm::tnfw_adjoint(C.tnfw[i], tn_st[i], Hx, Hy, acc + M::lens_off(i));
     }
 }
 
@@ -584,6 +602,14 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
         } else if (cd.kind == glm::K_SIS) {
             glm::SisC c{pv[0], pv[1], pv[2]};
             *reinterpret_cast<glm::SisC*>(dst) = c;
+        } else if (cd.kind == glm::K_SIE) {
+            glm::SieC c;
+            glm::sie_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], c);
+            *reinterpret_cast<glm::SieC*>(dst) = c;
+        } else if (cd.kind == glm::K_TNFW) {
+            glm::TnfwC c;
+            glm::tnfw_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], c);
+            *reinterpret_cast<glm::TnfwC*>(dst) = c;
         } else if (cd.kind == glm::K_SERSIC_ELLIPSE) {
             glm::SerC c;
             glm::sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.lstsq ? 1.0f : pv[6], 1, c);
@@ -920,6 +946,8 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
                 if (k == glm::K_EPL) { glm::epl_finish(C.epl[i], tot + M::lens_off(i), g); np = 6; }
                 else if (k == glm::K_SHEAR) { g[0] = tot[M::lens_off(i)]; g[1] = tot[M::lens_off(i) + 1]; np = 2; }
                 else if (k == glm::K_SIS) { glm::sis_finish(tot + M::lens_off(i), g); np = 3; }
+                else if (k == glm::K_SIE) { glm::sie_finish(C.sie[i], tot + M::lens_off(i), g); np = 5; }
+                else if (k == glm::K_TNFW) { glm::tnfw_finish(C.tnfw[i], tot + M::lens_off(i), g); np = 5; }
                 for (int p = 0; p < np; ++p)
                     if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[p]);
             }

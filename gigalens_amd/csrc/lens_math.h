@@ -65,7 +65,8 @@ GLM float rsq(float x) {
 // ---------------------------------------------------------------------------
 // Component constants (prepared once per particle)
 // ---------------------------------------------------------------------------
-enum Kind { K_EPL = 1, K_SHEAR = 2, K_SIS = 3, K_SERSIC = 10, K_SERSIC_ELLIPSE = 11, K_SHAPELETS = 12 };
+enum Kind { K_EPL = 1, K_SHEAR = 2, K_SIS = 3, K_SIE = 4, K_TNFW = 5, K_SERSIC = 10, K_SERSIC_ELLIPSE = 11,
+            K_SHAPELETS = 12 };
 
 // float slots of a component's constant block
 struct EplC {
@@ -80,6 +81,13 @@ struct SerC {
 };
 struct ShearC { float g1, g2; };
 struct SisC { float thetaE, cx, cy; };
+struct SieC {
+    float cx, cy, b, q, k2, cphi, sphi, thetaE, e1, e2, c, craw, sq;
+    int clipped;
+};
+struct TnfwC {
+    float cx, cy, Rs, alpha_Rs, rt, tau, T, P, dP, K0, K, xmin;
+};
 
 // Series length rule of epl.py:41,51-58 for a given f: the loop runs for
 // n = 1, 2, ... while n < log(1e-12)/log(f) + 2, at most niter times.
@@ -385,6 +393,238 @@ GLM void sersic_finish(const SerC& C, const float* acc, float* g) {
     g[4] = -2.0f * Ks * acc[6];
     g[5] = -2.0f * Ks * acc[7];
     g[6] = acc[0] / C.Ie;
+}
+
+// ---------------------------------------------------------------------------
+// SIE, src/gigalens/tf/profiles/mass/sie.py:13-43 (s_scale = 0).
+//   c = min(|e|, 0.9999), q = (1-c)/(1+c), b = theta_E sqrt(q) (the algebraic form of
+//   sie.py:19-21), k^2 = 1 - q^2 = 4c/(1+c)^2, (u, v) = R(phi)(x - c),
+//   psi = sqrt(q^2 u^2 + v^2), f_x = b/k atan(k u/psi), f_y = b/k atanh(k v/psi).
+// With s = u/psi (resp. v/psi) the two are written b s (1 + k^2 P_k) with
+//   P_k = (atan(k s)/(k s) - 1)/k^2,  Q_k = (atanh(k s)/(k s) - 1)/k^2,
+// which have the finite limits -s^2/3 and s^2/3 at k -> 0 (q = 1, where the
+// reference's expression is 0/0); power series below |k s| = 1/4.
+// ---------------------------------------------------------------------------
+GLM void sie_prepare(float thetaE, float e1, float e2, float cx, float cy, SieC& C) {
+    float phi = atan2f(e2, e1) * 0.5f;
+    float craw = sqrtf(e1 * e1 + e2 * e2);
+    float c = fminf(craw, 0.9999f);
+    float q = (1.0f - c) / (1.0f + c);
+    C.cx = cx; C.cy = cy;
+    C.q = q;
+    C.sq = sqrtf(q);
+    C.b = thetaE * C.sq;
+    C.k2 = 4.0f * c / ((1.0f + c) * (1.0f + c));
+    C.cphi = cosf(phi); C.sphi = sinf(phi);
+    C.thetaE = thetaE; C.e1 = e1; C.e2 = e2; C.c = c; C.craw = craw;
+    C.clipped = craw > 0.9999f ? 1 : 0;
+}
+
+// r = atan(A)/A (sign = -1) or atanh(A)/A (sign = +1) at A^2 = k2 s2, and
+// pk = (r - 1)/k2.
+GLM void sie_ratio(float k2, float s2, float sign, float& r, float& pk) {
+    const float A2 = k2 * s2;
+    if (A2 < 0.0625f) {
+        // (r - 1)/A^2 = sum_{n>=1} sign^n A^(2n-2)/(2n+1), through n = 7
+        const float a = sign * A2;
+        float t = fmaf(a, 1.0f / 15.0f, 1.0f / 13.0f);
+        t = fmaf(a, t, 1.0f / 11.0f);
+        t = fmaf(a, t, 1.0f / 9.0f);
+        t = fmaf(a, t, 1.0f / 7.0f);
+        t = fmaf(a, t, 1.0f / 5.0f);
+        t = fmaf(a, t, 1.0f / 3.0f);
+        pk = sign * s2 * t;
+        r = fmaf(k2, pk, 1.0f);
+    } else {
+        const float A = sqrtf(A2);
+        r = (sign < 0.0f ? atanf(A) : atanhf(A)) / A;
+        pk = (r - 1.0f) / k2;
+    }
+}
+
+struct SieState { float u, v, psi, fx, fy, dqx, dqy; };
+
+template <bool ADJ>
+GLM void sie_forward(const SieC& C, float X, float Y, float& ax, float& ay, SieState& S) {
+    float dx = X - C.cx, dy = Y - C.cy;
+    float u = fmaf(dx, C.cphi, dy * C.sphi);
+    float v = fmaf(-dx, C.sphi, dy * C.cphi);
+    float psi = sqrtf(fmaf(C.q * u, C.q * u, v * v));
+    float ip = 1.0f / psi;
+    float su = u * ip, sv = v * ip;
+    float ru, pu, rv, pv;
+    sie_ratio(C.k2, su * su, -1.0f, ru, pu);
+    sie_ratio(C.k2, sv * sv, 1.0f, rv, pv);
+    float fx = C.b * su * ru, fy = C.b * sv * rv;
+    ax = fx * C.cphi - fy * C.sphi;
+    ay = fx * C.sphi + fy * C.cphi;
+    if (ADJ) {
+        S.u = u; S.v = v; S.psi = psi; S.fx = fx; S.fy = fy;
+        // d f / d q at fixed b
+        S.dqx = C.q * C.b * su * pu;
+        S.dqy = C.q * C.b * sv * pv - C.b * sv / C.q;
+    }
+}
+
+// acc[0..4] = (sum h.f, S_u, S_v, S_phi, S_q), h = R(phi) H:
+//   J_f = b/(psi r^2) [[v^2, -uv], [-uv, u^2]],  (S_u, S_v) += J_f^T h,
+//   S_phi += h_v f_x - h_u f_y + b (v h_u - u h_v)/psi,  S_q += h . df/dq.
+GLM void sie_adjoint(const SieC& C, const SieState& S, float Hx, float Hy, float* acc) {
+    float hu = fmaf(Hx, C.cphi, Hy * C.sphi);
+    float hv = fmaf(-Hx, C.sphi, Hy * C.cphi);
+    acc[0] += fmaf(hu, S.fx, hv * S.fy);
+    float cr = fmaf(S.v, hu, -S.u * hv);
+    float bp = C.b / S.psi;
+    float w = bp * cr / fmaf(S.u, S.u, S.v * S.v);
+    acc[1] = fmaf(w, S.v, acc[1]);
+    acc[2] = fmaf(-w, S.u, acc[2]);
+    acc[3] += fmaf(hv, S.fx, fmaf(-hu, S.fy, bp * cr));
+    acc[4] += fmaf(hu, S.dqx, hv * S.dqy);
+}
+
+// g = d/d(theta_E, e1, e2, center_x, center_y)
+GLM void sie_finish(const SieC& C, const float* acc, float* g) {
+    float q = C.q;
+    float dLdb = acc[0] / C.b;
+    float dq = acc[4] + dLdb * C.b / (2.0f * q);
+    g[0] = dLdb * C.sq;
+    float dc = C.clipped ? 0.0f : dq * (-2.0f / ((1.0f + C.c) * (1.0f + C.c)));
+    float dphi = acc[3];
+    float cc = C.e1 * C.e1 + C.e2 * C.e2;
+    g[1] = dphi * (-C.e2 * 0.5f / cc) + dc * C.e1 / C.craw;
+    g[2] = dphi * (C.e1 * 0.5f / cc) + dc * C.e2 / C.craw;
+    g[3] = -(acc[1] * C.cphi - acc[2] * C.sphi);
+    g[4] = -(acc[1] * C.sphi + acc[2] * C.cphi);
+}
+
+// ---------------------------------------------------------------------------
+// Truncated NFW, src/gigalens/tf/profiles/mass/tnfw.py:14-58:
+//   alpha = K phi(x, tau) (dx, dy),  phi = g/x^2,  x = max(R, 0.001 Rs)/Rs,
+//   tau = r_trunc/Rs, K = alpha_Rs/(Rs (1 + ln 1/2)),  g = P(tau) Gb,
+//   P = tau^2/(tau^2+1)^2,
+//   Gb = (tau^2+1+2(x^2-1)) F + tau pi + (tau^2-1) ln tau + Sq (-pi + L (tau^2-1)/tau),
+//   Sq = sqrt(tau^2+x^2), L = ln(x/(tau+Sq)).
+// The terms of Gb are O(tau^2 ln x) and cancel to O(x^2 ln x) at small x, so
+// this evaluates the same function in the regrouped form
+//   Gb = (tau^2-1) B1 + x^2 C1,
+//   B1 = [F + ln(x/2)] + ln(2 tau/(tau+Sq)) + D L,   D = Sq/tau - 1 = x^2/(tau (tau+Sq)),
+//   C1 = 2F - pi/(tau+Sq),
+// with F + ln(x/2) = [ln((1+s)/2) + x^2 ln(2/x)/(1+s)]/s for x < 1 (s^2 = 1-x^2),
+// and the derivatives
+//   dB1/dx = x [(F-1)/s^2 + L/(tau Sq)],  dB1/dtau = -x^2 L/(tau^2 Sq),
+//   dC1/dx = 2F' + pi x/(Sq (tau+Sq)^2),  dC1/dtau = pi/(Sq (tau+Sq)).
+// F(x) = sum_n s2^n/(2n+1) (s2 = 1 - x^2) near x = 1, else atanh/atan forms.
+// ---------------------------------------------------------------------------
+GLM void tnfw_prepare(float Rs, float alpha_Rs, float rt, float cx, float cy, TnfwC& C) {
+    C.cx = cx; C.cy = cy; C.Rs = Rs; C.alpha_Rs = alpha_Rs; C.rt = rt;
+    float tau = rt / Rs;
+    float T = tau * tau;
+    C.tau = tau; C.T = T;
+    C.P = T / ((T + 1.0f) * (T + 1.0f));
+    C.dP = 2.0f * tau * (1.0f - T) / ((T + 1.0f) * (T + 1.0f) * (T + 1.0f));
+    C.K0 = 1.0f / (Rs * (1.0f - kLn2));
+    C.K = alpha_Rs * C.K0;
+    C.xmin = 0.001f;
+}
+
+// F(x), (F - 1)/s2 and F'(x) with s2 = 1 - x^2.
+GLM void tnfw_F(float x, float& F, float& Fm1, float& dF, float& FL2) {
+    const float s2 = (1.0f - x) * (1.0f + x);
+    if (fabsf(s2) < 0.1f) {
+        // F = sum_{n>=0} s2^n/(2n+1); (F-1)/s2 and dF/ds2 through n = 9
+        float t = fmaf(s2, 1.0f / 19.0f, 1.0f / 17.0f);
+        t = fmaf(s2, t, 1.0f / 15.0f);
+        t = fmaf(s2, t, 1.0f / 13.0f);
+        t = fmaf(s2, t, 1.0f / 11.0f);
+        t = fmaf(s2, t, 1.0f / 9.0f);
+        t = fmaf(s2, t, 1.0f / 7.0f);
+        t = fmaf(s2, t, 1.0f / 5.0f);
+        t = fmaf(s2, t, 1.0f / 3.0f);
+        Fm1 = t;
+        F = fmaf(s2, t, 1.0f);
+        float d = fmaf(s2, 9.0f / 19.0f, 8.0f / 17.0f);
+        d = fmaf(s2, d, 7.0f / 15.0f);
+        d = fmaf(s2, d, 6.0f / 13.0f);
+        d = fmaf(s2, d, 5.0f / 11.0f);
+        d = fmaf(s2, d, 4.0f / 9.0f);
+        d = fmaf(s2, d, 3.0f / 7.0f);
+        d = fmaf(s2, d, 2.0f / 5.0f);
+        d = fmaf(s2, d, 1.0f / 3.0f);
+        dF = -2.0f * x * d;
+        FL2 = F + logf(0.5f * x);
+    } else if (x < 1.0f) {
+        const float s = sqrtf(s2);
+        const float l2x = logf(2.0f / x);
+        F = (logf(1.0f + s) - logf(x)) / s;
+        Fm1 = (F - 1.0f) / s2;
+        dF = -(1.0f - x * x * F) / (x * s2);
+        FL2 = (log1pf(-x * x / (2.0f * (1.0f + s))) + x * x * l2x / (1.0f + s)) / s;
+    } else {
+        const float s = sqrtf(-s2);
+        F = atanf(s) / s;
+        Fm1 = (F - 1.0f) / s2;
+        dF = -(1.0f - x * x * F) / (x * s2);
+        FL2 = F + logf(0.5f * x);
+    }
+}
+
+struct TnfwState { float dx, dy, R, x, phi, phix, phit; int clamped; };
+
+template <bool ADJ>
+GLM void tnfw_forward(const TnfwC& C, float X, float Y, float& ax, float& ay, TnfwState& S) {
+    const float kPi = 3.14159265358979f;
+    float dx = X - C.cx, dy = Y - C.cy;
+    float R = sqrtf(fmaf(dx, dx, dy * dy));
+    float x = R / C.Rs;
+    int clamped = 0;
+    if (!(x >= C.xmin)) { x = C.xmin; clamped = 1; }
+    float F, Fm1, dF, FL2;
+    tnfw_F(x, F, Fm1, dF, FL2);
+    const float tau = C.tau, x2 = x * x;
+    const float Sq = sqrtf(fmaf(x, x, C.T));
+    const float ts = tau + Sq;
+    const float L = logf(x) - logf(ts);
+    const float D = x2 / (tau * ts);
+    const float B1 = FL2 - log1pf(x2 / (2.0f * tau * ts)) + D * L;
+    const float C1 = 2.0f * F - kPi / ts;
+    const float Gb = (C.T - 1.0f) * B1 + x2 * C1;
+    const float ix2 = 1.0f / x2;
+    const float phi = C.P * Gb * ix2;
+    const float a = C.K * phi;
+    ax = a * dx;
+    ay = a * dy;
+    if (ADJ) {
+        const float dB1x = x * (Fm1 + L / (tau * Sq));
+        const float dC1x = 2.0f * dF + kPi * x / (Sq * ts * ts);
+        const float Gbx = (C.T - 1.0f) * dB1x + 2.0f * x * C1 + x2 * dC1x;
+        const float dB1t = -x2 * L / (C.T * Sq);
+        const float Gbt = 2.0f * tau * B1 + (C.T - 1.0f) * dB1t + x2 * kPi / (Sq * ts);
+        S.dx = dx; S.dy = dy; S.R = R; S.x = x; S.phi = phi;
+        S.phix = clamped ? 0.0f : C.P * ix2 * (Gbx - 2.0f * Gb / x);
+        S.phit = (C.dP * Gb + C.P * Gbt) * ix2;
+        S.clamped = clamped;
+    }
+}
+
+// acc[0..4] = (sum phi H.d, sum x phi_x H.d, sum phi_tau H.d, S_cx, S_cy), all
+// without the factor K; (S_cx, S_cy) = d(H.alpha)/d(center) / K.
+GLM void tnfw_adjoint(const TnfwC& C, const TnfwState& S, float Hx, float Hy, float* acc) {
+    float Hd = fmaf(Hx, S.dx, Hy * S.dy);
+    acc[0] = fmaf(S.phi, Hd, acc[0]);
+    acc[1] = fmaf(S.x * S.phix, Hd, acc[1]);
+    acc[2] = fmaf(S.phit, Hd, acc[2]);
+    float w = S.clamped ? 0.0f : S.phix * Hd / (S.R * C.Rs);
+    acc[3] -= fmaf(S.phi, Hx, w * S.dx);
+    acc[4] -= fmaf(S.phi, Hy, w * S.dy);
+}
+
+// g = d/d(Rs, alpha_Rs, r_trunc, center_x, center_y)
+GLM void tnfw_finish(const TnfwC& C, const float* acc, float* g) {
+    g[0] = -C.K * (acc[0] + acc[1] + C.tau * acc[2]) / C.Rs;
+    g[1] = C.K0 * acc[0];
+    g[2] = C.K * acc[2] / C.Rs;
+    g[3] = C.K * acc[3];
+    g[4] = C.K * acc[4];
 }
 
 // ---------------------------------------------------------------------------

@@ -236,6 +236,14 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
         } else if (cd.kind == glm::K_SIS) {
             glm::SisC c{pv[0], pv[1], pv[2]};
             *reinterpret_cast<glm::SisC*>(dst) = c;
+        } else if (cd.kind == glm::K_SIE) {
+            glm::SieC c;
+            glm::sie_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], c);
+            *reinterpret_cast<glm::SieC*>(dst) = c;
+        } else if (cd.kind == glm::K_TNFW) {
+            glm::TnfwC c;
+            glm::tnfw_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], c);
+            *reinterpret_cast<glm::TnfwC*>(dst) = c;
         } else if (cd.kind == glm::K_SERSIC_ELLIPSE) {
             glm::SerC c;
             glm::sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.lstsq ? 1.0f : pv[6], 1, c);
@@ -269,10 +277,18 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
     if (!M.lstsq && tid == 0) M.coef[(size_t)q * M.D] = P.cf;  // fixed amplitudes: image = cf * conv
 }
 
-// Deflection of all lenses of particle p at (X, Y); keeps the EPL states when ADJ.
+// Per-lens states kept by the recompute for the deflection adjoint.
+struct LensStates {
+    glm::EplState es[kMaxLens];
+    float sis[kMaxLens][3];
+    glm::SieState sie[kMaxLens];
+    glm::TnfwState tn[kMaxLens];
+};
+
+// Deflection of all lenses of particle p at (X, Y); keeps the lens states when ADJ.
 template <bool ADJ>
 __device__ __forceinline__ void mc_beta(const DPlan& P, const McPlan& M, const float* pc, float X, float Y, float& bx,
-                                        float& by, glm::EplState* es, float (*sis_st)[3]) {
+                                        float& by, LensStates* ls) {
     const float* s_c = pc + ((6 * P.d + 3) & ~3);
     bx = X;
     by = Y;
@@ -288,7 +304,7 @@ __device__ __forceinline__ void mc_beta(const DPlan& P, const McPlan& M, const f
             const float4* cb = reinterpret_cast<const float4*>(ca + ((P.max_n + 4) & ~3));
             glm::EplState st;
             glm::epl_forward<ADJ>(C, ca, cb, X, Y, ax, ay, st);
-            if (ADJ) es[i] = st;
+            if (ADJ) ls->es[i] = st;
         } else if (kind == glm::K_SHEAR) {
             const glm::ShearC C = *reinterpret_cast<const glm::ShearC*>(cst);
             glm::shear_forward(C, X, Y, ax, ay);
@@ -296,7 +312,17 @@ __device__ __forceinline__ void mc_beta(const DPlan& P, const McPlan& M, const f
             const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
             float st[3];
             glm::sis_forward(C, X, Y, ax, ay, st);
-            if (ADJ) { sis_st[i][0] = st[0]; sis_st[i][1] = st[1]; sis_st[i][2] = st[2]; }
+            if (ADJ) { ls->sis[i][0] = st[0]; ls->sis[i][1] = st[1]; ls->sis[i][2] = st[2]; }
+        } else if (kind == glm::K_SIE) {
+            const glm::SieC C = *reinterpret_cast<const glm::SieC*>(cst);
+            glm::SieState st;
+            glm::sie_forward<ADJ>(C, X, Y, ax, ay, st);
+            if (ADJ) ls->sie[i] = st;
+        } else if (kind == glm::K_TNFW) {
+            const glm::TnfwC C = *reinterpret_cast<const glm::TnfwC*>(cst);
+            glm::TnfwState st;
+            glm::tnfw_forward<ADJ>(C, X, Y, ax, ay, st);
+            if (ADJ) ls->tn[i] = st;
         }
         bx -= ax;
         by -= ay;
@@ -330,7 +356,7 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
     const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
     const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
     float bx, by;
-    mc_beta<false>(P, M, pc, X, Y, bx, by, nullptr, nullptr);
+    mc_beta<false>(P, M, pc, X, Y, bx, by, nullptr);
     float* Bp = M.B + (size_t)p * M.D * nsub * M.BW + (size_t)y * M.BW + P.pb + x;
     const size_t cstride = (size_t)nsub * M.BW;
     if (!M.lstsq) {
@@ -927,10 +953,9 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
         const float g = M.B[((size_t)p * M.D * nsub + y) * M.BW + P.pb + x];
         const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
         const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
-        glm::EplState es[3];
-        float sis_st[3][3];
+        LensStates ls;
         float bx, by;
-        mc_beta<true>(P, M, pc, X, Y, bx, by, es, sis_st);
+        mc_beta<true>(P, M, pc, X, Y, bx, by, &ls);
         if (!M.lstsq) {
             // the sum of the lights: a NaN zeroed the whole sub-pixel in the forward pass
             float v = 0.0f;
@@ -999,12 +1024,18 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
             float* acc = st + 8 * i;
             if (kind == glm::K_EPL) {
                 const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
-                glm::epl_adjoint(C, es[i], Hx, Hy, acc);
+                glm::epl_adjoint(C, ls.es[i], Hx, Hy, acc);
             } else if (kind == glm::K_SHEAR) {
                 glm::shear_adjoint(X, Y, Hx, Hy, acc);
             } else if (kind == glm::K_SIS) {
                 const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
-                glm::sis_adjoint(C, sis_st[i], Hx, Hy, acc);
+                glm::sis_adjoint(C, ls.sis[i], Hx, Hy, acc);
+            } else if (kind == glm::K_SIE) {
+                const glm::SieC C = *reinterpret_cast<const glm::SieC*>(cst);
+                glm::sie_adjoint(C, ls.sie[i], Hx, Hy, acc);
+            } else if (kind == glm::K_TNFW) {
+                const glm::TnfwC C = *reinterpret_cast<const glm::TnfwC*>(cst);
+                glm::tnfw_adjoint(C, ls.tn[i], Hx, Hy, acc);
             }
         }
     }
@@ -1061,6 +1092,12 @@ __global__ void mc_finish(const McTable T, const McArgs A) {
             } else if (cd.kind == glm::K_SIS) {
                 glm::sis_finish(acc, g);
                 np_ = 3;
+            } else if (cd.kind == glm::K_SIE) {
+                glm::sie_finish(*reinterpret_cast<const glm::SieC*>(cst), acc, g);
+                np_ = 5;
+            } else if (cd.kind == glm::K_TNFW) {
+                glm::tnfw_finish(*reinterpret_cast<const glm::TnfwC*>(cst), acc, g);
+                np_ = 5;
             } else if (cd.kind == glm::K_SERSIC_ELLIPSE || cd.kind == glm::K_SERSIC) {
                 glm::sersic_finish(*reinterpret_cast<const glm::SerC*>(cst), acc, g);
                 np_ = cd.kind == glm::K_SERSIC_ELLIPSE ? 6 : 4;  // no Ie in least-squares mode

@@ -29,6 +29,8 @@ _FWD = {
     nat.GL_SERSIC_ELLIPSE: (19, 3),
     nat.GL_SHEAR: (6, 0),
     nat.GL_SIS: (12, 1),
+    nat.GL_SIE: (62, 2),
+    nat.GL_TNFW: (55, 10),
     nat.GL_EPL: (44, 3),  # + 3 per series term
 }
 # adjoint recompute + reverse statistics (the forward is recomputed)
@@ -37,6 +39,8 @@ _ADJ = {
     nat.GL_SERSIC_ELLIPSE: (19 + 24, 3 + 1),
     nat.GL_SHEAR: (6 + 8, 0),
     nat.GL_SIS: (12 + 12, 1),
+    nat.GL_SIE: (62 + 38, 2 + 3),
+    nat.GL_TNFW: (55 + 40, 10 + 4),
     nat.GL_EPL: (44 + 12 + 59, 3 + 1),  # + 9 per series term (three series)
 }
 

@@ -23,6 +23,8 @@ _CANON = {
     nat.GL_EPL: ["theta_E", "gamma", "e1", "e2", "center_x", "center_y"],
     nat.GL_SHEAR: ["gamma1", "gamma2"],
     nat.GL_SIS: ["theta_E", "center_x", "center_y"],
+    nat.GL_SIE: ["theta_E", "e1", "e2", "center_x", "center_y"],
+    nat.GL_TNFW: ["Rs", "alpha_Rs", "r_trunc", "center_x", "center_y"],
     nat.GL_SERSIC: ["R_sersic", "n_sersic", "center_x", "center_y", "Ie"],
     nat.GL_SERSIC_ELLIPSE: ["R_sersic", "n_sersic", "e1", "e2", "center_x", "center_y", "Ie"],
     nat.GL_SHAPELETS: ["beta", "center_x", "center_y"],

@@ -1,2 +1,2 @@
 """Mass profiles (mirrors ``gigalens.tf.profiles.mass``)."""
-from . import epl, shear, sis  # noqa: F401
+from . import epl, shear, sie, sis, tnfw  # noqa: F401

@@ -38,6 +38,8 @@ enum gl_profile_kind {
     GL_EPL = 1,            /* params: theta_E, gamma, e1, e2, center_x, center_y  (epl.py:17) */
     GL_SHEAR = 2,          /* params: gamma1, gamma2                              (shear.py:10) */
     GL_SIS = 3,            /* params: theta_E, center_x, center_y                 (sis.py:11) */
+    GL_SIE = 4,            /* params: theta_E, e1, e2, center_x, center_y         (sie.py:12) */
+    GL_TNFW = 5,           /* params: Rs, alpha_Rs, r_trunc, center_x, center_y   (tnfw.py:12) */
     GL_SERSIC = 10,        /* params: R_sersic, n_sersic, center_x, center_y, Ie   (sersic.py:23) */
     GL_SERSIC_ELLIPSE = 11,/* params: R_sersic, n_sersic, e1, e2, center_x, center_y, Ie (sersic.py:64) */
     GL_SHAPELETS = 12      /* params: beta, center_x, center_y (+ amplitudes)      (shapelets.py:17) */

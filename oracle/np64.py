@@ -13,6 +13,11 @@ Pinning (what checks each function beyond retyping):
   * ``epl_deriv``      -- scipy ``hyp2f1`` closed form, q=1 power law, the
                           reference SIE formula at gamma=2 (tests/test_oracle_profiles.py)
   * ``shear_deriv``    -- closed form
+  * ``sie_deriv``      -- equals ``epl_deriv`` (hyp2f1 form) at gamma = 2
+  * ``tnfw_deriv``     -- the NFW closed form in the limit r_trunc -> inf, and
+                          alpha R equal to the projected (cylinder) mass of the
+                          truncated-NFW 3-D density integrated with scipy quad
+                          (tests/test_oracle_tnfw.py)
   * ``sersic_*``       -- known answer light(0,1)=Ie (tests/test_profiles.py:17-26)
                           and an independent rotated-ellipse construction
   * ``shapelet_*``     -- numpy.polynomial.hermite, orthonormality integrals
@@ -293,6 +298,37 @@ def sie_deriv(x, y, theta_E, e1, e2, center_x, center_y):
     return _rotate(fx, fy, -phi)
 
 
+def tnfw_F(x):
+    """F(x) of ``src/gigalens/tf/profiles/mass/tnfw.py:40-58``: atanh/atan forms, 1 at x = 1."""
+    x = np.asarray(x, np.float64)
+    out = np.ones_like(x)
+    lo, hi = x < 1, x > 1
+    s = np.sqrt(1 - x[lo] ** 2)
+    out[lo] = np.arctanh(s) / s
+    s = np.sqrt(x[hi] ** 2 - 1)
+    out[hi] = np.arctan(s) / s
+    return out
+
+
+def tnfw_deriv(x, y, Rs, alpha_Rs, r_trunc, center_x, center_y):
+    """Truncated-NFW deflection, ``src/gigalens/tf/profiles/mass/tnfw.py:14-38``."""
+    Rs, aR, rt, cx, cy = [np.asarray(v, np.float64).reshape(-1, 1, 1)
+                          for v in (Rs, alpha_Rs, r_trunc, center_x, center_y)]
+    rho0 = aR / (4.0 * Rs ** 2 * (1.0 + np.log(0.5)))
+    dx, dy = x[None] - cx, y[None] - cy
+    R = np.sqrt(dx ** 2 + dy ** 2)
+    R = np.maximum(R, 0.001 * Rs)
+    xx = R / Rs
+    tau = rt / Rs
+    L = np.log(xx / (tau + np.sqrt(tau ** 2 + xx ** 2)))
+    F = tnfw_F(xx)
+    gx = (tau ** 2) / (tau ** 2 + 1) ** 2 * (
+        (tau ** 2 + 1 + 2 * (xx ** 2 - 1)) * F + tau * np.pi + (tau ** 2 - 1) * np.log(tau)
+        + np.sqrt(tau ** 2 + xx ** 2) * (-np.pi + L * (tau ** 2 - 1) / tau))
+    a = 4 * rho0 * Rs * gx / xx ** 2
+    return a * dx, a * dy
+
+
 def sis_deriv(x, y, theta_E, center_x, center_y):
     """SIS deflection, ``src/gigalens/tf/profiles/mass/sis.py:12-18``."""
     th, cx, cy = [np.asarray(v, np.float64).reshape(-1, 1, 1) for v in (theta_E, center_x, center_y)]
@@ -513,6 +549,7 @@ MASS = {
     "SHEAR": lambda x, y, **p: shear_deriv(x, y, **p),
     "SIS": lambda x, y, **p: sis_deriv(x, y, **p),
     "SIE": lambda x, y, **p: sie_deriv(x, y, **p),
+    "TNFW": lambda x, y, **p: tnfw_deriv(x, y, **p),
 }
 
 

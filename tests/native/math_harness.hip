@@ -74,6 +74,32 @@ extern "C" void h_sis(const float* p, int n, const float* x, const float* y, con
     sis_finish(acc, g);
 }
 
+extern "C" void h_sie(const float* p, int n, const float* x, const float* y, const float* Hx, const float* Hy,
+                      float* ax, float* ay, float* g) {
+    SieC C;
+    sie_prepare(p[0], p[1], p[2], p[3], p[4], C);
+    float acc[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        SieState S;
+        sie_forward<true>(C, x[i], y[i], ax[i], ay[i], S);
+        sie_adjoint(C, S, Hx[i], Hy[i], acc);
+    }
+    sie_finish(C, acc, g);
+}
+
+extern "C" void h_tnfw(const float* p, int n, const float* x, const float* y, const float* Hx, const float* Hy,
+                       float* ax, float* ay, float* g) {
+    TnfwC C;
+    tnfw_prepare(p[0], p[1], p[2], p[3], p[4], C);
+    float acc[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        TnfwState S;
+        tnfw_forward<true>(C, x[i], y[i], ax[i], ay[i], S);
+        tnfw_adjoint(C, S, Hx[i], Hy[i], acc);
+    }
+    tnfw_finish(C, acc, g);
+}
+
 extern "C" void h_prior(int kind, const float* args, float z, int fldj_at_x, float* out) {
     PriorOut o;
     prior_eval(kind, args, z, fldj_at_x, o);

@@ -202,6 +202,57 @@ def test_sis_forward_and_adjoint(mh):
     assert np.allclose(g, fd, rtol=1e-3, atol=1e-3)
 
 
+def _mass_check(mh, fn, oracle, prm, x, y, Hx, Hy, h, rtol_fwd, rtol_grad):
+    n = len(x)
+    ax, ay, g = np.empty(n, np.float32), np.empty(n, np.float32), np.zeros(len(prm), np.float32)
+    fn(p(f32(prm)), n, p(x), p(y), p(Hx), p(Hy), p(ax), p(ay), p(g))
+    X, Y = x.astype(np.float64), y.astype(np.float64)
+    rx, ry = oracle(X, Y, *prm)
+    rx, ry = rx.reshape(-1), ry.reshape(-1)
+    scale = max(np.abs(rx).max(), np.abs(ry).max())
+    assert np.abs(ax - rx).max() <= rtol_fwd * scale and np.abs(ay - ry).max() <= rtol_fwd * scale
+    fd = np.zeros(len(prm))
+    for j in range(len(prm)):
+        pp, pm = list(prm), list(prm)
+        step = h * max(abs(prm[j]), 0.1)
+        pp[j] += step
+        pm[j] -= step
+        a1, a0 = oracle(X, Y, *pp), oracle(X, Y, *pm)
+        fd[j] = np.sum(Hx * (a1[0] - a0[0]).reshape(-1) + Hy * (a1[1] - a0[1]).reshape(-1)) / (2 * step)
+    assert np.allclose(g, fd, rtol=rtol_grad, atol=rtol_grad * np.abs(fd).max()), (g, fd)
+
+
+@pytest.mark.parametrize("prm", [[1.0, 1e-3, 1e-3, 0.0, 0.0], [1.2, 0.1, -0.1, 0.0, 0.0],
+                                 [0.9, -0.3, 0.2, 0.05, -0.1], [1.1, 0.02, 0.0, -0.2, 0.1]])
+def test_sie_forward_and_adjoint(mh, prm):
+    """SIE (sie.py:13-43): deflection vs the FP64 reference formula at the points of
+    the reference's test_sie (test_profiles.py:82-95) and others, reverse-mode
+    statistics vs FP64 central differences."""
+    rng = np.random.default_rng(7)
+    n = 2000
+    x, y, Hx, Hy = (f32(rng.normal(size=n)) for _ in range(4))
+    _mass_check(mh, mh.h_sie, np64.sie_deriv, prm, x, y, Hx, Hy, 1e-5, 2e-6, 1e-4)
+
+
+@pytest.mark.parametrize("prm", [[1.0, 0.5, 5.0, 0.0, 0.0], [0.3, 0.05, 2.0, 0.1, -0.05],
+                                 [2.0, 1.2, 40.0, -0.3, 0.2]])
+def test_tnfw_forward_and_adjoint(mh, prm):
+    """Truncated NFW (tnfw.py:14-58): the device form (regrouped for FP32) vs the
+    oracle's literal FP64 expression, including points inside the 0.001 Rs clamp
+    and around x = 1; reverse-mode statistics vs FP64 central differences."""
+    rng = np.random.default_rng(8)
+    n = 2000
+    x, y = f32(rng.normal(size=n) * 1.5), f32(rng.normal(size=n) * 1.5)
+    Rs, cx, cy = prm[0], prm[3], prm[4]
+    ring = np.linspace(0, 2 * np.pi, 40)
+    x[:40] = f32(cx + Rs * (1 + 0.03 * np.sin(5 * ring)) * np.cos(ring))
+    y[:40] = f32(cy + Rs * (1 + 0.03 * np.sin(5 * ring)) * np.sin(ring))
+    x[40:45] = f32(cx + Rs * np.array([1e-4, 5e-4, 2e-3, 1e-2, 0.05]))
+    y[40:45] = f32(cy)
+    Hx, Hy = f32(rng.normal(size=n)), f32(rng.normal(size=n))
+    _mass_check(mh, mh.h_tnfw, np64.tnfw_deriv, prm, x, y, Hx, Hy, 1e-5, 5e-6, 1e-4)
+
+
 @pytest.mark.parametrize("kind,args", [(1, (0.0, 0.1)), (2, (math.log(1.25), 0.25)), (3, (0.5, 8.0)),
                                        (4, (2.0, 0.25, 1.2, 2.8))])
 @pytest.mark.parametrize("fldj_at_x", [0, 1])
