@@ -83,6 +83,7 @@ const ModelEntry kModels[] = {
 struct gl_plan {
     gl::DPlan P;
     const ModelEntry* model;
+    int mc_model;            // staged fixed-amplitude plan of the reference default model: 1 (specialised kernels)
     size_t lds_bytes;
     int epl_batch;
     int mc;                  // 1: multi-channel least-squares plan
@@ -318,6 +319,11 @@ static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int 
     M.gimg = (float*)dptr;
     if ((rc = dev_alloc(owned, (size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
     M.stats = (float*)dptr;
+    M.U = nullptr;
+    if (M.adj_rank > 0) {
+        if ((rc = dev_alloc(owned, (size_t)mb * M.adj_rank * N * ss * N * 4, &dptr, false))) return rc;
+        M.U = (float*)dptr;
+    }
     return GL_OK;
 }
 
@@ -394,11 +400,35 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     // conv on the shapelet configuration (the autotuner times both).
     M.sep_rank = 0;
     std::vector<float> sa, sbp;
+    std::vector<double> Wd(kf * kf), sv, Uv, Vv;
+    for (int i = 0; i < kf * kf; ++i) Wd[i] = W[i];
+    jacobi_svd(kf, Wd, sv, Uv, Vv);
+    // separable adjoint conv (default; psf_rank_tol < 0 keeps the direct form): rank at
+    // 1e-6 (or the given tolerance), at most 2, when its two passes cost under half the
+    // direct taps.  Higher ranks measured slower: the R planes of U per particle (1.1 MB
+    // at rank 7, 100x100, ss 4) fall out of L2, while the direct taps re-read a small image.
+    M.adj_rank = 0;
+    std::vector<float> asa, asb;
+    if (desc->psf_rank_tol >= 0.0f && sv.size() && sv[0] > 0.0) {
+        const double tol = desc->psf_rank_tol > 0.0f ? desc->psf_rank_tol : 1e-6;
+        int R = 0;
+        while (R < kf && sv[R] >= tol * sv[0]) ++R;
+        const double mt = (double)(kf + ss - 1) / ss;  // taps per phase, rounded up
+        const double direct = (double)nsub * nsub * mt * mt;
+        const double sep = (double)R * ((double)N * nsub * mt + (double)nsub * nsub * mt);
+        if (R >= 1 && R <= 2 && sep < 0.5 * direct) {
+            M.adj_rank = R;
+            asa.assign((size_t)R * kf, 0.0f);
+            asb.assign((size_t)R * kf, 0.0f);
+            for (int r = 0; r < R; ++r)
+                for (int i = 0; i < kf; ++i) {
+                    asa[(size_t)r * kf + i] = (float)(sv[r] * Uv[i * kf + r]);
+                    asb[(size_t)r * kf + i] = (float)Vv[i * kf + r];
+                }
+        }
+    }
     if (desc->psf_rank_tol > 0.0f) {
         const double tol = desc->psf_rank_tol;
-        std::vector<double> Wd(kf * kf), sv, Uv, Vv;
-        for (int i = 0; i < kf * kf; ++i) Wd[i] = W[i];
-        jacobi_svd(kf, Wd, sv, Uv, Vv);
         int R = 0;
         while (R < kf && sv[R] >= tol * sv[0]) ++R;
         const double ratio = (double)R * (nsub + N) / ((double)N * kf);
@@ -418,7 +448,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     const int BW = ((ss * (N - 1) + 4 * p->kf4 > pb + nsub ? ss * (N - 1) + 4 * p->kf4 : pb + nsub) + 3) & ~3;
     const double per_particle = 4.0 * ((double)M.pstride + (double)D * nsub * BW + (double)D * N * N + D +
                                        (double)GW * GW + gl::kMcStat +
-                                       (double)D * M.sep_rank * nsub * N) +
+                                       (double)D * M.sep_rank * nsub * N + (double)M.adj_rank * N * nsub) +
                                 8.0 * 4 * p->ksplit_cap * gram_nt * 256;
     int mb = desc->max_batch;
     if (mb <= 0) mb = (int)std::max(1.0, std::min(1024.0, 32.0 * 1073741824.0 / per_particle));
@@ -458,6 +488,12 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     }
     // workspace for max_batch particles (allocated here once; calls never allocate)
     if ((rc = mc_workspace(P, M, mb, p->kf4, p->ksplit_cap, W, p->dev))) return rc;
+    if (M.adj_rank > 0) {
+        if ((rc = upload(p, asa.data(), asa.size() * sizeof(float), &dptr))) return rc;
+        M.asa = (const float*)dptr;
+        if ((rc = upload(p, asb.data(), asb.size() * sizeof(float), &dptr))) return rc;
+        M.asb = (const float*)dptr;
+    }
     if (M.sep_rank > 0) {
         if ((rc = upload(p, sa.data(), sa.size() * sizeof(float), &dptr))) return rc;
         M.sa = (const float*)dptr;
@@ -481,7 +517,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
 
 // Work-unit totals of a lens with `count` particles (the prefix increments of a table).
 struct McUnits {
-    long part, basis, conv, adj, grad;
+    long part, basis, conv, adj, grad, adjh;
 };
 static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     McUnits u;
@@ -491,17 +527,25 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     u.grad = (long)count * (((long)P.nsub * P.nsub + 256 * gl::kMcGradK - 1) / (256 * gl::kMcGradK));
     u.conv = (long)count * M.D * ((P.N + M.conv_ib - 1) / M.conv_ib) * njb;
     u.adj = (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
+    u.adjh = M.adj_rank > 0 ? (long)count * (((long)P.N * P.ss * njb + 3) & ~3L) : 0;
     return u;
 }
 
 // One pass of the staged pipeline over the lenses of table T.
+// model 1: every lens is a fixed-amplitude plan of the reference default model
+// (EPL + shear | Sersic ellipse | Sersic ellipse): model-specialised basis and gradient.
+using DefaultModel = gl::ModelT<EPL, SHR, 0, SE, 0, SE, 0>;
+
 template <bool MULTI>
 static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch, int kf4, int lstsq, int gram_nb,
-                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st, bool shp = false) {
+                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st, bool shp = false, int model = 0) {
     const int np = T.tot_part;
     hipLaunchKernelGGL((gl::mc_prologue<MULTI>), dim3(np), dim3(128), 0, st, T, A);
     GL_HIP(hipGetLastError());
-    hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
+    if (model == 1)
+        hipLaunchKernelGGL((gl::mc_basis_m<MULTI, DefaultModel>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
+    else
+        hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
     GL_HIP(hipGetLastError());
     if (!MULTI && T.one.M.sep_rank > 0) {
         const gl::DPlan& P = T.one.P;
@@ -523,9 +567,15 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
     hipLaunchKernelGGL((gl::mc_image<MULTI>), dim3(np), dim3(256), 0, st, T, A, grad ? 1 : 0);
     GL_HIP(hipGetLastError());
     if (grad) {
+        if (T.tot_adjh > 0) {
+            hipLaunchKernelGGL((gl::mc_adj_h<MULTI>), dim3(T.tot_adjh / 4), dim3(256), T.lds_w, st, T);
+            GL_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), T.lds_w, st, T);
         GL_HIP(hipGetLastError());
-        if (shp)
+        if (model == 1)
+            hipLaunchKernelGGL((gl::mc_grad_m<MULTI, DefaultModel>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
+        else if (shp)
             hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
         else
             hipLaunchKernelGGL((gl::mc_grad<MULTI, false>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
@@ -569,14 +619,15 @@ static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStrea
         T.one.weight = 1.0f / (float)bs;
         T.n = 1;
         const McUnits u = mc_units(P, M, nb);
-        if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX)
+        if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX || u.adjh > INT32_MAX)
             return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
         T.tot_part = nb; T.tot_basis = (int)u.basis; T.tot_conv = (int)u.conv; T.tot_adj = (int)u.adj;
         T.tot_grad = (int)u.grad;
+        T.tot_adjh = (int)u.adjh;
         T.lds_consts = M.pstride * 4;
         T.lds_w = P.kf * P.kf * 4;
         int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
-                                    grad, st, M.shp_comp >= 0);
+                                    grad, st, M.shp_comp >= 0, plan->mc_model);
         if (rc) return rc;
     }
     return GL_OK;
@@ -758,6 +809,13 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         }
         p->mc = 1;
         p->model = nullptr;
+        p->mc_model = 0;
+        if (!lstsq) {
+            const int dflt[7] = {EPL, SHR, 0, SE, 0, SE, 0};
+            bool ok = true;
+            for (int j = 0; j < 7; ++j) ok = ok && sig[j] == dflt[j];
+            p->mc_model = ok ? 1 : 0;
+        }
         int rc = mc_create(desc, p, W, max_niter, lstsq);
         if (rc) { gl_plan_destroy(p); return rc; }
         *out = p;
@@ -1083,6 +1141,7 @@ extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* 
 // ---------------------------------------------------------------------------
 struct gl_batch {
     int n, d, total, ss_launch, kf4;
+    int mc_model;  // 1: every lens is of the reference default model (specialised kernels)
     gl::McTable T;
     std::vector<int> counts;
     std::vector<void*> dev;
@@ -1111,14 +1170,17 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     b->d = plans[0]->P.d;
     b->kf4 = 0;
     b->ss_launch = plans[0]->P.ss;
+    b->mc_model = 1;
+    for (int i = 0; i < n; ++i)
+        if (plans[i]->mc_model != 1) b->mc_model = 0;
     for (int i = 0; i < n; ++i) {
         b->kf4 = std::max(b->kf4, plans[i]->kf4);
         if (plans[i]->P.ss != b->ss_launch) b->ss_launch = 0;  // mixed supersampling: runtime ss
     }
     std::vector<gl::McLens> lens(n);
     std::vector<int> pre_part(n + 1, 0), pre_basis(n + 1, 0), pre_conv(n + 1, 0), pre_adj(n + 1, 0),
-        pre_grad(n + 1, 0);
-    long acc[5] = {0, 0, 0, 0, 0};
+        pre_grad(n + 1, 0), pre_adjh(n + 1, 0);
+    long acc[6] = {0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         gl::McLens& E = lens[i];
         memset(&E, 0, sizeof(E));
@@ -1133,13 +1195,15 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         E.weight = 1.0f / (float)counts[i];
         const McUnits u = mc_units(E.P, E.M, counts[i]);
         acc[0] += u.part; acc[1] += u.basis; acc[2] += u.conv; acc[3] += u.adj; acc[4] += u.grad;
-        for (int k = 0; k < 5; ++k)
+        acc[5] += u.adjh;
+        for (int k = 0; k < 6; ++k)
             if (acc[k] > INT32_MAX) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: batch too large"); }
         pre_part[i + 1] = (int)acc[0];
         pre_basis[i + 1] = (int)acc[1];
         pre_conv[i + 1] = (int)acc[2];
         pre_adj[i + 1] = (int)acc[3];
         pre_grad[i + 1] = (int)acc[4];
+        pre_adjh[i + 1] = (int)acc[5];
     }
     b->total = (int)acc[0];
     b->counts.assign(counts, counts + n);
@@ -1161,6 +1225,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     if (!rc) b->T.pre_conv = (const int*)put(pre_conv.data(), pre_conv.size() * sizeof(int));
     if (!rc) b->T.pre_adj = (const int*)put(pre_adj.data(), pre_adj.size() * sizeof(int));
     if (!rc) b->T.pre_grad = (const int*)put(pre_grad.data(), pre_grad.size() * sizeof(int));
+    if (!rc) b->T.pre_adjh = (const int*)put(pre_adjh.data(), pre_adjh.size() * sizeof(int));
     if (rc) { gl_batch_destroy(b); return rc; }
     b->T.n = n;
     b->T.tot_part = (int)acc[0];
@@ -1168,6 +1233,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     b->T.tot_conv = (int)acc[2];
     b->T.tot_adj = (int)acc[3];
     b->T.tot_grad = (int)acc[4];
+    b->T.tot_adjh = (int)acc[5];
     b->T.lds_consts = 0;
     b->T.lds_w = 0;
     for (int i = 0; i < n; ++i) {
@@ -1193,7 +1259,7 @@ extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, f
     A.lp = lp;
     A.chi2 = chi2;
     A.grad = grad_z;
-    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream);
+    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false, b->mc_model);
 }
 
 extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, float* v, int step, float lr,
@@ -1217,7 +1283,7 @@ extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, flo
     A.beta1 = beta1;
     A.beta2 = beta2;
     A.eps = epsilon;
-    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream);
+    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream, false, b->mc_model);
 }
 
 // ---------------------------------------------------------------------------

@@ -32,7 +32,9 @@
 //   mc_image      per particle: image = sum_c coef_c X_c, likelihood (forward: sigma
 //                 from the model image; backward: from the observed image), chi^2,
 //                 prior -> lp; dL/dimage
+//   mc_adj_h      (separable PSF) per image row segment: transposed horizontal pass
 //   mc_adjoint    per sub-pixel row segment: transposed correlation of dL/dimage
+//                 (separable PSF: the transposed vertical pass over mc_adj_h's rows)
 //   mc_grad       per sub-pixel: recompute and accumulate reverse-mode statistics
 //                 (least squares: amplitudes held at their optimum, envelope theorem)
 //   mc_finish     per particle: chain rule -> dlp/dz, optionally the Keras-Adam
@@ -59,6 +61,10 @@ struct McPlan {
     const float* sa;    // sep_rank x kf vertical factors
     const float* sbp;   // sep_rank x wpad_w horizontal factors, zero padded
     float* H;           // bs x D x sep_rank x nsub x N horizontal-pass intermediate
+    int adj_rank;       // > 0: separable adjoint conv with this rank (mc_adj_h + mc_adjoint's vertical pass)
+    const float* asa;   // adj_rank x kf vertical factors of the folded PSF
+    const float* asb;   // adj_rank x kf horizontal factors
+    float* U;           // bs x adj_rank x N x ss x N transposed horizontal pass, phase-major rows
     int D;              // channels
     int BW;             // staged row width (floats): pb + nsub + right margin, multiple of 4
     int pstride;        // floats of per-particle constants
@@ -130,7 +136,8 @@ struct McTable {
     const int* pre_conv;   // conv waves
     const int* pre_adj;    // adjoint waves (a multiple of 4 per particle)
     const int* pre_grad;   // mc_grad blocks (256 x kMcGradK sub-pixels each)
-    int tot_part, tot_basis, tot_conv, tot_adj, tot_grad;
+    const int* pre_adjh;   // separable adjoint horizontal-pass waves (a multiple of 4 per particle; 0 without)
+    int tot_part, tot_basis, tot_conv, tot_adj, tot_grad, tot_adjh;
     int lds_consts;        // bytes of LDS for one particle's constant block (max over lenses)
     int lds_w;             // bytes of LDS for the folded weights kf x kf (max over lenses)
 };
@@ -174,6 +181,9 @@ __device__ __forceinline__ const float* mc_stage_consts(const McPlan& M, int p, 
 }
 __device__ __forceinline__ int mc_adj_wpp4(const DPlan& P) {
     return ((P.nsub * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
+}
+__device__ __forceinline__ int mc_adjh_wpp4(const DPlan& P) {
+    return ((P.N * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
 }
 
 // gradient weight of particle p's log posterior: MAP -> -weight / N^2, else g_lp (default 1)
@@ -880,6 +890,9 @@ __global__ __launch_bounds__(256) void mc_image(const McTable T, const McArgs A,
 // Xc + db - nn (nn < Mb); the weights W[ss m + ra][ss nn + rb] are wave-uniform
 // and read from an LDS copy of the folded PSF.  Each particle owns a multiple
 // of 4 waves, so a block never spans two particles (or lenses).
+// With a separable PSF (W ~= sum_r asa_r asb_r^T, M.adj_rank > 0) the lane
+// instead sums the Ma rows of mc_adj_h's output U_r[Yc + da - m][b][Xc] with
+// the vertical factors asa_r[ss m + ra].
 // ---------------------------------------------------------------------------
 template <bool MULTI>
 __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
@@ -887,10 +900,14 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
     const McLens& E = mc_locate<MULTI>(T, T.pre_adj, blockIdx.x * 4, local);
     const DPlan& P = E.P;
     const McPlan& M = E.M;
-    const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb;
+    const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb, R = M.adj_rank;
     extern __shared__ float4 mc_dyn4[];
     float* s_w = reinterpret_cast<float*>(mc_dyn4);
-    for (int i = threadIdx.x; i < kf * kf; i += 256) s_w[i] = P.w[i];
+    if (R > 0) {
+        for (int i = threadIdx.x; i < R * kf; i += 256) s_w[i] = M.asa[i];
+    } else {
+        for (int i = threadIdx.x; i < kf * kf; i += 256) s_w[i] = P.w[i];
+    }
     __syncthreads();
     local += threadIdx.x >> 6;
     const int njb = (N + 63) / 64, wpp4 = mc_adj_wpp4(P);
@@ -904,16 +921,65 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
     if (Xc >= N) return;
     const int a = y % ss, Yc = y / ss;
     const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
-    const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
-    const int GW = N + 2 * M.gp;
-    const float* G = M.gimg + (size_t)p * GW * GW;
     float g = 0.0f;
-    for (int m = 0; m < Ma; ++m) {
-        const float* row = G + (size_t)(Yc + da - m + M.gp) * GW + (Xc + db + M.gp);
-        const float* wr = s_w + (ss * m + ra) * kf + rb;
-        for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
+    if (R > 0) {
+        const float* Ub = M.U + (size_t)p * R * N * ss * N + (size_t)b * N + Xc;
+        for (int r = 0; r < R; ++r) {
+            const float* Ur = Ub + (size_t)r * N * ss * N;
+            const float* ar = s_w + r * kf + ra;
+            for (int m = 0; m < Ma; ++m) {
+                const int I = Yc + da - m;  // rows outside the image carry dL/dimage = 0
+                if (I >= 0 && I < N) g = fmaf(ar[ss * m], Ur[(size_t)I * ss * N], g);
+            }
+        }
+    } else {
+        const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+        const int GW = N + 2 * M.gp;
+        const float* G = M.gimg + (size_t)p * GW * GW;
+        for (int m = 0; m < Ma; ++m) {
+            const float* row = G + (size_t)(Yc + da - m + M.gp) * GW + (Xc + db + M.gp);
+            const float* wr = s_w + (ss * m + ra) * kf + rb;
+            for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
+        }
     }
     M.B[((size_t)p * M.D * nsub + y) * M.BW + pb + ss * Xc + b] = g;
+}
+
+// mc_adj_h: separable adjoint, transposed horizontal pass.  block 256 = 4 waves;
+// one wave per (particle, image row I, column phase b, 64 columns Xc):
+//   U_r[I][b][Xc] = sum_{nn < Mb} asb_r[ss nn + rb] dL/dimage[I][Xc + db - nn]
+// (the padded gradient image supplies the zero columns outside the image).
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_adj_h(const McTable T) {
+    int local;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_adjh, blockIdx.x * 4, local);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
+    const int N = P.N, ss = P.ss, kf = P.kf, pb = P.pb, R = M.adj_rank;
+    extern __shared__ float4 mc_dyn4[];
+    float* s_b = reinterpret_cast<float*>(mc_dyn4);
+    for (int i = threadIdx.x; i < R * kf; i += 256) s_b[i] = M.asb[i];
+    __syncthreads();
+    local += threadIdx.x >> 6;
+    const int njb = (N + 63) / 64, wpp4 = mc_adjh_wpp4(P);
+    const int p = local / wpp4;
+    int t = local - p * wpp4;
+    if (t >= N * ss * njb) return;
+    const int jb = t % njb; t /= njb;
+    const int b = t % ss; t /= ss;
+    const int I = t;
+    const int Xc = jb * 64 + (threadIdx.x & 63);
+    if (Xc >= N) return;
+    const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+    const int GW = N + 2 * M.gp;
+    const float* row = M.gimg + (size_t)p * GW * GW + (size_t)(I + M.gp) * GW + (Xc + db + M.gp);
+    float* U = M.U + (size_t)p * R * N * ss * N + ((size_t)I * ss + b) * N + Xc;
+    for (int r = 0; r < R; ++r) {
+        const float* br = s_b + r * kf + rb;
+        float u = 0.0f;
+        for (int nn = 0; nn < Mb; ++nn) u = fmaf(br[ss * nn], row[-nn], u);
+        U[(size_t)r * N * ss * N] = u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1054,6 +1120,98 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
     if (threadIdx.x < nst) {
         const float v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
         atomicAdd(M.stats + (size_t)p * kMcStat + threadIdx.x, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Model-specialised basis and gradient for fixed-amplitude plans whose profile
+// kinds are those of MODEL (every lens of the launch): the constants come from
+// the LDS copy into scalar registers and only MODEL's states exist, through the
+// per-sub-pixel code of the fused kernel (eval_forward / eval_adjoint).  The
+// statistics land in the generic layout (8 per component) that mc_finish reads.
+// ---------------------------------------------------------------------------
+template <class M>
+__device__ __forceinline__ int mc_stat_slot(int e) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int k = M::lens[i];
+        const int n = k == glm::K_EPL ? LensAcc<glm::K_EPL>::n
+                    : k == glm::K_SHEAR ? LensAcc<glm::K_SHEAR>::n
+                    : k == glm::K_SIS ? LensAcc<glm::K_SIS>::n
+                    : k == glm::K_SIE ? LensAcc<glm::K_SIE>::n
+                    : k == glm::K_TNFW ? LensAcc<glm::K_TNFW>::n : 0;
+        if (e >= M::lens_off(i) && e < M::lens_off(i) + n) return 8 * i + (e - M::lens_off(i));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (M::light[j] == 0) continue;
+        const int off = M::light_off(light_acc_slot<M>(j));
+        if (e >= off && e < off + 8) return 8 * light_comp_index<M>(j) + (e - off);
+    }
+    return 0;
+}
+
+template <bool MULTI, class MODEL>
+__global__ __launch_bounds__(256) void mc_basis_m(const McTable T) {
+    int local;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_basis, blockIdx.x, local);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
+    const int bpp = mc_bpp(P);
+    const int p = local / bpp;
+    const int s = (local - p * bpp) * 256 + threadIdx.x;
+    const int nsub = P.nsub;
+    extern __shared__ float4 mc_dyn4[];
+    const float* pc = mc_stage_consts(M, p, reinterpret_cast<float*>(mc_dyn4));
+    Consts<MODEL> C;
+    load_consts<MODEL>(pc + ((6 * P.d + 3) & ~3), pc + M.coef_base, P, C);
+    if (s >= nsub * nsub) return;
+    const int y = s / nsub, x = s - y * nsub;
+    const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
+    const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
+    const float v = eval_forward<MODEL>(C, X, Y);
+    M.B[(size_t)p * nsub * M.BW + (size_t)y * M.BW + P.pb + x] = isnan(v) ? 0.0f : v;  // tf/simulator.py:81
+}
+
+template <bool MULTI, class MODEL>
+__global__ __launch_bounds__(256) void mc_grad_m(const McTable T) {
+    int local;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_grad, blockIdx.x, local);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
+    const int gpp = mc_gpp(P);
+    const int p = local / gpp;
+    const int blk = local - p * gpp;
+    const int nsub = P.nsub;
+    extern __shared__ float4 mc_dyn4[];
+    const float* pc = mc_stage_consts(M, p, reinterpret_cast<float*>(mc_dyn4));
+    Consts<MODEL> C;
+    load_consts<MODEL>(pc + ((6 * P.d + 3) & ~3), pc + M.coef_base, P, C);
+    const float cf = M.coef[(size_t)p * M.D];  // fixed amplitudes: image = cf * conv
+    float acc[MODEL::NACC];
+#pragma unroll
+    for (int e = 0; e < MODEL::NACC; ++e) acc[e] = 0.0f;
+    for (int it = 0; it < kMcGradK; ++it) {
+        const int s = (blk * kMcGradK + it) * 256 + threadIdx.x;
+        if (s >= nsub * nsub) break;
+        const int y = s / nsub, x = s - y * nsub;
+        const float g = M.B[(size_t)p * nsub * M.BW + (size_t)y * M.BW + P.pb + x];
+        const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
+        const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
+        eval_adjoint<MODEL>(C, X, Y, g * cf, acc);
+    }
+    __shared__ float red[MODEL::NACC][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int e = 0; e < MODEL::NACC; ++e) {
+        const float v = wave_sum(acc[e]);
+        if (lane == 0) red[e][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < MODEL::NACC) {
+        const int e = threadIdx.x;
+        const float v = red[e][0] + red[e][1] + red[e][2] + red[e][3];
+        atomicAdd(M.stats + (size_t)p * kMcStat + mc_stat_slot<MODEL>(e), v);
     }
 }
 
