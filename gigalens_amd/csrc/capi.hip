@@ -404,9 +404,9 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     for (int i = 0; i < kf * kf; ++i) Wd[i] = W[i];
     jacobi_svd(kf, Wd, sv, Uv, Vv);
     // separable adjoint conv (default; psf_rank_tol < 0 keeps the direct form): rank at
-    // 1e-6 (or the given tolerance), at most 2, when its two passes cost under half the
-    // direct taps.  Higher ranks measured slower: the R planes of U per particle (1.1 MB
-    // at rank 7, 100x100, ss 4) fall out of L2, while the direct taps re-read a small image.
+    // 1e-6 (or the given tolerance), at most 8, when its two passes cost under half the
+    // direct taps (measured: rank 1 at 100x100 ss 4 cuts the adjoint 5x, rank 7 of the
+    // 83x83 Moffat about 1.2x)
     M.adj_rank = 0;
     std::vector<float> asa, asb;
     if (desc->psf_rank_tol >= 0.0f && sv.size() && sv[0] > 0.0) {
@@ -416,7 +416,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
         const double mt = (double)(kf + ss - 1) / ss;  // taps per phase, rounded up
         const double direct = (double)nsub * nsub * mt * mt;
         const double sep = (double)R * ((double)N * nsub * mt + (double)nsub * nsub * mt);
-        if (R >= 1 && R <= 2 && sep < 0.5 * direct) {
+        if (R >= 1 && R <= gl::kMcMaxRank && sep < 0.5 * direct) {
             M.adj_rank = R;
             asa.assign((size_t)R * kf, 0.0f);
             asb.assign((size_t)R * kf, 0.0f);
@@ -517,7 +517,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
 
 // Work-unit totals of a lens with `count` particles (the prefix increments of a table).
 struct McUnits {
-    long part, basis, conv, adj, grad, adjh;
+    long part, basis, conv, adj, grad, adjh, adjv;
 };
 static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     McUnits u;
@@ -526,8 +526,11 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     u.basis = (long)count * ((P.nsub * P.nsub + 255) / 256);
     u.grad = (long)count * (((long)P.nsub * P.nsub + 256 * gl::kMcGradK - 1) / (256 * gl::kMcGradK));
     u.conv = (long)count * M.D * ((P.N + M.conv_ib - 1) / M.conv_ib) * njb;
-    u.adj = (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
+    // adjoint conv: direct (mc_adjoint) or separable (mc_adj_h + mc_adj_v)
+    const long nyb = (P.N + gl::kAdjKY - 1) / gl::kAdjKY;
+    u.adj = M.adj_rank > 0 ? 0 : (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
     u.adjh = M.adj_rank > 0 ? (long)count * (((long)P.N * P.ss * njb + 3) & ~3L) : 0;
+    u.adjv = M.adj_rank > 0 ? (long)count * (((long)P.ss * nyb * P.ss * njb + 3) & ~3L) : 0;
     return u;
 }
 
@@ -570,8 +573,11 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
         if (T.tot_adjh > 0) {
             hipLaunchKernelGGL((gl::mc_adj_h<MULTI>), dim3(T.tot_adjh / 4), dim3(256), T.lds_w, st, T);
             GL_HIP(hipGetLastError());
+            hipLaunchKernelGGL((gl::mc_adj_v<MULTI>), dim3(T.tot_adjv / 4), dim3(256), 0, st, T);
+            GL_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), T.lds_w, st, T);
+        if (T.tot_adj > 0)
+            hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), T.lds_w, st, T);
         GL_HIP(hipGetLastError());
         if (model == 1)
             hipLaunchKernelGGL((gl::mc_grad_m<MULTI, DefaultModel>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
@@ -619,11 +625,13 @@ static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStrea
         T.one.weight = 1.0f / (float)bs;
         T.n = 1;
         const McUnits u = mc_units(P, M, nb);
-        if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX || u.adjh > INT32_MAX)
+        if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX || u.adjh > INT32_MAX ||
+            u.adjv > INT32_MAX)
             return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
         T.tot_part = nb; T.tot_basis = (int)u.basis; T.tot_conv = (int)u.conv; T.tot_adj = (int)u.adj;
         T.tot_grad = (int)u.grad;
         T.tot_adjh = (int)u.adjh;
+        T.tot_adjv = (int)u.adjv;
         T.lds_consts = M.pstride * 4;
         T.lds_w = P.kf * P.kf * 4;
         int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
@@ -1179,8 +1187,8 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     }
     std::vector<gl::McLens> lens(n);
     std::vector<int> pre_part(n + 1, 0), pre_basis(n + 1, 0), pre_conv(n + 1, 0), pre_adj(n + 1, 0),
-        pre_grad(n + 1, 0), pre_adjh(n + 1, 0);
-    long acc[6] = {0, 0, 0, 0, 0, 0};
+        pre_grad(n + 1, 0), pre_adjh(n + 1, 0), pre_adjv(n + 1, 0);
+    long acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         gl::McLens& E = lens[i];
         memset(&E, 0, sizeof(E));
@@ -1196,7 +1204,8 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         const McUnits u = mc_units(E.P, E.M, counts[i]);
         acc[0] += u.part; acc[1] += u.basis; acc[2] += u.conv; acc[3] += u.adj; acc[4] += u.grad;
         acc[5] += u.adjh;
-        for (int k = 0; k < 6; ++k)
+        acc[6] += u.adjv;
+        for (int k = 0; k < 7; ++k)
             if (acc[k] > INT32_MAX) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: batch too large"); }
         pre_part[i + 1] = (int)acc[0];
         pre_basis[i + 1] = (int)acc[1];
@@ -1204,6 +1213,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         pre_adj[i + 1] = (int)acc[3];
         pre_grad[i + 1] = (int)acc[4];
         pre_adjh[i + 1] = (int)acc[5];
+        pre_adjv[i + 1] = (int)acc[6];
     }
     b->total = (int)acc[0];
     b->counts.assign(counts, counts + n);
@@ -1226,6 +1236,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     if (!rc) b->T.pre_adj = (const int*)put(pre_adj.data(), pre_adj.size() * sizeof(int));
     if (!rc) b->T.pre_grad = (const int*)put(pre_grad.data(), pre_grad.size() * sizeof(int));
     if (!rc) b->T.pre_adjh = (const int*)put(pre_adjh.data(), pre_adjh.size() * sizeof(int));
+    if (!rc) b->T.pre_adjv = (const int*)put(pre_adjv.data(), pre_adjv.size() * sizeof(int));
     if (rc) { gl_batch_destroy(b); return rc; }
     b->T.n = n;
     b->T.tot_part = (int)acc[0];
@@ -1234,6 +1245,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     b->T.tot_adj = (int)acc[3];
     b->T.tot_grad = (int)acc[4];
     b->T.tot_adjh = (int)acc[5];
+    b->T.tot_adjv = (int)acc[6];
     b->T.lds_consts = 0;
     b->T.lds_w = 0;
     for (int i = 0; i < n; ++i) {

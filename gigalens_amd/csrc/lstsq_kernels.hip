@@ -33,8 +33,8 @@
 //                 from the model image; backward: from the observed image), chi^2,
 //                 prior -> lp; dL/dimage
 //   mc_adj_h      (separable PSF) per image row segment: transposed horizontal pass
+//   mc_adj_v      (separable PSF) per sub-row block: transposed vertical pass
 //   mc_adjoint    per sub-pixel row segment: transposed correlation of dL/dimage
-//                 (separable PSF: the transposed vertical pass over mc_adj_h's rows)
 //   mc_grad       per sub-pixel: recompute and accumulate reverse-mode statistics
 //                 (least squares: amplitudes held at their optimum, envelope theorem)
 //   mc_finish     per particle: chain rule -> dlp/dz, optionally the Keras-Adam
@@ -61,7 +61,7 @@ struct McPlan {
     const float* sa;    // sep_rank x kf vertical factors
     const float* sbp;   // sep_rank x wpad_w horizontal factors, zero padded
     float* H;           // bs x D x sep_rank x nsub x N horizontal-pass intermediate
-    int adj_rank;       // > 0: separable adjoint conv with this rank (mc_adj_h + mc_adjoint's vertical pass)
+    int adj_rank;       // > 0: separable adjoint conv with this rank (mc_adj_h, mc_adj_v)
     const float* asa;   // adj_rank x kf vertical factors of the folded PSF
     const float* asb;   // adj_rank x kf horizontal factors
     float* U;           // bs x adj_rank x N x ss x N transposed horizontal pass, phase-major rows
@@ -137,7 +137,8 @@ struct McTable {
     const int* pre_adj;    // adjoint waves (a multiple of 4 per particle)
     const int* pre_grad;   // mc_grad blocks (256 x kMcGradK sub-pixels each)
     const int* pre_adjh;   // separable adjoint horizontal-pass waves (a multiple of 4 per particle; 0 without)
-    int tot_part, tot_basis, tot_conv, tot_adj, tot_grad, tot_adjh;
+    const int* pre_adjv;   // separable adjoint vertical-pass waves (a multiple of 4 per particle; 0 without)
+    int tot_part, tot_basis, tot_conv, tot_adj, tot_grad, tot_adjh, tot_adjv;
     int lds_consts;        // bytes of LDS for one particle's constant block (max over lenses)
     int lds_w;             // bytes of LDS for the folded weights kf x kf (max over lenses)
 };
@@ -184,6 +185,10 @@ __device__ __forceinline__ int mc_adj_wpp4(const DPlan& P) {
 }
 __device__ __forceinline__ int mc_adjh_wpp4(const DPlan& P) {
     return ((P.N * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
+}
+constexpr int kAdjKY = 8;  // sub-rows of one phase per lane in the separable vertical pass
+__device__ __forceinline__ int mc_adjv_wpp4(const DPlan& P) {
+    return ((P.ss * ((P.N + kAdjKY - 1) / kAdjKY) * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
 }
 
 // gradient weight of particle p's log posterior: MAP -> -weight / N^2, else g_lp (default 1)
@@ -890,9 +895,6 @@ __global__ __launch_bounds__(256) void mc_image(const McTable T, const McArgs A,
 // Xc + db - nn (nn < Mb); the weights W[ss m + ra][ss nn + rb] are wave-uniform
 // and read from an LDS copy of the folded PSF.  Each particle owns a multiple
 // of 4 waves, so a block never spans two particles (or lenses).
-// With a separable PSF (W ~= sum_r asa_r asb_r^T, M.adj_rank > 0) the lane
-// instead sums the Ma rows of mc_adj_h's output U_r[Yc + da - m][b][Xc] with
-// the vertical factors asa_r[ss m + ra].
 // ---------------------------------------------------------------------------
 template <bool MULTI>
 __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
@@ -900,14 +902,10 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
     const McLens& E = mc_locate<MULTI>(T, T.pre_adj, blockIdx.x * 4, local);
     const DPlan& P = E.P;
     const McPlan& M = E.M;
-    const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb, R = M.adj_rank;
+    const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb;
     extern __shared__ float4 mc_dyn4[];
     float* s_w = reinterpret_cast<float*>(mc_dyn4);
-    if (R > 0) {
-        for (int i = threadIdx.x; i < R * kf; i += 256) s_w[i] = M.asa[i];
-    } else {
-        for (int i = threadIdx.x; i < kf * kf; i += 256) s_w[i] = P.w[i];
-    }
+    for (int i = threadIdx.x; i < kf * kf; i += 256) s_w[i] = P.w[i];
     __syncthreads();
     local += threadIdx.x >> 6;
     const int njb = (N + 63) / 64, wpp4 = mc_adj_wpp4(P);
@@ -921,28 +919,69 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
     if (Xc >= N) return;
     const int a = y % ss, Yc = y / ss;
     const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
+    const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+    const int GW = N + 2 * M.gp;
+    const float* G = M.gimg + (size_t)p * GW * GW;
     float g = 0.0f;
-    if (R > 0) {
-        const float* Ub = M.U + (size_t)p * R * N * ss * N + (size_t)b * N + Xc;
-        for (int r = 0; r < R; ++r) {
-            const float* Ur = Ub + (size_t)r * N * ss * N;
-            const float* ar = s_w + r * kf + ra;
-            for (int m = 0; m < Ma; ++m) {
-                const int I = Yc + da - m;  // rows outside the image carry dL/dimage = 0
-                if (I >= 0 && I < N) g = fmaf(ar[ss * m], Ur[(size_t)I * ss * N], g);
-            }
-        }
-    } else {
-        const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
-        const int GW = N + 2 * M.gp;
-        const float* G = M.gimg + (size_t)p * GW * GW;
-        for (int m = 0; m < Ma; ++m) {
-            const float* row = G + (size_t)(Yc + da - m + M.gp) * GW + (Xc + db + M.gp);
-            const float* wr = s_w + (ss * m + ra) * kf + rb;
-            for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
-        }
+    for (int m = 0; m < Ma; ++m) {
+        const float* row = G + (size_t)(Yc + da - m + M.gp) * GW + (Xc + db + M.gp);
+        const float* wr = s_w + (ss * m + ra) * kf + rb;
+        for (int nn = 0; nn < Mb; ++nn) g = fmaf(wr[ss * nn], row[-nn], g);
     }
     M.B[((size_t)p * M.D * nsub + y) * M.BW + pb + ss * Xc + b] = g;
+}
+
+// mc_adj_v: separable adjoint, transposed vertical pass.  block 256 = 4 waves;
+// one wave per (particle, row phase a, kAdjKY sub-rows y = ss (Yc0 + k) + a,
+// column phase b, 64 columns Xc); lane = sub-pixel column x = ss Xc + b:
+//   dL/dsub[y][x] = sum_r sum_{m < Ma} asa_r[ss m + ra] U_r[Yc + da - m][b][Xc].
+// The lane walks the Ma + kAdjKY - 1 rows of U its sub-rows need once and adds
+// each into every sub-row it reaches; the factors are scalar loads.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_adj_v(const McTable T) {
+    int local;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_adjv, blockIdx.x * 4, local);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
+    const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb, R = M.adj_rank;
+    local += __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int njb = (N + 63) / 64, nyb = (N + kAdjKY - 1) / kAdjKY, wpp4 = mc_adjv_wpp4(P);
+    const int p = local / wpp4;
+    int t = local - p * wpp4;
+    if (t >= ss * nyb * ss * njb) return;
+    const int jb = t % njb; t /= njb;
+    const int b = t % ss; t /= ss;
+    const int yb = t % nyb; t /= nyb;
+    const int a = t;
+    const int Xc = jb * 64 + (threadIdx.x & 63);
+    if (Xc >= N) return;
+    const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
+    const int Yc0 = yb * kAdjKY;
+    float g[kAdjKY];
+#pragma unroll
+    for (int k = 0; k < kAdjKY; ++k) g[k] = 0.0f;
+    const float* Ub = M.U + (size_t)p * R * N * ss * N + (size_t)b * N + Xc;
+    // rows I = Yc0 + da - (Ma - 1) .. Yc0 + kAdjKY - 1 + da, inside the image
+    const int Ilo = max(Yc0 + da - (Ma - 1), 0), Ihi = min(Yc0 + kAdjKY - 1 + da, N - 1);
+    for (int r = 0; r < R; ++r) {
+        const float* Ur = Ub + (size_t)r * N * ss * N;
+        const float* ar = M.asa + r * kf + ra;
+        for (int I = Ilo; I <= Ihi; ++I) {
+            const float u = Ur[(size_t)I * ss * N];
+            const int m0 = Yc0 + da - I;  // tap of sub-row k is m0 + k
+#pragma unroll
+            for (int k = 0; k < kAdjKY; ++k) {
+                const int m = m0 + k;
+                if (m >= 0 && m < Ma) g[k] = fmaf(ar[ss * m], u, g[k]);
+            }
+        }
+    }
+    float* out = M.B + (size_t)p * M.D * nsub * M.BW + pb + ss * Xc + b;
+#pragma unroll
+    for (int k = 0; k < kAdjKY; ++k) {
+        const int Yc = Yc0 + k;
+        if (Yc < N) out[(size_t)(ss * Yc + a) * M.BW] = g[k];
+    }
 }
 
 // mc_adj_h: separable adjoint, transposed horizontal pass.  block 256 = 4 waves;
