@@ -14,12 +14,12 @@ LIB_PATH = os.environ.get("GIGALENS_AMD_LIB") or os.path.join(os.path.dirname(os
                                                                  "libgigalens_amd.so")
 
 GL_MAX_COMPONENTS = 8
-GL_MAX_COMPONENT_PARAMS = 8
+GL_MAX_COMPONENT_PARAMS = 10
 GL_MAX_PARAMS = 128
 GL_MAX_EPL_ITER = 100
 
 GL_EPL, GL_SHEAR, GL_SIS, GL_SIE, GL_TNFW = 1, 2, 3, 4, 5
-GL_SERSIC, GL_SERSIC_ELLIPSE, GL_SHAPELETS = 10, 11, 12
+GL_SERSIC, GL_SERSIC_ELLIPSE, GL_SHAPELETS, GL_CORE_SERSIC = 10, 11, 12, 13
 GL_ROLE_LENS, GL_ROLE_LENS_LIGHT, GL_ROLE_SOURCE = 0, 1, 2
 GL_PRIOR_NORMAL, GL_PRIOR_LOGNORMAL, GL_PRIOR_UNIFORM, GL_PRIOR_TRUNCNORMAL = 1, 2, 3, 4
 GL_LIK_FORWARD, GL_LIK_BACKWARD = 0, 1

@@ -58,7 +58,7 @@ struct ModelEntry {
 };
 
 constexpr int EPL = glm::K_EPL, SHR = glm::K_SHEAR, SIS = glm::K_SIS, SIE = glm::K_SIE, TNFW = glm::K_TNFW,
-              SE = glm::K_SERSIC_ELLIPSE, SC = glm::K_SERSIC;
+              SE = glm::K_SERSIC_ELLIPSE, SC = glm::K_SERSIC, CSE = glm::K_CORE_SERSIC;
 
 #define GL_ENTRY(L0, L1, L2, A0, A1, S0, S1)                                      \
     {{L0, L1, L2, A0, A1, S0, S1},                                                \
@@ -76,6 +76,7 @@ const ModelEntry kModels[] = {
     GL_ENTRY(EPL, SHR, 0, SE, SE, SE, 0),
     GL_ENTRY(SIE, SHR, 0, SE, 0, SE, 0),
     GL_ENTRY(EPL, SHR, TNFW, SE, 0, SE, 0),  // main deflector + one truncated-NFW subhalo
+    GL_ENTRY(EPL, SHR, 0, CSE, 0, SE, 0),    // core-Sersic lens light
 };
 
 }  // namespace
@@ -116,6 +117,7 @@ static int n_params_of(int kind) {
     case glm::K_TNFW: return 5;
     case glm::K_SERSIC: return 5;
     case glm::K_SERSIC_ELLIPSE: return 7;
+    case glm::K_CORE_SERSIC: return 10;
     case glm::K_SHAPELETS: return 3;
     default: return -1;
     }
@@ -744,7 +746,8 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             if (nl >= 3) { gl_plan_destroy(p); return fail(GL_ERR_UNSUPPORTED, "at most 3 lens components"); }
             sig[nl++] = c.kind;
         } else {
-            if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE && c.kind != glm::K_SHAPELETS) {
+            if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE && c.kind != glm::K_SHAPELETS &&
+                c.kind != glm::K_CORE_SERSIC) {
                 gl_plan_destroy(p);
                 return fail(GL_ERR_ARG, "mass profile given as a light component");
             }
@@ -815,6 +818,12 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             gl_plan_destroy(p);
             return fail(GL_ERR_UNSUPPORTED, "staged plans take one observed image (use gl_batch_* for many lenses)");
         }
+        for (int i = 0; i < desc->n_components; ++i)
+            if (desc->components[i].kind == glm::K_CORE_SERSIC) {
+                gl_plan_destroy(p);
+                return fail(GL_ERR_UNSUPPORTED, "the core-Sersic profile needs the resident-image kernel: an image "
+                                                "that fits LDS and a compiled profile combination");
+            }
         p->mc = 1;
         p->model = nullptr;
         p->mc_model = 0;
@@ -875,7 +884,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         if (sig[i] == glm::K_EPL) off += (coef_block + 3) & ~3;
     }
     P.off_red = off;
-    const int nred = 2 + 8 * 3 + 8 * 4;
+    const int nred = 2 + 8 * 3 + 11 * 4;
     off += (kBlock / 64 + 1) * nred;
 
     // ---- bounds proof of every LDS index the kernel forms --------------------

@@ -51,7 +51,9 @@ template <> struct LensAcc<glm::K_SHEAR> { static constexpr int n = 2; };
 template <> struct LensAcc<glm::K_SIS> { static constexpr int n = 3; };
 template <> struct LensAcc<glm::K_SIE> { static constexpr int n = 5; };
 template <> struct LensAcc<glm::K_TNFW> { static constexpr int n = 5; };
-template <int K> struct LightAcc { static constexpr int n = (K == glm::K_SERSIC || K == glm::K_SERSIC_ELLIPSE) ? 8 : 0; };
+template <int K> struct LightAcc {
+    static constexpr int n = (K == glm::K_SERSIC || K == glm::K_SERSIC_ELLIPSE) ? 8 : K == glm::K_CORE_SERSIC ? 11 : 0;
+};
 
 template <int L0, int L1, int L2, int A0, int A1, int S0, int S1>
 struct ModelT {
@@ -64,7 +66,16 @@ struct ModelT {
         return i == 0 ? 0 : i == 1 ? LensAcc<L0>::n : LensAcc<L0>::n + LensAcc<L1>::n;
     }
     static constexpr int NLENSACC = LensAcc<L0>::n + LensAcc<L1>::n + LensAcc<L2>::n;
-    static constexpr int light_off(int i) { return NLENSACC + 8 * i; }
+    // light accumulator slot j: lens lights first (j < NLL), then sources
+    static constexpr int light_kind_acc(int j) { return j < NLL ? light[j] : light[2 + j - NLL]; }
+    static constexpr int light_n(int k) {
+        return (k == glm::K_SERSIC || k == glm::K_SERSIC_ELLIPSE) ? 8 : k == glm::K_CORE_SERSIC ? 11 : 0;
+    }
+    static constexpr int light_off(int j) {
+        int o = NLENSACC;
+        for (int t = 0; t < j; ++t) o += light_n(light_kind_acc(t));
+        return o;
+    }
     static constexpr int NACC = NLENSACC + LightAcc<A0>::n + LightAcc<A1>::n + LightAcc<S0>::n + LightAcc<S1>::n;
     static constexpr int NRED = NACC + 2;  // + log-likelihood, chi^2
 };
@@ -373,6 +384,7 @@ struct Consts {
     glm::SieC sie[3];
     glm::TnfwC tnfw[3];
     glm::SerC ser[4];
+    glm::CoreSerC core[4];
     const float* coefA[3];
     const float4* coefB[3];
 };
@@ -410,9 +422,10 @@ __device__ __forceinline__ void load_consts(const float* s_c, const float* s_coe
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        if (M::light[i] != 0) {
-            const int ci = light_comp_index<M>(i);
-            C.ser[i] = uniform_load<glm::SerC>(s_c + ci * kConstStride);
+        if (M::light[i] == glm::K_CORE_SERSIC) {
+            C.core[i] = uniform_load<glm::CoreSerC>(s_c + light_comp_index<M>(i) * kConstStride);
+        } else if (M::light[i] != 0) {
+            C.ser[i] = uniform_load<glm::SerC>(s_c + light_comp_index<M>(i) * kConstStride);
         }
     }
 }
@@ -421,9 +434,12 @@ template <class M>
 __device__ __forceinline__ float eval_forward(const Consts<M>& C, float X, float Y) {
     float val = 0.0f;
     glm::SerState st;
+    glm::CoreSerState cst;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], X, Y, st);
+    for (int i = 0; i < 2; ++i) {
+        if (M::light[i] == glm::K_CORE_SERSIC) val += glm::core_sersic_forward(C.core[i], X, Y, cst);
+        else if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], X, Y, st);
+    }
     float bx = X, by = Y;
     glm::EplState es;
     float sis_st[3];
@@ -441,8 +457,10 @@ __device__ __forceinline__ float eval_forward(const Consts<M>& C, float X, float
         by -= ay;
     }
 #pragma unroll
-    for (int i = 2; i < 4; ++i)
-        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], bx, by, st);
+    for (int i = 2; i < 4; ++i) {
+        if (M::light[i] == glm::K_CORE_SERSIC) val += glm::core_sersic_forward(C.core[i], bx, by, cst);
+        else if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], bx, by, st);
+    }
     return val;
 }
 
@@ -450,9 +468,12 @@ template <class M>
 __device__ __forceinline__ void eval_adjoint(const Consts<M>& C, float X, float Y, float g, float* acc) {
     float val = 0.0f;
     glm::SerState lst[4];
+    glm::CoreSerState cst[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], X, Y, lst[i]);
+    for (int i = 0; i < 2; ++i) {
+        if (M::light[i] == glm::K_CORE_SERSIC) val += glm::core_sersic_forward(C.core[i], X, Y, cst[i]);
+        else if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], X, Y, lst[i]);
+    }
     float bx = X, by = Y;
     glm::EplState es[3];
     float sis_st[3][3];
@@ -470,15 +491,19 @@ __device__ __forceinline__ void eval_adjoint(const Consts<M>& C, float X, float 
         by -= ay;
     }
 #pragma unroll
-    for (int i = 2; i < 4; ++i)
-        if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], bx, by, lst[i]);
+    for (int i = 2; i < 4; ++i) {
+        if (M::light[i] == glm::K_CORE_SERSIC) val += glm::core_sersic_forward(C.core[i], bx, by, cst[i]);
+        else if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], bx, by, lst[i]);
+    }
     if (isnan(val)) return;  // zeroed in the forward pass: no contribution
     float gbx = 0.0f, gby = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (M::light[i] != 0) {
             float gdx, gdy;
-            glm::sersic_adjoint(C.ser[i], lst[i], g, acc + M::light_off(light_acc_slot<M>(i)), gdx, gdy);
+            float* la = acc + M::light_off(light_acc_slot<M>(i));
+            if (M::light[i] == glm::K_CORE_SERSIC) glm::core_sersic_adjoint(C.core[i], cst[i], g, la, gdx, gdy);
+            else glm::sersic_adjoint(C.ser[i], lst[i], g, la, gdx, gdy);
             if (i >= 2) { gbx += gdx; gby += gdy; }
         }
     }
@@ -618,6 +643,11 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
             glm::SerC c;
             glm::sersic_prepare(pv[0], pv[1], 0.0f, 0.0f, pv[2], pv[3], cd.lstsq ? 1.0f : pv[4], 0, c);
             *reinterpret_cast<glm::SerC*>(dst) = c;
+        } else if (cd.kind == glm::K_CORE_SERSIC) {
+            glm::CoreSerC c;
+            glm::core_sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], pv[6], pv[7], pv[8],
+                                     cd.lstsq ? 1.0f : pv[9], c);
+            *reinterpret_cast<glm::CoreSerC*>(dst) = c;
         }
     }
     __syncthreads();
@@ -955,7 +985,13 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
             for (int i = 0; i < 4; ++i) {
                 if (M::light[i] == 0 || w != 4 + i) continue;
                 const DComp& cd = P.comp[light_comp_index<M>(i)];
-                float g[8];
+                float g[10];
+                if (M::light[i] == glm::K_CORE_SERSIC) {
+                    glm::core_sersic_finish(C.core[i], tot + M::light_off(light_acc_slot<M>(i)), g);
+                    for (int p = 0; p < (cd.lstsq ? 9 : 10); ++p)
+                        if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[p]);
+                    continue;
+                }
                 glm::sersic_finish(C.ser[i], tot + M::light_off(light_acc_slot<M>(i)), g);
                 if (M::light[i] == glm::K_SERSIC_ELLIPSE) {
                     for (int p = 0; p < (cd.lstsq ? 6 : 7); ++p)

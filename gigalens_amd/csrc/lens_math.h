@@ -66,7 +66,7 @@ GLM float rsq(float x) {
 // Component constants (prepared once per particle)
 // ---------------------------------------------------------------------------
 enum Kind { K_EPL = 1, K_SHEAR = 2, K_SIS = 3, K_SIE = 4, K_TNFW = 5, K_SERSIC = 10, K_SERSIC_ELLIPSE = 11,
-            K_SHAPELETS = 12 };
+            K_SHAPELETS = 12, K_CORE_SERSIC = 13 };
 
 // float slots of a component's constant block
 struct EplC {
@@ -87,6 +87,10 @@ struct SieC {
 };
 struct TnfwC {
     float cx, cy, Rs, alpha_Rs, rt, tau, T, P, dP, K0, K, xmin;
+};
+struct CoreSerC {
+    float cx, cy, Axx, Axy, Ayy, q, cphi, sphi, c, e1, e2;
+    float Rs, n, Rb, al, gam, Ie, lnRs, lnRb, Rba, iRsa, kexp;
 };
 
 // Series length rule of epl.py:41,51-58 for a given f: the loop runs for
@@ -625,6 +629,100 @@ GLM void tnfw_finish(const TnfwC& C, const float* acc, float* g) {
     g[2] = C.K * acc[2] / C.Rs;
     g[3] = C.K * acc[3];
     g[4] = C.K * acc[4];
+}
+
+// ---------------------------------------------------------------------------
+// Core-Sersic light, src/gigalens/tf/profiles/light/sersic.py:75-121, as the
+// reference writes it (R_sersic ** alpha ** 1.0 == R_sersic^alpha):
+//   I = Ie (1 + (Rb/R)^a)^(g/a) exp(-bn (R^a + Rb^a) / (Rs^a a n) - 1),
+//   bn = 1.9992 n - 0.3271, R the elliptical distance of sersic.py:33-59.
+// ln I = ln Ie + (g/a) ln(1 + A) - kexp (R^a + Rb^a) Rs^-a - 1 with A = (Rb/R)^a,
+// kexp = bn/(a n).
+// ---------------------------------------------------------------------------
+GLM void core_sersic_prepare(float Rs, float n, float Rb, float al, float gam, float e1, float e2, float cx,
+                             float cy, float Ie, CoreSerC& C) {
+    C.cx = cx; C.cy = cy; C.e1 = e1; C.e2 = e2;
+    float phi = atan2f(e2, e1) * 0.5f;
+    float c = sqrtf(e1 * e1 + e2 * e2);
+    float q = (1.0f - c) / (1.0f + c);
+    float cp = cosf(phi), sp = sinf(phi);
+    C.Axx = q * cp * cp + sp * sp / q;
+    C.Ayy = q * sp * sp + cp * cp / q;
+    C.Axy = cp * sp * (q - 1.0f / q);
+    C.q = q; C.cphi = cp; C.sphi = sp; C.c = c;
+    C.Rs = Rs; C.n = n; C.Rb = Rb; C.al = al; C.gam = gam; C.Ie = Ie;
+    C.lnRs = logf(Rs);
+    C.lnRb = logf(Rb);
+    C.Rba = expf(al * C.lnRb);
+    C.iRsa = expf(-al * C.lnRs);
+    C.kexp = (1.9992f * n - 0.3271f) / (al * n);
+}
+
+struct CoreSerState { float dx, dy, gx, gy, Q, lnR, Ra, A, L1, I; };
+
+GLM float core_sersic_forward(const CoreSerC& C, float X, float Y, CoreSerState& S) {
+    float dx = X - C.cx, dy = Y - C.cy;
+    float gx = fmaf(C.Axx, dx, C.Axy * dy);
+    float gy = fmaf(C.Axy, dx, C.Ayy * dy);
+    float Q = fmaf(dx, gx, dy * gy);
+    float lnR = 0.5f * logf(Q);
+    float Ra = expf(C.al * lnR);
+    float A = C.Rba / Ra;
+    float L1 = log1pf(A);
+    float lnI = fmaf(C.gam / C.al, L1, fmaf(-C.kexp * C.iRsa, Ra + C.Rba, -1.0f));
+    float I = C.Ie * expf(lnI);
+    S.dx = dx; S.dy = dy; S.gx = gx; S.gy = gy; S.Q = Q; S.lnR = lnR; S.Ra = Ra; S.A = A; S.L1 = L1; S.I = I;
+    return I;
+}
+
+// acc[0..10] = (S_I, S_I L1, S_I A/(1+A), S_I R^a, S_I A lnR/(1+A), S_I R^a lnR,
+//               S_xx, S_xy, S_yy, S_dx, S_dy) with S_I = sum g I; the geometric sums
+// carry w = g I d lnI/dQ.  Returns dL/d(dx, dy) in (gdx, gdy).
+GLM void core_sersic_adjoint(const CoreSerC& C, const CoreSerState& S, float g, float* acc, float& gdx,
+                             float& gdy) {
+    float gI = g * S.I;
+    float t = S.A / (1.0f + S.A);
+    acc[0] += gI;
+    acc[1] = fmaf(gI, S.L1, acc[1]);
+    acc[2] = fmaf(gI, t, acc[2]);
+    acc[3] = fmaf(gI, S.Ra, acc[3]);
+    acc[4] = fmaf(gI * t, S.lnR, acc[4]);
+    acc[5] = fmaf(gI * S.Ra, S.lnR, acc[5]);
+    // d lnI / d lnR = -g t - a kexp R^a Rs^-a;  dQ: d lnR = dQ / (2 Q)
+    float dlnR = -C.gam * t - C.al * C.kexp * S.Ra * C.iRsa;
+    float w = gI * dlnR * 0.5f / S.Q;
+    acc[6] = fmaf(w * S.dx, S.dx, acc[6]);
+    acc[7] = fmaf(w * S.dx, S.dy, acc[7]);
+    acc[8] = fmaf(w * S.dy, S.dy, acc[8]);
+    acc[9] = fmaf(w, S.gx, acc[9]);
+    acc[10] = fmaf(w, S.gy, acc[10]);
+    gdx = 2.0f * w * S.gx;
+    gdy = 2.0f * w * S.gy;
+}
+
+// g = d/d(R_sersic, n_sersic, Rb, alpha, gamma, e1, e2, center_x, center_y, Ie)
+GLM void core_sersic_finish(const CoreSerC& C, const float* acc, float* g) {
+    const float al = C.al, ia = 1.0f / al;
+    const float base = C.iRsa * (acc[3] + C.Rba * acc[0]);  // sum g I (R^a + Rb^a) Rs^-a
+    g[0] = C.kexp * al * base / C.Rs;
+    g[1] = -0.3271f * ia / (C.n * C.n) * base;
+    g[2] = (C.gam * acc[2] - al * C.kexp * C.Rba * C.iRsa * acc[0]) / C.Rb;
+    g[3] = -C.gam * ia * ia * acc[1] + C.gam * ia * (C.lnRb * acc[2] - acc[4]) +
+           C.kexp * (base * (ia + C.lnRs) - C.iRsa * (acc[5] + C.Rba * C.lnRb * acc[0]));
+    g[4] = acc[1] * ia;
+    float dAxx = acc[6], dAxy = 2.0f * acc[7], dAyy = acc[8];
+    float q = C.q, cp = C.cphi, sp = C.sphi;
+    float s2p = 2.0f * sp * cp, c2p = cp * cp - sp * sp;
+    float iq = 1.0f / q, iq2 = iq * iq;
+    float dq = dAxx * (cp * cp - sp * sp * iq2) + dAyy * (sp * sp - cp * cp * iq2) + dAxy * 0.5f * s2p * (1.0f + iq2);
+    float dphi = dAxx * s2p * (iq - q) + dAyy * s2p * (q - iq) + dAxy * c2p * (q - iq);
+    float dc = dq * (-2.0f / ((1.0f + C.c) * (1.0f + C.c)));
+    float cc = C.e1 * C.e1 + C.e2 * C.e2;
+    g[5] = dphi * (-C.e2 * 0.5f / cc) + dc * C.e1 / C.c;
+    g[6] = dphi * (C.e1 * 0.5f / cc) + dc * C.e2 / C.c;
+    g[7] = -2.0f * acc[9];
+    g[8] = -2.0f * acc[10];
+    g[9] = acc[0] / C.Ie;
 }
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,7 @@ _CANON = {
     nat.GL_TNFW: ["Rs", "alpha_Rs", "r_trunc", "center_x", "center_y"],
     nat.GL_SERSIC: ["R_sersic", "n_sersic", "center_x", "center_y", "Ie"],
     nat.GL_SERSIC_ELLIPSE: ["R_sersic", "n_sersic", "e1", "e2", "center_x", "center_y", "Ie"],
+    nat.GL_CORE_SERSIC: ["R_sersic", "n_sersic", "Rb", "alpha", "gamma", "e1", "e2", "center_x", "center_y", "Ie"],
     nat.GL_SHAPELETS: ["beta", "center_x", "center_y"],
 }
 
@@ -46,7 +47,7 @@ def model_param_names(prof) -> List[str]:
     carry ``Ie`` when not solved by least squares (sersic.py:26,66)."""
     k = prof._kind
     names = list(_CANON[k])
-    if k in (nat.GL_SERSIC, nat.GL_SERSIC_ELLIPSE) and getattr(prof, "use_lstsq", False):
+    if k in (nat.GL_SERSIC, nat.GL_SERSIC_ELLIPSE, nat.GL_CORE_SERSIC) and getattr(prof, "use_lstsq", False):
         names.remove("Ie")
     if k == nat.GL_SHAPELETS and not prof.use_lstsq:
         names += list(prof._amp_names)

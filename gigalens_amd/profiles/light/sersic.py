@@ -1,4 +1,4 @@
-"""Sersic light profiles, ``src/gigalens/tf/profiles/light/sersic.py:9-72``."""
+"""Sersic light profiles, ``src/gigalens/tf/profiles/light/sersic.py:9-121``."""
 from ... import _native as nat
 from ...profile import LightProfile
 
@@ -17,3 +17,13 @@ class SersicEllipse(Sersic):
     _name = "SERSIC_ELLIPSE"
     _params = ["R_sersic", "n_sersic", "e1", "e2", "center_x", "center_y"]
     _kind = nat.GL_SERSIC_ELLIPSE
+
+
+class CoreSersic(Sersic):
+    """Core-Sersic as the reference writes it (sersic.py:75-121):
+    I = Ie (1 + (Rb/R)^alpha)^(gamma/alpha) exp(-b_n (R^alpha + Rb^alpha) / (R_s^alpha alpha n) - 1)
+    on the elliptical distance R.  Evaluated by the resident-image kernel only."""
+
+    _name = "CORE_SERSIC"
+    _params = ["R_sersic", "n_sersic", "Rb", "alpha", "gamma", "e1", "e2", "center_x", "center_y"]
+    _kind = nat.GL_CORE_SERSIC

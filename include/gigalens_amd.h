@@ -29,7 +29,7 @@ extern "C" {
 #define GL_ERR_UNSUPPORTED 3
 
 #define GL_MAX_COMPONENTS 8
-#define GL_MAX_COMPONENT_PARAMS 8
+#define GL_MAX_COMPONENT_PARAMS 10
 #define GL_MAX_PARAMS 128
 #define GL_MAX_EPL_ITER 100
 
@@ -42,7 +42,9 @@ enum gl_profile_kind {
     GL_TNFW = 5,           /* params: Rs, alpha_Rs, r_trunc, center_x, center_y   (tnfw.py:12) */
     GL_SERSIC = 10,        /* params: R_sersic, n_sersic, center_x, center_y, Ie   (sersic.py:23) */
     GL_SERSIC_ELLIPSE = 11,/* params: R_sersic, n_sersic, e1, e2, center_x, center_y, Ie (sersic.py:64) */
-    GL_SHAPELETS = 12      /* params: beta, center_x, center_y (+ amplitudes)      (shapelets.py:17) */
+    GL_SHAPELETS = 12,     /* params: beta, center_x, center_y (+ amplitudes)      (shapelets.py:17) */
+    GL_CORE_SERSIC = 13    /* params: R_sersic, n_sersic, Rb, alpha, gamma, e1, e2, center_x, center_y, Ie
+                              (sersic.py:75-88); resident-image (fused) kernel only */
 };
 
 /* Roles: which list of PhysicalModel the component belongs to (model.py:21-29). */

@@ -20,6 +20,11 @@ Pinning (what checks each function beyond retyping):
                           (tests/test_oracle_tnfw.py)
   * ``sersic_*``       -- known answer light(0,1)=Ie (tests/test_profiles.py:17-26)
                           and an independent rotated-ellipse construction
+  * ``core_sersic_light`` -- partly pinned: at Rb -> 0 and alpha = 1/n it is
+                          e^{-1-bn} times the (pinned) Sersic ellipse; at R = Rb its
+                          core factor is 2^{gamma/alpha}.  Beyond these reductions the
+                          alpha dependence follows the reference expression itself
+                          (parity unpinned: nothing else fixes it).
   * ``shapelet_*``     -- numpy.polynomial.hermite, orthonormality integrals
   * ``conv_pool``      -- scipy.signal.convolve2d + block mean
   * ``log_prob_*``     -- scipy.stats log-pdfs, closed-form log-Jacobians
@@ -382,6 +387,19 @@ def sersic_ellipse_light(x, y, R_sersic, n_sersic, e1, e2, center_x, center_y, I
         return Ie * np.exp(-bn * ((R / _as_b(R_sersic)) ** (1 / n) - 1.0))
 
 
+def core_sersic_light(x, y, R_sersic, n_sersic, Rb, alpha, gamma, e1, e2, center_x, center_y, Ie=None):
+    """``CoreSersic.light``, ``src/gigalens/tf/profiles/light/sersic.py:98-121``, as written there:
+    ``R_sersic ** alpha ** 1.0`` is ``R_sersic ** alpha`` and the ``- 1.0`` sits outside the
+    ``bn`` product."""
+    R = sersic_distance(x, y, center_x, center_y, e1, e2)
+    n, Rs, Rb, al, gam = (_as_b(v) for v in (n_sersic, R_sersic, Rb, alpha, gamma))
+    bn = 1.9992 * n - 0.3271
+    Ie = 1.0 if Ie is None else _as_b(Ie)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        return (Ie * (1 + (Rb / R) ** al) ** (gam / al)
+                * np.exp(-bn * ((R ** al + Rb ** al) / Rs ** al ** 1.0 / (al * n)) - 1.0))
+
+
 def sersic_light(x, y, R_sersic, n_sersic, center_x, center_y, Ie=None):
     """Circular ``Sersic.light``, ``src/gigalens/tf/profiles/light/sersic.py:21-31``."""
     R = sersic_distance(x, y, center_x, center_y)
@@ -558,6 +576,8 @@ def light_eval(kind: str, x, y, p: Dict, n_max=None, interpolate=True):
         return sersic_ellipse_light(x, y, **p)
     if kind == "SERSIC":
         return sersic_light(x, y, **p)
+    if kind == "CORE_SERSIC":
+        return core_sersic_light(x, y, **p)
     if kind == "SHAPELETS":
         amps = np.stack([np.asarray(p[k], np.float64) for k in sorted(p) if k.startswith("amp")])
         return shapelets_light(x, y, p["center_x"], p["center_y"], p["beta"], amps, n_max, interpolate)

@@ -100,6 +100,19 @@ extern "C" void h_tnfw(const float* p, int n, const float* x, const float* y, co
     tnfw_finish(C, acc, g);
 }
 
+extern "C" void h_core_sersic(const float* p, int n, const float* x, const float* y, const float* gup, float* I,
+                              float* gpar, float* gdx, float* gdy) {
+    CoreSerC C;
+    core_sersic_prepare(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], C);
+    float acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        CoreSerState S;
+        I[i] = core_sersic_forward(C, x[i], y[i], S);
+        core_sersic_adjoint(C, S, gup[i], acc, gdx[i], gdy[i]);
+    }
+    core_sersic_finish(C, acc, gpar);
+}
+
 extern "C" void h_prior(int kind, const float* args, float z, int fldj_at_x, float* out) {
     PriorOut o;
     prior_eval(kind, args, z, fldj_at_x, o);

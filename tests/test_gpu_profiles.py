@@ -1,7 +1,9 @@
 """GPU parity of the SIE and truncated-NFW mass profiles
 (src/gigalens/tf/profiles/mass/sie.py:13-43, tnfw.py:14-58), through the
 LDS-resident fused kernel (instantiated for SIE+shear and for EPL+shear+TNFW)
-and through the HBM-staged pipeline, against the FP64 oracle.
+and through the HBM-staged pipeline, and of the core-Sersic light profile
+(src/gigalens/tf/profiles/light/sersic.py:75-121) through the fused kernel,
+against the FP64 oracle.
 
 Bars: images within 1e-4 of the image peak; log_prob within 1e-4 relative;
 gradients against FP64 Richardson-extrapolated central differences of the
@@ -19,7 +21,8 @@ from gigalens_amd import ForwardProbModel, PhysicalModel
 from gigalens_amd import distributions as tfd
 from gigalens_amd import workloads as wl
 from gigalens_amd.plan import Plan, layout_from_paths
-from gigalens_amd.profiles.light.sersic import SersicEllipse
+from gigalens_amd import _native as nat
+from gigalens_amd.profiles.light.sersic import CoreSersic, SersicEllipse
 from gigalens_amd.profiles.mass.epl import EPL
 from gigalens_amd.profiles.mass.shear import Shear
 from gigalens_amd.profiles.mass.sie import SIE
@@ -66,8 +69,24 @@ def _tnfw_problem():
     return prior, phys
 
 
+def _core_problem():
+    """EPL + shear, core-Sersic lens light (sersic.py:75-121), Sersic source."""
+    _, src = _lights()
+    lens = wl.epl_shear_prior().model[0]
+    ll = tfd.JointDistributionSequential([
+        tfd.JointDistributionNamed(dict(
+            R_sersic=tfd.LogNormal(math.log(1.0), 0.15), n_sersic=tfd.Uniform(1.0, 4.0),
+            Rb=tfd.LogNormal(math.log(0.1), 0.2), alpha=tfd.Uniform(1.0, 3.0), gamma=tfd.Uniform(0.1, 0.8),
+            e1=tfd.Normal(0, 0.1), e2=tfd.Normal(0, 0.1), center_x=tfd.Normal(0, 0.05),
+            center_y=tfd.Normal(0, 0.05), Ie=tfd.LogNormal(math.log(500.0), 0.3))),
+    ])
+    prior = tfd.JointDistributionSequential([lens, ll, src])
+    phys = PhysicalModel([EPL(), Shear()], [CoreSersic()], [SersicEllipse()])
+    return prior, phys
+
+
 def _problem(name, bs=8):
-    prior, phys = _sie_problem() if name == "sie" else _tnfw_problem()
+    prior, phys = {"sie": _sie_problem, "tnfw": _tnfw_problem, "core": _core_problem}[name]()
     cfg = wl.epl_shear_config()
     cam = Camera(cfg)
     ocam = C.oracle_camera(cfg, cam)
@@ -84,7 +103,7 @@ def _plan(prob, staged):
                 observed=prob["obs"], background_rms=0.2, exp_time=100.0, staged=staged)
 
 
-CASES = [("sie", False), ("sie", True), ("tnfw", False), ("tnfw", True)]
+CASES = [("sie", False), ("sie", True), ("tnfw", False), ("tnfw", True), ("core", False)]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=[f"{n}-{'staged' if s else 'fused'}" for n, s in CASES])
@@ -149,3 +168,11 @@ def test_profile_gradient_vs_fd(case):
         err = np.abs(g[i] - gfd)
         _print(f"{name} particle {i}: max |g - fd| / max|fd| = {err.max() / scale:.2e}")
         assert np.all(err <= 2e-3 * np.abs(gfd) + 2e-4 * scale), (g[i], gfd)
+
+
+def test_core_sersic_staged_is_refused(cuda):
+    """The core-Sersic profile runs on the resident-image kernel only; a staged plan
+    is refused with an error instead of running without it."""
+    prob = _problem("core", bs=2)
+    with pytest.raises(nat.GlError):
+        _plan(prob, True)

@@ -202,6 +202,43 @@ def test_sis_forward_and_adjoint(mh):
     assert np.allclose(g, fd, rtol=1e-3, atol=1e-3)
 
 
+@pytest.mark.parametrize("prm", [[0.8, 2.5, 0.12, 1.6, 0.4, 0.09, 0.15, 0.1, -0.05, 499.0],
+                                 [0.5, 1.2, 0.05, 3.0, -0.3, -0.2, 0.05, -0.1, 0.2, 20.0]])
+def test_core_sersic_forward_and_adjoint(mh, prm):
+    """Core-Sersic (sersic.py:98-121): forward vs the FP64 restatement, the 10 parameter
+    gradients and the position adjoint vs FP64 central differences."""
+    names = ["R_sersic", "n_sersic", "Rb", "alpha", "gamma", "e1", "e2", "center_x", "center_y", "Ie"]
+    rng = np.random.default_rng(9)
+    n = 2000
+    x, y = rng.normal(size=n), rng.normal(size=n)
+    gup = rng.normal(size=n)
+    xs, ys, gs = f32(x), f32(y), f32(gup)
+    I = np.empty(n, np.float32)
+    gp = np.zeros(10, np.float32)
+    gdx, gdy = np.empty(n, np.float32), np.empty(n, np.float32)
+    mh.h_core_sersic(p(f32(prm)), n, p(xs), p(ys), p(gs), p(I), p(gp), p(gdx), p(gdy))
+
+    def light(pr, X, Y):
+        return np64.core_sersic_light(X, Y, **dict(zip(names, pr)))[0]
+    X, Y = xs.astype(np.float64).reshape(1, -1), ys.astype(np.float64).reshape(1, -1)
+    ref = light(prm, X, Y)[0]
+    assert np.allclose(I, ref, rtol=5e-6, atol=1e-6 * np.abs(ref).max())
+    fd = np.zeros(len(prm))
+    for j in range(len(prm)):
+        h = 1e-6 * max(abs(prm[j]), 1.0)
+        pp, pm = list(prm), list(prm)
+        pp[j] += h
+        pm[j] -= h
+        fd[j] = np.sum(gs * (light(pp, X, Y)[0] - light(pm, X, Y)[0])) / (2 * h)
+    scale = np.abs(fd).max()
+    assert np.all(np.abs(gp - fd) <= 1e-3 * np.abs(fd) + 1e-5 * scale), (gp, fd)
+    h = 1e-6
+    dIdx = (light(prm, X + h, Y)[0] - light(prm, X - h, Y)[0]) / (2 * h)
+    dIdy = (light(prm, X, Y + h)[0] - light(prm, X, Y - h)[0]) / (2 * h)
+    assert np.allclose(gdx, gs * dIdx, rtol=1e-3, atol=1e-4 * np.abs(gs * dIdx).max())
+    assert np.allclose(gdy, gs * dIdy, rtol=1e-3, atol=1e-4 * np.abs(gs * dIdy).max())
+
+
 def _mass_check(mh, fn, oracle, prm, x, y, Hx, Hy, h, rtol_fwd, rtol_grad):
     n = len(x)
     ax, ay, g = np.empty(n, np.float32), np.empty(n, np.float32), np.zeros(len(prm), np.float32)

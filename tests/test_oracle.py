@@ -245,3 +245,23 @@ def test_torch_ref_matches_np64():
     lp_t, chi_t = torch_ref.log_prob(sim, nested_prior, z, prob["obs"], 0.2, 100.0)
     lp_o, chi_o, _ = C.oracle_log_prob(prob["ocam"], prob["phys"], prob["prior"], z.numpy(), prob["obs"], 0.2, 100.0)
     assert np.allclose(lp_t.numpy(), lp_o, rtol=1e-10) and np.allclose(chi_t.numpy(), chi_o, rtol=1e-10)
+
+
+def test_core_sersic_reductions():
+    """Core-Sersic restatement (sersic.py:98-121): Rb -> 0 with alpha = 1/n gives
+    exp(-1 - bn) times the Sersic ellipse (pinned above), and the core factor at R = Rb
+    is 2^(gamma/alpha) relative to the core-less expression."""
+    rng = np.random.default_rng(3)
+    x, y = rng.normal(size=(2, 40, 40)) * 0.8
+    Rs, n, e1, e2, cx, cy, Ie = 0.7, 2.5, 0.1, -0.05, 0.02, -0.03, 3.0
+    bn = 1.9992 * n - 0.3271
+    core = np64.core_sersic_light(x, y, Rs, n, 1e-40, 1.0 / n, 0.4, e1, e2, cx, cy, Ie)
+    ser = np64.sersic_ellipse_light(x, y, Rs, n, e1, e2, cx, cy, Ie)
+    np.testing.assert_allclose(core, ser * np.exp(-1.0 - bn), rtol=1e-10)
+    # at R = Rb the core factor (1 + 1)^(gamma/alpha)
+    Rb, al, gam = 0.3, 1.7, 0.6
+    xs, ys = np.array([[cx + Rb]]), np.array([[cy]])
+    R = np64.sersic_distance(xs, ys, cx, cy, 0.0, 0.0)[0, 0, 0]
+    full = np64.core_sersic_light(xs, ys, Rs, n, Rb, al, gam, 0.0, 0.0, cx, cy, Ie)[0, 0, 0]
+    nocore = Ie * np.exp(-bn * (R ** al + Rb ** al) / Rs ** al / (al * n) - 1.0)
+    assert full == pytest.approx(nocore * 2.0 ** (gam / al), rel=1e-12)
