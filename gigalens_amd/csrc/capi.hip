@@ -87,6 +87,7 @@ struct gl_plan {
     int mc_model;            // staged fixed-amplitude plan of the reference default model: 1 (specialised kernels)
     size_t lds_bytes;
     int epl_batch;
+    int* order_buf;          // fused plans with an EPL lens: launch order of up to gl::kOrderCap particles
     int mc;                  // 1: multi-channel least-squares plan
     gl::McPlan M;
     int max_batch;
@@ -982,6 +983,13 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         if ((rc = upload(p, z8.data(), z8.size() * sizeof(int), &dptr))) { gl_plan_destroy(p); return rc; }
         P.batch_n = (const int*)dptr;
     }
+    p->order_buf = nullptr;
+    bool has_epl = false;
+    for (int i = 0; i < P.nlens; ++i) has_epl = has_epl || P.comp[i].kind == glm::K_EPL;
+    if (has_epl && !p->epl_batch) {
+        if ((rc = dev_alloc(p->dev, (size_t)gl::kOrderCap * sizeof(int), &dptr, false))) { gl_plan_destroy(p); return rc; }
+        p->order_buf = (int*)dptr;
+    }
     hipError_t e = p->model->attr(p->lds_bytes);
     if (e != hipSuccess) {
         gl_plan_destroy(p);
@@ -1048,6 +1056,15 @@ static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream)
         hipLaunchKernelGGL(gl::epl_batch_series_kernel, dim3(1), dim3(256), 0, stream, plan->P, A.in, A.bs, use_z,
                            const_cast<int*>(plan->P.batch_n));
         GL_HIP(hipGetLastError());
+    }
+    // launch order by cost when the launch has several rounds of workgroups (gradient launches)
+    A.order = nullptr;
+    if (plan->order_buf && grad && A.bs > 1024 && A.bs <= gl::kOrderCap) {
+        const int use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
+        hipLaunchKernelGGL(gl::order_kernel, dim3(1), dim3(1024), 0, stream, plan->P, A.in, A.bs, use_z,
+                           plan->order_buf);
+        GL_HIP(hipGetLastError());
+        A.order = plan->order_buf;
     }
     GL_HIP(plan->model->launch(A.bs, plan->lds_bytes, stream, plan->P, A, grad));
     return GL_OK;

@@ -40,6 +40,7 @@ struct KArgs {
     float* v;
     const int* lens;      // optional per-particle observed-image index (bs)
     const float* weight;  // MAP: optional per-particle loss weight (bs); default 1/bs
+    const int* order;     // optional: particle of workgroup b is order[b] (launch order, see order_kernel)
     float lr_t, beta1, beta2, eps, gscale;
     int mode;
     int bs;
@@ -573,7 +574,7 @@ template <int BLOCK, class M, bool GRAD>
 __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KArgs A) {
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
-    const int pid = blockIdx.x;
+    const int pid = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
     const int tid = threadIdx.x;
     const int d = P.d;
     float* s_sub = smem;
@@ -1023,6 +1024,63 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
             }
         }
     }
+}
+
+// Launch order of a fused launch: particles by descending EPL series length
+// (summed over the EPL slots), which sets a workgroup's run time, so the longest
+// workgroups start first and the tail of the launch holds the shortest.  The
+// result of every particle is independent of the order.  One workgroup: keys
+// in LDS, counting sort over the series lengths; the length is the epl.py:41
+// rule evaluated in FP32 (an ordering estimate only).
+constexpr int kOrderCap = 16384;
+constexpr int kOrderBins = 3 * GL_MAX_EPL_ITER + 1;
+__global__ __launch_bounds__(1024) void order_kernel(const DPlan P, const float* in, int bs, int use_z, int* order) {
+    __shared__ unsigned short key[kOrderCap];
+    __shared__ int cnt[kOrderBins];
+    for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    for (int p = threadIdx.x; p < bs; p += blockDim.x) {
+        int n = 0;
+#pragma unroll
+        for (int ci = 0; ci < kMaxLens; ++ci) {
+            if (ci >= P.nlens) break;
+            const DComp& cd = P.comp[ci];
+            if (cd.kind != glm::K_EPL) continue;
+            float e[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int col = cd.col[2 + j];
+                float v = col >= 0 ? in[(size_t)p * P.d + col] : cd.cst[2 + j];
+                if (col >= 0 && use_z) {
+                    glm::PriorOut o;
+                    glm::prior_eval(P.prior_kind[col], P.prior_args + 8 * col, v, 0, o);
+                    v = o.x;
+                }
+                e[j] = v;
+            }
+            const float c = fminf(fmaxf(sqrtf(e[0] * e[0] + e[1] * e[1]), 0.0f), 1.0f);
+            const float q = (1.0f - c) / (1.0f + c);
+            const float f = (1.0f - q) / (1.0f + q);
+            int m = cd.niter;
+            if (f > 0.0f && f < 1.0f) m = (int)fminf(ceilf(-27.631021f / logf(f) + 2.0f) - 1.0f, (float)cd.niter);
+            if (!(f > 0.0f)) m = 1;
+            n += m < P.max_n ? m : P.max_n;
+        }
+        const int b = kOrderBins - 1 - min(max(n, 0), kOrderBins - 1);
+        key[p] = (unsigned short)b;
+        atomicAdd(&cnt[b], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < kOrderBins; ++i) {
+            const int c = cnt[i];
+            cnt[i] = s;
+            s += c;
+        }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < bs; p += blockDim.x) order[atomicAdd(&cnt[key[p]], 1)] = p;
 }
 
 // Batch-coupled EPL series length (epl.py:41): one block reduces max f over

@@ -7,6 +7,9 @@
  * asynchronous, and no call allocates device memory or synchronises the
  * host except gl_plan_create / gl_plan_destroy.  Errors are reported as a
  * non-zero return code; gl_last_error() returns a thread-local message.
+ * A plan owns scratch memory (the staged pipeline's workspace, the fused
+ * kernel's launch-order buffer), so calls on one plan must be issued on one
+ * stream (or otherwise ordered); use one plan per stream for concurrency.
  *
  * Parameter vectors use the reference's flatten order (tf.nest order of a
  * prior sample: component lists in order, dict keys sorted), i.e. the
