@@ -538,13 +538,22 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
 }
 
 // One pass of the staged pipeline over the lenses of table T.
+// Conv launch group of a batch: the lenses of one (ss, kf4); the prefix table
+// counts only their conv waves, so the group's launch runs compile-time ss with
+// its own tap width.
+struct ConvGroup {
+    int ss, kf4, tot;
+    const int* pre;
+};
+
 // model 1: every lens is a fixed-amplitude plan of the reference default model
 // (EPL + shear | Sersic ellipse | Sersic ellipse): model-specialised basis and gradient.
 using DefaultModel = gl::ModelT<EPL, SHR, 0, SE, 0, SE, 0>;
 
 template <bool MULTI>
 static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch, int kf4, int lstsq, int gram_nb,
-                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st, bool shp = false, int model = 0) {
+                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st, bool shp = false, int model = 0,
+                       const std::vector<ConvGroup>* groups = nullptr) {
     const int np = T.tot_part;
     hipLaunchKernelGGL((gl::mc_prologue<MULTI>), dim3(np), dim3(128), 0, st, T, A);
     GL_HIP(hipGetLastError());
@@ -560,6 +569,14 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
         const long gv = (per * ((P.N + gl::kSepIB - 1) / gl::kSepIB) + 3) / 4;
         if (gh > INT32_MAX) return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
         GL_HIP(mc_sep_dispatch(ss_launch, kf4, (int)gh, (int)gv, st, T));
+    } else if (MULTI && groups && !groups->empty()) {
+        for (const ConvGroup& g : *groups) {
+            if (g.tot <= 0) continue;
+            gl::McTable Tg = T;
+            Tg.pre_conv = g.pre;
+            Tg.tot_conv = g.tot;
+            GL_HIP(mc_conv_dispatch<MULTI>(g.ss, g.kf4, gl::kMcIB, (g.tot + 3) / 4, st, Tg));
+        }
     } else {
         GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, MULTI ? 4 : T.one.M.conv_ib, (T.tot_conv + 3) / 4, st, T));
     }
@@ -1175,6 +1192,7 @@ extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* 
 // ---------------------------------------------------------------------------
 struct gl_batch {
     int n, d, total, ss_launch, kf4;
+    std::vector<ConvGroup> groups;  // conv launches by (ss, kf4)
     int mc_model;  // 1: every lens is of the reference default model (specialised kernels)
     gl::McTable T;
     std::vector<int> counts;
@@ -1222,7 +1240,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         E.M = plans[i]->M;
         E.M.conv_ib = gl::kMcIB;  // batches launch the direct conv at the default rows per lane
         E.M.sep_rank = 0;
-        int rc = mc_workspace(E.P, E.M, counts[i], b->kf4, 1, plans[i]->W_host, b->dev);
+        int rc = mc_workspace(E.P, E.M, counts[i], plans[i]->kf4, 1, plans[i]->W_host, b->dev);
         if (rc) { gl_batch_destroy(b); return rc; }
         E.first = (int)acc[0];
         E.count = counts[i];
@@ -1263,6 +1281,24 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     if (!rc) b->T.pre_grad = (const int*)put(pre_grad.data(), pre_grad.size() * sizeof(int));
     if (!rc) b->T.pre_adjh = (const int*)put(pre_adjh.data(), pre_adjh.size() * sizeof(int));
     if (!rc) b->T.pre_adjv = (const int*)put(pre_adjv.data(), pre_adjv.size() * sizeof(int));
+    // conv groups by (ss, kf4)
+    for (int i = 0; i < n && !rc; ++i) {
+        bool seen = false;
+        for (const ConvGroup& g : b->groups) seen = seen || (g.ss == plans[i]->P.ss && g.kf4 == plans[i]->kf4);
+        if (seen) continue;
+        ConvGroup g;
+        g.ss = plans[i]->P.ss;
+        g.kf4 = plans[i]->kf4;
+        std::vector<int> pre(n + 1, 0);
+        long a = 0;
+        for (int j = 0; j < n; ++j) {
+            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) a += mc_units(lens[j].P, lens[j].M, counts[j]).conv;
+            pre[j + 1] = (int)a;
+        }
+        g.tot = (int)a;
+        g.pre = (const int*)put(pre.data(), pre.size() * sizeof(int));
+        b->groups.push_back(g);
+    }
     if (rc) { gl_batch_destroy(b); return rc; }
     b->T.n = n;
     b->T.tot_part = (int)acc[0];
@@ -1297,7 +1333,8 @@ extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, f
     A.lp = lp;
     A.chi2 = chi2;
     A.grad = grad_z;
-    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false, b->mc_model);
+    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false, b->mc_model,
+                             &b->groups);
 }
 
 extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, float* v, int step, float lr,
@@ -1321,7 +1358,8 @@ extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, flo
     A.beta1 = beta1;
     A.beta2 = beta2;
     A.eps = epsilon;
-    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream, false, b->mc_model);
+    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream, false, b->mc_model,
+                             &b->groups);
 }
 
 // ---------------------------------------------------------------------------
