@@ -1472,6 +1472,8 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
         const int sp = cnn_wsplit(L, (int)B, &kc);
         part = std::max(part, (size_t)sp * L.cout * (9 * L.cin + 1));
     }
+    // the head's weight-gradient chunks share the partial buffer
+    part = std::max(part, (size_t)gl::cnn::kHeadChunks * ((size_t)Hd * c4 + Hd + (size_t)d2 * Hd + d2));
     net->part_elems = part;
     CNN_ALLOC(net->part, float, part);
     CNN_ALLOC(net->h0, float, B * c4);
@@ -1569,8 +1571,11 @@ extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* t
     GL_HIP(hipGetLastError());
     {
         const int n = net->hidden * net->L[3].cout + net->hidden + 2 * net->d * net->hidden + 2 * net->d;
-        hipLaunchKernelGGL(head_wgrad, dim3((n + 255) / 256), dim3(256), 0, stream, A, bs, net->grad + net->w1,
-                           net->grad + net->b1, net->grad + net->w2, net->grad + net->b2);
+        // the layers' weight-gradient partials reuse net->part after this (stream order)
+        hipLaunchKernelGGL(head_wgrad, dim3((n + 255) / 256, kHeadChunks), dim3(256), 0, stream, A, bs, net->part);
+        GL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(head_wgrad_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, A, net->part,
+                           net->grad + net->w1, net->grad + net->b1, net->grad + net->w2, net->grad + net->b2);
         GL_HIP(hipGetLastError());
     }
     if (loss) {

@@ -372,30 +372,52 @@ __global__ __launch_bounds__(256) void head_kernel(const HeadArgs A) {
     }
 }
 
-// Head weight gradients: sums over the batch of the per-image outer products.
-__global__ void head_wgrad(const HeadArgs A, int bs, float* gw1, float* gb1, float* gw2, float* gb2) {
+// Head weight gradients: sums over the batch of the per-image outer products,
+// in two passes.  Pass 1 (grid x: weights, y: kHeadChunks image ranges) writes
+// each range's sum to part[chunk][weight]; pass 2 adds the ranges in order, so
+// the result does not depend on scheduling.  Weight index i runs over
+// [gw1 (Hd x C) | gb1 (Hd) | gw2 (2d x Hd) | gb2 (2d)].
+constexpr int kHeadChunks = 16;
+__global__ void head_wgrad(const HeadArgs A, int bs, float* part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int C = A.C, Hd = A.Hd, d2 = 2 * A.d;
     const int n1 = Hd * C, n2 = Hd, n3 = d2 * Hd, n4 = d2;
+    const int n = n1 + n2 + n3 + n4;
+    if (i >= n) return;
+    const int per = (bs + kHeadChunks - 1) / kHeadChunks;
+    const int b0 = blockIdx.y * per, b1 = min(bs, b0 + per);
     float s = 0.0f;
     if (i < n1) {
         const int j = i / C, c = i - j * C;
-        for (int b = 0; b < bs; ++b) s = fmaf(A.da1[(size_t)b * Hd + j], A.h0[(size_t)b * C + c], s);
-        gw1[i] = s;
+        for (int b = b0; b < b1; ++b) s = fmaf(A.da1[(size_t)b * Hd + j], A.h0[(size_t)b * C + c], s);
     } else if (i < n1 + n2) {
         const int j = i - n1;
-        for (int b = 0; b < bs; ++b) s += A.da1[(size_t)b * Hd + j];
-        gb1[j] = s;
+        for (int b = b0; b < b1; ++b) s += A.da1[(size_t)b * Hd + j];
     } else if (i < n1 + n2 + n3) {
         const int q = i - n1 - n2;
         const int o = q / Hd, j = q - o * Hd;
-        for (int b = 0; b < bs; ++b) s = fmaf(A.dout[(size_t)b * d2 + o], A.h1[(size_t)b * Hd + j], s);
-        gw2[q] = s;
-    } else if (i < n1 + n2 + n3 + n4) {
+        for (int b = b0; b < b1; ++b) s = fmaf(A.dout[(size_t)b * d2 + o], A.h1[(size_t)b * Hd + j], s);
+    } else {
         const int o = i - n1 - n2 - n3;
-        for (int b = 0; b < bs; ++b) s += A.dout[(size_t)b * d2 + o];
-        gb2[o] = s;
+        for (int b = b0; b < b1; ++b) s += A.dout[(size_t)b * d2 + o];
     }
+    part[(size_t)blockIdx.y * n + i] = s;
+}
+
+__global__ void head_wgrad_reduce(const HeadArgs A, const float* part, float* gw1, float* gb1, float* gw2,
+                                  float* gb2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int C = A.C, Hd = A.Hd, d2 = 2 * A.d;
+    const int n1 = Hd * C, n2 = Hd, n3 = d2 * Hd, n4 = d2;
+    const int n = n1 + n2 + n3 + n4;
+    if (i >= n) return;
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kHeadChunks; ++c) s += part[(size_t)c * n + i];
+    if (i < n1) gw1[i] = s;
+    else if (i < n1 + n2) gb1[i - n1] = s;
+    else if (i < n1 + n2 + n3) gw2[i - n1 - n2] = s;
+    else gb2[i - n1 - n2 - n3] = s;
 }
 
 // mean of the per-image losses (one workgroup)
