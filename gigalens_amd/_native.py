@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("GIGALENS_AMD_LIB") or os.path.join(os.path.dirname(os
 
 GL_MAX_COMPONENTS = 8
 GL_MAX_COMPONENT_PARAMS = 10
+GL_OK, GL_ERR_ARG, GL_ERR_HIP, GL_ERR_UNSUPPORTED = 0, 1, 2, 3
 GL_MAX_PARAMS = 128
 GL_MAX_EPL_ITER = 100
 

@@ -708,8 +708,8 @@ static int upload_common(const gl_plan_desc* desc, gl_plan* p) {
 }
 
 extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
+    if (out) *out = nullptr;
     if (!desc || !out) return fail(GL_ERR_ARG, "gl_plan_create: null argument");
-    *out = nullptr;
     const int N = desc->num_pix, ss = desc->supersample;
     if (N <= 0 || ss <= 0) return fail(GL_ERR_ARG, "num_pix and supersample must be positive");
     if (desc->n_params <= 0 || desc->n_params > GL_MAX_PARAMS)
@@ -1207,8 +1207,8 @@ extern "C" int gl_batch_destroy(gl_batch* b) {
 }
 
 extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* counts, gl_batch** out) {
+    if (out) *out = nullptr;
     if (!plans || !counts || !out || n <= 0) return fail(GL_ERR_ARG, "gl_batch_create: bad arguments");
-    *out = nullptr;
     for (int i = 0; i < n; ++i) {
         const gl_plan* q = plans[i];
         if (!q || !q->mc || q->M.lstsq)
@@ -1415,8 +1415,8 @@ static int cnn_wsplit(const gl_cnn::Layer& l, int bs, int* kchunk) {
 }
 
 extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
+    if (out) *out = nullptr;
     if (!desc || !out) return fail(GL_ERR_ARG, "gl_cnn_create: null argument");
-    *out = nullptr;
     if (desc->height < 4 || desc->width < 4 || desc->max_batch <= 0)
         return fail(GL_ERR_ARG, "gl_cnn_create: bad image size or max_batch");
     for (int i = 0; i < 4; ++i)
