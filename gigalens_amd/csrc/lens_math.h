@@ -368,6 +368,21 @@ GLM void sersic_adjoint(const SerC& C, const SerState& S, float g, float* acc, f
     gdy = Ks2 * w * S.gy;
 }
 
+// d/d(e1, e2) from d/d(Axx, Axy, Ayy) of the quadratic form Q = A_xx dx^2 +
+// 2 A_xy dx dy + A_yy dy^2 of the elliptical distance (sersic.py:33-59), through
+// q = (1 - c)/(1 + c), c = |e| and phi = atan2(e2, e1)/2.
+GLM void ellipse_finish(float q, float cp, float sp, float c, float e1, float e2, float dAxx, float dAxy,
+                        float dAyy, float& ge1, float& ge2) {
+    float s2p = 2.0f * sp * cp, c2p = cp * cp - sp * sp;
+    float iq = 1.0f / q, iq2 = iq * iq;
+    float dq = dAxx * (cp * cp - sp * sp * iq2) + dAyy * (sp * sp - cp * cp * iq2) + dAxy * 0.5f * s2p * (1.0f + iq2);
+    float dphi = dAxx * s2p * (iq - q) + dAyy * s2p * (q - iq) + dAxy * c2p * (q - iq);
+    float dc = dq * (-2.0f / ((1.0f + c) * (1.0f + c)));
+    float cc = e1 * e1 + e2 * e2;
+    ge1 = dphi * (-e2 * 0.5f / cc) + dc * e1 / c;
+    ge2 = dphi * (e1 * 0.5f / cc) + dc * e2 / c;
+}
+
 // Epilogue: g receives d/d(R_sersic, n_sersic, e1, e2, center_x, center_y, Ie)
 // (e1,e2 only for the elliptical profile).
 GLM void sersic_finish(const SerC& C, const float* acc, float* g) {
@@ -379,18 +394,8 @@ GLM void sersic_finish(const SerC& C, const float* acc, float* g) {
     g[0] = dk0 * (-1.0f / (n * C.Rs * kLn2));
     g[1] = 1.9992f * dbn + dk1 * (-0.5f / (n * n)) + dk0 * (log2f(C.Rs) / (n * n));
     if (C.elliptical) {
-        float dAxx = Ks * acc[3], dAxy = 2.0f * Ks * acc[4], dAyy = Ks * acc[5];
-        float q = C.q, cp = C.cphi, sp = C.sphi;
-        float s2p = 2.0f * sp * cp, c2p = cp * cp - sp * sp;
-        float iq = 1.0f / q, iq2 = iq * iq;
-        float dq = dAxx * (cp * cp - sp * sp * iq2) + dAyy * (sp * sp - cp * cp * iq2) +
-                   dAxy * 0.5f * s2p * (1.0f + iq2);
-        float dphi = dAxx * s2p * (iq - q) + dAyy * s2p * (q - iq) + dAxy * c2p * (q - iq);
-        float c = C.c;
-        float dc = dq * (-2.0f / ((1.0f + c) * (1.0f + c)));
-        float cc = C.e1 * C.e1 + C.e2 * C.e2;
-        g[2] = dphi * (-C.e2 * 0.5f / cc) + dc * C.e1 / c;
-        g[3] = dphi * (C.e1 * 0.5f / cc) + dc * C.e2 / c;
+        ellipse_finish(C.q, C.cphi, C.sphi, C.c, C.e1, C.e2, Ks * acc[3], 2.0f * Ks * acc[4], Ks * acc[5], g[2],
+                       g[3]);
     } else {
         g[2] = 0.0f; g[3] = 0.0f;
     }
@@ -710,16 +715,7 @@ GLM void core_sersic_finish(const CoreSerC& C, const float* acc, float* g) {
     g[3] = -C.gam * ia * ia * acc[1] + C.gam * ia * (C.lnRb * acc[2] - acc[4]) +
            C.kexp * (base * (ia + C.lnRs) - C.iRsa * (acc[5] + C.Rba * C.lnRb * acc[0]));
     g[4] = acc[1] * ia;
-    float dAxx = acc[6], dAxy = 2.0f * acc[7], dAyy = acc[8];
-    float q = C.q, cp = C.cphi, sp = C.sphi;
-    float s2p = 2.0f * sp * cp, c2p = cp * cp - sp * sp;
-    float iq = 1.0f / q, iq2 = iq * iq;
-    float dq = dAxx * (cp * cp - sp * sp * iq2) + dAyy * (sp * sp - cp * cp * iq2) + dAxy * 0.5f * s2p * (1.0f + iq2);
-    float dphi = dAxx * s2p * (iq - q) + dAyy * s2p * (q - iq) + dAxy * c2p * (q - iq);
-    float dc = dq * (-2.0f / ((1.0f + C.c) * (1.0f + C.c)));
-    float cc = C.e1 * C.e1 + C.e2 * C.e2;
-    g[5] = dphi * (-C.e2 * 0.5f / cc) + dc * C.e1 / C.c;
-    g[6] = dphi * (C.e1 * 0.5f / cc) + dc * C.e2 / C.c;
+    ellipse_finish(C.q, C.cphi, C.sphi, C.c, C.e1, C.e2, acc[6], 2.0f * acc[7], acc[8], g[5], g[6]);
     g[7] = -2.0f * acc[9];
     g[8] = -2.0f * acc[10];
     g[9] = acc[0] / C.Ie;
