@@ -1597,7 +1597,8 @@ extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* t
             hipLaunchKernelGGL((gemm<false, false, LdPlain, LdIm2col, EpPartial>),
                                dim3((M + 63) / 64, (N + 63) / 64, splits), dim3(256), 0, stream, la, lb, ep, M, N, K, kc);
             GL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(wgrad_reduce, dim3((M * N + 255) / 256), dim3(256), 0, stream, net->part, splits, M, N,
+            hipLaunchKernelGGL(wgrad_reduce, dim3((M * N + kWrOut - 1) / kWrOut), dim3(256), 0, stream, net->part,
+                               splits, M, N,
                                net->grad + L.woff, net->grad + L.boff);
             GL_HIP(hipGetLastError());
         }

@@ -234,14 +234,30 @@ __global__ void input_kernel(const float* img, u16* x0, long n, float inv_scale)
 }
 
 // split-K reduction of the weight gradient: part[S][cout][9 cin + 1] -> grad
-__global__ void wgrad_reduce(const float* part, int splits, int cout, int ncol, float* gw, float* gb) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cout * ncol) return;
+// block 256 = kWrOut outputs x kWrLanes split lanes: lane l sums splits l, l + 32, ...
+// of its output in order, then the lanes are added in a fixed tree (deterministic)
+constexpr int kWrOut = 8, kWrLanes = 32;
+__global__ __launch_bounds__(256) void wgrad_reduce(const float* part, int splits, int cout, int ncol, float* gw,
+                                                    float* gb) {
+    __shared__ float red[kWrLanes][kWrOut];
+    const int n = cout * ncol;
+    const int o = threadIdx.x % kWrOut, l = threadIdx.x / kWrOut;
+    const int i = blockIdx.x * kWrOut + o;
     float s = 0.0f;
-    for (int q = 0; q < splits; ++q) s += part[(size_t)q * cout * ncol + i];
-    const int co = i / ncol, kk = i - co * ncol;
-    if (kk < ncol - 1) gw[co * (ncol - 1) + kk] = s;
-    else gb[co] = s;
+    if (i < n)
+        for (int q = l; q < splits; q += kWrLanes) s += part[(size_t)q * n + i];
+    red[l][o] = s;
+    __syncthreads();
+    for (int w = kWrLanes / 2; w > 0; w >>= 1) {
+        if (l < w) red[l][o] += red[l + w][o];
+        __syncthreads();
+    }
+    if (l == 0 && i < n) {
+        const float t = red[0][o];
+        const int co = i / ncol, kk = i - co * ncol;
+        if (kk < ncol - 1) gw[co * (ncol - 1) + kk] = t;
+        else gb[co] = t;
+    }
 }
 
 // bf16 GEMM operands from the FP32 master weights W[co][9 cin]:
