@@ -258,12 +258,12 @@ def shapelet_hmc(prior, phys, cfg, pm, z0, dev):
     t_svi = time.perf_counter() - t0
     t0 = time.perf_counter()
     samples, stats = seq.HMC(q, init_eps=0.01, init_l=10, n_hmc=256, num_burnin_steps=0, num_results=100,
-                             adapt=False)
+                             adapt=False, graph=True)
     torch.cuda.synchronize()
     t_hmc = time.perf_counter() - t0
     return {"map_steps": 200, "map_seconds": t_map, "best_chi2": float(chi2.detach()[best]),
             "svi_steps": 300, "svi_samples": 64, "svi_seconds": t_svi,
-            "chains": 256, "leapfrog_steps": 1000, "step_size": 0.01, "hmc_seconds": t_hmc,
+            "chains": 256, "leapfrog_steps": 1000, "step_size": 0.01, "hmc_seconds": t_hmc, "hmc_graph": True,
             "leapfrog_particle_evals_per_s": 256 * 1000 / t_hmc,
             "accept_rate_mean": float(np.mean(stats["accept_rate"]))}
 

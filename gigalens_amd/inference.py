@@ -258,14 +258,18 @@ class ModellingSequence(ModellingSequenceInterface):
 
     # ---- HMC ----------------------------------------------------------------
     def HMC(self, q_z, init_eps=0.3, init_l=3, n_hmc=50, num_burnin_steps=250, num_results=750,
-            max_leapfrog_steps=30, seed=3, adapt=True, target_accept=0.75, start=None):
+            max_leapfrog_steps=30, seed=3, adapt=True, target_accept=0.75, start=None, graph=False):
         """Preconditioned HMC (tf/inference.py:75-141): momentum ~ N(0, Sigma^-1)
         with Sigma the surrogate covariance (kinetic energy p^T Sigma p / 2),
         dual-averaging step-size adaptation and ChEES trajectory-length
         adaptation during 80% of the burn-in.  ``adapt=False`` runs fixed
         ``init_eps`` / ``init_l``.  ``q_z=None`` gives plain HMC (identity mass);
         ``start`` (n_hmc, d) overrides the chains' initial states (default: draws
-        from q_z).  Returns ``(samples (num_results, n_hmc, d), stats)``."""
+        from q_z).  ``graph=True`` records one leapfrog step (position update,
+        log_prob + gradient launch, momentum update) into a HIP graph on static
+        buffers, with the step size in a device scalar, and replays it L times
+        per trajectory: the same arithmetic in the same order as the eager loop.
+        Returns ``(samples (num_results, n_hmc, d), stats)``."""
         gen = torch.Generator().manual_seed(int(seed))
         if q_z is None:
             d0 = start.shape[1]
@@ -292,6 +296,19 @@ class ModellingSequence(ModellingSequenceInterface):
             return lp, g
 
         lp, g = lp_grad(z)
+        leap = None
+        if graph:
+            from .graphs import capture
+            st_z, st_p, st_g, st_lp = z.clone(), torch.zeros_like(z), g.clone(), lp.clone()
+            eps_t = torch.zeros((), dtype=torch.float32, device=self.device)
+
+            def step():
+                st_z.add_((eps_t * st_p) @ cov)
+                lpn_, gn_ = lp_grad(st_z)
+                st_lp.copy_(lpn_)
+                st_g.copy_(gn_)
+                st_p.add_(eps_t * st_g)
+            leap = capture(step)
         n_adapt = int(num_burnin_steps * 0.8)
         eps = float(init_eps)
         traj = float(init_eps) * int(init_l)  # trajectory length (ChEES adapts this)
@@ -310,11 +327,19 @@ class ModellingSequence(ModellingSequenceInterface):
             p = torch.randn((n_hmc, d), generator=gen).to(self.device) @ chol_inv
             # p ~ N(0, Sigma^-1):  p = eps_std @ L^-1 with Sigma = L L^T
             z0, lp0, g0, p0 = z, lp, g, p
-            zn, pn, gn = z.clone(), p + 0.5 * eps * g, g
-            for _ in range(L):
-                zn = zn + eps * pn @ cov
-                lpn, gn = lp_grad(zn)
-                pn = pn + eps * gn
+            if leap is not None:
+                st_z.copy_(z)
+                st_p.copy_(p + 0.5 * eps * g)
+                eps_t.fill_(eps)
+                for _ in range(L):
+                    leap.replay()
+                zn, pn, gn, lpn = st_z.clone(), st_p.clone(), st_g.clone(), st_lp.clone()
+            else:
+                zn, pn, gn = z.clone(), p + 0.5 * eps * g, g
+                for _ in range(L):
+                    zn = zn + eps * pn @ cov
+                    lpn, gn = lp_grad(zn)
+                    pn = pn + eps * gn
             pn = pn - 0.5 * eps * gn
             h0 = -lp0 + 0.5 * ((p0 @ cov) * p0).sum(1)
             h1 = -lpn + 0.5 * ((pn @ cov) * pn).sum(1)

@@ -51,3 +51,23 @@ def test_graph_replays_repeat(cuda):
         torch.cuda.synchronize()
         _ = plan.log_prob(z0)  # eager work between replays
         assert torch.equal(z, ze) and torch.equal(lp, lpe), rep
+
+
+def test_hmc_graph_leapfrog_equals_eager(cuda):
+    """HMC with graph-captured leapfrog steps (one recorded step replayed L times,
+    the step size in a device scalar) against the eager loop: identical samples and
+    acceptance, with dual-averaging step-size adaptation changing the step size
+    between trajectories."""
+    prob = C.epl_shear_problem(bs=32, seed=8)
+    pm = ForwardProbModel(prob["prior"], prob["obs"], 0.2, 100.0)
+    seq = ModellingSequence(prob["phys"], pm, prob["cfg"], device=cuda)
+    xt = torch.as_tensor(prob["truth_x"], dtype=torch.float32)
+    zt = pm.bij.inverse(C.nested_from_x(prob["prior"], xt))
+    start = (zt.repeat(32, 1) + 1e-3 * torch.randn(32, zt.shape[1], generator=torch.Generator().manual_seed(2)))
+    kw = dict(init_eps=1e-3, init_l=3, num_burnin_steps=10, num_results=6, max_leapfrog_steps=5, seed=4,
+              start=start.to(cuda))
+    s_e, st_e = seq.HMC(None, **kw)
+    s_g, st_g = seq.HMC(None, graph=True, **kw)
+    assert torch.equal(s_e, s_g)
+    assert np.array_equal(st_e["accept_rate"], st_g["accept_rate"])
+    assert np.array_equal(st_e["step_size"], st_g["step_size"])
