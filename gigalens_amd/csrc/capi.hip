@@ -374,7 +374,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     M.D = D;
     // per-particle constants: prior arrays, component blocks, EPL coefficient tables
     P.max_n = max_niter;
-    const int coef_block = ((max_niter + 4) & ~3) + 4 * (max_niter + 1);
+    const int coef_block = glm::epl_tab_floats(max_niter);
     const int prior_f = (6 * P.d + 3) & ~3;
     M.coef_base = prior_f + gl::kMaxComp * gl::kConstStride;
     int coff = 0;
@@ -896,7 +896,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     P.off_c = off; off += gl::kMaxComp * gl::kConstStride;
     P.off_coef = off;
     P.max_n = max_niter;
-    const int coef_block = ((max_niter + 4) & ~3) + 4 * (max_niter + 1);
+    const int coef_block = glm::epl_tab_floats(max_niter);
     for (int i = 0; i < 3; ++i) {
         P.coef_off[i] = (sig[i] == glm::K_EPL) ? (off - P.off_coef) : 0;
         if (sig[i] == glm::K_EPL) off += (coef_block + 3) & ~3;

@@ -410,7 +410,7 @@ __device__ __forceinline__ void load_consts(const float* s_c, const float* s_coe
         if (k == glm::K_EPL) {
             C.epl[i] = uniform_load<glm::EplC>(s_c + i * kConstStride);
             C.coefA[i] = s_coef + P.coef_off[i];
-            C.coefB[i] = reinterpret_cast<const float4*>(s_coef + P.coef_off[i] + ((P.max_n + 4) & ~3));
+            C.coefB[i] = reinterpret_cast<const float4*>(s_coef + P.coef_off[i] + glm::epl_tab_a_floats(P.max_n));
         } else if (k == glm::K_SHEAR) {
             C.shr[i] = uniform_load<glm::ShearC>(s_c + i * kConstStride);
         } else if (k == glm::K_SIS) {
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
         if (M::lens[i] == glm::K_EPL && (tid >> 6) == i) {
             const glm::EplC* c = reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
             float* ca = s_coef + P.coef_off[i];
-            float4* cb = reinterpret_cast<float4*>(ca + ((P.max_n + 4) & ~3));
+            float4* cb = reinterpret_cast<float4*>(ca + glm::epl_tab_a_floats(P.max_n));
             const int NN = c->N;
             const int n = tid & 63;
             if (NN < 64) {
@@ -675,6 +675,10 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
                     ca[k] = cv;
                     cb[k] = make_float4(cv, dt, df, 0.0f);
                 }
+            }
+            for (int k = NN + 1 + n; k <= glm::epl_pad(NN); k += 64) {  // zero padding above N
+                ca[k] = 0.0f;
+                cb[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
     }

@@ -182,31 +182,51 @@ struct EplState {
     float Etr, Eti, Efr, Efi;
 };
 
+// Coefficient tables of one EPL lens: c_n (floats), then (c_n, dc_n/dt, dc_n/df, 0)
+// (float4s), each holding indices 0..epl_pad(N) with zeros above N.  The Clenshaw
+// loops below step four terms at a time from epl_pad(N) down; a zero coefficient
+// above the last nonzero one keeps the recurrence state exactly 0, so the result
+// equals the unpadded N-term recurrence bit for bit, without a remainder loop.
+GLM int epl_pad(int n) { return (n + 3) & ~3; }
+// floats of the c_n table for series lengths up to max_n (a multiple of 4, so the
+// float4 table after it stays 16-byte aligned)
+GLM int epl_tab_a_floats(int max_n) { return epl_pad(max_n) + 4; }
+// floats of both tables of one lens
+GLM int epl_tab_floats(int max_n) { return epl_tab_a_floats(max_n) + 4 * (epl_pad(max_n) + 1); }
+
 // Clenshaw evaluation of sum_{n=0..N} c_n w^n, w = cos th + i sin th.
-// coef: c_n (float).  Returns real/imag parts.
+// coef: c_n (float), zero-padded to epl_pad(N).  Returns real/imag parts.
 GLM void clenshaw1(const float* __restrict__ coef, int N, float c2, float s2, float& re, float& im) {
     float x2 = c2 + c2;
     float b1 = 0.0f, b2 = 0.0f;
-    for (int k = N; k >= 1; --k) {
-        float b0 = fmaf(x2, b1, coef[k]) - b2;
-        b2 = b1;
-        b1 = b0;
+#pragma unroll 1
+    for (int k = epl_pad(N); k >= 4; k -= 4) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float b0 = fmaf(x2, b1, coef[k - j]) - b2;
+            b2 = b1;
+            b1 = b0;
+        }
     }
     re = fmaf(c2, b1, coef[0]) - b2;
     im = s2 * b1;
 }
 
-// Three series at once; coef4[n] = (c_n, dc_n/dt, dc_n/df, 0).
+// Three series at once; coef4[n] = (c_n, dc_n/dt, dc_n/df, 0), zero-padded to epl_pad(N).
 GLM void clenshaw3(const float4* __restrict__ coef4, int N, float c2, float s2, float& re, float& im,
                    float& tre, float& tim, float& fre, float& fim) {
     float x2 = c2 + c2;
     float a1 = 0.f, a2 = 0.f, d1 = 0.f, d2 = 0.f, g1 = 0.f, g2 = 0.f;
-    for (int k = N; k >= 1; --k) {
-        float4 cc = coef4[k];
-        float a0 = fmaf(x2, a1, cc.x) - a2;
-        float d0 = fmaf(x2, d1, cc.y) - d2;
-        float g0 = fmaf(x2, g1, cc.z) - g2;
-        a2 = a1; a1 = a0; d2 = d1; d1 = d0; g2 = g1; g1 = g0;
+#pragma unroll 1
+    for (int k = epl_pad(N); k >= 4; k -= 4) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float4 cc = coef4[k - j];
+            float a0 = fmaf(x2, a1, cc.x) - a2;
+            float d0 = fmaf(x2, d1, cc.y) - d2;
+            float g0 = fmaf(x2, g1, cc.z) - g2;
+            a2 = a1; a1 = a0; d2 = d1; d1 = d0; g2 = g1; g1 = g0;
+        }
     }
     re = fmaf(c2, a1, coef4[0].x) - a2;
     im = s2 * a1;

@@ -280,10 +280,11 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
         if (i >= P.nlens || P.comp[i].kind != glm::K_EPL) continue;
         const glm::EplC* c = reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
         float* ca = pc + M.coef_base + P.coef_off[i];
-        float4* cb = reinterpret_cast<float4*>(ca + ((P.max_n + 4) & ~3));
-        for (int n = tid; n <= c->N; n += blockDim.x) {
-            float cv, dt, df;
-            glm::epl_coef(n, (double)c->f, (double)c->t, cv, dt, df);
+        float4* cb = reinterpret_cast<float4*>(ca + glm::epl_tab_a_floats(P.max_n));
+        const int NN = c->N;
+        for (int n = tid; n <= glm::epl_pad(NN); n += blockDim.x) {
+            float cv = 0.0f, dt = 0.0f, df = 0.0f;  // zero padding above N
+            if (n <= NN) glm::epl_coef(n, (double)c->f, (double)c->t, cv, dt, df);
             ca[n] = cv;
             cb[n] = make_float4(cv, dt, df, 0.0f);
         }
@@ -316,7 +317,7 @@ __device__ __forceinline__ void mc_beta(const DPlan& P, const McPlan& M, const f
         if (kind == glm::K_EPL) {
             const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
             const float* ca = pc + M.coef_base + P.coef_off[i];
-            const float4* cb = reinterpret_cast<const float4*>(ca + ((P.max_n + 4) & ~3));
+            const float4* cb = reinterpret_cast<const float4*>(ca + glm::epl_tab_a_floats(P.max_n));
             glm::EplState st;
             glm::epl_forward<ADJ>(C, ca, cb, X, Y, ax, ay, st);
             if (ADJ) ls->es[i] = st;

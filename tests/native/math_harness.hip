@@ -7,8 +7,8 @@
 using namespace glm;
 
 static void coef_tables(const EplC& C, std::vector<float>& a, std::vector<float4>& b) {
-    a.resize(C.N + 1);
-    b.resize(C.N + 1);
+    a.assign(epl_pad(C.N) + 1, 0.0f);  // zero padding above N, as on the device
+    b.assign(epl_pad(C.N) + 1, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     for (int n = 0; n <= C.N; ++n) {
         float c, dt, df;
         epl_coef(n, (double)C.f, (double)C.t, c, dt, df);
