@@ -947,6 +947,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             const int TR = nsub + k - 1;
             const int xs = (N + gl::kTB - 1) / gl::kTB;
             const int urows = gp0 + N + dmax;
+            const int ucol = urows | 1;  // U column-major with an odd column stride
             // transposed horizontal pass (gl::SepGeom): phase b reads G columns from
             // X0 + off_b + gp0, off_b = db - (Mb - 1); one segment covers every phase
             int omin = 1 << 30, omax = -(1 << 30);
@@ -964,10 +965,11 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
                               (long)TR * nstr <= (long)gl::kMaxTT * kBlock && (long)N * nstr <= (long)gl::kMaxT * kBlock &&
                               // adjoint: G segments stay inside their row, U after G
                               omin + gp0 >= 0 && (long)(xs - 1) * gl::kTB + omin + gp0 + sega4 - 1 < GW &&
-                              (long)u_off + (long)R * urows * nsub <= P.sub_size;
+                              (long)u_off + (long)R * nsub * ucol <= P.sub_size;
             if (fits) {
                 P.sep_rank = R;
                 P.urows = urows;
+                P.ucol = ucol;
                 P.u_off = u_off;
                 P.sep_aligned = ((omin + gp0) % 4 == 0) ? 1 : 0;
                 sa.resize(R * kf);

@@ -297,12 +297,13 @@ struct SepGeom {
 };
 
 // One task: kTB adjoint columns X0.. of output row I, every phase b, every rank;
-// writes U_r[I][SS X + b] for X < N.  The phase loop is unrolled, so every tap
-// index is a compile-time constant and the factors arrive by scalar loads.
+// writes U_r[SS X + b][I] for X < N (U column-major, column stride US; the caller
+// passes U at row I of column 0).  The phase loop is unrolled, so every tap index
+// is a compile-time constant and the factors arrive by scalar loads.
 template <int SS, int KF>
 __device__ __forceinline__ void sep_upass(const float* __restrict__ G, float* __restrict__ U,
                                           const float* __restrict__ sb, int R, int GW, int gp0, int ustride,
-                                          int UW, bool aligned, bool vstore, int I, int X0, int N) {
+                                          int US, bool aligned, int I, int X0, int N) {
     using Gm = SepGeom<SS, KF>;
     constexpr int OMIN = Gm::off_min();
     const float* row = G + (I + gp0) * GW + (X0 + OMIN + gp0);
@@ -318,7 +319,6 @@ __device__ __forceinline__ void sep_upass(const float* __restrict__ G, float* __
 #pragma unroll
         for (int j = 0; j < Gm::SEGA4 * 4; ++j) seg[j] = j < Gm::SEGA ? row[j] : 0.0f;
     }
-    const bool full = vstore && X0 + kTB <= N;
 #pragma unroll
     for (int r = 0; r < kMaxRank; ++r) {
         if (r < R) {
@@ -338,25 +338,19 @@ __device__ __forceinline__ void sep_upass(const float* __restrict__ G, float* __
                     }
                 }
             }
-            float* dst = U + r * ustride + SS * X0;
-            if (full) {
+            float* dst = U + r * ustride + SS * X0 * US;
 #pragma unroll
-                for (int c = 0; c < SS * kTB / 4; ++c)
-                    reinterpret_cast<float4*>(dst)[c] = make_float4(o[4 * c], o[4 * c + 1], o[4 * c + 2], o[4 * c + 3]);
-            } else {
+            for (int j = 0; j < kTB; ++j)
+                if (X0 + j < N)
 #pragma unroll
-                for (int j = 0; j < kTB; ++j)
-                    if (X0 + j < N)
-#pragma unroll
-                        for (int b = 0; b < SS; ++b) dst[SS * j + b] = o[SS * j + b];
-            }
+                    for (int b = 0; b < SS; ++b) dst[(SS * j + b) * US] = o[SS * j + b];
         }
     }
 }
 
 template <int SS, int KF>
-__device__ __forceinline__ float sep_gsub(const float* __restrict__ U, const float* __restrict__ sa, int R, int UR,
-                                          int UW, int up0, int a, int Yc, int x) {
+__device__ __forceinline__ float sep_gsub(const float* __restrict__ U, const float* __restrict__ sa, int R,
+                                          int ustride, int US, int up0, int a, int Yc, int x) {
     constexpr int PB = (KF - SS) / 2;
     constexpr int MMAX = (KF + SS - 1) / SS;
     const int ra = (a + PB) % SS, da = (a + PB) / SS;
@@ -365,10 +359,11 @@ __device__ __forceinline__ float sep_gsub(const float* __restrict__ U, const flo
 #pragma unroll
     for (int r = 0; r < kMaxRank; ++r) {
         if (r < R) {
-            const float* col = U + r * UR * UW + (Yc + da + up0) * UW + x;
+            // column x of U is contiguous: the taps are constant offsets from one address
+            const float* col = U + r * ustride + x * US + (Yc + da + up0);
 #pragma unroll
             for (int m = 0; m < MMAX; ++m)
-                if (m < Ma) g = fmaf(sa[r * KF + SS * m + ra], col[-m * UW], g);
+                if (m < Ma) g = fmaf(sa[r * KF + SS * m + ra], col[-m], g);
         }
     }
     return g;
@@ -473,7 +468,7 @@ __device__ __forceinline__ void eval_adjoint(const Consts<M>& C, float X, float 
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         if (M::light[i] == glm::K_CORE_SERSIC) val += glm::core_sersic_forward(C.core[i], X, Y, cst[i]);
-        else if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], X, Y, lst[i]);
+        else if (M::light[i] != 0) val += glm::sersic_forward<true>(C.ser[i], X, Y, lst[i]);
     }
     float bx = X, by = Y;
     glm::EplState es[3];
@@ -494,7 +489,7 @@ __device__ __forceinline__ void eval_adjoint(const Consts<M>& C, float X, float 
 #pragma unroll
     for (int i = 2; i < 4; ++i) {
         if (M::light[i] == glm::K_CORE_SERSIC) val += glm::core_sersic_forward(C.core[i], bx, by, cst[i]);
-        else if (M::light[i] != 0) val += glm::sersic_forward(C.ser[i], bx, by, lst[i]);
+        else if (M::light[i] != 0) val += glm::sersic_forward<true>(C.ser[i], bx, by, lst[i]);
     }
     if (isnan(val)) return;  // zeroed in the forward pass: no contribution
     float gbx = 0.0f, gby = 0.0f;
@@ -862,24 +857,26 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
             // U lives after G in LDS (no overlap), so tasks store straight to it.
             const int XS = (N + kTB - 1) / kTB;
             const int utask = N * XS;
-            const int UR = P.urows, UW = P.nsub, up0 = gp0;
+            // U_r is column-major: nsub columns of P.urows rows, column stride US (odd,
+            // so lanes on consecutive columns hit distinct banks)
+            const int UR = P.urows, US = P.ucol, up0 = gp0;
+            const int ustride = nsub * US;
             float* s_u = s_sub + P.u_off;
-            for (int r = 0; r < R; ++r) {
-                float* ur = s_u + r * UR * UW;
-                for (int i = tid; i < up0 * UW; i += BLOCK) ur[i] = 0.0f;
-                for (int i = tid; i < (UR - up0 - N) * UW; i += BLOCK) ur[(up0 + N) * UW + i] = 0.0f;
+            const int zr = UR - N;  // rows of a column outside [up0, up0 + N): zero
+            for (int i = tid; i < R * nsub * zr; i += BLOCK) {
+                const int rc = i / zr, k = i - rc * zr;  // rc = r * nsub + column
+                s_u[rc * US + (k < up0 ? k : k + N)] = 0.0f;
             }
-            const bool vstore = (UW & 3) == 0;
             for (int task = tid; task < utask; task += BLOCK) {
                 const int I = task / XS;
                 const int X0 = (task - I * XS) * kTB;
-                float* urow = s_u + (I + up0) * UW;
+                float* urow = s_u + (I + up0);
                 if (sepk == 2022)
-                    sep_upass<2, 22>(s_sub, urow, P.sb, R, GW, gp0, UR * UW, UW, P.sep_aligned, vstore, I, X0, N);
+                    sep_upass<2, 22>(s_sub, urow, P.sb, R, GW, gp0, ustride, US, P.sep_aligned, I, X0, N);
                 else if (sepk == 1021)
-                    sep_upass<1, 21>(s_sub, urow, P.sb, R, GW, gp0, UR * UW, UW, P.sep_aligned, vstore, I, X0, N);
+                    sep_upass<1, 21>(s_sub, urow, P.sb, R, GW, gp0, ustride, US, P.sep_aligned, I, X0, N);
                 else
-                    sep_upass<3, 23>(s_sub, urow, P.sb, R, GW, gp0, UR * UW, UW, P.sep_aligned, vstore, I, X0, N);
+                    sep_upass<3, 23>(s_sub, urow, P.sb, R, GW, gp0, ustride, US, P.sep_aligned, I, X0, N);
             }
             __syncthreads();
             // ---- D2 + E: per sub-pixel transposed vertical pass + recompute, one row per wave ----
@@ -891,9 +888,9 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
                 const int a = y % ss, Yc = y / ss;
                 if (x < nsub) {
                     float g;
-                    if (sepk == 2022) g = sep_gsub<2, 22>(s_u, P.sa, R, UR, UW, up0, a, Yc, x);
-                    else if (sepk == 1021) g = sep_gsub<1, 21>(s_u, P.sa, R, UR, UW, up0, a, Yc, x);
-                    else g = sep_gsub<3, 23>(s_u, P.sa, R, UR, UW, up0, a, Yc, x);
+                    if (sepk == 2022) g = sep_gsub<2, 22>(s_u, P.sa, R, ustride, US, up0, a, Yc, x);
+                    else if (sepk == 1021) g = sep_gsub<1, 21>(s_u, P.sa, R, ustride, US, up0, a, Yc, x);
+                    else g = sep_gsub<3, 23>(s_u, P.sa, R, ustride, US, up0, a, Yc, x);
                     float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
                     float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
                     eval_adjoint<M>(C, X, Y, g, acc);

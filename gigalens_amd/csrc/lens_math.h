@@ -75,7 +75,7 @@ struct EplC {
     int N;
 };
 struct SerC {
-    float cx, cy, Axx, Axy, Ayy, k1, k0, m1, m0, Ie;
+    float cx, cy, Axx, Axy, Ayy, k1, k0, m1, m0, Ie, k1m1;
     float bn, n, Rs, e1, e2, q, cphi, sphi, c;
     int elliptical;
 };
@@ -168,6 +168,7 @@ GLM void sersic_prepare(float Rs, float n, float e1, float e2, float cx, float c
     }
     C.bn = 1.9992f * n - 0.3271f;
     C.k1 = 0.5f / n;
+    C.k1m1 = C.k1 - 1.0f;
     C.k0 = -log2f(Rs) / n;
     C.m1 = -C.bn * kLog2e;
     C.m0 = C.bn * kLog2e;
@@ -177,7 +178,7 @@ GLM void sersic_prepare(float Rs, float n, float e1, float e2, float cx, float c
 // Per sub-pixel forward
 // ---------------------------------------------------------------------------
 struct EplState {
-    float u, v, qu, r2, r2c, L, A, cp, sp, inrange;
+    float u, v, qu, ir, r2c, L, A, cp, sp, inrange;
     float Fr, Fi, aLu, aLv;
     float Etr, Eti, Efr, Efi;
 };
@@ -246,9 +247,9 @@ GLM void epl_forward(const EplC& C, const float* __restrict__ coef, const float4
     float v = fmaf(-dx, C.sphi, dy * C.cphi);
     float qu = C.q * u;
     float r2 = fmaf(qu, qu, v * v);
-    float cp, sp;
+    float cp, sp, ir = 0.0f;
     if (r2 > 0.0f) {
-        float ir = rsq(r2);
+        ir = rsq(r2);
         cp = qu * ir;
         sp = v * ir;
     } else {  // atan2(0, +-0)
@@ -275,7 +276,7 @@ GLM void epl_forward(const EplC& C, const float* __restrict__ coef, const float4
     ax = aLu * C.cphi - aLv * C.sphi;
     ay = aLu * C.sphi + aLv * C.cphi;
     if (ADJ) {
-        S.u = u; S.v = v; S.qu = qu; S.r2 = r2; S.r2c = r2c; S.L = L; S.A = A; S.cp = cp; S.sp = sp;
+        S.u = u; S.v = v; S.qu = qu; S.ir = ir; S.r2c = r2c; S.L = L; S.A = A; S.cp = cp; S.sp = sp;
         S.inrange = (r2 >= 1e-20f && r2 <= 1e20f) ? 1.0f : 0.0f;
         S.Fr = Fr; S.Fi = Fi; S.aLu = aLu; S.aLv = aLv;
     }
@@ -296,7 +297,7 @@ GLM void epl_adjoint(const EplC& C, const EplState& S, float Hx, float Hy, float
     float Yr = fmaf(f2, S.Efr, S.Fr);
     float Yi = fmaf(f2, S.Efi, S.Fi);
     float sphi = S.A * fmaf(-hu, Yi, hv * Yr);
-    float ir2 = S.r2 > 0.0f ? rcp(S.r2) : 0.0f;
+    float ir2 = S.ir * S.ir;  // 1/r^2, 0 at r = 0
     float sR = (1.0f - C.t) * sa * S.inrange;
     float dLdu = (sR * C.q * S.qu - sphi * C.q * S.v) * ir2;
     float dLdv = (sR * S.v + sphi * S.qu) * ir2;
@@ -355,29 +356,39 @@ GLM void sis_finish(const float* acc, float* g) {
     g[2] = -acc[2];
 }
 
-struct SerState { float dx, dy, gx, gy, Q, LQ, P, I; };
+struct SerState { float dx, dy, gx, gy, Pq, LQ, P, I; };
 
+// ADJ: also P/Q for sersic_adjoint, as one exp2 of (k1 - 1) log2 Q + k0 with P = Q (P/Q)
+// (no reciprocal of Q).
+template <bool ADJ = false>
 GLM float sersic_forward(const SerC& C, float X, float Y, SerState& S) {
     float dx = X - C.cx, dy = Y - C.cy;
     float gx = fmaf(C.Axx, dx, C.Axy * dy);
     float gy = fmaf(C.Axy, dx, C.Ayy * dy);
     float Q = fmaf(dx, gx, dy * gy);
     float LQ = lg2(Q);
-    float P = ex2(fmaf(C.k1, LQ, C.k0));
+    float P, Pq = 0.0f;
+    if (ADJ) {
+        Pq = ex2(fmaf(C.k1m1, LQ, C.k0));
+        P = Q > 0.0f ? Q * Pq : 0.0f;
+    } else {
+        P = ex2(fmaf(C.k1, LQ, C.k0));
+    }
     float I = C.Ie * ex2(fmaf(C.m1, P, C.m0));
-    S.dx = dx; S.dy = dy; S.gx = gx; S.gy = gy; S.Q = Q; S.LQ = LQ; S.P = P; S.I = I;
+    S.dx = dx; S.dy = dy; S.gx = gx; S.gy = gy; S.Pq = Pq; S.LQ = LQ; S.P = P; S.I = I;
     return I;
 }
 
 // acc[0..7] = (S_I, S_P, S_PL, S_xx, S_xy, S_yy, S_dx, S_dy); returns dL/d(dx,dy)
 // of this sub-pixel in (gdx, gdy) (needed to chain a source into the lenses).
+// S comes from sersic_forward<true>.
 GLM void sersic_adjoint(const SerC& C, const SerState& S, float g, float* acc, float& gdx, float& gdy) {
     float gI = g * S.I;
     float gP = gI * S.P;
     acc[0] += gI;
     acc[1] += gP;
     acc[2] = fmaf(gP, S.LQ, acc[2]);
-    float w = gP * rcp(S.Q);
+    float w = gI * S.Pq;  // g I P / Q
     acc[3] = fmaf(w * S.dx, S.dx, acc[3]);
     acc[4] = fmaf(w * S.dx, S.dy, acc[4]);
     acc[5] = fmaf(w * S.dy, S.dy, acc[5]);

@@ -40,6 +40,7 @@ struct DPlan {
     const int* batch_n;          // device int per EPL slot when batch_series (else null)
     int sep_rank;                // > 0: separable PSF path with this many rank-1 terms
     int urows;                   // rows of the transposed-horizontal buffer U (separable adjoint)
+    int ucol;                    // column stride of U (column-major, odd >= urows)
     int u_off;                   // LDS offset of U (after the adjoint image G, no overlap)
     int sep_aligned;             // the transposed-horizontal segments of G start 16-byte aligned
     const float* sa;             // sep_rank x kf vertical factors  (W ~= sum_r sa_r sb_r^T)

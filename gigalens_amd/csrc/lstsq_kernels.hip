@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
             if (cd.kind == glm::K_SERSIC || cd.kind == glm::K_SERSIC_ELLIPSE) {
                 const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
                 glm::SerState ss_;
-                const float v = glm::sersic_forward(C, ux, uy, ss_);
+                const float v = glm::sersic_forward<true>(C, ux, uy, ss_);
                 const int c = M.lstsq ? ch++ : 0;
                 if (isnan(v)) continue;  // least squares: this channel was zeroed
                 float gdx, gdy;

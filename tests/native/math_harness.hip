@@ -56,7 +56,7 @@ extern "C" void h_sersic(const float* p, int elliptical, int n, const float* x, 
     float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         SerState S;
-        I[i] = sersic_forward(C, x[i], y[i], S);
+        I[i] = sersic_forward<true>(C, x[i], y[i], S);
         sersic_adjoint(C, S, gup[i], acc, gdx[i], gdy[i]);
     }
     sersic_finish(C, acc, gpar);
