@@ -1036,10 +1036,14 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
 constexpr int kOrderCap = 16384;
 constexpr int kOrderBins = 3 * GL_MAX_EPL_ITER + 1;
 __global__ __launch_bounds__(1024) void order_kernel(const DPlan P, const float* in, int bs, int use_z, int* order) {
+    constexpr int kScanWaves = (kOrderBins + 63) / 64;
     __shared__ unsigned short key[kOrderCap];
     __shared__ int cnt[kOrderBins];
+    __shared__ int wtot[kScanWaves];
     for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) cnt[i] = 0;
-    __syncthreads();
+    // bin of every particle; no atomics in this loop, so the loads of several
+    // particles per thread can be in flight together
+#pragma unroll 4
     for (int p = threadIdx.x; p < bs; p += blockDim.x) {
         int n = 0;
 #pragma unroll
@@ -1067,18 +1071,30 @@ __global__ __launch_bounds__(1024) void order_kernel(const DPlan P, const float*
             if (!(f > 0.0f)) m = 1;
             n += m < P.max_n ? m : P.max_n;
         }
-        const int b = kOrderBins - 1 - min(max(n, 0), kOrderBins - 1);
-        key[p] = (unsigned short)b;
-        atomicAdd(&cnt[b], 1);
+        key[p] = (unsigned short)(kOrderBins - 1 - min(max(n, 0), kOrderBins - 1));
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int i = 0; i < kOrderBins; ++i) {
-            const int c = cnt[i];
-            cnt[i] = s;
-            s += c;
+    for (int p = threadIdx.x; p < bs; p += blockDim.x) atomicAdd(&cnt[key[p]], 1);
+    __syncthreads();
+    // exclusive prefix sum of the bins: an inclusive scan per wave of 64 bins (every
+    // lane of waves 0 .. kScanWaves-1 active), then the totals of the waves before
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int c = 0, incl = 0;
+    if (w < kScanWaves) {
+        c = threadIdx.x < kOrderBins ? cnt[threadIdx.x] : 0;
+        incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
         }
+        if (lane == 63) wtot[w] = incl;
+    }
+    __syncthreads();
+    if (threadIdx.x < kOrderBins) {
+        int base = 0;
+        for (int k = 0; k < w; ++k) base += wtot[k];
+        cnt[threadIdx.x] = base + incl - c;
     }
     __syncthreads();
     for (int p = threadIdx.x; p < bs; p += blockDim.x) order[atomicAdd(&cnt[key[p]], 1)] = p;
