@@ -197,7 +197,9 @@ def run_shapelets(args, dev):
     x0 = pm.bij.forward(z0.cpu())
     n = fl.series_length(x0[0][0]["e1"].numpy(), x0[0][0]["e2"].numpy(), 50)
     ksz = sim.camera.subgrid_kernel.shape[0]
-    c = fl.lstsq_per_particle(phys, cfg.num_pix, cfg.supersample, ksz, n, grad=True)
+    pinfo = plan.info()
+    c = fl.lstsq_per_particle(phys, cfg.num_pix, cfg.supersample, ksz, n, grad=True, sep_rank=pinfo["sep_rank"],
+                              adj_rank=pinfo["adj_rank"])
     fp32 = float(c["flops"].sum())
     achieved = fp32 / (step_ms * 1e-3) / 1e12
     value = bs * K / wall
@@ -221,7 +223,9 @@ def run_shapelets(args, dev):
                    "psf_subgrid": ksz, "params": z0.shape[1], "channels": c["channels"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "whole pipeline (mc_prologue .. mc_finish); mc_conv dominates",
+                     "kernel": "whole pipeline (mc_prologue .. mc_finish); the channel conv dominates ("
+                               + (f"separable rank {pinfo['sep_rank']}: mc_sep_h + mc_sep_v)" if pinfo["sep_rank"]
+                                  else "direct: mc_conv)"),
                      "step_ms": step_ms, "fp32_flop_per_step": fp32,
                      "conv_flop_per_step": c["conv_flops"] * bs,
                      "mfma_flop_per_step": float(c["mfma"].sum()), "fp64_flop_per_step": float(c["fp64"].sum()),

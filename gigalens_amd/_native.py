@@ -106,7 +106,7 @@ class PlanDesc(ctypes.Structure):
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("num_pix", "supersample", "n_params", "kernel_size", "folded_size",
-                                            "sep_rank", "lds_bytes", "block")]
+                                            "sep_rank", "lds_bytes", "block", "adj_rank")]
 
 
 class CnnDesc(ctypes.Structure):

@@ -54,11 +54,17 @@ constexpr int kMcStat = 64;  // statistics floats per particle
 constexpr int kMcMaxRank = 8;  // separable channel convolution: at most this rank
 constexpr int kMcIB = 4;     // default output rows per conv lane (McPlan.conv_ib: 2, 4 or 8)
 
+// Read-only factor tables addressed in the constant space: uniform reads become
+// scalar loads even in kernels that store to global memory in the same loop.
+typedef const float __attribute__((address_space(4))) CFloat;
+
 struct McPlan {
     int lstsq;          // 1: linear amplitudes solved by least squares; 0: fixed amplitudes (one channel)
     int conv_ib;        // output rows per lane of mc_conv (a tuning choice: 2, 4 or 8)
     int sep_rank;       // > 0: separable channel convolution with this rank (W ~= sum_r sa_r sb_r^T)
     const float* sa;    // sep_rank x kf vertical factors
+    const float* sap;   // sep_rank x ss x kfm polyphase vertical factors, sap[(r ss + a) kfm + m] =
+                        // sa_r[ss m + a] (zero where ss m + a >= kf), kfm = (4 kf4 + ss - 1) / ss
     const float* sbp;   // sep_rank x wpad_w horizontal factors, zero padded
     float* H;           // bs x D x sep_rank x nsub x N horizontal-pass intermediate
     int adj_rank;       // > 0: separable adjoint conv with this rank (mc_adj_h, mc_adj_v)
@@ -505,7 +511,7 @@ __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
 template <int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int task = blockIdx.x * 4 + wave;
+    int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
     const DPlan& P = T.one.P;
     const McPlan& M = T.one.M;
     const int N = P.N, nsub = P.nsub, R = M.sep_rank;
@@ -539,7 +545,7 @@ __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
 #pragma unroll
         for (int r = 0; r < kMcMaxRank; ++r) {
             if (r >= R) break;
-            const float* w = M.sbp + r * M.wpad_w;
+            const CFloat* w = (const CFloat*)M.sbp + r * M.wpad_w;
             float h = 0.0f;
 #pragma unroll
             for (int j = 0; j < 4 * KF4; ++j) h = fmaf(w[j], v[j], h);
@@ -548,16 +554,22 @@ __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
     }
 }
 
-// mc_sep_v: one wave per (particle, channel, kSepIB output rows, 64 columns J);
-// each intermediate row read serves up to kSepIB outputs of the lane.
+// mc_sep_v: one wave per (particle, channel, kSepIB output rows, 64 columns J),
+// in polyphase order: tap p = SS m + a of output row I reads H row SS (I + m) + a - pb,
+// so for each phase a the kSepIB outputs of a lane form a 1-D correlation over
+// H rows SS k + a - pb: each row read serves up to kSepIB outputs, and with the
+// loops unrolled every factor index is a compile-time constant (scalar loads of
+// the zero-padded polyphase table).  The padded taps (SS m + a >= kf) carry zero
+// factors and reach at most SS - 1 rows past the kernel window.
 constexpr int kSepIB = 16;
-template <int SS>
+template <int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_sep_v(const McTable T) {
+    constexpr int KFM = (4 * KF4 + SS - 1) / SS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int task = blockIdx.x * 4 + wave;
+    int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);  // rows, bounds tests: scalar
     const DPlan& P = T.one.P;
     const McPlan& M = T.one.M;
-    const int N = P.N, nsub = P.nsub, kf = P.kf, pb = P.pb, R = M.sep_rank;
+    const int N = P.N, nsub = P.nsub, pb = P.pb, R = M.sep_rank;
     const int nib = (N + kSepIB - 1) / kSepIB, njb = (N + 63) / 64;
     if (task >= T.tot_part * M.D * nib * njb) return;
     const int jb = task % njb; task /= njb;
@@ -567,22 +579,25 @@ __global__ __launch_bounds__(256) void mc_sep_v(const McTable T) {
     const int J = jb * 64 + lane;
     if (J >= N) return;
     const int I0 = ib * kSepIB;
-    const float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;
+    const float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;  // uniform; lanes index by J
     float acc[kSepIB];
 #pragma unroll
     for (int i = 0; i < kSepIB; ++i) acc[i] = 0.0f;
-    const int nrows = SS * (kSepIB - 1) + kf;
     for (int r = 0; r < R; ++r) {
         const float* Hr = Hc + (size_t)r * nsub * N;
-        const float* ar = M.sa + r * kf;
-        for (int rr = 0; rr < nrows; ++rr) {
-            const int y = SS * I0 + rr - pb;
-            if (y < 0 || y >= nsub) continue;
-            const float v = Hr[(size_t)y * N + J];
+#pragma unroll 1
+        for (int a = 0; a < SS; ++a) {
+            const CFloat* wa = (const CFloat*)M.sap + (r * SS + a) * KFM;
+            const int y0 = SS * I0 + a - pb;
 #pragma unroll
-            for (int i = 0; i < kSepIB; ++i) {
-                const int pp = rr - SS * i;
-                if (pp >= 0 && pp < kf) acc[i] = fmaf(ar[pp], v, acc[i]);
+            for (int k = 0; k < kSepIB + KFM - 1; ++k) {
+                const int y = y0 + SS * k;
+                const float v = (y >= 0 && y < nsub) ? Hr[(size_t)y * N + J] : 0.0f;
+#pragma unroll
+                for (int i = 0; i < kSepIB; ++i) {
+                    const int m = k - i;
+                    if (m >= 0 && m < KFM) acc[i] = fmaf(wa[m], v, acc[i]);
+                }
             }
         }
     }

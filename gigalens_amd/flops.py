@@ -141,12 +141,16 @@ def shapelet_1d(n_max: int, interpolate: bool, deriv: bool = False):
 
 
 def lstsq_per_particle(phys_model, num_pix: int, ss: int, kernel_size: int, n_terms: np.ndarray,
-                       grad: bool) -> Dict[str, np.ndarray]:
+                       grad: bool, sep_rank: int = 0, adj_rank: int = 0) -> Dict[str, np.ndarray]:
     """Operation counts of the least-squares pipeline for one particle.
     ``flops`` are FP32 vector flops (basis, D-channel conv, image and
     likelihood, adjoint conv, gradient recompute), ``mfma`` the Gram product
     [X | obs]^T [X | obs] (2 flops per pair per pixel, upper triangle), ``fp64``
-    the eigen-solve estimate (8 Jacobi sweeps of 6 (D+1)^3 flops)."""
+    the eigen-solve estimate (8 Jacobi sweeps of 6 (D+1)^3 flops).  The channel
+    conv is counted in the form the plan runs: direct (2 kf^2 per output) or, for
+    ``sep_rank`` R > 0, a horizontal pass over every sub-row and a vertical pass
+    (2 kf R (ss N N + N N)); the adjoint conv likewise direct or, for ``adj_rank``
+    > 0, its two polyphase passes (2 ceil(kf/ss) R ((ss N) N + (ss N)^2))."""
     S = (ss * num_pix) ** 2
     P = num_pix ** 2
     k = kernel_size if kernel_size > 0 else 1
@@ -180,13 +184,22 @@ def lstsq_per_particle(phys_model, num_pix: int, ss: int, kernel_size: int, n_te
         if kind == nat.GL_EPL:
             f += 3 * n_terms
             g += 9 * n_terms
-    flops = S * f + D * P * 2.0 * kf * kf + P * (2.0 * D + 12)
+    nsub = ss * num_pix
+    if sep_rank > 0:
+        conv = D * sep_rank * 2.0 * kf * (nsub * num_pix + P)
+    else:
+        conv = D * P * 2.0 * kf * kf
+    if adj_rank > 0:
+        mt = -(-kf // ss)
+        adj = adj_rank * 2.0 * mt * (nsub * num_pix + S)
+    else:
+        adj = P * 2.0 * kf * kf
+    flops = S * f + conv + P * (2.0 * D + 12)
     trans = S * tr + P
     mfma = P * (D + 1.0) * (D + 2.0) + P * (D + 1.0)
     fp64 = 8 * 6.0 * (D + 1) ** 3
     if grad:
-        flops = flops + P * 2.0 * kf * kf + S * g + P * 15
+        flops = flops + adj + S * g + P * 15
         trans = trans + S * gt
     return {"flops": flops, "transcendentals": trans, "mfma": np.full_like(n_terms, mfma),
-            "fp64": np.full_like(n_terms, fp64), "channels": D,
-            "conv_flops": float(D * P * 2.0 * kf * kf)}
+            "fp64": np.full_like(n_terms, fp64), "channels": D, "conv_flops": float(conv)}

@@ -146,14 +146,15 @@ def test_backward_map_fused_matches_autograd(cuda):
 
 
 def test_separable_channel_conv_matches_direct(cuda):
-    """The staged plan's separable channel convolution (explicit psf_rank_tol)
-    against its direct one at the full shapelet size (83x83 Moffat, rank 7 at 1e-6)."""
+    """The staged plan's separable channel convolution (the least-squares default)
+    against its direct one (psf_rank_tol = -1) at the full shapelet size (83x83
+    Moffat, rank 7 at 1e-6)."""
     from gigalens_amd.plan import Plan
     P = setup(100, 4, 0.04, 21, 10, True, bs=2, seed=3)
     sim = P["sim"]
     x = P["x"].to(cuda).contiguous()
-    direct = Plan(P["phys"], sim.camera, sim.layout, observed=P["obs"], noise_map=P["err"])
-    sep = Plan(P["phys"], sim.camera, sim.layout, observed=P["obs"], noise_map=P["err"], psf_rank_tol=1e-6)
+    direct = Plan(P["phys"], sim.camera, sim.layout, observed=P["obs"], noise_map=P["err"], psf_rank_tol=-1.0)
+    sep = Plan(P["phys"], sim.camera, sim.layout, observed=P["obs"], noise_map=P["err"])
     assert direct.info()["sep_rank"] == 0 and sep.info()["sep_rank"] == 7
     _, _, st_d = direct.lstsq_simulate(x, want_image=False, want_stacked=True)
     _, _, st_s = sep.lstsq_simulate(x, want_image=False, want_stacked=True)
