@@ -83,6 +83,10 @@ class _ProbModelBase(ProbabilisticModel):
             kw = dict(prior=self.prior, observed=self.observed_image.cpu().numpy(),
                       background_rms=self.background_rms, exp_time=self.exp_time, likelihood=self._likelihood,
                       fldj_at_x=self._fldj_at_x, epl_batch_series=epl_batch_series)
+            # staged plans size their workspace for the simulator's batch (one chunk per call)
+            sbs = int(getattr(simulator, "bs", 0) or 0)
+            if 0 < sbs <= 1024:
+                kw["max_batch"] = sbs
             variant = {}
             if self.autotune or os.environ.get("GIGALENS_AMD_AUTOTUNE") == "1":
                 from .autotune import tune
