@@ -220,6 +220,21 @@ static hipError_t mc_conv_dispatch(int ss, int kf4, int ib, int grid, hipStream_
     return hipErrorInvalidValue;
 }
 
+template <bool MULTI>
+static hipError_t mc_adjv_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::McTable& T) {
+#define GL_MCA(S, K)                                                                              \
+    if (ss == S && kf4 == K) {                                                                    \
+        hipLaunchKernelGGL((gl::mc_adj_v<MULTI, S, K>), dim3(grid), dim3(256), 0, st, T);        \
+        return hipGetLastError();                                                                 \
+    }
+    GL_MCA(1, 8); GL_MCA(1, 16); GL_MCA(1, 22); GL_MCA(1, 32);
+    GL_MCA(2, 8); GL_MCA(2, 16); GL_MCA(2, 22); GL_MCA(2, 32);
+    GL_MCA(3, 8); GL_MCA(3, 16); GL_MCA(3, 22); GL_MCA(3, 32);
+    GL_MCA(4, 8); GL_MCA(4, 16); GL_MCA(4, 22); GL_MCA(4, 32);
+#undef GL_MCA
+    return hipErrorInvalidValue;
+}
+
 template <int SS, int KF4>
 static hipError_t launch_mc_sep(int gh, int gv, hipStream_t st, const gl::McTable& T) {
     hipLaunchKernelGGL((gl::mc_sep_h<SS, KF4>), dim3(gh), dim3(256), 0, st, T);
@@ -496,6 +511,17 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     if (M.adj_rank > 0) {
         if ((rc = upload(p, asa.data(), asa.size() * sizeof(float), &dptr))) return rc;
         M.asa = (const float*)dptr;
+        // polyphase vertical factors for mc_adj_v: asap[(r ss + a) kfm + m] = asa_r[ss m + ra], m < Ma
+        const int kfm = (4 * p->kf4 + ss - 1) / ss;
+        std::vector<float> asap((size_t)M.adj_rank * ss * kfm, 0.0f);
+        for (int r = 0; r < M.adj_rank; ++r)
+            for (int a = 0; a < ss; ++a) {
+                const int ra = (a + pb) % ss, Ma = (kf - ra + ss - 1) / ss;
+                for (int m = 0; m < Ma && m < kfm; ++m)
+                    asap[((size_t)r * ss + a) * kfm + m] = asa[(size_t)r * kf + ss * m + ra];
+            }
+        if ((rc = upload(p, asap.data(), asap.size() * sizeof(float), &dptr))) return rc;
+        M.asap = (const float*)dptr;
         if ((rc = upload(p, asb.data(), asb.size() * sizeof(float), &dptr))) return rc;
         M.asb = (const float*)dptr;
     }
@@ -555,6 +581,8 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
 struct ConvGroup {
     int ss, kf4, tot;
     const int* pre;
+    int adjv_tot;         // separable-adjoint vertical-pass units of the group's lenses
+    const int* adjv_pre;
 };
 
 // model 1: every lens is a fixed-amplitude plan of the reference default model
@@ -604,8 +632,19 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
         if (T.tot_adjh > 0) {
             hipLaunchKernelGGL((gl::mc_adj_h<MULTI>), dim3(T.tot_adjh / 4), dim3(256), T.lds_w, st, T);
             GL_HIP(hipGetLastError());
-            hipLaunchKernelGGL((gl::mc_adj_v<MULTI>), dim3(T.tot_adjv / 4), dim3(256), 0, st, T);
-            GL_HIP(hipGetLastError());
+            // vertical pass: compile-time (ss, kf4), one launch per group of lenses
+            if (MULTI) {
+                if (!groups) return fail(GL_ERR_ARG, "lens batch without launch groups");
+                for (const ConvGroup& g : *groups) {
+                    if (g.adjv_tot <= 0) continue;
+                    gl::McTable Tg = T;
+                    Tg.pre_adjv = g.adjv_pre;
+                    Tg.tot_adjv = g.adjv_tot;
+                    GL_HIP(mc_adjv_dispatch<MULTI>(g.ss, g.kf4, g.adjv_tot / 4, st, Tg));
+                }
+            } else {
+                GL_HIP(mc_adjv_dispatch<MULTI>(ss_launch, kf4, T.tot_adjv / 4, st, T));
+            }
         }
         if (T.tot_adj > 0)
             hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), T.lds_w, st, T);
@@ -1311,6 +1350,14 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         }
         g.tot = (int)a;
         g.pre = (const int*)put(pre.data(), pre.size() * sizeof(int));
+        std::vector<int> prev(n + 1, 0);
+        long av = 0;
+        for (int j = 0; j < n; ++j) {
+            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) av += mc_units(lens[j].P, lens[j].M, counts[j]).adjv;
+            prev[j + 1] = (int)av;
+        }
+        g.adjv_tot = (int)av;
+        g.adjv_pre = (const int*)put(prev.data(), prev.size() * sizeof(int));
         b->groups.push_back(g);
     }
     if (rc) { gl_batch_destroy(b); return rc; }

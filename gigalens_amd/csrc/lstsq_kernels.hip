@@ -69,6 +69,8 @@ struct McPlan {
     float* H;           // bs x D x sep_rank x nsub x N horizontal-pass intermediate
     int adj_rank;       // > 0: separable adjoint conv with this rank (mc_adj_h, mc_adj_v)
     const float* asa;   // adj_rank x kf vertical factors of the folded PSF
+    const float* asap;  // adj_rank x ss x kfm polyphase vertical factors, asap[(r ss + a) kfm + m] =
+                        // asa_r[ss m + (a + pb) % ss] for m < Ma(a), else 0; kfm = (4 kf4 + ss - 1) / ss
     const float* asb;   // adj_rank x kf horizontal factors
     float* U;           // bs x adj_rank x N x ss x N transposed horizontal pass, phase-major rows
     int D;              // channels
@@ -951,15 +953,20 @@ __global__ __launch_bounds__(256) void mc_adjoint(const McTable T) {
 // one wave per (particle, row phase a, kAdjKY sub-rows y = ss (Yc0 + k) + a,
 // column phase b, 64 columns Xc); lane = sub-pixel column x = ss Xc + b:
 //   dL/dsub[y][x] = sum_r sum_{m < Ma} asa_r[ss m + ra] U_r[Yc + da - m][b][Xc].
-// The lane walks the Ma + kAdjKY - 1 rows of U its sub-rows need once and adds
-// each into every sub-row it reaches; the factors are scalar loads.
-template <bool MULTI>
+// The lane walks the KFM + kAdjKY - 1 rows of U its sub-rows need once and adds
+// each into every sub-row it reaches.  SS and KF4 are the launch group's: with
+// the loops unrolled, the tap of every (row, sub-row) pair is a compile-time
+// index into the zero-padded polyphase table (scalar loads, no per-tap test);
+// the padded taps (m >= Ma) carry zero factors.
+template <bool MULTI, int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_adj_v(const McTable T) {
+    constexpr int KFM = (4 * KF4 + SS - 1) / SS;
     int local;
     const McLens& E = mc_locate<MULTI>(T, T.pre_adjv, blockIdx.x * 4, local);
     const DPlan& P = E.P;
     const McPlan& M = E.M;
-    const int N = P.N, nsub = P.nsub, ss = P.ss, kf = P.kf, pb = P.pb, R = M.adj_rank;
+    constexpr int ss = SS;
+    const int N = P.N, nsub = P.nsub, pb = P.pb, R = M.adj_rank;
     local += __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int njb = (N + 63) / 64, nyb = (N + kAdjKY - 1) / kAdjKY, wpp4 = mc_adjv_wpp4(P);
     const int p = local / wpp4;
@@ -971,24 +978,27 @@ __global__ __launch_bounds__(256) void mc_adj_v(const McTable T) {
     const int a = t;
     const int Xc = jb * 64 + (threadIdx.x & 63);
     if (Xc >= N) return;
-    const int ra = (a + pb) % ss, da = (a + pb) / ss, Ma = (kf - ra + ss - 1) / ss;
+    const int da = (a + pb) / ss;
     const int Yc0 = yb * kAdjKY;
     float g[kAdjKY];
 #pragma unroll
     for (int k = 0; k < kAdjKY; ++k) g[k] = 0.0f;
-    const float* Ub = M.U + (size_t)p * R * N * ss * N + (size_t)b * N + Xc;
-    // rows I = Yc0 + da - (Ma - 1) .. Yc0 + kAdjKY - 1 + da, inside the image
-    const int Ilo = max(Yc0 + da - (Ma - 1), 0), Ihi = min(Yc0 + kAdjKY - 1 + da, N - 1);
+    // uniform base; lanes index by Xc
+    const float* Ub = M.U + (size_t)p * R * N * ss * N + (size_t)b * N;
+    // rows I = I0 + j, j < KFM + kAdjKY - 1; row j reaches sub-row k with tap m = KFM - 1 - j + k
+    const int I0 = Yc0 + da - (KFM - 1);
+    const long rstride = (long)ss * N;  // floats from one U row to the next
     for (int r = 0; r < R; ++r) {
-        const float* Ur = Ub + (size_t)r * N * ss * N;
-        const float* ar = M.asa + r * kf + ra;
-        for (int I = Ilo; I <= Ihi; ++I) {
-            const float u = Ur[(size_t)I * ss * N];
-            const int m0 = Yc0 + da - I;  // tap of sub-row k is m0 + k
+        const CFloat* wa = (const CFloat*)M.asap + (r * ss + a) * KFM;
+        const float* row = Ub + (size_t)r * N * ss * N + I0 * rstride;  // row I0 (may lie outside the image)
+#pragma unroll
+        for (int j = 0; j < KFM + kAdjKY - 1; ++j, row += rstride) {
+            const int I = I0 + j;
+            const float u = (I >= 0 && I < N) ? row[Xc] : 0.0f;
 #pragma unroll
             for (int k = 0; k < kAdjKY; ++k) {
-                const int m = m0 + k;
-                if (m >= 0 && m < Ma) g[k] = fmaf(ar[ss * m], u, g[k]);
+                const int m = KFM - 1 - j + k;
+                if (m >= 0 && m < KFM) g[k] = fmaf(wa[m], u, g[k]);
             }
         }
     }
