@@ -235,17 +235,19 @@ static hipError_t mc_adjv_dispatch(int ss, int kf4, int grid, hipStream_t st, co
     return hipErrorInvalidValue;
 }
 
-template <int SS, int KF4>
-static hipError_t launch_mc_sep(int gh, int gv, hipStream_t st, const gl::McTable& T) {
-    hipLaunchKernelGGL((gl::mc_sep_h<SS, KF4>), dim3(gh), dim3(256), 0, st, T);
+// separable channel conv: T.tot_seph / T.tot_sepv waves (one lens, or one launch group of a batch)
+template <bool MULTI, int SS, int KF4>
+static hipError_t launch_mc_sep(hipStream_t st, const gl::McTable& T) {
+    hipLaunchKernelGGL((gl::mc_sep_h<MULTI, SS, KF4>), dim3((T.tot_seph + 3) / 4), dim3(256), 0, st, T);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((gl::mc_sep_v<SS, KF4>), dim3(gv), dim3(256), 0, st, T);
+    hipLaunchKernelGGL((gl::mc_sep_v<MULTI, SS, KF4>), dim3((T.tot_sepv + 3) / 4), dim3(256), 0, st, T);
     return hipGetLastError();
 }
 
-static hipError_t mc_sep_dispatch(int ss, int kf4, int gh, int gv, hipStream_t st, const gl::McTable& T) {
-#define GL_MCS(S, K) if (ss == S && kf4 == K) return launch_mc_sep<S, K>(gh, gv, st, T)
+template <bool MULTI>
+static hipError_t mc_sep_dispatch(int ss, int kf4, hipStream_t st, const gl::McTable& T) {
+#define GL_MCS(S, K) if (ss == S && kf4 == K) return launch_mc_sep<MULTI, S, K>(st, T)
     GL_MCS(1, 8); GL_MCS(1, 16); GL_MCS(1, 22); GL_MCS(1, 32);
     GL_MCS(2, 8); GL_MCS(2, 16); GL_MCS(2, 22); GL_MCS(2, 32);
     GL_MCS(3, 8); GL_MCS(3, 16); GL_MCS(3, 22); GL_MCS(3, 32);
@@ -342,6 +344,11 @@ static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int 
         if ((rc = dev_alloc(owned, (size_t)mb * M.adj_rank * N * ss * N * 4, &dptr, false))) return rc;
         M.U = (float*)dptr;
     }
+    M.H = nullptr;
+    if (M.sep_rank > 0) {  // separable channel conv: horizontal-pass intermediate
+        if ((rc = dev_alloc(owned, (size_t)mb * D * M.sep_rank * nsub * N * 4, &dptr, false))) return rc;
+        M.H = (float*)dptr;
+    }
     return GL_OK;
 }
 
@@ -412,12 +419,11 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     const int gram_nt = M.gram_nb * (M.gram_nb + 1) / 2;
     const int nchunk = (N * N + 63) / 64;
     p->ksplit_cap = std::max(1, std::min(gl::kMcGramMaxSplit, nchunk / 4));
-    // separable channel convolution (mc_sep_h + mc_sep_v): when the folded PSF has
-    // rank <= 8 at the tolerance and the flop count is lower.  Default for
-    // least-squares plans (tolerance 1e-6; 100x100 ss 4 with the 83x83 Moffat, rank 7:
-    // 52 ms against 104 ms of direct conv per 512 particles); fixed-amplitude plans
-    // use it on request (psf_rank_tol > 0), as they may run in lens batches, which
-    // use the direct conv and would not use its intermediate buffer.
+    // separable channel convolution (mc_sep_h + mc_sep_v, single plans and lens
+    // batches): when the folded PSF has rank <= 8 at the tolerance (0 selects 1e-6;
+    // psf_rank_tol < 0 keeps the direct conv) and the flop count is lower.
+    // Measured: 100x100 ss 4 with the 83x83 Moffat (rank 7), 52 ms against 104 ms of
+    // direct conv per 512 particles.
     M.sep_rank = 0;
     std::vector<float> sa, sbp;
     std::vector<double> Wd(kf * kf), sv, Uv, Vv;
@@ -447,7 +453,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
                 }
         }
     }
-    if (desc->psf_rank_tol > 0.0f || (desc->psf_rank_tol == 0.0f && lstsq)) {
+    if (desc->psf_rank_tol >= 0.0f) {
         const double tol = desc->psf_rank_tol > 0.0f ? desc->psf_rank_tol : 1e-6;
         int R = 0;
         while (R < kf && sv[R] >= tol * sv[0]) ++R;
@@ -539,8 +545,6 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
         M.sap = (const float*)dptr;
         if ((rc = upload(p, sbp.data(), sbp.size() * sizeof(float), &dptr))) return rc;
         M.sbp = (const float*)dptr;
-        if ((rc = dev_alloc(p->dev, (size_t)mb * D * M.sep_rank * nsub * N * 4, &dptr, false))) return rc;
-        M.H = (float*)dptr;
     }
     const int n2 = D + (D & 1);
     // A, V (n2 x n2), rotations (n2), pairs (n2 ints), 2 reduction slots, rhs (n2)
@@ -557,7 +561,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
 
 // Work-unit totals of a lens with `count` particles (the prefix increments of a table).
 struct McUnits {
-    long part, basis, conv, adj, grad, adjh, adjv;
+    long part, basis, conv, adj, grad, adjh, adjv, seph, sepv;
 };
 static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     McUnits u;
@@ -565,7 +569,10 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     u.part = count;
     u.basis = (long)count * ((P.nsub * P.nsub + 255) / 256);
     u.grad = (long)count * (((long)P.nsub * P.nsub + 256 * gl::kMcGradK - 1) / (256 * gl::kMcGradK));
-    u.conv = (long)count * M.D * ((P.N + M.conv_ib - 1) / M.conv_ib) * njb;
+    // channel conv: direct (mc_conv) or separable (mc_sep_h + mc_sep_v)
+    u.conv = M.sep_rank > 0 ? 0 : (long)count * M.D * ((P.N + M.conv_ib - 1) / M.conv_ib) * njb;
+    u.seph = M.sep_rank > 0 ? (long)count * M.D * ((P.nsub + 3) / 4) * njb : 0;
+    u.sepv = M.sep_rank > 0 ? (long)count * M.D * ((P.N + gl::kSepIB - 1) / gl::kSepIB) * njb : 0;
     // adjoint conv: direct (mc_adjoint) or separable (mc_adj_h + mc_adj_v)
     const long nyb = (P.N + gl::kAdjKY - 1) / gl::kAdjKY;
     u.adj = M.adj_rank > 0 ? 0 : (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
@@ -583,6 +590,9 @@ struct ConvGroup {
     const int* pre;
     int adjv_tot;         // separable-adjoint vertical-pass units of the group's lenses
     const int* adjv_pre;
+    int seph_tot, sepv_tot;  // separable channel conv units of the group's separable lenses
+    const int* seph_pre;
+    const int* sepv_pre;
 };
 
 // model 1: every lens is a fixed-amplitude plan of the reference default model
@@ -602,19 +612,22 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
         hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
     GL_HIP(hipGetLastError());
     if (!MULTI && T.one.M.sep_rank > 0) {
-        const gl::DPlan& P = T.one.P;
-        const long per = (long)np * T.one.M.D * ((P.N + 63) / 64);
-        const long gh = (per * ((P.nsub + 3) / 4) + 3) / 4;
-        const long gv = (per * ((P.N + gl::kSepIB - 1) / gl::kSepIB) + 3) / 4;
-        if (gh > INT32_MAX) return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
-        GL_HIP(mc_sep_dispatch(ss_launch, kf4, (int)gh, (int)gv, st, T));
+        GL_HIP(mc_sep_dispatch<MULTI>(ss_launch, kf4, st, T));
     } else if (MULTI && groups && !groups->empty()) {
         for (const ConvGroup& g : *groups) {
-            if (g.tot <= 0) continue;
             gl::McTable Tg = T;
-            Tg.pre_conv = g.pre;
-            Tg.tot_conv = g.tot;
-            GL_HIP(mc_conv_dispatch<MULTI>(g.ss, g.kf4, gl::kMcIB, (g.tot + 3) / 4, st, Tg));
+            if (g.tot > 0) {
+                Tg.pre_conv = g.pre;
+                Tg.tot_conv = g.tot;
+                GL_HIP(mc_conv_dispatch<MULTI>(g.ss, g.kf4, gl::kMcIB, (g.tot + 3) / 4, st, Tg));
+            }
+            if (g.seph_tot > 0) {
+                Tg.pre_seph = g.seph_pre;
+                Tg.tot_seph = g.seph_tot;
+                Tg.pre_sepv = g.sepv_pre;
+                Tg.tot_sepv = g.sepv_tot;
+                GL_HIP(mc_sep_dispatch<MULTI>(g.ss, g.kf4, st, Tg));
+            }
         }
     } else {
         GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, MULTI ? 4 : T.one.M.conv_ib, (T.tot_conv + 3) / 4, st, T));
@@ -696,12 +709,14 @@ static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStrea
         T.n = 1;
         const McUnits u = mc_units(P, M, nb);
         if (u.basis > INT32_MAX || u.conv > INT32_MAX || u.adj > INT32_MAX || u.adjh > INT32_MAX ||
-            u.adjv > INT32_MAX)
+            u.adjv > INT32_MAX || u.seph > INT32_MAX || u.sepv > INT32_MAX)
             return fail(GL_ERR_UNSUPPORTED, "staged launch too large; lower max_batch");
         T.tot_part = nb; T.tot_basis = (int)u.basis; T.tot_conv = (int)u.conv; T.tot_adj = (int)u.adj;
         T.tot_grad = (int)u.grad;
         T.tot_adjh = (int)u.adjh;
         T.tot_adjv = (int)u.adjv;
+        T.tot_seph = (int)u.seph;
+        T.tot_sepv = (int)u.sepv;
         T.lds_consts = M.pstride * 4;
         T.lds_w = P.kf * P.kf * 4;
         int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
@@ -1285,14 +1300,14 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     std::vector<gl::McLens> lens(n);
     std::vector<int> pre_part(n + 1, 0), pre_basis(n + 1, 0), pre_conv(n + 1, 0), pre_adj(n + 1, 0),
         pre_grad(n + 1, 0), pre_adjh(n + 1, 0), pre_adjv(n + 1, 0);
-    long acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    long acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         gl::McLens& E = lens[i];
         memset(&E, 0, sizeof(E));
         E.P = plans[i]->P;
         E.M = plans[i]->M;
         E.M.conv_ib = gl::kMcIB;  // batches launch the direct conv at the default rows per lane
-        E.M.sep_rank = 0;
+        // a separable lens keeps its rank: the workspace below holds its own H
         int rc = mc_workspace(E.P, E.M, counts[i], plans[i]->kf4, 1, plans[i]->W_host, b->dev);
         if (rc) { gl_batch_destroy(b); return rc; }
         E.first = (int)acc[0];
@@ -1302,7 +1317,9 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         acc[0] += u.part; acc[1] += u.basis; acc[2] += u.conv; acc[3] += u.adj; acc[4] += u.grad;
         acc[5] += u.adjh;
         acc[6] += u.adjv;
-        for (int k = 0; k < 7; ++k)
+        acc[7] += u.seph;
+        acc[8] += u.sepv;
+        for (int k = 0; k < 9; ++k)
             if (acc[k] > INT32_MAX) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: batch too large"); }
         pre_part[i + 1] = (int)acc[0];
         pre_basis[i + 1] = (int)acc[1];
@@ -1358,6 +1375,21 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         }
         g.adjv_tot = (int)av;
         g.adjv_pre = (const int*)put(prev.data(), prev.size() * sizeof(int));
+        std::vector<int> preh(n + 1, 0), presv(n + 1, 0);
+        long ah = 0, asv = 0;
+        for (int j = 0; j < n; ++j) {
+            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) {
+                const McUnits uj = mc_units(lens[j].P, lens[j].M, counts[j]);
+                ah += uj.seph;
+                asv += uj.sepv;
+            }
+            preh[j + 1] = (int)ah;
+            presv[j + 1] = (int)asv;
+        }
+        g.seph_tot = (int)ah;
+        g.sepv_tot = (int)asv;
+        g.seph_pre = (const int*)put(preh.data(), preh.size() * sizeof(int));
+        g.sepv_pre = (const int*)put(presv.data(), presv.size() * sizeof(int));
         b->groups.push_back(g);
     }
     if (rc) { gl_batch_destroy(b); return rc; }
@@ -1369,6 +1401,8 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
     b->T.tot_grad = (int)acc[4];
     b->T.tot_adjh = (int)acc[5];
     b->T.tot_adjv = (int)acc[6];
+    b->T.tot_seph = (int)acc[7];
+    b->T.tot_sepv = (int)acc[8];
     b->T.lds_consts = 0;
     b->T.lds_w = 0;
     for (int i = 0; i < n; ++i) {

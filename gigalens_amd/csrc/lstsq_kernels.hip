@@ -146,7 +146,9 @@ struct McTable {
     const int* pre_grad;   // mc_grad blocks (256 x kMcGradK sub-pixels each)
     const int* pre_adjh;   // separable adjoint horizontal-pass waves (a multiple of 4 per particle; 0 without)
     const int* pre_adjv;   // separable adjoint vertical-pass waves (a multiple of 4 per particle; 0 without)
-    int tot_part, tot_basis, tot_conv, tot_adj, tot_grad, tot_adjh, tot_adjv;
+    const int* pre_seph;   // separable channel conv: horizontal-pass waves (0 for direct-conv lenses)
+    const int* pre_sepv;   // separable channel conv: vertical-pass waves
+    int tot_part, tot_basis, tot_conv, tot_adj, tot_grad, tot_adjh, tot_adjv, tot_seph, tot_sepv;
     int lds_consts;        // bytes of LDS for one particle's constant block (max over lenses)
     int lds_w;             // bytes of LDS for the folded weights kf x kf (max over lenses)
 };
@@ -508,17 +510,20 @@ __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
 //   H_r[y][J] = sum_q sb_r[q] B[y][ss J + q - pb],
 // and a vertical pass over the output rows,
 //   X[I][J] = sum_r sum_p sa_r[p] H_r[ss I + p - pb][J].
-// mc_sep_h: one wave per (particle, channel, 4 sub-rows, 64 columns J).
+// mc_sep_h: one wave per (particle, channel, 4 sub-rows, 64 columns J).  A batch
+// launches both passes once per (ss, kf4) group of its separable lenses.
 // ---------------------------------------------------------------------------
-template <int SS, int KF4>
+template <bool MULTI, int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
-    const DPlan& P = T.one.P;
-    const McPlan& M = T.one.M;
+    const int gtask = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+    if (gtask >= T.tot_seph) return;
+    int task;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_seph, gtask, task);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, R = M.sep_rank;
     const int nyb = (nsub + 3) / 4, njb = (N + 63) / 64;
-    if (task >= T.tot_part * M.D * nyb * njb) return;
     const int jb = task % njb; task /= njb;
     const int yb = task % nyb; task /= nyb;
     const int c = task % M.D;
@@ -564,16 +569,18 @@ __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
 // the zero-padded polyphase table).  The padded taps (SS m + a >= kf) carry zero
 // factors and reach at most SS - 1 rows past the kernel window.
 constexpr int kSepIB = 16;
-template <int SS, int KF4>
+template <bool MULTI, int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_sep_v(const McTable T) {
     constexpr int KFM = (4 * KF4 + SS - 1) / SS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int task = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);  // rows, bounds tests: scalar
-    const DPlan& P = T.one.P;
-    const McPlan& M = T.one.M;
+    const int gtask = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);  // rows, bounds tests: scalar
+    if (gtask >= T.tot_sepv) return;
+    int task;
+    const McLens& E = mc_locate<MULTI>(T, T.pre_sepv, gtask, task);
+    const DPlan& P = E.P;
+    const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, pb = P.pb, R = M.sep_rank;
     const int nib = (N + kSepIB - 1) / kSepIB, njb = (N + 63) / 64;
-    if (task >= T.tot_part * M.D * nib * njb) return;
     const int jb = task % njb; task /= njb;
     const int ib = task % nib; task /= nib;
     const int c = task % M.D;

@@ -103,9 +103,8 @@ typedef struct gl_plan_desc {
     int epl_batch_series;     /* 1: reference's batch-coupled EPL series length (epl.py:41) */
     float psf_rank_tol;       /* separable PSF: keep singular values >= tol * s_max of the folded
                                  PSF; 0 selects 1e-6, a negative value forces the direct 2-D path.
-                                 Staged least-squares plans use their separable channel
-                                 convolution by default (0 selects 1e-6); staged fixed-amplitude
-                                 plans only for an explicit tol > 0 */
+                                 Staged plans (and lens batches of them) use the same rule for
+                                 their channel convolution and adjoint convolution */
     int max_batch;            /* least-squares plans: particles per internal chunk (workspace is
                                  allocated for this many at plan creation); 0 selects 64 */
     const float* noise_map;   /* HOST N*N sigma per pixel; NULL: sqrt(bkg^2 + clip(obs,0)/t_exp) */
