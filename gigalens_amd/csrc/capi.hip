@@ -221,6 +221,21 @@ static hipError_t mc_conv_dispatch(int ss, int kf4, int ib, int grid, hipStream_
 }
 
 template <bool MULTI>
+static hipError_t mc_adjh_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::McTable& T) {
+#define GL_MCH(S, K)                                                                              \
+    if (ss == S && kf4 == K) {                                                                    \
+        hipLaunchKernelGGL((gl::mc_adj_h<MULTI, S, K>), dim3(grid), dim3(256), 0, st, T);        \
+        return hipGetLastError();                                                                 \
+    }
+    GL_MCH(1, 8); GL_MCH(1, 16); GL_MCH(1, 22); GL_MCH(1, 32);
+    GL_MCH(2, 8); GL_MCH(2, 16); GL_MCH(2, 22); GL_MCH(2, 32);
+    GL_MCH(3, 8); GL_MCH(3, 16); GL_MCH(3, 22); GL_MCH(3, 32);
+    GL_MCH(4, 8); GL_MCH(4, 16); GL_MCH(4, 22); GL_MCH(4, 32);
+#undef GL_MCH
+    return hipErrorInvalidValue;
+}
+
+template <bool MULTI>
 static hipError_t mc_adjv_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::McTable& T) {
 #define GL_MCA(S, K)                                                                              \
     if (ss == S && kf4 == K) {                                                                    \
@@ -412,6 +427,9 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
         if (Ma - 1 - da > gp) gp = Ma - 1 - da;
         if (da > gp) gp = da;
     }
+    // mc_adj_h loads a window of kfm + 1 columns per lane: margin >= kfm + 1
+    const int kfm_pad = (4 * p->kf4 + ss - 1) / ss;
+    if (gp < kfm_pad + 1) gp = kfm_pad + 1;
     M.gp = gp;
     const int GW = N + 2 * gp;
     // Gram tiles and pixel-range splits
@@ -528,6 +546,20 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
             }
         if ((rc = upload(p, asap.data(), asap.size() * sizeof(float), &dptr))) return rc;
         M.asap = (const float*)dptr;
+        // horizontal factors at mc_adj_h's window positions: phase b's tap nn sits at
+        // j = kfm - 1 + (db(b) - db(0)) - nn of the lane's kfm + 1 loaded columns
+        std::vector<float> asbq((size_t)M.adj_rank * ss * (kfm + 1), 0.0f);
+        const int db0 = pb / ss;
+        for (int r = 0; r < M.adj_rank; ++r)
+            for (int b2 = 0; b2 < ss; ++b2) {
+                const int rb = (b2 + pb) % ss, db = (b2 + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+                for (int nn = 0; nn < Mb; ++nn) {
+                    const int j = kfm - 1 + (db - db0) - nn;
+                    if (j >= 0 && j <= kfm) asbq[((size_t)r * ss + b2) * (kfm + 1) + j] = asb[(size_t)r * kf + ss * nn + rb];
+                }
+            }
+        if ((rc = upload(p, asbq.data(), asbq.size() * sizeof(float), &dptr))) return rc;
+        M.asbq = (const float*)dptr;
         if ((rc = upload(p, asb.data(), asb.size() * sizeof(float), &dptr))) return rc;
         M.asb = (const float*)dptr;
     }
@@ -576,7 +608,7 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     // adjoint conv: direct (mc_adjoint) or separable (mc_adj_h + mc_adj_v)
     const long nyb = (P.N + gl::kAdjKY - 1) / gl::kAdjKY;
     u.adj = M.adj_rank > 0 ? 0 : (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
-    u.adjh = M.adj_rank > 0 ? (long)count * (((long)P.N * P.ss * njb + 3) & ~3L) : 0;
+    u.adjh = M.adj_rank > 0 ? (long)count * (((long)P.N * njb + 3) & ~3L) : 0;
     u.adjv = M.adj_rank > 0 ? (long)count * (((long)P.ss * nyb * P.ss * njb + 3) & ~3L) : 0;
     return u;
 }
@@ -590,6 +622,8 @@ struct ConvGroup {
     const int* pre;
     int adjv_tot;         // separable-adjoint vertical-pass units of the group's lenses
     const int* adjv_pre;
+    int adjh_tot;         // separable-adjoint horizontal-pass units of the group's lenses
+    const int* adjh_pre;
     int seph_tot, sepv_tot;  // separable channel conv units of the group's separable lenses
     const int* seph_pre;
     const int* sepv_pre;
@@ -643,19 +677,21 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
     GL_HIP(hipGetLastError());
     if (grad) {
         if (T.tot_adjh > 0) {
-            hipLaunchKernelGGL((gl::mc_adj_h<MULTI>), dim3(T.tot_adjh / 4), dim3(256), T.lds_w, st, T);
-            GL_HIP(hipGetLastError());
-            // vertical pass: compile-time (ss, kf4), one launch per group of lenses
+            // both passes: compile-time (ss, kf4), one launch each per group of lenses
             if (MULTI) {
                 if (!groups) return fail(GL_ERR_ARG, "lens batch without launch groups");
                 for (const ConvGroup& g : *groups) {
                     if (g.adjv_tot <= 0) continue;
                     gl::McTable Tg = T;
+                    Tg.pre_adjh = g.adjh_pre;
+                    Tg.tot_adjh = g.adjh_tot;
                     Tg.pre_adjv = g.adjv_pre;
                     Tg.tot_adjv = g.adjv_tot;
+                    GL_HIP(mc_adjh_dispatch<MULTI>(g.ss, g.kf4, g.adjh_tot / 4, st, Tg));
                     GL_HIP(mc_adjv_dispatch<MULTI>(g.ss, g.kf4, g.adjv_tot / 4, st, Tg));
                 }
             } else {
+                GL_HIP(mc_adjh_dispatch<MULTI>(ss_launch, kf4, T.tot_adjh / 4, st, T));
                 GL_HIP(mc_adjv_dispatch<MULTI>(ss_launch, kf4, T.tot_adjv / 4, st, T));
             }
         }
@@ -1375,6 +1411,14 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         }
         g.adjv_tot = (int)av;
         g.adjv_pre = (const int*)put(prev.data(), prev.size() * sizeof(int));
+        std::vector<int> preah(n + 1, 0);
+        long aah = 0;
+        for (int j = 0; j < n; ++j) {
+            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) aah += mc_units(lens[j].P, lens[j].M, counts[j]).adjh;
+            preah[j + 1] = (int)aah;
+        }
+        g.adjh_tot = (int)aah;
+        g.adjh_pre = (const int*)put(preah.data(), preah.size() * sizeof(int));
         std::vector<int> preh(n + 1, 0), presv(n + 1, 0);
         long ah = 0, asv = 0;
         for (int j = 0; j < n; ++j) {

@@ -72,6 +72,8 @@ struct McPlan {
     const float* asap;  // adj_rank x ss x kfm polyphase vertical factors, asap[(r ss + a) kfm + m] =
                         // asa_r[ss m + (a + pb) % ss] for m < Ma(a), else 0; kfm = (4 kf4 + ss - 1) / ss
     const float* asb;   // adj_rank x kf horizontal factors
+    const float* asbq;  // adj_rank x ss x (kfm + 1) horizontal factors at the window positions of mc_adj_h:
+                        // asbq[(r ss + b)(kfm + 1) + j] = asb_r[ss nn + rb] with nn = kfm - 1 + db(b) - db(0) - j
     float* U;           // bs x adj_rank x N x ss x N transposed horizontal pass, phase-major rows
     int D;              // channels
     int BW;             // staged row width (floats): pb + nsub + right margin, multiple of 4
@@ -194,7 +196,7 @@ __device__ __forceinline__ int mc_adj_wpp4(const DPlan& P) {
     return ((P.nsub * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
 }
 __device__ __forceinline__ int mc_adjh_wpp4(const DPlan& P) {
-    return ((P.N * P.ss * ((P.N + 63) / 64)) + 3) & ~3;
+    return ((P.N * ((P.N + 63) / 64)) + 3) & ~3;
 }
 constexpr int kAdjKY = 8;  // sub-rows of one phase per lane in the separable vertical pass
 __device__ __forceinline__ int mc_adjv_wpp4(const DPlan& P) {
@@ -1018,39 +1020,48 @@ __global__ __launch_bounds__(256) void mc_adj_v(const McTable T) {
 }
 
 // mc_adj_h: separable adjoint, transposed horizontal pass.  block 256 = 4 waves;
-// one wave per (particle, image row I, column phase b, 64 columns Xc):
-//   U_r[I][b][Xc] = sum_{nn < Mb} asb_r[ss nn + rb] dL/dimage[I][Xc + db - nn]
-// (the padded gradient image supplies the zero columns outside the image).
-template <bool MULTI>
+// one wave per (particle, image row I, 64 columns Xc); each lane computes every
+// column phase b and rank r:
+//   U_r[I][b][Xc] = sum_{nn < Mb} asb_r[ss nn + rb] dL/dimage[I][Xc + db - nn].
+// The lane loads the KFM + 1 gradient values Xc + db(0) - (KFM - 1) .. Xc + db(0) + 1
+// once (db(b) - db(0) is 0 or 1); the host's table asbq places each phase's factors
+// at their window positions (zero elsewhere), so with SS and KF4 fixed per launch
+// group every tap index is a compile-time constant.  The padded gradient image
+// (margin gp >= KFM) supplies the zero columns outside the image.
+template <bool MULTI, int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_adj_h(const McTable T) {
+    constexpr int KFM = (4 * KF4 + SS - 1) / SS;
+    constexpr int WN = KFM + 1;
     int local;
     const McLens& E = mc_locate<MULTI>(T, T.pre_adjh, blockIdx.x * 4, local);
     const DPlan& P = E.P;
     const McPlan& M = E.M;
-    const int N = P.N, ss = P.ss, kf = P.kf, pb = P.pb, R = M.adj_rank;
-    extern __shared__ float4 mc_dyn4[];
-    float* s_b = reinterpret_cast<float*>(mc_dyn4);
-    for (int i = threadIdx.x; i < R * kf; i += 256) s_b[i] = M.asb[i];
-    __syncthreads();
-    local += threadIdx.x >> 6;
+    const int N = P.N, pb = P.pb, R = M.adj_rank;
+    local += __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int njb = (N + 63) / 64, wpp4 = mc_adjh_wpp4(P);
     const int p = local / wpp4;
     int t = local - p * wpp4;
-    if (t >= N * ss * njb) return;
-    const int jb = t % njb; t /= njb;
-    const int b = t % ss; t /= ss;
-    const int I = t;
+    if (t >= N * njb) return;
+    const int jb = t % njb;
+    const int I = t / njb;
     const int Xc = jb * 64 + (threadIdx.x & 63);
     if (Xc >= N) return;
-    const int rb = (b + pb) % ss, db = (b + pb) / ss, Mb = (kf - rb + ss - 1) / ss;
+    const int db0 = pb / SS;
     const int GW = N + 2 * M.gp;
-    const float* row = M.gimg + (size_t)p * GW * GW + (size_t)(I + M.gp) * GW + (Xc + db + M.gp);
-    float* U = M.U + (size_t)p * R * N * ss * N + ((size_t)I * ss + b) * N + Xc;
+    const float* row = M.gimg + (size_t)p * GW * GW + (size_t)(I + M.gp) * GW + (Xc + db0 - (KFM - 1) + M.gp);
+    float w[WN];
+#pragma unroll
+    for (int j = 0; j < WN; ++j) w[j] = row[j];
+    float* U = M.U + (size_t)p * R * N * SS * N + (size_t)I * SS * N + Xc;
     for (int r = 0; r < R; ++r) {
-        const float* br = s_b + r * kf + rb;
-        float u = 0.0f;
-        for (int nn = 0; nn < Mb; ++nn) u = fmaf(br[ss * nn], row[-nn], u);
-        U[(size_t)r * N * ss * N] = u;
+#pragma unroll
+        for (int b = 0; b < SS; ++b) {
+            const CFloat* q = (const CFloat*)M.asbq + (r * SS + b) * WN;
+            float u = 0.0f;
+#pragma unroll
+            for (int j = 0; j < WN; ++j) u = fmaf(q[j], w[j], u);
+            U[(size_t)r * N * SS * N + (size_t)b * N] = u;
+        }
     }
 }
 
