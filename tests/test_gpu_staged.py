@@ -197,3 +197,35 @@ def test_separable_adjoint_matches_direct(cuda):
     assert sep.info()["sep_rank"] == 1 and direct.info()["sep_rank"] == 0
     assert e_lp < 1e-6
     assert e_g < 1e-5
+
+
+def test_separable_rank5_ss3_matches_direct_and_oracle(cuda):
+    """ss 3 with an 11x11 Gaussian PSF: the folded sub-grid PSF has rank 5 at 1e-6, so
+    a staged plan runs the separable channel conv and the separable adjoint (both
+    launched at compile-time ss 3).  Bars: image and log_prob within 1e-4 of the FP64
+    oracle; log_prob within 1e-6 relative and gradients within 1e-5 of the largest
+    component of the direct staged plan (psf_rank_tol = -1)."""
+    prior, phys, cfg, cam, ocam, obs = _big_problem(n_pix=64, ss=3, seed=13)
+    paths = [p for p, _ in tfd.leaves(prior)]
+    mk = lambda tol: Plan(phys, cam, layout_from_paths(phys, paths), prior=prior, observed=obs,
+                          background_rms=0.2, exp_time=100.0, staged=True, psf_rank_tol=tol)
+    sep, direct = mk(0.0), mk(-1.0)
+    info = sep.info()
+    assert info["sep_rank"] == 5 and info["adj_rank"] == 5, info
+    assert direct.info()["sep_rank"] == 0 and direct.info()["adj_rank"] == 0
+    x = C.draw_prior(prior, 4, seed=14)
+    img = sep.simulate(x.to(cuda).contiguous()).cpu().numpy()
+    ref = C.oracle_simulate(ocam, phys, C.nested_from_x(prior, x.numpy().astype(np.float64)))
+    e_im = float((np.abs(img - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2))).max())
+    pm = ForwardProbModel(prior, obs, 0.2, 100.0)
+    z = pm.bij.inverse(C.nested_from_x(prior, x)).contiguous()
+    lp_s, _, g_s = sep.log_prob(z.to(cuda), want_grad=True)
+    lp_d, _, g_d = direct.log_prob(z.to(cuda), want_grad=True)
+    lp_ref, _, _ = C.oracle_log_prob(ocam, phys, prior, z.double().numpy(), obs, 0.2, 100.0)
+    e_lp = float(np.max(np.abs(lp_s.cpu().numpy() - lp_ref) / np.abs(lp_ref)))
+    e_sd = float(((lp_s - lp_d).abs() / lp_d.abs()).max())
+    e_g = float(((g_s - g_d).abs().max(1).values / g_d.abs().max(1).values).max())
+    _print(f"64x64 ss3 rank 5: image vs oracle {e_im:.2e}, lp vs oracle {e_lp:.2e}, "
+           f"separable vs direct lp {e_sd:.2e} grad {e_g:.2e}")
+    assert e_im < 1e-4 and e_lp < 1e-4
+    assert e_sd < 1e-6 and e_g < 1e-5
