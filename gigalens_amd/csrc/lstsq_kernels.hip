@@ -575,7 +575,12 @@ template <bool MULTI, int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_sep_v(const McTable T) {
     constexpr int KFM = (4 * KF4 + SS - 1) / SS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int gtask = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);  // rows, bounds tests: scalar
+    // workgroups go round-robin over the 8 XCDs: consecutive logical blocks (the row
+    // blocks of one channel, which share H rows) are placed on one XCD and its L2.
+    // The grid is a multiple of 8, so every logical block is covered.
+    const int per = gridDim.x >> 3;
+    const int lblk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int gtask = __builtin_amdgcn_readfirstlane(lblk * 4 + wave);  // rows, bounds tests: scalar
     if (gtask >= T.tot_sepv) return;
     int task;
     const McLens& E = mc_locate<MULTI>(T, T.pre_sepv, gtask, task);
