@@ -701,8 +701,17 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
         GL_HIP(hipGetLastError());
         if (model == 1)
             hipLaunchKernelGGL((gl::mc_grad_m<MULTI, DefaultModel>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
-        else if (shp)
-            hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
+        else if (shp) {
+            // EPL in slot 0, shear in slot 1, at most 4 components (the shapelet
+            // configuration): the instance without the other lens kinds' states
+            const gl::DPlan& P1 = T.one.P;
+            const bool lean = !MULTI && P1.ncomp <= 4 && P1.nlens >= 1 && P1.nlens <= 2 &&
+                              P1.comp[0].kind == glm::K_EPL && (P1.nlens < 2 || P1.comp[1].kind == glm::K_SHEAR);
+            if (lean)
+                hipLaunchKernelGGL((gl::mc_grad_lean<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
+            else
+                hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
+        }
         else
             hipLaunchKernelGGL((gl::mc_grad<MULTI, false>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
         GL_HIP(hipGetLastError());

@@ -313,8 +313,20 @@ struct LensStates {
     glm::TnfwState tn[kMaxLens];
 };
 
+// Lens kinds a kernel instance handles, per lens slot: byte i of the mask has bit k
+// set when slot i may hold glm kind k.  A narrower mask compiles the other kinds'
+// branches and per-slot states out (fewer registers).
+constexpr int lens_bit(int kind) { return 1 << kind; }
+constexpr int kLensAny = lens_bit(glm::K_EPL) | lens_bit(glm::K_SHEAR) | lens_bit(glm::K_SIS) |
+                         lens_bit(glm::K_SIE) | lens_bit(glm::K_TNFW);
+constexpr int kLensAll = kLensAny | (kLensAny << 8) | (kLensAny << 16);
+// EPL in slot 0, shear in slot 1, no third lens (the reference's default lens model)
+constexpr int kLensEplShear = lens_bit(glm::K_EPL) | (lens_bit(glm::K_SHEAR) << 8);
+__host__ __device__ constexpr bool lens_in(int LM, int slot, int kind) { return (LM >> (8 * slot)) & lens_bit(kind); }
+
 // Deflection of all lenses of particle p at (X, Y); keeps the lens states when ADJ.
-template <bool ADJ>
+// Every lens kind of the plan must be in LM.
+template <bool ADJ, int LM = kLensAll>
 __device__ __forceinline__ void mc_beta(const DPlan& P, const McPlan& M, const float* pc, float X, float Y, float& bx,
                                         float& by, LensStates* ls) {
     const float* s_c = pc + ((6 * P.d + 3) & ~3);
@@ -326,27 +338,27 @@ __device__ __forceinline__ void mc_beta(const DPlan& P, const McPlan& M, const f
         const int kind = P.comp[i].kind;
         const float* cst = s_c + i * kConstStride;
         float ax = 0.0f, ay = 0.0f;
-        if (kind == glm::K_EPL) {
+        if (lens_in(LM, i, glm::K_EPL) && kind == glm::K_EPL) {
             const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
             const float* ca = pc + M.coef_base + P.coef_off[i];
             const float4* cb = reinterpret_cast<const float4*>(ca + glm::epl_tab_a_floats(P.max_n));
             glm::EplState st;
             glm::epl_forward<ADJ>(C, ca, cb, X, Y, ax, ay, st);
             if (ADJ) ls->es[i] = st;
-        } else if (kind == glm::K_SHEAR) {
+        } else if (lens_in(LM, i, glm::K_SHEAR) && kind == glm::K_SHEAR) {
             const glm::ShearC C = *reinterpret_cast<const glm::ShearC*>(cst);
             glm::shear_forward(C, X, Y, ax, ay);
-        } else if (kind == glm::K_SIS) {
+        } else if (lens_in(LM, i, glm::K_SIS) && kind == glm::K_SIS) {
             const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
             float st[3];
             glm::sis_forward(C, X, Y, ax, ay, st);
             if (ADJ) { ls->sis[i][0] = st[0]; ls->sis[i][1] = st[1]; ls->sis[i][2] = st[2]; }
-        } else if (kind == glm::K_SIE) {
+        } else if (lens_in(LM, i, glm::K_SIE) && kind == glm::K_SIE) {
             const glm::SieC C = *reinterpret_cast<const glm::SieC*>(cst);
             glm::SieState st;
             glm::sie_forward<ADJ>(C, X, Y, ax, ay, st);
             if (ADJ) ls->sie[i] = st;
-        } else if (kind == glm::K_TNFW) {
+        } else if (lens_in(LM, i, glm::K_TNFW) && kind == glm::K_TNFW) {
             const glm::TnfwC C = *reinterpret_cast<const glm::TnfwC*>(cst);
             glm::TnfwState st;
             glm::tnfw_forward<ADJ>(C, X, Y, ax, ay, st);
@@ -1083,8 +1095,11 @@ __global__ __launch_bounds__(256) void mc_adj_h(const McTable T) {
 // sub-pixels.  A sub-pixel whose forward value was zeroed for NaN contributes
 // nothing: per channel with least squares (the channel was zeroed), the whole
 // sub-pixel with fixed amplitudes (their sum was zeroed, tf/simulator.py:81).
-template <bool MULTI, bool SHP>
-__global__ __launch_bounds__(256) void mc_grad(const McTable T) {
+// LM: the lens kinds of every lens of the launch; NC: an upper bound on its
+// components (statistics kept in registers: 8 per component).
+template <bool MULTI, bool SHP, int LM, int NC>
+__device__ __forceinline__ void mc_grad_body(const McTable& T) {
+    static_assert(8 * NC <= kMcStat, "statistics per particle");
     int local;
     const McLens& E = mc_locate<MULTI>(T, T.pre_grad, blockIdx.x, local);
     const DPlan& P = E.P;
@@ -1093,9 +1108,9 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
     const int p = local / gpp;
     const int blk = local - p * gpp;
     const int nsub = P.nsub;
-    float st[kMcStat];
+    float st[8 * NC];
 #pragma unroll
-    for (int e = 0; e < kMcStat; ++e) st[e] = 0.0f;
+    for (int e = 0; e < 8 * NC; ++e) st[e] = 0.0f;
     extern __shared__ float4 mc_dyn4[];
     const float* pc = mc_stage_consts(M, p, reinterpret_cast<float*>(mc_dyn4));
     const float* s_c = pc + ((6 * P.d + 3) & ~3);
@@ -1109,7 +1124,7 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
         const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
         LensStates ls;
         float bx, by;
-        mc_beta<true>(P, M, pc, X, Y, bx, by, &ls);
+        mc_beta<true, LM>(P, M, pc, X, Y, bx, by, &ls);
         if (!M.lstsq) {
             // the sum of the lights: a NaN zeroed the whole sub-pixel in the forward pass
             float v = 0.0f;
@@ -1124,7 +1139,7 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
         int ch = 0;
         float gbx = 0.0f, gby = 0.0f;
 #pragma unroll
-        for (int i = 0; i < kMaxComp; ++i) {
+        for (int i = 0; i < NC; ++i) {
             if (i < P.nlens) continue;
             if (i >= P.ncomp) break;
             const DComp& cd = P.comp[i];
@@ -1176,18 +1191,18 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
             const int kind = P.comp[i].kind;
             const float* cst = s_c + i * kConstStride;
             float* acc = st + 8 * i;
-            if (kind == glm::K_EPL) {
+            if (lens_in(LM, i, glm::K_EPL) && kind == glm::K_EPL) {
                 const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
                 glm::epl_adjoint(C, ls.es[i], Hx, Hy, acc);
-            } else if (kind == glm::K_SHEAR) {
+            } else if (lens_in(LM, i, glm::K_SHEAR) && kind == glm::K_SHEAR) {
                 glm::shear_adjoint(X, Y, Hx, Hy, acc);
-            } else if (kind == glm::K_SIS) {
+            } else if (lens_in(LM, i, glm::K_SIS) && kind == glm::K_SIS) {
                 const glm::SisC C = *reinterpret_cast<const glm::SisC*>(cst);
                 glm::sis_adjoint(C, ls.sis[i], Hx, Hy, acc);
-            } else if (kind == glm::K_SIE) {
+            } else if (lens_in(LM, i, glm::K_SIE) && kind == glm::K_SIE) {
                 const glm::SieC C = *reinterpret_cast<const glm::SieC*>(cst);
                 glm::sie_adjoint(C, ls.sie[i], Hx, Hy, acc);
-            } else if (kind == glm::K_TNFW) {
+            } else if (lens_in(LM, i, glm::K_TNFW) && kind == glm::K_TNFW) {
                 const glm::TnfwC C = *reinterpret_cast<const glm::TnfwC*>(cst);
                 glm::tnfw_adjoint(C, ls.tn[i], Hx, Hy, acc);
             }
@@ -1197,7 +1212,7 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nst = 8 * P.ncomp;
 #pragma unroll
-    for (int e = 0; e < kMcStat; ++e) {
+    for (int e = 0; e < 8 * NC; ++e) {
         if (e < nst) {
             float v = st[e];
             for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1209,6 +1224,18 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
         const float v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
         atomicAdd(M.stats + (size_t)p * kMcStat + threadIdx.x, v);
     }
+}
+
+template <bool MULTI, bool SHP>
+__global__ __launch_bounds__(256) void mc_grad(const McTable T) {
+    mc_grad_body<MULTI, SHP, kLensAll, kMaxComp>(T);
+}
+
+// The narrow instance (EPL + shear lenses, at most 4 components) fits two
+// workgroups per CU: 8 waves, up to 256 registers per lane.
+template <bool MULTI, bool SHP>
+__global__ __launch_bounds__(256, 2) void mc_grad_lean(const McTable T) {
+    mc_grad_body<MULTI, SHP, kLensEplShear, 4>(T);
 }
 
 // ---------------------------------------------------------------------------
