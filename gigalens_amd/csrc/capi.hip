@@ -581,7 +581,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     }
     const int n2 = D + (D & 1);
     // A, V (n2 x n2), rotations (n2), pairs (n2 ints), 2 reduction slots, rhs (n2)
-    p->solve_lds = (size_t)(2 * n2 * n2 + n2) * 8 + (size_t)n2 * 4 + 16 + (size_t)n2 * 8 + 64;
+    p->solve_lds = (size_t)(2 * n2 * (n2 + 1) + n2) * 8 + (size_t)n2 * 4 + 16 + (size_t)n2 * 8 + 64;  // A, V: n2 x (n2 + 1)
     if (lstsq)
         GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_solve<false>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->solve_lds));

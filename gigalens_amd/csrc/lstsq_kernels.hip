@@ -754,22 +754,23 @@ __global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, in
     const McPlan& M = mc_locate<MULTI>(T, T.pre_part, blockIdx.x, p).M;
     const int D = M.D;
     const int n = D + (D & 1);  // even size (dummy index n-1 when D is odd)
-    double* A = s_d;            // n x n
-    double* V = A + n * n;      // n x n
-    double* rot = V + n * n;    // (n/2) x 2
+    const int ns = n + 1;       // odd row stride: column sweeps hit distinct LDS banks
+    double* A = s_d;            // n x ns
+    double* V = A + n * ns;     // n x ns
+    double* rot = V + n * ns;   // (n/2) x 2
     int* pr = reinterpret_cast<int*>(rot + n);  // (n/2) x 2 pairs
     double* red = reinterpret_cast<double*>(pr + n + (n & 1));
     double* bv = red + 2;       // n: X^T W^2 obs
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int i = e / n, j = e - (e / n) * n;
-        A[e] = (i < D && j < D && i <= j) ? mc_gram_entry(M, p, parts, i, j) : 0.0;
-        V[e] = i == j ? 1.0 : 0.0;
+        A[i * ns + j] = (i < D && j < D && i <= j) ? mc_gram_entry(M, p, parts, i, j) : 0.0;
+        V[i * ns + j] = i == j ? 1.0 : 0.0;
     }
     for (int i = threadIdx.x; i < D; i += blockDim.x) bv[i] = mc_gram_entry(M, p, parts, i, D);
     __syncthreads();
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int i = e / n, j = e - (e / n) * n;
-        if (i > j) A[e] = A[j * n + i];
+        if (i > j) A[i * ns + j] = A[j * ns + i];
     }
     __syncthreads();
     const int half = n / 2;
@@ -780,7 +781,7 @@ __global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, in
         double off = 0.0, dia = 0.0;
         for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
             const int i = e / n, j = e - (e / n) * n;
-            const double a = A[e];
+            const double a = A[i * ns + j];
             if (i == j) dia += a * a; else off += a * a;
         }
         for (int o = 32; o >= 1; o >>= 1) {
@@ -803,9 +804,9 @@ __global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, in
                 int b = 1 + (n - 2 - k + round) % (n - 1);
                 if (a > b) { int t = a; a = b; b = t; }
                 double c = 1.0, s = 0.0;
-                const double apq = A[a * n + b];
+                const double apq = A[a * ns + b];
                 if (apq != 0.0) {
-                    const double th = (A[b * n + b] - A[a * n + a]) / (2.0 * apq);
+                    const double th = (A[b * ns + b] - A[a * ns + a]) / (2.0 * apq);
                     const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
                     c = 1.0 / sqrt(t * t + 1.0);
                     s = t * c;
@@ -821,9 +822,9 @@ __global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, in
                 const int k = e / n, j = e - (e / n) * n;
                 const int a = pr[2 * k], b = pr[2 * k + 1];
                 const double c = rot[2 * k], s = rot[2 * k + 1];
-                const double x = A[a * n + j], y = A[b * n + j];
-                A[a * n + j] = c * x - s * y;
-                A[b * n + j] = s * x + c * y;
+                const double x = A[a * ns + j], y = A[b * ns + j];
+                A[a * ns + j] = c * x - s * y;
+                A[b * ns + j] = s * x + c * y;
             }
             __syncthreads();
             // columns: A <- A J, V <- V J
@@ -831,12 +832,12 @@ __global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, in
                 const int k = e / n, i = e - (e / n) * n;
                 const int a = pr[2 * k], b = pr[2 * k + 1];
                 const double c = rot[2 * k], s = rot[2 * k + 1];
-                const double x = A[i * n + a], y = A[i * n + b];
-                A[i * n + a] = c * x - s * y;
-                A[i * n + b] = s * x + c * y;
-                const double vx = V[i * n + a], vy = V[i * n + b];
-                V[i * n + a] = c * vx - s * vy;
-                V[i * n + b] = s * vx + c * vy;
+                const double x = A[i * ns + a], y = A[i * ns + b];
+                A[i * ns + a] = c * x - s * y;
+                A[i * ns + b] = s * x + c * y;
+                const double vx = V[i * ns + a], vy = V[i * ns + b];
+                V[i * ns + a] = c * vx - s * vy;
+                V[i * ns + b] = s * vx + c * vy;
             }
             __syncthreads();
         }
@@ -845,21 +846,21 @@ __global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, in
     double* w = rot;  // reuse: n doubles of projections
     if (threadIdx.x == 0) {
         double mx = 0.0;
-        for (int i = 0; i < D; ++i) mx = fmax(mx, fabs(A[i * n + i]));
+        for (int i = 0; i < D; ++i) mx = fmax(mx, fabs(A[i * ns + i]));
         red[0] = mx * (double)rcond;
     }
     __syncthreads();
     const double cut = red[0];
     for (int i = threadIdx.x; i < D; i += blockDim.x) {
         double s = 0.0;
-        for (int j = 0; j < D; ++j) s += V[j * n + i] * bv[j];
-        const double l = A[i * n + i];
+        for (int j = 0; j < D; ++j) s += V[j * ns + i] * bv[j];
+        const double l = A[i * ns + i];
         w[i] = fabs(l) > cut ? s / l : 0.0;
     }
     __syncthreads();
     for (int j = threadIdx.x; j < D; j += blockDim.x) {
         double s = 0.0;
-        for (int i = 0; i < D; ++i) s += V[j * n + i] * w[i];
+        for (int i = 0; i < D; ++i) s += V[j * ns + i] * w[i];
         M.coef[(size_t)p * D + j] = (float)s;
     }
 }
