@@ -671,7 +671,7 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
         if (MULTI) return fail(GL_ERR_UNSUPPORTED, "least-squares lenses are not batched");
         const int ks = std::max(1, std::min(ksplit_cap, (1024 + np - 1) / np));
         GL_HIP(mc_gram_dispatch(gram_nb, np, ks, st, T));
-        hipLaunchKernelGGL((gl::mc_solve<MULTI>), dim3(np), dim3(256), solve_lds, st, T, T.one.M.rcond, 4 * ks);
+        hipLaunchKernelGGL((gl::mc_solve<MULTI>), dim3(np), dim3(1024), solve_lds, st, T, T.one.M.rcond, 4 * ks);
         GL_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((gl::mc_image<MULTI>), dim3(np), dim3(256), 0, st, T, A, grad ? 1 : 0);

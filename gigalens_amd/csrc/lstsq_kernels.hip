@@ -748,7 +748,7 @@ __device__ __forceinline__ double mc_gram_entry(const McPlan& M, int p, int part
 // mc_solve: grid bs, block 256.  FP64 parallel cyclic Jacobi + pseudo-inverse.
 // ---------------------------------------------------------------------------
 template <bool MULTI>
-__global__ __launch_bounds__(256) void mc_solve(const McTable T, float rcond, int parts) {
+__global__ __launch_bounds__(1024) void mc_solve(const McTable T, float rcond, int parts) {
     extern __shared__ double s_d[];
     int p;
     const McPlan& M = mc_locate<MULTI>(T, T.pre_part, blockIdx.x, p).M;
