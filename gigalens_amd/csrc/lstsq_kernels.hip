@@ -546,31 +546,48 @@ __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
     if (J >= N) return;
     const float* Bc = M.B + ((size_t)p * M.D + c) * nsub * M.BW;
     float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;
-#pragma unroll
+    // taps in two halves: half the window in registers at a time (more waves per SIMD)
+    constexpr int KH4 = (KF4 + 1) / 2;
+#pragma unroll 1
     for (int i = 0; i < 4; ++i) {
         const int y = yb * 4 + i;
         if (y >= nsub) break;
         const float* row = Bc + (size_t)y * M.BW + SS * J;
-        float v[4 * KF4];
-        if (SS == 4) {
-            const float4* r4 = reinterpret_cast<const float4*>(row);
+        float h[kMcMaxRank];
 #pragma unroll
-            for (int j = 0; j < KF4; ++j) {
-                const float4 t = r4[j];
-                v[4 * j] = t.x; v[4 * j + 1] = t.y; v[4 * j + 2] = t.z; v[4 * j + 3] = t.w;
+        for (int r = 0; r < kMcMaxRank; ++r) h[r] = 0.0f;
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            const int j0 = hf * KH4;
+            const int nj = hf == 0 ? KH4 : KF4 - KH4;
+            float v[4 * KH4];
+            if (SS == 4) {
+                const float4* r4 = reinterpret_cast<const float4*>(row) + j0;
+#pragma unroll
+                for (int j = 0; j < KH4; ++j) {
+                    if (j < nj) {
+                        const float4 t = r4[j];
+                        v[4 * j] = t.x; v[4 * j + 1] = t.y; v[4 * j + 2] = t.z; v[4 * j + 3] = t.w;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4 * KH4; ++j)
+                    if (j < 4 * nj) v[j] = row[4 * j0 + j];
             }
-        } else {
 #pragma unroll
-            for (int j = 0; j < 4 * KF4; ++j) v[j] = row[j];
+            for (int r = 0; r < kMcMaxRank; ++r) {
+                if (r >= R) break;
+                const CFloat* w = (const CFloat*)M.sbp + r * M.wpad_w + 4 * j0;
+#pragma unroll
+                for (int j = 0; j < 4 * KH4; ++j)
+                    if (j < 4 * nj) h[r] = fmaf(w[j], v[j], h[r]);
+            }
         }
 #pragma unroll
         for (int r = 0; r < kMcMaxRank; ++r) {
             if (r >= R) break;
-            const CFloat* w = (const CFloat*)M.sbp + r * M.wpad_w;
-            float h = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 4 * KF4; ++j) h = fmaf(w[j], v[j], h);
-            Hc[((size_t)r * nsub + y) * N + J] = h;
+            Hc[((size_t)r * nsub + y) * N + J] = h[r];
         }
     }
 }
