@@ -604,7 +604,7 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     u.grad = (long)count * (((long)P.nsub * P.nsub + 256 * gl::kMcGradK - 1) / (256 * gl::kMcGradK));
     // channel conv: direct (mc_conv) or separable (mc_sep_h + mc_sep_v)
     u.conv = M.sep_rank > 0 ? 0 : (long)count * M.D * ((P.N + M.conv_ib - 1) / M.conv_ib) * njb;
-    u.seph = M.sep_rank > 0 ? (long)count * M.D * ((P.nsub + 3) / 4) * njb : 0;
+    u.seph = M.sep_rank > 0 ? (long)count * M.D * (((long)P.nsub * P.N + 255) / 256) : 0;
     u.sepv = M.sep_rank > 0 ? (long)count * M.D * ((P.N + gl::kSepIB - 1) / gl::kSepIB) * njb : 0;
     // adjoint conv: direct (mc_adjoint) or separable (mc_adj_h + mc_adj_v)
     const long nyb = (P.N + gl::kAdjKY - 1) / gl::kAdjKY;

@@ -537,21 +537,21 @@ __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
     const DPlan& P = E.P;
     const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, R = M.sep_rank;
-    const int nyb = (nsub + 3) / 4, njb = (N + 63) / 64;
-    const int jb = task % njb; task /= njb;
-    const int yb = task % nyb; task /= nyb;
+    // a wave covers 4 x 64 consecutive outputs of the flattened (sub-row, column) range,
+    // so every lane works whatever N is
+    const int nout = nsub * N, nfb = (nout + 255) / 256;
+    const int fb = task % nfb; task /= nfb;
     const int c = task % M.D;
     const int p = task / M.D;
-    const int J = jb * 64 + lane;
-    if (J >= N) return;
     const float* Bc = M.B + ((size_t)p * M.D + c) * nsub * M.BW;
     float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;
     // taps in two halves: half the window in registers at a time (more waves per SIMD)
     constexpr int KH4 = (KF4 + 1) / 2;
 #pragma unroll 1
     for (int i = 0; i < 4; ++i) {
-        const int y = yb * 4 + i;
-        if (y >= nsub) break;
+        const int f = (fb * 4 + i) * 64 + lane;
+        if (f >= nout) break;
+        const int y = f / N, J = f - y * N;
         const float* row = Bc + (size_t)y * M.BW + SS * J;
         float h[kMcMaxRank];
 #pragma unroll
