@@ -9,77 +9,30 @@
 #include <string>
 #include <vector>
 
-#include "lens_kernels.hip"
-#include "lstsq_kernels.hip"
-#include "cnn_kernels.hip"
+#include "capi_internal.h"
+
+using namespace glc;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int glc::fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
 
-#define GL_HIP(expr)                                                                  \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess) return fail(GL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-constexpr int kBlock = 512;
-
-using LaunchFn = hipError_t (*)(int grid, size_t lds, hipStream_t, const gl::DPlan&, const gl::KArgs&, bool grad);
-using AttrFn = hipError_t (*)(size_t lds);
-
-template <class M>
-hipError_t launch_model(int grid, size_t lds, hipStream_t s, const gl::DPlan& P, const gl::KArgs& A, bool grad) {
-    if (grad)
-        hipLaunchKernelGGL((gl::particle_kernel<kBlock, M, true>), dim3(grid), dim3(kBlock), lds, s, P, A);
-    else
-        hipLaunchKernelGGL((gl::particle_kernel<kBlock, M, false>), dim3(grid), dim3(kBlock), lds, s, P, A);
-    return hipGetLastError();
+static const ModelEntry* find_model(const int sig[7]) {
+    int ns[3];
+    const ModelEntry* tabs[3] = {models_0(&ns[0]), models_1(&ns[1]), models_2(&ns[2])};
+    for (int t = 0; t < 3; ++t)
+        for (int i = 0; i < ns[t]; ++i) {
+            bool ok = true;
+            for (int j = 0; j < 7; ++j) ok = ok && tabs[t][i].sig[j] == sig[j];
+            if (ok) return &tabs[t][i];
+        }
+    return nullptr;
 }
-
-template <class M>
-hipError_t attr_model(size_t lds) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::particle_kernel<kBlock, M, true>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::particle_kernel<kBlock, M, false>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-}
-
-struct ModelEntry {
-    int sig[7];  // lens0..2, lens light 0..1, source 0..1
-    LaunchFn launch;
-    AttrFn attr;
-};
-
-constexpr int EPL = glm::K_EPL, SHR = glm::K_SHEAR, SIS = glm::K_SIS, SIE = glm::K_SIE, TNFW = glm::K_TNFW,
-              SE = glm::K_SERSIC_ELLIPSE, SC = glm::K_SERSIC, CSE = glm::K_CORE_SERSIC;
-
-#define GL_ENTRY(L0, L1, L2, A0, A1, S0, S1)                                      \
-    {{L0, L1, L2, A0, A1, S0, S1},                                                \
-     &launch_model<gl::ModelT<L0, L1, L2, A0, A1, S0, S1>>,                        \
-     &attr_model<gl::ModelT<L0, L1, L2, A0, A1, S0, S1>>}
-
-const ModelEntry kModels[] = {
-    GL_ENTRY(EPL, SHR, 0, SE, 0, SE, 0),   // reference default model (tests/conftest.py:76-80)
-    GL_ENTRY(EPL, 0, 0, SE, 0, SE, 0),
-    GL_ENTRY(EPL, SHR, 0, 0, 0, SE, 0),
-    GL_ENTRY(EPL, 0, 0, 0, 0, SE, 0),
-    GL_ENTRY(SIS, 0, 0, 0, 0, SE, 0),
-    GL_ENTRY(SIS, SHR, 0, SE, 0, SE, 0),
-    GL_ENTRY(EPL, SHR, 0, SE, 0, SC, 0),
-    GL_ENTRY(EPL, SHR, 0, SE, SE, SE, 0),
-    GL_ENTRY(SIE, SHR, 0, SE, 0, SE, 0),
-    GL_ENTRY(EPL, SHR, TNFW, SE, 0, SE, 0),  // main deflector + one truncated-NFW subhalo
-    GL_ENTRY(EPL, SHR, 0, CSE, 0, SE, 0),    // core-Sersic lens light
-};
-
-}  // namespace
 
 struct gl_plan {
     gl::DPlan P;
@@ -194,95 +147,6 @@ extern "C" int gl_plan_destroy(gl_plan* plan) {
 // for images / models the LDS-resident kernel does not take
 // ---------------------------------------------------------------------------
 static int upload_common(const gl_plan_desc* desc, gl_plan* p);
-
-template <int SS, int KF4, bool MULTI>
-static hipError_t launch_mc_conv(int ib, int grid, hipStream_t st, const gl::McTable& T) {
-    if (MULTI || ib == 4)
-        hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI, 4>), dim3(grid), dim3(256), 0, st, T);
-    else if (ib == 2)
-        hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI, 2>), dim3(grid), dim3(256), 0, st, T);
-    else if (ib == 8)
-        hipLaunchKernelGGL((gl::mc_conv<SS, KF4, MULTI, 8>), dim3(grid), dim3(256), 0, st, T);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
-}
-
-template <bool MULTI>
-static hipError_t mc_conv_dispatch(int ss, int kf4, int ib, int grid, hipStream_t st, const gl::McTable& T) {
-#define GL_MCC(S, K) if (ss == S && kf4 == K) return launch_mc_conv<S, K, MULTI>(ib, grid, st, T)
-    GL_MCC(0, 8); GL_MCC(0, 16); GL_MCC(0, 22); GL_MCC(0, 32);
-    GL_MCC(1, 8); GL_MCC(1, 16); GL_MCC(1, 22); GL_MCC(1, 32);
-    GL_MCC(2, 8); GL_MCC(2, 16); GL_MCC(2, 22); GL_MCC(2, 32);
-    GL_MCC(3, 8); GL_MCC(3, 16); GL_MCC(3, 22); GL_MCC(3, 32);
-    GL_MCC(4, 8); GL_MCC(4, 16); GL_MCC(4, 22); GL_MCC(4, 32);
-#undef GL_MCC
-    return hipErrorInvalidValue;
-}
-
-template <bool MULTI>
-static hipError_t mc_adjh_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::McTable& T) {
-#define GL_MCH(S, K)                                                                              \
-    if (ss == S && kf4 == K) {                                                                    \
-        hipLaunchKernelGGL((gl::mc_adj_h<MULTI, S, K>), dim3(grid), dim3(256), 0, st, T);        \
-        return hipGetLastError();                                                                 \
-    }
-    GL_MCH(1, 8); GL_MCH(1, 16); GL_MCH(1, 22); GL_MCH(1, 32);
-    GL_MCH(2, 8); GL_MCH(2, 16); GL_MCH(2, 22); GL_MCH(2, 32);
-    GL_MCH(3, 8); GL_MCH(3, 16); GL_MCH(3, 22); GL_MCH(3, 32);
-    GL_MCH(4, 8); GL_MCH(4, 16); GL_MCH(4, 22); GL_MCH(4, 32);
-#undef GL_MCH
-    return hipErrorInvalidValue;
-}
-
-template <bool MULTI>
-static hipError_t mc_adjv_dispatch(int ss, int kf4, int grid, hipStream_t st, const gl::McTable& T) {
-#define GL_MCA(S, K)                                                                              \
-    if (ss == S && kf4 == K) {                                                                    \
-        hipLaunchKernelGGL((gl::mc_adj_v<MULTI, S, K>), dim3(grid), dim3(256), 0, st, T);        \
-        return hipGetLastError();                                                                 \
-    }
-    GL_MCA(1, 8); GL_MCA(1, 16); GL_MCA(1, 22); GL_MCA(1, 32);
-    GL_MCA(2, 8); GL_MCA(2, 16); GL_MCA(2, 22); GL_MCA(2, 32);
-    GL_MCA(3, 8); GL_MCA(3, 16); GL_MCA(3, 22); GL_MCA(3, 32);
-    GL_MCA(4, 8); GL_MCA(4, 16); GL_MCA(4, 22); GL_MCA(4, 32);
-#undef GL_MCA
-    return hipErrorInvalidValue;
-}
-
-// separable channel conv: T.tot_seph / T.tot_sepv waves (one lens, or one launch group of a batch)
-template <bool MULTI, int SS, int KF4>
-static hipError_t launch_mc_sep(hipStream_t st, const gl::McTable& T) {
-    hipLaunchKernelGGL((gl::mc_sep_h<MULTI, SS, KF4>), dim3((T.tot_seph + 3) / 4), dim3(256), 0, st, T);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // a multiple of 8 workgroups (mc_sep_v's XCD-local block order)
-    hipLaunchKernelGGL((gl::mc_sep_v<MULTI, SS, KF4>), dim3((((T.tot_sepv + 3) / 4) + 7) & ~7), dim3(256), 0, st, T);
-    return hipGetLastError();
-}
-
-template <bool MULTI>
-static hipError_t mc_sep_dispatch(int ss, int kf4, hipStream_t st, const gl::McTable& T) {
-#define GL_MCS(S, K) if (ss == S && kf4 == K) return launch_mc_sep<MULTI, S, K>(st, T)
-    GL_MCS(1, 8); GL_MCS(1, 16); GL_MCS(1, 22); GL_MCS(1, 32);
-    GL_MCS(2, 8); GL_MCS(2, 16); GL_MCS(2, 22); GL_MCS(2, 32);
-    GL_MCS(3, 8); GL_MCS(3, 16); GL_MCS(3, 22); GL_MCS(3, 32);
-    GL_MCS(4, 8); GL_MCS(4, 16); GL_MCS(4, 22); GL_MCS(4, 32);
-#undef GL_MCS
-    return hipErrorInvalidValue;
-}
-
-static hipError_t mc_gram_dispatch(int nbt, int nb, int ks, hipStream_t st, const gl::McTable& T) {
-    switch (nbt) {
-        case 1: hipLaunchKernelGGL((gl::mc_gram<1, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
-        case 2: hipLaunchKernelGGL((gl::mc_gram<2, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
-        case 3: hipLaunchKernelGGL((gl::mc_gram<3, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
-        case 4: hipLaunchKernelGGL((gl::mc_gram<4, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
-        case 5: hipLaunchKernelGGL((gl::mc_gram<5, false>), dim3(nb, ks), dim3(256), 0, st, T, ks); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
 
 static int pick_kf4(int kf) {
     const int need = (kf + 3) / 4;
@@ -583,8 +447,7 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     // A, V (n2 x n2), rotations (n2), pairs (n2 ints), 2 reduction slots, rhs (n2)
     p->solve_lds = (size_t)(2 * n2 * (n2 + 1) + n2) * 8 + (size_t)n2 * 4 + 16 + (size_t)n2 * 8 + 64;  // A, V: n2 x (n2 + 1)
     if (lstsq)
-        GL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&gl::mc_solve<false>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->solve_lds));
+        GL_HIP(mc_solve_attr(p->solve_lds));
     p->epl_batch = 0;
     p->lds_bytes = 0;
     P.sep_rank = 0;
@@ -612,113 +475,6 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     u.adjh = M.adj_rank > 0 ? (long)count * (((long)P.N * njb + 3) & ~3L) : 0;
     u.adjv = M.adj_rank > 0 ? (long)count * (((long)P.ss * nyb * P.ss * njb + 3) & ~3L) : 0;
     return u;
-}
-
-// One pass of the staged pipeline over the lenses of table T.
-// Conv launch group of a batch: the lenses of one (ss, kf4); the prefix table
-// counts only their conv waves, so the group's launch runs compile-time ss with
-// its own tap width.
-struct ConvGroup {
-    int ss, kf4, tot;
-    const int* pre;
-    int adjv_tot;         // separable-adjoint vertical-pass units of the group's lenses
-    const int* adjv_pre;
-    int adjh_tot;         // separable-adjoint horizontal-pass units of the group's lenses
-    const int* adjh_pre;
-    int seph_tot, sepv_tot;  // separable channel conv units of the group's separable lenses
-    const int* seph_pre;
-    const int* sepv_pre;
-};
-
-// model 1: every lens is a fixed-amplitude plan of the reference default model
-// (EPL + shear | Sersic ellipse | Sersic ellipse): model-specialised basis and gradient.
-using DefaultModel = gl::ModelT<EPL, SHR, 0, SE, 0, SE, 0>;
-
-template <bool MULTI>
-static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch, int kf4, int lstsq, int gram_nb,
-                       int ksplit_cap, size_t solve_lds, bool grad, hipStream_t st, bool shp = false, int model = 0,
-                       const std::vector<ConvGroup>* groups = nullptr) {
-    const int np = T.tot_part;
-    hipLaunchKernelGGL((gl::mc_prologue<MULTI>), dim3(np), dim3(128), 0, st, T, A);
-    GL_HIP(hipGetLastError());
-    if (model == 1)
-        hipLaunchKernelGGL((gl::mc_basis_m<MULTI, DefaultModel>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
-    else
-        hipLaunchKernelGGL((gl::mc_basis<MULTI>), dim3(T.tot_basis), dim3(256), T.lds_consts, st, T);
-    GL_HIP(hipGetLastError());
-    if (!MULTI && T.one.M.sep_rank > 0) {
-        GL_HIP(mc_sep_dispatch<MULTI>(ss_launch, kf4, st, T));
-    } else if (MULTI && groups && !groups->empty()) {
-        for (const ConvGroup& g : *groups) {
-            gl::McTable Tg = T;
-            if (g.tot > 0) {
-                Tg.pre_conv = g.pre;
-                Tg.tot_conv = g.tot;
-                GL_HIP(mc_conv_dispatch<MULTI>(g.ss, g.kf4, gl::kMcIB, (g.tot + 3) / 4, st, Tg));
-            }
-            if (g.seph_tot > 0) {
-                Tg.pre_seph = g.seph_pre;
-                Tg.tot_seph = g.seph_tot;
-                Tg.pre_sepv = g.sepv_pre;
-                Tg.tot_sepv = g.sepv_tot;
-                GL_HIP(mc_sep_dispatch<MULTI>(g.ss, g.kf4, st, Tg));
-            }
-        }
-    } else {
-        GL_HIP(mc_conv_dispatch<MULTI>(ss_launch, kf4, MULTI ? 4 : T.one.M.conv_ib, (T.tot_conv + 3) / 4, st, T));
-    }
-    if (lstsq) {
-        if (MULTI) return fail(GL_ERR_UNSUPPORTED, "least-squares lenses are not batched");
-        const int ks = std::max(1, std::min(ksplit_cap, (1024 + np - 1) / np));
-        GL_HIP(mc_gram_dispatch(gram_nb, np, ks, st, T));
-        hipLaunchKernelGGL((gl::mc_solve<MULTI>), dim3(np), dim3(1024), solve_lds, st, T, T.one.M.rcond, 4 * ks);
-        GL_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL((gl::mc_image<MULTI>), dim3(np), dim3(256), 0, st, T, A, grad ? 1 : 0);
-    GL_HIP(hipGetLastError());
-    if (grad) {
-        if (T.tot_adjh > 0) {
-            // both passes: compile-time (ss, kf4), one launch each per group of lenses
-            if (MULTI) {
-                if (!groups) return fail(GL_ERR_ARG, "lens batch without launch groups");
-                for (const ConvGroup& g : *groups) {
-                    if (g.adjv_tot <= 0) continue;
-                    gl::McTable Tg = T;
-                    Tg.pre_adjh = g.adjh_pre;
-                    Tg.tot_adjh = g.adjh_tot;
-                    Tg.pre_adjv = g.adjv_pre;
-                    Tg.tot_adjv = g.adjv_tot;
-                    GL_HIP(mc_adjh_dispatch<MULTI>(g.ss, g.kf4, g.adjh_tot / 4, st, Tg));
-                    GL_HIP(mc_adjv_dispatch<MULTI>(g.ss, g.kf4, g.adjv_tot / 4, st, Tg));
-                }
-            } else {
-                GL_HIP(mc_adjh_dispatch<MULTI>(ss_launch, kf4, T.tot_adjh / 4, st, T));
-                GL_HIP(mc_adjv_dispatch<MULTI>(ss_launch, kf4, T.tot_adjv / 4, st, T));
-            }
-        }
-        if (T.tot_adj > 0)
-            hipLaunchKernelGGL((gl::mc_adjoint<MULTI>), dim3(T.tot_adj / 4), dim3(256), T.lds_w, st, T);
-        GL_HIP(hipGetLastError());
-        if (model == 1)
-            hipLaunchKernelGGL((gl::mc_grad_m<MULTI, DefaultModel>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
-        else if (shp) {
-            // EPL in slot 0, shear in slot 1, at most 4 components (the shapelet
-            // configuration): the instance without the other lens kinds' states
-            const gl::DPlan& P1 = T.one.P;
-            const bool lean = !MULTI && P1.ncomp <= 4 && P1.nlens >= 1 && P1.nlens <= 2 &&
-                              P1.comp[0].kind == glm::K_EPL && (P1.nlens < 2 || P1.comp[1].kind == glm::K_SHEAR);
-            if (lean)
-                hipLaunchKernelGGL((gl::mc_grad_lean<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
-            else
-                hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
-        }
-        else
-            hipLaunchKernelGGL((gl::mc_grad<MULTI, false>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
-        GL_HIP(hipGetLastError());
-        hipLaunchKernelGGL((gl::mc_finish<MULTI>), dim3(np), dim3(128), 0, st, T, A);
-        GL_HIP(hipGetLastError());
-    }
-    return GL_OK;
 }
 
 // Run a staged plan over bs particles in chunks of max_batch.  A carries the
@@ -765,8 +521,8 @@ static int run_mc(const gl_plan* plan, const gl::McArgs& A0, bool grad, hipStrea
         T.tot_sepv = (int)u.sepv;
         T.lds_consts = M.pstride * 4;
         T.lds_w = P.kf * P.kf * 4;
-        int rc = mc_pipeline<false>(T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
-                                    grad, st, M.shp_comp >= 0, plan->mc_model);
+        int rc = mc_pipeline_run(false, T, A, P.ss, plan->kf4, M.lstsq, M.gram_nb, plan->ksplit_cap, plan->solve_lds,
+                                 grad, st, M.shp_comp >= 0, plan->mc_model, nullptr);
         if (rc) return rc;
     }
     return GL_OK;
@@ -909,11 +665,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     const int lstsq = p->mc;
     bool staged = p->mc || desc->staged;
     if (!staged) {
-        for (const ModelEntry& e : kModels) {
-            bool ok = true;
-            for (int j = 0; j < 7; ++j) ok = ok && e.sig[j] == sig[j];
-            if (ok) { p->model = &e; break; }
-        }
+        p->model = find_model(sig);
         staged = !p->model;  // no fused kernel for this combination: the staged pipeline
     }
 
@@ -1184,17 +936,13 @@ static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream)
     if (!A.in) return fail(GL_ERR_ARG, "null parameter pointer");
     if (plan->epl_batch) {
         const int use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
-        hipLaunchKernelGGL(gl::epl_batch_series_kernel, dim3(1), dim3(256), 0, stream, plan->P, A.in, A.bs, use_z,
-                           const_cast<int*>(plan->P.batch_n));
-        GL_HIP(hipGetLastError());
+        GL_HIP(launch_epl_batch_series(plan->P, A.in, A.bs, use_z, const_cast<int*>(plan->P.batch_n), stream));
     }
     // launch order by cost when the launch has several rounds of workgroups (gradient launches)
     A.order = nullptr;
     if (plan->order_buf && grad && A.bs > 1024 && A.bs <= gl::kOrderCap) {
         const int use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
-        hipLaunchKernelGGL(gl::order_kernel, dim3(1), dim3(1024), 0, stream, plan->P, A.in, A.bs, use_z,
-                           plan->order_buf);
-        GL_HIP(hipGetLastError());
+        GL_HIP(launch_order(plan->P, A.in, A.bs, use_z, plan->order_buf, stream));
         A.order = plan->order_buf;
     }
     GL_HIP(plan->model->launch(A.bs, plan->lds_bytes, stream, plan->P, A, grad));
@@ -1482,8 +1230,8 @@ extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, f
     A.lp = lp;
     A.chi2 = chi2;
     A.grad = grad_z;
-    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false, b->mc_model,
-                             &b->groups);
+    return mc_pipeline_run(true, b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false,
+                           b->mc_model, &b->groups);
 }
 
 extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, float* v, int step, float lr,
@@ -1507,275 +1255,6 @@ extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, flo
     A.beta1 = beta1;
     A.beta2 = beta2;
     A.eps = epsilon;
-    return mc_pipeline<true>(b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream, false, b->mc_model,
-                             &b->groups);
-}
-
-// ---------------------------------------------------------------------------
-// Learned initialiser (no reference counterpart)
-// ---------------------------------------------------------------------------
-struct gl_cnn {
-    struct Layer {
-        int cin, cout, s, H, W, Ho, Wo, kp;
-        size_t woff, boff;         // offsets in the parameter vector
-        gl::cnn::u16* wt;          // [cout][kp]
-        gl::cnn::u16* w2t;         // [cin][9 cout] (layers > 0)
-        gl::cnn::u16* Y;           // [B][Ho][Wo][cout]
-        gl::cnn::u16* Z;
-        gl::cnn::u16* dZ;
-    };
-    int H, W, d, hidden, maxb, P;
-    float in_scale;
-    Layer L[4];
-    size_t w1, b1, w2, b2;
-    float *params, *m, *v, *grad;
-    gl::cnn::u16* x0;
-    float *h0, *h1, *da1, *out, *dout, *lossb, *part;
-    size_t part_elems;
-    std::vector<void*> dev;
-};
-
-extern "C" int gl_cnn_destroy(gl_cnn* net) {
-    if (!net) return GL_OK;
-    for (void* p : net->dev) (void)hipFree(p);
-    delete net;
-    return GL_OK;
-}
-
-static int cnn_alloc(gl_cnn* net, size_t bytes, void** out) {
-    void* d = nullptr;
-    GL_HIP(hipMalloc(&d, bytes ? bytes : 4));
-    net->dev.push_back(d);
-    GL_HIP(hipMemset(d, 0, bytes ? bytes : 4));
-    *out = d;
-    return GL_OK;
-}
-
-// split-K factor of the weight-gradient GEMM: about 1024 workgroups
-static int cnn_wsplit(const gl_cnn::Layer& l, int bs, int* kchunk) {
-    const int K = bs * l.Ho * l.Wo;
-    const int tiles = ((l.cout + 63) / 64) * ((9 * l.cin + 1 + 63) / 64);
-    int splits = std::max(1, std::min(1024 / std::max(tiles, 1), (K + 1023) / 1024));
-    int kc = (K + splits - 1) / splits;
-    kc = (kc + 31) / 32 * 32;
-    splits = (K + kc - 1) / kc;
-    *kchunk = kc;
-    return splits;
-}
-
-extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
-    if (out) *out = nullptr;
-    if (!desc || !out) return fail(GL_ERR_ARG, "gl_cnn_create: null argument");
-    if (desc->height < 4 || desc->width < 4 || desc->max_batch <= 0)
-        return fail(GL_ERR_ARG, "gl_cnn_create: bad image size or max_batch");
-    for (int i = 0; i < 4; ++i)
-        if (desc->channels[i] <= 0 || desc->channels[i] % 32 || desc->channels[i] > gl::cnn::kHeadMaxC)
-            return fail(GL_ERR_ARG, "gl_cnn_create: channels must be positive multiples of 32, at most 256");
-    if (desc->hidden <= 0 || desc->hidden > gl::cnn::kHeadMaxHd) return fail(GL_ERR_ARG, "gl_cnn_create: hidden must be in [1, 512]");
-    if (desc->n_out <= 0 || 2 * desc->n_out > gl::cnn::kHeadMaxOut) return fail(GL_ERR_ARG, "gl_cnn_create: n_out must be in [1, 128]");
-    if (!(desc->in_scale > 0.0f)) return fail(GL_ERR_ARG, "gl_cnn_create: in_scale must be positive");
-    gl_cnn* net = new gl_cnn();
-    net->H = desc->height; net->W = desc->width; net->d = desc->n_out; net->hidden = desc->hidden;
-    net->maxb = desc->max_batch; net->in_scale = desc->in_scale;
-    const int strides[4] = {1, 2, 1, 2};
-    size_t off = 0;
-    int H = net->H, W = net->W, cin = 1;
-    for (int l = 0; l < 4; ++l) {
-        gl_cnn::Layer& L = net->L[l];
-        L.cin = cin; L.cout = desc->channels[l]; L.s = strides[l];
-        L.H = H; L.W = W;
-        L.Ho = (H - 1) / L.s + 1; L.Wo = (W - 1) / L.s + 1;
-        L.kp = (9 * cin + 31) / 32 * 32;
-        L.woff = off; off += (size_t)L.cout * 9 * cin;
-        L.boff = off; off += L.cout;
-        H = L.Ho; W = L.Wo; cin = L.cout;
-    }
-    const int c4 = net->L[3].cout, Hd = net->hidden, d2 = 2 * net->d;
-    net->w1 = off; off += (size_t)Hd * c4;
-    net->b1 = off; off += Hd;
-    net->w2 = off; off += (size_t)d2 * Hd;
-    net->b2 = off; off += d2;
-    net->P = (int)off;
-    const size_t B = net->maxb;
-    void* p = nullptr;
-    int rc;
-#define CNN_ALLOC(field, type, count)                                                  \
-    if ((rc = cnn_alloc(net, (size_t)(count) * sizeof(type), &p))) { gl_cnn_destroy(net); return rc; } \
-    field = (type*)p;
-    CNN_ALLOC(net->params, float, net->P);
-    CNN_ALLOC(net->m, float, net->P);
-    CNN_ALLOC(net->v, float, net->P);
-    CNN_ALLOC(net->grad, float, net->P);
-    CNN_ALLOC(net->x0, gl::cnn::u16, B * net->H * net->W);
-    size_t part = 0;
-    for (int l = 0; l < 4; ++l) {
-        gl_cnn::Layer& L = net->L[l];
-        const size_t act = B * L.Ho * L.Wo * L.cout;
-        CNN_ALLOC(L.wt, gl::cnn::u16, (size_t)L.cout * L.kp);
-        L.w2t = nullptr;
-        if (l > 0) { CNN_ALLOC(L.w2t, gl::cnn::u16, (size_t)L.cin * 9 * L.cout); }
-        CNN_ALLOC(L.Y, gl::cnn::u16, act);
-        CNN_ALLOC(L.Z, gl::cnn::u16, act);
-        CNN_ALLOC(L.dZ, gl::cnn::u16, act);
-        int kc;
-        const int sp = cnn_wsplit(L, (int)B, &kc);
-        part = std::max(part, (size_t)sp * L.cout * (9 * L.cin + 1));
-    }
-    // the head's weight-gradient chunks share the partial buffer
-    part = std::max(part, (size_t)gl::cnn::kHeadChunks * ((size_t)Hd * c4 + Hd + (size_t)d2 * Hd + d2));
-    net->part_elems = part;
-    CNN_ALLOC(net->part, float, part);
-    CNN_ALLOC(net->h0, float, B * c4);
-    CNN_ALLOC(net->h1, float, B * Hd);
-    CNN_ALLOC(net->da1, float, B * Hd);
-    CNN_ALLOC(net->out, float, B * d2);
-    CNN_ALLOC(net->dout, float, B * d2);
-    CNN_ALLOC(net->lossb, float, B);
-#undef CNN_ALLOC
-    if (hipDeviceSynchronize() != hipSuccess) { gl_cnn_destroy(net); return fail(GL_ERR_HIP, "gl_cnn_create: sync"); }
-    *out = net;
-    return GL_OK;
-}
-
-extern "C" int gl_cnn_num_params(const gl_cnn* net) { return net ? net->P : -1; }
-
-static int cnn_pack(gl_cnn* net, hipStream_t st) {
-    for (int l = 0; l < 4; ++l) {
-        const gl_cnn::Layer& L = net->L[l];
-        const int n = L.cout * L.kp;
-        hipLaunchKernelGGL(gl::cnn::pack_weights, dim3((n + 255) / 256), dim3(256), 0, st, net->params + L.woff, L.cin,
-                           L.cout, L.kp, L.wt, L.w2t);
-        GL_HIP(hipGetLastError());
-    }
-    return GL_OK;
-}
-
-extern "C" int gl_cnn_set_params(gl_cnn* net, const float* params, hipStream_t stream) {
-    if (!net || !params) return fail(GL_ERR_ARG, "gl_cnn_set_params: null argument");
-    GL_HIP(hipMemcpyAsync(net->params, params, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
-    GL_HIP(hipMemsetAsync(net->m, 0, (size_t)net->P * 4, stream));
-    GL_HIP(hipMemsetAsync(net->v, 0, (size_t)net->P * 4, stream));
-    return cnn_pack(net, stream);
-}
-
-extern "C" int gl_cnn_get_params(const gl_cnn* net, float* params, hipStream_t stream) {
-    if (!net || !params) return fail(GL_ERR_ARG, "gl_cnn_get_params: null argument");
-    GL_HIP(hipMemcpyAsync(params, net->params, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
-    return GL_OK;
-}
-
-static gl::cnn::HeadArgs cnn_head_args(gl_cnn* net, const float* target, int bs) {
-    gl::cnn::HeadArgs A;
-    const gl_cnn::Layer& L = net->L[3];
-    A.y4 = L.Y; A.z4 = L.Z; A.P = L.Ho * L.Wo; A.C = L.cout; A.Hd = net->hidden; A.d = net->d;
-    A.w1 = net->params + net->w1; A.b1 = net->params + net->b1;
-    A.w2 = net->params + net->w2; A.b2 = net->params + net->b2;
-    A.h0 = net->h0; A.h1 = net->h1; A.da1 = net->da1; A.out = net->out; A.dout = net->dout;
-    A.lossb = net->lossb; A.dz4 = L.dZ; A.target = target; A.inv_bs = 1.0f / (float)bs;
-    return A;
-}
-
-static int cnn_forward_layers(gl_cnn* net, const float* images, int bs, hipStream_t st) {
-    using namespace gl::cnn;
-    const long n = (long)bs * net->H * net->W;
-    hipLaunchKernelGGL(input_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, images, net->x0, n,
-                       1.0f / net->in_scale);
-    GL_HIP(hipGetLastError());
-    const u16* x = net->x0;
-    for (int l = 0; l < 4; ++l) {
-        const gl_cnn::Layer& L = net->L[l];
-        const int M = bs * L.Ho * L.Wo, N = L.cout, K = L.kp;
-        LdIm2col la{x, L.H, L.W, L.cin, L.Ho, L.Wo, L.s, 0};
-        LdPlain lb{L.wt, L.kp, L.kp};
-        EpFwd ep{net->params + L.boff, L.Z, L.Y, N};
-        hipLaunchKernelGGL((gemm<true, true, LdIm2col, LdPlain, EpFwd>), dim3((M + 63) / 64, (N + 63) / 64, 1),
-                           dim3(256), 0, st, la, lb, ep, M, N, K, K);
-        GL_HIP(hipGetLastError());
-        x = L.Y;
-    }
-    return GL_OK;
-}
-
-extern "C" int gl_cnn_forward(gl_cnn* net, const float* images, int bs, float* out, hipStream_t stream) {
-    if (!net || !images || !out) return fail(GL_ERR_ARG, "gl_cnn_forward: null argument");
-    if (bs <= 0 || bs > net->maxb) return fail(GL_ERR_ARG, "gl_cnn_forward: bs must be in [1, max_batch]");
-    int rc;
-    if ((rc = cnn_forward_layers(net, images, bs, stream))) return rc;
-    gl::cnn::HeadArgs A = cnn_head_args(net, nullptr, bs);
-    A.out = out;
-    hipLaunchKernelGGL(gl::cnn::head_kernel, dim3(bs), dim3(256), 0, stream, A);
-    GL_HIP(hipGetLastError());
-    return GL_OK;
-}
-
-extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* target, int bs, float* loss,
-                                float* grad, hipStream_t stream) {
-    using namespace gl::cnn;
-    if (!net || !images || !target) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: null argument");
-    if (bs <= 0 || bs > net->maxb) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: bs must be in [1, max_batch]");
-    int rc;
-    if ((rc = cnn_forward_layers(net, images, bs, stream))) return rc;
-    HeadArgs A = cnn_head_args(net, target, bs);
-    hipLaunchKernelGGL(head_kernel, dim3(bs), dim3(256), 0, stream, A);
-    GL_HIP(hipGetLastError());
-    {
-        const int n = net->hidden * net->L[3].cout + net->hidden + 2 * net->d * net->hidden + 2 * net->d;
-        // the layers' weight-gradient partials reuse net->part after this (stream order)
-        hipLaunchKernelGGL(head_wgrad, dim3((n + 255) / 256, kHeadChunks), dim3(256), 0, stream, A, bs, net->part);
-        GL_HIP(hipGetLastError());
-        hipLaunchKernelGGL(head_wgrad_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, A, net->part,
-                           net->grad + net->w1, net->grad + net->b1, net->grad + net->w2, net->grad + net->b2);
-        GL_HIP(hipGetLastError());
-    }
-    if (loss) {
-        hipLaunchKernelGGL(loss_mean, dim3(1), dim3(256), 0, stream, net->lossb, bs, loss);
-        GL_HIP(hipGetLastError());
-    }
-    for (int l = 3; l >= 0; --l) {
-        const gl_cnn::Layer& L = net->L[l];
-        const u16* x = l > 0 ? net->L[l - 1].Y : net->x0;
-        // weight (and bias) gradient: C[co][kk] = sum_m dZ[m][co] X~(m, kk)
-        {
-            int kc;
-            const int splits = cnn_wsplit(L, bs, &kc);
-            const int M = L.cout, N = 9 * L.cin + 1, K = bs * L.Ho * L.Wo;
-            if ((size_t)splits * M * N > net->part_elems) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: partial buffer");
-            LdPlain la{L.dZ, L.cout, L.cout};
-            LdIm2col lb{x, L.H, L.W, L.cin, L.Ho, L.Wo, L.s, 1};
-            EpPartial ep{net->part, M, N};
-            hipLaunchKernelGGL((gemm<false, false, LdPlain, LdIm2col, EpPartial>),
-                               dim3((M + 63) / 64, (N + 63) / 64, splits), dim3(256), 0, stream, la, lb, ep, M, N, K, kc);
-            GL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(wgrad_reduce, dim3((M * N + kWrOut - 1) / kWrOut), dim3(256), 0, stream, net->part,
-                               splits, M, N,
-                               net->grad + L.woff, net->grad + L.boff);
-            GL_HIP(hipGetLastError());
-        }
-        if (l == 0) break;
-        // data gradient into the previous layer: dZprev = (sum_k dZ~ W2T) * gelu'(Zprev)
-        {
-            const gl_cnn::Layer& Lp = net->L[l - 1];
-            const int M = bs * L.H * L.W, N = L.cin, K = 9 * L.cout;
-            LdDzGather la{L.dZ, L.H, L.W, L.Ho, L.Wo, L.cout, L.s};
-            LdPlain lb{L.w2t, 9 * L.cout, 9 * L.cout};
-            EpDataGrad ep{Lp.Z, Lp.dZ, N};
-            hipLaunchKernelGGL((gemm<true, true, LdDzGather, LdPlain, EpDataGrad>),
-                               dim3((M + 63) / 64, (N + 63) / 64, 1), dim3(256), 0, stream, la, lb, ep, M, N, K, K);
-            GL_HIP(hipGetLastError());
-        }
-    }
-    if (grad) GL_HIP(hipMemcpyAsync(grad, net->grad, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
-    return GL_OK;
-}
-
-extern "C" int gl_cnn_adamw_step(gl_cnn* net, int step, float lr, float beta1, float beta2, float eps,
-                                 float weight_decay, hipStream_t stream) {
-    if (!net) return fail(GL_ERR_ARG, "gl_cnn_adamw_step: null network");
-    if (step < 1) return fail(GL_ERR_ARG, "gl_cnn_adamw_step: step is 1-based");
-    const float bc1 = (float)(1.0 - pow((double)beta1, step));
-    const float bc2s = (float)sqrt(1.0 - pow((double)beta2, step));
-    hipLaunchKernelGGL(gl::cnn::adamw, dim3((net->P + 255) / 256), dim3(256), 0, stream, net->params, net->m, net->v,
-                       net->grad, net->P, lr, beta1, beta2, eps, weight_decay, bc1, bc2s);
-    GL_HIP(hipGetLastError());
-    return cnn_pack(net, stream);
+    return mc_pipeline_run(true, b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream, false, b->mc_model,
+                           &b->groups);
 }

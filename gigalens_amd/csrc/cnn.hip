@@ -1,0 +1,283 @@
+// C ABI of the learned initialiser's network (gl_cnn_*): workspace, parameter
+// packing and the launch sequences of the BF16 MFMA kernels in cnn_kernels.hip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "abi_common.h"
+#include "cnn_kernels.hip"
+
+using glc::fail;
+
+// ---------------------------------------------------------------------------
+// Learned initialiser (no reference counterpart)
+// ---------------------------------------------------------------------------
+struct gl_cnn {
+    struct Layer {
+        int cin, cout, s, H, W, Ho, Wo, kp;
+        size_t woff, boff;         // offsets in the parameter vector
+        gl::cnn::u16* wt;          // [cout][kp]
+        gl::cnn::u16* w2t;         // [cin][9 cout] (layers > 0)
+        gl::cnn::u16* Y;           // [B][Ho][Wo][cout]
+        gl::cnn::u16* Z;
+        gl::cnn::u16* dZ;
+    };
+    int H, W, d, hidden, maxb, P;
+    float in_scale;
+    Layer L[4];
+    size_t w1, b1, w2, b2;
+    float *params, *m, *v, *grad;
+    gl::cnn::u16* x0;
+    float *h0, *h1, *da1, *out, *dout, *lossb, *part;
+    size_t part_elems;
+    std::vector<void*> dev;
+};
+
+extern "C" int gl_cnn_destroy(gl_cnn* net) {
+    if (!net) return GL_OK;
+    for (void* p : net->dev) (void)hipFree(p);
+    delete net;
+    return GL_OK;
+}
+
+static int cnn_alloc(gl_cnn* net, size_t bytes, void** out) {
+    void* d = nullptr;
+    GL_HIP(hipMalloc(&d, bytes ? bytes : 4));
+    net->dev.push_back(d);
+    GL_HIP(hipMemset(d, 0, bytes ? bytes : 4));
+    *out = d;
+    return GL_OK;
+}
+
+// split-K factor of the weight-gradient GEMM: about 1024 workgroups
+static int cnn_wsplit(const gl_cnn::Layer& l, int bs, int* kchunk) {
+    const int K = bs * l.Ho * l.Wo;
+    const int tiles = ((l.cout + 63) / 64) * ((9 * l.cin + 1 + 63) / 64);
+    int splits = std::max(1, std::min(1024 / std::max(tiles, 1), (K + 1023) / 1024));
+    int kc = (K + splits - 1) / splits;
+    kc = (kc + 31) / 32 * 32;
+    splits = (K + kc - 1) / kc;
+    *kchunk = kc;
+    return splits;
+}
+
+extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
+    if (out) *out = nullptr;
+    if (!desc || !out) return fail(GL_ERR_ARG, "gl_cnn_create: null argument");
+    if (desc->height < 4 || desc->width < 4 || desc->max_batch <= 0)
+        return fail(GL_ERR_ARG, "gl_cnn_create: bad image size or max_batch");
+    for (int i = 0; i < 4; ++i)
+        if (desc->channels[i] <= 0 || desc->channels[i] % 32 || desc->channels[i] > gl::cnn::kHeadMaxC)
+            return fail(GL_ERR_ARG, "gl_cnn_create: channels must be positive multiples of 32, at most 256");
+    if (desc->hidden <= 0 || desc->hidden > gl::cnn::kHeadMaxHd) return fail(GL_ERR_ARG, "gl_cnn_create: hidden must be in [1, 512]");
+    if (desc->n_out <= 0 || 2 * desc->n_out > gl::cnn::kHeadMaxOut) return fail(GL_ERR_ARG, "gl_cnn_create: n_out must be in [1, 128]");
+    if (!(desc->in_scale > 0.0f)) return fail(GL_ERR_ARG, "gl_cnn_create: in_scale must be positive");
+    gl_cnn* net = new gl_cnn();
+    net->H = desc->height; net->W = desc->width; net->d = desc->n_out; net->hidden = desc->hidden;
+    net->maxb = desc->max_batch; net->in_scale = desc->in_scale;
+    const int strides[4] = {1, 2, 1, 2};
+    size_t off = 0;
+    int H = net->H, W = net->W, cin = 1;
+    for (int l = 0; l < 4; ++l) {
+        gl_cnn::Layer& L = net->L[l];
+        L.cin = cin; L.cout = desc->channels[l]; L.s = strides[l];
+        L.H = H; L.W = W;
+        L.Ho = (H - 1) / L.s + 1; L.Wo = (W - 1) / L.s + 1;
+        L.kp = (9 * cin + 31) / 32 * 32;
+        L.woff = off; off += (size_t)L.cout * 9 * cin;
+        L.boff = off; off += L.cout;
+        H = L.Ho; W = L.Wo; cin = L.cout;
+    }
+    const int c4 = net->L[3].cout, Hd = net->hidden, d2 = 2 * net->d;
+    net->w1 = off; off += (size_t)Hd * c4;
+    net->b1 = off; off += Hd;
+    net->w2 = off; off += (size_t)d2 * Hd;
+    net->b2 = off; off += d2;
+    net->P = (int)off;
+    const size_t B = net->maxb;
+    void* p = nullptr;
+    int rc;
+#define CNN_ALLOC(field, type, count)                                                  \
+    if ((rc = cnn_alloc(net, (size_t)(count) * sizeof(type), &p))) { gl_cnn_destroy(net); return rc; } \
+    field = (type*)p;
+    CNN_ALLOC(net->params, float, net->P);
+    CNN_ALLOC(net->m, float, net->P);
+    CNN_ALLOC(net->v, float, net->P);
+    CNN_ALLOC(net->grad, float, net->P);
+    CNN_ALLOC(net->x0, gl::cnn::u16, B * net->H * net->W);
+    size_t part = 0;
+    for (int l = 0; l < 4; ++l) {
+        gl_cnn::Layer& L = net->L[l];
+        const size_t act = B * L.Ho * L.Wo * L.cout;
+        CNN_ALLOC(L.wt, gl::cnn::u16, (size_t)L.cout * L.kp);
+        L.w2t = nullptr;
+        if (l > 0) { CNN_ALLOC(L.w2t, gl::cnn::u16, (size_t)L.cin * 9 * L.cout); }
+        CNN_ALLOC(L.Y, gl::cnn::u16, act);
+        CNN_ALLOC(L.Z, gl::cnn::u16, act);
+        CNN_ALLOC(L.dZ, gl::cnn::u16, act);
+        int kc;
+        const int sp = cnn_wsplit(L, (int)B, &kc);
+        part = std::max(part, (size_t)sp * L.cout * (9 * L.cin + 1));
+    }
+    // the head's weight-gradient chunks share the partial buffer
+    part = std::max(part, (size_t)gl::cnn::kHeadChunks * ((size_t)Hd * c4 + Hd + (size_t)d2 * Hd + d2));
+    net->part_elems = part;
+    CNN_ALLOC(net->part, float, part);
+    CNN_ALLOC(net->h0, float, B * c4);
+    CNN_ALLOC(net->h1, float, B * Hd);
+    CNN_ALLOC(net->da1, float, B * Hd);
+    CNN_ALLOC(net->out, float, B * d2);
+    CNN_ALLOC(net->dout, float, B * d2);
+    CNN_ALLOC(net->lossb, float, B);
+#undef CNN_ALLOC
+    if (hipDeviceSynchronize() != hipSuccess) { gl_cnn_destroy(net); return fail(GL_ERR_HIP, "gl_cnn_create: sync"); }
+    *out = net;
+    return GL_OK;
+}
+
+extern "C" int gl_cnn_num_params(const gl_cnn* net) { return net ? net->P : -1; }
+
+static int cnn_pack(gl_cnn* net, hipStream_t st) {
+    for (int l = 0; l < 4; ++l) {
+        const gl_cnn::Layer& L = net->L[l];
+        const int n = L.cout * L.kp;
+        hipLaunchKernelGGL(gl::cnn::pack_weights, dim3((n + 255) / 256), dim3(256), 0, st, net->params + L.woff, L.cin,
+                           L.cout, L.kp, L.wt, L.w2t);
+        GL_HIP(hipGetLastError());
+    }
+    return GL_OK;
+}
+
+extern "C" int gl_cnn_set_params(gl_cnn* net, const float* params, hipStream_t stream) {
+    if (!net || !params) return fail(GL_ERR_ARG, "gl_cnn_set_params: null argument");
+    GL_HIP(hipMemcpyAsync(net->params, params, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
+    GL_HIP(hipMemsetAsync(net->m, 0, (size_t)net->P * 4, stream));
+    GL_HIP(hipMemsetAsync(net->v, 0, (size_t)net->P * 4, stream));
+    return cnn_pack(net, stream);
+}
+
+extern "C" int gl_cnn_get_params(const gl_cnn* net, float* params, hipStream_t stream) {
+    if (!net || !params) return fail(GL_ERR_ARG, "gl_cnn_get_params: null argument");
+    GL_HIP(hipMemcpyAsync(params, net->params, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
+    return GL_OK;
+}
+
+static gl::cnn::HeadArgs cnn_head_args(gl_cnn* net, const float* target, int bs) {
+    gl::cnn::HeadArgs A;
+    const gl_cnn::Layer& L = net->L[3];
+    A.y4 = L.Y; A.z4 = L.Z; A.P = L.Ho * L.Wo; A.C = L.cout; A.Hd = net->hidden; A.d = net->d;
+    A.w1 = net->params + net->w1; A.b1 = net->params + net->b1;
+    A.w2 = net->params + net->w2; A.b2 = net->params + net->b2;
+    A.h0 = net->h0; A.h1 = net->h1; A.da1 = net->da1; A.out = net->out; A.dout = net->dout;
+    A.lossb = net->lossb; A.dz4 = L.dZ; A.target = target; A.inv_bs = 1.0f / (float)bs;
+    return A;
+}
+
+static int cnn_forward_layers(gl_cnn* net, const float* images, int bs, hipStream_t st) {
+    using namespace gl::cnn;
+    const long n = (long)bs * net->H * net->W;
+    hipLaunchKernelGGL(input_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, images, net->x0, n,
+                       1.0f / net->in_scale);
+    GL_HIP(hipGetLastError());
+    const u16* x = net->x0;
+    for (int l = 0; l < 4; ++l) {
+        const gl_cnn::Layer& L = net->L[l];
+        const int M = bs * L.Ho * L.Wo, N = L.cout, K = L.kp;
+        LdIm2col la{x, L.H, L.W, L.cin, L.Ho, L.Wo, L.s, 0};
+        LdPlain lb{L.wt, L.kp, L.kp};
+        EpFwd ep{net->params + L.boff, L.Z, L.Y, N};
+        hipLaunchKernelGGL((gemm<true, true, LdIm2col, LdPlain, EpFwd>), dim3((M + 63) / 64, (N + 63) / 64, 1),
+                           dim3(256), 0, st, la, lb, ep, M, N, K, K);
+        GL_HIP(hipGetLastError());
+        x = L.Y;
+    }
+    return GL_OK;
+}
+
+extern "C" int gl_cnn_forward(gl_cnn* net, const float* images, int bs, float* out, hipStream_t stream) {
+    if (!net || !images || !out) return fail(GL_ERR_ARG, "gl_cnn_forward: null argument");
+    if (bs <= 0 || bs > net->maxb) return fail(GL_ERR_ARG, "gl_cnn_forward: bs must be in [1, max_batch]");
+    int rc;
+    if ((rc = cnn_forward_layers(net, images, bs, stream))) return rc;
+    gl::cnn::HeadArgs A = cnn_head_args(net, nullptr, bs);
+    A.out = out;
+    hipLaunchKernelGGL(gl::cnn::head_kernel, dim3(bs), dim3(256), 0, stream, A);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* target, int bs, float* loss,
+                                float* grad, hipStream_t stream) {
+    using namespace gl::cnn;
+    if (!net || !images || !target) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: null argument");
+    if (bs <= 0 || bs > net->maxb) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: bs must be in [1, max_batch]");
+    int rc;
+    if ((rc = cnn_forward_layers(net, images, bs, stream))) return rc;
+    HeadArgs A = cnn_head_args(net, target, bs);
+    hipLaunchKernelGGL(head_kernel, dim3(bs), dim3(256), 0, stream, A);
+    GL_HIP(hipGetLastError());
+    {
+        const int n = net->hidden * net->L[3].cout + net->hidden + 2 * net->d * net->hidden + 2 * net->d;
+        // the layers' weight-gradient partials reuse net->part after this (stream order)
+        hipLaunchKernelGGL(head_wgrad, dim3((n + 255) / 256, kHeadChunks), dim3(256), 0, stream, A, bs, net->part);
+        GL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(head_wgrad_reduce, dim3((n + 255) / 256), dim3(256), 0, stream, A, net->part,
+                           net->grad + net->w1, net->grad + net->b1, net->grad + net->w2, net->grad + net->b2);
+        GL_HIP(hipGetLastError());
+    }
+    if (loss) {
+        hipLaunchKernelGGL(loss_mean, dim3(1), dim3(256), 0, stream, net->lossb, bs, loss);
+        GL_HIP(hipGetLastError());
+    }
+    for (int l = 3; l >= 0; --l) {
+        const gl_cnn::Layer& L = net->L[l];
+        const u16* x = l > 0 ? net->L[l - 1].Y : net->x0;
+        // weight (and bias) gradient: C[co][kk] = sum_m dZ[m][co] X~(m, kk)
+        {
+            int kc;
+            const int splits = cnn_wsplit(L, bs, &kc);
+            const int M = L.cout, N = 9 * L.cin + 1, K = bs * L.Ho * L.Wo;
+            if ((size_t)splits * M * N > net->part_elems) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: partial buffer");
+            LdPlain la{L.dZ, L.cout, L.cout};
+            LdIm2col lb{x, L.H, L.W, L.cin, L.Ho, L.Wo, L.s, 1};
+            EpPartial ep{net->part, M, N};
+            hipLaunchKernelGGL((gemm<false, false, LdPlain, LdIm2col, EpPartial>),
+                               dim3((M + 63) / 64, (N + 63) / 64, splits), dim3(256), 0, stream, la, lb, ep, M, N, K, kc);
+            GL_HIP(hipGetLastError());
+            hipLaunchKernelGGL(wgrad_reduce, dim3((M * N + kWrOut - 1) / kWrOut), dim3(256), 0, stream, net->part,
+                               splits, M, N,
+                               net->grad + L.woff, net->grad + L.boff);
+            GL_HIP(hipGetLastError());
+        }
+        if (l == 0) break;
+        // data gradient into the previous layer: dZprev = (sum_k dZ~ W2T) * gelu'(Zprev)
+        {
+            const gl_cnn::Layer& Lp = net->L[l - 1];
+            const int M = bs * L.H * L.W, N = L.cin, K = 9 * L.cout;
+            LdDzGather la{L.dZ, L.H, L.W, L.Ho, L.Wo, L.cout, L.s};
+            LdPlain lb{L.w2t, 9 * L.cout, 9 * L.cout};
+            EpDataGrad ep{Lp.Z, Lp.dZ, N};
+            hipLaunchKernelGGL((gemm<true, true, LdDzGather, LdPlain, EpDataGrad>),
+                               dim3((M + 63) / 64, (N + 63) / 64, 1), dim3(256), 0, stream, la, lb, ep, M, N, K, K);
+            GL_HIP(hipGetLastError());
+        }
+    }
+    if (grad) GL_HIP(hipMemcpyAsync(grad, net->grad, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
+    return GL_OK;
+}
+
+extern "C" int gl_cnn_adamw_step(gl_cnn* net, int step, float lr, float beta1, float beta2, float eps,
+                                 float weight_decay, hipStream_t stream) {
+    if (!net) return fail(GL_ERR_ARG, "gl_cnn_adamw_step: null network");
+    if (step < 1) return fail(GL_ERR_ARG, "gl_cnn_adamw_step: step is 1-based");
+    const float bc1 = (float)(1.0 - pow((double)beta1, step));
+    const float bc2s = (float)sqrt(1.0 - pow((double)beta2, step));
+    hipLaunchKernelGGL(gl::cnn::adamw, dim3((net->P + 255) / 256), dim3(256), 0, stream, net->params, net->m, net->v,
+                       net->grad, net->P, lr, beta1, beta2, eps, weight_decay, bc1, bc2s);
+    GL_HIP(hipGetLastError());
+    return cnn_pack(net, stream);
+}

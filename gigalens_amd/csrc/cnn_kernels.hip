@@ -19,6 +19,7 @@
 // asinh(img / scale), the weights, Z = conv + bias and Y = gelu(Z) are
 // rounded to bfloat16; dZ = dY * gelu'(Z) is rounded to bfloat16; dY, the
 // weight gradients and the head are FP32.
+#pragma once
 #include <hip/hip_runtime.h>
 
 namespace gl {

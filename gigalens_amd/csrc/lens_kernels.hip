@@ -18,6 +18,7 @@
 // Reference semantics: src/gigalens/tf/simulator.py:57-97 (simulate),
 // src/gigalens/tf/model.py:53-86 (ForwardProbModel.log_prob),
 // src/gigalens/tf/inference.py:28-35 (MAP train_step).
+#pragma once
 #include <hip/hip_runtime.h>
 #include "lens_math.h"
 #include "lens_plan.h"
@@ -1035,6 +1036,7 @@ __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KA
 // rule evaluated in FP32 (an ordering estimate only).
 constexpr int kOrderCap = 16384;
 constexpr int kOrderBins = 3 * GL_MAX_EPL_ITER + 1;
+#ifdef GL_FUSED_AUX_KERNELS  // defined by the one translation unit that launches them
 __global__ __launch_bounds__(1024) void order_kernel(const DPlan P, const float* in, int bs, int use_z, int* order) {
     constexpr int kScanWaves = (kOrderBins + 63) / 64;
     __shared__ unsigned short key[kOrderCap];
@@ -1144,5 +1146,6 @@ __global__ void epl_batch_series_kernel(const DPlan P, const float* in, int bs, 
         __syncthreads();
     }
 }
+#endif  // GL_FUSED_AUX_KERNELS
 
 }  // namespace gl

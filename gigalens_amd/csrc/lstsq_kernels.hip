@@ -39,6 +39,7 @@
 //                 (least squares: amplitudes held at their optimum, envelope theorem)
 //   mc_finish     per particle: chain rule -> dlp/dz, optionally the Keras-Adam
 //                 update of a MAP step (tf/inference.py:28-35)
+#pragma once
 #include <hip/hip_runtime.h>
 #include "lens_math.h"
 #include "lens_plan.h"
