@@ -86,7 +86,7 @@ def cpu_baseline(obs, budget_s=20.0):
     cfg = wl.epl_shear_config()
     from gigalens_amd.simulator import Camera
     cam = Camera(cfg)
-    ocam = np64.Camera(cfg.delta_pix, cfg.num_pix, cfg.supersample, cfg.kernel, None, subgrid=cam.subgrid_kernel)
+    ocam = np64.Camera(cfg.delta_pix, cfg.num_pix, cfg.supersample, cfg.kernel, None)
     model = {"lenses": [("EPL", {"niter": 50}), ("SHEAR", {})], "lens_light": [("SERSIC_ELLIPSE", {})],
              "source_light": [("SERSIC_ELLIPSE", {})]}
     kind = {1: "Normal", 2: "LogNormal", 3: "Uniform", 4: "TruncatedNormal"}
@@ -148,7 +148,7 @@ def shapelet_cpu_baseline(prior, phys, cfg, sim, obs, z0, n=1):
     from oracle import torch_ref
     from oracle import np64
     cam = sim.camera
-    ocam = np64.Camera(cfg.delta_pix, cfg.num_pix, cfg.supersample, cfg.kernel, None, subgrid=cam.subgrid_kernel)
+    ocam = np64.Camera(cfg.delta_pix, cfg.num_pix, cfg.supersample, cfg.kernel, None)
     model = {"lenses": [("EPL", {"niter": 50}), ("SHEAR", {})], "lens_light": [("SERSIC_ELLIPSE", {})],
              "source_light": [("SHAPELETS", {"n_max": 10, "interpolate": phys.source_light[0].interpolate})]}
     kind = {1: "Normal", 2: "LogNormal", 3: "Uniform", 4: "TruncatedNormal"}

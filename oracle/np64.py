@@ -20,19 +20,24 @@ Pinning (what checks each function beyond retyping):
                           (tests/test_oracle_tnfw.py)
   * ``sersic_*``       -- known answer light(0,1)=Ie (tests/test_profiles.py:17-26)
                           and an independent rotated-ellipse construction
-  * ``core_sersic_light`` -- partly pinned: at Rb -> 0 and alpha = 1/n it is
-                          e^{-1-bn} times the (pinned) Sersic ellipse; at R = Rb its
-                          core factor is 2^{gamma/alpha}.  Beyond these reductions the
-                          alpha dependence follows the reference expression itself
-                          (parity unpinned: nothing else fixes it).
+  * ``core_sersic_light`` -- at Rb -> 0 and alpha = 1/n it is e^{-1-bn} times the
+                          (pinned) Sersic ellipse; at R = Rb its core factor is
+                          2^{gamma/alpha}; at general alpha the Rb -> 0 log-profile has
+                          log-log slope alpha and value bn/(alpha n) at R = R_sersic, the
+                          inner logarithmic slope tends to -gamma and the outer core
+                          factor to 1 + (gamma/alpha)(Rb/R)^alpha (tests/test_oracle.py)
   * ``shapelet_*``     -- numpy.polynomial.hermite, orthonormality integrals
   * ``conv_pool``      -- scipy.signal.convolve2d + block mean
   * ``log_prob_*``     -- scipy.stats log-pdfs, closed-form log-Jacobians
   * ``lstsq``          -- numpy.linalg.pinv / lstsq
-  * ``subgrid_kernel`` -- parity unpinned against lenstronomy (absent); pinned
-                          only by invariants (normalisation, re-averaging
-                          reproduces the input, symmetry) and statistically by
-                          the reference's demo image.
+  * ``subgrid_kernel`` -- lenstronomy 1.9.3 (absent here): its re-sampling equals
+                          scipy RegularGridInterpolator, its even-factor averaging the
+                          sub-pixel area overlap, its even-ss output the directly solved
+                          fixed point of "re-averaging reproduces the input", its odd-ss
+                          output re-averages exactly; planted mistakes (nearest
+                          up-sampling, shifted grid, no correction, whole-pixel
+                          averaging) each fail a check (tests/test_oracle_subgrid.py).
+                          lenstronomy's own floating-point output is not available.
 """
 from __future__ import annotations
 
@@ -129,7 +134,8 @@ def subgrid_kernel(kernel: np.ndarray, subgrid_res: int, odd: bool = True, num_i
     Restates lenstronomy 1.9.3 ``Util.kernel_util.subgrid_kernel`` (third-party,
     not vendored in the reference): bilinear up-sampling, then ``num_iter``
     corrections so that re-averaging to the coarse grid reproduces the input,
-    normalised to unit sum.  Parity unpinned against lenstronomy itself.
+    normalised to unit sum.  Pinned by independent constructions
+    (tests/test_oracle_subgrid.py); lenstronomy's own output is not available.
     """
     kernel = np.asarray(kernel, dtype=np.float64)
     ss = int(subgrid_res)

@@ -23,7 +23,7 @@ def oracle_prior(prior):
 
 def oracle_camera(sim_config, camera: Camera):
     return np64.Camera(sim_config.delta_pix, sim_config.num_pix, sim_config.supersample, sim_config.kernel,
-                       sim_config.transform_pix2angle, subgrid=camera.subgrid_kernel)
+                       sim_config.transform_pix2angle)
 
 
 def nested_from_x(prior, x):

@@ -265,3 +265,40 @@ def test_core_sersic_reductions():
     full = np64.core_sersic_light(xs, ys, Rs, n, Rb, al, gam, 0.0, 0.0, cx, cy, Ie)[0, 0, 0]
     nocore = Ie * np.exp(-bn * (R ** al + Rb ** al) / Rs ** al / (al * n) - 1.0)
     assert full == pytest.approx(nocore * 2.0 ** (gam / al), rel=1e-12)
+
+
+@pytest.mark.parametrize("alpha", [0.5, 1.3, 2.0, 3.5])
+def test_core_sersic_general_alpha(alpha):
+    """Core-Sersic restatement at general alpha, from properties of the profile rather
+    than its expression: with Rb -> 0 the log-profile is ln Ie - 1 - b (R/Rs)^alpha, so
+    -(ln I - ln Ie + 1) has log-log slope alpha and equals b = bn / (alpha n) at R = Rs;
+    inside the core (R << Rb) the logarithmic slope d ln I / d ln R tends to -gamma;
+    outside it (R >> Rb) the core factor tends to 1 + (gamma/alpha) (Rb/R)^alpha."""
+    # centre at the origin so R = x exactly (a shifted centre loses R's low digits at R << 1)
+    Rs, n, gam, Ie, cx, cy = 0.7, 2.5, 0.6, 3.0, 0.0, 0.0
+    bn = 1.9992 * n - 0.3271
+    b = bn / (alpha * n)
+
+    def light(R, Rb):
+        xs, ys = np.array([[cx + R]]), np.array([[cy]])
+        return float(np64.core_sersic_light(xs, ys, Rs, n, Rb, alpha, gam, 0.0, 0.0, cx, cy, Ie)[0, 0, 0])
+
+    # Rb -> 0: log-log slope alpha (central differences in ln R) and the value at R = Rs
+    for R in (0.1, 0.5, 1.3):
+        h = 1e-4
+        fp = -(math.log(light(R * math.exp(h), 1e-300)) - math.log(Ie) + 1.0)
+        fm = -(math.log(light(R * math.exp(-h), 1e-300)) - math.log(Ie) + 1.0)
+        assert (math.log(fp) - math.log(fm)) / (2 * h) == pytest.approx(alpha, rel=1e-7)
+    assert -(math.log(light(Rs, 1e-300)) - math.log(Ie) + 1.0) == pytest.approx(b, rel=1e-12)
+    # inner logarithmic slope -> -gamma
+    Rb = 0.4
+    R, h = 1e-12, 1e-4
+    slope = (math.log(light(R * math.exp(h), Rb)) - math.log(light(R * math.exp(-h), Rb))) / (2 * h)
+    assert slope == pytest.approx(-gam, rel=1e-4)
+    # outer core factor
+    R = 4.0
+    ratio = light(R, Rb) / light(R, 1e-300)
+    # the Rb^alpha term of the exponent is part of the core factor at finite Rb
+    ratio *= math.exp(b * (Rb / Rs) ** alpha)
+    x = (Rb / R) ** alpha
+    assert ratio - 1.0 == pytest.approx(gam / alpha * x, rel=abs(gam / alpha - 1.0) * x + 1e-9)
