@@ -109,7 +109,9 @@ GLM int epl_series_len(double f, int niter) {
 }
 
 // EPL constants, epl.py:25-30,37,59.  b = theta_E sqrt(q) is the algebraic
-// simplification of epl.py:28-29.
+// simplification of epl.py:28-29.  The prefactor 2b/(1+q) (b/R)^(t-1) is evaluated
+// as exp2(K0 + (1-t) log2 R); at b = 0 (|e| >= 1 clips q to 0) the reference's product
+// 0 * 0^(t-1) is NaN for t < 1 (gamma < 2) and 0 otherwise, so K0 is NaN there too.
 GLM void epl_prepare(float thetaE, float gamma, float e1, float e2, float cx, float cy, int niter,
                      int n_override, EplC& C) {
     float phi = atan2f(e2, e1) * 0.5f;
@@ -125,6 +127,7 @@ GLM void epl_prepare(float thetaE, float gamma, float e1, float e2, float cx, fl
     C.t = t; C.tm1 = t - 1.0f;
     C.lnb = logf(b);
     C.K0 = log2f(2.0f / (1.0f + q)) + t * log2f(b);
+    if (b == 0.0f && C.tm1 < 0.0f) C.K0 = __builtin_nanf("");
     C.thetaE = thetaE; C.e1 = e1; C.e2 = e2; C.c = c; C.craw = craw;
     C.N = n_override >= 0 ? n_override : epl_series_len((double)C.f, niter);
 }
@@ -369,8 +372,13 @@ GLM float sersic_forward(const SerC& C, float X, float Y, SerState& S) {
     float LQ = lg2(Q);
     float P, Pq = 0.0f;
     if (ADJ) {
-        Pq = ex2(fmaf(C.k1m1, LQ, C.k0));
-        P = Q > 0.0f ? Q * Pq : 0.0f;
+        // Q > 0: P = Q Pq.  Q < 0 (|e| > 1, q < 0): NaN like the forward pass, so the
+        // sub-pixel is skipped as zeroed.  Q = 0 (a sub-pixel on the centre): P = 0 and the
+        // radial terms vanish -- R is even in (dx, dy), so the symmetric derivative there
+        // is 0 (its one-sided derivatives are infinite for n > 1/2).
+        Pq = Q > 0.0f ? ex2(fmaf(C.k1m1, LQ, C.k0)) : 0.0f;
+        P = Q > 0.0f ? Q * Pq : (Q == 0.0f ? 0.0f : __builtin_nanf(""));
+        LQ = Q > 0.0f ? LQ : 0.0f;
     } else {
         P = ex2(fmaf(C.k1, LQ, C.k0));
     }

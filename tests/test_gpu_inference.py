@@ -72,6 +72,24 @@ def test_vi_lr0_unchanged_lr_positive_changes(cuda):
     assert losses.shape == (5,)
 
 
+def test_vi_diagonal_lr0_unchanged_lr_positive_changes(cuda):
+    """tests/tf/test_model.py:45-57 for the diagonal surrogate (full_rank=False): the
+    scale parameters are a vector, lr 0 leaves mean and scale unchanged."""
+    prior, phys, data = _default()
+    pm = ForwardProbModel(prior, *data)
+    cfg = SimulatorConfig(delta_pix=0.05, num_pix=20)
+    seq = ModellingSequence(phys, pm, cfg)
+    start = pm.bij.inverse(prior.sample(2, seed=1))[0]
+    q, losses = seq.SVI(Adam(0), start, n_vi=5, num_steps=5, full_rank=False)
+    assert q.raw.shape == (22,)
+    assert np.allclose(q.mean().cpu(), start, atol=1e-6)
+    assert np.allclose(q.stddev().cpu(), 1e-3, rtol=1e-5)
+    q, losses = seq.SVI(Adam(1e-3), start, n_vi=5, num_steps=5, full_rank=False)
+    assert not np.allclose(q.mean().cpu(), start)
+    assert not np.allclose(q.stddev().cpu(), 1e-3, rtol=1e-5)
+    assert losses.shape == (5,) and torch.isfinite(losses).all()
+
+
 def test_hmc_sample_count(cuda):
     """tests/tf/test_model.py:60-72"""
     prior, phys, data = _default()

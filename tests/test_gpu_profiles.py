@@ -1,7 +1,9 @@
-"""GPU parity of the SIE and truncated-NFW mass profiles
-(src/gigalens/tf/profiles/mass/sie.py:13-43, tnfw.py:14-58), through the
-LDS-resident fused kernel (instantiated for SIE+shear and for EPL+shear+TNFW)
-and through the HBM-staged pipeline, and of the core-Sersic light profile
+"""GPU parity of the SIS, SIE and truncated-NFW mass profiles
+(src/gigalens/tf/profiles/mass/sis.py:12-18, sie.py:13-43, tnfw.py:14-58), of the
+circular Sersic light profile (src/gigalens/tf/profiles/light/sersic.py:21-31) and
+of a non-diagonal pixel-to-angle transform (src/gigalens/tf/simulator.py:19-29,
+src/gigalens/simulator.py:91-125), through the LDS-resident fused kernel and
+through the HBM-staged pipeline, and of the core-Sersic light profile
 (src/gigalens/tf/profiles/light/sersic.py:75-121) through the fused kernel,
 against the FP64 oracle.
 
@@ -22,10 +24,11 @@ from gigalens_amd import distributions as tfd
 from gigalens_amd import workloads as wl
 from gigalens_amd.plan import Plan, layout_from_paths
 from gigalens_amd import _native as nat
-from gigalens_amd.profiles.light.sersic import CoreSersic, SersicEllipse
+from gigalens_amd.profiles.light.sersic import CoreSersic, Sersic, SersicEllipse
 from gigalens_amd.profiles.mass.epl import EPL
 from gigalens_amd.profiles.mass.shear import Shear
 from gigalens_amd.profiles.mass.sie import SIE
+from gigalens_amd.profiles.mass.sis import SIS
 from gigalens_amd.profiles.mass.tnfw import TNFW
 from gigalens_amd.simulator import Camera
 
@@ -69,6 +72,32 @@ def _tnfw_problem():
     return prior, phys
 
 
+def _sis_problem():
+    ll, src = _lights()
+    lens = tfd.JointDistributionSequential([
+        tfd.JointDistributionNamed(dict(
+            theta_E=tfd.LogNormal(math.log(1.2), 0.2), center_x=tfd.Normal(0, 0.05),
+            center_y=tfd.Normal(0, 0.05))),
+        tfd.JointDistributionNamed(dict(gamma1=tfd.Normal(0, 0.05), gamma2=tfd.Normal(0, 0.05))),
+    ])
+    prior = tfd.JointDistributionSequential([lens, ll, src])
+    phys = PhysicalModel([SIS(), Shear()], [SersicEllipse()], [SersicEllipse()])
+    return prior, phys
+
+
+def _round_source_problem():
+    """EPL + shear, Sersic-ellipse lens light, circular Sersic source (sersic.py:21-31)."""
+    prior0 = wl.epl_shear_prior()
+    src = tfd.JointDistributionSequential([
+        tfd.JointDistributionNamed(dict(
+            R_sersic=tfd.LogNormal(math.log(0.25), 0.15), n_sersic=tfd.Uniform(0.5, 4),
+            center_x=tfd.Normal(0, 0.25), center_y=tfd.Normal(0, 0.25), Ie=tfd.LogNormal(math.log(150.0), 0.5))),
+    ])
+    prior = tfd.JointDistributionSequential([prior0.model[0], prior0.model[1], src])
+    phys = PhysicalModel([EPL(), Shear()], [SersicEllipse()], [Sersic()])
+    return prior, phys
+
+
 def _core_problem():
     """EPL + shear, core-Sersic lens light (sersic.py:75-121), Sersic source."""
     _, src = _lights()
@@ -85,9 +114,20 @@ def _core_problem():
     return prior, phys
 
 
+# a rotated and slightly sheared pixel grid (columns 0.0655"/px at -7 deg, rows 0.0645"/px)
+ROTATED_T = np.array([[0.0655 * math.cos(-0.122), -0.0645 * math.sin(-0.122) + 0.004],
+                      [0.0655 * math.sin(-0.122), 0.0645 * math.cos(-0.122)]], np.float32)
+
+
 def _problem(name, bs=8):
-    prior, phys = {"sie": _sie_problem, "tnfw": _tnfw_problem, "core": _core_problem}[name]()
+    if name == "rotated":
+        prior, phys = wl.epl_shear_prior(), wl.epl_shear_model()
+    else:
+        prior, phys = {"sie": _sie_problem, "tnfw": _tnfw_problem, "core": _core_problem, "sis": _sis_problem,
+                       "round": _round_source_problem}[name]()
     cfg = wl.epl_shear_config()
+    if name == "rotated":
+        cfg.transform_pix2angle = ROTATED_T
     cam = Camera(cfg)
     ocam = C.oracle_camera(cfg, cam)
     truth_x = C.draw_prior(prior, 1, seed=0).numpy().astype(np.float64)
@@ -103,7 +143,8 @@ def _plan(prob, staged):
                 observed=prob["obs"], background_rms=0.2, exp_time=100.0, staged=staged)
 
 
-CASES = [("sie", False), ("sie", True), ("tnfw", False), ("tnfw", True), ("core", False)]
+CASES = [("sie", False), ("sie", True), ("tnfw", False), ("tnfw", True), ("core", False), ("sis", False),
+         ("sis", True), ("round", False), ("round", True), ("rotated", False), ("rotated", True)]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=[f"{n}-{'staged' if s else 'fused'}" for n, s in CASES])
