@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "gl_lstsq_simulate",
     "gl_log_prob_lenses",
     "gl_map_adam_step_lenses",
+    "gl_map_adam_step_noalloc",
     "gl_batch_create",
     "gl_batch_destroy",
     "gl_batch_size",
@@ -138,6 +139,8 @@ _SIGS = {
     "gl_log_prob_lenses": ([_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "gl_map_adam_step_lenses": ([_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
+    "gl_map_adam_step_noalloc": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
     "gl_batch_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_vp)],
                         ctypes.c_int),
     "gl_batch_destroy": ([_vp], ctypes.c_int),

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -896,8 +897,42 @@ extern "C" int gl_plan_query(const gl_plan* plan, gl_plan_info* info) {
     return GL_OK;
 }
 
-static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream) {
+// Scratch of the fused path.  Plans own it (the batch-coupled EPL series counts and the
+// launch-order buffer); gl_map_adam_step_noalloc takes it from `scratch` instead: the
+// first kNoAllocSeries ints hold the series counts, the next gl::kOrderCap the launch
+// order.  Every element a launch reads is written earlier in the same launch sequence
+// (the series kernel writes every count, the order kernel a full permutation of the
+// particles), so nothing is cleared between calls.
+constexpr int kNoAllocSeries = 8;
+static __device__ __align__(256) int scratch[kNoAllocSeries + gl::kOrderCap];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device, looked up once per device (each
+// GPU holds its own copy of the array at its own address).
+static int scratch_ptr(int** out) {
+    int dev = -1;
+    GL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GL_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        GL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (int*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return GL_OK;
+}
+
+// One pass of a plan over A.bs particles.  scratch (fused plans only): device memory of
+// kNoAllocSeries + gl::kOrderCap ints used in place of the plan's own buffers.
+static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream, int* scratch_dev = nullptr) {
     if (!plan) return fail(GL_ERR_ARG, "null plan");
+    if (plan->mc && scratch_dev) return fail(GL_ERR_UNSUPPORTED, "staged plans run on their own workspace");
     if (plan->mc) {
         if (A.lens) return fail(GL_ERR_UNSUPPORTED, "staged plans take one observed image (use gl_batch_*)");
         gl::McArgs B;
@@ -934,18 +969,21 @@ static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream)
     }
     if (A.bs <= 0) return fail(GL_ERR_ARG, "bs must be positive");
     if (!A.in) return fail(GL_ERR_ARG, "null parameter pointer");
+    gl::DPlan P = plan->P;
+    const int use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
     if (plan->epl_batch) {
-        const int use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
-        GL_HIP(launch_epl_batch_series(plan->P, A.in, A.bs, use_z, const_cast<int*>(plan->P.batch_n), stream));
+        int* counts = scratch_dev ? scratch_dev : const_cast<int*>(plan->P.batch_n);
+        GL_HIP(launch_epl_batch_series(P, A.in, A.bs, use_z, counts, stream));
+        P.batch_n = counts;
     }
     // launch order by cost when the launch has several rounds of workgroups (gradient launches)
     A.order = nullptr;
-    if (plan->order_buf && grad && A.bs > 1024 && A.bs <= gl::kOrderCap) {
-        const int use_z = (A.mode == gl::M_LOGPROB || A.mode == gl::M_MAP) ? 1 : 0;
-        GL_HIP(launch_order(plan->P, A.in, A.bs, use_z, plan->order_buf, stream));
-        A.order = plan->order_buf;
+    int* order = plan->order_buf ? (scratch_dev ? scratch_dev + kNoAllocSeries : plan->order_buf) : nullptr;
+    if (order && grad && A.bs > 1024 && A.bs <= gl::kOrderCap) {
+        GL_HIP(launch_order(P, A.in, A.bs, use_z, order, stream));
+        A.order = order;
     }
-    GL_HIP(plan->model->launch(A.bs, plan->lds_bytes, stream, plan->P, A, grad));
+    GL_HIP(plan->model->launch(A.bs, plan->lds_bytes, stream, P, A, grad));
     return GL_OK;
 }
 
@@ -1017,9 +1055,9 @@ extern "C" int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, fl
     return run_mc(plan, B, false, stream);
 }
 
-extern "C" int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, float* v, const int* lens,
-                                       const float* weight, int bs, int step, float lr, float beta1, float beta2,
-                                       float epsilon, float* lp, float* chi2, hipStream_t stream) {
+static int map_step(const gl_plan* plan, float* z, float* m, float* v, const int* lens, const float* weight, int bs,
+                    int step, float lr, float beta1, float beta2, float epsilon, float* lp, float* chi2,
+                    hipStream_t stream, int* scratch_dev) {
     if (!z || !m || !v) return fail(GL_ERR_ARG, "null optimizer state");
     if (step < 1) return fail(GL_ERR_ARG, "step is 1-based");
     gl::KArgs A = blank(gl::M_MAP, bs);
@@ -1038,13 +1076,35 @@ extern "C" int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, 
                       : -1.0f / ((float)bs * (float)(plan ? plan->P.N * plan->P.N : 1));
     A.lens = lens;
     A.weight = weight;
-    return run(plan, A, true, stream);
+    return run(plan, A, true, stream, scratch_dev);
+}
+
+extern "C" int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, float* v, const int* lens,
+                                       const float* weight, int bs, int step, float lr, float beta1, float beta2,
+                                       float epsilon, float* lp, float* chi2, hipStream_t stream) {
+    return map_step(plan, z, m, v, lens, weight, bs, step, lr, beta1, beta2, epsilon, lp, chi2, stream, nullptr);
 }
 
 extern "C" int gl_map_adam_step(const gl_plan* plan, float* z, float* m, float* v, int bs, int step, float lr,
                                 float beta1, float beta2, float epsilon, float* lp, float* chi2, hipStream_t stream) {
     return gl_map_adam_step_lenses(plan, z, m, v, nullptr, nullptr, bs, step, lr, beta1, beta2, epsilon, lp, chi2,
                                    stream);
+}
+
+extern "C" int gl_map_adam_step_noalloc(const gl_plan* plan, float* z, float* m, float* v, int bs, int step,
+                                        float lr, float beta1, float beta2, float epsilon, float* lp, float* chi2,
+                                        hipStream_t stream) {
+    if (!plan) return fail(GL_ERR_ARG, "null plan");
+    if (plan->mc)
+        return fail(GL_ERR_UNSUPPORTED, "gl_map_adam_step_noalloc takes resident-image plans (gl_plan_info.lds_bytes > 0); "
+                                        "staged plans need their own workspace");
+    if (bs > gl::kOrderCap)
+        return fail(GL_ERR_ARG, "gl_map_adam_step_noalloc: bs " + std::to_string(bs) + " exceeds the static scratch (" +
+                                    std::to_string(gl::kOrderCap) + " particles)");
+    int* sp = nullptr;
+    int rc = scratch_ptr(&sp);
+    if (rc) return rc;
+    return map_step(plan, z, m, v, nullptr, nullptr, bs, step, lr, beta1, beta2, epsilon, lp, chi2, stream, sp);
 }
 
 // ---------------------------------------------------------------------------

@@ -199,6 +199,19 @@ int gl_map_adam_step_lenses(const gl_plan* plan, float* z, float* m, float* v, c
                             const float* weight, int bs, int step, float lr, float beta1, float beta2,
                             float epsilon, float* lp, float* chi2, hipStream_t stream);
 
+/* gl_map_adam_step without any device memory the library allocates: no counterpart in
+ * the reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Resident-image plans only
+ * (gl_plan_info.lds_bytes > 0; GL_ERR_UNSUPPORTED otherwise) and bs <= 16384
+ * (GL_ERR_ARG beyond).  The launch-order buffer and the batch-coupled EPL series
+ * counts come from a static device array of 16,392 ints (64 KiB) per GPU, held until
+ * the process ends, instead of the plan's own buffers; results equal
+ * gl_map_adam_step's.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (several streams or host threads) race and must be ordered by the caller. */
+int gl_map_adam_step_noalloc(const gl_plan* plan, float* z, float* m, float* v, int bs, int step,
+                             float lr, float beta1, float beta2, float epsilon,
+                             float* lp, float* chi2, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Batches of lens systems with different image sizes, supersampling, PSFs and
  * particle counts.  No counterpart in the reference (one simulator per lens,
