@@ -147,7 +147,8 @@ class LensBatch:
 
     def log_prob(self, z: torch.Tensor, want_grad: bool = False):
         z = z.to(self.device, torch.float32).contiguous()
-        assert z.shape == (self.total, self.d)
+        if z.shape != (self.total, self.d):
+            raise ValueError(f"z must be ({self.total}, {self.d}): every particle of every lens, lens-major")
         lp = torch.empty(self.total, device=self.device)
         chi2 = torch.empty(self.total, device=self.device)
         grad = torch.empty_like(z) if want_grad else None
@@ -161,6 +162,8 @@ class LensBatch:
         MultiLensMAP; ``graph=True`` records the steps into one HIP graph first."""
         optimizer = optimizer if optimizer is not None else Adam(1e-2)
         z = z0.to(self.device, torch.float32).contiguous().clone()
+        if z.shape != (self.total, self.d):
+            raise ValueError(f"z0 must be ({self.total}, {self.d}): every particle of every lens, lens-major")
         m, v = torch.zeros_like(z), torch.zeros_like(z)
         nrec = (num_steps - 1) // record_every + 1
         lp_h = torch.empty((nrec, self.total), device=self.device)

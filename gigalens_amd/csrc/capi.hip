@@ -50,6 +50,7 @@ struct gl_plan {
     int ksplit_cap;          // Gram pixel-range splits allowed by the image size
     std::vector<float> W_host;  // folded PSF (kf x kf) of staged plans, for batches
     std::vector<void*> dev;  // owned device buffers
+    mutable StreamGuard guard;  // orders calls across streams (the scratch above is shared)
 };
 
 extern "C" const char* gl_last_error(void) { return g_err.c_str(); }
@@ -139,6 +140,7 @@ static void jacobi_svd(int n, const std::vector<double>& A, std::vector<double>&
 extern "C" int gl_plan_destroy(gl_plan* plan) {
     if (!plan) return GL_OK;
     for (void* d : plan->dev) (void)hipFree(d);
+    plan->guard.release();
     delete plan;
     return GL_OK;
 }
@@ -354,14 +356,16 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
                 }
         }
     }
-    // particles per chunk: the caller's, else as many as fit a 32 GiB workspace (at most 1024)
     const int BW = ((ss * (N - 1) + 4 * p->kf4 > pb + nsub ? ss * (N - 1) + 4 * p->kf4 : pb + nsub) + 3) & ~3;
     const double per_particle = 4.0 * ((double)M.pstride + (double)D * nsub * BW + (double)D * N * N + D +
                                        (double)GW * GW + gl::kMcStat +
                                        (double)D * M.sep_rank * nsub * N + (double)M.adj_rank * N * nsub) +
                                 8.0 * 4 * p->ksplit_cap * gram_nt * 256;
+    // particles per chunk: the caller's max_batch, at most what a 64 GiB workspace holds;
+    // by default as many as fit 32 GiB, at most 1024
     int mb = desc->max_batch;
     if (mb <= 0) mb = (int)std::max(1.0, std::min(1024.0, 32.0 * 1073741824.0 / per_particle));
+    else mb = (int)std::max(1.0, std::min((double)mb, 64.0 * 1073741824.0 / per_particle));
     p->max_batch = mb;
     M.rcond = 1e-6f;
     if ((rc = upload(p, W.data(), W.size() * sizeof(float), &dptr))) return rc;
@@ -717,6 +721,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         }
         int rc = mc_create(desc, p, W, max_niter, lstsq);
         if (rc) { gl_plan_destroy(p); return rc; }
+        if (p->guard.init() != hipSuccess) { gl_plan_destroy(p); return fail(GL_ERR_HIP, "gl_plan_create: event"); }
         *out = p;
         return GL_OK;
     };
@@ -878,6 +883,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         gl_plan_destroy(p);
         return fail(GL_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
     }
+    if (p->guard.init() != hipSuccess) { gl_plan_destroy(p); return fail(GL_ERR_HIP, "gl_plan_create: event"); }
     *out = p;
     return GL_OK;
 }
@@ -933,6 +939,7 @@ static int scratch_ptr(int** out) {
 static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream, int* scratch_dev = nullptr) {
     if (!plan) return fail(GL_ERR_ARG, "null plan");
     if (plan->mc && scratch_dev) return fail(GL_ERR_UNSUPPORTED, "staged plans run on their own workspace");
+    GL_HIP(plan->guard.enter(stream));
     if (plan->mc) {
         if (A.lens) return fail(GL_ERR_UNSUPPORTED, "staged plans take one observed image (use gl_batch_*)");
         gl::McArgs B;
@@ -1052,6 +1059,7 @@ extern "C" int gl_lstsq_simulate(const gl_plan* plan, const float* x, int bs, fl
     B.img = img;
     B.coef_out = coeffs;
     B.stacked = stacked;
+    GL_HIP(plan->guard.enter(stream));
     return run_mc(plan, B, false, stream);
 }
 
@@ -1119,11 +1127,13 @@ struct gl_batch {
     gl::McTable T;
     std::vector<int> counts;
     std::vector<void*> dev;
+    mutable StreamGuard guard;  // orders calls across streams (the workspace is shared)
 };
 
 extern "C" int gl_batch_destroy(gl_batch* b) {
     if (!b) return GL_OK;
     for (void* p : b->dev) (void)hipFree(p);
+    b->guard.release();
     delete b;
     return GL_OK;
 }
@@ -1140,6 +1150,10 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
             return fail(GL_ERR_ARG, "gl_batch_create: plans must share the parameter count and likelihood");
     }
     gl_batch* b = new gl_batch();
+    if (b->guard.init() != hipSuccess) {
+        delete b;
+        return fail(GL_ERR_HIP, "gl_batch_create: event creation failed");
+    }
     b->n = n;
     b->d = plans[0]->P.d;
     b->kf4 = 0;
@@ -1290,6 +1304,7 @@ extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, f
     A.lp = lp;
     A.chi2 = chi2;
     A.grad = grad_z;
+    GL_HIP(b->guard.enter(stream));
     return mc_pipeline_run(true, b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false,
                            b->mc_model, &b->groups);
 }
@@ -1315,6 +1330,7 @@ extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, flo
     A.beta1 = beta1;
     A.beta2 = beta2;
     A.eps = epsilon;
+    GL_HIP(b->guard.enter(stream));
     return mc_pipeline_run(true, b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream, false, b->mc_model,
                            &b->groups);
 }

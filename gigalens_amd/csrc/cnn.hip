@@ -12,6 +12,7 @@
 #include "cnn_kernels.hip"
 
 using glc::fail;
+using glc::StreamGuard;
 
 // ---------------------------------------------------------------------------
 // Learned initialiser (no reference counterpart)
@@ -35,11 +36,13 @@ struct gl_cnn {
     float *h0, *h1, *da1, *out, *dout, *lossb, *part;
     size_t part_elems;
     std::vector<void*> dev;
+    mutable StreamGuard guard;  // orders calls across streams (the activations are shared)
 };
 
 extern "C" int gl_cnn_destroy(gl_cnn* net) {
     if (!net) return GL_OK;
     for (void* p : net->dev) (void)hipFree(p);
+    net->guard.release();
     delete net;
     return GL_OK;
 }
@@ -77,6 +80,10 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
     if (desc->n_out <= 0 || 2 * desc->n_out > gl::cnn::kHeadMaxOut) return fail(GL_ERR_ARG, "gl_cnn_create: n_out must be in [1, 128]");
     if (!(desc->in_scale > 0.0f)) return fail(GL_ERR_ARG, "gl_cnn_create: in_scale must be positive");
     gl_cnn* net = new gl_cnn();
+    if (net->guard.init() != hipSuccess) {
+        delete net;
+        return fail(GL_ERR_HIP, "gl_cnn_create: event creation failed");
+    }
     net->H = desc->height; net->W = desc->width; net->d = desc->n_out; net->hidden = desc->hidden;
     net->maxb = desc->max_batch; net->in_scale = desc->in_scale;
     const int strides[4] = {1, 2, 1, 2};
@@ -154,6 +161,7 @@ static int cnn_pack(gl_cnn* net, hipStream_t st) {
 
 extern "C" int gl_cnn_set_params(gl_cnn* net, const float* params, hipStream_t stream) {
     if (!net || !params) return fail(GL_ERR_ARG, "gl_cnn_set_params: null argument");
+    GL_HIP(net->guard.enter(stream));
     GL_HIP(hipMemcpyAsync(net->params, params, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
     GL_HIP(hipMemsetAsync(net->m, 0, (size_t)net->P * 4, stream));
     GL_HIP(hipMemsetAsync(net->v, 0, (size_t)net->P * 4, stream));
@@ -162,6 +170,7 @@ extern "C" int gl_cnn_set_params(gl_cnn* net, const float* params, hipStream_t s
 
 extern "C" int gl_cnn_get_params(const gl_cnn* net, float* params, hipStream_t stream) {
     if (!net || !params) return fail(GL_ERR_ARG, "gl_cnn_get_params: null argument");
+    GL_HIP(net->guard.enter(stream));
     GL_HIP(hipMemcpyAsync(params, net->params, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
     return GL_OK;
 }
@@ -201,6 +210,7 @@ static int cnn_forward_layers(gl_cnn* net, const float* images, int bs, hipStrea
 extern "C" int gl_cnn_forward(gl_cnn* net, const float* images, int bs, float* out, hipStream_t stream) {
     if (!net || !images || !out) return fail(GL_ERR_ARG, "gl_cnn_forward: null argument");
     if (bs <= 0 || bs > net->maxb) return fail(GL_ERR_ARG, "gl_cnn_forward: bs must be in [1, max_batch]");
+    GL_HIP(net->guard.enter(stream));
     int rc;
     if ((rc = cnn_forward_layers(net, images, bs, stream))) return rc;
     gl::cnn::HeadArgs A = cnn_head_args(net, nullptr, bs);
@@ -215,6 +225,7 @@ extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* t
     using namespace gl::cnn;
     if (!net || !images || !target) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: null argument");
     if (bs <= 0 || bs > net->maxb) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: bs must be in [1, max_batch]");
+    GL_HIP(net->guard.enter(stream));
     int rc;
     if ((rc = cnn_forward_layers(net, images, bs, stream))) return rc;
     HeadArgs A = cnn_head_args(net, target, bs);
@@ -274,6 +285,7 @@ extern "C" int gl_cnn_adamw_step(gl_cnn* net, int step, float lr, float beta1, f
                                  float weight_decay, hipStream_t stream) {
     if (!net) return fail(GL_ERR_ARG, "gl_cnn_adamw_step: null network");
     if (step < 1) return fail(GL_ERR_ARG, "gl_cnn_adamw_step: step is 1-based");
+    GL_HIP(net->guard.enter(stream));
     const float bc1 = (float)(1.0 - pow((double)beta1, step));
     const float bc2s = (float)sqrt(1.0 - pow((double)beta2, step));
     hipLaunchKernelGGL(gl::cnn::adamw, dim3((net->P + 255) / 256), dim3(256), 0, stream, net->params, net->m, net->v,

@@ -525,8 +525,10 @@ __global__ __launch_bounds__(256) void mc_conv(const McTable T) {
 //   H_r[y][J] = sum_q sb_r[q] B[y][ss J + q - pb],
 // and a vertical pass over the output rows,
 //   X[I][J] = sum_r sum_p sa_r[p] H_r[ss I + p - pb][J].
-// mc_sep_h: one wave per (particle, channel, 4 sub-rows, 64 columns J).  A batch
-// launches both passes once per (ss, kf4) group of its separable lenses.
+// mc_sep_h: one wave per (particle, channel, 256 consecutive outputs of the
+// flattened (sub-row y, column J) range), each lane 4 of them, so no lane idles
+// when N is not a multiple of 64.  A batch launches both passes once per (ss, kf4)
+// group of its separable lenses.
 // ---------------------------------------------------------------------------
 template <bool MULTI, int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {

@@ -194,7 +194,7 @@ class ModellingSequence(ModellingSequenceInterface):
         hist = torch.empty((nrec, n), device=self.device)
         lp_hist = torch.empty((nrec, n), device=self.device)
         scratch = torch.empty((2, n), device=self.device)
-        def steps(it):
+        def steps(it, show):
             for s in it:
                 step = optimizer.iterations + 1
                 rec = s % record_every == 0
@@ -203,17 +203,21 @@ class ModellingSequence(ModellingSequenceInterface):
                 plan.map_adam_step(z, m, v, step, optimizer.lr(optimizer.iterations), optimizer.beta_1,
                                    optimizer.beta_2, optimizer.epsilon, lp=lp, chi2=row)
                 optimizer.iterations += 1
-                if progress and s % 10 == 0:
+                if show and s % 10 == 0:
                     it.set_description(f"Chi Squared: {float(torch.nan_to_num(row, nan=float('inf')).min()):.4f}")
         if graph:
+            # the progress text needs a host read per step, which a graph cannot hold: the
+            # steps are recorded without it and the final value is reported after the replay
             from .graphs import capture
-            capture(lambda: steps(range(num_steps))).replay()
+            capture(lambda: steps(range(num_steps), False)).replay()
+            if progress:
+                print(f"Chi Squared: {float(torch.nan_to_num(hist[-1], nan=float('inf')).min()):.4f}", flush=True)
         else:
             it = range(num_steps)
             if progress:
                 from tqdm.auto import trange
                 it = trange(num_steps)
-            steps(it)
+            steps(it, progress)
         self.chi2_history = hist
         self.lp_history = lp_hist
         return z

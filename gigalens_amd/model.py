@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import os
 from abc import ABC, abstractmethod
+from collections import OrderedDict
 from typing import List
 
 import torch
@@ -70,21 +71,35 @@ class _ProbModelBase(ProbabilisticModel):
         self.unconstraining_bij = prior.experimental_default_event_space_bijector()
         self.bij = dist.ChainBijector(self.unconstraining_bij, self.pack_bij)
         self._paths = [p for p, _ in dist.leaves(prior)]
-        self._plans = {}
+        self._plans = OrderedDict()
 
     #: pick the fastest execution variant of each new plan by timing (autotune.py); also
     #: switched on by the environment variable GIGALENS_AMD_AUTOTUNE=1
     autotune = False
 
+    #: plans kept per prob model (least recently used evicted first; a plan holds its
+    #: device constants and, for staged plans, a workspace sized for its batch)
+    plan_cache_size = 2
+
+    @staticmethod
+    def _camera_key(simulator):
+        cam = simulator.camera
+        k = None if cam.subgrid_kernel is None else cam.subgrid_kernel.tobytes()
+        return (cam.num_pix, cam.supersample, cam.transform_sub.tobytes(), cam.origin, cam.conversion_factor, k)
+
     def plan_for(self, simulator, epl_batch_series=False) -> Plan:
-        key = (id(simulator), bool(epl_batch_series))
-        if key not in self._plans:
+        """The device plan of (physical model, camera, batch size, series rule): built on
+        first use, then reused by every simulator of the same model, camera and batch."""
+        sbs = int(getattr(simulator, "bs", 0) or 0)
+        key = (id(simulator.phys_model), self._camera_key(simulator), sbs, bool(epl_batch_series))
+        hit = self._plans.pop(key, None)
+        if hit is None:
             lay = layout_from_paths(simulator.phys_model, self._paths)
             kw = dict(prior=self.prior, observed=self.observed_image.cpu().numpy(),
                       background_rms=self.background_rms, exp_time=self.exp_time, likelihood=self._likelihood,
                       fldj_at_x=self._fldj_at_x, epl_batch_series=epl_batch_series)
-            # staged plans size their workspace for the simulator's batch (one chunk per call)
-            sbs = int(getattr(simulator, "bs", 0) or 0)
+            # staged plans size their workspace for the simulator's batch (one chunk per call),
+            # capped by the library at a 64 GiB workspace
             if 0 < sbs <= 1024:
                 kw["max_batch"] = sbs
             variant = {}
@@ -93,8 +108,12 @@ class _ProbModelBase(ProbabilisticModel):
                 bs = max(1, min(int(getattr(simulator, "bs", 256)), 1024))
                 z = self.bij.inverse(self.prior.sample(bs, seed=12345)).to(simulator.device, torch.float32)
                 variant, _ = tune(simulator.phys_model, simulator.camera, lay, z.contiguous(), kw)
-            self._plans[key] = (simulator, Plan(simulator.phys_model, simulator.camera, lay, **kw, **variant))
-        return self._plans[key][1]
+            while len(self._plans) >= max(1, self.plan_cache_size):
+                self._plans.pop(next(iter(self._plans)))  # least recently used
+            # the entry keeps the physical model alive, so its id stays unique while cached
+            hit = (simulator.phys_model, Plan(simulator.phys_model, simulator.camera, lay, **kw, **variant))
+        self._plans[key] = hit
+        return hit[1]
 
     def log_prob(self, simulator, z):
         """Log posterior of unconstrained ``z`` (bs, d): ``(lp, chi2)``."""

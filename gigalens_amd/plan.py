@@ -92,6 +92,16 @@ def _dev(t: torch.Tensor, name: str) -> torch.Tensor:
     return t
 
 
+def _devn(t, name: str, numel: int):
+    """Optional float32 device tensor of exactly ``numel`` elements (None passes)."""
+    if t is None:
+        return None
+    _dev(t, name)
+    if t.numel() != numel:
+        raise ValueError(f"{name} must hold {numel} elements, got {t.numel()}")
+    return t
+
+
 def _lens(t: torch.Tensor, bs: int) -> torch.Tensor:
     if not t.is_cuda or t.dtype != torch.int32 or not t.is_contiguous() or t.numel() != bs:
         raise TypeError("lens must be a contiguous int32 CUDA tensor with one entry per particle")
@@ -232,6 +242,10 @@ class Plan:
         each particle's observed image of a multi-lens plan."""
         z = _dev(z, "z")
         bs = z.shape[0]
+        if z.dim() != 2 or z.shape[1] != self.d:
+            raise ValueError(f"z must be (bs, {self.d})")
+        g_lp = _devn(g_lp, "g_lp", bs)
+        g_chi2 = _devn(g_chi2, "g_chi2", bs)
         lp = torch.empty(bs, device=z.device, dtype=torch.float32)
         chi2 = torch.empty(bs, device=z.device, dtype=torch.float32)
         grad = torch.empty_like(z) if want_grad else None
@@ -258,15 +272,23 @@ class Plan:
 
     def map_adam_step(self, z, m, v, step: int, lr: float, beta1=0.9, beta2=0.999, epsilon=1e-7, lp=None,
                       chi2=None, lens=None, weight=None):
+        z = _dev(z, "z")
         bs = z.shape[0]
+        if z.dim() != 2 or z.shape[1] != self.d:
+            raise ValueError(f"z must be (bs, {self.d})")
+        m = _devn(m, "m", z.numel())
+        v = _devn(v, "v", z.numel())
+        if m is None or v is None:
+            raise ValueError("m and v (Adam moments) are required")
+        lp = _devn(lp, "lp", bs)
+        chi2 = _devn(chi2, "chi2", bs)
         if lens is None and weight is None:
             nat.check(self._lib.gl_map_adam_step(self._h, _ptr(z), _ptr(m), _ptr(v), bs, int(step), float(lr),
                                                  float(beta1), float(beta2), float(epsilon), _ptr(lp), _ptr(chi2),
                                                  _stream()), "gl_map_adam_step")
         else:
             lens = None if lens is None else _lens(lens, bs)
-            if weight is not None:
-                weight = _dev(weight, "weight")
+            weight = _devn(weight, "weight", bs)
             nat.check(self._lib.gl_map_adam_step_lenses(self._h, _ptr(z), _ptr(m), _ptr(v), _ptr(lens), _ptr(weight),
                                                         bs, int(step), float(lr), float(beta1), float(beta2),
                                                         float(epsilon), _ptr(lp), _ptr(chi2), _stream()),

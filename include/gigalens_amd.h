@@ -4,12 +4,20 @@
  * The ABI is what a binding of the reference's LensSimulator / prob-model
  * path would call.  Every tensor argument is caller-allocated DEVICE memory
  * (row-major float32 unless stated), every call is stream-ordered and
- * asynchronous, and no call allocates device memory or synchronises the
- * host except gl_plan_create / gl_plan_destroy.  Errors are reported as a
- * non-zero return code; gl_last_error() returns a thread-local message.
- * A plan owns scratch memory (the staged pipeline's workspace, the fused
- * kernel's launch-order buffer), so calls on one plan must be issued on one
- * stream (or otherwise ordered); use one plan per stream for concurrency.
+ * asynchronous.  Only the create / destroy calls allocate or free device memory
+ * and synchronise the host: gl_plan_create, gl_plan_destroy, gl_batch_create,
+ * gl_batch_destroy, gl_cnn_create, gl_cnn_destroy.  gl_cnn_set_params clears the
+ * AdamW moments with hipMemsetAsync and must not be recorded into a HIP graph.
+ * Errors are reported as a non-zero return code; gl_last_error() returns a
+ * thread-local message.
+ * A plan, a batch and a network own scratch memory (the staged pipeline's
+ * workspace, the fused kernel's launch-order buffer, the network's activations)
+ * that their calls share.  The library orders the calls on one object across
+ * streams: a call on another stream than the previous call makes its stream wait
+ * for the work queued on the previous one (an event wait; calls on one stream add
+ * nothing), so concurrent streams on one object are serialised, not raced; a
+ * stream under graph capture is not ordered this way (order the work before the
+ * capture).  Use one plan per stream for concurrent work.
  *
  * Parameter vectors use the reference's flatten order (tf.nest order of a
  * prior sample: component lists in order, dict keys sorted), i.e. the
@@ -105,8 +113,9 @@ typedef struct gl_plan_desc {
                                  PSF; 0 selects 1e-6, a negative value forces the direct 2-D path.
                                  Staged plans (and lens batches of them) use the same rule for
                                  their channel convolution and adjoint convolution */
-    int max_batch;            /* least-squares plans: particles per internal chunk (workspace is
-                                 allocated for this many at plan creation); 0 selects 64 */
+    int max_batch;            /* staged plans: particles per internal chunk (the workspace is
+                                 allocated for this many at plan creation), at most what a 64 GiB
+                                 workspace holds; 0 selects as many as fit 32 GiB, at most 1024 */
     const float* noise_map;   /* HOST N*N sigma per pixel; NULL: sqrt(bkg^2 + clip(obs,0)/t_exp) */
     int n_observed;           /* observed images stacked in `observed` (and `noise_map`), each
                                  N*N; 0 selects 1.  Particles pick one with the lens index of
