@@ -222,6 +222,11 @@ static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int 
     M.gimg = (float*)dptr;
     if ((rc = dev_alloc(owned, (size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
     M.stats = (float*)dptr;
+    M.gamp = nullptr;
+    if (M.famp) {
+        if ((rc = dev_alloc(owned, (size_t)mb * D * 4, &dptr, true))) return rc;
+        M.gamp = (float*)dptr;
+    }
     M.U = nullptr;
     if (M.adj_rank > 0) {
         if ((rc = dev_alloc(owned, (size_t)mb * M.adj_rank * N * ss * N * 4, &dptr, false))) return rc;
@@ -235,11 +240,13 @@ static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int 
     return GL_OK;
 }
 
-static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<float>& W, int max_niter, int lstsq) {
+static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<float>& W, int max_niter, int lstsq,
+                     int famp) {
     gl::DPlan& P = p->P;
     gl::McPlan& M = p->M;
     memset(&M, 0, sizeof(M));
     M.lstsq = lstsq;
+    M.famp = famp;
     M.conv_ib = desc->conv_rows > 0 ? desc->conv_rows : gl::kMcIB;
     if (M.conv_ib != 2 && M.conv_ib != 4 && M.conv_ib != 8)
         return fail(GL_ERR_ARG, "conv_rows must be 0, 2, 4 or 8");
@@ -256,11 +263,13 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     M.shp_comp = -1;
     for (int i = P.nlens; i < P.ncomp; ++i) {
         const gl::DComp& c = P.comp[i];
-        if (!lstsq) {
+        if (!lstsq && !famp) {
             if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE)
                 return fail(GL_ERR_UNSUPPORTED, "fixed-amplitude staged plans take Sersic lights");
             continue;
         }
+        if (famp && c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE && c.kind != glm::K_SHAPELETS)
+            return fail(GL_ERR_UNSUPPORTED, "plans with a fixed-amplitude shapelet light take Sersic and shapelet lights");
         if (c.kind == glm::K_SHAPELETS) {
             if (++nshp > 1) return fail(GL_ERR_UNSUPPORTED, "at most one shapelet component");
             if (c.n_max < 0 || c.n_max > glm::kMaxShapeletN)
@@ -269,12 +278,14 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
             M.shp_nmax = c.n_max;
             M.shp_interp = c.interpolate;
             M.shp_chan0 = D;
+            if (famp && (c.amp_col0 < 0 || c.amp_col0 + (c.n_max + 1) * (c.n_max + 2) / 2 > P.d))
+                return fail(GL_ERR_ARG, "fixed-amplitude shapelets need amp_col0 with every amplitude column in range");
             D += (c.n_max + 1) * (c.n_max + 2) / 2;
         } else {
             D += 1;
         }
     }
-    if (!lstsq) D = 1;
+    if (!lstsq && !famp) D = 1;
     if (D < 1 || D > gl::kMcMaxChan) return fail(GL_ERR_UNSUPPORTED, "number of linear channels out of range");
     M.D = D;
     // per-particle constants: prior arrays, component blocks, EPL coefficient tables
@@ -607,9 +618,15 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     int sig[7] = {0, 0, 0, 0, 0, 0, 0};
     int nl = 0, nll = 0, ns = 0, last_role = 0, max_niter = 0;
     P.ncomp = desc->n_components;
-    p->mc = 0;
-    for (int i = 0; i < desc->n_components; ++i)
-        if (desc->components[i].use_lstsq || desc->components[i].kind == glm::K_SHAPELETS) p->mc = 1;
+    // least squares: some light is linear (use_lstsq), and then every light must be;
+    // fixed per-channel amplitudes: a shapelet light with sampled amplitudes
+    int any_lstsq = 0, any_shp = 0;
+    for (int i = 0; i < desc->n_components; ++i) {
+        any_lstsq |= desc->components[i].use_lstsq ? 1 : 0;
+        any_shp |= desc->components[i].kind == glm::K_SHAPELETS ? 1 : 0;
+    }
+    const int famp = any_shp && !any_lstsq;
+    p->mc = (any_lstsq || any_shp) ? 1 : 0;
     for (int i = 0; i < desc->n_components; ++i) {
         const gl_component_desc& c = desc->components[i];
         if (c.role < last_role) {
@@ -641,7 +658,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
                 gl_plan_destroy(p);
                 return fail(GL_ERR_ARG, "mass profile given as a light component");
             }
-            if (p->mc && !c.use_lstsq) {
+            if (any_lstsq && !c.use_lstsq) {
                 gl_plan_destroy(p);
                 return fail(GL_ERR_UNSUPPORTED, "least-squares plans need every light component linear (use_lstsq)");
             }
@@ -667,7 +684,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     }
     P.nlens = nl; P.nll = nll; P.nsrc = ns;
     p->model = nullptr;
-    const int lstsq = p->mc;
+    const int lstsq = any_lstsq;
     bool staged = p->mc || desc->staged;
     if (!staged) {
         p->model = find_model(sig);
@@ -713,13 +730,13 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         p->mc = 1;
         p->model = nullptr;
         p->mc_model = 0;
-        if (!lstsq) {
+        if (!lstsq && !famp) {
             const int dflt[7] = {EPL, SHR, 0, SE, 0, SE, 0};
             bool ok = true;
             for (int j = 0; j < 7; ++j) ok = ok && sig[j] == dflt[j];
             p->mc_model = ok ? 1 : 0;
         }
-        int rc = mc_create(desc, p, W, max_niter, lstsq);
+        int rc = mc_create(desc, p, W, max_niter, lstsq, famp);
         if (rc) { gl_plan_destroy(p); return rc; }
         if (p->guard.init() != hipSuccess) { gl_plan_destroy(p); return fail(GL_ERR_HIP, "gl_plan_create: event"); }
         *out = p;
@@ -1145,6 +1162,7 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         const gl_plan* q = plans[i];
         if (!q || !q->mc || q->M.lstsq)
             return fail(GL_ERR_ARG, "gl_batch_create: every plan must be a fixed-amplitude staged plan (desc.staged = 1)");
+        if (q->M.famp) return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: shapelet lights are not batched");
         if (counts[i] <= 0) return fail(GL_ERR_ARG, "gl_batch_create: particle counts must be positive");
         if (q->P.d != plans[0]->P.d || q->P.lik != plans[0]->P.lik)
             return fail(GL_ERR_ARG, "gl_batch_create: plans must share the parameter count and likelihood");

@@ -60,7 +60,10 @@ constexpr int kMcIB = 4;     // default output rows per conv lane (McPlan.conv_i
 typedef const float __attribute__((address_space(4))) CFloat;
 
 struct McPlan {
-    int lstsq;          // 1: linear amplitudes solved by least squares; 0: fixed amplitudes (one channel)
+    int lstsq;          // 1: linear amplitudes solved by least squares; 0: fixed amplitudes
+    int famp;           // fixed amplitudes per channel (a shapelet light with sampled amplitudes): the
+                        // unit-amplitude channels of least squares, weighted by cf x the parameters
+                        // (Sersic Ie, shapelet amplitudes); 0 with lstsq = 0: one channel (the sum)
     int conv_ib;        // output rows per lane of mc_conv (a tuning choice: 2, 4 or 8)
     int sep_rank;       // > 0: separable channel convolution with this rank (W ~= sum_r sa_r sb_r^T)
     const float* sa;    // sep_rank x kf vertical factors
@@ -102,6 +105,7 @@ struct McPlan {
     // the sub-grid adjoint image dL/dsub reuses B (dead after mc_conv): channel 0's
     // interior, row stride BW, column offset pb -- the zero margins stay untouched
     float* stats;            // bs x kMcStat
+    float* gamp;             // famp: bs x D gradient w.r.t. each channel's amplitude parameter
     float* lp_like;          // bs (log-likelihood + chi^2 scratch)
 };
 
@@ -274,11 +278,11 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
             *reinterpret_cast<glm::TnfwC*>(dst) = c;
         } else if (cd.kind == glm::K_SERSIC_ELLIPSE) {
             glm::SerC c;
-            glm::sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.lstsq ? 1.0f : pv[6], 1, c);
+            glm::sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], (cd.lstsq || M.famp) ? 1.0f : pv[6], 1, c);
             *reinterpret_cast<glm::SerC*>(dst) = c;
         } else if (cd.kind == glm::K_SERSIC) {
             glm::SerC c;
-            glm::sersic_prepare(pv[0], pv[1], 0.0f, 0.0f, pv[2], pv[3], cd.lstsq ? 1.0f : pv[4], 0, c);
+            glm::sersic_prepare(pv[0], pv[1], 0.0f, 0.0f, pv[2], pv[3], (cd.lstsq || M.famp) ? 1.0f : pv[4], 0, c);
             *reinterpret_cast<glm::SerC*>(dst) = c;
         } else if (cd.kind == glm::K_SHAPELETS) {
             dst[0] = pv[0];  // beta
@@ -303,7 +307,22 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
         }
     }
     if (tid < kMcStat) M.stats[(size_t)q * kMcStat + tid] = 0.0f;
-    if (!M.lstsq && tid == 0) M.coef[(size_t)q * M.D] = P.cf;  // fixed amplitudes: image = cf * conv
+    if (!M.lstsq && !M.famp && tid == 0) M.coef[(size_t)q * M.D] = P.cf;  // fixed amplitudes: image = cf * conv
+    if (M.famp && tid == 0) {
+        // per-channel amplitudes: image = sum_c cf a_c X_c (the sampled Ie / amp parameters)
+        int ch = 0;
+        for (int i = P.nlens; i < P.ncomp; ++i) {
+            const DComp& cd = P.comp[i];
+            if (cd.kind == glm::K_SHAPELETS) {
+                const int nl = (cd.n_max + 1) * (cd.n_max + 2) / 2;
+                for (int k = 0; k < nl; ++k) M.coef[(size_t)q * M.D + ch + k] = P.cf * s_x[cd.amp_col0 + k];
+                ch += nl;
+            } else {
+                const int ie = cd.kind == glm::K_SERSIC ? 4 : 6;
+                M.coef[(size_t)q * M.D + ch++] = P.cf * (cd.col[ie] >= 0 ? s_x[cd.col[ie]] : cd.cst[ie]);
+            }
+        }
+    }
 }
 
 // Per-lens states kept by the recompute for the deflection adjoint.
@@ -376,6 +395,38 @@ __device__ __forceinline__ void mc_phi(const McPlan& M, float u, float* phi, flo
     else glm::shapelet_1d_direct(M.shp_pref, M.shp_nmax, u, phi, dphi);
 }
 
+// Fixed per-channel amplitudes: the simulated sub-pixel sum_c coef_c channel_c (NaN when
+// the forward pass zeroed the sub-pixel).
+__device__ __forceinline__ float mc_famp_sum(const DPlan& P, const McPlan& M, const float* s_c, const float* cp,
+                                             float X, float Y, float bx, float by) {
+    float v = 0.0f;
+    int ch = 0;
+    for (int i = P.nlens; i < P.ncomp; ++i) {
+        const DComp& cd = P.comp[i];
+        const float* cst = s_c + i * kConstStride;
+        const bool src = cd.role == GL_ROLE_SOURCE;
+        const float ux = src ? bx : X, uy = src ? by : Y;
+        if (cd.kind == glm::K_SHAPELETS) {
+            const float beta = cst[0];
+            float pu[glm::kMaxShapeletN + 1], pw[glm::kMaxShapeletN + 1];
+            mc_phi(M, (ux - cst[1]) / beta, pu, nullptr);
+            mc_phi(M, (uy - cst[2]) / beta, pw, nullptr);
+#pragma unroll
+            for (int so = 0; so <= glm::kMaxShapeletN; ++so) {
+                if (so > M.shp_nmax) break;
+#pragma unroll
+                for (int j = 0; j <= so; ++j) v = fmaf(cp[ch + so * (so + 1) / 2 + j], pu[so - j] * pw[j], v);
+            }
+            ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
+        } else {
+            const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
+            glm::SerState st;
+            v = fmaf(cp[ch++], glm::sersic_forward(C, ux, uy, st), v);
+        }
+    }
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // mc_basis: grid (bs, ceil(nsub*nsub/256)), block 256
 // ---------------------------------------------------------------------------
@@ -400,7 +451,7 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
     mc_beta<false>(P, M, pc, X, Y, bx, by, nullptr);
     float* Bp = M.B + (size_t)p * M.D * nsub * M.BW + (size_t)y * M.BW + P.pb + x;
     const size_t cstride = (size_t)nsub * M.BW;
-    if (!M.lstsq) {
+    if (!M.lstsq && !M.famp) {
         // fixed amplitudes: the sum of every light, NaN -> 0 (tf/simulator.py:74-81)
         float v = 0.0f;
         for (int i = P.nlens; i < P.ncomp; ++i) {
@@ -413,6 +464,11 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
         Bp[0] = isnan(v) ? 0.0f : v;
         return;
     }
+    // least squares: NaN -> 0 per channel (tf/simulator.py:125); fixed per-channel
+    // amplitudes: the weighted sum of the channels is the simulated sub-pixel, and a NaN
+    // there zeroes every channel of it (tf/simulator.py:74-81)
+    const float* cp = M.coef + (size_t)p * M.D;
+    float wsum = 0.0f;
     int ch = 0;
     for (int i = P.nlens; i < P.ncomp; ++i) {
         const DComp& cd = P.comp[i];
@@ -423,7 +479,8 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
             const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
             glm::SerState st;
             float v = glm::sersic_forward(C, ux, uy, st);
-            Bp[cstride * ch++] = isnan(v) ? 0.0f : v;
+            if (M.famp) wsum = fmaf(cp[ch], v, wsum);
+            Bp[cstride * ch++] = (isnan(v) && !M.famp) ? 0.0f : v;
         } else if (cd.kind == glm::K_SHAPELETS) {
             const float beta = cst[0];
             const float u = (ux - cst[1]) / beta, w = (uy - cst[2]) / beta;
@@ -436,12 +493,16 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
 #pragma unroll
                 for (int j = 0; j <= so; ++j) {  // (n1, n2) = (so - j, j): order of shapelets.py:27-46
                     const float v = pu[so - j] * pw[j];
-                    Bp[cstride * (ch + so * (so + 1) / 2 + j)] = isnan(v) ? 0.0f : v;
+                    const int c = ch + so * (so + 1) / 2 + j;
+                    if (M.famp) wsum = fmaf(cp[c], v, wsum);
+                    Bp[cstride * c] = (isnan(v) && !M.famp) ? 0.0f : v;
                 }
             }
             ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
         }
     }
+    if (M.famp && isnan(wsum))
+        for (int c = 0; c < M.D; ++c) Bp[cstride * c] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1146,7 +1207,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
         LensStates ls;
         float bx, by;
         mc_beta<true, LM>(P, M, pc, X, Y, bx, by, &ls);
-        if (!M.lstsq) {
+        if (!M.lstsq && !M.famp) {
             // the sum of the lights: a NaN zeroed the whole sub-pixel in the forward pass
             float v = 0.0f;
             for (int i = P.nlens; i < P.ncomp; ++i) {
@@ -1157,6 +1218,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
             }
             if (isnan(v)) continue;
         }
+        if (SHP && M.famp && isnan(mc_famp_sum(P, M, s_c, cp, X, Y, bx, by))) continue;
         int ch = 0;
         float gbx = 0.0f, gby = 0.0f;
 #pragma unroll
@@ -1172,7 +1234,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
                 const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
                 glm::SerState ss_;
                 const float v = glm::sersic_forward<true>(C, ux, uy, ss_);
-                const int c = M.lstsq ? ch++ : 0;
+                const int c = (M.lstsq || M.famp) ? ch++ : 0;
                 if (isnan(v)) continue;  // least squares: this channel was zeroed
                 float gdx, gdy;
                 glm::sersic_adjoint(C, ss_, g * cp[c], acc, gdx, gdy);
@@ -1352,6 +1414,33 @@ __global__ __launch_bounds__(256) void mc_grad_m(const McTable T) {
 }
 
 // ---------------------------------------------------------------------------
+// mc_ampgrad (fixed per-channel amplitudes, single plans): the image is
+// sum_c cf a_c X_c, so dL/da_c = cf sum_pix dL/dimage X_c.  grid (particles x D),
+// block 256: one channel of one particle per block, a fixed-order reduction.
+// ---------------------------------------------------------------------------
+template <bool MULTI>
+__global__ __launch_bounds__(256) void mc_ampgrad(const McTable T) {
+    static_assert(!MULTI, "single plans only");
+    const DPlan& P = T.one.P;
+    const McPlan& M = T.one.M;
+    const int D = M.D;
+    const int p = blockIdx.x / D, c = blockIdx.x - (blockIdx.x / D) * D;
+    const int N = P.N, NP = N * N, GW = N + 2 * M.gp;
+    const float* Xc = M.X + ((size_t)p * D + c) * NP;
+    const float* G = M.gimg + (size_t)p * GW * GW;
+    float acc = 0.0f;
+    for (int pix = threadIdx.x; pix < NP; pix += blockDim.x) {
+        const int I = pix / N, J = pix - (pix / N) * N;
+        acc = fmaf(G[(I + M.gp) * GW + (J + M.gp)], Xc[pix], acc);
+    }
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) M.gamp[(size_t)p * D + c] = P.cf * (red[0] + red[1] + red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------
 // mc_finish: grid bs, block 64.  Statistics -> dlp/dz (or dL/dx).
 // ---------------------------------------------------------------------------
 template <bool MULTI>
@@ -1395,7 +1484,11 @@ __global__ void mc_finish(const McTable T, const McArgs A) {
             } else if (cd.kind == glm::K_SERSIC_ELLIPSE || cd.kind == glm::K_SERSIC) {
                 glm::sersic_finish(*reinterpret_cast<const glm::SerC*>(cst), acc, g);
                 np_ = cd.kind == glm::K_SERSIC_ELLIPSE ? 6 : 4;  // no Ie in least-squares mode
-                if (!cd.lstsq) np_ += 1;                        // fixed amplitudes: Ie too
+                if (!cd.lstsq && !M.famp) np_ += 1;             // fixed amplitudes: Ie too
+                if (M.famp) {                                   // per-channel amplitudes: Ie from mc_ampgrad
+                    const int ie = cd.kind == glm::K_SERSIC ? 4 : 6;
+                    if (cd.col[ie] >= 0) gx[cd.col[ie]] += M.gamp[(size_t)p * M.D + ch];
+                }
                 circ = cd.kind == glm::K_SERSIC;
                 ++ch;
             } else if (cd.kind == glm::K_SHAPELETS) {
@@ -1404,7 +1497,10 @@ __global__ void mc_finish(const McTable T, const McArgs A) {
                 g[1] = -acc[0] / beta;  // d/d center_x
                 g[2] = -acc[1] / beta;  // d/d center_y
                 np_ = 3;
-                ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
+                const int nl = (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
+                if (M.famp)  // sampled amplitudes amp00.. (columns amp_col0..)
+                    for (int k = 0; k < nl; ++k) gx[cd.amp_col0 + k] += M.gamp[(size_t)p * M.D + ch + k];
+                ch += nl;
             }
             for (int q = 0; q < np_; ++q)
                 if (cd.col[q] >= 0) gx[cd.col[q]] += circ ? g[q < 2 ? q : q + 2] : g[q];

@@ -140,6 +140,10 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
     }
     hipLaunchKernelGGL((gl::mc_image<MULTI>), dim3(np), dim3(256), 0, st, T, A, grad ? 1 : 0);
     GL_HIP(hipGetLastError());
+    if (grad && !MULTI && T.one.M.famp) {  // amplitude gradients from dL/dimage (single plans)
+        hipLaunchKernelGGL(gl::mc_ampgrad<false>, dim3(np * T.one.M.D), dim3(256), 0, st, T);
+        GL_HIP(hipGetLastError());
+    }
     if (grad) {
         if (T.tot_adjh > 0) {
             // both passes: compile-time (ss, kf4), one launch each per group of lenses

@@ -142,6 +142,16 @@ class Plan:
             c.interpolate = int(getattr(prof, "interpolate", 0))
             c.use_lstsq = int(getattr(prof, "use_lstsq", False))
             c.amp_col0 = -1
+            if prof._kind == nat.GL_SHAPELETS and not prof.use_lstsq:
+                # sampled amplitudes amp00.. (shapelets.py:32-36): contiguous columns in basis order
+                try:
+                    acols = [col_of[(role, i, a)] for a in prof._amp_names]
+                except KeyError as e:
+                    raise ValueError(f"shapelet amplitude {e.args[0][2]!r} is not in the layout") from None
+                if acols != list(range(acols[0], acols[0] + len(acols))):
+                    raise ValueError("shapelet amplitudes must occupy contiguous columns in basis order")
+                c.amp_col0 = acols[0]
+                used.update((role, i, a) for a in prof._amp_names)
         extra = [k for k in col_of if k not in used]
         if extra:
             raise ValueError(f"layout columns not used by the physical model: {extra}")

@@ -53,7 +53,9 @@ enum gl_profile_kind {
     GL_TNFW = 5,           /* params: Rs, alpha_Rs, r_trunc, center_x, center_y   (tnfw.py:12) */
     GL_SERSIC = 10,        /* params: R_sersic, n_sersic, center_x, center_y, Ie   (sersic.py:23) */
     GL_SERSIC_ELLIPSE = 11,/* params: R_sersic, n_sersic, e1, e2, center_x, center_y, Ie (sersic.py:64) */
-    GL_SHAPELETS = 12,     /* params: beta, center_x, center_y (+ amplitudes)      (shapelets.py:17) */
+    GL_SHAPELETS = 12,     /* params: beta, center_x, center_y; amplitudes solved by least squares
+                              (use_lstsq) or sampled at columns amp_col0.. (shapelets.py:17,53-75);
+                              HBM-staged pipeline only */
     GL_CORE_SERSIC = 13    /* params: R_sersic, n_sersic, Rb, alpha, gamma, e1, e2, center_x, center_y, Ie
                               (sersic.py:75-88); resident-image (fused) kernel only */
 };

@@ -42,8 +42,12 @@ def oracle_model_lists(phys_model, nested):
             lenses.append(("EPL", dict(p, niter=prof.niter)))
         else:
             lenses.append((prof._name, dict(p)))
-    ll = [(prof._name, dict(p)) for prof, p in zip(phys_model.lens_light, groups[1])]
-    src = [(prof._name, dict(p)) for prof, p in zip(phys_model.source_light, groups[2])]
+    def light(prof, p):
+        if prof._name == "SHAPELETS":  # sampled amplitudes amp00.. in p
+            return (prof._name, dict(p), prof.n_max, prof.interpolate)
+        return (prof._name, dict(p))
+    ll = [light(prof, p) for prof, p in zip(phys_model.lens_light, groups[1])]
+    src = [light(prof, p) for prof, p in zip(phys_model.source_light, groups[2])]
     return lenses, ll, src
 
 
@@ -71,10 +75,10 @@ def oracle_simulate(cam, phys_model, nested, per_particle_series=True):
             ax, ay = np64.MASS[kind](X, Y, **p)
         bx, by = bx - ax, by - ay
     img = 0.0
-    for kind, p in ll:
-        img = img + np64.light_eval(kind, X.astype(np.float64), Y.astype(np.float64), p)
-    for kind, p in src:
-        img = img + np64.light_eval(kind, bx, by, p)
+    for kind, p, *extra in ll:
+        img = img + np64.light_eval(kind, X.astype(np.float64), Y.astype(np.float64), p, *extra)
+    for kind, p, *extra in src:
+        img = img + np64.light_eval(kind, bx, by, p, *extra)
     img = np.where(np.isnan(img), 0.0, img)
     return np64.conv_pool(img, cam.kernel, cam.ss) * cam.conversion_factor
 
