@@ -135,11 +135,28 @@ class LensSimulator:
         cols = [_as_batch(by_role[role][i][key], bs, self.device) for role, i, key in self.layout]
         return torch.stack(cols, dim=1).contiguous()
 
+    def undeflected(self):
+        """(plan, light columns) of the lights alone: ``no_deflection`` evaluates the
+        sources on the image-plane grid (tf/simulator.py:74-75, 116-117), which is the
+        same model without its lenses; the lens parameters take no part."""
+        if getattr(self, "_undeflected", None) is None:
+            from .model import PhysicalModel
+            phys = PhysicalModel([], self.phys_model.lens_light, self.phys_model.source_light)
+            lay = layout_from_model(phys)
+            col_of = {k: c for c, k in enumerate(self.layout)}
+            self._undeflected = (phys, lay, [col_of[k] for k in lay])
+        return self._undeflected
+
     def simulate(self, params, no_deflection=False):
-        """Images (bs, N, N), squeezed like the reference (tf/simulator.py:65-97)."""
-        if no_deflection:
-            raise NotImplementedError("no_deflection is not supported by the fused kernels")
+        """Images (bs, N, N), squeezed like the reference (tf/simulator.py:65-97).
+        ``no_deflection`` evaluates the sources on the undeflected grid (:74-75)."""
         x = params if torch.is_tensor(params) else self.params_to_x(params)
+        if no_deflection:
+            phys, lay, cols = self.undeflected()
+            if getattr(self, "_plan_nd", None) is None:
+                self._plan_nd = Plan(phys, self.camera, lay)
+            img = _SimulateFn.apply(self._plan_nd, x[:, cols].contiguous())
+            return img.squeeze()
         img = _SimulateFn.apply(self.plan, x)
         return img.squeeze()
 
