@@ -39,10 +39,15 @@ EXPORTED_SYMBOLS = (
     "gl_map_adam_step_lenses",
     "gl_map_adam_step_noalloc",
     "gl_batch_create",
+    "gl_batch_create_ws",
+    "gl_batch_workspace_bytes",
+    "gl_batch_num_chunks",
     "gl_batch_destroy",
     "gl_batch_size",
     "gl_batch_log_prob",
     "gl_batch_map_adam_step",
+    "gl_svi_diag_sample",
+    "gl_svi_diag_update",
     "gl_cnn_create",
     "gl_cnn_destroy",
     "gl_cnn_num_params",
@@ -143,11 +148,20 @@ _SIGS = {
                                   ctypes.c_float, ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
     "gl_batch_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_vp)],
                         ctypes.c_int),
+    "gl_batch_create_ws": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t,
+                            ctypes.POINTER(_vp)], ctypes.c_int),
+    "gl_batch_workspace_bytes": ([_vp], ctypes.c_size_t),
+    "gl_batch_num_chunks": ([_vp], ctypes.c_int),
     "gl_batch_destroy": ([_vp], ctypes.c_int),
     "gl_batch_size": ([_vp], ctypes.c_int),
     "gl_batch_log_prob": ([_vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "gl_batch_map_adam_step": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                 ctypes.c_float, _vp, _vp, _vp], ctypes.c_int),
+    "gl_svi_diag_sample": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _vp, _vp, _vp,
+                            _vp], ctypes.c_int),
+    "gl_svi_diag_update": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, ctypes.c_int, _vp],
+                           ctypes.c_int),
     "gl_cnn_create": ([ctypes.POINTER(CnnDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "gl_cnn_destroy": ([_vp], ctypes.c_int),
     "gl_cnn_num_params": ([_vp], ctypes.c_int),

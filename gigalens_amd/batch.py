@@ -103,12 +103,16 @@ class MultiLensMAP:
 
 
 class LensBatch:
-    """Lens systems with different cameras and particle counts in one launch per
-    stage.  ``lenses`` is a sequence of (sim_config, observed_image, n_particles);
-    all share the physical model, the prior and the noise model
-    (``ForwardProbModel``, tf/model.py:53-86).  Particles are lens-major."""
+    """Lens systems with different cameras and particle counts, fitted together.
+    ``lenses`` is a sequence of (sim_config, observed_image, n_particles); all
+    share the physical model, the prior and the noise model (``ForwardProbModel``,
+    tf/model.py:53-86).  Particles are lens-major.  Lenses whose image fits the
+    resident-image kernel run there (one fused launch each); the others run the
+    HBM-staged pipeline in chunks whose workspaces share an arena of at most
+    ``workspace_gib``."""
 
-    def __init__(self, phys_model, prior, lenses: Sequence, background_rms: float, exp_time: float, device="cuda"):
+    def __init__(self, phys_model, prior, lenses: Sequence, background_rms: float, exp_time: float, device="cuda",
+                 workspace_gib: float = 32.0, fused: bool = True):
         self.device = torch.device(device)
         self.prior = prior
         self.bij = dist.ChainBijector(prior.experimental_default_event_space_bijector(), dist.PackBijector(prior))
@@ -117,20 +121,25 @@ class LensBatch:
         self.plans = []
         self.counts = []
         for cfg, obs, n in lenses:
-            # the batch owns the workspace of its particles: the plan itself keeps one particle's
+            # staged lenses keep their workspace in the batch's arena: the plan itself holds one particle's
             self.plans.append(Plan(phys_model, Camera(cfg), layout_from_paths(phys_model, paths), prior=prior,
                                    observed=np.asarray(obs, np.float32), background_rms=background_rms,
-                                   exp_time=exp_time, likelihood=nat.GL_LIK_FORWARD, staged=True, max_batch=1))
+                                   exp_time=exp_time, likelihood=nat.GL_LIK_FORWARD, staged=not fused, max_batch=1))
             self.counts.append(int(n))
         self.n_lenses = len(self.plans)
+        self.fused = [p.info()["lds_bytes"] > 0 for p in self.plans]
         handles = (ctypes.c_void_p * self.n_lenses)(*[p._h.value for p in self.plans])
         counts = (ctypes.c_int * self.n_lenses)(*self.counts)
         h = ctypes.c_void_p()
         lib = nat.lib()
-        nat.check(lib.gl_batch_create(handles, self.n_lenses, counts, ctypes.byref(h)), "gl_batch_create")
+        nat.check(lib.gl_batch_create_ws(handles, self.n_lenses, counts, int(workspace_gib * (1 << 30)),
+                                         ctypes.byref(h)), "gl_batch_create_ws")
         self._h = h
         self._lib = lib
         self.total = int(lib.gl_batch_size(h))
+        self.workspace_bytes = int(lib.gl_batch_workspace_bytes(h))
+        self.num_chunks = int(lib.gl_batch_num_chunks(h))
+        self.first = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)
         self.lens = torch.arange(self.n_lenses, device=self.device, dtype=torch.int32).repeat_interleave(
             torch.as_tensor(self.counts, device=self.device))
         self.lp_history = None
@@ -145,21 +154,28 @@ class LensBatch:
                 pass
             self._h = None
 
-    def log_prob(self, z: torch.Tensor, want_grad: bool = False):
+    def log_prob(self, z: torch.Tensor, want_grad: bool = False, out=None):
+        """(lp, chi2, dlp/dz or None) of every particle; ``out`` = (lp, chi2, grad)
+        buffers to write instead of new tensors."""
         z = z.to(self.device, torch.float32).contiguous()
         if z.shape != (self.total, self.d):
             raise ValueError(f"z must be ({self.total}, {self.d}): every particle of every lens, lens-major")
-        lp = torch.empty(self.total, device=self.device)
-        chi2 = torch.empty(self.total, device=self.device)
-        grad = torch.empty_like(z) if want_grad else None
+        if out is None:
+            lp = torch.empty(self.total, device=self.device)
+            chi2 = torch.empty(self.total, device=self.device)
+            grad = torch.empty_like(z) if want_grad else None
+        else:
+            lp, chi2, grad = out
         nat.check(self._lib.gl_batch_log_prob(self._h, _ptr(z), _ptr(lp), _ptr(chi2), _ptr(grad), _stream()),
                   "gl_batch_log_prob")
         return lp, chi2, grad
 
     def run(self, z0: torch.Tensor, num_steps: int = 350, optimizer: Optional[Adam] = None,
-            record_every: int = 1, graph: bool = False) -> torch.Tensor:
+            record_every: int = 1, graph: bool = False, graph_steps: int = 0) -> torch.Tensor:
         """Keras-Adam MAP of every lens at once from z0 (total, d); histories as in
-        MultiLensMAP; ``graph=True`` records the steps into one HIP graph first."""
+        MultiLensMAP.  ``graph=True`` records the steps into HIP graphs first (of
+        ``graph_steps`` steps each, default all) and replays them; ``step_seconds``
+        then holds the measured time of each graph (per step)."""
         optimizer = optimizer if optimizer is not None else Adam(1e-2)
         z = z0.to(self.device, torch.float32).contiguous().clone()
         if z.shape != (self.total, self.d):
@@ -170,60 +186,109 @@ class LensBatch:
         chi_h = torch.empty((nrec, self.total), device=self.device)
         scratch = torch.empty((2, self.total), device=self.device)
 
-        def steps():
-            for s in range(num_steps):
+        def steps(s0, s1):
+            for s in range(s0, s1):
                 rec = s % record_every == 0
                 lp = lp_h[s // record_every] if rec else scratch[0]
                 chi = chi_h[s // record_every] if rec else scratch[1]
                 nat.check(self._lib.gl_batch_map_adam_step(
-                    self._h, _ptr(z), _ptr(m), _ptr(v), optimizer.iterations + 1,
-                    float(optimizer.lr(optimizer.iterations)), float(optimizer.beta_1), float(optimizer.beta_2),
+                    self._h, _ptr(z), _ptr(m), _ptr(v), s + 1,
+                    float(optimizer.lr(optimizer.iterations + s)), float(optimizer.beta_1), float(optimizer.beta_2),
                     float(optimizer.epsilon), _ptr(lp), _ptr(chi), _stream()), "gl_batch_map_adam_step")
-                optimizer.iterations += 1
+        self.step_seconds = []
         if graph:
             from .graphs import capture
-            capture(steps).replay()
+            gs = graph_steps if graph_steps > 0 else num_steps
+            for s0 in range(0, num_steps, gs):
+                s1 = min(num_steps, s0 + gs)
+                g = capture(lambda: steps(s0, s1))
+                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ev0.record()
+                g.replay()
+                ev1.record()
+                self.step_seconds.append((s0, s1, ev0, ev1))
+                del g
+            torch.cuda.synchronize()
+            self.step_seconds = [(s0, s1, a.elapsed_time(b) * 1e-3 / (s1 - s0)) for s0, s1, a, b in self.step_seconds]
         else:
-            steps()
+            steps(0, num_steps)
+        optimizer.iterations += num_steps
         self.lp_history, self.chi2_history = lp_h, chi_h
         return z
 
+    def per_lens_best(self):
+        """(best lp, best chi2) of every lens at every recorded step, each (steps, lenses), on the host."""
+        idx = self.lens.long()[None].expand(self.lp_history.shape[0], -1)
+        lp = torch.nan_to_num(self.lp_history, nan=-float("inf"))
+        chi = torch.nan_to_num(self.chi2_history, nan=float("inf"))
+        L = self.n_lenses
+        best_lp = torch.full((lp.shape[0], L), -float("inf"), device=self.device).scatter_reduce(
+            1, idx, lp, reduce="amax")
+        best_chi = torch.full((chi.shape[0], L), float("inf"), device=self.device).scatter_reduce(
+            1, idx, chi, reduce="amin")
+        return best_lp.cpu().numpy(), best_chi.cpu().numpy()
+
+    def best_particles(self, z: torch.Tensor) -> torch.Tensor:
+        """Each lens's highest-log-posterior particle of z (total, d) -> (lenses, d)."""
+        lp, _, _ = self.log_prob(z)
+        lp = torch.nan_to_num(lp, nan=-float("inf"))
+        best = torch.full((self.n_lenses,), -float("inf"), device=self.device).scatter_reduce(
+            0, self.lens.long(), lp, reduce="amax")
+        is_best = lp == best[self.lens.long()]
+        pos = torch.arange(self.total, device=self.device)
+        first = torch.full((self.n_lenses,), self.total, device=self.device, dtype=torch.long).scatter_reduce(
+            0, self.lens.long(), torch.where(is_best, pos, torch.full_like(pos, self.total)), reduce="amin")
+        return z[first.clamp(max=self.total - 1)].contiguous()
+
     def svi_diag(self, loc: torch.Tensor, num_steps: int = 1500, optimizer: Optional[Adam] = None,
-                 init_scale: float = 1e-3, seed: int = 2):
+                 init_scale: float = 1e-3, seed: int = 2, graph: bool = False):
         """Diagonal-Gaussian SVI of every lens at once (the surrogate of
-        tf/inference.py:43-73 restricted to a diagonal scale).  The batch's
-        particle count per lens is the number of Monte-Carlo samples per step.
-        loc: (lenses, d) starting means.  Reparameterised ELBO gradient from one
-        batched log_prob + gradient launch per step:
-          d/d loc = mean_s grad_z lp,  d/d log sigma = mean_s grad_z lp * eps * sigma + 1.
-        Returns (loc, log_sigma, losses (steps, lenses))."""
+        tf/inference.py:43-73 restricted to a diagonal scale): every lens of this
+        batch holds the same number of particles, used as the Monte-Carlo samples of
+        each step.  One step = ``gl_svi_diag_sample`` (on-device Philox normals),
+        the batched log_prob + gradient, ``gl_svi_diag_update`` (per-lens ELBO
+        gradient + Keras Adam); ``graph=True`` records one step into a HIP graph and
+        replays it (a constant learning rate).  loc: (lenses, d) starting means.
+        Returns (loc, log_scale, losses (steps, lenses))."""
         optimizer = optimizer if optimizer is not None else Adam(1e-3)
-        L, d = loc.shape
-        counts = torch.as_tensor(self.counts, device=self.device)
-        loc = loc.to(self.device, torch.float32).clone()
-        logs = torch.full_like(loc, float(np.log(init_scale)))
-        g = torch.Generator(device=self.device)
-        g.manual_seed(int(seed))
-        idx = self.lens.long()
-        inv_n = (1.0 / counts.float())[:, None]
-        m = [torch.zeros_like(loc), torch.zeros_like(logs)]
-        v = [torch.zeros_like(loc), torch.zeros_like(logs)]
+        n_vi = self.counts[0]
+        if any(c != n_vi for c in self.counts):
+            raise ValueError("svi_diag needs the same number of particles (samples) for every lens")
+        if graph and callable(optimizer.learning_rate):
+            raise ValueError("graph=True records a constant learning rate")
+        L, d = self.n_lenses, self.d
+        loc = loc.to(self.device, torch.float32).contiguous().clone()
+        if loc.shape != (L, d):
+            raise ValueError(f"loc must be ({L}, {d})")
+        raw = torch.full_like(loc, float(np.log(init_scale - 1e-6)))
+        m = torch.zeros((2, L, d), device=self.device)
+        v = torch.zeros((2, L, d), device=self.device)
+        step = torch.zeros(L, device=self.device, dtype=torch.int32)
+        z = torch.empty((self.total, d), device=self.device)
+        eps = torch.empty_like(z)
+        lp = torch.empty(self.total, device=self.device)
+        chi2 = torch.empty(self.total, device=self.device)
+        gz = torch.empty_like(z)
         losses = torch.empty((num_steps, L), device=self.device)
-        for s in range(num_steps):
-            eps = torch.randn((self.total, d), device=self.device, generator=g)
-            sig = torch.exp(logs)
-            z = (loc[idx] + sig[idx] * eps).contiguous()
-            lp, _, gz = self.log_prob(z, want_grad=True)
-            # loss = -ELBO = -(mean lp + sum log sigma) + const
-            lp_mean = torch.zeros(L, device=self.device).index_add_(0, idx, lp) * inv_n[:, 0]
-            losses[s] = -(lp_mean + logs.sum(1))
-            g_loc = -torch.zeros_like(loc).index_add_(0, idx, gz) * inv_n
-            g_logs = -(torch.zeros_like(logs).index_add_(0, idx, gz * eps * sig[idx]) * inv_n + 1.0)
-            lr_t = optimizer.lr(optimizer.iterations) * np.sqrt(1 - optimizer.beta_2 ** (s + 1)) / \
-                (1 - optimizer.beta_1 ** (s + 1))
-            for k, (par, gr) in enumerate(((loc, g_loc), (logs, g_logs))):
-                m[k].mul_(optimizer.beta_1).add_((1 - optimizer.beta_1) * gr)
-                v[k].mul_(optimizer.beta_2).add_((1 - optimizer.beta_2) * gr * gr)
-                par.sub_(lr_t * m[k] / (torch.sqrt(v[k]) + optimizer.epsilon))
-            optimizer.iterations += 1
-        return loc, logs, losses
+        lib = self._lib
+
+        def one(s):
+            nat.check(lib.gl_svi_diag_sample(_ptr(loc), _ptr(raw), L, n_vi, d, ctypes.c_ulonglong(int(seed)),
+                                             _ptr(step), _ptr(z), _ptr(eps), _stream()), "gl_svi_diag_sample")
+            self.log_prob(z, out=(lp, chi2, gz))
+            nat.check(lib.gl_svi_diag_update(_ptr(lp), _ptr(gz), _ptr(eps), L, n_vi, d, _ptr(loc), _ptr(raw), _ptr(m),
+                                             _ptr(v), _ptr(step), float(optimizer.lr(optimizer.iterations + s)),
+                                             float(optimizer.beta_1), float(optimizer.beta_2),
+                                             float(optimizer.epsilon), _ptr(losses), num_steps, _stream()),
+                      "gl_svi_diag_update")
+        if graph:
+            from .graphs import capture
+            g = capture(lambda: one(0))
+            for _ in range(num_steps):
+                g.replay()
+        else:
+            for s in range(num_steps):
+                one(s)
+        optimizer.iterations += num_steps
+        log_scale = torch.log(torch.exp(raw) + 1e-6)
+        return loc, log_scale, losses

@@ -185,12 +185,39 @@ static int dev_alloc(std::vector<void*>& owned, size_t bytes, void** out, bool z
     return GL_OK;
 }
 
-// Workspace of a staged lens for `mb` particles, with the conv launched at
-// template width kf4 (>= the lens's own): staged row width BW and the zero-padded
-// weight rows follow it.  W: the lens's folded weights (kf x kf, host).
-static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int ksplit_cap,
-                        const std::vector<float>& W, std::vector<void*>& owned) {
-    const int N = P.N, ss = P.ss, nsub = P.nsub, kf = P.kf, pb = P.pb, D = M.D;
+// Where the scratch buffers of a staged workspace go: device allocations owned by a
+// plan or batch (zeroed when asked), or consecutive 256-byte aligned slices of a
+// shared arena.  Arena memory is not cleared: the kernels write every element they
+// later read (the conv's column margins and the padded adjoint image's margins
+// included), so lenses of different shapes may use it one after the other.
+struct ScratchAlloc {
+    std::vector<void*>* owned = nullptr;
+    char* arena = nullptr;
+    size_t off = 0, cap = 0;
+    bool measure = false;  // count the bytes only
+};
+static int scratch_alloc(ScratchAlloc& a, size_t bytes, void** out, bool zero) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (a.measure) {
+        a.off += bytes;
+        *out = nullptr;
+        return GL_OK;
+    }
+    if (a.arena) {
+        if (a.off + bytes > a.cap) return fail(GL_ERR_ARG, "staged workspace exceeds its arena");
+        *out = a.arena + a.off;
+        a.off += bytes;
+        return GL_OK;
+    }
+    return dev_alloc(*a.owned, bytes, out, zero);
+}
+
+// Constants of a staged lens whose conv is launched at template width kf4 (>= the
+// lens's own): the staged row width BW and the zero-padded weight rows follow it.
+// W: the lens's folded weights (kf x kf, host).
+static int mc_consts_upload(const gl::DPlan& P, gl::McPlan& M, int kf4, const std::vector<float>& W,
+                            std::vector<void*>& owned) {
+    const int N = P.N, ss = P.ss, nsub = P.nsub, kf = P.kf, pb = P.pb;
     int rc;
     void* dptr = nullptr;
     M.BW = ((ss * (N - 1) + 4 * kf4 > pb + nsub ? ss * (N - 1) + 4 * kf4 : pb + nsub) + 3) & ~3;
@@ -201,40 +228,48 @@ static int mc_workspace(const gl::DPlan& P, gl::McPlan& M, int mb, int kf4, int 
     if ((rc = dev_alloc(owned, wp.size() * sizeof(float), &dptr, false))) return rc;
     GL_HIP(hipMemcpy(dptr, wp.data(), wp.size() * sizeof(float), hipMemcpyHostToDevice));
     M.wpad = (const float*)dptr;
+    return GL_OK;
+}
+
+// Scratch of a staged lens for `mb` particles (after mc_consts_upload).
+static int mc_scratch(const gl::DPlan& P, gl::McPlan& M, int mb, int ksplit_cap, ScratchAlloc& sa) {
+    const int N = P.N, ss = P.ss, nsub = P.nsub, D = M.D;
+    int rc;
+    void* dptr = nullptr;
     const int GW = N + 2 * M.gp;
     const int gram_nt = M.gram_nb * (M.gram_nb + 1) / 2;
-    if ((rc = dev_alloc(owned, (size_t)mb * M.pstride * 4, &dptr, true))) return rc;
+    if ((rc = scratch_alloc(sa, (size_t)mb * M.pstride * 4, &dptr, true))) return rc;
     M.consts = (float*)dptr;
-    if ((rc = dev_alloc(owned, (size_t)mb * D * nsub * M.BW * 4, &dptr, true))) return rc;  // zero margins stay zero
+    if ((rc = scratch_alloc(sa, (size_t)mb * D * nsub * M.BW * 4, &dptr, true))) return rc;
     M.B = (float*)dptr;
-    if ((rc = dev_alloc(owned, (size_t)mb * D * N * N * 4, &dptr, false))) return rc;
+    if ((rc = scratch_alloc(sa, (size_t)mb * D * N * N * 4, &dptr, false))) return rc;
     M.X = (float*)dptr;
     M.gpart = nullptr;
     if (M.lstsq) {
-        if ((rc = dev_alloc(owned, (size_t)std::min(mb * ksplit_cap, 1024 + mb) * 4 * gram_nt * 256 * 8, &dptr,
-                            false)))
+        if ((rc = scratch_alloc(sa, (size_t)std::min(mb * ksplit_cap, 1024 + mb) * 4 * gram_nt * 256 * 8, &dptr,
+                                false)))
             return rc;
         M.gpart = (double*)dptr;
     }
-    if ((rc = dev_alloc(owned, (size_t)mb * D * 4, &dptr, false))) return rc;
+    if ((rc = scratch_alloc(sa, (size_t)mb * D * 4, &dptr, false))) return rc;
     M.coef = (float*)dptr;
-    if ((rc = dev_alloc(owned, (size_t)mb * GW * GW * 4, &dptr, true))) return rc;  // zero margins stay zero
+    if ((rc = scratch_alloc(sa, (size_t)mb * GW * GW * 4, &dptr, true))) return rc;
     M.gimg = (float*)dptr;
-    if ((rc = dev_alloc(owned, (size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
+    if ((rc = scratch_alloc(sa, (size_t)mb * gl::kMcStat * 4, &dptr, true))) return rc;
     M.stats = (float*)dptr;
     M.gamp = nullptr;
     if (M.famp) {
-        if ((rc = dev_alloc(owned, (size_t)mb * D * 4, &dptr, true))) return rc;
+        if ((rc = scratch_alloc(sa, (size_t)mb * D * 4, &dptr, true))) return rc;
         M.gamp = (float*)dptr;
     }
     M.U = nullptr;
     if (M.adj_rank > 0) {
-        if ((rc = dev_alloc(owned, (size_t)mb * M.adj_rank * N * ss * N * 4, &dptr, false))) return rc;
+        if ((rc = scratch_alloc(sa, (size_t)mb * M.adj_rank * N * ss * N * 4, &dptr, false))) return rc;
         M.U = (float*)dptr;
     }
     M.H = nullptr;
     if (M.sep_rank > 0) {  // separable channel conv: horizontal-pass intermediate
-        if ((rc = dev_alloc(owned, (size_t)mb * D * M.sep_rank * nsub * N * 4, &dptr, false))) return rc;
+        if ((rc = scratch_alloc(sa, (size_t)mb * D * M.sep_rank * nsub * N * 4, &dptr, false))) return rc;
         M.H = (float*)dptr;
     }
     return GL_OK;
@@ -412,7 +447,12 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
         M.shp_pref = (const float*)dptr;
     }
     // workspace for max_batch particles (allocated here once; calls never allocate)
-    if ((rc = mc_workspace(P, M, mb, p->kf4, p->ksplit_cap, W, p->dev))) return rc;
+    if ((rc = mc_consts_upload(P, M, p->kf4, W, p->dev))) return rc;
+    {
+        ScratchAlloc sa;
+        sa.owned = &p->dev;
+        if ((rc = mc_scratch(P, M, mb, p->ksplit_cap, sa))) return rc;
+    }
     if (M.adj_rank > 0) {
         if ((rc = upload(p, asa.data(), asa.size() * sizeof(float), &dptr))) return rc;
         M.asa = (const float*)dptr;
@@ -1134,15 +1174,28 @@ extern "C" int gl_map_adam_step_noalloc(const gl_plan* plan, float* z, float* m,
 
 // ---------------------------------------------------------------------------
 // Batches of lens systems with their own image sizes, PSFs and particle counts
-// (no reference counterpart): every stage of the staged pipeline is one launch
-// over all lenses, addressed through a device lens table.
+// (no reference counterpart).  Staged lenses are grouped into chunks whose
+// workspaces share one arena of bounded size; each chunk runs every stage of the
+// staged pipeline as one launch over its lenses, addressed through a device lens
+// table.  Lenses whose plan runs on the resident-image (fused) kernel take one
+// fused launch each.  All launches of a call are ordered on its stream.
 // ---------------------------------------------------------------------------
-struct gl_batch {
-    int n, d, total, ss_launch, kf4;
-    std::vector<ConvGroup> groups;  // conv launches by (ss, kf4)
-    int mc_model;  // 1: every lens is of the reference default model (specialised kernels)
+struct BatchChunk {
     gl::McTable T;
+    std::vector<ConvGroup> groups;  // conv launches by (ss, kf4)
+    int ss_launch, kf4;
+    int mc_model;  // 1: every lens is of the reference default model (specialised kernels)
+};
+struct FusedLens {
+    const gl_plan* plan;
+    int first, count;
+};
+struct gl_batch {
+    int n, d, total;
+    std::vector<BatchChunk> chunks;
+    std::vector<FusedLens> fused;
     std::vector<int> counts;
+    size_t arena_bytes;
     std::vector<void*> dev;
     mutable StreamGuard guard;  // orders calls across streams (the workspace is shared)
 };
@@ -1155,58 +1208,28 @@ extern "C" int gl_batch_destroy(gl_batch* b) {
     return GL_OK;
 }
 
-extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* counts, gl_batch** out) {
-    if (out) *out = nullptr;
-    if (!plans || !counts || !out || n <= 0) return fail(GL_ERR_ARG, "gl_batch_create: bad arguments");
-    for (int i = 0; i < n; ++i) {
-        const gl_plan* q = plans[i];
-        if (!q || !q->mc || q->M.lstsq)
-            return fail(GL_ERR_ARG, "gl_batch_create: every plan must be a fixed-amplitude staged plan (desc.staged = 1)");
-        if (q->M.famp) return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: shapelet lights are not batched");
-        if (counts[i] <= 0) return fail(GL_ERR_ARG, "gl_batch_create: particle counts must be positive");
-        if (q->P.d != plans[0]->P.d || q->P.lik != plans[0]->P.lik)
-            return fail(GL_ERR_ARG, "gl_batch_create: plans must share the parameter count and likelihood");
-    }
-    gl_batch* b = new gl_batch();
-    if (b->guard.init() != hipSuccess) {
-        delete b;
-        return fail(GL_ERR_HIP, "gl_batch_create: event creation failed");
-    }
-    b->n = n;
-    b->d = plans[0]->P.d;
-    b->kf4 = 0;
-    b->ss_launch = plans[0]->P.ss;
-    b->mc_model = 1;
-    for (int i = 0; i < n; ++i)
-        if (plans[i]->mc_model != 1) b->mc_model = 0;
-    for (int i = 0; i < n; ++i) {
-        b->kf4 = std::max(b->kf4, plans[i]->kf4);
-        if (plans[i]->P.ss != b->ss_launch) b->ss_launch = 0;  // mixed supersampling: runtime ss
-    }
-    std::vector<gl::McLens> lens(n);
+// Device tables of one chunk: its lenses (global particle offsets in goff), the
+// prefix sums of every stage's work units, and the conv launch groups.
+static int batch_chunk(gl_batch* b, const std::vector<gl::McLens>& lens, const std::vector<int>& gfirst,
+                       const std::vector<int>& kf4s, BatchChunk& C) {
+    const int n = (int)lens.size();
+    std::vector<gl::McLens> L(lens);
     std::vector<int> pre_part(n + 1, 0), pre_basis(n + 1, 0), pre_conv(n + 1, 0), pre_adj(n + 1, 0),
         pre_grad(n + 1, 0), pre_adjh(n + 1, 0), pre_adjv(n + 1, 0);
     long acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<McUnits> units(n);
+    C.kf4 = 0;
+    C.ss_launch = L[0].P.ss;
     for (int i = 0; i < n; ++i) {
-        gl::McLens& E = lens[i];
-        memset(&E, 0, sizeof(E));
-        E.P = plans[i]->P;
-        E.M = plans[i]->M;
-        E.M.conv_ib = gl::kMcIB;  // batches launch the direct conv at the default rows per lane
-        // a separable lens keeps its rank: the workspace below holds its own H
-        int rc = mc_workspace(E.P, E.M, counts[i], plans[i]->kf4, 1, plans[i]->W_host, b->dev);
-        if (rc) { gl_batch_destroy(b); return rc; }
+        gl::McLens& E = L[i];
         E.first = (int)acc[0];
-        E.count = counts[i];
-        E.weight = 1.0f / (float)counts[i];
-        const McUnits u = mc_units(E.P, E.M, counts[i]);
+        E.goff = gfirst[i] - (int)acc[0];
+        const McUnits u = mc_units(E.P, E.M, E.count);
+        units[i] = u;
         acc[0] += u.part; acc[1] += u.basis; acc[2] += u.conv; acc[3] += u.adj; acc[4] += u.grad;
-        acc[5] += u.adjh;
-        acc[6] += u.adjv;
-        acc[7] += u.seph;
-        acc[8] += u.sepv;
+        acc[5] += u.adjh; acc[6] += u.adjv; acc[7] += u.seph; acc[8] += u.sepv;
         for (int k = 0; k < 9; ++k)
-            if (acc[k] > INT32_MAX) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: batch too large"); }
+            if (acc[k] > INT32_MAX) return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: chunk too large");
         pre_part[i + 1] = (int)acc[0];
         pre_basis[i + 1] = (int)acc[1];
         pre_conv[i + 1] = (int)acc[2];
@@ -1214,12 +1237,11 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         pre_grad[i + 1] = (int)acc[4];
         pre_adjh[i + 1] = (int)acc[5];
         pre_adjv[i + 1] = (int)acc[6];
+        C.kf4 = std::max(C.kf4, kf4s[i]);
+        if (E.P.ss != C.ss_launch) C.ss_launch = 0;  // mixed supersampling: runtime ss
     }
-    b->total = (int)acc[0];
-    b->counts.assign(counts, counts + n);
-    memset(&b->T, 0, sizeof(b->T));
+    int rc = GL_OK;
     void* dptr = nullptr;
-    int rc;
     auto put = [&](const void* host, size_t bytes) -> void* {
         if ((rc = dev_alloc(b->dev, bytes, &dptr, false))) return nullptr;
         if (hipMemcpy(dptr, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1228,92 +1250,186 @@ extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* co
         }
         return dptr;
     };
-    rc = GL_OK;
-    b->T.lens = (const gl::McLens*)put(lens.data(), lens.size() * sizeof(gl::McLens));
-    if (!rc) b->T.pre_part = (const int*)put(pre_part.data(), pre_part.size() * sizeof(int));
-    if (!rc) b->T.pre_basis = (const int*)put(pre_basis.data(), pre_basis.size() * sizeof(int));
-    if (!rc) b->T.pre_conv = (const int*)put(pre_conv.data(), pre_conv.size() * sizeof(int));
-    if (!rc) b->T.pre_adj = (const int*)put(pre_adj.data(), pre_adj.size() * sizeof(int));
-    if (!rc) b->T.pre_grad = (const int*)put(pre_grad.data(), pre_grad.size() * sizeof(int));
-    if (!rc) b->T.pre_adjh = (const int*)put(pre_adjh.data(), pre_adjh.size() * sizeof(int));
-    if (!rc) b->T.pre_adjv = (const int*)put(pre_adjv.data(), pre_adjv.size() * sizeof(int));
-    // conv groups by (ss, kf4)
+    memset(&C.T, 0, sizeof(C.T));
+    C.T.lens = (const gl::McLens*)put(L.data(), L.size() * sizeof(gl::McLens));
+    if (!rc) C.T.pre_part = (const int*)put(pre_part.data(), pre_part.size() * sizeof(int));
+    if (!rc) C.T.pre_basis = (const int*)put(pre_basis.data(), pre_basis.size() * sizeof(int));
+    if (!rc) C.T.pre_conv = (const int*)put(pre_conv.data(), pre_conv.size() * sizeof(int));
+    if (!rc) C.T.pre_adj = (const int*)put(pre_adj.data(), pre_adj.size() * sizeof(int));
+    if (!rc) C.T.pre_grad = (const int*)put(pre_grad.data(), pre_grad.size() * sizeof(int));
+    if (!rc) C.T.pre_adjh = (const int*)put(pre_adjh.data(), pre_adjh.size() * sizeof(int));
+    if (!rc) C.T.pre_adjv = (const int*)put(pre_adjv.data(), pre_adjv.size() * sizeof(int));
+    // conv launch groups by (ss, kf4): each group's prefix tables count only its lenses' units
     for (int i = 0; i < n && !rc; ++i) {
         bool seen = false;
-        for (const ConvGroup& g : b->groups) seen = seen || (g.ss == plans[i]->P.ss && g.kf4 == plans[i]->kf4);
+        for (const ConvGroup& g : C.groups) seen = seen || (g.ss == L[i].P.ss && g.kf4 == kf4s[i]);
         if (seen) continue;
         ConvGroup g;
-        g.ss = plans[i]->P.ss;
-        g.kf4 = plans[i]->kf4;
-        std::vector<int> pre(n + 1, 0);
-        long a = 0;
+        g.ss = L[i].P.ss;
+        g.kf4 = kf4s[i];
+        std::vector<int> pc(n + 1, 0), pv(n + 1, 0), pah(n + 1, 0), psh(n + 1, 0), psv(n + 1, 0);
+        long ac = 0, av = 0, aah = 0, ash = 0, asv = 0;
         for (int j = 0; j < n; ++j) {
-            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) a += mc_units(lens[j].P, lens[j].M, counts[j]).conv;
-            pre[j + 1] = (int)a;
-        }
-        g.tot = (int)a;
-        g.pre = (const int*)put(pre.data(), pre.size() * sizeof(int));
-        std::vector<int> prev(n + 1, 0);
-        long av = 0;
-        for (int j = 0; j < n; ++j) {
-            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) av += mc_units(lens[j].P, lens[j].M, counts[j]).adjv;
-            prev[j + 1] = (int)av;
-        }
-        g.adjv_tot = (int)av;
-        g.adjv_pre = (const int*)put(prev.data(), prev.size() * sizeof(int));
-        std::vector<int> preah(n + 1, 0);
-        long aah = 0;
-        for (int j = 0; j < n; ++j) {
-            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) aah += mc_units(lens[j].P, lens[j].M, counts[j]).adjh;
-            preah[j + 1] = (int)aah;
-        }
-        g.adjh_tot = (int)aah;
-        g.adjh_pre = (const int*)put(preah.data(), preah.size() * sizeof(int));
-        std::vector<int> preh(n + 1, 0), presv(n + 1, 0);
-        long ah = 0, asv = 0;
-        for (int j = 0; j < n; ++j) {
-            if (plans[j]->P.ss == g.ss && plans[j]->kf4 == g.kf4) {
-                const McUnits uj = mc_units(lens[j].P, lens[j].M, counts[j]);
-                ah += uj.seph;
-                asv += uj.sepv;
+            if (L[j].P.ss == g.ss && kf4s[j] == g.kf4) {
+                ac += units[j].conv;
+                av += units[j].adjv;
+                aah += units[j].adjh;
+                ash += units[j].seph;
+                asv += units[j].sepv;
             }
-            preh[j + 1] = (int)ah;
-            presv[j + 1] = (int)asv;
+            pc[j + 1] = (int)ac;
+            pv[j + 1] = (int)av;
+            pah[j + 1] = (int)aah;
+            psh[j + 1] = (int)ash;
+            psv[j + 1] = (int)asv;
         }
-        g.seph_tot = (int)ah;
+        g.tot = (int)ac;
+        g.adjv_tot = (int)av;
+        g.adjh_tot = (int)aah;
+        g.seph_tot = (int)ash;
         g.sepv_tot = (int)asv;
-        g.seph_pre = (const int*)put(preh.data(), preh.size() * sizeof(int));
-        g.sepv_pre = (const int*)put(presv.data(), presv.size() * sizeof(int));
-        b->groups.push_back(g);
+        g.pre = (const int*)put(pc.data(), pc.size() * sizeof(int));
+        if (!rc) g.adjv_pre = (const int*)put(pv.data(), pv.size() * sizeof(int));
+        if (!rc) g.adjh_pre = (const int*)put(pah.data(), pah.size() * sizeof(int));
+        if (!rc) g.seph_pre = (const int*)put(psh.data(), psh.size() * sizeof(int));
+        if (!rc) g.sepv_pre = (const int*)put(psv.data(), psv.size() * sizeof(int));
+        C.groups.push_back(g);
+    }
+    if (rc) return rc;
+    C.T.n = n;
+    C.T.tot_part = (int)acc[0];
+    C.T.tot_basis = (int)acc[1];
+    C.T.tot_conv = (int)acc[2];
+    C.T.tot_adj = (int)acc[3];
+    C.T.tot_grad = (int)acc[4];
+    C.T.tot_adjh = (int)acc[5];
+    C.T.tot_adjv = (int)acc[6];
+    C.T.tot_seph = (int)acc[7];
+    C.T.tot_sepv = (int)acc[8];
+    C.T.lds_consts = 0;
+    C.T.lds_w = 0;
+    for (int i = 0; i < n; ++i) {
+        C.T.lds_consts = std::max(C.T.lds_consts, L[i].M.pstride * 4);
+        C.T.lds_w = std::max(C.T.lds_w, L[i].P.kf * L[i].P.kf * 4);
+    }
+    if (C.T.lds_consts > 64 * 1024) return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: constant block too large");
+    return GL_OK;
+}
+
+extern "C" int gl_batch_create_ws(const gl_plan* const* plans, int n, const int* counts, size_t workspace_bytes,
+                                  gl_batch** out) {
+    if (out) *out = nullptr;
+    if (!plans || !counts || !out || n <= 0) return fail(GL_ERR_ARG, "gl_batch_create: bad arguments");
+    for (int i = 0; i < n; ++i) {
+        const gl_plan* q = plans[i];
+        if (!q) return fail(GL_ERR_ARG, "gl_batch_create: null plan");
+        if (q->mc && (q->M.lstsq || q->M.famp))
+            return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: least-squares and shapelet lights are not batched");
+        if (counts[i] <= 0) return fail(GL_ERR_ARG, "gl_batch_create: particle counts must be positive");
+        if (!q->mc && counts[i] > gl::kOrderCap)
+            return fail(GL_ERR_ARG, "gl_batch_create: a resident-image lens takes at most 16384 particles");
+        if (q->P.d != plans[0]->P.d || q->P.lik != plans[0]->P.lik)
+            return fail(GL_ERR_ARG, "gl_batch_create: plans must share the parameter count and likelihood");
+    }
+    const size_t budget = workspace_bytes ? workspace_bytes : (size_t)32 << 30;
+    gl_batch* b = new gl_batch();
+    b->n = n;
+    b->d = plans[0]->P.d;
+    b->counts.assign(counts, counts + n);
+    b->arena_bytes = 0;
+    int rc = GL_OK;
+    // lens-major particle offsets; fused lenses run on their own plans
+    std::vector<int> gfirst(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        if ((long)gfirst[i] + counts[i] > INT32_MAX) { gl_batch_destroy(b); return fail(GL_ERR_ARG, "gl_batch_create: too many particles"); }
+        gfirst[i + 1] = gfirst[i] + counts[i];
+    }
+    b->total = gfirst[n];
+    // staged lenses: constants (per lens, kept) and scratch sizes; chunks in lens order
+    std::vector<gl::McLens> lens;
+    std::vector<int> firsts, kf4s, mcm;
+    std::vector<size_t> bytes;
+    for (int i = 0; i < n && !rc; ++i) {
+        if (!plans[i]->mc) {
+            b->fused.push_back(FusedLens{plans[i], gfirst[i], counts[i]});
+            continue;
+        }
+        gl::McLens E;
+        memset(&E, 0, sizeof(E));
+        E.P = plans[i]->P;
+        E.M = plans[i]->M;
+        E.M.conv_ib = gl::kMcIB;  // batches launch the direct conv at the default rows per lane
+        E.count = counts[i];
+        E.weight = 1.0f / (float)counts[i];
+        if ((rc = mc_consts_upload(E.P, E.M, plans[i]->kf4, plans[i]->W_host, b->dev))) break;
+        ScratchAlloc m;
+        m.measure = true;
+        if ((rc = mc_scratch(E.P, E.M, counts[i], 1, m))) break;
+        lens.push_back(E);
+        firsts.push_back(gfirst[i]);
+        kf4s.push_back(plans[i]->kf4);
+        mcm.push_back(plans[i]->mc_model);
+        bytes.push_back(m.off);
     }
     if (rc) { gl_batch_destroy(b); return rc; }
-    b->T.n = n;
-    b->T.tot_part = (int)acc[0];
-    b->T.tot_basis = (int)acc[1];
-    b->T.tot_conv = (int)acc[2];
-    b->T.tot_adj = (int)acc[3];
-    b->T.tot_grad = (int)acc[4];
-    b->T.tot_adjh = (int)acc[5];
-    b->T.tot_adjv = (int)acc[6];
-    b->T.tot_seph = (int)acc[7];
-    b->T.tot_sepv = (int)acc[8];
-    b->T.lds_consts = 0;
-    b->T.lds_w = 0;
-    for (int i = 0; i < n; ++i) {
-        b->T.lds_consts = std::max(b->T.lds_consts, lens[i].M.pstride * 4);
-        b->T.lds_w = std::max(b->T.lds_w, lens[i].P.kf * lens[i].P.kf * 4);
+    std::vector<std::vector<int>> groups;
+    size_t cur = 0;
+    for (size_t i = 0; i < lens.size(); ++i) {
+        if (groups.empty() || cur + bytes[i] > budget) {
+            groups.emplace_back();
+            cur = 0;
+        }
+        groups.back().push_back((int)i);
+        cur += bytes[i];
+        b->arena_bytes = std::max(b->arena_bytes, cur);
     }
-    if (b->T.lds_consts > 64 * 1024) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: constant block too large"); }
+    char* arena = nullptr;
+    if (b->arena_bytes) {
+        void* dptr = nullptr;
+        if ((rc = dev_alloc(b->dev, b->arena_bytes, &dptr, false))) { gl_batch_destroy(b); return rc; }
+        arena = (char*)dptr;
+    }
+    for (const std::vector<int>& g : groups) {
+        ScratchAlloc sa;
+        sa.arena = arena;
+        sa.cap = b->arena_bytes;
+        std::vector<gl::McLens> cl;
+        std::vector<int> cf, ck;
+        for (int i : g) {
+            gl::McLens E = lens[i];
+            if ((rc = mc_scratch(E.P, E.M, E.count, 1, sa))) break;
+            cl.push_back(E);
+            cf.push_back(firsts[i]);
+            ck.push_back(kf4s[i]);
+        }
+        if (rc) break;
+        BatchChunk C;
+        if ((rc = batch_chunk(b, cl, cf, ck, C))) break;
+        C.mc_model = 1;
+        for (int i : g) C.mc_model = C.mc_model && mcm[i] == 1;
+        b->chunks.push_back(C);
+    }
+    if (rc) { gl_batch_destroy(b); return rc; }
     if (hipDeviceSynchronize() != hipSuccess) { gl_batch_destroy(b); return fail(GL_ERR_HIP, "gl_batch_create: sync"); }
+    if (b->guard.init() != hipSuccess) { gl_batch_destroy(b); return fail(GL_ERR_HIP, "gl_batch_create: event"); }
     *out = b;
     return GL_OK;
 }
 
+extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* counts, gl_batch** out) {
+    return gl_batch_create_ws(plans, n, counts, 0, out);
+}
+
 extern "C" int gl_batch_size(const gl_batch* b) { return b ? b->total : -1; }
+
+extern "C" size_t gl_batch_workspace_bytes(const gl_batch* b) { return b ? b->arena_bytes : 0; }
+
+extern "C" int gl_batch_num_chunks(const gl_batch* b) { return b ? (int)b->chunks.size() : -1; }
 
 extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, float* chi2, float* grad_z,
                                  hipStream_t stream) {
     if (!b || !z) return fail(GL_ERR_ARG, "gl_batch_log_prob: null argument");
+    GL_HIP(b->guard.enter(stream));
     gl::McArgs A;
     memset(&A, 0, sizeof(A));
     A.in = z;
@@ -1322,9 +1438,22 @@ extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, f
     A.lp = lp;
     A.chi2 = chi2;
     A.grad = grad_z;
-    GL_HIP(b->guard.enter(stream));
-    return mc_pipeline_run(true, b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false,
-                           b->mc_model, &b->groups);
+    for (const BatchChunk& C : b->chunks) {
+        int rc = mc_pipeline_run(true, C.T, A, C.ss_launch, C.kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false,
+                                 C.mc_model, &C.groups);
+        if (rc) return rc;
+    }
+    const int d = b->d;
+    for (const FusedLens& F : b->fused) {
+        gl::KArgs K = blank(gl::M_LOGPROB, F.count);
+        K.in = z + (size_t)F.first * d;
+        K.lp = lp ? lp + F.first : nullptr;
+        K.chi2 = chi2 ? chi2 + F.first : nullptr;
+        K.grad = grad_z ? grad_z + (size_t)F.first * d : nullptr;
+        int rc = run(F.plan, K, grad_z != nullptr, stream);
+        if (rc) return rc;
+    }
+    return GL_OK;
 }
 
 extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, float* v, int step, float lr,
@@ -1332,6 +1461,7 @@ extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, flo
                                       hipStream_t stream) {
     if (!b || !z || !m || !v) return fail(GL_ERR_ARG, "gl_batch_map_adam_step: null argument");
     if (step < 1) return fail(GL_ERR_ARG, "step is 1-based");
+    GL_HIP(b->guard.enter(stream));
     gl::McArgs A;
     memset(&A, 0, sizeof(A));
     A.in = z;
@@ -1348,7 +1478,18 @@ extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, flo
     A.beta1 = beta1;
     A.beta2 = beta2;
     A.eps = epsilon;
-    GL_HIP(b->guard.enter(stream));
-    return mc_pipeline_run(true, b->T, A, b->ss_launch, b->kf4, 0, 0, 1, 0, true, stream, false, b->mc_model,
-                           &b->groups);
+    for (const BatchChunk& C : b->chunks) {
+        int rc = mc_pipeline_run(true, C.T, A, C.ss_launch, C.kf4, 0, 0, 1, 0, true, stream, false, C.mc_model,
+                                 &C.groups);
+        if (rc) return rc;
+    }
+    const int d = b->d;
+    for (const FusedLens& F : b->fused) {
+        // a lens's loss is the mean over its particles of -lp / N^2 (tf/inference.py:28-35)
+        int rc = map_step(F.plan, z + (size_t)F.first * d, m + (size_t)F.first * d, v + (size_t)F.first * d, nullptr,
+                          nullptr, F.count, step, lr, beta1, beta2, epsilon, lp ? lp + F.first : nullptr,
+                          chi2 ? chi2 + F.first : nullptr, stream, nullptr);
+        if (rc) return rc;
+    }
+    return GL_OK;
 }

@@ -136,7 +136,7 @@ struct McLens {
     McPlan M;
     int first, count;
     float weight;
-    int pad_;
+    int goff;  // a launch's particle gp of this lens is particle gp + goff of the caller's arrays
 };
 
 // Lens table of a launch.  Single-lens launches use `one` (a kernel argument);
@@ -232,7 +232,7 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
     float* s_x = pc;  // [x | dxdz | dlpdx | dfldjdz | gx | lpj]
     float* s_c = pc + ((6 * d + 3) & ~3);
     if (tid < d) {
-        const float zin = A.in[(size_t)p * d + tid];
+        const float zin = A.in[(size_t)(p + E.goff) * d + tid];
         glm::PriorOut o;
         if (A.use_z) {
             glm::prior_eval(P.prior_kind[tid], P.prior_args + 8 * tid, zin, P.fldj_at_x, o);
@@ -427,6 +427,19 @@ __device__ __forceinline__ float mc_famp_sum(const DPlan& P, const McPlan& M, co
     return v;
 }
 
+// The column margins of a staged row (pb zeros on the left, BW - pb - nsub on the
+// right) that the channel convolution reads: written by the thread of column x of the
+// row, for every channel, so a workspace shared by lenses of different shapes needs
+// no clearing.
+__device__ __forceinline__ void mc_zero_margins(const McPlan& M, const DPlan& P, float* row, int x, int nch) {
+    const int nsub = P.nsub, rm = M.BW - P.pb - nsub;
+    const size_t cstride = (size_t)nsub * M.BW;
+    for (int c = 0; c < nch; ++c) {
+        for (int m = x; m < P.pb; m += nsub) row[cstride * c + m] = 0.0f;
+        for (int m = x; m < rm; m += nsub) row[cstride * c + P.pb + nsub + m] = 0.0f;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // mc_basis: grid (bs, ceil(nsub*nsub/256)), block 256
 // ---------------------------------------------------------------------------
@@ -451,6 +464,7 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
     mc_beta<false>(P, M, pc, X, Y, bx, by, nullptr);
     float* Bp = M.B + (size_t)p * M.D * nsub * M.BW + (size_t)y * M.BW + P.pb + x;
     const size_t cstride = (size_t)nsub * M.BW;
+    mc_zero_margins(M, P, Bp - P.pb - x, x, M.D);
     if (!M.lstsq && !M.famp) {
         // fixed amplitudes: the sum of every light, NaN -> 0 (tf/simulator.py:74-81)
         float v = 0.0f;
@@ -951,9 +965,9 @@ __global__ __launch_bounds__(1024) void mc_solve(const McTable T, float rcond, i
 // ---------------------------------------------------------------------------
 template <bool MULTI>
 __global__ __launch_bounds__(256) void mc_image(const McTable T, const McArgs A, int want_grad) {
-    const int gp = blockIdx.x;  // particle of the launch
-    int p;                      // particle inside its lens
-    const McLens& E = mc_locate<MULTI>(T, T.pre_part, gp, p);
+    int p;  // particle inside its lens
+    const McLens& E = mc_locate<MULTI>(T, T.pre_part, blockIdx.x, p);
+    const int gp = blockIdx.x + E.goff;  // particle of the caller's arrays
     const DPlan& P = E.P;
     const McPlan& M = E.M;
     const int N = P.N, NP = N * N, D = M.D;
@@ -987,6 +1001,13 @@ __global__ __launch_bounds__(256) void mc_image(const McTable T, const McArgs A,
         if (want_grad) {
             const int I = pix / N, J = pix - (pix / N) * N;
             M.gimg[(size_t)p * GW * GW + (I + M.gp) * GW + (J + M.gp)] = glp * dll + gch * inv_np * dr2;
+        }
+    }
+    if (want_grad) {  // the zero margins of the padded dL/dimage (a shared workspace is not cleared)
+        float* Gp = M.gimg + (size_t)p * GW * GW;
+        for (int i = threadIdx.x; i < GW * GW; i += blockDim.x) {
+            const int r = i / GW, c = i - (i / GW) * GW;
+            if (r < M.gp || r >= M.gp + N || c < M.gp || c >= M.gp + N) Gp[i] = 0.0f;
         }
     }
     for (int o = 32; o >= 1; o >>= 1) {
@@ -1368,7 +1389,9 @@ __global__ __launch_bounds__(256) void mc_basis_m(const McTable T) {
     const float X = fmaf(P.M00, (float)x, fmaf(P.M01, (float)y, P.ox));
     const float Y = fmaf(P.M10, (float)x, fmaf(P.M11, (float)y, P.oy));
     const float v = eval_forward<MODEL>(C, X, Y);
-    M.B[(size_t)p * nsub * M.BW + (size_t)y * M.BW + P.pb + x] = isnan(v) ? 0.0f : v;  // tf/simulator.py:81
+    float* row = M.B + (size_t)p * nsub * M.BW + (size_t)y * M.BW;
+    row[P.pb + x] = isnan(v) ? 0.0f : v;  // tf/simulator.py:81
+    mc_zero_margins(M, P, row, x, 1);
 }
 
 template <bool MULTI, class MODEL>
@@ -1445,9 +1468,9 @@ __global__ __launch_bounds__(256) void mc_ampgrad(const McTable T) {
 // ---------------------------------------------------------------------------
 template <bool MULTI>
 __global__ void mc_finish(const McTable T, const McArgs A) {
-    const int gp = blockIdx.x;
     int p;
-    const McLens& E = mc_locate<MULTI>(T, T.pre_part, gp, p);
+    const McLens& E = mc_locate<MULTI>(T, T.pre_part, blockIdx.x, p);
+    const int gp = blockIdx.x + E.goff;  // particle of the caller's arrays
     const DPlan& P = E.P;
     const McPlan& M = E.M;
     float* px = pconst(M, p);

@@ -226,16 +226,23 @@ int gl_map_adam_step_noalloc(const gl_plan* plan, float* z, float* m, float* v, 
 /* ---------------------------------------------------------------------------
  * Batches of lens systems with different image sizes, supersampling, PSFs and
  * particle counts.  No counterpart in the reference (one simulator per lens,
- * tf/inference.py:21).  Each lens is a fixed-amplitude staged plan
- * (gl_plan_desc.staged = 1, same parameter count and likelihood); the plans
- * must outlive the batch.  Particles are lens-major: lens 0's counts[0]
- * particles first.  Every stage of the staged pipeline is one launch over all
- * lenses; the batch owns the workspace for its particle counts.
+ * tf/inference.py:21).  Each lens is a fixed-amplitude plan with Sersic lights (same
+ * parameter count and likelihood); the plans must outlive the batch.  Particles are
+ * lens-major: lens 0's counts[0] particles first.  Lenses whose plan runs on the
+ * resident-image kernel (gl_plan_info.lds_bytes > 0, at most 16384 particles) take
+ * one fused launch each; the others run the staged pipeline, grouped into chunks
+ * whose workspaces share one arena of at most workspace_bytes (0 selects 32 GiB; a
+ * lens larger than that forms a chunk alone), every stage one launch over a chunk.
+ * gl_batch_create is gl_batch_create_ws with workspace_bytes 0.
  * ------------------------------------------------------------------------- */
 typedef struct gl_batch gl_batch;
 int gl_batch_create(const gl_plan* const* plans, int n_lenses, const int* counts, gl_batch** out);
+int gl_batch_create_ws(const gl_plan* const* plans, int n_lenses, const int* counts, size_t workspace_bytes,
+                       gl_batch** out);
 int gl_batch_destroy(gl_batch* batch);
 int gl_batch_size(const gl_batch* batch);  /* total particles */
+size_t gl_batch_workspace_bytes(const gl_batch* batch);  /* bytes of the staged lenses' arena */
+int gl_batch_num_chunks(const gl_batch* batch);          /* staged chunks (launch sequences) per call */
 /* lp, chi2 (total) and, when grad_z is non-NULL, dlp/dz (total x d) */
 int gl_batch_log_prob(const gl_batch* batch, const float* z, float* lp, float* chi2, float* grad_z,
                       hipStream_t stream);
@@ -245,6 +252,30 @@ int gl_batch_log_prob(const gl_batch* batch, const float* z, float* lp, float* c
 int gl_batch_map_adam_step(const gl_batch* batch, float* z, float* m, float* v, int step, float lr,
                            float beta1, float beta2, float epsilon, float* lp, float* chi2,
                            hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Diagonal-Gaussian SVI of many lens systems on the device (the surrogate of
+ * tf/inference.py:43-73 restricted to a diagonal scale, as the uneven-batch
+ * workload asks).  Lens L's surrogate is N(loc[L], diag(sigma^2)), sigma =
+ * exp(raw[L]) + 1e-6 (FillScaleTriL's Exp bijector and diag_shift, :56-63), with
+ * n_vi samples per lens laid out lens-major (particle L n_vi + s), i.e. a batch
+ * whose counts are all n_vi.  One SVI step is gl_svi_diag_sample, the batch's
+ * log_prob with gradient on z, then gl_svi_diag_update; no host work, so the
+ * three calls can be recorded into a HIP graph and replayed.  step: per-lens
+ * device counters (n_lenses ints, start at 0) that the update advances.
+ * ------------------------------------------------------------------------- */
+/* z, eps (n_lenses n_vi x d): eps ~ N(0, 1) from Philox4x32-10 keyed by seed with
+ * counter (particle, dimension / 4, step[L], 0) and Box-Muller; z = loc + sigma eps. */
+int gl_svi_diag_sample(const float* loc, const float* raw, int n_lenses, int n_vi, int d,
+                       unsigned long long seed, const int* step, float* z, float* eps, hipStream_t stream);
+/* Per lens, from lp (n_lenses n_vi) and grad_z = dlp/dz: the reparameterised
+ * gradient of the loss mean_s [log q(z_s) - lp_s] (tfp.vi.fit_surrogate_posterior,
+ * tf/inference.py:65-71) w.r.t. loc and raw, then Keras Adam (m, v: 2 n_lenses d,
+ * [loc | raw]) with lr_t from step[L] + 1; step[L] += 1.  loss_hist (hist_len x
+ * n_lenses, may be NULL) receives the loss at row step[L] (before the increment). */
+int gl_svi_diag_update(const float* lp, const float* grad_z, const float* eps, int n_lenses, int n_vi, int d,
+                       float* loc, float* raw, float* m, float* v, int* step, float lr, float beta1,
+                       float beta2, float epsilon, float* loss_hist, int hist_len, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Learned initialiser.  No counterpart in the reference: a small BF16 CNN

@@ -131,8 +131,9 @@ def test_uneven_batch_matches_per_lens_plans(cuda):
             clean = sp.simulate(truth.contiguous()).cpu().numpy()[0]
         obs = wl.noisy_observation(clean, 0.2, 100.0, seed=40 + k)
         lenses.append((cfg, obs, n))
+        # the same execution path as inside the batch: fused where the image fits LDS
         plans.append(Plan(phys, Camera(cfg), layout_from_paths(phys, paths), prior=prior, observed=obs,
-                          background_rms=0.2, exp_time=100.0, staged=True))
+                          background_rms=0.2, exp_time=100.0))
     batch = LensBatch(phys, prior, lenses, 0.2, 100.0, device=cuda)
     assert batch.total == sum(n for *_, n in specs)
     z = batch.bij.inverse(prior.sample(batch.total, seed=3)).to(cuda).float().contiguous()
