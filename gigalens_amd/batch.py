@@ -171,11 +171,12 @@ class LensBatch:
         return lp, chi2, grad
 
     def run(self, z0: torch.Tensor, num_steps: int = 350, optimizer: Optional[Adam] = None,
-            record_every: int = 1, graph: bool = False, graph_steps: int = 0) -> torch.Tensor:
+            record_every: int = 1, graph: bool = False, graph_steps: int = 0, progress=None) -> torch.Tensor:
         """Keras-Adam MAP of every lens at once from z0 (total, d); histories as in
         MultiLensMAP.  ``graph=True`` records the steps into HIP graphs first (of
         ``graph_steps`` steps each, default all) and replays them; ``step_seconds``
-        then holds the measured time of each graph (per step)."""
+        then holds the measured time of each graph (per step).  ``progress(done,
+        total)`` is called after each graph."""
         optimizer = optimizer if optimizer is not None else Adam(1e-2)
         z = z0.to(self.device, torch.float32).contiguous().clone()
         if z.shape != (self.total, self.d):
@@ -208,6 +209,8 @@ class LensBatch:
                 ev1.record()
                 self.step_seconds.append((s0, s1, ev0, ev1))
                 del g
+                if progress is not None:
+                    progress(s1, num_steps)
             torch.cuda.synchronize()
             self.step_seconds = [(s0, s1, a.elapsed_time(b) * 1e-3 / (s1 - s0)) for s0, s1, a, b in self.step_seconds]
         else:

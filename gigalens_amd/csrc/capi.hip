@@ -1372,15 +1372,30 @@ extern "C" int gl_batch_create_ws(const gl_plan* const* plans, int n, const int*
         bytes.push_back(m.off);
     }
     if (rc) { gl_batch_destroy(b); return rc; }
+    // a chunk stays within the byte budget and keeps every stage's launch under 2^31
+    // work-items (a dispatch holds at most 2^32)
+    auto stage_threads = [](const McUnits& u) {
+        return std::max({u.basis * 256, u.grad * 256, u.conv * 64, u.adj * 64, u.adjh * 64, u.adjv * 64,
+                         u.seph * 64, u.sepv * 64, u.part * 256});
+    };
+    const long kMaxThreads = 1L << 31;
     std::vector<std::vector<int>> groups;
     size_t cur = 0;
+    McUnits acc_u{};
     for (size_t i = 0; i < lens.size(); ++i) {
-        if (groups.empty() || cur + bytes[i] > budget) {
+        const McUnits u = mc_units(lens[i].P, lens[i].M, lens[i].count);
+        McUnits sum = acc_u;
+        sum.part += u.part; sum.basis += u.basis; sum.conv += u.conv; sum.adj += u.adj; sum.grad += u.grad;
+        sum.adjh += u.adjh; sum.adjv += u.adjv; sum.seph += u.seph; sum.sepv += u.sepv;
+        if (groups.empty() || cur + bytes[i] > budget || stage_threads(sum) > kMaxThreads) {
             groups.emplace_back();
             cur = 0;
+            sum = u;
         }
+        if (stage_threads(u) > kMaxThreads) { gl_batch_destroy(b); return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: a lens's launch exceeds 2^31 work-items; lower its particle count"); }
         groups.back().push_back((int)i);
         cur += bytes[i];
+        acc_u = sum;
         b->arena_bytes = std::max(b->arena_bytes, cur);
     }
     char* arena = nullptr;

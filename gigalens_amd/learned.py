@@ -397,3 +397,60 @@ def compare_initialisation(init: "LearnedInitializer", n_lenses: int = 256, n_pa
         res["time_to_prior_best_lp_within_1nat"][name]["time_ratio_vs_prior_p50"] = \
             float(np.median(ratio)) if both.any() else None
     return res
+
+
+def central_view(image, size: int, background_rms: float, seed: int) -> torch.Tensor:
+    """The central size x size pixels of a centred image on the same pixel grid:
+    a crop when the image is larger, else the image inside a border of Gaussian
+    background noise (rms ``background_rms``, seeded).  Both grids are centred on the
+    optical axis, so pixel centres coincide when N - size is even."""
+    img = torch.as_tensor(image, dtype=torch.float32)
+    N = img.shape[-1]
+    if (N - size) % 2:
+        raise ValueError("the image and the view must differ by an even number of pixels")
+    if N >= size:
+        o = (N - size) // 2
+        return img[o:o + size, o:o + size].contiguous()
+    g = torch.Generator().manual_seed(int(seed))
+    out = background_rms * torch.randn((size, size), generator=g)
+    o = (size - N) // 2
+    out[o:o + N, o:o + N] = img
+    return out
+
+
+def propose_batch(init: "LearnedInitializer", batch, images: Sequence, prior_fraction: float = 0.25,
+                  oversample: int = 2, seed: int = 0) -> torch.Tensor:
+    """Starting points for every particle of a ``LensBatch`` (lens-major): each
+    lens's network Gaussian (predicted from the central view of its observed image)
+    with ``prior_fraction`` of prior draws as the fallback; ``oversample`` candidate
+    sets are evaluated through the batch's log_prob and each lens keeps its best
+    ``count`` candidates."""
+    dev = batch.device
+    views = torch.stack([central_view(im, init.sim_config.num_pix, init.background_rms, seed + k)
+                         for k, im in enumerate(images)]).to(dev)
+    mean, std = init.predict(views)
+    lens = batch.lens.long()
+    counts = torch.as_tensor(batch.counts, device=dev)
+    first = torch.as_tensor(batch.first, device=dev)
+    pos = torch.arange(batch.total, device=dev) - first[lens]
+    n_prior = torch.round(prior_fraction * counts.float()).long()
+    use_prior = pos >= (counts - n_prior)[lens]
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed) + 4242)
+    cands, lps = [], []
+    for r in range(max(1, int(oversample))):
+        z = mean[lens] + std[lens] * torch.randn((batch.total, init.d), device=dev, generator=g)
+        if r == 0:
+            z[pos == 0] = mean  # the network's point estimate is one of the candidates
+        zp = init.bij.inverse(init.prior.sample(batch.total, seed=seed + 100 + r)).to(dev, torch.float32)
+        z = torch.where(use_prior[:, None], zp, z).contiguous()
+        lp, _, _ = batch.log_prob(z)
+        cands.append(z)
+        lps.append(torch.nan_to_num(lp, nan=-1e30).clamp(-1e11, 1e11))
+    R = len(cands)
+    zc = torch.cat(cands)
+    lpc = torch.cat(lps).double()
+    lens_c = lens.repeat(R)
+    order = torch.argsort(lens_c.double() * 1e12 - lpc)  # by lens, then descending log-posterior
+    pick = order[R * first[lens] + pos]
+    return zc[pick].contiguous()
