@@ -298,12 +298,12 @@ static int mc_create(const gl_plan_desc* desc, gl_plan* p, const std::vector<flo
     M.shp_comp = -1;
     for (int i = P.nlens; i < P.ncomp; ++i) {
         const gl::DComp& c = P.comp[i];
+        const bool sersic = c.kind == glm::K_SERSIC || c.kind == glm::K_SERSIC_ELLIPSE || c.kind == glm::K_CORE_SERSIC;
         if (!lstsq && !famp) {
-            if (c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE)
-                return fail(GL_ERR_UNSUPPORTED, "fixed-amplitude staged plans take Sersic lights");
+            if (!sersic) return fail(GL_ERR_UNSUPPORTED, "fixed-amplitude staged plans take Sersic-family lights");
             continue;
         }
-        if (famp && c.kind != glm::K_SERSIC && c.kind != glm::K_SERSIC_ELLIPSE && c.kind != glm::K_SHAPELETS)
+        if (famp && !sersic && c.kind != glm::K_SHAPELETS)
             return fail(GL_ERR_UNSUPPORTED, "plans with a fixed-amplitude shapelet light take Sersic and shapelet lights");
         if (c.kind == glm::K_SHAPELETS) {
             if (++nshp > 1) return fail(GL_ERR_UNSUPPORTED, "at most one shapelet component");
@@ -761,12 +761,6 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
             gl_plan_destroy(p);
             return fail(GL_ERR_UNSUPPORTED, "staged plans take one observed image (use gl_batch_* for many lenses)");
         }
-        for (int i = 0; i < desc->n_components; ++i)
-            if (desc->components[i].kind == glm::K_CORE_SERSIC) {
-                gl_plan_destroy(p);
-                return fail(GL_ERR_UNSUPPORTED, "the core-Sersic profile needs the resident-image kernel: an image "
-                                                "that fits LDS and a compiled profile combination");
-            }
         p->mc = 1;
         p->model = nullptr;
         p->mc_model = 0;

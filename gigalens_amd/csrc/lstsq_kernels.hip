@@ -51,7 +51,8 @@ namespace gl {
 constexpr int kMcMaxChan = 79;
 constexpr int kMcGramMaxNB = 5;
 constexpr int kMcGramMaxSplit = 8;  // pixel-range splits per particle (x 4 waves = partials)
-constexpr int kMcStat = 64;  // statistics floats per particle
+constexpr int kStatStride = 16;  // statistics floats per component (core-Sersic: 11)
+constexpr int kMcStat = kStatStride * kMaxComp;  // statistics floats per particle
 constexpr int kMcMaxRank = 8;  // separable channel convolution: at most this rank
 constexpr int kMcIB = 4;     // default output rows per conv lane (McPlan.conv_ib: 2, 4 or 8)
 
@@ -219,6 +220,21 @@ __device__ __forceinline__ float mc_glp(const McArgs& A, const McLens& E, int p)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float* pconst(const McPlan& M, int p) { return M.consts + (size_t)p * M.pstride; }
 
+// Position of Ie among a light profile's parameters (the canonical order of the C ABI).
+__host__ __device__ constexpr int mc_ie_index(int kind) {
+    return kind == glm::K_SERSIC ? 4 : kind == glm::K_CORE_SERSIC ? 9 : 6;
+}
+
+// Value of a Sersic-family light (circular, ellipse, core) with constants cst at (x, y).
+__device__ __forceinline__ float mc_sersic_light(int kind, const float* cst, float x, float y) {
+    if (kind == glm::K_CORE_SERSIC) {
+        glm::CoreSerState st;
+        return glm::core_sersic_forward(*reinterpret_cast<const glm::CoreSerC*>(cst), x, y, st);
+    }
+    glm::SerState st;
+    return glm::sersic_forward(*reinterpret_cast<const glm::SerC*>(cst), x, y, st);
+}
+
 template <bool MULTI>
 __global__ void mc_prologue(const McTable T, const McArgs A) {
     const int p = blockIdx.x;  // particle of the launch (inputs, outputs)
@@ -284,6 +300,11 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
             glm::SerC c;
             glm::sersic_prepare(pv[0], pv[1], 0.0f, 0.0f, pv[2], pv[3], (cd.lstsq || M.famp) ? 1.0f : pv[4], 0, c);
             *reinterpret_cast<glm::SerC*>(dst) = c;
+        } else if (cd.kind == glm::K_CORE_SERSIC) {
+            glm::CoreSerC c;
+            glm::core_sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], pv[6], pv[7], pv[8],
+                                     (cd.lstsq || M.famp) ? 1.0f : pv[9], c);
+            *reinterpret_cast<glm::CoreSerC*>(dst) = c;
         } else if (cd.kind == glm::K_SHAPELETS) {
             dst[0] = pv[0];  // beta
             dst[1] = pv[1];  // center_x
@@ -318,7 +339,7 @@ __global__ void mc_prologue(const McTable T, const McArgs A) {
                 for (int k = 0; k < nl; ++k) M.coef[(size_t)q * M.D + ch + k] = P.cf * s_x[cd.amp_col0 + k];
                 ch += nl;
             } else {
-                const int ie = cd.kind == glm::K_SERSIC ? 4 : 6;
+                const int ie = mc_ie_index(cd.kind);
                 M.coef[(size_t)q * M.D + ch++] = P.cf * (cd.col[ie] >= 0 ? s_x[cd.col[ie]] : cd.cst[ie]);
             }
         }
@@ -419,9 +440,7 @@ __device__ __forceinline__ float mc_famp_sum(const DPlan& P, const McPlan& M, co
             }
             ch += (M.shp_nmax + 1) * (M.shp_nmax + 2) / 2;
         } else {
-            const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
-            glm::SerState st;
-            v = fmaf(cp[ch++], glm::sersic_forward(C, ux, uy, st), v);
+            v = fmaf(cp[ch++], mc_sersic_light(cd.kind, cst, ux, uy), v);
         }
     }
     return v;
@@ -470,10 +489,8 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
         float v = 0.0f;
         for (int i = P.nlens; i < P.ncomp; ++i) {
             const DComp& cd = P.comp[i];
-            const glm::SerC C = *reinterpret_cast<const glm::SerC*>(s_c + i * kConstStride);
             const bool src = cd.role == GL_ROLE_SOURCE;
-            glm::SerState st;
-            v += glm::sersic_forward(C, src ? bx : X, src ? by : Y, st);
+            v += mc_sersic_light(cd.kind, s_c + i * kConstStride, src ? bx : X, src ? by : Y);
         }
         Bp[0] = isnan(v) ? 0.0f : v;
         return;
@@ -489,10 +506,8 @@ __global__ __launch_bounds__(256) void mc_basis(const McTable T) {
         const float* cst = s_c + i * kConstStride;
         const bool src = cd.role == GL_ROLE_SOURCE;
         const float ux = src ? bx : X, uy = src ? by : Y;
-        if (cd.kind == glm::K_SERSIC || cd.kind == glm::K_SERSIC_ELLIPSE) {
-            const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
-            glm::SerState st;
-            float v = glm::sersic_forward(C, ux, uy, st);
+        if (cd.kind == glm::K_SERSIC || cd.kind == glm::K_SERSIC_ELLIPSE || cd.kind == glm::K_CORE_SERSIC) {
+            float v = mc_sersic_light(cd.kind, cst, ux, uy);
             if (M.famp) wsum = fmaf(cp[ch], v, wsum);
             Bp[cstride * ch++] = (isnan(v) && !M.famp) ? 0.0f : v;
         } else if (cd.kind == glm::K_SHAPELETS) {
@@ -1202,7 +1217,7 @@ __global__ __launch_bounds__(256) void mc_adj_h(const McTable T) {
 // components (statistics kept in registers: 8 per component).
 template <bool MULTI, bool SHP, int LM, int NC>
 __device__ __forceinline__ void mc_grad_body(const McTable& T) {
-    static_assert(8 * NC <= kMcStat, "statistics per particle");
+    static_assert(kStatStride * NC <= kMcStat, "statistics per particle");
     int local;
     const McLens& E = mc_locate<MULTI>(T, T.pre_grad, blockIdx.x, local);
     const DPlan& P = E.P;
@@ -1211,9 +1226,9 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
     const int p = local / gpp;
     const int blk = local - p * gpp;
     const int nsub = P.nsub;
-    float st[8 * NC];
+    float st[kStatStride * NC];
 #pragma unroll
-    for (int e = 0; e < 8 * NC; ++e) st[e] = 0.0f;
+    for (int e = 0; e < kStatStride * NC; ++e) st[e] = 0.0f;
     extern __shared__ float4 mc_dyn4[];
     const float* pc = mc_stage_consts(M, p, reinterpret_cast<float*>(mc_dyn4));
     const float* s_c = pc + ((6 * P.d + 3) & ~3);
@@ -1232,10 +1247,8 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
             // the sum of the lights: a NaN zeroed the whole sub-pixel in the forward pass
             float v = 0.0f;
             for (int i = P.nlens; i < P.ncomp; ++i) {
-                const glm::SerC C = *reinterpret_cast<const glm::SerC*>(s_c + i * kConstStride);
                 const bool src = P.comp[i].role == GL_ROLE_SOURCE;
-                glm::SerState ss0;
-                v += glm::sersic_forward(C, src ? bx : X, src ? by : Y, ss0);
+                v += mc_sersic_light(P.comp[i].kind, s_c + i * kConstStride, src ? bx : X, src ? by : Y);
             }
             if (isnan(v)) continue;
         }
@@ -1250,7 +1263,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
             const float* cst = s_c + i * kConstStride;
             const bool src = cd.role == GL_ROLE_SOURCE;
             const float ux = src ? bx : X, uy = src ? by : Y;
-            float* acc = st + 8 * i;
+            float* acc = st + kStatStride * i;
             if (cd.kind == glm::K_SERSIC || cd.kind == glm::K_SERSIC_ELLIPSE) {
                 const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
                 glm::SerState ss_;
@@ -1259,6 +1272,15 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
                 if (isnan(v)) continue;  // least squares: this channel was zeroed
                 float gdx, gdy;
                 glm::sersic_adjoint(C, ss_, g * cp[c], acc, gdx, gdy);
+                if (src) { gbx += gdx; gby += gdy; }
+            } else if (cd.kind == glm::K_CORE_SERSIC) {
+                const glm::CoreSerC C = *reinterpret_cast<const glm::CoreSerC*>(cst);
+                glm::CoreSerState cs;
+                const float v = glm::core_sersic_forward(C, ux, uy, cs);
+                const int c = (M.lstsq || M.famp) ? ch++ : 0;
+                if (isnan(v)) continue;
+                float gdx, gdy;
+                glm::core_sersic_adjoint(C, cs, g * cp[c], acc, gdx, gdy);
                 if (src) { gbx += gdx; gby += gdy; }
             } else if (SHP && cd.kind == glm::K_SHAPELETS) {
                 const float beta = cst[0];
@@ -1294,7 +1316,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
             if (i >= P.nlens) break;
             const int kind = P.comp[i].kind;
             const float* cst = s_c + i * kConstStride;
-            float* acc = st + 8 * i;
+            float* acc = st + kStatStride * i;
             if (lens_in(LM, i, glm::K_EPL) && kind == glm::K_EPL) {
                 const glm::EplC C = *reinterpret_cast<const glm::EplC*>(cst);
                 glm::epl_adjoint(C, ls.es[i], Hx, Hy, acc);
@@ -1314,17 +1336,17 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
     }
     __shared__ float red[kMcStat][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nst = 8 * P.ncomp;
+    const int nst = kStatStride * P.ncomp;
 #pragma unroll
-    for (int e = 0; e < 8 * NC; ++e) {
-        if (e < nst) {
+    for (int e = 0; e < kStatStride * NC; ++e) {
+        if (e < nst && (e % kStatStride) < 11) {
             float v = st[e];
             for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
             if (lane == 0) red[e][wv] = v;
         }
     }
     __syncthreads();
-    if (threadIdx.x < nst) {
+    if (threadIdx.x < nst && (threadIdx.x % kStatStride) < 11) {
         const float v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
         atomicAdd(M.stats + (size_t)p * kMcStat + threadIdx.x, v);
     }
@@ -1359,13 +1381,13 @@ __device__ __forceinline__ int mc_stat_slot(int e) {
                     : k == glm::K_SIS ? LensAcc<glm::K_SIS>::n
                     : k == glm::K_SIE ? LensAcc<glm::K_SIE>::n
                     : k == glm::K_TNFW ? LensAcc<glm::K_TNFW>::n : 0;
-        if (e >= M::lens_off(i) && e < M::lens_off(i) + n) return 8 * i + (e - M::lens_off(i));
+        if (e >= M::lens_off(i) && e < M::lens_off(i) + n) return kStatStride * i + (e - M::lens_off(i));
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (M::light[j] == 0) continue;
         const int off = M::light_off(light_acc_slot<M>(j));
-        if (e >= off && e < off + 8) return 8 * light_comp_index<M>(j) + (e - off);
+        if (e >= off && e < off + 8) return kStatStride * light_comp_index<M>(j) + (e - off);
     }
     return 0;
 }
@@ -1485,7 +1507,7 @@ __global__ void mc_finish(const McTable T, const McArgs A) {
         for (int i = 0; i < P.ncomp; ++i) {
             const DComp& cd = P.comp[i];
             const float* cst = s_c + i * kConstStride;
-            const float* acc = st + 8 * i;
+            const float* acc = st + kStatStride * i;
             float g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             int np_ = 0;
             bool circ = false;  // circular Sersic: canonical (Rs, n, cx, cy) -> finish order (Rs, n, e1, e2, cx, cy)
@@ -1509,10 +1531,18 @@ __global__ void mc_finish(const McTable T, const McArgs A) {
                 np_ = cd.kind == glm::K_SERSIC_ELLIPSE ? 6 : 4;  // no Ie in least-squares mode
                 if (!cd.lstsq && !M.famp) np_ += 1;             // fixed amplitudes: Ie too
                 if (M.famp) {                                   // per-channel amplitudes: Ie from mc_ampgrad
-                    const int ie = cd.kind == glm::K_SERSIC ? 4 : 6;
+                    const int ie = mc_ie_index(cd.kind);
                     if (cd.col[ie] >= 0) gx[cd.col[ie]] += M.gamp[(size_t)p * M.D + ch];
                 }
                 circ = cd.kind == glm::K_SERSIC;
+                ++ch;
+            } else if (cd.kind == glm::K_CORE_SERSIC) {
+                float gc[GL_MAX_COMPONENT_PARAMS];
+                glm::core_sersic_finish(*reinterpret_cast<const glm::CoreSerC*>(cst), acc, gc);
+                const int npc = (!cd.lstsq && !M.famp) ? 10 : 9;  // Ie too with fixed amplitudes
+                for (int q = 0; q < npc; ++q)
+                    if (cd.col[q] >= 0) gx[cd.col[q]] += gc[q];
+                if (M.famp && cd.col[9] >= 0) gx[cd.col[9]] += M.gamp[(size_t)p * M.D + ch];
                 ++ch;
             } else if (cd.kind == glm::K_SHAPELETS) {
                 const float beta = cst[0];

@@ -22,7 +22,7 @@ class SersicEllipse(Sersic):
 class CoreSersic(Sersic):
     """Core-Sersic as the reference writes it (sersic.py:75-121):
     I = Ie (1 + (Rb/R)^alpha)^(gamma/alpha) exp(-b_n (R^alpha + Rb^alpha) / (R_s^alpha alpha n) - 1)
-    on the elliptical distance R.  Evaluated by the resident-image kernel only."""
+    on the elliptical distance R."""
 
     _name = "CORE_SERSIC"
     _params = ["R_sersic", "n_sersic", "Rb", "alpha", "gamma", "e1", "e2", "center_x", "center_y"]

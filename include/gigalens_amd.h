@@ -57,7 +57,7 @@ enum gl_profile_kind {
                               (use_lstsq) or sampled at columns amp_col0.. (shapelets.py:17,53-75);
                               HBM-staged pipeline only */
     GL_CORE_SERSIC = 13    /* params: R_sersic, n_sersic, Rb, alpha, gamma, e1, e2, center_x, center_y, Ie
-                              (sersic.py:75-88); resident-image (fused) kernel only */
+                              (sersic.py:75-88) */
 };
 
 /* Roles: which list of PhysicalModel the component belongs to (model.py:21-29). */

@@ -4,8 +4,8 @@ circular Sersic light profile (src/gigalens/tf/profiles/light/sersic.py:21-31) a
 of a non-diagonal pixel-to-angle transform (src/gigalens/tf/simulator.py:19-29,
 src/gigalens/simulator.py:91-125), through the LDS-resident fused kernel and
 through the HBM-staged pipeline, and of the core-Sersic light profile
-(src/gigalens/tf/profiles/light/sersic.py:75-121) through the fused kernel,
-against the FP64 oracle.
+(src/gigalens/tf/profiles/light/sersic.py:75-121) through both, against the FP64
+oracle.
 
 Bars: images within 1e-4 of the image peak; log_prob within 1e-4 relative;
 gradients against FP64 Richardson-extrapolated central differences of the
@@ -143,7 +143,7 @@ def _plan(prob, staged):
                 observed=prob["obs"], background_rms=0.2, exp_time=100.0, staged=staged)
 
 
-CASES = [("sie", False), ("sie", True), ("tnfw", False), ("tnfw", True), ("core", False), ("sis", False),
+CASES = [("sie", False), ("sie", True), ("tnfw", False), ("tnfw", True), ("core", False), ("core", True), ("sis", False),
          ("sis", True), ("round", False), ("round", True), ("rotated", False), ("rotated", True)]
 
 
@@ -209,11 +209,3 @@ def test_profile_gradient_vs_fd(case):
         err = np.abs(g[i] - gfd)
         _print(f"{name} particle {i}: max |g - fd| / max|fd| = {err.max() / scale:.2e}")
         assert np.all(err <= 2e-3 * np.abs(gfd) + 2e-4 * scale), (g[i], gfd)
-
-
-def test_core_sersic_staged_is_refused(cuda):
-    """The core-Sersic profile runs on the resident-image kernel only; a staged plan
-    is refused with an error instead of running without it."""
-    prob = _problem("core", bs=2)
-    with pytest.raises(nat.GlError):
-        _plan(prob, True)
