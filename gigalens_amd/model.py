@@ -2,9 +2,9 @@
 ``gigalens.tf.model``).
 
 ``ForwardProbModel.log_prob(simulator, z)`` returns ``(lp, chi2)`` like
-src/gigalens/tf/model.py:53-86 and is differentiable: its backward is the
-hand-written adjoint kernel, which recomputes the ray-trace instead of
-storing per-particle grids.
+src/gigalens/tf/model.py:53-86 through the ``gigalens::log_prob`` operator
+(ops.py) and is differentiable: its backward is the hand-written adjoint
+kernel, which recomputes the ray-trace instead of storing per-particle grids.
 """
 from __future__ import annotations
 
@@ -17,6 +17,7 @@ import torch
 
 from . import _native as nat
 from . import distributions as dist
+from . import ops
 from .plan import Plan, layout_from_paths
 
 
@@ -39,23 +40,6 @@ class ProbabilisticModel(ABC):
     @abstractmethod
     def log_prob(self, simulator, z):
         pass
-
-
-class _LogProbFn(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, plan, z):
-        lp, chi2, _ = plan.log_prob(z)
-        ctx.plan = plan
-        ctx.save_for_backward(z)
-        return lp, chi2
-
-    @staticmethod
-    def backward(ctx, g_lp, g_chi2):
-        (z,) = ctx.saved_tensors
-        g_lp = torch.zeros_like(z[:, 0]) if g_lp is None else g_lp.contiguous()
-        g_chi2 = None if g_chi2 is None else g_chi2.contiguous()
-        _, _, grad = ctx.plan.log_prob(z, g_lp=g_lp, g_chi2=g_chi2, want_grad=True)
-        return None, grad
 
 
 class _ProbModelBase(ProbabilisticModel):
@@ -119,7 +103,7 @@ class _ProbModelBase(ProbabilisticModel):
         """Log posterior of unconstrained ``z`` (bs, d): ``(lp, chi2)``."""
         plan = self.plan_for(simulator)
         z = torch.as_tensor(z, dtype=torch.float32, device=simulator.device).contiguous()
-        return _LogProbFn.apply(plan, z)
+        return ops.log_prob(plan.handle, z)
 
     def log_prob_and_grad(self, simulator, z, g_lp=None):
         """Fused value and gradient in one launch: ``(lp, chi2, g_lp * dlp/dz)``."""

@@ -216,6 +216,13 @@ class Plan:
         nat.check(lib.gl_plan_create(ctypes.byref(desc), ctypes.byref(h)), "gl_plan_create")
         self._h = h
         self._lib = lib
+        from . import ops
+        ops.register(self)
+
+    @property
+    def handle(self) -> int:
+        """The plan's handle: how the ``gigalens::*`` operators (ops.py) refer to it."""
+        return int(self._h.value)
 
     def __del__(self):
         h = getattr(self, "_h", None)

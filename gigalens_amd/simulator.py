@@ -2,7 +2,8 @@
 
 ``LensSimulator(phys_model, sim_config, bs)`` keeps the reference's
 constructor and methods (src/gigalens/tf/simulator.py:10-165); the work is
-done by the HIP kernels behind ``gigalens_amd.plan.Plan``.
+done by the HIP kernels behind ``gigalens_amd.plan.Plan``, reached through the
+``gigalens::simulate`` operator (ops.py), whose backward is the adjoint kernel.
 """
 from __future__ import annotations
 
@@ -13,6 +14,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import ops
 from . import psf as _psf
 from .plan import Plan, components, layout_from_model, model_param_names
 
@@ -70,19 +72,6 @@ def _as_batch(v, bs, device):
     if t.numel() == 1 and bs > 1:
         t = t.expand(bs)
     return t
-
-
-class _SimulateFn(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, plan, x):
-        ctx.plan = plan
-        ctx.save_for_backward(x)
-        return plan.simulate(x)
-
-    @staticmethod
-    def backward(ctx, g):
-        (x,) = ctx.saved_tensors
-        return None, ctx.plan.simulate_vjp(x, g.contiguous())
 
 
 class LensSimulator:
@@ -155,9 +144,9 @@ class LensSimulator:
             phys, lay, cols = self.undeflected()
             if getattr(self, "_plan_nd", None) is None:
                 self._plan_nd = Plan(phys, self.camera, lay)
-            img = _SimulateFn.apply(self._plan_nd, x[:, cols].contiguous())
+            img = ops.simulate(self._plan_nd.handle, x[:, cols].contiguous())
             return img.squeeze()
-        img = _SimulateFn.apply(self.plan, x)
+        img = ops.simulate(self.plan.handle, x.contiguous())
         return img.squeeze()
 
     def lstsq_simulate(self, params, observed_image, err_map, return_stacked=False, return_coeffs=False,
