@@ -48,6 +48,12 @@ EXPORTED_SYMBOLS = (
     "gl_batch_map_adam_step",
     "gl_svi_diag_sample",
     "gl_svi_diag_update",
+    "gl_log_prob_masked",
+    "gl_hmc_begin",
+    "gl_hmc_position",
+    "gl_hmc_momentum",
+    "gl_hmc_end",
+    "gl_hmc_adapt",
     "gl_cnn_create",
     "gl_cnn_destroy",
     "gl_cnn_num_params",
@@ -162,6 +168,15 @@ _SIGS = {
     "gl_svi_diag_update": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
                             ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp, ctypes.c_int, _vp],
                            ctypes.c_int),
+    "gl_log_prob_masked": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    "gl_hmc_begin": ([ctypes.c_int, ctypes.c_int, _vp, ctypes.c_ulonglong, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                      _vp, _vp, _vp], ctypes.c_int),
+    "gl_hmc_position": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+    "gl_hmc_momentum": ([ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+    "gl_hmc_end": ([ctypes.c_int, ctypes.c_int, _vp, ctypes.c_ulonglong, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                    _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "gl_hmc_adapt": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_float, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "gl_cnn_create": ([ctypes.POINTER(CnnDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "gl_cnn_destroy": ([_vp], ctypes.c_int),
     "gl_cnn_num_params": ([_vp], ctypes.c_int),

@@ -1037,7 +1037,7 @@ static int run(const gl_plan* plan, gl::KArgs& A, bool grad, hipStream_t stream,
     // launch order by cost when the launch has several rounds of workgroups (gradient launches)
     A.order = nullptr;
     int* order = plan->order_buf ? (scratch_dev ? scratch_dev + kNoAllocSeries : plan->order_buf) : nullptr;
-    if (order && grad && A.bs > 1024 && A.bs <= gl::kOrderCap) {
+    if (order && grad && !A.limit && A.bs > 1024 && A.bs <= gl::kOrderCap) {
         GL_HIP(launch_order(P, A.in, A.bs, use_z, order, stream));
         A.order = order;
     }
@@ -1080,6 +1080,24 @@ extern "C" int gl_log_prob(const gl_plan* plan, const float* z, int bs, float* l
     A.g_lp = g_lp;
     A.g_chi2 = g_chi2;
     A.grad = grad_z;
+    return run(plan, A, grad_z != nullptr, stream);
+}
+
+extern "C" int gl_log_prob_masked(const gl_plan* plan, const float* z, int bs, float* lp, float* chi2,
+                                  const float* g_lp, const float* g_chi2, float* grad_z, const int* limit, int k,
+                                  hipStream_t stream) {
+    if (!plan) return fail(GL_ERR_ARG, "null plan");
+    if (plan->mc) return fail(GL_ERR_UNSUPPORTED, "gl_log_prob_masked takes resident-image plans (lds_bytes > 0)");
+    if (!limit) return fail(GL_ERR_ARG, "gl_log_prob_masked: null limit");
+    gl::KArgs A = blank(gl::M_LOGPROB, bs);
+    A.in = z;
+    A.lp = lp;
+    A.chi2 = chi2;
+    A.g_lp = g_lp;
+    A.g_chi2 = g_chi2;
+    A.grad = grad_z;
+    A.limit = limit;
+    A.limit_k = k;
     return run(plan, A, grad_z != nullptr, stream);
 }
 

@@ -42,6 +42,9 @@ struct KArgs {
     const int* lens;      // optional per-particle observed-image index (bs)
     const float* weight;  // MAP: optional per-particle loss weight (bs); default 1/bs
     const int* order;     // optional: particle of workgroup b is order[b] (launch order, see order_kernel)
+    const int* limit;     // optional device scalar: the launch does nothing when limit_k >= *limit (a step
+                          // of a graph-recorded loop whose length is decided on the device)
+    int limit_k;
     float lr_t, beta1, beta2, eps, gscale;
     int mode;
     int bs;
@@ -568,6 +571,7 @@ __device__ __forceinline__ void epl_coef_scan(int n, int NN, double f, double t,
 
 template <int BLOCK, class M, bool GRAD>
 __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KArgs A) {
+    if (A.limit && A.limit_k >= *A.limit) return;  // uniform over the launch
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
     const int pid = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;

@@ -3,13 +3,14 @@
 
 All three drivers evaluate the log posterior and its gradient through the
 fused HIP kernels.  MAP runs one fused launch per step (log_prob + adjoint +
-Keras-Adam update, tf/inference.py:28-35).  SVI and HMC call the fused
-value-and-gradient entry point once per log-density evaluation; the few
-small vector operations around it (reparameterisation, leapfrog updates) run
-as torch ops on the same stream.
+Keras-Adam update, tf/inference.py:28-35).  SVI calls the fused
+value-and-gradient entry point once per step with the reparameterisation as
+torch ops on the same stream.  HMC keeps its whole state, adaptation included,
+on the device (``DeviceHMC``, csrc/hmc.hip).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from typing import Callable, Optional, Union
 
@@ -263,125 +264,131 @@ class ModellingSequence(ModellingSequenceInterface):
     # ---- HMC ----------------------------------------------------------------
     def HMC(self, q_z, init_eps=0.3, init_l=3, n_hmc=50, num_burnin_steps=250, num_results=750,
             max_leapfrog_steps=30, seed=3, adapt=True, target_accept=0.75, start=None, graph=False):
-        """Preconditioned HMC (tf/inference.py:75-141): momentum ~ N(0, Sigma^-1)
-        with Sigma the surrogate covariance (kinetic energy p^T Sigma p / 2),
-        dual-averaging step-size adaptation and ChEES trajectory-length
-        adaptation during 80% of the burn-in.  ``adapt=False`` runs fixed
-        ``init_eps`` / ``init_l``.  ``q_z=None`` gives plain HMC (identity mass);
-        ``start`` (n_hmc, d) overrides the chains' initial states (default: draws
-        from q_z).  ``graph=True`` records one leapfrog step (position update,
-        log_prob + gradient launch, momentum update) into a HIP graph on static
-        buffers, with the step size in a device scalar, and replays it L times
-        per trajectory: the same arithmetic in the same order as the eager loop.
-        Returns ``(samples (num_results, n_hmc, d), stats)``."""
+        """Preconditioned HMC (tf/inference.py:75-141) on the device: momentum ~ N(0,
+        Sigma^-1) with Sigma the surrogate covariance (kinetic energy p^T Sigma p / 2),
+        dual-averaging step-size adaptation and ChEES trajectory-length adaptation during
+        80% of the burn-in, all state kept on the GPU (``DeviceHMC``).  ``adapt=False``
+        runs fixed ``init_eps`` / ``init_l``.  ``q_z=None`` gives identity mass; ``start``
+        (n_hmc, d) overrides the chains' initial states (default: draws from q_z).
+        ``graph=True`` records one whole iteration (momentum draw, every leapfrog step,
+        Metropolis correction, adaptation) into a HIP graph and replays it.  Returns
+        ``(samples (num_results, n_hmc, d), stats)``."""
         gen = torch.Generator().manual_seed(int(seed))
         if q_z is None:
             d0 = start.shape[1]
-            chol = torch.eye(d0, dtype=torch.float64, device=self.device)
+            cov64 = torch.eye(d0, dtype=torch.float64, device=self.device)
         else:
-            # Sigma = L L^T with L the surrogate's scale_tril (positive diagonal): L is
-            # already the Cholesky factor; build Sigma in FP64 for the kinetic energy
             chol = q_z.scale_tril().detach().to(self.device, torch.float64)
-        cov = (chol @ chol.T).to(torch.float32)
-        chol_inv = torch.linalg.solve_triangular(chol, torch.eye(chol.shape[0], dtype=torch.float64,
-                                                                  device=self.device), upper=False)
-        chol_inv = chol_inv.to(torch.float32)
+            cov64 = chol @ chol.T
         if start is not None:
             z = torch.as_tensor(start, dtype=torch.float32).to(self.device).contiguous()
             n_hmc = z.shape[0]
         else:
             z = q_z.sample(n_hmc, generator=gen).detach().contiguous()
         sim = self._sim(n_hmc)
-        pm = self.prob_model
-        d = z.shape[1]
+        plan = self.prob_model.plan_for(sim)
+        hmc = DeviceHMC(plan_target(plan, n_hmc), n_hmc, z.shape[1], cov64, init_eps, init_l,
+                        max_leapfrog_steps if adapt else int(init_l), num_burnin_steps, num_results, adapt,
+                        target_accept, seed, self.device)
+        return hmc.run(z, graph=graph)
 
-        def lp_grad(zz):
-            lp, _, g = pm.log_prob_and_grad(sim, zz.contiguous())
-            return lp, g
 
-        lp, g = lp_grad(z)
-        leap = None
+def plan_target(plan, n: int):
+    """Target log-density and gradient of a device plan for ``DeviceHMC``: resident-image
+    plans skip leapfrog steps beyond the trajectory on the device (gl_log_prob_masked);
+    staged plans evaluate every recorded step (unchanged positions give unchanged values)."""
+    from . import _native as nat
+    from .plan import _ptr, _stream
+    lib = nat.lib()
+    masked = plan.info()["lds_bytes"] > 0
+    chi2 = torch.empty(n, device="cuda")
+
+    def lp_grad(z, lp, g, k, limit):
+        if limit is not None and masked:
+            nat.check(lib.gl_log_prob_masked(plan._h, _ptr(z), n, _ptr(lp), _ptr(chi2), None, None, _ptr(g),
+                                             _ptr(limit), int(k), _stream()), "gl_log_prob_masked")
+        else:
+            nat.check(lib.gl_log_prob(plan._h, _ptr(z), n, _ptr(lp), _ptr(chi2), None, None, _ptr(g), _stream()),
+                      "gl_log_prob")
+    return lp_grad
+
+
+class DeviceHMC:
+    """Preconditioned HMC with dual-averaging step size and ChEES trajectory-length
+    adaptation, state on the device (the kernel stack of tf/inference.py:98-131; the
+    arithmetic is in csrc/hmc.hip).  ``lp_grad(z, lp, g, k, limit)`` writes the target
+    log-density (n,) and its gradient (n, d) at z on the current stream; k is the
+    leapfrog step and ``limit`` the device scalar L (the call may skip k >= L; k = -1,
+    limit None for the initial evaluation)."""
+
+    def __init__(self, lp_grad, n: int, d: int, cov, init_eps=0.3, init_l=3, max_leapfrog_steps=30,
+                 num_burnin_steps=250, num_results=750, adapt=True, target_accept=0.75, seed=3, device="cuda"):
+        self.lp_grad = lp_grad
+        self.n, self.d = int(n), int(d)
+        dev = torch.device(device)
+        self.device = dev
+        cov64 = torch.as_tensor(cov, dtype=torch.float64, device=dev)
+        self.cov = cov64.float().contiguous()
+        L = torch.linalg.cholesky(cov64)
+        self.chol_inv = torch.linalg.solve_triangular(L, torch.eye(self.d, dtype=torch.float64, device=dev),
+                                                      upper=False).float().contiguous()
+        self.init_eps, self.init_l = float(init_eps), int(init_l)
+        self.max_l = max(1, int(max_leapfrog_steps))
+        self.burnin, self.n_results = int(num_burnin_steps), int(num_results)
+        self.adapt = bool(adapt)
+        self.n_adapt = int(num_burnin_steps * 0.8)  # tf/inference.py:112
+        self.target = float(target_accept)
+        self.seed = int(seed) * 0x9E3779B97F4A7C15 % (1 << 64)
+
+    def _launch_iteration(self, B):
+        from . import _native as nat
+        from .plan import _ptr, _stream
+        lib = nat.lib()
+        n, d = self.n, self.d
+        nat.check(lib.gl_hmc_begin(n, d, _ptr(self.chol_inv), ctypes.c_ulonglong(self.seed), _ptr(B["si"]),
+                                   _ptr(B["sf"]), _ptr(B["z"]), _ptr(B["lp"]), _ptr(B["g"]), _ptr(B["p"]),
+                                   _ptr(B["z0"]), _ptr(B["lp0"]), _ptr(B["g0"]), _ptr(B["p0"]), _stream()),
+                  "gl_hmc_begin")
+        limit = B["si"][1:2]
+        for k in range(self.max_l):
+            nat.check(lib.gl_hmc_position(n, d, _ptr(self.cov), _ptr(B["si"]), _ptr(B["sf"]), k, _ptr(B["z"]),
+                                          _ptr(B["p"]), _stream()), "gl_hmc_position")
+            self.lp_grad(B["z"], B["lp"], B["g"], k, limit)
+            nat.check(lib.gl_hmc_momentum(n, d, _ptr(B["si"]), _ptr(B["sf"]), k, _ptr(B["p"]), _ptr(B["g"]),
+                                          _stream()), "gl_hmc_momentum")
+        nat.check(lib.gl_hmc_end(n, d, _ptr(self.cov), ctypes.c_ulonglong(self.seed), _ptr(B["si"]), _ptr(B["z"]),
+                                 _ptr(B["lp"]), _ptr(B["g"]), _ptr(B["p"]), _ptr(B["z0"]), _ptr(B["lp0"]),
+                                 _ptr(B["g0"]), _ptr(B["p0"]), _ptr(B["zp"]), _ptr(B["vel"]), _ptr(B["acc"]),
+                                 _ptr(B["samples"]), self.burnin, self.n_results, _stream()), "gl_hmc_end")
+        nat.check(lib.gl_hmc_adapt(n, d, _ptr(B["si"]), _ptr(B["sf"]), _ptr(B["z0"]), _ptr(B["zp"]), _ptr(B["vel"]),
+                                   _ptr(B["acc"]), int(self.adapt), self.n_adapt, self.target, self.max_l,
+                                   _ptr(B["eps_h"]), _ptr(B["acc_h"]), _ptr(B["l_h"]), _ptr(B["t_h"]), B["total"],
+                                   _stream()), "gl_hmc_adapt")
+
+    def run(self, z0: torch.Tensor, graph: bool = False):
+        n, d, dev = self.n, self.d, self.device
+        total = self.burnin + self.n_results
+        f = lambda *shape: torch.zeros(shape, device=dev, dtype=torch.float32)  # noqa: E731
+        B = {"z": z0.to(dev, torch.float32).contiguous().clone(), "lp": f(n), "g": f(n, d), "p": f(n, d),
+             "z0": f(n, d), "lp0": f(n), "g0": f(n, d), "p0": f(n, d), "zp": f(n, d), "vel": f(n, d), "acc": f(n),
+             "samples": f(max(1, self.n_results), n, d), "eps_h": f(total), "acc_h": f(total),
+             "l_h": torch.zeros(total, device=dev, dtype=torch.int32), "t_h": f(total), "total": total}
+        eps, T = self.init_eps, self.init_eps * self.init_l
+        u0 = 0.5  # base-2 Halton sequence, first element
+        L0 = min(self.max_l, max(1, math.ceil(u0 * T / eps))) if self.adapt else self.init_l
+        B["si"] = torch.tensor([0, L0, 0, 0], device=dev, dtype=torch.int32)
+        B["sf"] = torch.tensor([eps, u0, math.log(T), 0.0, 0.0, math.log(10 * eps), 0.0, math.log(T)], device=dev,
+                               dtype=torch.float32)
+        self.lp_grad(B["z"], B["lp"], B["g"], -1, None)
         if graph:
             from .graphs import capture
-            st_z, st_p, st_g, st_lp = z.clone(), torch.zeros_like(z), g.clone(), lp.clone()
-            eps_t = torch.zeros((), dtype=torch.float32, device=self.device)
-
-            def step():
-                st_z.add_((eps_t * st_p) @ cov)
-                lpn_, gn_ = lp_grad(st_z)
-                st_lp.copy_(lpn_)
-                st_g.copy_(gn_)
-                st_p.add_(eps_t * st_g)
-            leap = capture(step)
-        n_adapt = int(num_burnin_steps * 0.8)
-        eps = float(init_eps)
-        traj = float(init_eps) * int(init_l)  # trajectory length (ChEES adapts this)
-        # dual averaging (Hoffman & Gelman 2014, Alg. 5) state
-        mu = math.log(10 * eps)
-        h_bar, log_eps_bar = 0.0, 0.0
-        gamma, t0, kappa = 0.05, 10.0, 0.75
-        # ChEES (Hoffman, Radul & Sountsov 2021): Adam on log trajectory length
-        log_traj = math.log(traj)
-        ch_m, ch_v = 0.0, 0.0
-        samples, accept_hist, eps_hist = [], [], []
-        total = num_burnin_steps + num_results
-        for it in range(total):
-            jitter = (it * 0.6180339887498949) % 1.0 if adapt else 1.0
-            L = int(min(max_leapfrog_steps, max(1, math.ceil(jitter * traj / eps)))) if adapt else int(init_l)
-            p = torch.randn((n_hmc, d), generator=gen).to(self.device) @ chol_inv
-            # p ~ N(0, Sigma^-1):  p = eps_std @ L^-1 with Sigma = L L^T
-            z0, lp0, g0, p0 = z, lp, g, p
-            if leap is not None:
-                st_z.copy_(z)
-                st_p.copy_(p + 0.5 * eps * g)
-                eps_t.fill_(eps)
-                for _ in range(L):
-                    leap.replay()
-                zn, pn, gn, lpn = st_z.clone(), st_p.clone(), st_g.clone(), st_lp.clone()
-            else:
-                zn, pn, gn = z.clone(), p + 0.5 * eps * g, g
-                for _ in range(L):
-                    zn = zn + eps * pn @ cov
-                    lpn, gn = lp_grad(zn)
-                    pn = pn + eps * gn
-            pn = pn - 0.5 * eps * gn
-            h0 = -lp0 + 0.5 * ((p0 @ cov) * p0).sum(1)
-            h1 = -lpn + 0.5 * ((pn @ cov) * pn).sum(1)
-            log_acc = torch.clamp(-(h1 - h0), max=0.0)
-            log_acc = torch.nan_to_num(log_acc, nan=-float("inf"))
-            u = torch.rand(n_hmc, generator=gen).to(self.device)
-            acc = torch.log(u) < log_acc
-            acc_prob = torch.exp(log_acc)
-            z = torch.where(acc[:, None], zn, z0)
-            lp = torch.where(acc, lpn, lp0)
-            g = torch.where(acc[:, None], gn, g0)
-            if adapt and it < n_adapt:
-                a = float(acc_prob.mean())
-                m_ = it + 1
-                h_bar = (1 - 1 / (m_ + t0)) * h_bar + (target_accept - a) / (m_ + t0)
-                log_eps = mu - math.sqrt(m_) / gamma * h_bar
-                w = m_ ** (-kappa)
-                log_eps_bar = w * log_eps + (1 - w) * log_eps_bar
-                eps = math.exp(log_eps)
-                # ChEES gradient of E[(|z'-mean|^2 - |z-mean|^2)^2] w.r.t. trajectory length
-                zc = z0 - z0.mean(0)
-                znc = zn - zn.mean(0)
-                diff = (znc ** 2).sum(1) - (zc ** 2).sum(1)
-                vel = (pn @ cov)
-                term = acc_prob * diff * (znc * vel).sum(1) * jitter
-                term = term[torch.isfinite(term)]
-                gch = float(term.mean()) * traj if term.numel() else 0.0
-                if math.isfinite(gch):
-                    ch_m = 0.9 * ch_m + 0.1 * gch
-                    ch_v = 0.999 * ch_v + 0.001 * gch * gch
-                    log_traj += 0.025 * (ch_m / (1 - 0.9 ** m_)) / (math.sqrt(ch_v / (1 - 0.999 ** m_)) + 1e-8)
-                log_traj = min(max(log_traj, math.log(eps)), math.log(max_leapfrog_steps * eps))
-                traj = math.exp(log_traj)
-            elif adapt and it == n_adapt:
-                eps = math.exp(log_eps_bar)
-            if it >= num_burnin_steps:
-                samples.append(z.detach().clone())
-            accept_hist.append(float(acc.float().mean()))
-            eps_hist.append(eps)
-        stats = {"accept_rate": np.array(accept_hist), "step_size": np.array(eps_hist)}
-        return torch.stack(samples), stats
+            g = capture(lambda: self._launch_iteration(B))
+            for _ in range(total):
+                g.replay()
+        else:
+            for _ in range(total):
+                self._launch_iteration(B)
+        stats = {"accept_rate": B["acc_h"].cpu().numpy(), "step_size": B["eps_h"].cpu().numpy(),
+                 "leapfrog_steps": B["l_h"].cpu().numpy(), "trajectory_length": B["t_h"].cpu().numpy()}
+        self.buffers = B
+        return B["samples"][:self.n_results], stats

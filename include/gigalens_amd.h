@@ -277,6 +277,47 @@ int gl_svi_diag_update(const float* lp, const float* grad_z, const float* eps, i
                        float* loc, float* raw, float* m, float* v, int* step, float lr, float beta1,
                        float beta2, float epsilon, float* loss_hist, int hist_len, hipStream_t stream);
 
+/* Log posterior (and gradient) like gl_log_prob, as step k of a loop whose trip count
+ * *limit lives in device memory: the launch does nothing when k >= *limit, so a
+ * loop of data-dependent length can be recorded into a HIP graph at its maximum
+ * length.  Resident-image plans only (GL_ERR_UNSUPPORTED otherwise).  No counterpart
+ * in the reference (its HMC runs a tf.while_loop, tfp.mcmc). */
+int gl_log_prob_masked(const gl_plan* plan, const float* z, int bs, float* lp, float* chi2,
+                       const float* g_lp, const float* g_chi2, float* grad_z, const int* limit, int k,
+                       hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Preconditioned HMC with dual-averaging step size and ChEES trajectory-length
+ * adaptation on the device (the kernel stack of tf/inference.py:98-131).  n chains of
+ * d <= 128 parameters.  State: si (4 ints: iteration, leapfrog steps L of the current
+ * trajectory), sf (8 floats: step size, trajectory jitter, log T, dual-averaging
+ * h_bar, log eps_bar, mu, averaged squared trajectory gradient).  One iteration:
+ * gl_hmc_begin; for k < max_leapfrog: gl_hmc_position(k), target log-density and
+ * gradient at z (gl_log_prob_masked with limit si + 1), gl_hmc_momentum(k);
+ * gl_hmc_end; gl_hmc_adapt.  Steps k >= L change nothing, so the sequence can be
+ * recorded once into a HIP graph.  Mass matrix: cov (d x d) with chol_inv its
+ * inverse Cholesky factor; momenta p ~ N(0, cov^-1).
+ * ------------------------------------------------------------------------- */
+int gl_hmc_begin(int n, int d, const float* chol_inv, unsigned long long seed, const int* si, const float* sf,
+                 const float* z, const float* lp, const float* g, float* p, float* z0, float* lp0, float* g0,
+                 float* p0, hipStream_t stream);
+int gl_hmc_position(int n, int d, const float* cov, const int* si, const float* sf, int k, float* z,
+                    const float* p, hipStream_t stream);
+int gl_hmc_momentum(int n, int d, const int* si, const float* sf, int k, float* p, const float* g,
+                    hipStream_t stream);
+/* Metropolis correction (energy -lp + p^T cov p / 2); proposal zp and velocity cov p
+ * kept; acc (n) the acceptance probabilities; samples (n_results x n x d) from
+ * iteration burnin on. */
+int gl_hmc_end(int n, int d, const float* cov, unsigned long long seed, const int* si, float* z, float* lp,
+               float* g, const float* p, const float* z0, const float* lp0, const float* g0, const float* p0,
+               float* zp, float* vel, float* acc, float* samples, int burnin, int n_results,
+               hipStream_t stream);
+/* adapt = 0: fixed step size and L.  Histories (hist_len entries, by iteration): step
+ * size, mean acceptance, L, trajectory length. */
+int gl_hmc_adapt(int n, int d, int* si, float* sf, const float* z0, const float* zp, const float* vel,
+                 const float* acc, int adapt, int n_adapt, float target, int max_leapfrog, float* eps_hist,
+                 float* acc_hist, int* l_hist, float* t_hist, int hist_len, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Learned initialiser.  No counterpart in the reference: a small BF16 CNN
  * (hand-written MFMA kernels, FP32 accumulation) mapping an observed image to
