@@ -77,15 +77,26 @@ def setup(bs, device):
     return prior, phys, cfg, sim, pm, obs, z0
 
 
-def cpu_baseline(obs, budget_s=20.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(obs, n=1000, chunk=250, runs=3):
     """Oracle port (torch-CPU FP32, op-for-op restatement of the reference's TF
-    path, autograd) timed on this host: 2 warm-ups, median of 5 runs."""
+    path, autograd) timed on this host: log_prob + gradient of ``n`` prior
+    particles (in batches of ``chunk`` to bound memory), one small warm-up, median
+    of ``runs`` runs."""
     from oracle import torch_ref
     from oracle import np64
     prior = wl.epl_shear_prior()
     cfg = wl.epl_shear_config()
-    from gigalens_amd.simulator import Camera
-    cam = Camera(cfg)
     ocam = np64.Camera(cfg.delta_pix, cfg.num_pix, cfg.supersample, cfg.kernel, None)
     model = {"lenses": [("EPL", {"niter": 50}), ("SHEAR", {})], "lens_light": [("SERSIC_ELLIPSE", {})],
              "source_light": [("SERSIC_ELLIPSE", {})]}
@@ -99,22 +110,50 @@ def cpu_baseline(obs, budget_s=20.0):
         return (kind[node.kind], tuple(node.args()))
     nested = rec(prior)
     sim = torch_ref.Sim(model, ocam, dtype=torch.float32)
-    x = flat_sample(prior, 4000, seed=3)
+    x = flat_sample(prior, 16 + n * runs, seed=3)
     z_all = torch.as_tensor(torch_ref.x_to_z(nested, x.double().numpy()), dtype=torch.float32)
 
     def run(bs, off):
-        z = z_all[off:off + bs].clone().requires_grad_(True)
         t = time.perf_counter()
-        lp, _ = torch_ref.log_prob(sim, nested, z, obs, 0.2, 100.0)
-        (-lp.mean()).backward()
+        for a in range(off, off + bs, chunk):
+            z = z_all[a:min(a + chunk, off + bs)].clone().requires_grad_(True)
+            lp, _ = torch_ref.log_prob(sim, nested, z, obs, 0.2, 100.0)
+            (-lp.mean()).backward()
         return time.perf_counter() - t
-    t1 = run(16, 0)
-    bs = int(max(16, min(512, 16 * budget_s / 7.0 / max(t1, 1e-3))))
-    times = [run(bs, 16 + i * bs) for i in range(7)]
-    med = float(np.median(times[2:]))
-    return {"value": bs / med, "unit": "particle-evals/s", "cores": torch.get_num_threads(), "kind": "port",
-            "sample": f"{bs} prior particles per run, sim+log_prob+autograd grad in FP32 (torch CPU, "
-                      f"op-for-op restatement of tf/simulator.py + tf/model.py), 2 warm-ups, median of 5 runs"}
+    run(16, 0)
+    times = [run(n, 16 + i * n) for i in range(runs)]
+    med = float(np.median(times))
+    return {"value": n / med, "unit": "particle-evals/s", "cores": torch.get_num_threads(), "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"{n} prior particles per run (batches of {chunk}), sim+log_prob+autograd grad in FP32 (torch "
+                      f"CPU, op-for-op restatement of tf/simulator.py + tf/model.py), 1 warm-up of 16, median of "
+                      f"{runs} runs"}
+
+
+def oracle_error_map(prior, phys, cfg, pm, obs, z0, n=8):
+    """The GPU path against the FP64 oracle on the workload's first ``n`` particles:
+    image (from the same physical parameters), log_prob, chi2 and the gradient
+    (Richardson-extrapolated central differences of the oracle)."""
+    from oracle import bridge as ob
+    z = z0[:n].contiguous()
+    z64 = z.double().cpu().numpy()
+    x64 = ob.z_to_x(prior, z64)
+    sim = LensSimulator(phys, cfg, bs=n, device=z.device)
+    img = sim.simulate(torch.as_tensor(x64, dtype=torch.float32, device=z.device)).double().cpu().numpy()
+    lp, chi2, g = pm.log_prob_and_grad(sim, z)
+    ocam = ob.oracle_camera(cfg)
+    img_ref = ob.oracle_simulate(ocam, phys, ob.nested_from_x(prior, x64.astype(np.float32).astype(np.float64)))
+    lp_ref, chi_ref, _ = ob.oracle_log_prob(ocam, phys, prior, z64, obs, 0.2, 100.0)
+    g_ref = ob.oracle_grad_richardson(ocam, phys, prior, z64, obs, 0.2, 100.0)
+    lp, chi2, g = (t.double().cpu().numpy() for t in (lp, chi2, g))
+    peak = np.abs(img_ref).max(axis=(1, 2))
+    return {"particles": n, "oracle": "FP64 (oracle/np64.py)",
+            "image_max_abs": float(np.abs(img - img_ref).max()),
+            "image_max_abs_over_peak": float((np.abs(img - img_ref).max(axis=(1, 2)) / peak).max()),
+            "log_prob_max_abs": float(np.abs(lp - lp_ref).max()),
+            "log_prob_max_rel": float((np.abs(lp - lp_ref) / np.abs(lp_ref)).max()),
+            "chi2_max_rel": float((np.abs(chi2 - chi_ref) / np.abs(chi_ref)).max()),
+            "grad_max_abs_over_particle_max": float((np.abs(g - g_ref).max(1) / np.abs(g_ref).max(1)).max())}
 
 
 def shapelet_setup(bs, device, interpolate=True):
@@ -171,8 +210,34 @@ def shapelet_cpu_baseline(prior, phys, cfg, sim, obs, z0, n=1):
         return time.perf_counter() - t
     dt = run()
     return {"value": n / dt, "unit": "particle-evals/s", "cores": torch.get_num_threads(), "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{n} prior particle(s), BackwardProbModel log_prob + autograd gradient in FP32 (torch CPU "
                       f"restatement of tf/simulator.py lstsq_simulate + tf/model.py), 1 timed run"}
+
+
+def oracle_error_shapelets(prior, phys, cfg, pm, obs, z0, n=2):
+    """The least-squares path against the FP64 oracle (pinv solve) on the workload's
+    first ``n`` particles: image and BackwardProbModel log_prob / chi2."""
+    from oracle import bridge as ob
+    from oracle import np64
+    z = z0[:n].contiguous()
+    z64 = z.double().cpu().numpy()
+    x64 = ob.z_to_x(prior, z64)
+    sim = LensSimulator(phys, cfg, bs=n, device=z.device)
+    lp, chi2 = pm.log_prob(sim, z)
+    err = np64.backward_err_map(np.asarray(obs, np.float64), 0.2, 100.0)
+    img = sim.lstsq_simulate(torch.as_tensor(x64, dtype=torch.float32, device=z.device), obs,
+                             err.astype(np.float32)).double().cpu().numpy()
+    ocam = ob.oracle_camera(cfg)
+    lp_ref, chi_ref, img_ref = ob.oracle_backward_log_prob(ocam, phys, prior, z64, obs, 0.2, 100.0)
+    lp, chi2 = lp.double().cpu().numpy(), chi2.double().cpu().numpy()
+    peak = np.abs(img_ref).max(axis=(1, 2))
+    return {"particles": n, "oracle": "FP64 (oracle/np64.py, pinv rcond 1e-6)",
+            "image_max_abs": float(np.abs(img - img_ref).max()),
+            "image_max_abs_over_peak": float((np.abs(img - img_ref).max(axis=(1, 2)) / peak).max()),
+            "log_prob_max_abs": float(np.abs(lp - lp_ref).max()),
+            "log_prob_max_rel": float((np.abs(lp - lp_ref) / np.abs(lp_ref)).max()),
+            "chi2_max_rel": float((np.abs(chi2 - chi_ref) / np.abs(chi_ref)).max())}
 
 
 def run_shapelets(args, dev):
@@ -236,6 +301,7 @@ def run_shapelets(args, dev):
         out["hmc"] = shapelet_hmc(prior, phys, cfg, pm, z0, dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = shapelet_cpu_baseline(prior, phys, cfg, sim, obs, z0)
+        out["oracle_error"] = oracle_error_shapelets(prior, phys, cfg, pm, obs, z0)
     print(json.dumps(out), flush=True)
 
 
@@ -332,9 +398,48 @@ def run_cnn(args, dev):
         cnn_ref.loss_and_grad(p, init.net.layout, x, tt, 0.2, ch)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": n / dt, "unit": "images/s", "cores": torch.get_num_threads(), "kind": "port",
+                               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
                                "sample": f"{n} images, forward + NLL + autograd backward of the torch-CPU restatement "
                                          "(FP32 with the same bf16 rounding points), 1 warm-up, 1 timed run"}
+        lref, gref = cnn_ref.loss_and_grad(p, init.net.layout, x, tt, 0.2, ch)
+        loss, g = init.net.loss_grad(imgs[:n], t[:n], want_grad=True)
+        gref = gref.double()
+        out["oracle_error"] = {"images": n, "oracle": "torch-CPU FP32 restatement with the kernels' bf16 rounding "
+                                                      "points (oracle/cnn_ref.py)",
+                               "loss_rel": abs(float(loss) - float(lref)) / abs(float(lref)),
+                               "grad_max_abs_over_max": float((g.double().cpu() - gref).abs().max()
+                                                              / gref.abs().max())}
     print(json.dumps(out), flush=True)
+
+
+def oracle_error_uneven(prior, phys, lens_set, batch, z, per_lens=4):
+    """The batched log_prob and gradient (after MAP) against the FP64 oracle for the
+    first ``per_lens`` particles of four lenses: the smallest and the largest image,
+    the first lens on the resident-image (fused) path and the first on the staged
+    path."""
+    from oracle import bridge as ob
+    lp, chi2, g = batch.log_prob(z, want_grad=True)
+    npix = [c.num_pix * c.supersample for c, _, _ in lens_set]
+    sel = [int(np.argmin(npix)), int(np.argmax(npix))]
+    for want in (True, False):
+        k = next((i for i, f in enumerate(batch.fused) if f == want), None)
+        if k is not None and k not in sel:
+            sel.append(k)
+    rows = []
+    for k in sel:
+        cfg, obs, cnt = lens_set[k]
+        a, m = int(batch.first[k]), min(per_lens, cnt)
+        z64 = z[a:a + m].double().cpu().numpy()
+        ocam = ob.oracle_camera(cfg)
+        lp_ref, chi_ref, _ = ob.oracle_log_prob(ocam, phys, prior, z64, obs, 0.2, 100.0)
+        g_ref = ob.oracle_grad_richardson(ocam, phys, prior, z64, obs, 0.2, 100.0)
+        lpk, chk, gk = (t[a:a + m].double().cpu().numpy() for t in (lp, chi2, g))
+        rows.append({"lens": k, "num_pix": cfg.num_pix, "supersample": cfg.supersample, "fused": bool(batch.fused[k]),
+                     "particles": m, "log_prob_max_rel": float((np.abs(lpk - lp_ref) / np.abs(lp_ref)).max()),
+                     "chi2_max_rel": float((np.abs(chk - chi_ref) / np.abs(chi_ref)).max()),
+                     "grad_max_abs_over_particle_max": float((np.abs(gk - g_ref).max(1)
+                                                              / np.abs(g_ref).max(1)).max())})
+    return {"oracle": "FP64 (oracle/np64.py)", "lenses": rows}
 
 
 def uneven_lenses(args, dev):
@@ -442,6 +547,8 @@ def run_uneven(args, dev):
                "final_best_chi2_p50": float(np.median(best_chi[-1])),
                "svi_final_loss_finite": int(torch.isfinite(losses[-1]).sum()),
                "particle_evals_per_s": sum(c for *_, c in lens_set) / float(step_s.mean())}
+        if tag == "batched" and not args.no_cpu_baseline:
+            res["oracle_error"] = oracle_error_uneven(prior, phys, lens_set, batch, z)
         print(f"[uneven] {tag}: {json.dumps({k: v for k, v in res.items()})}", flush=True)
         del batch, svi
         return res
@@ -460,8 +567,8 @@ def run_uneven(args, dev):
                                   f"{args.uneven_scale}, 11x11 Gaussian PSF; learned-init MAP 350 steps + diagonal "
                                   f"SVI 1500 steps x 64 samples, every step measured", "lenses": L,
                       "particles": total, "scale": args.uneven_scale},
-           "batched": dict(b, latency_p50_s=b["wall_s"], latency_p95_s=b["wall_s"],
-                           latency_note="every lens completes with the batch"),
+           "batched": dict(b, latency_note="every lens's fit completes with the batch (wall_s); "
+                                            "time_to_map_chi2_p50/p95 are per-lens"),
            "learned_training_s": t_train,
            "lenses_per_hour_incl_training": L / (b["wall_s"] + t_train) * 3600.0}
     if seq_rows:
@@ -611,7 +718,12 @@ def main():
                      "kernel_ms": kern_ms, "flop_per_launch": tot, "transcendentals_per_launch": tot_tr,
                      "psf_path": ("separable rank %d" % plan.info()["sep_rank"]) if plan.info()["sep_rank"]
                      else "direct 2-D",
-                     "reference_formulation_flop_per_launch": tot_survey},
+                     "reference_formulation_flop_per_launch": tot_survey,
+                     "reference_formulation_achieved": tot_survey / (kern_ms * 1e-3) / 1e12,
+                     "reference_formulation_frac": tot_survey / (kern_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                     "formulation_note": "achieved/frac count the flops the kernel executes (Clenshaw series, "
+                                         "separable PSF, adjoint); reference_formulation_* count the reference's "
+                                         "per-particle-eval operations (series, dense PSF, autodiff backward)"},
         "hip_graph": {"ms_per_step": graph_ms, "eager_ms_per_step": ms_per_step,
                       "particle_evals_per_s": bs / (graph_ms * 1e-3), "same_result_as_eager": graph_same},
         "time_to_map": {"chi2_threshold": 1.1,
@@ -623,6 +735,7 @@ def main():
         out["time_to_map"]["learned_vs_prior"] = learned_time_to_map(prior, phys, cfg, dev, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(obs)
+        out["oracle_error"] = oracle_error_map(prior, phys, cfg, pm, obs, z0)
     print(json.dumps(out), flush=True)
 
 

@@ -42,12 +42,12 @@ struct KArgs {
     const int* lens;      // optional per-particle observed-image index (bs)
     const float* weight;  // MAP: optional per-particle loss weight (bs); default 1/bs
     const int* order;     // optional: particle of workgroup b is order[b] (launch order, see order_kernel)
-    const int* limit;     // optional device scalar: the launch does nothing when limit_k >= *limit (a step
-                          // of a graph-recorded loop whose length is decided on the device)
-    int limit_k;
     float lr_t, beta1, beta2, eps, gscale;
     int mode;
     int bs;
+    int limit_k;
+    const int* limit;     // optional device scalar: the launch does nothing when limit_k >= *limit (a step
+                          // of a graph-recorded loop whose length is decided on the device)
 };
 
 template <int K> struct LensAcc { static constexpr int n = 0; };
