@@ -367,6 +367,10 @@ def run_cnn(args, dev):
     init.train(n_images=bs * K, batch=bs, lr=1e-3, seed=4)
     torch.cuda.synchronize()
     wall_step = (time.perf_counter() - t0) / K
+    if args.copy_probe:
+        a = torch.empty(64 * 1024 * 1024, device=dev)
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
     N = cfg.num_pix
     ch = init.net.channels
     fwd = 0.0

@@ -54,6 +54,7 @@ EXPORTED_SYMBOLS = (
     "gl_hmc_momentum",
     "gl_hmc_end",
     "gl_hmc_adapt",
+    "gl_noise_poisson_gauss",
     "gl_cnn_create",
     "gl_cnn_destroy",
     "gl_cnn_num_params",
@@ -177,6 +178,8 @@ _SIGS = {
                     _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "gl_hmc_adapt": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                       ctypes.c_float, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    "gl_noise_poisson_gauss": ([_vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_float, ctypes.c_ulonglong, _vp, _vp],
+                               ctypes.c_int),
     "gl_cnn_create": ([ctypes.POINTER(CnnDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "gl_cnn_destroy": ([_vp], ctypes.c_int),
     "gl_cnn_num_params": ([_vp], ctypes.c_int),

@@ -2,7 +2,7 @@
 
 The library is several translation units compiled in parallel (the fused
 particle kernels by profile combination, the staged pipeline, the network, the
-batched SVI and HMC kernels and the C ABI) and linked into
+batched SVI and HMC kernels, the noise sampler and the C ABI) and linked into
 ``_lib/libgigalens_amd.so``.  ``build()`` compiles
 every unit each time it is called; ``build(incremental=True)`` recompiles only
 units whose recorded dependencies (``-MMD`` files) changed.
@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "_lib", "libgigalens_amd.so")
-UNITS = ["fused_0.hip", "fused_1.hip", "fused_2.hip", "staged.hip", "cnn.hip", "svi.hip", "hmc.hip", "capi.hip"]
+UNITS = ["fused_0.hip", "fused_1.hip", "fused_2.hip", "staged.hip", "cnn.hip", "svi.hip", "hmc.hip", "noise.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

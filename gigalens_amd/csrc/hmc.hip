@@ -31,6 +31,7 @@
 #include <string>
 
 #include "abi_common.h"
+#include "philox.h"
 
 namespace gl {
 namespace hmc {
@@ -40,24 +41,10 @@ enum { SI_ITER = 0, SI_L = 1, SI_N = 4 };
 // float state
 enum { SF_EPS = 0, SF_JITTER, SF_LOGT, SF_HBAR, SF_LOGEPSBAR, SF_MU, SF_SQGRAD, SF_LOGTBAR, SF_N = 8 };
 
-struct U4 {
-    unsigned x, y, z, w;
-};
+using rng::U4;
+using rng::unit;
 
-__device__ __forceinline__ U4 philox(U4 c, unsigned long long seed) {
-    const unsigned M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-    unsigned k0 = (unsigned)seed, k1 = (unsigned)(seed >> 32);
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const unsigned long long p0 = (unsigned long long)M0 * c.x, p1 = (unsigned long long)M1 * c.z;
-        c = U4{(unsigned)(p1 >> 32) ^ c.y ^ k0, (unsigned)p1, (unsigned)(p0 >> 32) ^ c.w ^ k1, (unsigned)p0};
-        k0 += W0;
-        k1 += W1;
-    }
-    return c;
-}
-
-__device__ __forceinline__ float unit(unsigned u) { return ((float)(u >> 8) + 1.0f) * (1.0f / 16777216.0f); }
+__device__ __forceinline__ U4 philox(U4 c, unsigned long long seed) { return rng::philox4x32_10(c, seed); }
 
 // base-2 radical inverse of i >= 1, in (0, 1)
 __device__ __forceinline__ float halton2(unsigned i) {
@@ -82,10 +69,7 @@ __global__ void begin(int n, int d, const float* chol_inv, unsigned long long se
     for (int i = 0; i < d; ++i) p0[(size_t)c * d + i] = 0.0f;
     for (int k = 0; k < d; ++k) {
         if ((k & 3) == 0) {
-            const U4 r = philox(U4{(unsigned)c, (unsigned)(k >> 2), (unsigned)it, 0u}, seed);
-            const float r0 = sqrtf(-2.0f * logf(unit(r.x))), r1 = sqrtf(-2.0f * logf(unit(r.z)));
-            const float t0 = 6.283185307179586f * unit(r.y), t1 = 6.283185307179586f * unit(r.w);
-            xi[0] = r0 * cosf(t0); xi[1] = r0 * sinf(t0); xi[2] = r1 * cosf(t1); xi[3] = r1 * sinf(t1);
+            rng::normals4(philox(U4{(unsigned)c, (unsigned)(k >> 2), (unsigned)it, 0u}, seed), xi);
         }
         const float x = xi[k & 3];
         for (int i = 0; i < d; ++i) p0[(size_t)c * d + i] = fmaf(x, chol_inv[(size_t)k * d + i], p0[(size_t)c * d + i]);

@@ -10,50 +10,23 @@
 // tf/inference.py:65-71): mean over the lens's samples of log q(z_s) - log p(z_s),
 // with z_s = loc + sigma * eps_s.
 //
-// Normal draws: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
-// as easy as 1, 2, 3", SC 2011) keyed by the seed, counter (particle, dimension
-// group, step, 0), then the Box-Muller transform.
+// Normal draws: Philox4x32-10 (philox.h) keyed by the seed, counter (particle,
+// dimension group, step, 0), then the Box-Muller transform.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include <string>
 
 #include "abi_common.h"
+#include "philox.h"
 
 namespace gl {
 namespace svi {
 
-struct U4 {
-    unsigned x, y, z, w;
-};
-
-__device__ __forceinline__ U4 philox4x32_10(U4 c, unsigned k0, unsigned k1) {
-    const unsigned M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const unsigned long long p0 = (unsigned long long)M0 * c.x, p1 = (unsigned long long)M1 * c.z;
-        const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
-        const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
-        k0 += W0;
-        k1 += W1;
-    }
-    return c;
-}
-
-// (0, 1] from 32 random bits
-__device__ __forceinline__ float unit(unsigned u) { return ((float)(u >> 8) + 1.0f) * (1.0f / 16777216.0f); }
-
 // Four standard normals for (particle, dimension group g, step).
 __device__ __forceinline__ void normals4(unsigned long long seed, unsigned particle, unsigned g, unsigned step,
                                          float out[4]) {
-    const U4 r = philox4x32_10(U4{particle, g, step, 0u}, (unsigned)seed, (unsigned)(seed >> 32));
-    const float r0 = sqrtf(-2.0f * logf(unit(r.x))), r1 = sqrtf(-2.0f * logf(unit(r.z)));
-    const float t0 = 6.283185307179586f * unit(r.y), t1 = 6.283185307179586f * unit(r.w);
-    out[0] = r0 * cosf(t0);
-    out[1] = r0 * sinf(t0);
-    out[2] = r1 * cosf(t1);
-    out[3] = r1 * sinf(t1);
+    rng::normals4(rng::philox4x32_10(rng::U4{particle, g, step, 0u}, seed), out);
 }
 
 constexpr float kShift = 1e-6f;  // diag_shift of the surrogate scale

@@ -145,6 +145,23 @@ class ConvNet:
                                               float(weight_decay), _stream()), "gl_cnn_adamw_step")
 
 
+def poisson_gauss_noise(img: torch.Tensor, exp_time: float, background_rms: float, seed: int,
+                        out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Poisson(max(img, 0) exp_time) / exp_time + background_rms N(0, 1) per pixel on the
+    device (csrc/noise.hip); ``out`` may be ``img``."""
+    img = img.contiguous()
+    if img.device.type != "cuda" or img.dtype != torch.float32:
+        raise ValueError("poisson_gauss_noise: img must be a float32 CUDA tensor")
+    if out is None:
+        out = torch.empty_like(img)
+    elif out.shape != img.shape or out.dtype != torch.float32 or out.device != img.device or not out.is_contiguous():
+        raise ValueError("poisson_gauss_noise: out must match img")
+    nat.check(nat.lib().gl_noise_poisson_gauss(_ptr(img), img.numel(), float(exp_time), float(background_rms),
+                                               ctypes.c_ulonglong(int(seed) % (2 ** 64)), _ptr(out), _stream()),
+              "gl_noise_poisson_gauss")
+    return out
+
+
 def _flat(prior, sample) -> torch.Tensor:
     return torch.stack([torch.as_tensor(v, dtype=torch.float32) for v in dist.flatten(sample)], dim=1)
 
@@ -181,13 +198,10 @@ class LearnedInitializer:
         z = self.bij.inverse(x_nested).to(self.device, torch.float32)
         x = _flat(self.prior, x_nested).to(self.device).contiguous()
         img = self.sim.plan.simulate(x)
-        g = torch.Generator(device=self.device)
         ns = seed if noise_seed is None else noise_seed
-        g.manual_seed(int(ns) * 2654435761 % (2 ** 62))
-        lam = torch.clamp(img, min=0.0) * self.exp_time
-        noisy = torch.poisson(lam, generator=g) / self.exp_time
-        noisy = noisy + self.background_rms * torch.randn(img.shape, device=self.device, generator=g)
-        return noisy.contiguous(), z
+        noisy = poisson_gauss_noise(img, self.exp_time, self.background_rms, int(ns) * 2654435761 % (2 ** 64),
+                                    out=img)
+        return noisy, z
 
     def standardise(self, z: torch.Tensor) -> torch.Tensor:
         return (z - self.z_mean) / self.z_std

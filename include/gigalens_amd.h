@@ -291,7 +291,8 @@ int gl_log_prob_masked(const gl_plan* plan, const float* z, int bs, float* lp, f
  * adaptation on the device (the kernel stack of tf/inference.py:98-131).  n chains of
  * d <= 128 parameters.  State: si (4 ints: iteration, leapfrog steps L of the current
  * trajectory), sf (8 floats: step size, trajectory jitter, log T, dual-averaging
- * h_bar, log eps_bar, mu, averaged squared trajectory gradient).  One iteration:
+ * h_bar, log eps_bar, mu, averaged squared trajectory gradient, averaged log T).
+ * One iteration:
  * gl_hmc_begin; for k < max_leapfrog: gl_hmc_position(k), target log-density and
  * gradient at z (gl_log_prob_masked with limit si + 1), gl_hmc_momentum(k);
  * gl_hmc_end; gl_hmc_adapt.  Steps k >= L change nothing, so the sequence can be
@@ -317,6 +318,17 @@ int gl_hmc_end(int n, int d, const float* cov, unsigned long long seed, const in
 int gl_hmc_adapt(int n, int d, int* si, float* sf, const float* z0, const float* zp, const float* vel,
                  const float* acc, int adapt, int n_adapt, float target, int max_leapfrog, float* eps_hist,
                  float* acc_hist, int* l_hist, float* t_hist, int hist_len, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Observation noise on the device (training images of the learned initialiser):
+ * out[i] = Poisson(max(img[i], 0) exp_time) / exp_time + background_rms N(0, 1), the
+ * noise model behind the likelihood variance of tf/model.py:73-76.  Draws from
+ * Philox4x32-10 keyed by seed with counter (i, block, 0x5A17): a pure function of
+ * (seed, i).  Poisson by CDF inversion below mean 10 and PTRS (Hoermann 1993) above.
+ * img and out may be the same buffer.  No allocation; graph-capturable.
+ * ------------------------------------------------------------------------- */
+int gl_noise_poisson_gauss(const float* img, long long n, float exp_time, float background_rms,
+                           unsigned long long seed, float* out, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Learned initialiser.  No counterpart in the reference: a small BF16 CNN
