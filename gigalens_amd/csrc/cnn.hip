@@ -1,5 +1,6 @@
 // C ABI of the learned initialiser's network (gl_cnn_*): workspace, parameter
-// packing and the launch sequences of the BF16 MFMA kernels in cnn_kernels.hip.
+// packing and the launch sequences of the BF16 MFMA kernels (cnn_conv.hip: band-tiled
+// convolutions; cnn_kernels.hip: input transform, head, reductions, AdamW).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -9,7 +10,7 @@
 #include <vector>
 
 #include "abi_common.h"
-#include "cnn_kernels.hip"
+#include "cnn_conv.hip"
 
 using glc::fail;
 using glc::StreamGuard;
@@ -20,6 +21,8 @@ using glc::StreamGuard;
 struct gl_cnn {
     struct Layer {
         int cin, cout, s, H, W, Ho, Wo, kp;
+        int rf, rd, rw;            // band rows of the forward, data- and weight-gradient passes
+        int wblk, wncg, wnp, wchunks;  // weight-gradient grid: blocks over images, tile chunks
         size_t woff, boff;         // offsets in the parameter vector
         gl::cnn::u16* wt;          // [cout][kp]
         gl::cnn::u16* w2t;         // [cin][9 cout] (layers > 0)
@@ -27,7 +30,7 @@ struct gl_cnn {
         gl::cnn::u16* Z;
         gl::cnn::u16* dZ;
     };
-    int H, W, d, hidden, maxb, P;
+    int H, W, d, hidden, maxb, P, cus;
     float in_scale;
     Layer L[4];
     size_t w1, b1, w2, b2;
@@ -56,16 +59,91 @@ static int cnn_alloc(gl_cnn* net, size_t bytes, void** out) {
     return GL_OK;
 }
 
-// split-K factor of the weight-gradient GEMM: about 1024 workgroups
-static int cnn_wsplit(const gl_cnn::Layer& l, int bs, int* kchunk) {
-    const int K = bs * l.Ho * l.Wo;
-    const int tiles = ((l.cout + 63) / 64) * ((9 * l.cin + 1 + 63) / 64);
-    int splits = std::max(1, std::min(1024 / std::max(tiles, 1), (K + 1023) / 1024));
-    int kc = (K + splits - 1) / splits;
-    kc = (kc + 31) / 32 * 32;
-    splits = (K + kc - 1) / kc;
-    *kchunk = kc;
-    return splits;
+// LDS budget of one band workgroup: two fit in a CU's 160 KB
+constexpr size_t kCnnLds = 80 * 1024;
+
+static int dgrad_groups(const gl_cnn::Layer& L, int R);
+// band + weight tile + output staging (one pass of at most 16 pixel groups)
+static size_t fwd_lds(const gl_cnn::Layer& L, int R) {
+    const size_t slots = (size_t)std::min(gl::cnn::kGroupsPerPass, (R * L.Wo + 15) / 16) * 16;
+    return ((size_t)(L.s * (R - 1) + 3) * (L.W + 2) * (L.cin + 8) + (size_t)9 * 32 * (L.cin + 8)) * 2 +
+           slots * gl::cnn::kStageF * 2;
+}
+static size_t dgrad_lds(const gl_cnn::Layer& L, int R) {
+    const size_t slots = (size_t)std::min(gl::cnn::kGroupsPerPass, dgrad_groups(L, R)) * 16;
+    return ((size_t)((R + L.s) / L.s + 1) * (L.Wo + 2) * (L.cout + 8) + (size_t)9 * 32 * (L.cout + 8)) * 2 +
+           slots * gl::cnn::kStageD * 4;
+}
+// first layer: rows of a forward band (about one pixel per thread)
+static int conv1_rows(const gl_cnn::Layer& L) { return std::max(1, std::min(L.H, (256 + L.W - 1) / L.W)); }
+static size_t conv1_fwd_lds(const gl_cnn::Layer& L) {
+    return ((size_t)L.cout * 10 + (size_t)(conv1_rows(L) + 2) * (L.W + 2)) * 4;
+}
+static size_t conv1_wgrad_lds(const gl_cnn::Layer& L) { return (size_t)(L.H + 2) * (L.W + 2) * 4; }
+static size_t wgrad_lds(const gl_cnn::Layer& L, int R) {
+    return ((size_t)R * L.Wo * (L.cout + 8) + (size_t)(L.s * (R - 1) + 3) * (L.W + 2) * (L.cin + 8)) * 2;
+}
+// pixel groups of 16 of a data-gradient band (worst case over the band's row parity)
+static int dgrad_groups(const gl_cnn::Layer& L, int R) {
+    int worst = 0;
+    for (int h0 = 0; h0 < L.s; ++h0) {
+        int g = 0;
+        for (int c = 0; c < L.s * L.s; ++c) {
+            const int ph = c / L.s, pw = c % L.s;
+            const int fr = ((ph - h0) % L.s + L.s) % L.s;
+            const int nr = fr < R ? (R - fr + L.s - 1) / L.s : 0;
+            const int nc = (L.W - pw + L.s - 1) / L.s;
+            g += (nr * nc + 15) / 16;
+        }
+        worst = std::max(worst, g);
+    }
+    return worst;
+}
+// largest band (rows <= n) meeting the budget, then evened out over the bands
+static int pick_rows(int n, bool (*ok)(const gl_cnn::Layer&, int), const gl_cnn::Layer& L) {
+    int R = 0;
+    for (int r = 1; r <= n; ++r)
+        if (ok(L, r)) R = r;
+    if (R == 0) return 0;
+    const int bands = (n + R - 1) / R;
+    return (n + bands - 1) / bands;
+}
+// 16-byte vectors of a band: the register prefetch holds at most kPre per thread
+constexpr int kBandVecMax = gl::cnn::kPre * gl::cnn::kWaves * 64;
+static bool fwd_ok(const gl_cnn::Layer& L, int R) {
+    return fwd_lds(L, R) <= kCnnLds && (L.s * (R - 1) + 3) * (L.W + 2) * (L.cin / 8) <= kBandVecMax;
+}
+static bool dgrad_ok(const gl_cnn::Layer& L, int R) {
+    return dgrad_lds(L, R) <= kCnnLds && ((R + L.s) / L.s + 1) * (L.Wo + 2) * (L.cout / 8) <= kBandVecMax;
+}
+// persistent grid: every CU holds as many band workgroups as its 160 KB of LDS allows
+static unsigned persistent_grid(int items, size_t lds, int cus) {
+    const long per_cu = std::max<long>(1, (long)(160 * 1024) / (long)std::max<size_t>(lds, 1));
+    return (unsigned)std::min<long>(items, per_cu * cus);
+}
+static bool wgrad_ok(const gl_cnn::Layer& L, int R) { return wgrad_lds(L, R) <= kCnnLds; }
+
+static hipError_t cnn_lds_attrs() {
+    const int lds = (int)kCnnLds;
+    hipError_t e;
+    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_fwd<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds)))
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_fwd<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds)))
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_dgrad<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 lds)))
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_dgrad<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 lds)))
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_wgrad<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 lds)))
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv1_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds)))
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv1_wgrad, hipFuncAttributeMaxDynamicSharedMemorySize, lds)))
+        return e;
+    return hipFuncSetAttribute((const void*)&gl::cnn::conv_wgrad<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
@@ -86,6 +164,15 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
     }
     net->H = desc->height; net->W = desc->width; net->d = desc->n_out; net->hidden = desc->hidden;
     net->maxb = desc->max_batch; net->in_scale = desc->in_scale;
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+            gl_cnn_destroy(net);
+            return fail(GL_ERR_HIP, "gl_cnn_create: device properties");
+        }
+        net->cus = std::max(1, prop.multiProcessorCount);
+    }
     const int strides[4] = {1, 2, 1, 2};
     size_t off = 0;
     int H = net->H, W = net->W, cin = 1;
@@ -95,6 +182,27 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
         L.H = H; L.W = W;
         L.Ho = (H - 1) / L.s + 1; L.Wo = (W - 1) / L.s + 1;
         L.kp = (9 * cin + 31) / 32 * 32;
+        L.rf = L.rd = L.rw = 1;
+        L.wblk = L.wncg = L.wnp = L.wchunks = 0;
+        if (l == 0 && (conv1_fwd_lds(L) > kCnnLds || conv1_wgrad_lds(L) > kCnnLds)) {
+            gl_cnn_destroy(net);
+            return fail(GL_ERR_UNSUPPORTED, "gl_cnn_create: image too large for the first layer's LDS tiles");
+        }
+        if (l > 0) {
+            L.rf = pick_rows(L.Ho, fwd_ok, L);
+            L.rd = pick_rows(L.H, dgrad_ok, L);
+            L.rw = pick_rows(L.Ho, wgrad_ok, L);
+            if (!L.rf || !L.rd || !L.rw) {
+                gl_cnn_destroy(net);
+                return fail(GL_ERR_UNSUPPORTED, "gl_cnn_create: image rows too wide for the LDS band of layer " +
+                                                    std::to_string(l + 1));
+            }
+            L.wblk = std::min(desc->max_batch, 512);
+            L.wncg = std::min(L.cout / 16, 4);
+            L.wnp = gl::cnn::kWaves * gl::cnn::kMaxPairs;
+            const int pairs = 9 * (cin / 16) + 1;
+            L.wchunks = ((L.cout / 16 + L.wncg - 1) / L.wncg) * ((pairs + L.wnp - 1) / L.wnp);
+        }
         L.woff = off; off += (size_t)L.cout * 9 * cin;
         L.boff = off; off += L.cout;
         H = L.Ho; W = L.Wo; cin = L.cout;
@@ -126,9 +234,8 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
         CNN_ALLOC(L.Y, gl::cnn::u16, act);
         CNN_ALLOC(L.Z, gl::cnn::u16, act);
         CNN_ALLOC(L.dZ, gl::cnn::u16, act);
-        int kc;
-        const int sp = cnn_wsplit(L, (int)B, &kc);
-        part = std::max(part, (size_t)sp * L.cout * (9 * L.cin + 1));
+        const size_t sp = l > 0 ? (size_t)L.wblk : B;
+        part = std::max(part, sp * L.cout * (9 * L.cin + 1));
     }
     // the head's weight-gradient chunks share the partial buffer
     part = std::max(part, (size_t)gl::cnn::kHeadChunks * ((size_t)Hd * c4 + Hd + (size_t)d2 * Hd + d2));
@@ -141,6 +248,7 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
     CNN_ALLOC(net->dout, float, B * d2);
     CNN_ALLOC(net->lossb, float, B);
 #undef CNN_ALLOC
+    if (cnn_lds_attrs() != hipSuccess) { gl_cnn_destroy(net); return fail(GL_ERR_HIP, "gl_cnn_create: LDS attribute"); }
     if (hipDeviceSynchronize() != hipSuccess) { gl_cnn_destroy(net); return fail(GL_ERR_HIP, "gl_cnn_create: sync"); }
     *out = net;
     return GL_OK;
@@ -192,17 +300,21 @@ static int cnn_forward_layers(gl_cnn* net, const float* images, int bs, hipStrea
     hipLaunchKernelGGL(input_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, images, net->x0, n,
                        1.0f / net->in_scale);
     GL_HIP(hipGetLastError());
-    const u16* x = net->x0;
-    for (int l = 0; l < 4; ++l) {
-        const gl_cnn::Layer& L = net->L[l];
-        const int M = bs * L.Ho * L.Wo, N = L.cout, K = L.kp;
-        LdIm2col la{x, L.H, L.W, L.cin, L.Ho, L.Wo, L.s, 0};
-        LdPlain lb{L.wt, L.kp, L.kp};
-        EpFwd ep{net->params + L.boff, L.Z, L.Y, N};
-        hipLaunchKernelGGL((gemm<true, true, LdIm2col, LdPlain, EpFwd>), dim3((M + 63) / 64, (N + 63) / 64, 1),
-                           dim3(256), 0, st, la, lb, ep, M, N, K, K);
+    {
+        const gl_cnn::Layer& L = net->L[0];
+        const int R = conv1_rows(L);
+        hipLaunchKernelGGL(conv1_fwd, dim3((L.H + R - 1) / R, bs), dim3(256), conv1_fwd_lds(L), st, net->x0, L.wt,
+                           L.kp, net->params + L.boff, L.H, L.W, L.cout, R, L.Z, L.Y);
         GL_HIP(hipGetLastError());
-        x = L.Y;
+    }
+    for (int l = 1; l < 4; ++l) {
+        const gl_cnn::Layer& L = net->L[l];
+        FwdArgs A{net->L[l - 1].Y, L.wt, net->params + L.boff, L.Z, L.Y, bs, L.H, L.W, L.cin, L.cout, L.Ho, L.Wo, L.rf};
+        const size_t lds = fwd_lds(L, L.rf);
+        const dim3 grid(persistent_grid(bs * ((L.Ho + L.rf - 1) / L.rf), lds, net->cus));
+        if (L.s == 1) hipLaunchKernelGGL(conv_fwd<1>, grid, dim3(512), lds, st, A);
+        else hipLaunchKernelGGL(conv_fwd<2>, grid, dim3(512), lds, st, A);
+        GL_HIP(hipGetLastError());
     }
     return GL_OK;
 }
@@ -246,36 +358,35 @@ extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* t
     }
     for (int l = 3; l >= 0; --l) {
         const gl_cnn::Layer& L = net->L[l];
-        const u16* x = l > 0 ? net->L[l - 1].Y : net->x0;
-        // weight (and bias) gradient: C[co][kk] = sum_m dZ[m][co] X~(m, kk)
-        {
-            int kc;
-            const int splits = cnn_wsplit(L, bs, &kc);
-            const int M = L.cout, N = 9 * L.cin + 1, K = bs * L.Ho * L.Wo;
-            if ((size_t)splits * M * N > net->part_elems) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: partial buffer");
-            LdPlain la{L.dZ, L.cout, L.cout};
-            LdIm2col lb{x, L.H, L.W, L.cin, L.Ho, L.Wo, L.s, 1};
-            EpPartial ep{net->part, M, N};
-            hipLaunchKernelGGL((gemm<false, false, LdPlain, LdIm2col, EpPartial>),
-                               dim3((M + 63) / 64, (N + 63) / 64, splits), dim3(256), 0, stream, la, lb, ep, M, N, K, kc);
-            GL_HIP(hipGetLastError());
-            hipLaunchKernelGGL(wgrad_reduce, dim3((M * N + kWrOut - 1) / kWrOut), dim3(256), 0, stream, net->part,
-                               splits, M, N,
-                               net->grad + L.woff, net->grad + L.boff);
-            GL_HIP(hipGetLastError());
+        const int ncol = 9 * L.cin + 1;
+        // weight (and bias) gradient: per-block sums over the pixels, then an ordered reduction
+        int splits;
+        if (l == 0) {
+            splits = bs;
+            hipLaunchKernelGGL(conv1_wgrad, dim3(bs), dim3(256), conv1_wgrad_lds(L), stream, net->x0, L.dZ, L.H, L.W,
+                               L.cout, net->part);
+        } else {
+            splits = std::min(L.wblk, bs);
+            WgradArgs A{L.dZ, net->L[l - 1].Y, net->part, bs, L.H, L.W, L.cin, L.cout, L.Ho, L.Wo, L.rw, L.wncg, L.wnp};
+            const dim3 grid(splits, L.wchunks);
+            const size_t lds = wgrad_lds(L, L.rw);
+            if (L.s == 1) hipLaunchKernelGGL(conv_wgrad<1>, grid, dim3(512), lds, stream, A);
+            else hipLaunchKernelGGL(conv_wgrad<2>, grid, dim3(512), lds, stream, A);
         }
+        GL_HIP(hipGetLastError());
+        if ((size_t)splits * L.cout * ncol > net->part_elems) return fail(GL_ERR_ARG, "gl_cnn_loss_grad: partial buffer");
+        hipLaunchKernelGGL(wgrad_reduce, dim3((L.cout * ncol + kWrOut - 1) / kWrOut), dim3(256), 0, stream, net->part,
+                           splits, L.cout, ncol, net->grad + L.woff, net->grad + L.boff);
+        GL_HIP(hipGetLastError());
         if (l == 0) break;
-        // data gradient into the previous layer: dZprev = (sum_k dZ~ W2T) * gelu'(Zprev)
-        {
-            const gl_cnn::Layer& Lp = net->L[l - 1];
-            const int M = bs * L.H * L.W, N = L.cin, K = 9 * L.cout;
-            LdDzGather la{L.dZ, L.H, L.W, L.Ho, L.Wo, L.cout, L.s};
-            LdPlain lb{L.w2t, 9 * L.cout, 9 * L.cout};
-            EpDataGrad ep{Lp.Z, Lp.dZ, N};
-            hipLaunchKernelGGL((gemm<true, true, LdDzGather, LdPlain, EpDataGrad>),
-                               dim3((M + 63) / 64, (N + 63) / 64, 1), dim3(256), 0, stream, la, lb, ep, M, N, K, K);
-            GL_HIP(hipGetLastError());
-        }
+        // data gradient into the previous layer: dZprev = conv^T(dZ, W) * gelu'(Zprev)
+        const gl_cnn::Layer& Lp = net->L[l - 1];
+        DgradArgs A{L.dZ, L.w2t, Lp.Z, Lp.dZ, bs, L.H, L.W, L.cin, L.cout, L.Ho, L.Wo, L.rd};
+        const size_t lds = dgrad_lds(L, L.rd);
+        const dim3 grid(persistent_grid(bs * ((L.H + L.rd - 1) / L.rd), lds, net->cus));
+        if (L.s == 1) hipLaunchKernelGGL(conv_dgrad<1>, grid, dim3(512), lds, stream, A);
+        else hipLaunchKernelGGL(conv_dgrad<2>, grid, dim3(512), lds, stream, A);
+        GL_HIP(hipGetLastError());
     }
     if (grad) GL_HIP(hipMemcpyAsync(grad, net->grad, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
     return GL_OK;

@@ -6,14 +6,10 @@
 //
 // Architecture: 4 x (3x3 conv, pad 1, bias, tanh-GELU), strides 1,2,1,2,
 // global average pool, Linear(C4 -> hidden) + GELU, Linear(hidden -> 2 d)
-// giving (mean, log-std).  Activations are NHWC bfloat16; every convolution
-// pass is one BF16 MFMA GEMM (v_mfma_f32_16x16x32_bf16, FP32 accumulation)
-// whose operand loaders gather straight from the NHWC tensors:
-//   forward      C[m][co]  = sum_kk X~(m, kk) W[co][kk]           (m = output pixel)
-//   data grad    C[mi][ci] = sum_k  dZ~(mi, k) W2T[ci][k]         (mi = input pixel)
-//   weight grad  C[co][kk] = sum_m  dZ[m][co] X~(m, kk)           (split over m)
-// with kk = (kh*3 + kw)*Cin + ci and k = (kh*3 + kw)*Cout + co.  The head,
-// the Gaussian NLL and AdamW run in FP32 on the vector ALUs.
+// giving (mean, log-std).  Activations are NHWC bfloat16.  The convolution
+// passes (cnn_conv.hip) run on BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32
+// accumulation); this file holds the bf16 helpers, the input transform, the
+// head, the Gaussian NLL, the reductions and AdamW (FP32 on the vector ALUs).
 //
 // Rounding points (the CPU oracle applies the same ones): the input
 // asinh(img / scale), the weights, Z = conv + bias and Y = gelu(Z) are
@@ -48,181 +44,6 @@ __device__ inline float gelu_grad(float x) {
     const float u = kGeluC * (x + kGeluA * x * x * x);
     const float t = tanhf(u);
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
-}
-
-// ---------------------------------------------------------------------------
-// Operand loaders: load8(major, minor8) returns the 8 consecutive minor-index
-// elements at one major index (zeros outside the operand).
-// ---------------------------------------------------------------------------
-struct LdPlain {  // element (major, minor) = p[major * ld + minor], minor < cols (cols % 8 == 0)
-    const u16* p;
-    int ld, cols;
-    __device__ v8u load8(int major, int minor8) const {
-        if (minor8 + 8 <= cols) return *reinterpret_cast<const v8u*>(p + (size_t)major * ld + minor8);
-        return v8u{0, 0, 0, 0, 0, 0, 0, 0};
-    }
-};
-
-// im2col view of an NHWC tensor: (m = (b, ho, wo), kk = (kh*3+kw)*cin + ci);
-// column kk == 9 cin reads 1 when `ones` (bias gradient column).
-struct LdIm2col {
-    const u16* x;
-    int H, W, cin, Ho, Wo, s, ones;
-    __device__ v8u load8(int m, int kk8) const {
-        v8u v = {0, 0, 0, 0, 0, 0, 0, 0};
-        const int hw = Ho * Wo;
-        const int b = m / hw, r = m - b * hw;
-        const int ho = r / Wo, wo = r - ho * Wo;
-        const int kv = 9 * cin;
-        if ((cin & 7) == 0) {
-            if (kk8 < kv) {
-                const int khw = kk8 / cin, ci = kk8 - khw * cin;
-                const int kh = khw / 3, kw = khw - 3 * kh;
-                const int hi = ho * s + kh - 1, wi = wo * s + kw - 1;
-                if (hi >= 0 && hi < H && wi >= 0 && wi < W)
-                    v = *reinterpret_cast<const v8u*>(x + ((size_t)(b * H + hi) * W + wi) * cin + ci);
-            } else if (ones && kk8 == kv) {
-                v[0] = 0x3f80;
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int kk = kk8 + e;
-                if (kk < kv) {
-                    const int khw = kk / cin, ci = kk - khw * cin;
-                    const int kh = khw / 3, kw = khw - 3 * kh;
-                    const int hi = ho * s + kh - 1, wi = wo * s + kw - 1;
-                    if (hi >= 0 && hi < H && wi >= 0 && wi < W) v[e] = x[((size_t)(b * H + hi) * W + wi) * cin + ci];
-                } else if (ones && kk == kv) {
-                    v[e] = 0x3f80;
-                }
-            }
-        }
-        return v;
-    }
-};
-
-// Transposed-convolution view of dZ (NHWC over Ho x Wo x cout) seen from the
-// input pixel mi = (b, hi, wi): k = (kh*3+kw)*cout + co reads dZ at
-// ho = (hi + 1 - kh) / s, wo = (wi + 1 - kw) / s when those divide exactly.
-struct LdDzGather {
-    const u16* dz;
-    int H, W, Ho, Wo, cout, s;
-    __device__ v8u load8(int mi, int k8) const {
-        const int hw = H * W;
-        const int b = mi / hw, r = mi - b * hw;
-        const int hi = r / W, wi = r - hi * W;
-        const int khw = k8 / cout, co = k8 - khw * cout;
-        const int kh = khw / 3, kw = khw - 3 * kh;
-        const int hn = hi + 1 - kh, wn = wi + 1 - kw;
-        if (khw >= 9 || hn < 0 || wn < 0 || hn % s || wn % s) return v8u{0, 0, 0, 0, 0, 0, 0, 0};
-        const int ho = hn / s, wo = wn / s;
-        if (ho >= Ho || wo >= Wo) return v8u{0, 0, 0, 0, 0, 0, 0, 0};
-        return *reinterpret_cast<const v8u*>(dz + ((size_t)(b * Ho + ho) * Wo + wo) * cout + co);
-    }
-};
-
-// ---------------------------------------------------------------------------
-// Epilogues
-// ---------------------------------------------------------------------------
-struct EpFwd {  // Z = bf16(acc + bias), Y = bf16(gelu(Z)), row-major [M][N]
-    const float* bias;
-    u16* Z;
-    u16* Y;
-    int N;
-    __device__ void operator()(int m, int n, float acc, int) const {
-        const u16 zb = f2bf(acc + bias[n]);
-        Z[(size_t)m * N + n] = zb;
-        Y[(size_t)m * N + n] = f2bf(gelu(bf2f(zb)));
-    }
-};
-struct EpDataGrad {  // dZprev = bf16(acc * gelu'(Zprev))
-    const u16* Zp;
-    u16* dZp;
-    int N;
-    __device__ void operator()(int m, int n, float acc, int) const {
-        const size_t i = (size_t)m * N + n;
-        dZp[i] = f2bf(acc * gelu_grad(bf2f(Zp[i])));
-    }
-};
-struct EpPartial {  // split-K partial sums part[split][M][N]
-    float* part;
-    int M, N;
-    __device__ void operator()(int m, int n, float acc, int split) const {
-        part[((size_t)split * M + m) * N + n] = acc;
-    }
-};
-
-// ---------------------------------------------------------------------------
-// GEMM: C[M][N] = sum_{k in split} A(m, k) B(k, n), 64x64x32 tiles, 4 waves of
-// 32x32 (2x2 MFMA 16x16x32 tiles).  Operands are staged in LDS as [row][k]
-// (row = m for A, n for B), padded to 40 elements.  KMAJ: the loader's 8-vector
-// runs along k (major = row); otherwise along rows (major = k).
-// MFMA lane layout: lane l feeds row (l % 16) with the 8 k-values 8(l/16)..+7
-// for both operands (the same k order on both sides, so the sum covers the 32
-// k-values of the step once); the 16x16 result holds (4(l/16) + r, l % 16).
-// ---------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 32, GLS = GBK + 8;
-
-template <bool KMAJ, class L>
-__device__ __forceinline__ void stage(const L& ld, u16* S, int r0, int k0, int rows, int kend) {
-    const int t = threadIdx.x;
-    if (KMAJ) {
-        const int row = t >> 2, k8 = (t & 3) * 8;
-        v8u v = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (r0 + row < rows && k0 + k8 < kend) v = ld.load8(r0 + row, k0 + k8);
-        *reinterpret_cast<v8u*>(S + row * GLS + k8) = v;
-    } else {
-        const int kk = t >> 3, r8 = (t & 7) * 8;
-        v8u v = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (k0 + kk < kend) v = ld.load8(k0 + kk, r0 + r8);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) S[(r8 + e) * GLS + kk] = v[e];
-    }
-}
-
-template <bool AK, bool BK, class LA, class LB, class EP>
-__global__ __launch_bounds__(256) void gemm(LA la, LB lb, EP ep, int M, int N, int K, int kchunk) {
-    __shared__ __attribute__((aligned(16))) u16 As[GBM * GLS];
-    __shared__ __attribute__((aligned(16))) u16 Bs[GBN * GLS];
-    const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
-    const int kbeg = blockIdx.z * kchunk;
-    const int kend = K < kbeg + kchunk ? K : kbeg + kchunk;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    v4f acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = kbeg; k0 < kend; k0 += GBK) {
-        stage<AK>(la, As, m0, k0, M, kend);
-        stage<BK>(lb, Bs, n0, k0, N, kend);
-        __syncthreads();
-        v8bf a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const v8u ua = *reinterpret_cast<const v8u*>(As + (wm * 32 + i * 16 + (lane & 15)) * GLS + 8 * (lane >> 4));
-            const v8u ub = *reinterpret_cast<const v8u*>(Bs + (wn * 32 + i * 16 + (lane & 15)) * GLS + 8 * (lane >> 4));
-            a[i] = __builtin_bit_cast(v8bf, ua);
-            b[i] = __builtin_bit_cast(v8bf, ub);
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
-                if (m < M && n < N) ep(m, n, acc[i][j][r], blockIdx.z);
-            }
 }
 
 // ---------------------------------------------------------------------------
