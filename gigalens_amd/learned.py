@@ -304,66 +304,82 @@ def screen(log_prob_fn, z: torch.Tensor, lens: torch.Tensor, n_lenses: int, per_
     return z[idx.reshape(-1)].contiguous()
 
 
-def compare_initialisation(init: "LearnedInitializer", n_lenses: int = 256, n_particles: int = 1000,
-                           num_steps: int = 350, lr: float = 5e-3, prior_fraction: float = 0.25,
-                           oversample: int = 8, seed0: int = 1000, chi2_threshold: float = 1.1):
+def compare_initialisation(init, n_lenses: int = 256, n_particles: int = 1000, num_steps: int = 350,
+                           lr: float = 5e-3, prior_fraction: float = 0.25, oversample: int = 8, seed0: int = 1000,
+                           chi2_threshold: float = 1.1, measure_lenses: int = 16):
     """Held-out evaluation of learned vs prior-sampled MAP starts.
 
-    Lens k is a prior draw simulated with fresh noise (seed seed0 + k).  Three
-    start strategies, each fitting all lenses in one multi-lens MAP
-    (``n_particles`` starts per lens, Keras Adam ``lr``):
+    ``init`` is a ``LearnedInitializer`` or a dict name -> initializer (several
+    trained networks against the same prior runs).  Lens k is a prior draw
+    simulated with fresh noise (seed seed0 + k).  Start strategies, each fitting all
+    lenses in one multi-lens MAP (``n_particles`` starts per lens, Keras Adam ``lr``):
 
       * ``prior``          -- prior draws (the reference's MAP, tf/inference.py:21-24)
       * ``learned``        -- ``oversample * n_particles`` candidates, (1 - prior_fraction)
                               from the network's Gaussian and the rest prior draws,
-                              screened by the log-posterior (keep the best n_particles)
+                              screened by the log-posterior (keep the best n_particles);
+                              ``learned:<name>`` for each network of a dict
       * ``prior_screened`` -- the same screening applied to prior draws only
                               (what the screening alone buys)
 
     Per lens: steps until the best particle reaches chi2 <= ``chi2_threshold``
     and until it comes within 1 nat of the ``prior`` run's final best
-    log-posterior.  Wall times are steps times the measured time of one
-    single-lens MAP step of ``n_particles``, plus the measured per-lens
-    start-up cost (network inference, screening evaluation)."""
+    log-posterior; seconds = steps times the measured time of one single-lens MAP
+    step of ``n_particles``, plus the measured per-lens start-up cost (network
+    inference, screening evaluation).  ``measured_single_lens``: the first
+    ``measure_lenses`` lenses fitted one at a time per strategy -- start-up, then
+    ``num_steps`` fused MAP steps -- each run timed, and each lens's time to the
+    chi2 threshold read from its own run (start-up + its steps at that run's
+    measured step rate)."""
     from .batch import MultiLensMAP
     from .inference import Adam, ModellingSequence
     from .model import ForwardProbModel
-    dev = init.device
-    d = init.d
+    inits = dict(init) if isinstance(init, dict) else {"": init}
+    first = next(iter(inits.values()))
+    dev = first.device
+    d = first.d
     imgs = []
     for k in range(n_lenses):
-        im, _ = init.simulate_batch(1, seed=seed0 + k)
+        im, _ = first.simulate_batch(1, seed=seed0 + k)
         imgs.append(im[0])
     images = torch.stack(imgs).contiguous()
-    mlm = MultiLensMAP(init.phys_model, init.sim_config, init.prior, images.cpu().numpy(), init.background_rms,
-                       init.exp_time, device=dev)
+    mlm = MultiLensMAP(first.phys_model, first.sim_config, first.prior, images.cpu().numpy(), first.background_rms,
+                       first.exp_time, device=dev)
     lens = torch.arange(n_lenses, device=dev, dtype=torch.int32).repeat_interleave(n_particles)
     K = oversample * n_particles
     lens_c = torch.arange(n_lenses, device=dev, dtype=torch.int32).repeat_interleave(K)
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed0 + 7)
     n_prior = int(round(prior_fraction * K))
     n_net = K - n_prior
-    # learned candidates: network Gaussian + prior fallback, then screening
+
+    def learned_candidates(net, imgs_, n_l, gen, fb_seed):
+        mean, std = net.predict(imgs_)
+        z_net = (mean[:, None, :] + std[:, None, :] * torch.randn((n_l, n_net, d), device=dev, generator=gen))
+        z_net[:, 0] = mean  # the network's point estimate is one of the candidates
+        z_fb = net.bij.inverse(net.prior.sample(n_l * n_prior, seed=fb_seed)).to(dev, torch.float32)
+        return torch.cat([z_net, z_fb.reshape(n_l, n_prior, d)], 1).reshape(-1, d).contiguous()
+
+    starts, setup = {}, {}
+    for name, net in inits.items():
+        key = "learned" if name == "" else f"learned:{name}"
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed0 + 7)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        z_c = learned_candidates(net, images, n_lenses, g, seed0 + 11)
+        starts[key] = screen(mlm.log_prob, z_c, lens_c, n_lenses, K, n_particles)
+        torch.cuda.synchronize()
+        setup[key] = (time.perf_counter() - t0) / n_lenses
+    z_pc = first.bij.inverse(first.prior.sample(n_lenses * K, seed=seed0 + 17)).to(dev, torch.float32).contiguous()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    mean, std = init.predict(images)
-    z_net = (mean[:, None, :] + std[:, None, :] * torch.randn((n_lenses, n_net, d), device=dev, generator=g))
-    z_net[:, 0] = mean  # the network's point estimate is one of the candidates
-    z_fb = init.bij.inverse(init.prior.sample(n_lenses * n_prior, seed=seed0 + 11)).to(dev, torch.float32)
-    z_c = torch.cat([z_net, z_fb.reshape(n_lenses, n_prior, d)], 1).reshape(-1, d).contiguous()
-    z_learned = screen(mlm.log_prob, z_c, lens_c, n_lenses, K, n_particles)
+    starts["prior_screened"] = screen(mlm.log_prob, z_pc, lens_c, n_lenses, K, n_particles)
     torch.cuda.synchronize()
-    t_learned_setup = (time.perf_counter() - t0) / n_lenses
-    z_pc = init.bij.inverse(init.prior.sample(n_lenses * K, seed=seed0 + 17)).to(dev, torch.float32).contiguous()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    z_pscreen = screen(mlm.log_prob, z_pc, lens_c, n_lenses, K, n_particles)
-    torch.cuda.synchronize()
-    t_screen_setup = (time.perf_counter() - t0) / n_lenses
-    z_prior = init.bij.inverse(init.prior.sample(n_lenses * n_particles, seed=seed0 + 13)).to(dev, torch.float32)
+    setup["prior_screened"] = (time.perf_counter() - t0) / n_lenses
+    starts["prior"] = first.bij.inverse(first.prior.sample(n_lenses * n_particles, seed=seed0 + 13)).to(
+        dev, torch.float32)
+    setup["prior"] = 0.0
     runs = {}
-    for name, z0 in (("learned", z_learned), ("prior", z_prior), ("prior_screened", z_pscreen)):
+    for name, z0 in starts.items():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         mlm.run(z0, lens, num_steps, Adam(lr))
@@ -372,15 +388,14 @@ def compare_initialisation(init: "LearnedInitializer", n_lenses: int = 256, n_pa
         best_lp, best_chi = mlm.per_lens_best(lens)
         runs[name] = {"best_lp": best_lp, "best_chi": best_chi, "batched_seconds": wall}
     # one single-lens MAP step of n_particles (the per-lens cost of each step)
-    pm = ForwardProbModel(init.prior, images[0].cpu().numpy(), init.background_rms, init.exp_time)
-    fitter = ModellingSequence(init.phys_model, pm, init.sim_config, device=dev)
-    fitter.MAP(optimizer=Adam(lr), start=z_prior[:n_particles], num_steps=5)
+    pm = ForwardProbModel(first.prior, images[0].cpu().numpy(), first.background_rms, first.exp_time)
+    fitter = ModellingSequence(first.phys_model, pm, first.sim_config, device=dev)
+    fitter.MAP(optimizer=Adam(lr), start=starts["prior"][:n_particles], num_steps=5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    fitter.MAP(optimizer=Adam(lr), start=z_prior[:n_particles], num_steps=50)
+    fitter.MAP(optimizer=Adam(lr), start=starts["prior"][:n_particles], num_steps=50)
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t0) / 50
-    setup = {"learned": t_learned_setup, "prior": 0.0, "prior_screened": t_screen_setup}
     target = runs["prior"]["best_lp"][-1] - 1.0
 
     def stats(s, extra):
@@ -405,11 +420,63 @@ def compare_initialisation(init: "LearnedInitializer", n_lenses: int = 256, n_pa
         res["time_to_prior_best_lp_within_1nat"][name] = stats(s_lp, setup[name])
         res["final_best_lp_gain_vs_prior_p50"][name] = float(np.median(r["best_lp"][-1] - runs["prior"]["best_lp"][-1]))
         res["batched_seconds"][name] = r["batched_seconds"]
-    for name in ("learned", "prior_screened"):
+    for name in runs:
+        if name == "prior":
+            continue
         both = (steps_lp[name] > 0) & (steps_lp["prior"] > 0)
         ratio = (steps_lp[name][both] * t_step + setup[name]) / (steps_lp["prior"][both] * t_step)
         res["time_to_prior_best_lp_within_1nat"][name]["time_ratio_vs_prior_p50"] = \
             float(np.median(ratio)) if both.any() else None
+    # single-lens runs, each timed
+    m = min(measure_lenses, n_lenses)
+    if m > 0:
+        meas = {name: [] for name in runs}
+        for k in range(m):
+            pm_k = ForwardProbModel(first.prior, images[k].cpu().numpy(), first.background_rms, first.exp_time)
+            fit_k = ModellingSequence(first.phys_model, pm_k, first.sim_config, device=dev)
+            sim_k = fit_k._sim(K)
+            for name in runs:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if name.startswith("learned"):
+                    net = inits["" if name == "learned" else name.split(":", 1)[1]]
+                    g = torch.Generator(device=dev)
+                    g.manual_seed(seed0 + 7 + 101 * k)
+                    z_c = learned_candidates(net, images[k:k + 1], 1, g, seed0 + 11 + 101 * k)
+                elif name == "prior_screened":
+                    z_c = first.bij.inverse(first.prior.sample(K, seed=seed0 + 17 + 101 * k)).to(
+                        dev, torch.float32).contiguous()
+                else:
+                    z_c = None
+                if z_c is not None:
+                    lp_c, _ = pm_k.log_prob(sim_k, z_c)
+                    z0 = z_c[torch.topk(torch.nan_to_num(lp_c, nan=-float("inf")), n_particles).indices]
+                else:
+                    z0 = first.bij.inverse(first.prior.sample(n_particles, seed=seed0 + 13 + 101 * k)).to(
+                        dev, torch.float32)
+                torch.cuda.synchronize()
+                t_setup = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                fit_k.MAP(optimizer=Adam(lr), start=z0.contiguous(), num_steps=num_steps)
+                torch.cuda.synchronize()
+                t_map = time.perf_counter() - t0
+                chi = torch.nan_to_num(fit_k.chi2_history, nan=float("inf")).min(1).values.cpu().numpy()
+                hit = np.nonzero(chi <= chi2_threshold)[0]
+                meas[name].append(t_setup + (hit[0] + 1) * t_map / num_steps if len(hit) else np.nan)
+        res["measured_single_lens"] = {"lenses": m, "chi2_threshold": chi2_threshold,
+                                       "note": "each lens fitted alone per strategy; seconds to its best chi2 <= "
+                                               "threshold, start-up and MAP timed on that lens's run"}
+        for name, v in meas.items():
+            v = np.array(v)
+            ok = np.isfinite(v)
+            row = {"reached": int(ok.sum())}
+            if ok.any():
+                row.update(seconds_p50=float(np.median(v[ok])), seconds_p95=float(np.percentile(v[ok], 95)))
+            if name != "prior":
+                p = np.array(meas["prior"])
+                both = ok & np.isfinite(p)
+                row["time_ratio_vs_prior_p50"] = float(np.median(v[both] / p[both])) if both.any() else None
+            res["measured_single_lens"][name] = row
     return res
 
 
