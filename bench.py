@@ -435,18 +435,26 @@ def run_cnn(args, dev):
 
 
 def oracle_error_uneven(prior, phys, lens_set, batch, z, per_lens=4):
-    """The batched log_prob and gradient (after MAP) against the FP64 oracle for the
-    first ``per_lens`` particles of four lenses: the smallest and the largest image,
-    the first lens on the resident-image (fused) path and the first on the staged
-    path."""
+    """The batched log_prob and gradient against the FP64 oracle for ``per_lens``
+    prior draws (seed 4242, in place of the first particles of z) of four lenses:
+    the smallest and the largest image, the first lens on the resident-image (fused)
+    path and the first on the staged path.  Prior draws rather than fitted particles:
+    at a converged MAP the gradient is near zero and the posterior curvature large,
+    where finite differences of the oracle lose their accuracy."""
     from oracle import bridge as ob
-    lp, chi2, g = batch.log_prob(z, want_grad=True)
+    z = z.clone()
     npix = [c.num_pix * c.supersample for c, _, _ in lens_set]
     sel = [int(np.argmin(npix)), int(np.argmax(npix))]
     for want in (True, False):
         k = next((i for i, f in enumerate(batch.fused) if f == want), None)
         if k is not None and k not in sel:
             sel.append(k)
+    for k in sel:
+        a, m = int(batch.first[k]), min(per_lens, lens_set[k][2])
+        xk = flat_sample(prior, m, seed=4242 + k)
+        zk = batch.bij.inverse(tfd.pack_as(prior, [xk[:, j] for j in range(xk.shape[1])]))
+        z[a:a + m] = torch.as_tensor(zk, dtype=torch.float32).to(z.device)
+    lp, chi2, g = batch.log_prob(z.contiguous(), want_grad=True)
     rows = []
     for k in sel:
         cfg, obs, cnt = lens_set[k]

@@ -55,6 +55,7 @@ EXPORTED_SYMBOLS = (
     "gl_hmc_end",
     "gl_hmc_adapt",
     "gl_noise_poisson_gauss",
+    "gl_leapfrog_step",
     "gl_cnn_create",
     "gl_cnn_destroy",
     "gl_cnn_num_params",
@@ -178,6 +179,7 @@ _SIGS = {
                     _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
     "gl_hmc_adapt": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                       ctypes.c_float, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
+    "gl_leapfrog_step": ([_vp, ctypes.c_int, _vp, ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "gl_noise_poisson_gauss": ([_vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_float, ctypes.c_ulonglong, _vp, _vp],
                                ctypes.c_int),
     "gl_cnn_create": ([ctypes.POINTER(CnnDesc), ctypes.POINTER(_vp)], ctypes.c_int),

@@ -246,6 +246,26 @@ __global__ __launch_bounds__(256) void adapt(int n, int d, int* si, float* sf, c
     }
 }
 
+// p += h g, element-wise over n d
+__global__ void kick(long nd, float h, const float* g, float* p) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nd) p[t] = fmaf(h, g[t], p[t]);
+}
+
+// z += eps Sigma p (Sigma = identity when cov is null); one thread per (chain, dimension)
+__global__ void drift(int n, int d, const float* cov, float eps, const float* p, float* z) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n * d) return;
+    const int c = (int)(t / d), i = (int)(t - (long)c * d);
+    float v = 0.0f;
+    if (cov) {
+        for (int j = 0; j < d; ++j) v = fmaf(p[(size_t)c * d + j], cov[(size_t)j * d + i], v);
+    } else {
+        v = p[t];
+    }
+    z[t] = fmaf(eps, v, z[t]);
+}
+
 }  // namespace hmc
 }  // namespace gl
 
@@ -297,6 +317,25 @@ extern "C" int gl_hmc_adapt(int n, int d, int* si, float* sf, const float* z0, c
     if (d > 128) return fail(GL_ERR_ARG, "gl_hmc_adapt: d <= 128");
     hipLaunchKernelGGL(gl::hmc::adapt, dim3(1), dim3(256), 0, stream, n, d, si, sf, z0, zp, vel, acc, adapt, n_adapt,
                        target, max_leapfrog, eps_hist, acc_hist, l_hist, t_hist, hist_len);
+    GL_HIP(hipGetLastError());
+    return GL_OK;
+}
+
+extern "C" int gl_leapfrog_step(const gl_plan* plan, int n, const float* cov, float eps, float* z, float* p,
+                                float* lp, float* chi2, float* grad_z, hipStream_t stream) {
+    if (!plan || !z || !p || !lp || !chi2 || !grad_z) return fail(GL_ERR_ARG, "gl_leapfrog_step: null argument");
+    gl_plan_info info;
+    int rc = gl_plan_query(plan, &info);
+    if (rc) return rc;
+    const int d = info.n_params;
+    if (n <= 0 || d <= 0) return fail(GL_ERR_ARG, "gl_leapfrog_step: bad sizes");
+    const long nd = (long)n * d;
+    hipLaunchKernelGGL(gl::hmc::kick, dim3(blocks(nd)), dim3(kT), 0, stream, nd, 0.5f * eps, grad_z, p);
+    GL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gl::hmc::drift, dim3(blocks(nd)), dim3(kT), 0, stream, n, d, cov, eps, p, z);
+    GL_HIP(hipGetLastError());
+    if ((rc = gl_log_prob(plan, z, n, lp, chi2, nullptr, nullptr, grad_z, stream))) return rc;
+    hipLaunchKernelGGL(gl::hmc::kick, dim3(blocks(nd)), dim3(kT), 0, stream, nd, 0.5f * eps, grad_z, p);
     GL_HIP(hipGetLastError());
     return GL_OK;
 }

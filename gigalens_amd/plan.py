@@ -275,6 +275,26 @@ class Plan:
                                                    _ptr(g_chi2), _ptr(grad), _stream()), "gl_log_prob_lenses")
         return lp, chi2, grad
 
+    def leapfrog_step(self, z, p, grad, eps: float, cov=None, lp=None, chi2=None):
+        """One leapfrog step of preconditioned HMC in place (gl_leapfrog_step): p +=
+        eps/2 grad, z += eps cov p, grad = dlp/dz at the new z, p += eps/2 grad.
+        ``grad`` must hold dlp/dz at z on entry; ``cov`` (d x d) the inverse mass
+        matrix (None: identity).  Returns (lp, chi2) at the new z."""
+        z = _dev(z, "z")
+        bs = z.shape[0]
+        if z.dim() != 2 or z.shape[1] != self.d:
+            raise ValueError(f"z must be (bs, {self.d})")
+        p = _devn(p, "p", z.numel())
+        grad = _devn(grad, "grad", z.numel())
+        if p is None or grad is None:
+            raise ValueError("p and grad are required")
+        cov = _devn(cov, "cov", self.d * self.d)
+        lp = _devn(lp, "lp", bs) if lp is not None else torch.empty(bs, device=z.device, dtype=torch.float32)
+        chi2 = _devn(chi2, "chi2", bs) if chi2 is not None else torch.empty(bs, device=z.device, dtype=torch.float32)
+        nat.check(self._lib.gl_leapfrog_step(self._h, bs, _ptr(cov), float(eps), _ptr(z), _ptr(p), _ptr(lp),
+                                             _ptr(chi2), _ptr(grad), _stream()), "gl_leapfrog_step")
+        return lp, chi2
+
     def lstsq_simulate(self, x: torch.Tensor, want_image=True, want_coeffs=False, want_stacked=False):
         """Least-squares simulation (tf/simulator.py:99-165): (image, coeffs, stacked)."""
         x = _dev(x, "x")

@@ -6,7 +6,7 @@
  * (row-major float32 unless stated), every call is stream-ordered and
  * asynchronous.  Only the create / destroy calls allocate or free device memory
  * and synchronise the host: gl_plan_create, gl_plan_destroy, gl_batch_create,
- * gl_batch_destroy, gl_cnn_create, gl_cnn_destroy.  gl_cnn_set_params clears the
+ * gl_batch_create_ws, gl_batch_destroy, gl_cnn_create, gl_cnn_destroy.  gl_cnn_set_params clears the
  * AdamW moments with hipMemsetAsync and must not be recorded into a HIP graph.
  * Errors are reported as a non-zero return code; gl_last_error() returns a
  * thread-local message.
@@ -318,6 +318,15 @@ int gl_hmc_end(int n, int d, const float* cov, unsigned long long seed, const in
 int gl_hmc_adapt(int n, int d, int* si, float* sf, const float* z0, const float* zp, const float* vel,
                  const float* acc, int adapt, int n_adapt, float target, int max_leapfrog, float* eps_hist,
                  float* acc_hist, int* l_hist, float* t_hist, int hist_len, hipStream_t stream);
+
+/* One leapfrog step of preconditioned HMC on a plan's log posterior (the integrator
+ * of tfp PreconditionedHamiltonianMonteCarlo as used by tf/inference.py:109-117):
+ * p += eps/2 grad; z += eps cov p; (lp, chi2, grad) at the new z; p += eps/2 grad.
+ * z, p, grad_z (n x d, d = the plan's parameters): grad_z holds d lp / dz at z on
+ * entry and at the new z on return.  cov (d x d) is the mass matrix's inverse
+ * (kinetic energy p^T cov p / 2); NULL: identity. */
+int gl_leapfrog_step(const gl_plan* plan, int n, const float* cov, float eps, float* z, float* p, float* lp,
+                     float* chi2, float* grad_z, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Observation noise on the device (training images of the learned initialiser):

@@ -136,3 +136,49 @@ def test_graph_equals_eager(cuda):
     assert torch.equal(se, sg)
     for k in ("accept_rate", "step_size", "leapfrog_steps", "trajectory_length"):
         assert np.array_equal(ste[k], stg[k])
+
+
+def test_leapfrog_step_composition_and_reversibility(cuda):
+    """gl_leapfrog_step on the 60x60 EPL+shear posterior: equal to the composition of
+    its parts through the plan's own log_prob (kick, drift with the mass matrix,
+    gradient, kick) and time-reversible: ten steps, momentum negated, ten steps back
+    return to the start."""
+    from tests import common as C
+    from gigalens_amd import ForwardProbModel, LensSimulator
+    prob = C.epl_shear_problem(bs=8)
+    sim = LensSimulator(prob["phys"], prob["cfg"], bs=8)
+    pm = ForwardProbModel(prob["prior"], prob["obs"], 0.2, 100.0)
+    plan = pm.plan_for(sim)
+    xt = torch.as_tensor(prob["truth_x"], dtype=torch.float32)
+    x = xt.repeat(8, 1) * (1 + 0.01 * torch.randn(8, xt.shape[1], generator=torch.Generator().manual_seed(2)))
+    z = pm.bij.inverse(C.nested_from_x(prob["prior"], x)).to(cuda).contiguous()
+    d = z.shape[1]
+    g = torch.Generator(device="cuda").manual_seed(4)
+    # inverse mass ~ 1e-6 (the posterior's stiffest directions have curvature ~1e7, so
+    # eps sqrt(curvature cov) stays well inside the leapfrog's stability range 2)
+    A = torch.randn((d, d), device="cuda", generator=g) * 1e-4
+    cov = (A @ A.T + 1e-6 * torch.eye(d, device="cuda")).contiguous()
+    p = torch.randn((8, d), device="cuda", generator=g) * 1e3
+    _, _, grad = plan.log_prob(z, want_grad=True)
+    eps = 0.05
+    # composition through the plan's log_prob
+    p1 = p + 0.5 * eps * grad
+    z1 = z + eps * (p1 @ cov)
+    lp_ref, _, g1 = plan.log_prob(z1.contiguous(), want_grad=True)
+    p2 = p1 + 0.5 * eps * g1
+    zl, pl, gl = z.clone(), p.clone(), grad.clone()
+    lp, _ = plan.leapfrog_step(zl, pl, gl, eps, cov=cov)
+    assert torch.allclose(zl, z1, rtol=1e-6, atol=1e-6)
+    assert torch.allclose(lp, lp_ref, rtol=1e-5)
+    assert torch.allclose(pl, p2, rtol=1e-4, atol=1e-3)
+    # reversibility
+    zr, pr, gr = z.clone(), p.clone(), grad.clone()
+    for _ in range(10):
+        plan.leapfrog_step(zr, pr, gr, eps, cov=cov)
+    pr.neg_()
+    for _ in range(10):
+        plan.leapfrog_step(zr, pr, gr, eps, cov=cov)
+    err = float((zr - z).abs().max() / z.abs().max())
+    print(f"leapfrog 10 + 10 reversed: max |z - z0| / max |z0| = {err:.2e}", flush=True)
+    assert err < 1e-4
+    assert torch.allclose(-pr, p, rtol=1e-3, atol=1e-2 * float(p.abs().max()))
