@@ -40,6 +40,8 @@ EXPORTED_SYMBOLS = (
     "gl_map_adam_step_noalloc",
     "gl_batch_create",
     "gl_batch_create_ws",
+    "gl_batch_create_ex",
+    "gl_batch_query",
     "gl_batch_workspace_bytes",
     "gl_batch_num_chunks",
     "gl_batch_destroy",
@@ -123,6 +125,21 @@ class PlanInfo(ctypes.Structure):
                                             "sep_rank", "lds_bytes", "block", "adj_rank")]
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [
+        ("workspace_bytes", ctypes.c_size_t),
+        ("band", ctypes.c_int),
+        ("band_rows", ctypes.c_int),
+        ("band_lds_bytes", ctypes.c_int),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("total", "lenses_fused", "lenses_staged", "lenses_band", "chunks",
+                                            "band_grid", "band_lds_bytes", "band_rows_min", "band_rows_max")] + [
+        ("band_scratch_bytes", ctypes.c_size_t), ("workspace_bytes", ctypes.c_size_t)]
+
+
 class CnnDesc(ctypes.Structure):
     _fields_ = [
         ("height", ctypes.c_int),
@@ -158,6 +175,9 @@ _SIGS = {
                         ctypes.c_int),
     "gl_batch_create_ws": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_size_t,
                             ctypes.POINTER(_vp)], ctypes.c_int),
+    "gl_batch_create_ex": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(BatchOpts), ctypes.POINTER(_vp)], ctypes.c_int),
+    "gl_batch_query": ([_vp, ctypes.POINTER(BatchInfo)], ctypes.c_int),
     "gl_batch_workspace_bytes": ([_vp], ctypes.c_size_t),
     "gl_batch_num_chunks": ([_vp], ctypes.c_int),
     "gl_batch_destroy": ([_vp], ctypes.c_int),

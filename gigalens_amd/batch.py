@@ -106,13 +106,17 @@ class LensBatch:
     """Lens systems with different cameras and particle counts, fitted together.
     ``lenses`` is a sequence of (sim_config, observed_image, n_particles); all
     share the physical model, the prior and the noise model (``ForwardProbModel``,
-    tf/model.py:53-86).  Particles are lens-major.  Lenses whose image fits the
-    resident-image kernel run there (one fused launch each); the others run the
-    HBM-staged pipeline in chunks whose workspaces share an arena of at most
-    ``workspace_gib``."""
+    tf/model.py:53-86).  Particles are lens-major.  ``band`` 1 puts every lens the
+    banded persistent kernel takes into one launch per call (2: only the lenses the
+    resident-image kernel cannot hold; 0: none), with at most ``band_rows`` sub-rows
+    per band and ``band_lds_bytes`` of LDS per workgroup (0: defaults).  Other lenses
+    whose image fits the resident-image kernel run there (one fused launch each); the
+    rest run the HBM-staged pipeline in chunks whose workspaces share an arena of at
+    most ``workspace_gib``.  ``info`` reports where the lenses went."""
 
     def __init__(self, phys_model, prior, lenses: Sequence, background_rms: float, exp_time: float, device="cuda",
-                 workspace_gib: float = 32.0, fused: bool = True):
+                 workspace_gib: float = 32.0, fused: bool = True, band: int = 0, band_rows: int = 0,
+                 band_lds_bytes: int = 0):
         self.device = torch.device(device)
         self.prior = prior
         self.bij = dist.ChainBijector(prior.experimental_default_event_space_bijector(), dist.PackBijector(prior))
@@ -132,11 +136,16 @@ class LensBatch:
         counts = (ctypes.c_int * self.n_lenses)(*self.counts)
         h = ctypes.c_void_p()
         lib = nat.lib()
-        nat.check(lib.gl_batch_create_ws(handles, self.n_lenses, counts, int(workspace_gib * (1 << 30)),
-                                         ctypes.byref(h)), "gl_batch_create_ws")
+        opts = nat.BatchOpts(workspace_bytes=int(workspace_gib * (1 << 30)), band=int(band), band_rows=int(band_rows),
+                             band_lds_bytes=int(band_lds_bytes))
+        nat.check(lib.gl_batch_create_ex(handles, self.n_lenses, counts, ctypes.byref(opts), ctypes.byref(h)),
+                  "gl_batch_create_ex")
         self._h = h
         self._lib = lib
         self.total = int(lib.gl_batch_size(h))
+        info = nat.BatchInfo()
+        nat.check(lib.gl_batch_query(h, ctypes.byref(info)), "gl_batch_query")
+        self.info = {name: getattr(info, name) for name, _ in nat.BatchInfo._fields_}
         self.workspace_bytes = int(lib.gl_batch_workspace_bytes(h))
         self.num_chunks = int(lib.gl_batch_num_chunks(h))
         self.first = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "_lib", "libgigalens_amd.so")
-UNITS = ["fused_0.hip", "fused_1.hip", "fused_2.hip", "staged.hip", "cnn.hip", "svi.hip", "hmc.hip", "noise.hip", "capi.hip"]
+UNITS = ["fused_0.hip", "fused_1.hip", "fused_2.hip", "band.hip", "staged.hip", "cnn.hip", "svi.hip", "hmc.hip", "noise.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

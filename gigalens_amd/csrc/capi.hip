@@ -48,7 +48,8 @@ struct gl_plan {
     int kf4;                 // conv template parameter of the multi-channel path
     size_t solve_lds;
     int ksplit_cap;          // Gram pixel-range splits allowed by the image size
-    std::vector<float> W_host;  // folded PSF (kf x kf) of staged plans, for batches
+    std::vector<float> W_host;  // folded PSF (kf x kf), for batches
+    int sig[7];              // profile kinds: lens 0..2, lens light 0..1, source 0..1
     std::vector<void*> dev;  // owned device buffers
     mutable StreamGuard guard;  // orders calls across streams (the scratch above is shared)
 };
@@ -723,6 +724,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
         }
     }
     P.nlens = nl; P.nll = nll; P.nsrc = ns;
+    for (int j = 0; j < 7; ++j) p->sig[j] = sig[j];
     p->model = nullptr;
     const int lstsq = any_lstsq;
     bool staged = p->mc || desc->staged;
@@ -908,6 +910,7 @@ extern "C" int gl_plan_create(const gl_plan_desc* desc, gl_plan** out) {
     int rc;
     if ((rc = upload(p, W.data(), W.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
     P.w = (const float*)dptr;
+    p->W_host = W;
     if (P.sep_rank > 0) {
         if ((rc = upload(p, sa.data(), sa.size() * sizeof(float), &dptr))) { gl_plan_destroy(p); return rc; }
         P.sa = (const float*)dptr;
@@ -1202,15 +1205,195 @@ struct FusedLens {
     const gl_plan* plan;
     int first, count;
 };
+// Lenses on the banded persistent kernel: one launch over all of them.
+struct BandSet {
+    const BandEntry* model = nullptr;
+    gl::BArgs A;            // device tables and LDS layout; A.K is set per call
+    int n = 0;              // lenses
+    int grid = 0;           // workgroups (CUs x workgroups per CU at the LDS size)
+    size_t lds = 0;         // dynamic LDS bytes per workgroup
+    int rows_min = 0, rows_max = 0;
+    size_t scratch_bytes = 0;
+};
 struct gl_batch {
     int n, d, total;
     std::vector<BatchChunk> chunks;
     std::vector<FusedLens> fused;
+    BandSet band;
+    int n_staged = 0;
     std::vector<int> counts;
     size_t arena_bytes;
     std::vector<void*> dev;
     mutable StreamGuard guard;  // orders calls across streams (the workspace is shared)
 };
+
+static const BandEntry* find_band_model(const int sig[7]) {
+    int nm = 0;
+    const BandEntry* tab = band_models(&nm);
+    for (int i = 0; i < nm; ++i) {
+        bool ok = true;
+        for (int j = 0; j < 7; ++j) ok = ok && tab[i].sig[j] == sig[j];
+        if (ok) return &tab[i];
+    }
+    return nullptr;
+}
+
+// Rank of a folded PSF at relative singular value 1e-6 and its factors for the banded
+// kernel (gl::band::fac_floats layout: a_r = sv_r u_r and b_r = v_r, KFP floats each,
+// zero padded, then a_r with ZP zeros in front, KFV floats each).  0 when the rank
+// exceeds gl::band::kMaxRank.
+static int band_factors(const std::vector<float>& W, int k, int ss, std::vector<float>& fac) {
+    const int kf = k + ss - 1;
+    std::vector<double> Wd(kf * kf), sv, Uv, Vv;
+    for (int i = 0; i < kf * kf; ++i) Wd[i] = W[i];
+    jacobi_svd(kf, Wd, sv, Uv, Vv);
+    if (sv.empty() || !(sv[0] > 0.0)) return 0;
+    int R = 0;
+    while (R < kf && sv[R] >= 1e-6 * sv[0]) ++R;
+    if (R > gl::band::kMaxRank) return 0;
+    const gl::band::Geom g = gl::band::make_geom(1, ss, k, R, 1);
+    fac.assign((size_t)gl::band::fac_floats(g), 0.0f);
+    for (int r = 0; r < R; ++r)
+        for (int i = 0; i < kf; ++i) {
+            const float a = (float)(sv[r] * Uv[i * kf + r]);
+            fac[(size_t)r * g.KFP + i] = a;
+            fac[(size_t)(R + r) * g.KFP + i] = (float)Vv[i * kf + r];
+            fac[(size_t)2 * R * g.KFP + (size_t)r * g.KFV + g.ZP + i] = a;
+        }
+    return R;
+}
+
+// Puts the lenses marked in `want` that the banded kernel can take on it; clears
+// want[i] for the others.  gfirst: lens-major particle offsets.
+static int band_setup(gl_batch* b, const gl_plan* const* plans, const int* counts, const std::vector<int>& gfirst,
+                      const gl_batch_opts& o, std::vector<char>& want) {
+    const int n = b->n;
+    BandSet& B = b->band;
+    std::vector<int> R(n, 0);
+    std::vector<std::vector<float>> fac(n);
+    for (int i = 0; i < n; ++i) {
+        if (!want[i]) continue;
+        const gl_plan* q = plans[i];
+        const BandEntry* e = find_band_model(q->sig);
+        want[i] = 0;
+        if (!e || q->epl_batch || q->P.n_obs != 1 || (B.model && e != B.model)) continue;
+        R[i] = band_factors(q->W_host, q->P.k, q->P.ss, fac[i]);
+        if (R[i] < 1) continue;
+        B.model = e;
+        want[i] = 1;
+    }
+    // LDS layout shared by every lens (floats)
+    int max_n = 0;
+    for (int i = 0; i < n; ++i)
+        if (want[i]) max_n = std::max(max_n, plans[i]->P.max_n);
+    if (!B.model) return GL_OK;
+    const int* sig = B.model->sig;
+    gl::BArgs& A = B.A;
+    memset(&A, 0, sizeof(A));
+    int off = 0;
+    A.off_x = off; off += (6 * b->d + 3) & ~3;
+    A.off_c = off; off += gl::kMaxComp * gl::kConstStride;
+    A.off_coef = off;
+    int coef_off[3] = {0, 0, 0};
+    const int coef_block = (glm::epl_tab_floats(max_n) + 3) & ~3;
+    for (int i = 0; i < 3; ++i)
+        if (sig[i] == glm::K_EPL) { coef_off[i] = off - A.off_coef; off += coef_block; }
+    A.off_red = off; off += (gl::kBandBlock / 64 + 1) * (2 + 8 * 3 + 11 * 4);
+    off = (off + 3) & ~3;
+    A.off_band = off;
+    const int lds_cap = o.band_lds_bytes > 0 ? o.band_lds_bytes : 80 * 1024 - 256;
+    const int region = lds_cap / 4 - off;
+    if (region <= 0) return fail(GL_ERR_ARG, "gl_batch_create: band_lds_bytes too small for the banded kernel");
+    // per lens: band rows that fit the region; lenses with none stay where they were
+    std::vector<gl::BLens> L;
+    std::vector<int> idx;
+    long slot = 0;
+    int band_max = 0;
+    B.rows_min = 1 << 30;
+    B.rows_max = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!want[i]) continue;
+        const gl_plan* q = plans[i];
+        gl::band::Geom g = gl::band::make_geom(q->P.N, q->P.ss, q->P.k, R[i], 1);
+        int rows = region / std::max(g.BW, g.R * g.VW + g.nsub);
+        rows = std::min(rows, o.band_rows > 0 ? o.band_rows : 64);
+        rows = std::min(rows, g.nsub);
+        if (rows < 1) { want[i] = 0; continue; }
+        g.BY = rows;
+        gl::BLens E;
+        memset(&E, 0, sizeof(E));
+        E.P = q->P;
+        E.P.batch_n = nullptr;
+        E.P.max_n = max_n;
+        for (int j = 0; j < 3; ++j) E.P.coef_off[j] = coef_off[j];
+        E.g = g;
+        E.first = gfirst[i];
+        E.count = counts[i];
+        E.glp_map = -1.0f / ((float)counts[i] * (float)q->P.N * (float)q->P.N);
+        void* dptr = nullptr;
+        int rc = dev_alloc(b->dev, fac[i].size() * sizeof(float), &dptr, false);
+        if (rc) return rc;
+        if (hipMemcpy(dptr, fac[i].data(), fac[i].size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(GL_ERR_HIP, "gl_batch_create: upload");
+        E.fac = (const float*)dptr;
+        slot = std::max(slot, gl::band::slot_floats(g));
+        band_max = std::max(band_max, gl::band::band_floats(g));
+        B.rows_min = std::min(B.rows_min, rows);
+        B.rows_max = std::max(B.rows_max, rows);
+        L.push_back(E);
+        idx.push_back(i);
+    }
+    B.n = (int)L.size();
+    if (!B.n) { B.model = nullptr; return GL_OK; }
+    B.lds = (size_t)(A.off_band + band_max) * sizeof(float);
+    int wg_per_cu = 0, dev = 0, cus = 0;
+    GL_HIP(B.model->attr(B.lds, &wg_per_cu));
+    GL_HIP(hipGetDevice(&dev));
+    GL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (wg_per_cu < 1) return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: the banded kernel does not fit a CU");
+    // lenses by descending cost (sub-pixels per particle): the longest items start first
+    std::vector<int> order(B.n);
+    for (int j = 0; j < B.n; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return L[a].g.nsub > L[c].g.nsub; });
+    std::vector<int> pre(B.n + 1, 0);
+    for (int j = 0; j < B.n; ++j) pre[j + 1] = pre[j] + L[order[j]].count;
+    A.n = B.n;
+    A.total = pre[B.n];
+    B.grid = std::max(1, std::min(cus * wg_per_cu, A.total));
+    slot = (slot + 63) & ~63L;
+    A.slot = slot;
+    B.scratch_bytes = (size_t)B.grid * (size_t)slot * sizeof(float);
+    void* dptr = nullptr;
+    int rc;
+    if ((rc = dev_alloc(b->dev, B.scratch_bytes, &dptr, false))) return rc;
+    A.scratch = (float*)dptr;
+    if ((rc = dev_alloc(b->dev, 2 * sizeof(int), &dptr, false))) return rc;
+    A.queue = (int*)dptr;
+    GL_HIP(hipMemset(A.queue, 0, 2 * sizeof(int)));
+    auto put = [&](const void* host, size_t bytes, void** out) -> int {
+        int r = dev_alloc(b->dev, bytes, out, false);
+        if (r) return r;
+        if (hipMemcpy(*out, host, bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(GL_ERR_HIP, "gl_batch_create: upload");
+        return GL_OK;
+    };
+    if ((rc = put(L.data(), L.size() * sizeof(gl::BLens), &dptr))) return rc;
+    A.lens = (const gl::BLens*)dptr;
+    if ((rc = put(order.data(), order.size() * sizeof(int), &dptr))) return rc;
+    A.order = (const int*)dptr;
+    if ((rc = put(pre.data(), pre.size() * sizeof(int), &dptr))) return rc;
+    A.pre = (const int*)dptr;
+    (void)idx;
+    return GL_OK;
+}
+
+// One launch of the banded kernel over every lens on it.
+static int band_run(const gl_batch* b, const gl::KArgs& K, bool grad, hipStream_t stream) {
+    gl::BArgs A = b->band.A;
+    A.K = K;
+    GL_HIP(b->band.model->launch(b->band.grid, b->band.lds, stream, A, grad));
+    return GL_OK;
+}
 
 extern "C" int gl_batch_destroy(gl_batch* b) {
     if (!b) return GL_OK;
@@ -1328,22 +1511,25 @@ static int batch_chunk(gl_batch* b, const std::vector<gl::McLens>& lens, const s
     return GL_OK;
 }
 
-extern "C" int gl_batch_create_ws(const gl_plan* const* plans, int n, const int* counts, size_t workspace_bytes,
+extern "C" int gl_batch_create_ex(const gl_plan* const* plans, int n, const int* counts, const gl_batch_opts* opts,
                                   gl_batch** out) {
     if (out) *out = nullptr;
     if (!plans || !counts || !out || n <= 0) return fail(GL_ERR_ARG, "gl_batch_create: bad arguments");
+    gl_batch_opts o;
+    memset(&o, 0, sizeof(o));
+    if (opts) o = *opts;
+    if (o.band < 0 || o.band > 2 || o.band_rows < 0 || o.band_lds_bytes < 0 || o.band_lds_bytes > 160 * 1024)
+        return fail(GL_ERR_ARG, "gl_batch_create: band must be 0, 1 or 2, band_rows >= 0, band_lds_bytes in [0, 160 KiB]");
     for (int i = 0; i < n; ++i) {
         const gl_plan* q = plans[i];
         if (!q) return fail(GL_ERR_ARG, "gl_batch_create: null plan");
         if (q->mc && (q->M.lstsq || q->M.famp))
             return fail(GL_ERR_UNSUPPORTED, "gl_batch_create: least-squares and shapelet lights are not batched");
         if (counts[i] <= 0) return fail(GL_ERR_ARG, "gl_batch_create: particle counts must be positive");
-        if (!q->mc && counts[i] > gl::kOrderCap)
-            return fail(GL_ERR_ARG, "gl_batch_create: a resident-image lens takes at most 16384 particles");
         if (q->P.d != plans[0]->P.d || q->P.lik != plans[0]->P.lik)
             return fail(GL_ERR_ARG, "gl_batch_create: plans must share the parameter count and likelihood");
     }
-    const size_t budget = workspace_bytes ? workspace_bytes : (size_t)32 << 30;
+    const size_t budget = o.workspace_bytes ? o.workspace_bytes : (size_t)32 << 30;
     gl_batch* b = new gl_batch();
     b->n = n;
     b->d = plans[0]->P.d;
@@ -1357,15 +1543,26 @@ extern "C" int gl_batch_create_ws(const gl_plan* const* plans, int n, const int*
         gfirst[i + 1] = gfirst[i] + counts[i];
     }
     b->total = gfirst[n];
+    // lenses on the banded kernel (band 1: any lens it takes; 2: staged lenses only)
+    std::vector<char> on_band(n, 0);
+    if (o.band)
+        for (int i = 0; i < n; ++i) on_band[i] = o.band == 1 || plans[i]->mc;
+    if ((rc = band_setup(b, plans, counts, gfirst, o, on_band))) { gl_batch_destroy(b); return rc; }
     // staged lenses: constants (per lens, kept) and scratch sizes; chunks in lens order
     std::vector<gl::McLens> lens;
     std::vector<int> firsts, kf4s, mcm;
     std::vector<size_t> bytes;
     for (int i = 0; i < n && !rc; ++i) {
+        if (on_band[i]) continue;
         if (!plans[i]->mc) {
+            if (counts[i] > gl::kOrderCap) {
+                gl_batch_destroy(b);
+                return fail(GL_ERR_ARG, "gl_batch_create: a resident-image lens takes at most 16384 particles");
+            }
             b->fused.push_back(FusedLens{plans[i], gfirst[i], counts[i]});
             continue;
         }
+        ++b->n_staged;
         gl::McLens E;
         memset(&E, 0, sizeof(E));
         E.P = plans[i]->P;
@@ -1443,8 +1640,33 @@ extern "C" int gl_batch_create_ws(const gl_plan* const* plans, int n, const int*
     return GL_OK;
 }
 
+extern "C" int gl_batch_create_ws(const gl_plan* const* plans, int n, const int* counts, size_t workspace_bytes,
+                                  gl_batch** out) {
+    gl_batch_opts o;
+    memset(&o, 0, sizeof(o));
+    o.workspace_bytes = workspace_bytes;
+    return gl_batch_create_ex(plans, n, counts, &o, out);
+}
+
 extern "C" int gl_batch_create(const gl_plan* const* plans, int n, const int* counts, gl_batch** out) {
     return gl_batch_create_ws(plans, n, counts, 0, out);
+}
+
+extern "C" int gl_batch_query(const gl_batch* b, gl_batch_info* info) {
+    if (!b || !info) return fail(GL_ERR_ARG, "gl_batch_query: null argument");
+    memset(info, 0, sizeof(*info));
+    info->total = b->total;
+    info->lenses_fused = (int)b->fused.size();
+    info->lenses_staged = b->n_staged;
+    info->lenses_band = b->band.n;
+    info->chunks = (int)b->chunks.size();
+    info->band_grid = b->band.grid;
+    info->band_lds_bytes = (int)b->band.lds;
+    info->band_rows_min = b->band.n ? b->band.rows_min : 0;
+    info->band_rows_max = b->band.rows_max;
+    info->band_scratch_bytes = b->band.scratch_bytes;
+    info->workspace_bytes = b->arena_bytes;
+    return GL_OK;
 }
 
 extern "C" int gl_batch_size(const gl_batch* b) { return b ? b->total : -1; }
@@ -1465,6 +1687,15 @@ extern "C" int gl_batch_log_prob(const gl_batch* b, const float* z, float* lp, f
     A.lp = lp;
     A.chi2 = chi2;
     A.grad = grad_z;
+    if (b->band.n) {
+        gl::KArgs K = blank(gl::M_LOGPROB, b->total);
+        K.in = z;
+        K.lp = lp;
+        K.chi2 = chi2;
+        K.grad = grad_z;
+        int rc = band_run(b, K, grad_z != nullptr, stream);
+        if (rc) return rc;
+    }
     for (const BatchChunk& C : b->chunks) {
         int rc = mc_pipeline_run(true, C.T, A, C.ss_launch, C.kf4, 0, 0, 1, 0, grad_z != nullptr, stream, false,
                                  C.mc_model, &C.groups);
@@ -1505,6 +1736,21 @@ extern "C" int gl_batch_map_adam_step(const gl_batch* b, float* z, float* m, flo
     A.beta1 = beta1;
     A.beta2 = beta2;
     A.eps = epsilon;
+    if (b->band.n) {
+        gl::KArgs K = blank(gl::M_MAP, b->total);
+        K.in = z;
+        K.z_io = z;
+        K.m = m;
+        K.v = v;
+        K.lp = lp;
+        K.chi2 = chi2;
+        K.lr_t = A.lr_t;
+        K.beta1 = beta1;
+        K.beta2 = beta2;
+        K.eps = epsilon;
+        int rc = band_run(b, K, true, stream);
+        if (rc) return rc;
+    }
     for (const BatchChunk& C : b->chunks) {
         int rc = mc_pipeline_run(true, C.T, A, C.ss_launch, C.kf4, 0, 0, 1, 0, true, stream, false, C.mc_model,
                                  &C.groups);

@@ -1,7 +1,7 @@
 // Host-side declarations shared by the translation units of the C ABI
 // (capi.hip: plans, batches and entry points; fused_*.hip: the resident-image
 // particle kernels by profile combination; staged.hip: the HBM-staged pipeline;
-// cnn.hip: the learned initialiser).  Kernels are launched only from the unit
+// cnn.hip: the learned initialiser; band.hip: the banded persistent kernel).  Kernels are launched only from the unit
 // that instantiates them.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "abi_common.h"
+#include "band_kernels.hip"
 #include "lens_kernels.hip"
 #include "lstsq_kernels.hip"
 
@@ -33,6 +34,16 @@ constexpr int EPL = glm::K_EPL, SHR = glm::K_SHEAR, SIS = glm::K_SIS, SIE = glm:
 const ModelEntry* models_0(int* n);
 const ModelEntry* models_1(int* n);
 const ModelEntry* models_2(int* n);
+
+// Banded persistent kernel (band.hip): launch, and LDS attribute + workgroups per CU.
+using BandLaunchFn = hipError_t (*)(int grid, size_t lds, hipStream_t, const gl::BArgs&, bool grad);
+using BandAttrFn = hipError_t (*)(size_t lds, int* wg_per_cu);
+struct BandEntry {
+    int sig[7];
+    BandLaunchFn launch;
+    BandAttrFn attr;
+};
+const BandEntry* band_models(int* n);
 
 // Auxiliary kernels of the fused path (fused_0.hip).
 hipError_t launch_epl_batch_series(const gl::DPlan& P, const float* in, int bs, int use_z, int* out, hipStream_t st);

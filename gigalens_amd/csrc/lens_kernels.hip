@@ -569,6 +569,225 @@ __device__ __forceinline__ void epl_coef_scan(int n, int NN, double f, double t,
     df = (float)(-(double)n * r);
 }
 
+// ---- Per-particle steps of the banded kernel (band_kernels.hip) -----------------------
+// The resident-image particle_kernel below performs the same prologue (A) and reductions
+// (F) written out inline: calling these helpers from it costs the headline instantiation
+// 68 bytes of scratch spills per lane (its register allocation sits at the limit).
+//
+// A: z -> x through the bijectors, log prior + fldj (thread tid < d handles parameter tid;
+// zin is its z, or its x when !use_z).  s_x = [x | dxdz | dlpdx | dfldjdz | gx | lpj].
+__device__ __forceinline__ void prologue_prior(const DPlan& P, bool use_z, int tid, float zin, float* s_x) {
+    const int d = P.d;
+    if (tid < d) {
+        glm::PriorOut o;
+        if (use_z) {
+            glm::prior_eval(P.prior_kind[tid], P.prior_args + 8 * tid, zin, P.fldj_at_x, o);
+        } else {
+            o.x = zin; o.dxdz = 1.0f; o.lp = 0.0f; o.dlpdx = 0.0f; o.fldj = 0.0f; o.dfldjdz = 0.0f;
+        }
+        s_x[tid] = o.x;
+        s_x[d + tid] = o.dxdz;
+        s_x[2 * d + tid] = o.dlpdx;
+        s_x[3 * d + tid] = o.dfldjdz;
+        s_x[4 * d + tid] = 0.0f;
+        s_x[5 * d + tid] = o.lp + o.fldj;
+    }
+}
+
+// Component constants: component ci is prepared by lane 0 of wave ci (the components run
+// side by side) into s_c + ci * kConstStride.
+__device__ __forceinline__ void prologue_components(const DPlan& P, int tid, const float* s_x, float* s_c) {
+    if ((tid & 63) == 0 && (tid >> 6) < P.ncomp) {
+        const int ci = tid >> 6;
+        const DComp& cd = P.comp[ci];
+        float pv[GL_MAX_COMPONENT_PARAMS];
+#pragma unroll
+        for (int j = 0; j < GL_MAX_COMPONENT_PARAMS; ++j) pv[j] = cd.col[j] >= 0 ? s_x[cd.col[j]] : cd.cst[j];
+        float* dst = s_c + ci * kConstStride;
+        if (cd.kind == glm::K_EPL) {
+            glm::EplC c;
+            int nov = P.batch_n ? P.batch_n[ci] : -1;
+            glm::epl_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.niter, nov, c);
+            if (c.N > P.max_n) c.N = P.max_n;
+            *reinterpret_cast<glm::EplC*>(dst) = c;
+        } else if (cd.kind == glm::K_SHEAR) {
+            glm::ShearC c{pv[0], pv[1]};
+            *reinterpret_cast<glm::ShearC*>(dst) = c;
+        } else if (cd.kind == glm::K_SIS) {
+            glm::SisC c{pv[0], pv[1], pv[2]};
+            *reinterpret_cast<glm::SisC*>(dst) = c;
+        } else if (cd.kind == glm::K_SIE) {
+            glm::SieC c;
+            glm::sie_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], c);
+            *reinterpret_cast<glm::SieC*>(dst) = c;
+        } else if (cd.kind == glm::K_TNFW) {
+            glm::TnfwC c;
+            glm::tnfw_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], c);
+            *reinterpret_cast<glm::TnfwC*>(dst) = c;
+        } else if (cd.kind == glm::K_SERSIC_ELLIPSE) {
+            glm::SerC c;
+            glm::sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], cd.lstsq ? 1.0f : pv[6], 1, c);
+            *reinterpret_cast<glm::SerC*>(dst) = c;
+        } else if (cd.kind == glm::K_SERSIC) {
+            glm::SerC c;
+            glm::sersic_prepare(pv[0], pv[1], 0.0f, 0.0f, pv[2], pv[3], cd.lstsq ? 1.0f : pv[4], 0, c);
+            *reinterpret_cast<glm::SerC*>(dst) = c;
+        } else if (cd.kind == glm::K_CORE_SERSIC) {
+            glm::CoreSerC c;
+            glm::core_sersic_prepare(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], pv[6], pv[7], pv[8],
+                                     cd.lstsq ? 1.0f : pv[9], c);
+            *reinterpret_cast<glm::CoreSerC*>(dst) = c;
+        }
+    }
+}
+
+// EPL series coefficient tables: the table of EPL slot i is built by wave i at
+// s_coef + P.coef_off[i] (c_n, then (c_n, dc_n/dt, dc_n/df, 0)), zero padded above N.
+template <class M>
+__device__ __forceinline__ void prologue_tables(const DPlan& P, int tid, const float* s_c, float* s_coef) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (M::lens[i] == glm::K_EPL && (tid >> 6) == i) {
+            const glm::EplC* c = reinterpret_cast<const glm::EplC*>(s_c + i * kConstStride);
+            float* ca = s_coef + P.coef_off[i];
+            float4* cb = reinterpret_cast<float4*>(ca + glm::epl_tab_a_floats(P.max_n));
+            const int NN = c->N;
+            const int n = tid & 63;
+            if (NN < 64) {
+                float cv, dt, df;
+                epl_coef_scan(n, NN, (double)c->f, (double)c->t, cv, dt, df);
+                if (n <= NN) {
+                    ca[n] = cv;
+                    cb[n] = make_float4(cv, dt, df, 0.0f);
+                }
+            } else {
+                for (int k = n; k <= NN; k += 64) {
+                    float cv, dt, df;
+                    glm::epl_coef(k, (double)c->f, (double)c->t, cv, dt, df);
+                    ca[k] = cv;
+                    cb[k] = make_float4(cv, dt, df, 0.0f);
+                }
+            }
+            for (int k = NN + 1 + n; k <= glm::epl_pad(NN); k += 64) {  // zero padding above N
+                ca[k] = 0.0f;
+                cb[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
+}
+
+// F: reductions, per-particle chain rule, outputs.  Every thread of the workgroup calls it (after a barrier) with its partial sums of
+// the log-likelihood, chi^2 and the reverse-mode statistics acc[].  Writes lp / chi2
+// (LOGPROB, MAP), grad (LOGPROB: d/dz, VJP: d/dx) or the Keras-Adam update of z, m, v
+// (MAP; zin, m_in, v_in: thread tid < d's entries).  glp: the weight of lp in the
+// gradient (MAP: minus the loss weight).
+template <int BLOCK, class M, bool GRAD>
+__device__ __forceinline__ void finish_particle(const DPlan& P, const Consts<M>& C, const KArgs& A, int tid, int pid,
+                                                float ll_acc, float chi_acc, const float* acc, float glp,
+                                                bool bad_lens, float zin, float m_in, float v_in, float* s_x,
+                                                float* s_red) {
+    const int d = P.d;
+    const int N = P.N;
+    const float inv_np = 1.0f / (float)(N * N);
+    const int lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = BLOCK / 64;
+    float part = wave_sum(ll_acc);
+    if (lane == 0) s_red[wave * M::NRED + 0] = part;
+    part = wave_sum(chi_acc);
+    if (lane == 0) s_red[wave * M::NRED + 1] = part;
+    if (GRAD) {
+#pragma unroll
+        for (int e = 0; e < M::NACC; ++e) {
+            part = wave_sum(acc[e]);
+            if (lane == 0) s_red[wave * M::NRED + 2 + e] = part;
+        }
+    }
+    __syncthreads();
+    float* s_tot = s_red + NW * M::NRED;
+    if (tid < M::NRED) {
+        float t = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += s_red[w * M::NRED + tid];
+        s_tot[tid] = t;
+    }
+    __syncthreads();
+
+    if (tid == 0) {
+        float lp_like = s_tot[0] - (float)(N * N) * glm::kHalfLn2Pi;
+        float chi2 = s_tot[1] * inv_np;
+        float lp_prior = 0.0f;
+        for (int j = 0; j < d; ++j) lp_prior += s_x[5 * d + j];
+        if (A.mode == M_LOGPROB || A.mode == M_MAP) {
+            if (A.lp) A.lp[pid] = bad_lens ? __builtin_nanf("") : lp_like + lp_prior;
+            if (A.chi2) A.chi2[pid] = bad_lens ? __builtin_nanf("") : chi2;
+        }
+    }
+    if (GRAD && A.mode != M_SIM) {
+        // per-component chain rule: lens slot i on lane 0 of wave 1 + i, light slot i on
+        // lane 0 of wave 4 + i; contributions meet in gx through LDS atomics
+        const float* tot = s_tot + 2;
+        float* gx = s_x + 4 * d;
+        const int w = tid >> 6;
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int k = M::lens[i];
+                if (k == 0 || w != 1 + i) continue;
+                const DComp& cd = P.comp[i];
+                float g[8];
+                int np = 0;
+                if (k == glm::K_EPL) { glm::epl_finish(C.epl[i], tot + M::lens_off(i), g); np = 6; }
+                else if (k == glm::K_SHEAR) { g[0] = tot[M::lens_off(i)]; g[1] = tot[M::lens_off(i) + 1]; np = 2; }
+                else if (k == glm::K_SIS) { glm::sis_finish(tot + M::lens_off(i), g); np = 3; }
+                else if (k == glm::K_SIE) { glm::sie_finish(C.sie[i], tot + M::lens_off(i), g); np = 5; }
+                else if (k == glm::K_TNFW) { glm::tnfw_finish(C.tnfw[i], tot + M::lens_off(i), g); np = 5; }
+                for (int p = 0; p < np; ++p)
+                    if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[p]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (M::light[i] == 0 || w != 4 + i) continue;
+                const DComp& cd = P.comp[light_comp_index<M>(i)];
+                float g[10];
+                if (M::light[i] == glm::K_CORE_SERSIC) {
+                    glm::core_sersic_finish(C.core[i], tot + M::light_off(light_acc_slot<M>(i)), g);
+                    for (int p = 0; p < (cd.lstsq ? 9 : 10); ++p)
+                        if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[p]);
+                    continue;
+                }
+                glm::sersic_finish(C.ser[i], tot + M::light_off(light_acc_slot<M>(i)), g);
+                if (M::light[i] == glm::K_SERSIC_ELLIPSE) {
+                    for (int p = 0; p < (cd.lstsq ? 6 : 7); ++p)
+                        if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[p]);
+                } else {
+                    const int map[5] = {0, 1, 4, 5, 6};
+                    for (int p = 0; p < (cd.lstsq ? 4 : 5); ++p)
+                        if (cd.col[p] >= 0) atomicAdd(&gx[cd.col[p]], g[map[p]]);
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < d) {
+            const float gxj = s_x[4 * d + tid];
+            if (A.mode == M_VJP) {
+                A.grad[(size_t)pid * d + tid] = gxj;
+            } else {
+                const float gz = (gxj + glp * s_x[2 * d + tid]) * s_x[d + tid] + glp * s_x[3 * d + tid];
+                if (A.mode == M_LOGPROB) {
+                    A.grad[(size_t)pid * d + tid] = gz;
+                } else {  // M_MAP: Keras Adam
+                    const size_t o = (size_t)pid * d + tid;
+                    float mm = A.beta1 * m_in + (1.0f - A.beta1) * gz;
+                    float vv = A.beta2 * v_in + (1.0f - A.beta2) * gz * gz;
+                    A.m[o] = mm;
+                    A.v[o] = vv;
+                    A.z_io[o] = zin - A.lr_t * mm / (sqrtf(vv) + A.eps);
+                }
+            }
+        }
+    }
+}
+
 template <int BLOCK, class M, bool GRAD>
 __global__ __launch_bounds__(BLOCK) void particle_kernel(const DPlan P, const KArgs A) {
     if (A.limit && A.limit_k >= *A.limit) return;  // uniform over the launch

@@ -239,6 +239,40 @@ typedef struct gl_batch gl_batch;
 int gl_batch_create(const gl_plan* const* plans, int n_lenses, const int* counts, gl_batch** out);
 int gl_batch_create_ws(const gl_plan* const* plans, int n_lenses, const int* counts, size_t workspace_bytes,
                        gl_batch** out);
+/* Options of gl_batch_create_ex.  band selects the banded persistent kernel: every
+ * lens on it runs in ONE launch per call -- workgroups as many as the CUs hold at
+ * once take (lens, particle) items from a device queue, longest lenses first, and
+ * stream each particle's supersampled image through LDS in bands of sub-rows, so
+ * images of any size share the launch.  It takes lenses of the profile combinations
+ * it is instantiated for (EPL + shear with a Sersic ellipse source, with or without a
+ * Sersic ellipse lens light) whose folded PSF has rank <= 8 at singular value 1e-6;
+ * other lenses keep the resident-image or staged path.  band 0: off (gl_batch_create_ws);
+ * 1: every lens it takes; 2: only lenses the resident-image kernel cannot hold.
+ * band_rows caps the sub-rows per band (0: as many as the LDS holds, at most 64);
+ * band_lds_bytes is the LDS per workgroup (0: 80 KiB less 256 bytes, two workgroups
+ * per CU).  The kernel keeps R x nsub x N + N x N floats per workgroup in HBM
+ * (allocated here). */
+typedef struct gl_batch_opts {
+    size_t workspace_bytes;  /* arena of the staged lenses (0: 32 GiB) */
+    int band;
+    int band_rows;
+    int band_lds_bytes;
+} gl_batch_opts;
+int gl_batch_create_ex(const gl_plan* const* plans, int n_lenses, const int* counts, const gl_batch_opts* opts,
+                       gl_batch** out);
+typedef struct gl_batch_info {
+    int total;          /* particles */
+    int lenses_fused;   /* lenses on the resident-image kernel (one launch each) */
+    int lenses_staged;  /* lenses on the staged pipeline */
+    int lenses_band;    /* lenses on the banded persistent kernel (one launch for all) */
+    int chunks;         /* staged chunks */
+    int band_grid;      /* workgroups of the banded launch */
+    int band_lds_bytes;
+    int band_rows_min, band_rows_max;  /* sub-rows per band over its lenses */
+    size_t band_scratch_bytes;         /* HBM of the banded kernel's per-workgroup slots */
+    size_t workspace_bytes;            /* staged arena */
+} gl_batch_info;
+int gl_batch_query(const gl_batch* batch, gl_batch_info* info);
 int gl_batch_destroy(gl_batch* batch);
 int gl_batch_size(const gl_batch* batch);  /* total particles */
 size_t gl_batch_workspace_bytes(const gl_batch* batch);  /* bytes of the staged lenses' arena */

@@ -38,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layout_matches_header():
-    """ctypes mirrors of gl_component_desc / gl_plan_desc have the C sizes."""
+    """ctypes mirrors of the ABI structs have the C sizes and field offsets."""
     src = os.path.join(ROOT, "_abi_probe.c")
     out = os.path.join(ROOT, "_abi_probe")
     import subprocess
@@ -46,14 +46,19 @@ def test_struct_layout_matches_header():
     d = tempfile.mkdtemp()
     src = os.path.join(d, "probe.c")
     out = os.path.join(d, "probe")
-    open(src, "w").write('#include <stdio.h>\n#include "gigalens_amd.h"\nint main(void){printf("%zu %zu %zu\\n",'
-                         'sizeof(gl_component_desc), sizeof(gl_plan_desc), sizeof(gl_plan_info));return 0;}\n')
+    open(src, "w").write('#include <stdio.h>\n#include <stddef.h>\n#include "gigalens_amd.h"\n'
+                         'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(gl_component_desc), '
+                         'sizeof(gl_plan_desc), sizeof(gl_plan_info), sizeof(gl_batch_opts), sizeof(gl_batch_info), '
+                         'offsetof(gl_batch_opts, band_lds_bytes), offsetof(gl_batch_info, workspace_bytes));'
+                         'return 0;}\n')
     r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
                         src, "-o", out], capture_output=True, text=True)
     if r.returncode != 0:
         pytest.skip("no C compiler for the layout probe: " + r.stderr[-200:])
     sizes = [int(v) for v in subprocess.run([out], capture_output=True, text=True).stdout.split()]
-    assert sizes == [ctypes.sizeof(nat.ComponentDesc), ctypes.sizeof(nat.PlanDesc), ctypes.sizeof(nat.PlanInfo)]
+    assert sizes == [ctypes.sizeof(nat.ComponentDesc), ctypes.sizeof(nat.PlanDesc), ctypes.sizeof(nat.PlanInfo),
+                     ctypes.sizeof(nat.BatchOpts), ctypes.sizeof(nat.BatchInfo), nat.BatchOpts.band_lds_bytes.offset,
+                     nat.BatchInfo.workspace_bytes.offset]
 
 
 def test_layout_and_prior_table():
