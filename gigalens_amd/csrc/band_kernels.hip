@@ -14,8 +14,9 @@
 //   C  conv+pool  horizontal pass of each band into T (the workgroup's HBM slot),
 //                 then the vertical pass, the Gaussian log-likelihood and dL/dimage
 //                 (HBM slot) -- band_passes.h
-//   D  adjoint    per band: transposed vertical pass of dL/dimage into LDS, then the
-//                 transposed horizontal pass to every sub-pixel of the band (LDS)
+//   D  adjoint    per band: the rows of dL/dimage it reaches staged in LDS, the transposed
+//                 vertical pass into LDS, then the transposed horizontal pass to every
+//                 sub-pixel of the band (LDS)
 //   E  recompute  the ray-trace again per sub-pixel, accumulating the reverse-mode
 //                 statistics
 //   F  reduce     as the resident-image kernel: chain rule, lp / chi2 / gradient or
@@ -61,23 +62,11 @@ struct BArgs {
     float* scratch;    // slot floats per workgroup
     long slot;
     int* queue;        // [0] next item, [1] workgroups done; both 0 between launches
-    int off_x, off_c, off_coef, off_red, off_band;  // LDS layout (floats)
-};
-
-// Per-lens values the per-sub-pixel and per-pixel loops use, held in scalar registers.
-struct BHot {
-    band::Geom g;
-    float M00, M01, M10, M11, ox, oy, cf, bkg2, inv_texp;
-    int lik;
-    float glp_map;
-    const float* obs;
-    const float* inv_var_obs;
-    const float* log_sig_obs;
-    const float* fac;
+    int off_x, off_c, off_coef, off_red, off_lens, off_band;  // LDS layout (floats)
 };
 
 // Copy of a lens-uniform struct of 32-bit words into scalar registers (one
-// readfirstlane per word: the loads themselves may be vector loads).
+// readfirstlane per word).
 template <class T>
 __device__ __forceinline__ T uniform_copy(const T& src) {
     static_assert(sizeof(T) % 4 == 0, "32-bit words");
@@ -88,42 +77,54 @@ __device__ __forceinline__ T uniform_copy(const T& src) {
     for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = __builtin_amdgcn_readfirstlane(w[i]);
     return out;
 }
-
-__device__ __forceinline__ BHot band_hot(const BLens* L) {
-    BHot h;
-    h.g = L->g;
-    h.M00 = L->P.M00; h.M01 = L->P.M01; h.M10 = L->P.M10; h.M11 = L->P.M11; h.ox = L->P.ox; h.oy = L->P.oy;
-    h.cf = L->P.cf; h.bkg2 = L->P.bkg2; h.inv_texp = L->P.inv_texp; h.lik = L->P.lik;
-    h.glp_map = L->glp_map;
-    h.obs = L->P.obs; h.inv_var_obs = L->P.inv_var_obs; h.log_sig_obs = L->P.log_sig_obs;
-    h.fac = L->fac;
-    return uniform_copy(h);
+template <class T>
+__device__ __forceinline__ T uniform_get(const T& src) {
+    return uniform_copy(src);
 }
 
+// Sub-pixel coordinates: theta = transform (x, y) + origin (tf/simulator.py:19-29).
+struct BXform {
+    float M00, M01, M10, M11, ox, oy;
+};
+typedef const BLens __attribute__((address_space(4))) CLensT;
+__device__ __forceinline__ BXform band_xform(const CLensT& L) {
+    BXform t;
+    t.M00 = L.P.M00; t.M01 = L.P.M01; t.M10 = L.P.M10; t.M11 = L.P.M11; t.ox = L.P.ox; t.oy = L.P.oy;
+    return t;
+}
+
+// One particle on the workgroup.  The lens's table entry is copied into LDS (s_lens) and
+// every phase reads the few lens values it uses from there into scalar registers, so
+// they are not held across the phases: the per-sub-pixel loops keep the scalar
+// registers for the profile constants.
 template <int BLOCK, class M, bool GRAD>
 __device__ __forceinline__ void band_particle(const BArgs& A, const BLens* Lp, int pid, float* s_x, float* s_c,
-                                              float* s_coef, float* s_red, float* s_band, float* slot, int tid) {
-    const DPlan& P = Lp->P;  // prologue and epilogue only
-    const BHot L = band_hot(Lp);
-    const band::Geom& g = L.g;
+                                              float* s_coef, float* s_red, float* s_band, float* slot, int tid,
+                                              BLens* s_lens) {
+    // its previous reader, the last particle's epilogue, ended before the item barrier
+    static_assert(sizeof(BLens) % 4 == 0, "lens entry copied as 32-bit words");
+    for (int i = tid; i < (int)(sizeof(BLens) / 4); i += BLOCK)
+        reinterpret_cast<int*>(s_lens)[i] = reinterpret_cast<const int*>(Lp)[i];
+    __syncthreads();
+    const DPlan& P = s_lens->P;  // prologue and epilogue
     const KArgs& K = A.K;
-    const int d = P.d;
-    const float* fa = L.fac;                  // vertical factors a_r (read by scalar loads)
-    const float* fb = L.fac + g.R * g.KFP;    // horizontal factors b_r
-    const float* fv = L.fac + 2 * g.R * g.KFP;  // a_r zero padded for the vertical pass
-    float* T = slot;
-    float* G = slot + band::t_floats(g);
+    const CLensT& Lc = *(const CLensT*)Lp;  // the same entry, read-only: scalar loads the compiler may repeat
 
     // ---- A: prologue -------------------------------------------------------
+    const int d = uniform_get(P.d);
     float zin = 0.0f, m_in = 0.0f, v_in = 0.0f;
     if (tid < d) {
         const size_t o = (size_t)pid * d + tid;
         zin = K.in[o];
         if (K.mode == M_MAP) { m_in = K.m[o]; v_in = K.v[o]; }
     }
-    const int nband = band::band_floats(g);
-    for (int i = tid; i < nband; i += BLOCK) s_band[i] = 0.0f;  // halo columns of the forward bands
-    band::zero_pads(g, tid, BLOCK, T, G);
+    float4* b4 = reinterpret_cast<float4*>(s_band);
+    {
+        const band::Geom g = uniform_get(s_lens->g);
+        for (int i = tid; i < (band::band_floats(g) + 3) / 4; i += BLOCK)
+            b4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // halo columns of the forward bands
+        band::zero_pads(g, tid, BLOCK, slot, slot + band::t_floats(g));
+    }
     prologue_prior(P, true, tid, zin, s_x);
     __syncthreads();
     prologue_components(P, tid, s_x, s_c);
@@ -134,74 +135,108 @@ __device__ __forceinline__ void band_particle(const BArgs& A, const BLens* Lp, i
     load_consts<M>(s_c, s_coef, P, C);
 
     // ---- B + C1: per band, ray-trace then the horizontal pass ------------------
-    for (int y0 = 0; y0 < g.nsub; y0 += g.BY) {
-        const int rows = g.BY < g.nsub - y0 ? g.BY : g.nsub - y0;
-        const int cnt = rows * g.nsub;
-        int yy = tid / g.nsub, x = tid - yy * g.nsub;
-        for (int s = tid; s < cnt; s += BLOCK) {
-            const int y = y0 + yy;
-            const float X = fmaf(L.M00, (float)x, fmaf(L.M01, (float)y, L.ox));
-            const float Y = fmaf(L.M10, (float)x, fmaf(L.M11, (float)y, L.oy));
-            float val = eval_forward<M>(C, X, Y);
-            if (isnan(val)) val = 0.0f;
-            s_band[yy * g.BW + g.pb + x] = val;
-            x += BLOCK;
-            while (x >= g.nsub) { x -= g.nsub; ++yy; }
+    for (int y0 = 0;;) {
+        const int nsub = Lc.g.nsub, BY = Lc.g.BY;
+        if (y0 >= nsub) break;
+        const int rows = BY < nsub - y0 ? BY : nsub - y0;
+        {
+            const BXform t = band_xform(Lc);
+            const int BW = Lc.g.BW, pb = Lc.g.pb;
+            const int cnt = rows * nsub;
+            int yy = tid / nsub, x = tid - yy * nsub;
+            for (int s = tid; s < cnt; s += BLOCK) {
+                const int y = y0 + yy;
+                const float X = fmaf(t.M00, (float)x, fmaf(t.M01, (float)y, t.ox));
+                const float Y = fmaf(t.M10, (float)x, fmaf(t.M11, (float)y, t.oy));
+                float val = eval_forward<M>(C, X, Y);
+                if (isnan(val)) val = 0.0f;
+                s_band[yy * BW + pb + x] = val;
+                x += BLOCK;
+                while (x >= nsub) { x -= nsub; ++yy; }
+            }
         }
         __syncthreads();
-        band::hpass_any(g, tid, BLOCK, y0, rows, s_band, fb, T);
+        {
+            const band::Geom g = uniform_get(s_lens->g);
+            const float* fb = uniform_get(s_lens->fac) + g.R * g.KFP;  // horizontal factors b_r
+            band::hpass_any(g, tid, BLOCK, y0, rows, s_band, fb, slot);
+        }
         __syncthreads();
+        y0 += rows;
     }
 
     // ---- C2: vertical pass, likelihood, dL/dimage ---------------------------------
     float ll_acc = 0.0f, chi_acc = 0.0f;
-    const float cf = L.cf;
-    const float inv_np = 1.0f / (float)(g.N * g.N);
-    const float glp = K.mode == M_MAP ? L.glp_map : (K.g_lp ? K.g_lp[pid] : 1.0f);
-    const float gch = (K.mode == M_LOGPROB && K.g_chi2) ? K.g_chi2[pid] : 0.0f;
-    float* Gi = G + g.GT * g.N;  // row 0 of the image in the padded G
-    band::vpass_any(g, tid, BLOCK, T, fv, [&](int I, int J, float conv) {
-        const float im = cf * conv;
-        const int pix = I * g.N + J;
-        float ll, r2, dll, dr2;
-        if (L.lik == GL_LIK_FORWARD)
-            glm::lik_forward(im, L.obs[pix], L.bkg2, L.inv_texp, ll, r2, dll, dr2);
-        else
-            glm::lik_backward(im, L.obs[pix], L.inv_var_obs[pix], L.log_sig_obs[pix], ll, r2, dll, dr2);
-        ll_acc += ll;
-        chi_acc += r2;
-        if (GRAD) Gi[pix] = cf * (glp * dll + gch * inv_np * dr2);
-    });
+    const float glp = K.mode == M_MAP ? uniform_get(s_lens->glp_map) : (K.g_lp ? K.g_lp[pid] : 1.0f);
+    {
+        const band::Geom g = uniform_get(s_lens->g);
+        const float cf = uniform_get(P.cf), bkg2 = uniform_get(P.bkg2), inv_texp = uniform_get(P.inv_texp);
+        const int lik = uniform_get(P.lik);
+        const float* obs = uniform_get(P.obs);
+        const float* ivar = uniform_get(P.inv_var_obs);
+        const float* lsig = uniform_get(P.log_sig_obs);
+        const float* fv = uniform_get(s_lens->fac) + 2 * g.R * g.KFP;  // a_r zero padded
+        const float inv_np = 1.0f / (float)(g.N * g.N);
+        const float gch = (K.mode == M_LOGPROB && K.g_chi2) ? K.g_chi2[pid] : 0.0f;
+        float* Gi = slot + band::t_floats(g) + g.GT * g.N;  // row 0 of the image in the padded G
+        band::vpass_any(g, tid, BLOCK, slot, fv, [&](int I, int J, float conv) {
+            const float im = cf * conv;
+            const int pix = I * g.N + J;
+            float ll, r2, dll, dr2;
+            if (lik == GL_LIK_FORWARD)
+                glm::lik_forward(im, obs[pix], bkg2, inv_texp, ll, r2, dll, dr2);
+            else
+                glm::lik_backward(im, obs[pix], ivar[pix], lsig[pix], ll, r2, dll, dr2);
+            ll_acc += ll;
+            chi_acc += r2;
+            if (GRAD) Gi[pix] = cf * (glp * dll + gch * inv_np * dr2);
+        });
+    }
 
     float acc[M::NACC > 0 ? M::NACC : 1];
 #pragma unroll
     for (int e = 0; e < M::NACC; ++e) acc[e] = 0.0f;
 
     if (GRAD) {
-        // ---- D + E: per band, the transposed passes into LDS (V, then the sub-pixel
-        // adjoint values gb), then the recompute over the band's sub-pixels -------------
-        float* gb = s_band + band::g_offset(g);
+        // ---- D + E: per band, the transposed passes into LDS (the rows of dL/dimage it
+        // reaches, V, then the sub-pixel adjoint values gb), then the recompute -------------
         __syncthreads();
-        for (int i = tid; i < nband; i += BLOCK) s_band[i] = 0.0f;  // V margins
-        __syncthreads();
-        for (int y0 = 0; y0 < g.nsub; y0 += g.BY) {
-            const int rows = g.BY < g.nsub - y0 ? g.BY : g.nsub - y0;
-            band::vband_any(g, tid, BLOCK, y0, rows, G, fa, s_band);
-            __syncthreads();
-            band::gpass_any(g, tid, BLOCK, rows, s_band, fb, gb);
-            __syncthreads();
-            const int cnt = rows * g.nsub;
-            int yy = tid / g.nsub, x = tid - yy * g.nsub;
+        {
+            const band::Geom g = uniform_get(s_lens->g);
+            for (int i = tid; i < (band::g_offset(g) + 3) / 4; i += BLOCK)
+                b4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // V margins
+        }
+        for (int y0 = 0;;) {
+            const int nsub = Lc.g.nsub, BY = Lc.g.BY;
+            if (y0 >= nsub) break;
+            const int rows = BY < nsub - y0 ? BY : nsub - y0;
+            float* gb;
+            {
+                const band::Geom g = uniform_get(s_lens->g);
+                const float* fac = uniform_get(s_lens->fac);
+                float* Gw = s_band + band::gw_offset(g);
+                gb = s_band + band::g_offset(g);
+                band::stage_g(g, tid, BLOCK, y0, slot + band::t_floats(g), Gw);
+                __syncthreads();
+                band::vband_any(g, tid, BLOCK, y0, rows, Gw, fac, s_band);
+                __syncthreads();
+                band::gpass_any(g, tid, BLOCK, rows, s_band, fac + g.R * g.KFP, gb);
+                __syncthreads();
+            }
+            const BXform t = band_xform(Lc);
+            const int cnt = rows * nsub;
+            int yy = tid / nsub, x = tid - yy * nsub;
             for (int s = tid; s < cnt; s += BLOCK) {
                 const int y = y0 + yy;
-                const float X = fmaf(L.M00, (float)x, fmaf(L.M01, (float)y, L.ox));
-                const float Y = fmaf(L.M10, (float)x, fmaf(L.M11, (float)y, L.oy));
+                const float X = fmaf(t.M00, (float)x, fmaf(t.M01, (float)y, t.ox));
+                const float Y = fmaf(t.M10, (float)x, fmaf(t.M11, (float)y, t.oy));
                 eval_adjoint<M>(C, X, Y, gb[s], acc);
                 x += BLOCK;
-                while (x >= g.nsub) { x -= g.nsub; ++yy; }
+                while (x >= nsub) { x -= nsub; ++yy; }
             }
-            // the next band's transposed vertical pass writes V only; its barrier orders
-            // this band's reads of gb before the next gpass
+            // the next band's staging writes Gw only and its transposed vertical pass V only;
+            // the barrier between them orders this band's reads of gb before the next gpass
+            y0 += rows;
         }
     }
 
@@ -239,7 +274,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         const int pid = __builtin_amdgcn_readfirstlane(((const CLens*)A.lens)[((const CInt*)A.order)[lo]].first) +
                         item - pre[lo];
         band_particle<BLOCK, M, GRAD>(A, L, pid, smem + A.off_x, smem + A.off_c, smem + A.off_coef,
-                                      smem + A.off_red, smem + A.off_band, slot, tid);
+                                      smem + A.off_red, smem + A.off_band, slot, tid,
+                                      reinterpret_cast<BLens*>(smem + A.off_lens));
     }
     // every workgroup has taken its last item once the done count reaches the grid size:
     // the last one clears the queue for the next launch on the stream

@@ -46,6 +46,7 @@ struct Geom {
     int GR;   // rows of G
     int vm;   // zero columns on each side of a V row
     int VW;   // V row width: N + 2 vm
+    int GWR;  // rows of G one band's transposed vertical pass reads (staged in LDS)
     int ZP;   // zeros before the vertical pass's copy of a_r: ss (kRows - 1)
     int KFV;  // its row length: kf + 2 ZP (a_r[p] at ZP + p, zero for p outside [0, kf))
     int fixed;  // compile-time pass set (fixed_code), 0: generic
@@ -86,6 +87,7 @@ __host__ __device__ inline Geom make_geom(int N, int ss, int k, int R, int BY) {
     g.vm = g.Mt - 1 - db_min > db_max ? g.Mt - 1 - db_min : db_max;
     if (g.vm < 1) g.vm = 1;
     g.VW = g.N + 2 * g.vm;
+    g.GWR = (BY + ss - 2) / ss + g.Mt + 1;
     g.ZP = ss * (kRows - 1);
     g.KFV = g.kf + 2 * g.ZP;
     g.fixed = fixed_code(ss, g.kf, R);
@@ -97,12 +99,35 @@ __host__ __device__ inline Geom make_geom(int N, int ss, int k, int R, int BY) {
 __host__ __device__ inline int fac_floats(const Geom& g) { return 2 * g.R * g.KFP + g.R * g.KFV; }
 
 // Floats of LDS per band: the forward band (BY rows of BW), or the adjoint's R planes of V
-// (BY rows of VW) followed by the band's adjoint sub-pixels g (BY rows of nsub).
+// (BY rows of VW), the band's adjoint sub-pixels g (BY rows of nsub) and the window of G
+// rows its transposed vertical pass reads (GWR rows of N).
 __host__ __device__ inline int band_floats(const Geom& g) {
-    const int f = g.BY * g.BW, a = g.R * g.BY * g.VW + g.BY * g.nsub;
+    const int f = g.BY * g.BW, a = g.R * g.BY * g.VW + g.BY * g.nsub + g.GWR * g.N;
     return f > a ? f : a;
 }
 __host__ __device__ inline int g_offset(const Geom& g) { return g.R * g.BY * g.VW; }
+__host__ __device__ inline int gw_offset(const Geom& g) { return g.R * g.BY * g.VW + g.BY * g.nsub; }
+
+// First padded G row (index I + GT) the band from sub-row y0 reads.
+__host__ __device__ inline int gw_first(const Geom& g, int y0) { return (y0 + g.pb) / g.ss - g.Mt + 1 + g.GT; }
+
+// Copy the band's window of padded G rows from HBM into LDS (rows past GR are not read).
+__host__ __device__ inline void stage_g(const Geom& g, int tid, int nthr, int y0, const float* __restrict__ G,
+                                        float* __restrict__ Gw) {
+    const int w0 = gw_first(g, y0);
+    const int nr = g.GR - w0 < g.GWR ? g.GR - w0 : g.GWR;
+    const float* src = G + (long)w0 * g.N;
+    const int total = nr * g.N;
+    constexpr int kIn = 8;  // loads in flight per thread
+    for (int base = tid; base < total; base += kIn * nthr) {
+        float v[kIn];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) v[k] = base + k * nthr < total ? src[base + k * nthr] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kIn; ++k)
+            if (base + k * nthr < total) Gw[base + k * nthr] = v[k];
+    }
+}
 
 // HBM floats of one particle: T (R x TR x N) then G (GR x N).
 __host__ __device__ inline long t_floats(const Geom& g) { return (long)g.R * g.TR * g.N; }
@@ -197,17 +222,19 @@ __host__ __device__ inline void vpass(const Geom& g, int tid, int nthr, const fl
 }
 
 // Transposed vertical pass for the `rows` band rows from sub-row y0:
-// V[r][yy][vm + J] (the vm margins stay as they are: zero).
+// V[r][yy][vm + J] (the vm margins stay as they are: zero).  Gw: the band's window of G
+// rows (stage_g).
 __host__ __device__ inline void vband(const Geom& g, int tid, int nthr, int y0, int rows,
-                                      const float* __restrict__ G, const float* __restrict__ fa,
+                                      const float* __restrict__ Gw, const float* __restrict__ fa,
                                       float* __restrict__ V) {
+    const int w0 = gw_first(g, y0);
     const int cnt = rows * g.N;
     for (int s = tid; s < cnt; s += nthr) {
         const int yy = s / g.N, J = s - yy * g.N;
         const int y = y0 + yy;
         const int Ihi = (y + g.pb) / g.ss;  // the I of the smallest tap, y + pb - ss Ihi in [0, ss)
         const int q0 = y + g.pb - g.ss * Ihi;
-        const float* gcol = G + (Ihi + g.GT) * g.N + J;
+        const float* gcol = Gw + (Ihi + g.GT - w0) * g.N + J;
         float v[kMaxRank];
 #pragma unroll
         for (int r = 0; r < kMaxRank; ++r) v[r] = 0.0f;
@@ -313,14 +340,17 @@ __host__ __device__ inline void vpass_fixed(const Geom& g, int tid, int nthr, co
         for (int i = 0; i < kRows; ++i) o[i] = 0.0f;
 #pragma unroll 1
         for (int r = 0; r < R; ++r) {
+            // the column window first (every load in flight at once), then the taps
             const float* col = T + ((long)r * g.TR + SS * I0) * g.N + J;
             const float* a = fv + r * KFV + ZP;
-#pragma unroll 4
-            for (int c = 0; c < F::SPAN; ++c) {
-                const float v = col[(long)c * g.N];
+            float win[F::SPAN];
 #pragma unroll
-                for (int i = 0; i < kRows; ++i) o[i] = fmaf(ldc(a, c - SS * i), v, o[i]);
-            }
+            for (int c = 0; c < F::SPAN; ++c) win[c] = col[(long)c * g.N];
+#pragma unroll
+            for (int c = 0; c < F::SPAN; ++c)
+#pragma unroll
+                for (int i = 0; i < kRows; ++i)
+                    if (c - SS * i >= 0 && c - SS * i < KF) o[i] = fmaf(ldc(a, c - SS * i), win[c], o[i]);
         }
 #pragma unroll
         for (int i = 0; i < kRows; ++i)
@@ -332,9 +362,10 @@ __host__ __device__ inline void vpass_fixed(const Geom& g, int tid, int nthr, co
 // phase of the row and its factors are wave-uniform.
 template <int SS, int KF, int R>
 __host__ __device__ inline void vband_fixed(const Geom& g, int tid, int nthr, int y0, int rows,
-                                            const float* __restrict__ G, const float* __restrict__ fa,
+                                            const float* __restrict__ Gw, const float* __restrict__ fa,
                                             float* __restrict__ V) {
     using F = Fix<SS, KF>;
+    const int w0 = gw_first(g, y0);
     const int NC = (g.N + 63) / 64;
     const int lane = tid & 63;
     for (int w = uniform_int(tid >> 6); w < rows * NC; w += nthr >> 6) {
@@ -343,7 +374,7 @@ __host__ __device__ inline void vband_fixed(const Geom& g, int tid, int nthr, in
         const int Ihi = (y + F::PB) / SS;
         const int q0 = y + F::PB - SS * Ihi;
         if (J < g.N) {
-            const float* gcol = G + (Ihi + g.GT) * g.N + J;
+            const float* gcol = Gw + (Ihi + g.GT - w0) * g.N + J;
             float v[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) v[r] = 0.0f;
@@ -411,13 +442,13 @@ __host__ __device__ inline void vpass_any(const Geom& g, int tid, int nthr, cons
     default: vpass(g, tid, nthr, T, fv, out);
     }
 }
-__host__ __device__ inline void vband_any(const Geom& g, int tid, int nthr, int y0, int rows, const float* G,
+__host__ __device__ inline void vband_any(const Geom& g, int tid, int nthr, int y0, int rows, const float* Gw,
                                           const float* fa, float* V) {
     switch (g.fixed) {
-    case 1: vband_fixed<2, 22, 1>(g, tid, nthr, y0, rows, G, fa, V); break;
-    case 2: vband_fixed<3, 35, 5>(g, tid, nthr, y0, rows, G, fa, V); break;
-    case 3: vband_fixed<4, 46, 1>(g, tid, nthr, y0, rows, G, fa, V); break;
-    default: vband(g, tid, nthr, y0, rows, G, fa, V);
+    case 1: vband_fixed<2, 22, 1>(g, tid, nthr, y0, rows, Gw, fa, V); break;
+    case 2: vband_fixed<3, 35, 5>(g, tid, nthr, y0, rows, Gw, fa, V); break;
+    case 3: vband_fixed<4, 46, 1>(g, tid, nthr, y0, rows, Gw, fa, V); break;
+    default: vband(g, tid, nthr, y0, rows, Gw, fa, V);
     }
 }
 __host__ __device__ inline void gpass_any(const Geom& g, int tid, int nthr, int rows, const float* V,

@@ -1300,9 +1300,10 @@ static int band_setup(gl_batch* b, const gl_plan* const* plans, const int* count
         if (sig[i] == glm::K_EPL) { coef_off[i] = off - A.off_coef; off += coef_block; }
     A.off_red = off; off += (gl::kBandBlock / 64 + 1) * (2 + 8 * 3 + 11 * 4);
     off = (off + 3) & ~3;
+    A.off_lens = off; off += (int)((sizeof(gl::BLens) / 4 + 3) & ~(size_t)3);
     A.off_band = off;
     const int lds_cap = o.band_lds_bytes > 0 ? o.band_lds_bytes : 80 * 1024 - 256;
-    const int region = lds_cap / 4 - off;
+    const int region = lds_cap / 4 - off - 3;  // room to round the band up to whole float4s
     if (region <= 0) return fail(GL_ERR_ARG, "gl_batch_create: band_lds_bytes too small for the banded kernel");
     // per lens: band rows that fit the region; lenses with none stay where they were
     std::vector<gl::BLens> L;
@@ -1314,12 +1315,12 @@ static int band_setup(gl_batch* b, const gl_plan* const* plans, const int* count
     for (int i = 0; i < n; ++i) {
         if (!want[i]) continue;
         const gl_plan* q = plans[i];
-        gl::band::Geom g = gl::band::make_geom(q->P.N, q->P.ss, q->P.k, R[i], 1);
-        int rows = region / std::max(g.BW, g.R * g.VW + g.nsub);
-        rows = std::min(rows, o.band_rows > 0 ? o.band_rows : 64);
-        rows = std::min(rows, g.nsub);
-        if (rows < 1) { want[i] = 0; continue; }
-        g.BY = rows;
+        // the most sub-rows per band (at most band_rows, 64 by default) whose LDS fits the region
+        int rows = std::min(o.band_rows > 0 ? o.band_rows : 64, q->P.N * q->P.ss);
+        gl::band::Geom g = gl::band::make_geom(q->P.N, q->P.ss, q->P.k, R[i], rows);
+        while (rows > 1 && gl::band::band_floats(g) > region)
+            g = gl::band::make_geom(q->P.N, q->P.ss, q->P.k, R[i], --rows);
+        if (gl::band::band_floats(g) > region) { want[i] = 0; continue; }
         gl::BLens E;
         memset(&E, 0, sizeof(E));
         E.P = q->P;
@@ -1345,7 +1346,8 @@ static int band_setup(gl_batch* b, const gl_plan* const* plans, const int* count
     }
     B.n = (int)L.size();
     if (!B.n) { B.model = nullptr; return GL_OK; }
-    B.lds = (size_t)(A.off_band + band_max) * sizeof(float);
+    // the kernel clears the band with 16-byte stores: whole float4s
+    B.lds = (size_t)(A.off_band + ((band_max + 3) & ~3)) * sizeof(float);
     int wg_per_cu = 0, dev = 0, cus = 0;
     GL_HIP(B.model->attr(B.lds, &wg_per_cu));
     GL_HIP(hipGetDevice(&dev));

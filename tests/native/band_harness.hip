@@ -130,10 +130,13 @@ extern "C" int h_band_check_m(int N, int ss, int k, int R, int BY, unsigned seed
     for (double v : aref) aref_max = std::fmax(aref_max, std::fabs(v));
     for (size_t i = 0; i < band.n; ++i) band.data()[i] = 0.0f;  // V margins
     float* gbuf = band.data() + gl::band::g_offset(g0);
+    // the staged window in a buffer of its own (exact size, guarded); the kernel keeps it in the band
+    Guarded gw((size_t)g0.GWR * N);
     double ea = 0.0;
     for (int y0 = 0; y0 < ns; y0 += g.BY) {
         const int rows = g.BY < ns - y0 ? g.BY : ns - y0;
-        for (int t = 0; t < kThreads; ++t) gl::band::vband_any(g, t, kThreads, y0, rows, G, fa, band.data());
+        for (int t = 0; t < kThreads; ++t) gl::band::stage_g(g, t, kThreads, y0, G, gw.data());
+        for (int t = 0; t < kThreads; ++t) gl::band::vband_any(g, t, kThreads, y0, rows, gw.data(), fa, band.data());
         for (int t = 0; t < kThreads; ++t) gl::band::gpass_any(g, t, kThreads, rows, band.data(), fb, gbuf);
         for (int yy = 0; yy < rows; ++yy)
             for (int x = 0; x < ns; ++x) {
@@ -143,7 +146,7 @@ extern "C" int h_band_check_m(int N, int ss, int k, int R, int BY, unsigned seed
             }
     }
     *err_adj = ea / (aref_max > 0 ? aref_max : 1.0);
-    if (!facg.guards_intact() || !band.guards_intact() || !slot.guards_intact()) return 1;
+    if (!facg.guards_intact() || !band.guards_intact() || !slot.guards_intact() || !gw.guards_intact()) return 1;
     return 0;
 }
 
