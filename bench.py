@@ -503,7 +503,9 @@ def uneven_lenses(args, dev):
 
 def run_uneven(args, dev):
     """Uneven batched fitting: every lens of ``uneven_lenses`` fitted at once by the
-    learned-init MAP -> diagonal SVI pipeline.  Starts: the learned initialiser
+    learned-init MAP -> diagonal SVI pipeline, on the path the batch autotuner picks
+    (``--uneven-band auto``; by default every lens in one launch of the banded
+    persistent kernel per step), tuned before the timed runs.  Starts: the learned initialiser
     (trained on the device on the 60x60 workload: ``--learned-images`` draws,
     ``--learned-epochs`` passes, AdamW ``--learned-lr``) predicts each lens's
     Gaussian from the central 60x60 pixels of its image; 3/4 of the candidates come
@@ -535,13 +537,28 @@ def run_uneven(args, dev):
     t_train = time.perf_counter() - t0
     print(f"[uneven] {L} lenses, {total} particles; network trained in {t_train:.1f} s", flush=True)
 
+    band = args.uneven_band if args.uneven_band == "auto" else int(args.uneven_band)
+
+    def tune(lens_set):
+        """The autotuner's choice for the MAP and SVI batches of these lenses (untimed; the
+        pipeline's own batches then find it in the cache)."""
+        out = {}
+        for name, ls in (("map", lens_set), ("svi", [(c, o, n_vi) for c, o, _ in lens_set])):
+            b = LensBatch(phys, prior, ls, 0.2, 100.0, device=dev, workspace_gib=args.uneven_workspace_gib,
+                          band=band)
+            out[name] = {"config": b.config, "info": b.info,
+                         "timings_ms": b.tuning[1] if b.tuning is not None else None}
+            del b
+        return out
+
     def pipeline(lens_set, tag):
         images = [o for _, o, _ in lens_set]
         torch.cuda.synchronize()
         t_start = time.perf_counter()
-        batch = LensBatch(phys, prior, lens_set, 0.2, 100.0, device=dev, workspace_gib=args.uneven_workspace_gib)
+        batch = LensBatch(phys, prior, lens_set, 0.2, 100.0, device=dev, workspace_gib=args.uneven_workspace_gib,
+                          band=band)
         svi = LensBatch(phys, prior, [(c, o, n_vi) for c, o, _ in lens_set], 0.2, 100.0, device=dev,
-                        workspace_gib=args.uneven_workspace_gib)
+                        workspace_gib=args.uneven_workspace_gib, band=band)
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t_start
         z0 = propose_batch(init, batch, images, prior_fraction=0.25, oversample=2, seed=5)
@@ -568,6 +585,10 @@ def run_uneven(args, dev):
         first = np.where(hit.any(0), np.argmax(hit, 0) + 1, -1)
         t_chi = np.array([t_setup + t_prop + cum[f] if f > 0 else np.nan for f in first])
         res = {"wall_s": wall, "setup_s": t_setup, "proposals_s": t_prop, "map_s": t_map, "svi_s": t_svi,
+               "map_batch": {"config": batch.config, "lenses_band": batch.info["lenses_band"],
+                             "lenses_fused": batch.info["lenses_fused"], "lenses_staged": batch.info["lenses_staged"],
+                             "launches_per_step": int(batch.info["lenses_band"] > 0) + batch.info["lenses_fused"] +
+                             batch.info["chunks"]},
                "map_step_s_mean": float(step_s.mean()), "svi_step_s": t_svi / n_svi,
                "fused_lenses": int(sum(batch.fused)), "staged_chunks": batch.num_chunks,
                "workspace_gib": batch.workspace_bytes / 2 ** 30,
@@ -583,6 +604,10 @@ def run_uneven(args, dev):
         del batch, svi
         return res
 
+    tuned = tune(lenses)
+    print(f"[uneven] autotuned: map {tuned['map']['config']}, svi {tuned['svi']['config']}", flush=True)
+    for k in range(min(args.uneven_seq_lenses, L)):
+        tune([lenses[k]])
     b = pipeline(lenses, "batched")
     seq_rows = []
     for k in range(min(args.uneven_seq_lenses, L)):
@@ -599,6 +624,7 @@ def run_uneven(args, dev):
                       "particles": total, "scale": args.uneven_scale},
            "batched": dict(b, latency_note="every lens's fit completes with the batch (wall_s); "
                                             "time_to_map_chi2_p50/p95 are per-lens"),
+           "autotune": tuned,
            "learned_training_s": t_train,
            "lenses_per_hour_incl_training": L / (b["wall_s"] + t_train) * 3600.0}
     if seq_rows:
@@ -625,6 +651,8 @@ def main():
     ap.add_argument("--uneven-scale", type=float, default=1.0)
     ap.add_argument("--uneven-workspace-gib", type=float, default=48.0)
     ap.add_argument("--uneven-seq-lenses", type=int, default=16)
+    ap.add_argument("--uneven-band", default="auto",
+                    help="lens-batch path: auto (autotuned), 1 (banded persistent kernel), 2, 0 (resident + staged)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-prior time-to-MAP evaluation")
     ap.add_argument("--learned-images", type=int, default=2 ** 20)
     ap.add_argument("--learned-lenses", type=int, default=256)

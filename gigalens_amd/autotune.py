@@ -1,4 +1,5 @@
-"""Autotuner: picks the fastest of the equivalent execution variants of a plan.
+"""Autotuner: picks the fastest of the equivalent execution variants of a plan,
+and the path and tile shape of a lens batch.
 
 Every variant computes the same model (results agree to FP32 rounding):
 
@@ -12,8 +13,15 @@ Every variant computes the same model (results agree to FP32 rounding):
 ``tune`` builds each candidate plan, warms it up and times ``log_prob`` with its
 gradient on a batch of the caller's z; the fastest configuration is kept per
 key (image size, supersampling, PSF size, profile kinds, likelihood, batch size)
-in memory and in ``~/.cache/gigalens_amd/autotune.json``.  No counterpart in the
-reference (TensorFlow picks its own kernels).
+in memory and in ``~/.cache/gigalens_amd/autotune.json``.
+
+``tune_batch`` does the same for a ``LensBatch``: every lens on the banded
+persistent kernel with bands of 64, 32 or 16 sub-rows (the band is the tile of
+the supersampled image a workgroup holds in LDS; taller bands mean fewer
+barriers and passes, shorter ones a smaller LDS footprint), only the lenses the
+resident-image kernel cannot hold on it, or none (resident-image and staged
+launches); the key is the batch's lens shapes and particle counts.  No
+counterpart in the reference (TensorFlow picks its own kernels).
 """
 from __future__ import annotations
 
@@ -108,6 +116,63 @@ def tune(phys_model, camera, layout, z: torch.Tensor, plan_kwargs: dict, repeats
         timings[name] = ms
         if ms < best_ms:
             best, best_ms = cand, ms
+    rec = {"best": best, "timings": timings}
+    _memory[key] = rec
+    if use_cache:
+        _store(key, rec)
+    return dict(best), timings
+
+
+def batch_candidates() -> List[dict]:
+    """Batch configurations ``tune_batch`` times (``gl_batch_opts`` fields)."""
+    return [{"band": 1, "band_rows": 0}, {"band": 1, "band_rows": 32}, {"band": 1, "band_rows": 16},
+            {"band": 2, "band_rows": 0}, {"band": 0, "band_rows": 0}]
+
+
+def _batch_key(batch) -> str:
+    import hashlib
+    h = hashlib.sha1()
+    for plan, n in zip(batch.plans, batch.counts):
+        i = plan.info()
+        h.update(f"{i['num_pix']},{i['supersample']},{i['kernel_size']},{n};".encode())
+    return f"batch-{batch.n_lenses}-{batch.total}-{h.hexdigest()[:16]}"
+
+
+def tune_batch(batch, z: torch.Tensor, repeats: int = 3, use_cache: bool = True) -> Tuple[dict, Dict[str, float]]:
+    """Fastest configuration of ``batch`` (a ``LensBatch``) for log_prob + gradient of
+    z (total, d); the batch is left in that configuration.  Returns (configuration,
+    timings ms by configuration)."""
+    key = _batch_key(batch)
+    if use_cache:
+        rec = _memory.get(key) or _load().get(key)
+        if rec is not None:
+            _memory[key] = rec
+            batch.configure(**rec["best"])
+            return dict(rec["best"]), dict(rec["timings"])
+    timings: Dict[str, float] = {}
+    best: Optional[dict] = None
+    best_ms = float("inf")
+    out = None
+    for cand in batch_candidates():
+        try:
+            batch.configure(**cand)
+        except nat.GlError:
+            continue
+        if out is None:
+            out = batch.log_prob(z, want_grad=True)
+        batch.log_prob(z, want_grad=True, out=out)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(repeats):
+            batch.log_prob(z, want_grad=True, out=out)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / repeats
+        timings[json.dumps(cand, sort_keys=True)] = ms
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    batch.configure(**best)
     rec = {"best": best, "timings": timings}
     _memory[key] = rec
     if use_cache:

@@ -109,13 +109,15 @@ class LensBatch:
     tf/model.py:53-86).  Particles are lens-major.  ``band`` 1 puts every lens the
     banded persistent kernel takes into one launch per call (2: only the lenses the
     resident-image kernel cannot hold; 0: none), with at most ``band_rows`` sub-rows
-    per band and ``band_lds_bytes`` of LDS per workgroup (0: defaults).  Other lenses
+    per band and ``band_lds_bytes`` of LDS per workgroup (0: defaults); ``"auto"``
+    times the candidate configurations on this batch and keeps the fastest
+    (``autotune.tune_batch``; ``tuning`` holds the timings).  Other lenses
     whose image fits the resident-image kernel run there (one fused launch each); the
     rest run the HBM-staged pipeline in chunks whose workspaces share an arena of at
     most ``workspace_gib``.  ``info`` reports where the lenses went."""
 
     def __init__(self, phys_model, prior, lenses: Sequence, background_rms: float, exp_time: float, device="cuda",
-                 workspace_gib: float = 32.0, fused: bool = True, band: int = 0, band_rows: int = 0,
+                 workspace_gib: float = 32.0, fused: bool = True, band=0, band_rows: int = 0,
                  band_lds_bytes: int = 0):
         self.device = torch.device(device)
         self.prior = prior
@@ -132,27 +134,44 @@ class LensBatch:
             self.counts.append(int(n))
         self.n_lenses = len(self.plans)
         self.fused = [p.info()["lds_bytes"] > 0 for p in self.plans]
-        handles = (ctypes.c_void_p * self.n_lenses)(*[p._h.value for p in self.plans])
-        counts = (ctypes.c_int * self.n_lenses)(*self.counts)
-        h = ctypes.c_void_p()
-        lib = nat.lib()
-        opts = nat.BatchOpts(workspace_bytes=int(workspace_gib * (1 << 30)), band=int(band), band_rows=int(band_rows),
-                             band_lds_bytes=int(band_lds_bytes))
-        nat.check(lib.gl_batch_create_ex(handles, self.n_lenses, counts, ctypes.byref(opts), ctypes.byref(h)),
-                  "gl_batch_create_ex")
-        self._h = h
-        self._lib = lib
-        self.total = int(lib.gl_batch_size(h))
-        info = nat.BatchInfo()
-        nat.check(lib.gl_batch_query(h, ctypes.byref(info)), "gl_batch_query")
-        self.info = {name: getattr(info, name) for name, _ in nat.BatchInfo._fields_}
-        self.workspace_bytes = int(lib.gl_batch_workspace_bytes(h))
-        self.num_chunks = int(lib.gl_batch_num_chunks(h))
+        self._lib = nat.lib()
+        self._h = None
+        self.workspace_gib = float(workspace_gib)
+        self.total = int(sum(self.counts))
         self.first = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)
         self.lens = torch.arange(self.n_lenses, device=self.device, dtype=torch.int32).repeat_interleave(
             torch.as_tensor(self.counts, device=self.device))
         self.lp_history = None
         self.chi2_history = None
+        self.tuning = None
+        if band == "auto":
+            from .autotune import tune_batch
+            z = self.bij.inverse(prior.sample(self.total, seed=12345)).to(self.device, torch.float32).contiguous()
+            self.tuning = tune_batch(self, z)
+        else:
+            self.configure(int(band), int(band_rows), int(band_lds_bytes))
+
+    def configure(self, band: int = 0, band_rows: int = 0, band_lds_bytes: int = 0) -> None:
+        """(Re)build the device batch over the same plans with these ``gl_batch_opts``."""
+        lib = self._lib
+        if self._h is not None and self._h.value:
+            lib.gl_batch_destroy(self._h)
+            self._h = None
+        handles = (ctypes.c_void_p * self.n_lenses)(*[p._h.value for p in self.plans])
+        counts = (ctypes.c_int * self.n_lenses)(*self.counts)
+        h = ctypes.c_void_p()
+        opts = nat.BatchOpts(workspace_bytes=int(self.workspace_gib * (1 << 30)), band=int(band),
+                             band_rows=int(band_rows), band_lds_bytes=int(band_lds_bytes))
+        nat.check(lib.gl_batch_create_ex(handles, self.n_lenses, counts, ctypes.byref(opts), ctypes.byref(h)),
+                  "gl_batch_create_ex")
+        self._h = h
+        assert int(lib.gl_batch_size(h)) == self.total
+        info = nat.BatchInfo()
+        nat.check(lib.gl_batch_query(h, ctypes.byref(info)), "gl_batch_query")
+        self.info = {name: getattr(info, name) for name, _ in nat.BatchInfo._fields_}
+        self.config = {"band": int(band), "band_rows": int(band_rows), "band_lds_bytes": int(band_lds_bytes)}
+        self.workspace_bytes = int(lib.gl_batch_workspace_bytes(h))
+        self.num_chunks = int(lib.gl_batch_num_chunks(h))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -10,6 +10,8 @@ uneven batch in one launch.
 * Results do not depend on the band height; repeated launches (the device work queue
   is reset by its last workgroup) and a HIP graph of MAP steps reproduce the eager
   launches bit for bit; a batch with fewer particles than workgroups works.
+* The batch autotuner times every candidate configuration and leaves the batch in
+  the fastest, which computes what the resident-image and staged paths compute.
 """
 import numpy as np
 import pytest
@@ -17,6 +19,7 @@ import torch
 
 from tests import common as C
 from gigalens_amd import SimulatorConfig
+from gigalens_amd import autotune
 from gigalens_amd import psf
 from gigalens_amd import workloads as wl
 from gigalens_amd.batch import LensBatch
@@ -141,3 +144,24 @@ def test_band_graph_and_small_batch(cuda):
     scale = g_b.abs().max(dim=1, keepdim=True).values
     assert torch.allclose(lp_m, lp_b, rtol=2e-6, atol=0)
     assert ((g_m - g_b).abs() <= 1e-4 * scale + 1e-5 * g_b.abs()).all()
+
+
+def test_batch_autotune(cuda, tmp_path, monkeypatch):
+    monkeypatch.setattr(autotune, "CACHE_PATH", str(tmp_path / "autotune.json"))
+    autotune._memory.clear()
+    prior, phys, lenses, ocams = _lenses([(48, 2, 40), (100, 4, 30), (64, 3, 50)], seed0=5000)
+    b = LensBatch(phys, prior, lenses, 0.2, 100.0, device=cuda, band="auto")
+    best, timings = b.tuning
+    _print(f"batch autotune: best {best}, timings (ms) {timings}")
+    assert best in autotune.batch_candidates() and len(timings) == len(autotune.batch_candidates())
+    assert b.config["band"] == best["band"] and b.config["band_rows"] == best["band_rows"]
+    z = b.bij.inverse(prior.sample(b.total, seed=21)).to(cuda).float().contiguous()
+    lp, _, g = b.log_prob(z, want_grad=True)
+    ref = LensBatch(phys, prior, lenses, 0.2, 100.0, device=cuda, band=0)
+    lp0, _, g0 = ref.log_prob(z, want_grad=True)
+    scale = g0.abs().max(dim=1, keepdim=True).values
+    assert torch.allclose(lp, lp0, rtol=1e-5, atol=0)
+    assert ((g - g0).abs() <= 1e-3 * scale + 1e-4 * g0.abs()).all()
+    # a second batch of the same shapes takes the cached choice without timing
+    again = LensBatch(phys, prior, lenses, 0.2, 100.0, device=cuda, band="auto")
+    assert again.config == b.config and again.tuning[0] == best
