@@ -526,7 +526,7 @@ def run_uneven(args, dev):
     prior, phys, lenses = uneven_lenses(args, dev)
     L = len(lenses)
     total = sum(c for *_, c in lenses)
-    n_map, n_svi, n_vi = 350, 1500, 64
+    n_map, n_svi, n_vi = args.uneven_map_steps, args.uneven_svi_steps, 64
     # learned initialiser (trained once; its time is reported apart)
     init = LearnedInitializer(phys, wl.epl_shear_config(), prior, 0.2, 100.0, max_batch=1024, seed=0, device=dev)
     torch.cuda.synchronize()
@@ -604,23 +604,28 @@ def run_uneven(args, dev):
         del batch, svi
         return res
 
+    t0 = time.perf_counter()
     tuned = tune(lenses)
-    print(f"[uneven] autotuned: map {tuned['map']['config']}, svi {tuned['svi']['config']}", flush=True)
+    print(f"[uneven] autotuned in {time.perf_counter() - t0:.1f} s: map {tuned['map']['config']}, "
+          f"svi {tuned['svi']['config']}", flush=True)
     for k in range(min(args.uneven_seq_lenses, L)):
-        tune([lenses[k]])
+        t0 = time.perf_counter()
+        tk = tune([lenses[k]])
+        print(f"[uneven] lens {k} autotuned in {time.perf_counter() - t0:.1f} s: map {tk['map']['config']}, "
+              f"svi {tk['svi']['config']}", flush=True)
     b = pipeline(lenses, "batched")
     seq_rows = []
     for k in range(min(args.uneven_seq_lenses, L)):
         r = pipeline([lenses[k]], f"sequential lens {k}")
         seq_rows.append(r["wall_s"])
-    out = {"metric": "lenses/hour (uneven batch, learned-init MAP 350 + diagonal SVI 1500 steps)",
+    out = {"metric": f"lenses/hour (uneven batch, learned-init MAP {n_map} + diagonal SVI {n_svi} steps)",
            "value": L / b["wall_s"] * 3600.0, "unit": "lenses/hour", "n_gpus": 1, "steps": n_map + n_svi,
            "warmup": 0, "ms_per_step": b["map_step_s_mean"] * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (prior draws, Poisson + Gaussian noise; lens seeds 2000.., noise seeds 3000..)",
            "config": {"workload": f"{L} lenses, N in {{48,64,80,100,128}}, ss 2-4, particles U{{256..4096}} x "
-                                  f"{args.uneven_scale}, 11x11 Gaussian PSF; learned-init MAP 350 steps + diagonal "
-                                  f"SVI 1500 steps x 64 samples, every step measured", "lenses": L,
+                                  f"{args.uneven_scale}, 11x11 Gaussian PSF; learned-init MAP {n_map} steps + diagonal "
+                                  f"SVI {n_svi} steps x 64 samples, every step measured", "lenses": L,
                       "particles": total, "scale": args.uneven_scale},
            "batched": dict(b, latency_note="every lens's fit completes with the batch (wall_s); "
                                             "time_to_map_chi2_p50/p95 are per-lens"),
@@ -651,6 +656,8 @@ def main():
     ap.add_argument("--uneven-scale", type=float, default=1.0)
     ap.add_argument("--uneven-workspace-gib", type=float, default=48.0)
     ap.add_argument("--uneven-seq-lenses", type=int, default=16)
+    ap.add_argument("--uneven-map-steps", type=int, default=350, help="MAP steps (the configuration: 350)")
+    ap.add_argument("--uneven-svi-steps", type=int, default=1500, help="SVI steps (the configuration: 1500)")
     ap.add_argument("--uneven-band", default="auto",
                     help="lens-batch path: auto (autotuned), 1 (banded persistent kernel), 2, 0 (resident + staged)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-prior time-to-MAP evaluation")

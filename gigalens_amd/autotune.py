@@ -138,7 +138,7 @@ def _batch_key(batch) -> str:
     return f"batch-{batch.n_lenses}-{batch.total}-{h.hexdigest()[:16]}"
 
 
-def tune_batch(batch, z: torch.Tensor, repeats: int = 3, use_cache: bool = True) -> Tuple[dict, Dict[str, float]]:
+def tune_batch(batch, z: torch.Tensor, repeats: int = 2, use_cache: bool = True) -> Tuple[dict, Dict[str, float]]:
     """Fastest configuration of ``batch`` (a ``LensBatch``) for log_prob + gradient of
     z (total, d); the batch is left in that configuration.  Returns (configuration,
     timings ms by configuration)."""

@@ -2,11 +2,11 @@
 // lens system of a batch -- lenses of different image sizes, supersampling,
 // PSFs and particle counts -- in one launch.
 //
-// The grid is as many workgroups as the CUs hold at once.  Each workgroup takes
-// (lens, particle) items from a device work queue (one atomic per item; lenses
-// in descending order of cost, so the longest items start first and the launch
-// ends on short ones) until the queue is drained, and runs the whole
-// log-probability (and gradient, and MAP step) of the item:
+// The grid is as many workgroups as the CUs hold at once.  The (lens, particle)
+// items, lenses in descending order of cost, are dealt round-robin over the
+// workgroups (the longest items start first and every workgroup ends on short
+// ones; hundreds of items per workgroup even out the rest), and each workgroup
+// runs the whole log-probability (and gradient, and MAP step) of its items:
 //
 //   A  prologue   z -> x, log prior, component constants, EPL series tables (LDS)
 //   B  ray-trace  per band of BY sub-rows: lens light(theta) + source(beta), NaN -> 0,
@@ -25,9 +25,8 @@
 // Only one band of the supersampled image is ever in LDS, so an image of any size
 // runs at the LDS footprint of the band; the separable (rank <= 8) folded PSF keeps
 // the HBM intermediates at R x nsub x N floats per particle, written and read by
-// the same workgroup while they are in L2.  The last workgroup to leave resets
-// the queue for the next launch (no host work between launches: HIP graphs can
-// record it).
+// the same workgroup while they are in L2.  A launch keeps no state between
+// calls: HIP graphs can record it.
 //
 // Reference semantics as the resident-image kernel (lens_kernels.hip):
 // src/gigalens/tf/simulator.py:57-97, src/gigalens/tf/model.py:53-86,
@@ -61,7 +60,6 @@ struct BArgs {
     int n, total;      // lenses, items (particles)
     float* scratch;    // slot floats per workgroup
     long slot;
-    int* queue;        // [0] next item, [1] workgroups done; both 0 between launches
     int off_x, off_c, off_coef, off_red, off_lens, off_band;  // LDS layout (floats)
 };
 
@@ -101,7 +99,7 @@ template <int BLOCK, class M, bool GRAD>
 __device__ __forceinline__ void band_particle(const BArgs& A, const BLens* Lp, int pid, float* s_x, float* s_c,
                                               float* s_coef, float* s_red, float* s_band, float* slot, int tid,
                                               BLens* s_lens) {
-    // its previous reader, the last particle's epilogue, ended before the item barrier
+    // its previous reader, the last item's epilogue, ended at the barrier after it
     static_assert(sizeof(BLens) % 4 == 0, "lens entry copied as 32-bit words");
     for (int i = tid; i < (int)(sizeof(BLens) / 4); i += BLOCK)
         reinterpret_cast<int*>(s_lens)[i] = reinterpret_cast<const int*>(Lp)[i];
@@ -250,38 +248,29 @@ template <int BLOCK, class M, bool GRAD>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void band_kernel(const BArgs A) {
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
-    __shared__ int s_item;
     const int tid = threadIdx.x;
     float* slot = A.scratch + (long)blockIdx.x * A.slot;
-    for (;;) {
-        if (tid == 0) s_item = atomicAdd(A.queue, 1);
-        __syncthreads();
-        // every thread reads s_item before thread 0 writes it again: band_particle has barriers
-        const int item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item >= A.total) break;  // the queue is drained: every wave of the workgroup leaves
+    // items in processing order (lenses longest first), dealt round-robin: workgroup b
+    // takes items b, b + grid, ...; the loop bound is the same for every thread of the
+    // workgroup, so every barrier of an item is reached by all of its waves
+    typedef const int __attribute__((address_space(4))) CInt;
+    const CInt* pre = (const CInt*)A.pre;
+    const CInt* order = (const CInt*)A.order;
+    for (int item = blockIdx.x; item < A.total; item += gridDim.x) {
         // the tables are read-only for the launch: addressed in the constant space, their
         // fields come in through scalar loads
-        typedef const int __attribute__((address_space(4))) CInt;
-        typedef const BLens __attribute__((address_space(4))) CLens;
-        const CInt* pre = (const CInt*)A.pre;
         int lo = 0, hi = A.n;  // pre[lo] <= item < pre[hi]
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (pre[mid] <= item) lo = mid;
             else hi = mid;
         }
-        const BLens* L = A.lens + ((const CInt*)A.order)[lo];
-        const int pid = __builtin_amdgcn_readfirstlane(((const CLens*)A.lens)[((const CInt*)A.order)[lo]].first) +
-                        item - pre[lo];
+        const BLens* L = A.lens + order[lo];
+        const int pid = ((const CLensT*)A.lens)[order[lo]].first + item - pre[lo];
         band_particle<BLOCK, M, GRAD>(A, L, pid, smem + A.off_x, smem + A.off_c, smem + A.off_coef,
                                       smem + A.off_red, smem + A.off_band, slot, tid,
                                       reinterpret_cast<BLens*>(smem + A.off_lens));
-    }
-    // every workgroup has taken its last item once the done count reaches the grid size:
-    // the last one clears the queue for the next launch on the stream
-    if (tid == 0 && atomicAdd(A.queue + 1, 1) == (int)gridDim.x - 1) {
-        atomicExch(A.queue, 0);
-        atomicExch(A.queue + 1, 0);
+        __syncthreads();  // the next item's copy of its lens entry overwrites this one's
     }
 }
 

@@ -1369,9 +1369,6 @@ static int band_setup(gl_batch* b, const gl_plan* const* plans, const int* count
     int rc;
     if ((rc = dev_alloc(b->dev, B.scratch_bytes, &dptr, false))) return rc;
     A.scratch = (float*)dptr;
-    if ((rc = dev_alloc(b->dev, 2 * sizeof(int), &dptr, false))) return rc;
-    A.queue = (int*)dptr;
-    GL_HIP(hipMemset(A.queue, 0, 2 * sizeof(int)));
     auto put = [&](const void* host, size_t bytes, void** out) -> int {
         int r = dev_alloc(b->dev, bytes, out, false);
         if (r) return r;

@@ -6,8 +6,9 @@
  * (row-major float32 unless stated), every call is stream-ordered and
  * asynchronous.  Only the create / destroy calls allocate or free device memory
  * and synchronise the host: gl_plan_create, gl_plan_destroy, gl_batch_create,
- * gl_batch_create_ws, gl_batch_destroy, gl_cnn_create, gl_cnn_destroy.  gl_cnn_set_params clears the
- * AdamW moments with hipMemsetAsync and must not be recorded into a HIP graph.
+ * gl_batch_create_ws, gl_batch_create_ex, gl_batch_destroy, gl_cnn_create, gl_cnn_destroy.
+ * gl_cnn_set_params clears the AdamW moments with hipMemsetAsync and must not be recorded
+ * into a HIP graph.
  * Errors are reported as a non-zero return code; gl_last_error() returns a
  * thread-local message.
  * A plan, a batch and a network own scratch memory (the staged pipeline's
@@ -241,7 +242,7 @@ int gl_batch_create_ws(const gl_plan* const* plans, int n_lenses, const int* cou
                        gl_batch** out);
 /* Options of gl_batch_create_ex.  band selects the banded persistent kernel: every
  * lens on it runs in ONE launch per call -- workgroups as many as the CUs hold at
- * once take (lens, particle) items from a device queue, longest lenses first, and
+ * once take the (lens, particle) items round-robin, longest lenses first, and
  * stream each particle's supersampled image through LDS in bands of sub-rows, so
  * images of any size share the launch.  It takes lenses of the profile combinations
  * it is instantiated for (EPL + shear with a Sersic ellipse source, with or without a

@@ -165,3 +165,15 @@ def test_batch_autotune(cuda, tmp_path, monkeypatch):
     # a second batch of the same shapes takes the cached choice without timing
     again = LensBatch(phys, prior, lenses, 0.2, 100.0, device=cuda, band="auto")
     assert again.config == b.config and again.tuning[0] == best
+
+
+def test_band_value_only_matches_gradient_launch(mixed, cuda):
+    """log_prob without the gradient (the variant without the adjoint phases) gives the
+    values of the gradient launch, bit for bit (the forward passes are the same code)."""
+    prior, phys, lenses, ocams, batch = mixed
+    z = batch.bij.inverse(prior.sample(batch.total, seed=13)).to(cuda).float().contiguous()
+    _print("value-only banded launch")
+    lp0, chi0, _ = batch.log_prob(z)
+    torch.cuda.synchronize()
+    lp1, chi1, _ = batch.log_prob(z, want_grad=True)
+    assert torch.equal(lp0, lp1) and torch.equal(chi0, chi1)
