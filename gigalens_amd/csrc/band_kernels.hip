@@ -250,13 +250,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     float* smem = reinterpret_cast<float*>(smem4);
     const int tid = threadIdx.x;
     float* slot = A.scratch + (long)blockIdx.x * A.slot;
-    // items in processing order (lenses longest first), dealt round-robin: workgroup b
-    // takes items b, b + grid, ...; the loop bound is the same for every thread of the
-    // workgroup, so every barrier of an item is reached by all of its waves
+    // items in processing order (lenses longest first), dealt round-robin in two halves
+    // that run in opposite directions: even workgroups take the even items from the
+    // front, odd ones the odd items from the back, so workgroups that share a CU are
+    // seldom in the same phase of equally long items (their latency-bound phases
+    // overlap the other's arithmetic) and each half still holds half of every lens.
+    // The loop bound is the same for every thread of the workgroup, so every barrier of
+    // an item is reached by all of its waves.
     typedef const int __attribute__((address_space(4))) CInt;
     const CInt* pre = (const CInt*)A.pre;
     const CInt* order = (const CInt*)A.order;
-    for (int item = blockIdx.x; item < A.total; item += gridDim.x) {
+    const int split = gridDim.x > 1 ? 1 : 0;     // one workgroup takes every item in order
+    const int half = split & blockIdx.x;
+    const int nwg = split ? (gridDim.x + 1 - half) >> 1 : 1;  // workgroups of this half
+    const int nitems = split ? (A.total + 1 - half) >> 1 : A.total;  // items of this half
+    for (int k = blockIdx.x >> split; k < nitems; k += nwg) {
+        const int item = !split ? k : half ? 2 * (nitems - 1 - k) + 1 : 2 * k;
         // the tables are read-only for the launch: addressed in the constant space, their
         // fields come in through scalar loads
         int lo = 0, hi = A.n;  // pre[lo] <= item < pre[hi]
