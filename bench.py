@@ -642,6 +642,89 @@ def run_uneven(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_uneven_step(args, dev):
+    """One MAP step of every lens of the uneven configuration (``uneven_lenses``: 128
+    lenses, full particle counts) on the path the batch autotuner picks (or
+    ``--uneven-band``): W untimed steps, then K steps timed with HIP events on the
+    launch stream.  Counted work: the flop model of the implemented formulation
+    (``gigalens_amd/flops.py``) per particle with its own EPL series length, the
+    separable PSF passes at each lens's rank, summed over the batch."""
+    from gigalens_amd.batch import LensBatch
+    from gigalens_amd.simulator import Camera
+    prior, phys, lenses = uneven_lenses(args, dev)
+    band = args.uneven_band if args.uneven_band == "auto" else int(args.uneven_band)
+    batch = LensBatch(phys, prior, lenses, 0.2, 100.0, device=dev, workspace_gib=args.uneven_workspace_gib,
+                      band=band)
+    lib = batch._lib
+    z0 = batch.bij.inverse(prior.sample(batch.total, seed=7)).to(dev).float().contiguous()
+    z, m, v = z0.clone(), torch.zeros_like(z0), torch.zeros_like(z0)
+    lp = torch.empty(batch.total, device=dev)
+    chi2 = torch.empty(batch.total, device=dev)
+    from gigalens_amd.plan import _ptr, _stream
+    from gigalens_amd import _native as nat
+
+    def step(s):
+        nat.check(lib.gl_batch_map_adam_step(batch._h, _ptr(z), _ptr(m), _ptr(v), s, args.lr, 0.9, 0.999, 1e-7,
+                                             _ptr(lp), _ptr(chi2), _stream()), "gl_batch_map_adam_step")
+    for s in range(1, args.warmup + 1):
+        step(s)
+    torch.cuda.synchronize()
+    K = args.steps
+    stream = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record(stream)
+    for s in range(K):
+        step(args.warmup + s + 1)
+    b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = a.elapsed_time(b) / K
+    if args.copy_probe:
+        c0 = torch.empty(64 * 1024 * 1024, device=dev)
+        torch.empty_like(c0).copy_(c0)
+        torch.cuda.synchronize()
+    # counted work of one step
+    kinds, roles = fl.model_kinds(phys)
+    x0 = batch.bij.forward(z0.cpu())
+    nt = fl.series_length(x0[0][0]["e1"].numpy(), x0[0][0]["e2"].numpy(), 50)
+    flops, subpix = 0.0, 0.0
+    for (cfg, _, n), first in zip(lenses, batch.first):
+        cam = Camera(cfg)
+        k = cam.subgrid_kernel.shape[0]
+        ss = cfg.supersample
+        kf = k + ss - 1
+        W = np.zeros((kf, kf))
+        for aa in range(ss):
+            for bb in range(ss):
+                W[aa:aa + k, bb:bb + k] += cam.subgrid_kernel[::-1, ::-1]
+        sv = np.linalg.svd(W, compute_uv=False)
+        rank = int((sv >= 1e-6 * sv[0]).sum())
+        c = fl.per_particle(kinds, roles, cfg.num_pix, ss, k, nt[first:first + n], grad=True,
+                            sep_rank=rank if rank <= 8 else 0)
+        flops += float(c["flops"].sum())
+        subpix += float(n) * (ss * cfg.num_pix) ** 2
+    achieved = flops / (step_ms * 1e-3) / 1e12
+    out = {"metric": "sim+grad particle-evals/s (uneven lens batch, one MAP step)",
+           "value": batch.total / (step_ms * 1e-3), "unit": "particle-evals/s", "n_gpus": 1, "steps": K,
+           "warmup": args.warmup, "ms_per_step": step_ms, "wall_ms_per_step": wall / K * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (prior draws, Poisson + Gaussian noise; lens seeds 2000.., noise seeds 3000..)",
+           "config": {"workload": f"{len(lenses)} lenses, N in {{48,64,80,100,128}}, ss 2-4, particles U{{256..4096}} "
+                                  f"x {args.uneven_scale}, 11x11 Gaussian PSF; one Keras-Adam MAP step of every "
+                                  "particle", "lenses": len(lenses), "particles": batch.total,
+                      "path": batch.config, "batch": batch.info},
+           "sub_pixels_per_step": subpix, "sub_pixels_per_s": subpix / (step_ms * 1e-3),
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                        "kernel": "gl::band_kernel (one launch per step)" if batch.info["lenses_fused"] +
+                        batch.info["chunks"] == 0 else "all launches of the step",
+                        "fp32_flop_per_step": flops}}
+    if batch.tuning is not None:
+        out["autotune_ms"] = batch.tuning[1]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -651,7 +734,7 @@ def main():
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="also launch a 256 MiB device copy (PMC calibration)")
-    ap.add_argument("--workload", choices=["map", "shapelets", "cnn", "uneven"], default="map")
+    ap.add_argument("--workload", choices=["map", "shapelets", "cnn", "uneven", "uneven-step"], default="map")
     ap.add_argument("--uneven-lenses", type=int, default=128)
     ap.add_argument("--uneven-scale", type=float, default=1.0)
     ap.add_argument("--uneven-workspace-gib", type=float, default=48.0)
@@ -684,6 +767,10 @@ def main():
         return run_cnn(args, dev)
     if args.workload == "uneven":
         return run_uneven(args, dev)
+    if args.workload == "uneven-step":
+        if args.steps == 350:
+            args.steps = 10
+        return run_uneven_step(args, dev)
     bs = args.particles
     prior, phys, cfg, sim, pm, obs, z0 = setup(bs, dev)
     plan = pm.plan_for(sim)
