@@ -177,3 +177,86 @@ def test_band_value_only_matches_gradient_launch(mixed, cuda):
     torch.cuda.synchronize()
     lp1, chi1, _ = batch.log_prob(z, want_grad=True)
     assert torch.equal(lp0, lp1) and torch.equal(chi0, chi1)
+
+
+def test_band_source_only_model_vs_oracle(cuda):
+    """The second compiled model (EPL + shear, no lens light, Sersic ellipse source) on
+    the banded kernel: log_prob and gradient against the FP64 oracle."""
+    from gigalens_amd import distributions as tfd
+    from gigalens_amd.model import PhysicalModel
+    from gigalens_amd.profiles.light.sersic import SersicEllipse
+    from gigalens_amd.profiles.mass.epl import EPL
+    from gigalens_amd.profiles.mass.shear import Shear
+    full = wl.epl_shear_prior()
+    prior = tfd.JointDistributionSequential([full.model[0], full.model[2]])
+    phys = PhysicalModel([EPL(50), Shear()], [], [SersicEllipse()])
+    kernel = psf.gaussian(0.1, 0.065, 11)
+    lenses, ocams = [], []
+    for k, (n_pix, ss, n) in enumerate([(64, 3, 2), (100, 4, 2), (48, 2, 3)]):
+        cfg = SimulatorConfig(delta_pix=0.065, num_pix=n_pix, supersample=ss, kernel=kernel)
+        ocam = C.oracle_camera(cfg, Camera(cfg))
+        truth = C.draw_prior(prior, 1, seed=6000 + k).numpy().astype(np.float64)
+        clean = C.oracle_simulate(ocam, phys, C.nested_from_x(prior, truth))[0]
+        lenses.append((cfg, wl.noisy_observation(clean, 0.2, 100.0, seed=7000 + k), n))
+        ocams.append(ocam)
+    batch = LensBatch(phys, prior, lenses, 0.2, 100.0, device=cuda, band=1)
+    assert batch.info["lenses_band"] == 3
+    z = batch.bij.inverse(prior.sample(batch.total, seed=8)).to(cuda).float().contiguous()
+    lp, chi2, g = batch.log_prob(z, want_grad=True)
+    lp, g = lp.cpu().numpy(), g.cpu().numpy().astype(np.float64)
+    zz = z.cpu().numpy().astype(np.float64)
+    off = 0
+    for k, ((cfg, obs, n), ocam) in enumerate(zip(lenses, ocams)):
+        lp_ref, _, _ = C.oracle_log_prob(ocam, phys, prior, zz[off:off + n], obs, 0.2, 100.0)
+        g_fd = C.oracle_grad_richardson(ocam, phys, prior, zz[off:off + 1], obs, 0.2, 100.0)[0]
+        rel = np.abs(lp[off:off + n] - lp_ref) / np.abs(lp_ref)
+        e = np.abs(g[off] - g_fd)
+        _print(f"source-only lens {k} ({cfg.num_pix}px ss{cfg.supersample}): lp {rel.max():.2e}, "
+               f"grad {e.max() / np.abs(g_fd).max():.2e}")
+        assert rel.max() < 1e-4 and np.all(e <= 2e-3 * np.abs(g_fd) + 2e-4 * np.abs(g_fd).max())
+        off += n
+
+
+def test_band_backward_likelihood_matches_plans(cuda):
+    """A batch of plans with the observed-image noise model (GL_LIK_BACKWARD, a noise map)
+    on the banded kernel equals each plan's own launch (resident or staged)."""
+    import ctypes
+    from gigalens_amd import _native as nat
+    from gigalens_amd import distributions as tfd
+    from gigalens_amd.plan import Plan, _ptr, _stream, layout_from_paths
+    prior, phys, lenses, ocams = _lenses([(60, 2, 5), (128, 4, 3)], seed0=8000)
+    paths = [p for p, _ in tfd.leaves(prior)]
+    plans = []
+    for k, (cfg, obs, n) in enumerate(lenses):
+        nm = 0.2 + 0.05 * np.random.default_rng(k).random(obs.shape).astype(np.float32)
+        plans.append(Plan(phys, Camera(cfg), layout_from_paths(phys, paths), prior=prior, observed=obs,
+                          background_rms=0.2, exp_time=100.0, likelihood=nat.GL_LIK_BACKWARD, noise_map=nm))
+    counts = [n for *_, n in lenses]
+    total = sum(counts)
+    lib = nat.lib()
+    h = ctypes.c_void_p()
+    handles = (ctypes.c_void_p * 2)(*[p._h.value for p in plans])
+    cts = (ctypes.c_int * 2)(*counts)
+    nat.check(lib.gl_batch_create_ex(handles, 2, cts, ctypes.byref(nat.BatchOpts(0, 1, 0, 0)), ctypes.byref(h)),
+              "gl_batch_create_ex")
+    try:
+        info = nat.BatchInfo()
+        nat.check(lib.gl_batch_query(h, ctypes.byref(info)), "gl_batch_query")
+        assert info.lenses_band == 2
+        bij = tfd.ChainBijector(prior.experimental_default_event_space_bijector(), tfd.PackBijector(prior))
+        z = bij.inverse(prior.sample(total, seed=9)).to(cuda).float().contiguous()
+        lp = torch.empty(total, device=cuda)
+        chi2 = torch.empty(total, device=cuda)
+        g = torch.empty_like(z)
+        nat.check(lib.gl_batch_log_prob(h, _ptr(z), _ptr(lp), _ptr(chi2), _ptr(g), _stream()), "gl_batch_log_prob")
+        off = 0
+        for p, n in zip(plans, counts):
+            lp0, chi0, g0 = p.log_prob(z[off:off + n].contiguous(), want_grad=True)
+            scale = g0.abs().max(dim=1, keepdim=True).values
+            _print(f"backward likelihood, {p.N}px: lp {((lp[off:off + n] - lp0).abs() / lp0.abs()).max().item():.2e}")
+            assert torch.allclose(lp[off:off + n], lp0, rtol=1e-5, atol=0)
+            assert torch.allclose(chi2[off:off + n], chi0, rtol=1e-5, atol=1e-7)
+            assert ((g[off:off + n] - g0).abs() <= 1e-3 * scale + 1e-4 * g0.abs()).all()
+            off += n
+    finally:
+        lib.gl_batch_destroy(h)
