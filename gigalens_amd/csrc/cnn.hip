@@ -18,10 +18,18 @@ using glc::StreamGuard;
 // ---------------------------------------------------------------------------
 // Learned initialiser (no reference counterpart)
 // ---------------------------------------------------------------------------
+// One wave-streaming convolution pass (gl::cnn::conv_wave): its kernel instance, LDS
+// bytes (the weight tiles) and the most workgroups that are resident at once.
+struct WavePass {
+    const void* fn;
+    int kind, nmt, nk, lds, grid;
+};
+
 struct gl_cnn {
     struct Layer {
         int cin, cout, s, H, W, Ho, Wo, kp;
-        int rf, rd, rw;            // band rows of the forward, data- and weight-gradient passes
+        int rw;                    // band rows of the weight-gradient pass
+        WavePass fwd, dgrad;       // wave-streaming forward and data-gradient passes
         int wblk, wncg, wnp, wchunks;  // weight-gradient grid: blocks over images, tile chunks
         size_t woff, boff;         // offsets in the parameter vector
         gl::cnn::u16* wt;          // [cout][kp]
@@ -61,19 +69,9 @@ static int cnn_alloc(gl_cnn* net, size_t bytes, void** out) {
 
 // LDS budget of one band workgroup: two fit in a CU's 160 KB
 constexpr size_t kCnnLds = 80 * 1024;
+// zeros after each convolution operand tensor: at least the most channels of a layer
+constexpr size_t kZeroTail = gl::cnn::kHeadMaxC;
 
-static int dgrad_groups(const gl_cnn::Layer& L, int R);
-// band + weight tile + output staging (one pass of at most 16 pixel groups)
-static size_t fwd_lds(const gl_cnn::Layer& L, int R) {
-    const size_t slots = (size_t)std::min(gl::cnn::kGroupsPerPass, (R * L.Wo + 15) / 16) * 16;
-    return ((size_t)(L.s * (R - 1) + 3) * (L.W + 2) * (L.cin + 8) + (size_t)9 * 32 * (L.cin + 8)) * 2 +
-           slots * gl::cnn::kStageF * 2;
-}
-static size_t dgrad_lds(const gl_cnn::Layer& L, int R) {
-    const size_t slots = (size_t)std::min(gl::cnn::kGroupsPerPass, dgrad_groups(L, R)) * 16;
-    return ((size_t)((R + L.s) / L.s + 1) * (L.Wo + 2) * (L.cout + 8) + (size_t)9 * 32 * (L.cout + 8)) * 2 +
-           slots * gl::cnn::kStageD * 4;
-}
 // first layer: rows of a forward band (about one pixel per thread)
 static int conv1_rows(const gl_cnn::Layer& L) { return std::max(1, std::min(L.H, (256 + L.W - 1) / L.W)); }
 static size_t conv1_fwd_lds(const gl_cnn::Layer& L) {
@@ -82,22 +80,6 @@ static size_t conv1_fwd_lds(const gl_cnn::Layer& L) {
 static size_t conv1_wgrad_lds(const gl_cnn::Layer& L) { return (size_t)(L.H + 2) * (L.W + 2) * 4; }
 static size_t wgrad_lds(const gl_cnn::Layer& L, int R) {
     return ((size_t)R * L.Wo * (L.cout + 8) + (size_t)(L.s * (R - 1) + 3) * (L.W + 2) * (L.cin + 8)) * 2;
-}
-// pixel groups of 16 of a data-gradient band (worst case over the band's row parity)
-static int dgrad_groups(const gl_cnn::Layer& L, int R) {
-    int worst = 0;
-    for (int h0 = 0; h0 < L.s; ++h0) {
-        int g = 0;
-        for (int c = 0; c < L.s * L.s; ++c) {
-            const int ph = c / L.s, pw = c % L.s;
-            const int fr = ((ph - h0) % L.s + L.s) % L.s;
-            const int nr = fr < R ? (R - fr + L.s - 1) / L.s : 0;
-            const int nc = (L.W - pw + L.s - 1) / L.s;
-            g += (nr * nc + 15) / 16;
-        }
-        worst = std::max(worst, g);
-    }
-    return worst;
 }
 // largest band (rows <= n) meeting the budget, then evened out over the bands
 static int pick_rows(int n, bool (*ok)(const gl_cnn::Layer&, int), const gl_cnn::Layer& L) {
@@ -108,34 +90,55 @@ static int pick_rows(int n, bool (*ok)(const gl_cnn::Layer&, int), const gl_cnn:
     const int bands = (n + R - 1) / R;
     return (n + bands - 1) / bands;
 }
-// 16-byte vectors of a band: the register prefetch holds at most kPre per thread
-constexpr int kBandVecMax = gl::cnn::kPre * gl::cnn::kWaves * 64;
-static bool fwd_ok(const gl_cnn::Layer& L, int R) {
-    return fwd_lds(L, R) <= kCnnLds && (L.s * (R - 1) + 3) * (L.W + 2) * (L.cin / 8) <= kBandVecMax;
+// Wave-streaming pass of K operand channels into Nout output channels: 64-channel
+// output tiles when the weights of all K channels fit 72 KB (two workgroups per CU),
+// 32-channel tiles otherwise (at most 144 KB for K = 256).  K steps of 64 channels
+// only for the stride-2 data gradient (at most 4 taps per group); the 9-tap passes
+// take 32-channel steps, which keeps them at 128 VGPRs (two workgroups per CU).
+typedef void (*WaveKernelFn)(gl::cnn::WaveArgs);
+static WaveKernelFn wave_fn(int kind, int s, int nmt, int nk) {
+    using gl::cnn::conv_wave;
+    if (kind == 1 && s == 2) {
+        static const WaveKernelFn tab[2][2] = {{conv_wave<1, 2, 2, 1>, conv_wave<1, 2, 2, 2>},
+                                               {conv_wave<1, 2, 4, 1>, conv_wave<1, 2, 4, 2>}};
+        return tab[nmt == 4][nk == 2];
+    }
+    static const WaveKernelFn tab[3][2] = {{conv_wave<0, 1, 2, 1>, conv_wave<0, 1, 4, 1>},
+                                           {conv_wave<0, 2, 2, 1>, conv_wave<0, 2, 4, 1>},
+                                           {conv_wave<1, 1, 2, 1>, conv_wave<1, 1, 4, 1>}};
+    return tab[kind == 1 ? 2 : s - 1][nmt == 4];
 }
-static bool dgrad_ok(const gl_cnn::Layer& L, int R) {
-    return dgrad_lds(L, R) <= kCnnLds && ((R + L.s) / L.s + 1) * (L.Wo + 2) * (L.cout / 8) <= kBandVecMax;
+static hipError_t wave_setup(WavePass& P, int kind, int s, int K, int Nout, int cus) {
+    P.kind = kind;
+    P.nmt = (Nout % 64 == 0 && gl::cnn::wave_lds(K, 4) <= 72 * 1024) ? 4 : 2;
+    P.nk = (kind == 1 && s == 2 && K % 64 == 0) ? 2 : 1;
+    P.lds = gl::cnn::wave_lds(K, P.nmt);
+    P.fn = (const void*)wave_fn(kind, s, P.nmt, P.nk);
+    hipError_t e = hipFuncSetAttribute(P.fn, hipFuncAttributeMaxDynamicSharedMemorySize, P.lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, P.fn, 512, P.lds);
+    if (e != hipSuccess) return e;
+    P.grid = std::max(1, per_cu) * cus;
+    return hipSuccess;
 }
-// persistent grid: every CU holds as many band workgroups as its 160 KB of LDS allows
-static unsigned persistent_grid(int items, size_t lds, int cus) {
-    const long per_cu = std::max<long>(1, (long)(160 * 1024) / (long)std::max<size_t>(lds, 1));
-    return (unsigned)std::min<long>(items, per_cu * cus);
+// Items of R output rows of one image: bands short enough that the batch makes about
+// four items per resident workgroup.
+static int wave_launch(const WavePass& P, gl::cnn::WaveArgs A, hipStream_t st) {
+    const long target = 4L * P.grid;
+    const int nb = (int)std::min<long>(A.Ho, std::max<long>(1, (target + A.B - 1) / A.B));
+    A.R = (A.Ho + nb - 1) / nb;
+    const long items = (long)A.B * ((A.Ho + A.R - 1) / A.R);
+    const dim3 grid((unsigned)std::min<long>(items, P.grid));
+    void* args[] = {&A};
+    GL_HIP(hipLaunchKernel(P.fn, grid, dim3(512), args, (size_t)P.lds, st));
+    return GL_OK;
 }
 static bool wgrad_ok(const gl_cnn::Layer& L, int R) { return wgrad_lds(L, R) <= kCnnLds; }
 
 static hipError_t cnn_lds_attrs() {
     const int lds = (int)kCnnLds;
     hipError_t e;
-    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_fwd<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds)))
-        return e;
-    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_fwd<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds)))
-        return e;
-    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_dgrad<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 lds)))
-        return e;
-    if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_dgrad<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 lds)))
-        return e;
     if ((e = hipFuncSetAttribute((const void*)&gl::cnn::conv_wgrad<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  lds)))
         return e;
@@ -182,20 +185,24 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
         L.H = H; L.W = W;
         L.Ho = (H - 1) / L.s + 1; L.Wo = (W - 1) / L.s + 1;
         L.kp = (9 * cin + 31) / 32 * 32;
-        L.rf = L.rd = L.rw = 1;
+        L.rw = 1;
+        L.fwd = L.dgrad = WavePass{nullptr, 0, 0, 0, 0, 0};
         L.wblk = L.wncg = L.wnp = L.wchunks = 0;
         if (l == 0 && (conv1_fwd_lds(L) > kCnnLds || conv1_wgrad_lds(L) > kCnnLds)) {
             gl_cnn_destroy(net);
             return fail(GL_ERR_UNSUPPORTED, "gl_cnn_create: image too large for the first layer's LDS tiles");
         }
         if (l > 0) {
-            L.rf = pick_rows(L.Ho, fwd_ok, L);
-            L.rd = pick_rows(L.H, dgrad_ok, L);
             L.rw = pick_rows(L.Ho, wgrad_ok, L);
-            if (!L.rf || !L.rd || !L.rw) {
+            if (!L.rw) {
                 gl_cnn_destroy(net);
                 return fail(GL_ERR_UNSUPPORTED, "gl_cnn_create: image rows too wide for the LDS band of layer " +
                                                     std::to_string(l + 1));
+            }
+            if (wave_setup(L.fwd, 0, L.s, cin, L.cout, net->cus) != hipSuccess ||
+                wave_setup(L.dgrad, 1, L.s, L.cout, cin, net->cus) != hipSuccess) {
+                gl_cnn_destroy(net);
+                return fail(GL_ERR_HIP, "gl_cnn_create: convolution kernel attributes");
             }
             L.wblk = std::min(desc->max_batch, 512);
             L.wncg = std::min(L.cout / 16, 4);
@@ -207,6 +214,12 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
         L.boff = off; off += L.cout;
         H = L.Ho; W = L.Wo; cin = L.cout;
     }
+    // the convolution passes index activations with 32-bit element offsets
+    for (int l = 0; l < 4; ++l)
+        if ((size_t)net->maxb * net->L[l].H * net->L[l].W * std::max(net->L[l].cin, net->L[l].cout) >= (1u << 31)) {
+            gl_cnn_destroy(net);
+            return fail(GL_ERR_UNSUPPORTED, "gl_cnn_create: max_batch x height x width x channels must stay below 2^31");
+        }
     const int c4 = net->L[3].cout, Hd = net->hidden, d2 = 2 * net->d;
     net->w1 = off; off += (size_t)Hd * c4;
     net->b1 = off; off += Hd;
@@ -231,9 +244,11 @@ extern "C" int gl_cnn_create(const gl_cnn_desc* desc, gl_cnn** out) {
         CNN_ALLOC(L.wt, gl::cnn::u16, (size_t)L.cout * L.kp);
         L.w2t = nullptr;
         if (l > 0) { CNN_ALLOC(L.w2t, gl::cnn::u16, (size_t)L.cin * 9 * L.cout); }
-        CNN_ALLOC(L.Y, gl::cnn::u16, act);
+        // Y and dZ, the operands of the convolution passes, end in kZeroTail zeros that
+        // out-of-image taps read (never written: every pass writes the first bs images)
+        CNN_ALLOC(L.Y, gl::cnn::u16, act + kZeroTail);
         CNN_ALLOC(L.Z, gl::cnn::u16, act);
-        CNN_ALLOC(L.dZ, gl::cnn::u16, act);
+        CNN_ALLOC(L.dZ, gl::cnn::u16, act + kZeroTail);
         const size_t sp = l > 0 ? (size_t)L.wblk : B;
         part = std::max(part, sp * L.cout * (9 * L.cin + 1));
     }
@@ -309,11 +324,11 @@ static int cnn_forward_layers(gl_cnn* net, const float* images, int bs, hipStrea
     }
     for (int l = 1; l < 4; ++l) {
         const gl_cnn::Layer& L = net->L[l];
-        FwdArgs A{net->L[l - 1].Y, L.wt, net->params + L.boff, L.Z, L.Y, bs, L.H, L.W, L.cin, L.cout, L.Ho, L.Wo, L.rf};
-        const size_t lds = fwd_lds(L, L.rf);
-        const dim3 grid(persistent_grid(bs * ((L.Ho + L.rf - 1) / L.rf), lds, net->cus));
-        if (L.s == 1) hipLaunchKernelGGL(conv_fwd<1>, grid, dim3(512), lds, st, A);
-        else hipLaunchKernelGGL(conv_fwd<2>, grid, dim3(512), lds, st, A);
+        gl::cnn::WaveArgs A{net->L[l - 1].Y, L.wt, net->params + L.boff, nullptr, L.Z, L.Y,
+                            bs, L.Ho, L.Wo, L.H, L.W, L.cin, L.cout, 0,
+                            (unsigned)((size_t)net->maxb * L.H * L.W * L.cin)};
+        int rc;
+        if ((rc = wave_launch(L.fwd, A, st))) return rc;
         GL_HIP(hipGetLastError());
     }
     return GL_OK;
@@ -381,12 +396,9 @@ extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* t
         if (l == 0) break;
         // data gradient into the previous layer: dZprev = conv^T(dZ, W) * gelu'(Zprev)
         const gl_cnn::Layer& Lp = net->L[l - 1];
-        DgradArgs A{L.dZ, L.w2t, Lp.Z, Lp.dZ, bs, L.H, L.W, L.cin, L.cout, L.Ho, L.Wo, L.rd};
-        const size_t lds = dgrad_lds(L, L.rd);
-        const dim3 grid(persistent_grid(bs * ((L.H + L.rd - 1) / L.rd), lds, net->cus));
-        if (L.s == 1) hipLaunchKernelGGL(conv_dgrad<1>, grid, dim3(512), lds, stream, A);
-        else hipLaunchKernelGGL(conv_dgrad<2>, grid, dim3(512), lds, stream, A);
-        GL_HIP(hipGetLastError());
+        gl::cnn::WaveArgs A{L.dZ, L.w2t, nullptr, Lp.Z, Lp.dZ, nullptr, bs, L.H, L.W, L.Ho, L.Wo, L.cout, L.cin, 0,
+                            (unsigned)((size_t)net->maxb * L.Ho * L.Wo * L.cout)};
+        if ((rc = wave_launch(L.dgrad, A, stream))) return rc;
     }
     if (grad) GL_HIP(hipMemcpyAsync(grad, net->grad, (size_t)net->P * 4, hipMemcpyDeviceToDevice, stream));
     return GL_OK;

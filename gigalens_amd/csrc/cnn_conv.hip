@@ -1,14 +1,9 @@
-// Convolution passes of the learned initialiser's network as band-tiled direct
-// convolutions.  A workgroup takes one image and a band of rows, copies the band of
-// its input (with the 3x3 halo and zero padding) from the NHWC tensor into LDS with
-// contiguous 16-byte loads, and reads the MFMA operands straight from LDS at the
-// shifted pixel of each tap, so every input element is fetched from HBM about once.
-// LDS rows hold C + 8 bf16 per pixel (16-byte aligned, and 16 consecutive pixels of
-// a b128 read fall on distinct banks).
+// Convolution passes of the learned initialiser's network on the MFMA
+// (v_mfma_f32_16x16x32_bf16), NHWC bfloat16 activations.
 //
-//   conv_fwd<S>    Z = conv(X, W) + b, Y = gelu(Z)          rows: output pixels, cols: cout
-//   conv_dgrad<S>  dZprev = conv^T(dZ, W) * gelu'(Zprev)      rows: input pixels, cols: cin
-//   conv_wgrad<S>  dW = sum_pixels dZ (x) X~, db = sum dZ     rows: cout, cols: (tap, cin)
+//   conv_wave<0, S>  Z = conv(X, W) + b, Y = gelu(Z)          D[cout][pixel], operands from global memory
+//   conv_wave<1, S>  dZprev = conv^T(dZ, W) * gelu'(Zprev)      D[cin][pixel], operands from global memory
+//   conv_wgrad<S>    dW = sum_pixels dZ (x) X~, db = sum dZ     rows: cout, cols: (tap, cin), band in LDS
 //   conv1_fwd / conv1_wgrad: the single-channel first layer on the vector ALUs.
 //
 // v_mfma_f32_16x16x32_bf16 lane layout: lane l supplies row (l % 16) of A and column
@@ -27,10 +22,6 @@ __device__ __forceinline__ v4f mfma(v8bf a, v8bf b, v4f c) { return __builtin_am
 __host__ __device__ inline int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
 constexpr int kWaves = 8;           // waves of a band workgroup (512 threads)
-constexpr int kGPW = 2;             // pixel groups of 16 per wave and pass
-constexpr int kGroupsPerPass = kWaves * kGPW;
-constexpr int kStageF = 40;         // bf16 row stride of the forward output staging (32 channels + 8)
-constexpr int kStageD = 36;         // fp32 row stride of the data-gradient staging (32 channels + 4)
 
 // a / d for 0 <= a < 2^22 through a float reciprocal and one correction step
 struct Div {
@@ -46,9 +37,12 @@ struct Div {
 };
 
 // GELU (tanh form) and its derivative with tanh(u) = 1 - 2 / (exp(2u) + 1) on the
-// hardware exp and reciprocal: the results are rounded to bfloat16 right after, so
-// the last-bit differences from tanhf change a rounded value only at a tie.
-__device__ __forceinline__ float tanh_fast(float u) { return 1.0f - 2.0f * __frcp_rn(__expf(2.0f * u) + 1.0f); }
+// hardware exp and reciprocal (v_rcp_f32, 1 ulp; exp overflow gives rcp(inf) = 0 and
+// tanh = 1): the results are rounded to bfloat16 right after, so the last-bit
+// differences from tanhf change a rounded value only at a tie.
+__device__ __forceinline__ float tanh_fast(float u) {
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * u) + 1.0f);
+}
 __device__ __forceinline__ float gelu_f(float x) {
     return 0.5f * x * (1.0f + tanh_fast(kGeluC * (x + kGeluA * x * x * x)));
 }
@@ -100,187 +94,8 @@ __device__ __forceinline__ void load_band(const u16* src, int b, int H, int W, i
     copy16(nr * (W + 2) * (C >> 3), dst, m);
 }
 
-// Register-staged band copy: issue() starts the (at most kPre per thread) 16-byte
-// loads of a band, commit() writes them to LDS, so the loads of the next band can
-// be in flight while the current one is computed.
-constexpr int kPre = 8;
-struct BandPrefetch {
-    v8u val[kPre];
-    int off[kPre];
-    __device__ void issue(const u16* src, int b, int H, int W, int C, int r_lo, int nr) {
-        const int vec = C >> 3, n = nr * (W + 2) * vec;
-        const BandMap m{src, b, H, W, C, r_lo, Div(vec), Div(W + 2)};
-#pragma unroll
-        for (int u = 0; u < kPre; ++u) {
-            const int i = threadIdx.x + u * blockDim.x;
-            val[u] = v8u{0, 0, 0, 0, 0, 0, 0, 0};
-            off[u] = -1;
-            if (i < n) {
-                const u16* p = m(i, off[u]);
-                if (p) val[u] = *reinterpret_cast<const v8u*>(p);
-            }
-        }
-    }
-    __device__ void commit(u16* dst) const {
-#pragma unroll
-        for (int u = 0; u < kPre; ++u)
-            if (off[u] >= 0) *reinterpret_cast<v8u*>(dst + off[u]) = val[u];
-    }
-};
-
-// 32-row weight tile [9][32][C + 8] of rows j0 .. j0 + 31 of w[rows][9][C]
-struct WeightMap {
-    const u16* w;
-    int C, j0;
-    Div dv;
-    __device__ const u16* operator()(int i, int& off) const {
-        const int tj = dv.q(i), v = i - tj * dv.d;
-        const int tap = tj >> 5, j = tj & 31;
-        off = (tap * 32 + j) * (C + 8) + v * 8;
-        return w + ((size_t)(j0 + j) * 9 + tap) * C + v * 8;
-    }
-};
-__device__ __forceinline__ void load_weights(const u16* w, int C, int j0, u16* dst) {
-    const WeightMap m{w, C, j0, Div(C >> 3)};
-    copy16(9 * 32 * (C >> 3), dst, m);
-}
-
-// ---------------------------------------------------------------------------
-// Forward: persistent workgroups of 512 threads over the (image, band) items, one
-// 32-channel tile of cout at a time (weight tile loaded once per tile).  w:
-// [cout][9][cin] bf16.  LDS: input band, weight tile, output staging.  The next
-// item's band is loaded into registers while the current one is computed; each pass
-// of up to 16 pixel groups leaves Z in the staging, then Z and Y = gelu(Z) go out
-// as 16-byte stores of 8 channels.
-// ---------------------------------------------------------------------------
-struct FwdArgs {
-    const u16* x;
-    const u16* w;
-    const float* bias;
-    u16* Z;
-    u16* Y;
-    int B, H, W, cin, cout, Ho, Wo, R;
-};
-
-template <int S>
-__global__ __launch_bounds__(512) void conv_fwd(const FwdArgs A) {
-    extern __shared__ __attribute__((aligned(16))) u16 lds[];
-    const int CP = A.cin + 8, NC = A.W + 2;
-    const int NRmax = S * (A.R - 1) + 3;
-    u16* band = lds;
-    u16* wl = band + (size_t)NRmax * NC * CP;  // [9][32][CP]
-    u16* stage = wl + (size_t)9 * 32 * CP;     // [<= 256][kStageF]
-    const int nb = (A.Ho + A.R - 1) / A.R, items = A.B * nb;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nk = A.cin >> 5;
-    const Div dwo(A.Wo);
-    BandPrefetch pf;
-    for (int ct = 0; ct < A.cout / 32; ++ct) {
-        __syncthreads();
-        load_weights(A.w, A.cin, ct * 32, wl);
-        int item = blockIdx.x;
-        if (item < items) {
-            const int b = item / nb, ho0 = (item - b * nb) * A.R, R = min(A.R, A.Ho - ho0);
-            pf.issue(A.x, b, A.H, A.W, A.cin, S * ho0 - 1, S * (R - 1) + 3);
-        }
-        while (item < items) {
-            const int b = item / nb, ho0 = (item - b * nb) * A.R, R = min(A.R, A.Ho - ho0);
-            __syncthreads();
-            pf.commit(band);
-            __syncthreads();
-            const int next = item + gridDim.x;
-            if (next < items) {
-                const int b2 = next / nb, h2 = (next - b2 * nb) * A.R, R2 = min(A.R, A.Ho - h2);
-                pf.issue(A.x, b2, A.H, A.W, A.cin, S * h2 - 1, S * (R2 - 1) + 3);
-            }
-            const int P = R * A.Wo, G = (P + 15) / 16;
-            for (int g0 = 0; g0 < G; g0 += kGroupsPerPass) {
-                int boff[kGPW];
-                int ng = 0;
-#pragma unroll
-                for (int gi = 0; gi < kGPW; ++gi) {
-                    const int g = g0 + wave + kWaves * gi;
-                    boff[gi] = 0;
-                    if (g < G && g < g0 + kGroupsPerPass) {
-                        const int q = min(g * 16 + (lane & 15), P - 1);
-                        const int hr = dwo.q(q), wo = q - hr * A.Wo;
-                        boff[gi] = ((S * hr) * NC + S * wo) * CP + 8 * (lane >> 4);
-                        ng = gi + 1;
-                    }
-                }
-                v4f acc[kGPW][2];
-#pragma unroll
-                for (int gi = 0; gi < kGPW; ++gi) acc[gi][0] = acc[gi][1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-                for (int tap = 0; tap < 9; ++tap) {
-                    const int kh = tap / 3, kw = tap - 3 * kh;
-                    const int toff = (kh * NC + kw) * CP;
-                    for (int kc = 0; kc < nk; ++kc) {
-                        const v8bf b0 = ld8(wl + ((size_t)tap * 32 + (lane & 15)) * CP + kc * 32 + 8 * (lane >> 4));
-                        const v8bf b1 =
-                            ld8(wl + ((size_t)tap * 32 + 16 + (lane & 15)) * CP + kc * 32 + 8 * (lane >> 4));
-#pragma unroll
-                        for (int gi = 0; gi < kGPW; ++gi) {
-                            if (gi < ng) {
-                                const v8bf a = ld8(band + boff[gi] + toff + kc * 32);
-                                acc[gi][0] = mfma(a, b0, acc[gi][0]);
-                                acc[gi][1] = mfma(a, b1, acc[gi][1]);
-                            }
-                        }
-                    }
-                }
-                // Z = bf16(acc + bias) into the staging, row = pixel within the pass
-#pragma unroll
-                for (int gi = 0; gi < kGPW; ++gi) {
-                    if (gi >= ng) continue;
-                    const int g = g0 + wave + kWaves * gi;
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        const int cl = 16 * j + (lane & 15);
-                        const float bias = A.bias[ct * 32 + cl];
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            stage[((g - g0) * 16 + 4 * (lane >> 4) + r) * kStageF + cl] = f2bf(acc[gi][j][r] + bias);
-                    }
-                }
-                __syncthreads();
-                const int npx = min(P - g0 * 16, kGroupsPerPass * 16);
-                for (int i = threadIdx.x; i < npx * 4; i += blockDim.x) {
-                    const int slot = i >> 2, v = i & 3;
-                    const int q = g0 * 16 + slot;
-                    const int hr = dwo.q(q), wo = q - hr * A.Wo;
-                    const size_t o = (((size_t)b * A.Ho + ho0 + hr) * A.Wo + wo) * A.cout + ct * 32 + 8 * v;
-                    const v8u z = *reinterpret_cast<const v8u*>(stage + slot * kStageF + 8 * v);
-                    v8u y;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) y[e] = f2bf(gelu_f(bf2f(z[e])));
-                    *reinterpret_cast<v8u*>(A.Z + o) = z;
-                    *reinterpret_cast<v8u*>(A.Y + o) = y;
-                }
-                if (g0 + kGroupsPerPass < G) __syncthreads();
-            }
-            item = next;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Data gradient: persistent workgroups of 512 threads over the (image, band of input
-// rows) items, one 32-channel tile of cin at a time.  w2t: [cin][9][cout] bf16.
-// Input pixels (h, w) of the band are taken class by class of (h mod S, w mod S), so
-// that every pixel group of 16 uses the same taps: with stride 2 only
-// kh = (h + 1) mod 2 (+ 2) reach an output row.  The next item's dZ band is loaded
-// into registers during the current item, and each pass's Zprev vectors before its
-// MFMAs; the pass leaves its FP32 sums in the staging, then dZprev = bf16(sum *
-// gelu'(Zprev)) goes out 8 channels at a time with 16-byte stores.
-// ---------------------------------------------------------------------------
-struct DgradArgs {
-    const u16* dz;   // [B][Ho][Wo][cout]
-    const u16* w2t;  // [cin][9][cout]
-    const u16* zp;   // [B][H][W][cin]
-    u16* dzp;        // [B][H][W][cin]
-    int B, H, W, cin, cout, Ho, Wo, R;
-};
-
+// Pixels of a band [h0, h0 + R) of an image W pixels wide taken class by class of
+// (h mod S, w mod S); 16-pixel groups numbered class after class.
 template <int S>
 struct PixelClasses {  // pixel classes c = (ph, pw) of a band of rows [h0, h0 + R)
     int fr[S], nc[S], cnt[S * S], gstart[S * S + 1];  // constant indices only (registers)
@@ -333,133 +148,246 @@ struct PixelClasses {  // pixel classes c = (ph, pw) of a band of rows [h0, h0 +
     }
 };
 
-template <int S>
-__global__ __launch_bounds__(512) void conv_dgrad(const DgradArgs A) {
-    extern __shared__ __attribute__((aligned(16))) u16 lds[];
-    const int CP = A.cout + 8, NC = A.Wo + 2;
-    const int NRmax = (A.R + S) / S + 1;  // most dZ rows any band of A.R rows needs
-    u16* band = lds;
-    u16* wl = band + (size_t)NRmax * NC * CP;                           // [9][32][CP]
-    float* stage = reinterpret_cast<float*>(wl + (size_t)9 * 32 * CP);  // [<= 256][kStageD]
-    const int nb = (A.H + A.R - 1) / A.R, items = A.B * nb;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nk = A.cout >> 5;
-    constexpr int U = 2;  // Zprev vectors per thread and pass: 256 pixels x 4 vectors / 512 threads
-    BandPrefetch pf;
-    for (int ct = 0; ct < A.cin / 32; ++ct) {
-        __syncthreads();
-        load_weights(A.w2t, A.cout, ct * 32, wl);
-        int item = blockIdx.x;
-        if (item < items) {
-            const int b = item / nb, h0 = (item - b * nb) * A.R, R = min(A.R, A.H - h0);
-            const int lo = floordiv(h0 - 1, S);
-            pf.issue(A.dz, b, A.Ho, A.Wo, A.cout, lo, floordiv(h0 + R, S) - lo + 1);
+// ---------------------------------------------------------------------------
+// Wave-streaming forward and data-gradient passes.  The MFMA runs in the transposed
+// form D[channel][pixel] = sum_k A[channel][k] B[k][pixel]: A is a 16-channel tile of
+// the weights, held in LDS for the whole launch and stored lane-major so that a wave's
+// operand is one contiguous 1 KB read; B holds the operand activations of 16 output
+// pixels at one tap, each lane loading 8 channels (16 bytes) of one pixel straight
+// from global memory; the result leaves lane l with 4 consecutive channels of pixel
+// l % 16, the 8-byte vectors the epilogue loads (Zprev) and stores.  The waves of a
+// workgroup take the 16-pixel groups of its (image, band of rows) items on their own:
+// after the weight load there is no barrier, so every wave keeps its loads in flight,
+// and the rows that neighbouring groups share come from the L1 and L2 of one CU.  A
+// wave issues the loads of its next group before the epilogue of the current one:
+// vector stores and loads retire in order, so a load issued after a group's stores
+// would wait for them.
+//   KIND 0 (forward):       output pixel (ho, wo), tap (kh, kw) -> operand pixel
+//                           (S ho + kh - 1, S wo + kw - 1);  Z = acc + b, Y = gelu(Z).
+//   KIND 1 (data gradient): output pixel (h, w), tap (kh, kw) -> dZ pixel
+//                           ((h + 1 - kh) / S, (w + 1 - kw) / S) where S divides;
+//                           out = acc * gelu'(Zprev).  With S = 2 the pixels are taken
+//                           class by class of (ph, pw) = (h mod 2, w mod 2), so a group
+//                           uses 1, 2 or 4 taps: slot (i, j), i <= ph, j <= pw, is tap
+//                           (kh, kw) = (ph ? 2 i : 1, pw ? 2 j : 1), dZ pixel
+//                           ((h - ph) / 2 + (ph && i == 0), (w - pw) / 2 + (pw && j == 0)).
+// Zero padding: a tap whose operand pixel lies outside the image loads from a zeroed
+// tail the host allocates after the operand tensor (zoff), so all loads of a group
+// issue together and need no select.  Lanes past a class's last pixel repeat its last
+// pixel and store nothing.  Offsets are 32-bit element indices (the host keeps every
+// activation tensor below 2^31 elements).
+// ---------------------------------------------------------------------------
+constexpr int kWvWaves = 8;  // waves of a wave-streaming workgroup (512 threads)
+typedef unsigned short v4u __attribute__((ext_vector_type(4)));
+
+struct WaveArgs {
+    const u16* x;       // operand activations [B][Hx][Wx][K] (forward: Y of the previous layer; data gradient: dZ)
+    const u16* w;       // weights [Nout][9][K] (forward: wt; data gradient: w2t)
+    const float* bias;  // forward: [Nout]
+    const u16* zp;      // data gradient: Zprev [B][Ho][Wo][Nout]
+    u16* out0;          // forward: Z; data gradient: dZprev; [B][Ho][Wo][Nout]
+    u16* out1;          // forward: Y
+    int B, Ho, Wo, Hx, Wx, K, Nout;
+    int R;          // output rows per item
+    unsigned zoff;  // element offset of K zeros after the operand tensor
+};
+
+// bfloat16 rounding of 4 values on v_cvt_pk_bf16_f32: round to nearest even, as f2bf
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4u bf4(v4f x) { return __builtin_bit_cast(v4u, __builtin_convertvector(x, v4bf)); }
+
+// LDS bytes of the weight tiles: 9 taps x K / 32 chunks x NMT 16-channel tiles of 1 KB
+__host__ __device__ constexpr int wave_lds(int K, int nmt) { return 9 * (K >> 5) * nmt * 1024; }
+
+// one 16-pixel group of a wave: where its outputs go and where its operands are
+template <int KIND, int S>
+struct WaveGroup {
+    static constexpr bool kClasses = KIND == 1 && S == 2;
+    static constexpr int T = kClasses ? 4 : 9;  // operand slots
+    unsigned obase;  // the lane's output vector (channel c0 + 4 (lane / 16))
+    int ocen;        // the lane's operand vector at dy = dx = 0
+    unsigned vmask;  // bit t: slot t's operand pixel lies in the image
+    bool ok;         // the lane holds a pixel of the group
+    int ph, pw;      // pixel class (uniform)
+    __device__ bool used(int t) const { return !kClasses || ((t >> 1) <= ph && (t & 1) <= pw); }
+    __device__ int tap(int t) const {
+        if (!kClasses) return t;
+        const int kh = ph ? 2 * (t >> 1) : 1, kw = pw ? 2 * (t & 1) : 1;
+        return 3 * kh + kw;
+    }
+    __device__ int dy(int t) const { return kClasses ? (ph && (t >> 1) == 0) : KIND == 0 ? t / 3 : -(t / 3); }
+    __device__ int dx(int t) const { return kClasses ? (pw && (t & 1) == 0) : KIND == 0 ? t % 3 : -(t % 3); }
+};
+
+template <int KIND, int S, int SC>
+__device__ __forceinline__ void wave_group_setup(const WaveArgs& A, const PixelClasses<SC>& pc, int b, int h0, int g,
+                                                 int lane, int c0, WaveGroup<KIND, S>& u) {
+    const int c = pc.cls(g);
+    const int cnt = pc.count(c);
+    const int k = (g - pc.first_group(c)) * 16 + (lane & 15);
+    int hr, w;
+    pc.pixel(c, min(k, cnt - 1), hr, w);
+    const int h = h0 + hr;
+    u.ok = k < cnt;
+    u.ph = SC == 2 ? c / 2 : 0;
+    u.pw = SC == 2 ? c % 2 : 0;
+    u.obase = (unsigned)(((b * A.Ho + h) * A.Wo + w) * A.Nout + c0 + 4 * (lane >> 4));
+    int yc, xc;  // operand pixel at dy = dx = 0
+    if (KIND == 0) {
+        yc = S * h - 1;
+        xc = S * w - 1;
+    } else if (S == 1) {
+        yc = h + 1;
+        xc = w + 1;
+    } else {
+        yc = (h - u.ph) / 2;
+        xc = (w - u.pw) / 2;
+    }
+    u.ocen = ((b * A.Hx + yc) * A.Wx + xc) * A.K + 8 * (lane >> 4);
+    unsigned m = 0;
+#pragma unroll
+    for (int t = 0; t < WaveGroup<KIND, S>::T; ++t) {
+        const bool v = (unsigned)(yc + u.dy(t)) < (unsigned)A.Hx && (unsigned)(xc + u.dx(t)) < (unsigned)A.Wx;
+        m |= (unsigned)v << t;
+    }
+    u.vmask = m;
+}
+
+template <int KIND, int S, int NK>
+__device__ __forceinline__ void wave_group_load(const WaveArgs& A, const WaveGroup<KIND, S>& u, int lane, int kq,
+                                                v8u (&bv)[WaveGroup<KIND, S>::T][NK]) {
+    const unsigned zlane = A.zoff + 8 * (lane >> 4);
+#pragma unroll
+    for (int t = 0; t < WaveGroup<KIND, S>::T; ++t) {
+        const unsigned off = (u.vmask >> t) & 1u ? (unsigned)(u.ocen + (u.dy(t) * A.Wx + u.dx(t)) * A.K) : zlane;
+#pragma unroll
+        for (int kc = 0; kc < NK; ++kc) {
+            bv[t][kc] = v8u{0, 0, 0, 0, 0, 0, 0, 0};
+            if (!u.used(t)) continue;  // uniform
+            bv[t][kc] = *reinterpret_cast<const v8u*>(A.x + off + 32 * (kq + kc));
         }
-        while (item < items) {
-            const int b = item / nb, h0 = (item - b * nb) * A.R, R = min(A.R, A.H - h0);
-            const int ho_lo = floordiv(h0 - 1, S);
-            __syncthreads();
-            pf.commit(band);
-            __syncthreads();
-            const int next = item + gridDim.x;
-            if (next < items) {
-                const int b2 = next / nb, h2 = (next - b2 * nb) * A.R, R2 = min(A.R, A.H - h2);
-                const int lo2 = floordiv(h2 - 1, S);
-                pf.issue(A.dz, b2, A.Ho, A.Wo, A.cout, lo2, floordiv(h2 + R2, S) - lo2 + 1);
+    }
+}
+
+template <int KIND, int S, int NMT, int NK>
+__device__ __forceinline__ void wave_group_mfma(const WaveGroup<KIND, S>& u, const v8u* wl, int lane, int nkc, int kq,
+                                                const v8u (&bv)[WaveGroup<KIND, S>::T][NK], v4f (&acc)[NMT]) {
+#pragma unroll
+    for (int t = 0; t < WaveGroup<KIND, S>::T; ++t) {
+        if (!u.used(t)) continue;  // uniform
+        const int tap = u.tap(t);
+#pragma unroll
+        for (int kc = 0; kc < NK; ++kc) {
+            const v8bf bop = __builtin_bit_cast(v8bf, bv[t][kc]);
+#pragma unroll
+            for (int mt = 0; mt < NMT; ++mt) {
+                const v8bf aop = __builtin_bit_cast(v8bf, wl[((tap * nkc + kq + kc) * NMT + mt) * 64 + lane]);
+                acc[mt] = mfma(aop, bop, acc[mt]);
             }
-            const PixelClasses<S> pc(h0, R, A.W);
+        }
+    }
+}
+
+template <int KIND, int S, int NMT>
+__device__ __forceinline__ void wave_group_store(const WaveArgs& A, const WaveGroup<KIND, S>& u, const v4f (&acc)[NMT],
+                                                 const v4u (&zv)[NMT], const v4f (&bias)[NMT]) {
+    if (!u.ok) return;
+#pragma unroll
+    for (int mt = 0; mt < NMT; ++mt) {
+        if (KIND == 0) {
+            const v4u zo = bf4(acc[mt] + bias[mt]);
+            v4f y;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) y[r] = gelu_f(bf2f(zo[r]));
+            *reinterpret_cast<v4u*>(A.out0 + u.obase + 16 * mt) = zo;
+            *reinterpret_cast<v4u*>(A.out1 + u.obase + 16 * mt) = bf4(y);
+        } else {
+            v4f o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) o[r] = acc[mt][r] * gelu_grad_f(bf2f(zv[mt][r]));
+            *reinterpret_cast<v4u*>(A.out0 + u.obase + 16 * mt) = bf4(o);
+        }
+    }
+}
+
+// Persistent workgroups of 512 threads over the (image, band of R output rows) items;
+// the output channels in chunks of 16 NMT (one chunk when they fit, weights reloaded
+// per chunk otherwise); the K operand channels in steps of 32 NK.  Needs K % (32 NK)
+// == 0, Nout % (16 NMT) == 0 and wave_lds(K, NMT) bytes of dynamic LDS.
+template <int KIND, int S, int NMT, int NK>
+__global__ __launch_bounds__(512) void conv_wave(const WaveArgs A) {
+    extern __shared__ __attribute__((aligned(16))) v8u wl[];
+    constexpr int SC = KIND == 1 ? S : 1;
+    constexpr int T = WaveGroup<KIND, S>::T;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nkc = A.K >> 5;
+    const int nvec = 9 * nkc * NMT * 64;
+    const int nb = (A.Ho + A.R - 1) / A.R, items = A.B * nb;
+    for (int c0 = 0; c0 < A.Nout; c0 += 16 * NMT) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nvec; i += blockDim.x) {
+            const int t = i >> 6, l = i & 63;
+            const int mt = t % NMT, kk = (t / NMT) % nkc, tap = t / (NMT * nkc);
+            wl[i] = *reinterpret_cast<const v8u*>(A.w + ((size_t)(c0 + 16 * mt + (l & 15)) * 9 + tap) * A.K + 32 * kk +
+                                                  8 * (l >> 4));
+        }
+        __syncthreads();
+        v4f bias[NMT];
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt)
+            bias[mt] = KIND == 0 ? *reinterpret_cast<const v4f*>(A.bias + c0 + 16 * mt + 4 * (lane >> 4))
+                                 : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int item = blockIdx.x; item < items; item += gridDim.x) {
+            const int b = item / nb, h0 = (item - b * nb) * A.R, R = min(A.R, A.Ho - h0);
+            const PixelClasses<SC> pc(h0, R, A.Wo);
             const int G = pc.groups();
-            for (int g0 = 0; g0 < G; g0 += kGroupsPerPass) {
-                const int nslot = min(G - g0, kGroupsPerPass) * 16;
-                // this pass's Zprev vectors (in flight during the MFMAs)
-                v8u zv[U];
-                size_t zo[U];
-                bool zok[U];
+            // the first group rotates with the item so that no wave always takes the remainder
+            int g = (wave + item) & (kWvWaves - 1);
+            if (g >= G) continue;
+            WaveGroup<KIND, S> u;
+            v8u bv[T][NK];
+            v4u zv[NMT];
+            wave_group_setup<KIND, S, SC>(A, pc, b, h0, g, lane, c0, u);
+            wave_group_load<KIND, S, NK>(A, u, lane, 0, bv);
+            if (KIND == 1) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = threadIdx.x + u * blockDim.x;
-                    const int slot = i >> 2, v = i & 3;
-                    zok[u] = false;
-                    zo[u] = 0;
-                    zv[u] = v8u{0, 0, 0, 0, 0, 0, 0, 0};
-                    if (i < nslot * 4) {
-                        const int g = g0 + (slot >> 4);
-                        const int c = pc.cls(g);
-                        const int k = (g - pc.first_group(c)) * 16 + (slot & 15);
-                        if (k < pc.count(c)) {
-                            int hr, w;
-                            pc.pixel(c, k, hr, w);
-                            zo[u] = (((size_t)b * A.H + h0 + hr) * A.W + w) * A.cin + ct * 32 + 8 * v;
-                            zv[u] = *reinterpret_cast<const v8u*>(A.zp + zo[u]);
-                            zok[u] = true;
-                        }
-                    }
-                }
-                int boff[kGPW], cl[kGPW];
-                int ng = 0;
-#pragma unroll
-                for (int gi = 0; gi < kGPW; ++gi) {
-                    const int g = g0 + wave + kWaves * gi;
-                    boff[gi] = 0;
-                    cl[gi] = 0;
-                    if (g < G && g < g0 + kGroupsPerPass) {
-                        const int c = pc.cls(g);
-                        const int ph = c / S, pw = c % S;
-                        const int k = min((g - pc.first_group(c)) * 16 + (lane & 15), pc.count(c) - 1);
-                        int hr, w;
-                        pc.pixel(c, k, hr, w);
-                        // dZ row of tap kh: (h + 1 - kh) / S = (h - ph) / S + (ph + 1 - kh) / S
-                        boff[gi] = (((h0 + hr - ph) / S - ho_lo) * NC + (w - pw) / S + 1) * CP + 8 * (lane >> 4);
-                        cl[gi] = c;
-                        ng = gi + 1;
-                    }
-                }
-                v4f acc[kGPW][2];
-#pragma unroll
-                for (int gi = 0; gi < kGPW; ++gi) acc[gi][0] = acc[gi][1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-                for (int tap = 0; tap < 9; ++tap) {
-                    const int kh = tap / 3, kw = tap - 3 * kh;
-                    for (int kc = 0; kc < nk; ++kc) {
-                        const v8bf b0 = ld8(wl + ((size_t)tap * 32 + (lane & 15)) * CP + kc * 32 + 8 * (lane >> 4));
-                        const v8bf b1 =
-                            ld8(wl + ((size_t)tap * 32 + 16 + (lane & 15)) * CP + kc * 32 + 8 * (lane >> 4));
-#pragma unroll
-                        for (int gi = 0; gi < kGPW; ++gi) {
-                            if (gi >= ng) continue;
-                            const int ph = cl[gi] / S, pw = cl[gi] % S;
-                            if ((ph + 1 - kh) % S || (pw + 1 - kw) % S) continue;  // tap misses every output pixel
-                            const int dh = floordiv(ph + 1 - kh, S), dw = floordiv(pw + 1 - kw, S);
-                            const v8bf a = ld8(band + boff[gi] + (dh * NC + dw) * CP + kc * 32);
-                            acc[gi][0] = mfma(a, b0, acc[gi][0]);
-                            acc[gi][1] = mfma(a, b1, acc[gi][1]);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int gi = 0; gi < kGPW; ++gi) {
-                    if (gi >= ng) continue;
-                    const int g = g0 + wave + kWaves * gi;
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            stage[((g - g0) * 16 + 4 * (lane >> 4) + r) * kStageD + 16 * j + (lane & 15)] =
-                                acc[gi][j][r];
-                }
-                __syncthreads();
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (!zok[u]) continue;
-                    const int i = threadIdx.x + u * blockDim.x;
-                    const float* sv = stage + (i >> 2) * kStageD + 8 * (i & 3);
-                    v8u o;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) o[e] = f2bf(sv[e] * gelu_grad_f(bf2f(zv[u][e])));
-                    *reinterpret_cast<v8u*>(A.dzp + zo[u]) = o;
-                }
-                if (g0 + kGroupsPerPass < G) __syncthreads();
+                for (int mt = 0; mt < NMT; ++mt) zv[mt] = *reinterpret_cast<const v4u*>(A.zp + u.obase + 16 * mt);
             }
-            item = next;
+            for (;;) {
+                v4f acc[NMT];
+#pragma unroll
+                for (int mt = 0; mt < NMT; ++mt) acc[mt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+                // the weight tiles are read from LDS per group: an opaque lane index keeps the
+                // compiler from hoisting all of them into registers (up to 144 VGPRs)
+                int wlane = lane;
+                asm volatile("" : "+v"(wlane));
+                wave_group_mfma<KIND, S, NMT, NK>(u, wl, wlane, nkc, 0, bv, acc);
+                for (int kq = NK; kq < nkc; kq += NK) {
+                    wave_group_load<KIND, S, NK>(A, u, lane, kq, bv);
+                    wave_group_mfma<KIND, S, NMT, NK>(u, wl, wlane, nkc, kq, bv, acc);
+                }
+                // the next group's loads go out before this group's stores
+                const int gn = g + kWvWaves;
+                const bool more = gn < G;
+                WaveGroup<KIND, S> un;
+                v4u zn[NMT];
+                if (more) {
+                    wave_group_setup<KIND, S, SC>(A, pc, b, h0, gn, lane, c0, un);
+                    wave_group_load<KIND, S, NK>(A, un, lane, 0, bv);
+                    if (KIND == 1) {
+#pragma unroll
+                        for (int mt = 0; mt < NMT; ++mt)
+                            zn[mt] = *reinterpret_cast<const v4u*>(A.zp + un.obase + 16 * mt);
+                    }
+                }
+                wave_group_store<KIND, S, NMT>(A, u, acc, zv, bias);
+                if (!more) break;
+                u = un;
+#pragma unroll
+                for (int mt = 0; mt < NMT; ++mt) zv[mt] = zn[mt];
+                g = gn;
+            }
         }
     }
 }
@@ -629,26 +557,42 @@ __global__ __launch_bounds__(256) void conv1_fwd(const u16* x0, const u16* wt, i
         xs[i] = (h >= 0 && h < H && w >= 0 && w < W) ? bf2f(x0[((size_t)b * H + h) * W + w]) : 0.0f;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < Rr * W * ng; t += blockDim.x) {
-        const int p = t / ng, g = t - p * ng;
+    // thread -> (channel group g, first pixel): g stays fixed, so its 72 weights and 8
+    // biases sit in registers for the whole band
+    const int pstep = blockDim.x / ng, g = threadIdx.x % ng;
+    if (threadIdx.x >= pstep * ng) return;
+    float wr[8][9], br[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) wr[e][tap] = ws[(8 * g + e) * 9 + tap];
+        br[e] = ws[cout * 9 + 8 * g + e];
+    }
+    for (int p = threadIdx.x / ng; p < Rr * W; p += pstep) {
         const int hr = p / W, w = p - hr * W;
         float xv[9];
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) xv[tap] = xs[(hr + tap / 3) * NC + w + tap % 3];
         const size_t o = (((size_t)b * H + h0 + hr) * W + w) * cout + 8 * g;
-        v8u zo, yo;
+        v4f z0, z1;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int co = 8 * g + e;
             float a = 0.0f;
 #pragma unroll
-            for (int tap = 0; tap < 9; ++tap) a = fmaf(ws[co * 9 + tap], xv[tap], a);
-            const u16 zb = f2bf(a + ws[cout * 9 + co]);
-            zo[e] = zb;
-            yo[e] = f2bf(gelu_f(bf2f(zb)));
+            for (int tap = 0; tap < 9; ++tap) a = fmaf(wr[e][tap], xv[tap], a);
+            if (e < 4) z0[e] = a + br[e];
+            else z1[e - 4] = a + br[e];
         }
-        *reinterpret_cast<v8u*>(Z + o) = zo;
-        *reinterpret_cast<v8u*>(Y + o) = yo;
+        const v4u zo0 = bf4(z0), zo1 = bf4(z1);
+        v4f y0, y1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            y0[e] = gelu_f(bf2f(zo0[e]));
+            y1[e] = gelu_f(bf2f(zo1[e]));
+        }
+        const v4u yo0 = bf4(y0), yo1 = bf4(y1);
+        *reinterpret_cast<v8u*>(Z + o) = v8u{zo0[0], zo0[1], zo0[2], zo0[3], zo1[0], zo1[1], zo1[2], zo1[3]};
+        *reinterpret_cast<v8u*>(Y + o) = v8u{yo0[0], yo0[1], yo0[2], yo0[3], yo1[0], yo1[1], yo1[2], yo1[3]};
     }
 }
 

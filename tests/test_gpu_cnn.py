@@ -62,6 +62,40 @@ def test_loss_and_gradient_match_oracle(cuda):
         assert e < 3e-2, name
 
 
+# Channel counts and odd image sizes that take the convolution passes' other tile
+# shapes: 32- and 64-channel output tiles, 32- and 64-channel K steps, several output
+# chunks (weights reloaded per chunk) and several K steps, images whose rows and
+# columns do not fill 16-pixel groups, and batches smaller than the resident grid.
+_SHAPES = [((64, 96, 32, 128), 37, 45, 3), ((32, 64, 32, 64), 33, 29, 5), ((32, 64, 64, 32), 20, 24, 2)]
+
+
+@pytest.mark.parametrize("channels,height,width,bs", _SHAPES)
+def test_other_shapes_match_oracle(cuda, channels, height, width, bs):
+    d = 7
+    net = ConvNet(height, width, d, channels, 64, max_batch=8, in_scale=0.2, seed=11, device=cuda)
+    rng = np.random.default_rng(12)
+    img = torch.as_tensor(rng.gamma(1.0, 2.0, size=(bs, height, width)) + rng.normal(0, 0.2, size=(bs, height, width)),
+                          dtype=torch.float32)
+    tgt = torch.as_tensor(rng.normal(size=(bs, d)), dtype=torch.float32)
+    p = net.get_params().cpu()
+    out = net.forward(img.to(cuda)).cpu()
+    ref = cnn_ref.forward(cnn_ref.unpack(p, net.layout), img, 0.2, channels).detach()
+    err = float((out - ref).abs().max() / ref.abs().max())
+    loss, g = net.loss_grad(img.to(cuda), tgt.to(cuda), want_grad=True)
+    lref, gref = cnn_ref.loss_and_grad(p, net.layout, img, tgt, 0.2, channels)
+    rel = abs(float(loss) - float(lref)) / abs(float(lref))
+    _print(f"{channels} {height}x{width} bs {bs}: forward err {err:.2e}, loss rel {rel:.2e}")
+    assert err < 1e-2
+    assert rel < 1e-3
+    g = g.cpu()
+    for name, shape, off in net.layout:
+        n = int(np.prod(shape))
+        a, b = g[off:off + n], gref[off:off + n]
+        e = float((a - b).abs().max() / b.abs().max().clamp_min(1e-20))
+        _print(f"  grad {name:8s} rel err {e:.2e}")
+        assert e < 3e-2, name
+
+
 def test_adamw_matches_torch(cuda):
     net, img, tgt = _net_and_batch(cuda, bs=4, seed=5)
     p0 = net.get_params().cpu().clone()
