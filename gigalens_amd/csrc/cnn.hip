@@ -78,8 +78,10 @@ static size_t conv1_fwd_lds(const gl_cnn::Layer& L) {
     return ((size_t)L.cout * 10 + (size_t)(conv1_rows(L) + 2) * (L.W + 2)) * 4;
 }
 static size_t conv1_wgrad_lds(const gl_cnn::Layer& L) { return (size_t)(L.H + 2) * (L.W + 2) * 4; }
+// weight gradient: the transposed dZ band and the three column planes of the input band
+static int wgrad_wp(const gl_cnn::Layer& L) { return (L.Wo + 7) / 8 * 8; }
 static size_t wgrad_lds(const gl_cnn::Layer& L, int R) {
-    return ((size_t)R * L.Wo * (L.cout + 8) + (size_t)(L.s * (R - 1) + 3) * (L.W + 2) * (L.cin + 8)) * 2;
+    return (size_t)gl::cnn::wgrad_lds_elems(L.s, R, wgrad_wp(L), L.cin, L.cout) * 2;
 }
 // largest band (rows <= n) meeting the budget, then evened out over the bands
 static int pick_rows(int n, bool (*ok)(const gl_cnn::Layer&, int), const gl_cnn::Layer& L) {
@@ -382,7 +384,8 @@ extern "C" int gl_cnn_loss_grad(gl_cnn* net, const float* images, const float* t
                                L.cout, net->part);
         } else {
             splits = std::min(L.wblk, bs);
-            WgradArgs A{L.dZ, net->L[l - 1].Y, net->part, bs, L.H, L.W, L.cin, L.cout, L.Ho, L.Wo, L.rw, L.wncg, L.wnp};
+            WgradArgs A{L.dZ, net->L[l - 1].Y, net->part, bs, L.H, L.W, L.cin, L.cout, L.Ho, L.Wo, L.rw, L.wncg, L.wnp,
+                        wgrad_wp(L)};
             const dim3 grid(splits, L.wchunks);
             const size_t lds = wgrad_lds(L, L.rw);
             if (L.s == 1) hipLaunchKernelGGL(conv_wgrad<1>, grid, dim3(512), lds, stream, A);

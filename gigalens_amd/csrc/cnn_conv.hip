@@ -51,49 +51,6 @@ __device__ __forceinline__ float gelu_grad_f(float x) {
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
 }
 
-// 16-byte copies into LDS with eight loads in flight per thread: for i in [0, n),
-// m(i, &off) returns the source vector (or null for zeros) and sets its LDS offset.
-template <class Map>
-__device__ __forceinline__ void copy16(int n, u16* dst, const Map& m) {
-    constexpr int U = 8;
-    for (int i0 = threadIdx.x; i0 < n; i0 += U * blockDim.x) {
-        v8u val[U];
-        int off[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = i0 + u * blockDim.x;
-            val[u] = v8u{0, 0, 0, 0, 0, 0, 0, 0};
-            off[u] = -1;
-            if (i < n) {
-                const u16* src = m(i, off[u]);
-                if (src) val[u] = *reinterpret_cast<const v8u*>(src);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (off[u] >= 0) *reinterpret_cast<v8u*>(dst + off[u]) = val[u];
-    }
-}
-
-// Rows [r_lo, r_lo + nr) x columns [-1, W] of image b of an NHWC tensor (C channels,
-// C % 8 == 0) into LDS as [nr][W + 2][C + 8], zeros outside the image.
-struct BandMap {
-    const u16* src;
-    int b, H, W, C, r_lo;
-    Div dv, dc;
-    __device__ const u16* operator()(int i, int& off) const {
-        const int pc = dv.q(i), v = i - pc * dv.d;
-        const int r = dc.q(pc), c = pc - r * dc.d;
-        off = (r * dc.d + c) * (C + 8) + v * 8;
-        const int h = r_lo + r, w = c - 1;
-        return (h >= 0 && h < H && w >= 0 && w < W) ? src + (((size_t)b * H + h) * W + w) * C + v * 8 : nullptr;
-    }
-};
-__device__ __forceinline__ void load_band(const u16* src, int b, int H, int W, int C, int r_lo, int nr, u16* dst) {
-    const BandMap m{src, b, H, W, C, r_lo, Div(C >> 3), Div(W + 2)};
-    copy16(nr * (W + 2) * (C >> 3), dst, m);
-}
-
 // Pixels of a band [h0, h0 + R) of an image W pixels wide taken class by class of
 // (h mod S, w mod S); 16-pixel groups numbered class after class.
 template <int S>
@@ -397,9 +354,22 @@ __global__ __launch_bounds__(512) void conv_wave(const WaveArgs A) {
 // tiles are (co group of 16) x (tap, ci group of 16) plus one bias tile per co
 // group (B = ones); a chunk covers co groups [cg0, cg0 + ncg) (ncg <= 4) and pairs
 // [p0, p0 + np) of the 9 cin / 16 + 1 (tap, ci group) pairs, wave w taking pairs
-// p0 + w, p0 + w + kWaves, ... (<= kMaxPairs).  Pixels are the reduction dimension:
-// k-steps of 32 pixels of the band; the operands gather 8 pixels per lane from LDS.
-// part[blockIdx.x][cout][9 cin + 1]: per-block sums, added by wgrad_reduce.
+// p0 + w, p0 + w + kWaves, ... (<= kMaxPairs) with every co group of the chunk.
+// Pixels are the reduction dimension, so each MFMA operand lane needs 8 consecutive
+// pixels of one channel: a band of R output rows is stored in LDS transposed,
+//   dZT[co][hr][wo]        wo < Wp (Wo rounded up to 8; zeros past Wo), row stride DS
+//   XT[kw][ci][r][cc] = X[S ho0 - 1 + r][S cc + kw - 1]   (zeros outside the image)
+// one plane per tap column kw, so that the operand of tap (kh, kw) at output pixel
+// (hr, wo) is XT[kw][ci][S hr + kh][wo .. wo + 7], and every operand is one aligned
+// 16-byte LDS read whatever the stride and tap.  The fill transposes in registers: a
+// thread loads the 16-byte vectors (8 channels) of 8 or 9 pixels of one row, turns
+// each 8 x 8 block into 8 channel rows of 8 pixels with byte permutes, and writes
+// them as aligned 16-byte LDS stores.  9 pixels one step apart feed two planes (S = 1:
+// columns cc0 - 1 .. cc0 + 7 for kw = 0, 1; S = 2: the odd columns for kw = 0, 2),
+// 8 pixels the third (S = 1: cc0 + 1 .. cc0 + 8 for kw = 2; S = 2: the even columns
+// for kw = 1).  k-steps of 32 pixels = 4 segments of 8
+// (one per quarter-wave).  part[blockIdx.x][cout][9 cin + 1]: per-block sums, added
+// by wgrad_reduce.
 // ---------------------------------------------------------------------------
 constexpr int kMaxPairs = 3;
 
@@ -409,24 +379,44 @@ struct WgradArgs {
     float* part;
     int B, H, W, cin, cout, Ho, Wo, R;
     int ncg, npairs;  // per chunk; gridDim.y = co chunks x pair chunks
+    int Wp;           // Wo rounded up to a multiple of 8
 };
 
-struct RowsMap {  // P consecutive pixels of C channels into LDS rows of C + 8
-    const u16* src;
-    int C;
-    Div dv;
-    __device__ const u16* operator()(int i, int& off) const {
-        const int q = dv.q(i), v = i - q * dv.d;
-        off = q * (C + 8) + v * 8;
-        return src + (size_t)q * C + v * 8;
+// LDS elements of the transposed band (host and device agree on the layout)
+__host__ __device__ inline int wgrad_ds(int R, int Wp) { return R * Wp + 8; }
+__host__ __device__ inline int wgrad_xs(int S, int R, int Wp) { return (S * (R - 1) + 3) * Wp + 8; }
+__host__ __device__ inline long wgrad_lds_elems(int S, int R, int Wp, int cin, int cout) {
+    return (long)cout * wgrad_ds(R, Wp) + 3L * cin * wgrad_xs(S, R, Wp);
+}
+
+// 8 pixels x 8 channels (p[i] = the channels of pixel i) -> 8 channels x 8 pixels,
+// written as out + c * stride for channel c (16-byte aligned)
+__device__ __forceinline__ void transpose_store8(const v8u* p, u16* out, int stride) {
+    typedef unsigned int v4w __attribute__((ext_vector_type(4)));
+    v4w d[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) d[i] = __builtin_bit_cast(v4w, p[i]);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        v4w o;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const unsigned a = d[2 * m][c >> 1], b = d[2 * m + 1][c >> 1];
+            o[m] = (c & 1) ? __builtin_amdgcn_perm(b, a, 0x07060302u) : __builtin_amdgcn_perm(b, a, 0x05040100u);
+        }
+        *reinterpret_cast<v4w*>(out + c * stride) = o;
     }
-};
+}
 
 template <int S>
-__global__ __launch_bounds__(512) void conv_wgrad(const WgradArgs A) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void conv_wgrad(const WgradArgs A) {
     extern __shared__ __attribute__((aligned(16))) u16 lds[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int CPo = A.cout + 8, CPi = A.cin + 8, NC = A.W + 2;
+    const int Wp = A.Wp, spr = Wp >> 3;
+    const int DS = wgrad_ds(A.R, Wp), XS = wgrad_xs(S, A.R, Wp);
+    const int PL = A.cin * XS;  // elements of one XT plane
+    u16* dzt = lds;
+    u16* xt = lds + A.cout * DS;
     const int n_pairs_total = 9 * (A.cin >> 4) + 1;
     const int n_pchunks = (n_pairs_total + A.npairs - 1) / A.npairs;
     const int cchunk = blockIdx.y / n_pchunks, pchunk = blockIdx.y % n_pchunks;
@@ -434,8 +424,6 @@ __global__ __launch_bounds__(512) void conv_wgrad(const WgradArgs A) {
     const int ncg = min(A.ncg, (A.cout >> 4) - cg0);
     const int p0 = pchunk * A.npairs;
     const int np = min(A.npairs, n_pairs_total - p0);
-    u16* dzb = lds;                            // [R Wo][CPo]
-    u16* xb = lds + (size_t)A.R * A.Wo * CPo;  // [S (R - 1) + 3][NC][CPi]
     // my pairs: (tap, ci group) or the bias pair (index 9 cin / 16)
     int ptap[kMaxPairs], pci[kMaxPairs];
     int mp = 0;
@@ -460,40 +448,76 @@ __global__ __launch_bounds__(512) void conv_wgrad(const WgradArgs A) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[i][c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     const v8u ones = {0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80, 0x3f80};
+    const v8u zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
     const int nbands = (A.Ho + A.R - 1) / A.R;
-    const Div dwo(A.Wo);
+    const int vo = A.cout >> 3, vi = A.cin >> 3, nblk = Wp >> 3;
+    constexpr int NP = 9;  // pixels a fill thread loads (one row)
     for (int b = blockIdx.x; b < A.B; b += gridDim.x) {
         for (int band = 0; band < nbands; ++band) {
             const int ho0 = band * A.R, R = min(A.R, A.Ho - ho0);
-            const int P = R * A.Wo;
+            const int nrx = S * (R - 1) + 3, xr0 = S * ho0 - 1;
             __syncthreads();
-            {  // dZ rows [ho0, ho0 + R): contiguous pixels
-                const RowsMap m{A.dz + (((size_t)b * A.Ho + ho0) * A.Wo) * A.cout, A.cout, Div(A.cout >> 3)};
-                copy16(P * (A.cout >> 3), dzb, m);
-            }
-            load_band(A.x, b, A.H, A.W, A.cin, S * ho0 - 1, S * (R - 1) + 3, xb);
-            __syncthreads();
-            for (int k0 = 0; k0 < P; k0 += 32) {
-                // the 8 pixels of this lane
-                int dofs[8], xofs[8];
-                bool ok[8];
+            // fill units: X (kind e, row r, column block, channel block) then dZ (row, column block, channel block)
+            const int nxu = 2 * nrx * nblk * vi, ndu = R * nblk * vo;
+            for (int u = threadIdx.x; u < nxu + ndu; u += blockDim.x) {
+                v8u px[NP];
+                if (u < nxu) {
+                    int q = u;
+                    const int v = q % vi;
+                    q /= vi;
+                    const int cb = q % nblk;
+                    q /= nblk;
+                    const int r = q % nrx, e = q / nrx;
+                    const int xr = xr0 + r, cc0 = cb * 8;
+                    // pixel i at column x0 + S i.  e = 1: 9 pixels for the planes kw = 0 and
+                    // (S = 1) 1 or (S = 2) 2, one pixel apart; e = 0: 8 pixels for the third plane
+                    const int x0 = S == 1 ? (e ? cc0 - 1 : cc0 + 1) : (e ? 2 * cc0 - 1 : 2 * cc0);
+                    const int np = e ? 9 : 8;
+                    const bool rin = xr >= 0 && xr < A.H;
+                    const u16* row = A.x + (((size_t)b * A.H + (rin ? xr : 0)) * A.W) * A.cin + 8 * v;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int q = k0 + 8 * (lane >> 4) + j;
-                    ok[j] = q < P;
-                    const int qq = ok[j] ? q : 0;
-                    const int hr = dwo.q(qq), wo = qq - hr * A.Wo;
-                    dofs[j] = qq * CPo;
-                    xofs[j] = ((S * hr) * NC + S * wo) * CPi;
+                    for (int i = 0; i < NP; ++i) {
+                        const int x = x0 + S * i;
+                        px[i] = v8u{0, 0, 0, 0, 0, 0, 0, 0};
+                        if (i < np && rin && x >= 0 && x < A.W) px[i] = *reinterpret_cast<const v8u*>(row + (size_t)x * A.cin);
+                    }
+                    u16* dst = xt + 8 * v * XS + r * Wp + cc0;
+                    if (e) {
+                        transpose_store8(px, dst, XS);                       // kw = 0
+                        transpose_store8(px + 1, dst + (S == 1 ? 1 : 2) * PL, XS);  // kw = 1 (S = 1), kw = 2 (S = 2)
+                    } else {
+                        transpose_store8(px, dst + (S == 1 ? 2 : 1) * PL, XS);  // kw = 2 (S = 1), kw = 1 (S = 2)
+                    }
+                } else {
+                    int q = u - nxu;
+                    const int v = q % vo;
+                    q /= vo;
+                    const int wb = q % nblk, hr = q / nblk;
+                    const int wo0 = wb * 8;
+                    const u16* row = A.dz + (((size_t)b * A.Ho + ho0 + hr) * A.Wo) * A.cout + 8 * v;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        px[i] = v8u{0, 0, 0, 0, 0, 0, 0, 0};
+                        if (wo0 + i < A.Wo) px[i] = *reinterpret_cast<const v8u*>(row + (size_t)(wo0 + i) * A.cout);
+                    }
+                    transpose_store8(px, dzt + 8 * v * DS + hr * Wp + wo0, DS);
                 }
+            }
+            __syncthreads();
+            const int nseg = R * spr;
+            for (int s0 = 0; s0 < nseg; s0 += 4) {
+                int seg = s0 + (lane >> 4);
+                const bool sv = seg < nseg;
+                seg = sv ? seg : nseg - 1;
+                const int hr = seg / spr, wo0 = (seg - hr * spr) * 8;
                 v8bf av[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    v8u t = {0, 0, 0, 0, 0, 0, 0, 0};
+                    v8u t = zero8;
                     if (c < ncg) {
-                        const int co = (cg0 + c) * 16 + (lane & 15);
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) t[j] = ok[j] ? dzb[dofs[j] + co] : (u16)0;
+                        const v8u d = *reinterpret_cast<const v8u*>(dzt + ((cg0 + c) * 16 + (lane & 15)) * DS +
+                                                                     hr * Wp + wo0);
+                        t = sv ? d : zero8;
                     }
                     av[c] = __builtin_bit_cast(v8bf, t);
                 }
@@ -503,9 +527,8 @@ __global__ __launch_bounds__(512) void conv_wgrad(const WgradArgs A) {
                     v8u t = ones;
                     if (ptap[i] < 9) {
                         const int kh = ptap[i] / 3, kw = ptap[i] - 3 * kh;
-                        const int toff = (kh * NC + kw) * CPi + pci[i] + (lane & 15);
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) t[j] = xb[xofs[j] + toff];
+                        t = *reinterpret_cast<const v8u*>(xt + kw * PL + (pci[i] + (lane & 15)) * XS +
+                                                          (S * hr + kh) * Wp + wo0);
                     }
                     const v8bf bv = __builtin_bit_cast(v8bf, t);
 #pragma unroll
