@@ -406,7 +406,7 @@ def run_cnn(args, dev):
                                   "60x60 images, batch 1024, AdamW", "batch": bs, "num_pix": N},
            "roofline": {"bound": "hbm", "achieved": hbm, "peak": 8000.0, "unit": "GB/s", "frac": hbm / 8000.0,
                         "traffic": None, "algorithmic_bytes_per_step": bytes_img * bs,
-                        "kernel": "network step (band-tiled BF16 MFMA convolutions + head + AdamW)",
+                        "kernel": "network step (BF16 MFMA convolutions: wave-streaming forward and data gradient, LDS-transposed weight gradient; head; AdamW)",
                         "network_ms": net_ms, "network_wall_ms": wall_net * 1e3, "flop_per_image": flop_img,
                         "mfma_achieved_tflops": achieved, "mfma_peak_tflops": 2500.0, "mfma_frac": achieved / 2500.0,
                         "note": "traffic per step from rocprofv3 FETCH_SIZE/WRITE_SIZE is in the committed profile"}}
