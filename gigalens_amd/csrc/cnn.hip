@@ -1,6 +1,6 @@
 // C ABI of the learned initialiser's network (gl_cnn_*): workspace, parameter
-// packing and the launch sequences of the BF16 MFMA kernels (cnn_conv.hip: band-tiled
-// convolutions; cnn_kernels.hip: input transform, head, reductions, AdamW).
+// packing and the launch sequences of the BF16 MFMA kernels (cnn_conv.hip: wave-streaming
+// and LDS-transposed convolutions; cnn_kernels.hip: input transform, head, reductions, AdamW).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
