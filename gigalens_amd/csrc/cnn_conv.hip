@@ -457,19 +457,24 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void c
             const int ho0 = band * A.R, R = min(A.R, A.Ho - ho0);
             const int nrx = S * (R - 1) + 3, xr0 = S * ho0 - 1;
             __syncthreads();
-            // fill units: X (kind e, channel block, row, column block) then dZ (channel block, row, column block)
+            // fill units: X (kind e, channel block pair, row, column block, channel block) then dZ (channel
+            // block pair, row, column block, channel block)
             const int nxu = 2 * nrx * nblk * vi, ndu = R * nblk * vo;
             for (int u = threadIdx.x; u < nxu + ndu; u += blockDim.x) {
                 v8u px[NP];
                 if (u < nxu) {
-                    // column block fastest, then row: 16 consecutive lanes store one contiguous
-                    // 256-byte run of a channel row (no LDS bank conflicts)
+                    // two channel blocks fastest (32 contiguous bytes of a pixel per lane pair),
+                    // then column block, then row: the channel-block rows lie an odd multiple
+                    // of 128 bytes apart, so 16 consecutive lanes store 16 distinct 16-byte bank
+                    // slots (no LDS bank conflicts)
                     int q = u;
+                    const int v_lo = q & 1;
+                    q >>= 1;
                     const int cb = q % nblk;
                     q /= nblk;
                     const int r = q % nrx;
                     q /= nrx;
-                    const int v = q % vi, e = q / vi;
+                    const int v = v_lo + 2 * (q % (vi >> 1)), e = q / (vi >> 1);
                     const int xr = xr0 + r, cc0 = cb * 8;
                     // pixel i at column x0 + S i.  e = 1: 9 pixels for the planes kw = 0 and
                     // (S = 1) 1 or (S = 2) 2, one pixel apart; e = 0: 8 pixels for the third plane
@@ -492,9 +497,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void c
                     }
                 } else {
                     int q = u - nxu;
+                    const int v_lo = q & 1;
+                    q >>= 1;
                     const int wb = q % nblk;
                     q /= nblk;
-                    const int hr = q % R, v = q / R;
+                    const int hr = q % R, v = v_lo + 2 * (q / R);
                     const int wo0 = wb * 8;
                     const u16* row = A.dz + (((size_t)b * A.Ho + ho0 + hr) * A.Wo) * A.cout + 8 * v;
 #pragma unroll
