@@ -525,7 +525,7 @@ static McUnits mc_units(const gl::DPlan& P, const gl::McPlan& M, int count) {
     // channel conv: direct (mc_conv) or separable (mc_sep_h + mc_sep_v)
     u.conv = M.sep_rank > 0 ? 0 : (long)count * M.D * ((P.N + M.conv_ib - 1) / M.conv_ib) * njb;
     u.seph = M.sep_rank > 0 ? (long)count * M.D * (((long)P.nsub * P.N + 255) / 256) : 0;
-    u.sepv = M.sep_rank > 0 ? (long)count * M.D * ((P.N + gl::kSepIB - 1) / gl::kSepIB) * njb : 0;
+    u.sepv = M.sep_rank > 0 ? (long)count * M.D * ((P.N + gl::kSepRows - 1) / gl::kSepRows) * njb : 0;
     // adjoint conv: direct (mc_adjoint) or separable (mc_adj_h + mc_adj_v)
     const long nyb = (P.N + gl::kAdjKY - 1) / gl::kAdjKY;
     u.adj = M.adj_rank > 0 ? 0 : (long)count * (((long)P.nsub * P.ss * njb + 3) & ~3L);
@@ -1584,7 +1584,7 @@ extern "C" int gl_batch_create_ex(const gl_plan* const* plans, int n, const int*
     // work-items (a dispatch holds at most 2^32)
     auto stage_threads = [](const McUnits& u) {
         return std::max({u.basis * 256, u.grad * 256, u.conv * 64, u.adj * 64, u.adjh * 64, u.adjv * 64,
-                         u.seph * 64, u.sepv * 64, u.part * 256});
+                         u.seph * 64, u.sepv * 256, u.part * 256});
     };
     const long kMaxThreads = 1L << 31;
     std::vector<std::vector<int>> groups;

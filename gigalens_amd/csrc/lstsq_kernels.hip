@@ -155,7 +155,7 @@ struct McTable {
     const int* pre_adjh;   // separable adjoint horizontal-pass waves (a multiple of 4 per particle; 0 without)
     const int* pre_adjv;   // separable adjoint vertical-pass waves (a multiple of 4 per particle; 0 without)
     const int* pre_seph;   // separable channel conv: horizontal-pass waves (0 for direct-conv lenses)
-    const int* pre_sepv;   // separable channel conv: vertical-pass waves
+    const int* pre_sepv;   // separable channel conv: vertical-pass workgroups
     int tot_part, tot_basis, tot_conv, tot_adj, tot_grad, tot_adjh, tot_adjv, tot_seph, tot_sepv;
     int lds_consts;        // bytes of LDS for one particle's constant block (max over lenses)
     int lds_w;             // bytes of LDS for the folded weights kf x kf (max over lenses)
@@ -685,64 +685,112 @@ __global__ __launch_bounds__(256) void mc_sep_h(const McTable T) {
     }
 }
 
-// mc_sep_v: one wave per (particle, channel, kSepIB output rows, 64 columns J),
+// mc_sep_v: one workgroup per (particle, channel, kSepRows output rows, 64 columns J),
 // in polyphase order: tap p = SS m + a of output row I reads H row SS (I + m) + a - pb,
-// so for each phase a the kSepIB outputs of a lane form a 1-D correlation over
-// H rows SS k + a - pb: each row read serves up to kSepIB outputs, and with the
-// loops unrolled every factor index is a compile-time constant (scalar loads of
-// the zero-padded polyphase table).  The padded taps (SS m + a >= kf) carry zero
-// factors and reach at most SS - 1 rows past the kernel window.
-constexpr int kSepIB = 16;
+// so for each phase a the output rows of a column form a 1-D correlation over H rows
+// SS k + a - pb.  For each (rank, phase) the workgroup stages the kSepRows + KFM - 1 H
+// rows of its block in LDS once (one wave-wide 256-byte row load per row), and each wave
+// takes 16 output rows from them: every H row is read from memory once per block
+// rather than once per 16 rows.  The next (rank, phase)'s rows are loaded into registers
+// while the current one's taps run.  With the tap loops unrolled every factor index is
+// a compile-time constant (scalar loads of the zero-padded polyphase table); the padded
+// taps (SS m + a >= kf) carry zero factors and reach at most SS - 1 rows past the window.
+constexpr int kSepRows = 64;
 template <bool MULTI, int SS, int KF4>
 __global__ __launch_bounds__(256) void mc_sep_v(const McTable T) {
     constexpr int KFM = (4 * KF4 + SS - 1) / SS;
+    constexpr int KR = kSepRows + KFM - 1;  // staged H rows per (rank, phase)
+    constexpr int NL = (KR + 3) / 4;        // staged rows per wave
+    __shared__ float hs[KR * 64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // workgroups go round-robin over the 8 XCDs: consecutive logical blocks (the row
-    // blocks of one channel, which share H rows) are placed on one XCD and its L2.
-    // The grid is a multiple of 8, so every logical block is covered.
+    // blocks of one channel, which share KFM - 1 H rows) are placed on one XCD and its
+    // L2.  The grid is a multiple of 8, so every logical block is covered.
     const int per = gridDim.x >> 3;
-    const int lblk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    const int gtask = __builtin_amdgcn_readfirstlane(lblk * 4 + wave);  // rows, bounds tests: scalar
-    if (gtask >= T.tot_sepv) return;
+    const int lblk = __builtin_amdgcn_readfirstlane((blockIdx.x & 7) * per + (blockIdx.x >> 3));
+    if (lblk >= T.tot_sepv) return;  // whole workgroup
     int task;
-    const McLens& E = mc_locate<MULTI>(T, T.pre_sepv, gtask, task);
+    const McLens& E = mc_locate<MULTI>(T, T.pre_sepv, lblk, task);
     const DPlan& P = E.P;
     const McPlan& M = E.M;
     const int N = P.N, nsub = P.nsub, pb = P.pb, R = M.sep_rank;
-    const int nib = (N + kSepIB - 1) / kSepIB, njb = (N + 63) / 64;
+    const int nib = (N + kSepRows - 1) / kSepRows, njb = (N + 63) / 64;
     const int jb = task % njb; task /= njb;
     const int ib = task % nib; task /= nib;
     const int c = task % M.D;
     const int p = task / M.D;
     const int J = jb * 64 + lane;
-    if (J >= N) return;
-    const int I0 = ib * kSepIB;
-    const float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;  // uniform; lanes index by J
-    float acc[kSepIB];
-#pragma unroll
-    for (int i = 0; i < kSepIB; ++i) acc[i] = 0.0f;
-    for (int r = 0; r < R; ++r) {
+    const bool jok = J < N;
+    const int I0 = ib * kSepRows;
+    const float* Hc = M.H + ((size_t)p * M.D + c) * R * nsub * N;
+    // A wave stages whole rows, so every row test below is wave-uniform.  The loads take
+    // the nearest image row (always in bounds); the window's zero padding is applied
+    // when the rows are stored to LDS.  Columns past N load column N - 1, whose values
+    // reach only outputs that are not stored.  Only the last row slot can fall past KR.
+    const int swave = __builtin_amdgcn_readfirstlane(wave);
+    const unsigned jbyte = 4u * (unsigned)(jok ? J : N - 1);  // lane's byte offset in a row
+    float pre[NL];
+    auto fetch = [&](int it) {
+        const int r = it / SS, a = it - (it / SS) * SS;
         const float* Hr = Hc + (size_t)r * nsub * N;
+        const int y0 = SS * I0 + a - pb;
+#pragma unroll
+        for (int t = 0; t < NL; ++t) {
+            if (t < NL - 1 || swave + 4 * t < KR) {
+                const int y = y0 + SS * (swave + 4 * t);
+                const char* row = (const char*)(Hr + (size_t)min(max(y, 0), nsub - 1) * N);  // uniform
+                pre[t] = *(const float*)(row + jbyte);
+            }
+        }
+    };
+    auto stage = [&](int it) {
+        const int a = it - (it / SS) * SS;
+        const int y0 = SS * I0 + a - pb;
+#pragma unroll
+        for (int t = 0; t < NL; ++t) {
+            if (t < NL - 1 || swave + 4 * t < KR) {
+                const int y = y0 + SS * (swave + 4 * t);
+                hs[(swave + 4 * t) * 64 + lane] = (y >= 0 && y < nsub) ? pre[t] : 0.0f;
+            }
+        }
+    };
+    const int i0 = wave * 16;
+    const bool live = I0 + i0 < N;  // wave-uniform: this wave has output rows
+    float acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    const int nit = R * SS;
+    fetch(0);
+    stage(0);
+    __syncthreads();
 #pragma unroll 1
-        for (int a = 0; a < SS; ++a) {
+    for (int it = 0; it < nit; ++it) {
+        if (it + 1 < nit) fetch(it + 1);
+        if (live) {
+            const int r = it / SS, a = it - (it / SS) * SS;
             const CFloat* wa = (const CFloat*)M.sap + (r * SS + a) * KFM;
-            const int y0 = SS * I0 + a - pb;
+            const float* hw = hs + i0 * 64 + lane;
 #pragma unroll
-            for (int k = 0; k < kSepIB + KFM - 1; ++k) {
-                const int y = y0 + SS * k;
-                const float v = (y >= 0 && y < nsub) ? Hr[(size_t)y * N + J] : 0.0f;
+            for (int k = 0; k < 16 + KFM - 1; ++k) {
+                const float v = hw[k * 64];
 #pragma unroll
-                for (int i = 0; i < kSepIB; ++i) {
+                for (int i = 0; i < 16; ++i) {
                     const int m = k - i;
                     if (m >= 0 && m < KFM) acc[i] = fmaf(wa[m], v, acc[i]);
                 }
             }
         }
+        if (it + 1 < nit) {
+            __syncthreads();  // every wave is done with this (rank, phase)'s rows
+            stage(it + 1);
+            __syncthreads();
+        }
     }
+    if (!live || !jok) return;
     float* Xc = M.X + ((size_t)p * M.D + c) * N * N;
 #pragma unroll
-    for (int i = 0; i < kSepIB; ++i) {
-        const int I = I0 + i;
+    for (int i = 0; i < 16; ++i) {
+        const int I = I0 + i0 + i;
         if (I < N) Xc[I * N + J] = isnan(acc[i]) ? 0.0f : acc[i];  // NaN -> 0 after pooling
     }
 }

@@ -66,8 +66,8 @@ static hipError_t launch_mc_sep(hipStream_t st, const gl::McTable& T) {
     hipLaunchKernelGGL((gl::mc_sep_h<MULTI, SS, KF4>), dim3((T.tot_seph + 3) / 4), dim3(256), 0, st, T);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // a multiple of 8 workgroups (mc_sep_v's XCD-local block order)
-    hipLaunchKernelGGL((gl::mc_sep_v<MULTI, SS, KF4>), dim3((((T.tot_sepv + 3) / 4) + 7) & ~7), dim3(256), 0, st, T);
+    // a multiple of 8 workgroups (mc_sep_v's XCD-local block order), one per unit
+    hipLaunchKernelGGL((gl::mc_sep_v<MULTI, SS, KF4>), dim3((T.tot_sepv + 7) & ~7), dim3(256), 0, st, T);
     return hipGetLastError();
 }
 
