@@ -1262,8 +1262,18 @@ __global__ __launch_bounds__(256) void mc_adj_h(const McTable T) {
 // nothing: per channel with least squares (the channel was zeroed), the whole
 // sub-pixel with fixed amplitudes (their sum was zeroed, tf/simulator.py:81).
 // LM: the lens kinds of every lens of the launch; NC: an upper bound on its
-// components (statistics kept in registers: 8 per component).
-template <bool MULTI, bool SHP, int LM, int NC>
+// components (statistics kept in registers: up to 11 per component).  KS: the kind of
+// each component slot, 8 bits per slot (0: read from the plan), so that an instance for
+// one model keeps only the statistics its kinds update.
+__host__ __device__ constexpr int slot_kinds(int k0, int k1, int k2, int k3) {
+    return k0 | (k1 << 8) | (k2 << 16) | (k3 << 24);
+}
+template <int KS>
+__device__ __forceinline__ int slot_kind(int i, int plan_kind) {
+    const int k = (KS >> (8 * i)) & 255;
+    return k ? k : plan_kind;
+}
+template <bool MULTI, bool SHP, int LM, int NC, int KS = 0>
 __device__ __forceinline__ void mc_grad_body(const McTable& T) {
     static_assert(kStatStride * NC <= kMcStat, "statistics per particle");
     int local;
@@ -1308,11 +1318,12 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
             if (i < P.nlens) continue;
             if (i >= P.ncomp) break;
             const DComp& cd = P.comp[i];
+            const int kind = slot_kind<KS>(i, cd.kind);
             const float* cst = s_c + i * kConstStride;
             const bool src = cd.role == GL_ROLE_SOURCE;
             const float ux = src ? bx : X, uy = src ? by : Y;
             float* acc = st + kStatStride * i;
-            if (cd.kind == glm::K_SERSIC || cd.kind == glm::K_SERSIC_ELLIPSE) {
+            if (kind == glm::K_SERSIC || kind == glm::K_SERSIC_ELLIPSE) {
                 const glm::SerC C = *reinterpret_cast<const glm::SerC*>(cst);
                 glm::SerState ss_;
                 const float v = glm::sersic_forward<true>(C, ux, uy, ss_);
@@ -1321,7 +1332,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
                 float gdx, gdy;
                 glm::sersic_adjoint(C, ss_, g * cp[c], acc, gdx, gdy);
                 if (src) { gbx += gdx; gby += gdy; }
-            } else if (cd.kind == glm::K_CORE_SERSIC) {
+            } else if (kind == glm::K_CORE_SERSIC) {
                 const glm::CoreSerC C = *reinterpret_cast<const glm::CoreSerC*>(cst);
                 glm::CoreSerState cs;
                 const float v = glm::core_sersic_forward(C, ux, uy, cs);
@@ -1330,7 +1341,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
                 float gdx, gdy;
                 glm::core_sersic_adjoint(C, cs, g * cp[c], acc, gdx, gdy);
                 if (src) { gbx += gdx; gby += gdy; }
-            } else if (SHP && cd.kind == glm::K_SHAPELETS) {
+            } else if (SHP && kind == glm::K_SHAPELETS) {
                 const float beta = cst[0];
                 const float ib = 1.0f / beta;
                 const float u = (ux - cst[1]) * ib, w = (uy - cst[2]) * ib;
@@ -1362,7 +1373,7 @@ __device__ __forceinline__ void mc_grad_body(const McTable& T) {
 #pragma unroll
         for (int i = 0; i < kMaxLens; ++i) {
             if (i >= P.nlens) break;
-            const int kind = P.comp[i].kind;
+            const int kind = slot_kind<KS>(i, P.comp[i].kind);
             const float* cst = s_c + i * kConstStride;
             float* acc = st + kStatStride * i;
             if (lens_in(LM, i, glm::K_EPL) && kind == glm::K_EPL) {
@@ -1410,6 +1421,14 @@ __global__ __launch_bounds__(256) void mc_grad(const McTable T) {
 template <bool MULTI, bool SHP>
 __global__ __launch_bounds__(256, 2) void mc_grad_lean(const McTable T) {
     mc_grad_body<MULTI, SHP, kLensEplShear, 4>(T);
+}
+
+// The shapelet configuration's components in their slots: EPL, shear, a Sersic-ellipse
+// light and a shapelet light (21 statistics in registers instead of 44).
+constexpr int kSlotsEplShearSerShp = slot_kinds(glm::K_EPL, glm::K_SHEAR, glm::K_SERSIC_ELLIPSE, glm::K_SHAPELETS);
+template <bool MULTI>
+__global__ __launch_bounds__(256, 2) void mc_grad_shp4(const McTable T) {
+    mc_grad_body<MULTI, true, kLensEplShear, 4, kSlotsEplShearSerShp>(T);
 }
 
 // ---------------------------------------------------------------------------

@@ -175,7 +175,11 @@ static int mc_pipeline(const gl::McTable& T, const gl::McArgs& A, int ss_launch,
             const gl::DPlan& P1 = T.one.P;
             const bool lean = !MULTI && P1.ncomp <= 4 && P1.nlens >= 1 && P1.nlens <= 2 &&
                               P1.comp[0].kind == glm::K_EPL && (P1.nlens < 2 || P1.comp[1].kind == glm::K_SHEAR);
-            if (lean)
+            const bool shp4 = lean && P1.ncomp == 4 && P1.nlens == 2 && P1.comp[2].kind == glm::K_SERSIC_ELLIPSE &&
+                              P1.comp[3].kind == glm::K_SHAPELETS;
+            if (shp4)
+                hipLaunchKernelGGL((gl::mc_grad_shp4<MULTI>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
+            else if (lean)
                 hipLaunchKernelGGL((gl::mc_grad_lean<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
             else
                 hipLaunchKernelGGL((gl::mc_grad<MULTI, true>), dim3(T.tot_grad), dim3(256), T.lds_consts, st, T);
