@@ -25,7 +25,11 @@
 //                 stride-ss correlation with the PSF folded with the pooling box
 //                 (depthwise conv2d SAME + avg_pool2d); one lane per output
 //                 column streams the contributing sub-rows through L1
-//   mc_gram       least squares: per particle and pixel range [X | obs]^T W^2 [X | obs]
+//   mc_sep_h      (separable PSF, in place of mc_conv) horizontal pass of every
+//                 sub-row and rank into H, 256 flattened outputs per wave
+//   mc_sep_v      (separable PSF) polyphase vertical pass: a workgroup stages the
+//                 H rows of 64 output rows in LDS per (rank, phase)
+//   mc_gram      least squares: per particle and pixel range [X | obs]^T W^2 [X | obs]
 //                 on FP32 MFMA (16x16x4) tiles, flushed into FP64 every 128 pixels
 //   mc_solve      least squares: per particle FP64 parallel cyclic Jacobi, then the
 //                 pseudo-inverse with the reference's cutoff (1e-6, tf/simulator.py:161)
@@ -36,7 +40,10 @@
 //   mc_adj_v      (separable PSF) per sub-row block: transposed vertical pass
 //   mc_adjoint    per sub-pixel row segment: transposed correlation of dL/dimage
 //   mc_grad       per sub-pixel: recompute and accumulate reverse-mode statistics
-//                 (least squares: amplitudes held at their optimum, envelope theorem)
+//                 (least squares: amplitudes held at their optimum, envelope theorem);
+//                 instances with the lens kinds (mc_grad_lean) or every component
+//                 kind (mc_grad_shp4, mc_grad_m) fixed at compile time keep fewer
+//                 statistics in registers
 //   mc_finish     per particle: chain rule -> dlp/dz, optionally the Keras-Adam
 //                 update of a MAP step (tf/inference.py:28-35)
 #pragma once
